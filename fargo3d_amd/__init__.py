@@ -1,0 +1,214 @@
+"""fargo3d_amd -- FARGO3D's hydro time step on AMD MI355X (gfx950).
+
+The product is the shared library ``fargo3d_amd/lib/libfargo3d_amd.so``:
+hand-written HIP kernels behind the C ABI of ``include/fargo3d_hip.h`` and a C
+step driver (``csrc/host/f3d_driver.h``) that keeps FARGO3D's host structure
+(.par set-up, DT loop, per-routine function table).  This module is a thin
+ctypes binding over that library; it loads nothing else and has no fallback:
+without the built library, or without a HIP device, it raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["Simulation", "lib", "library_path", "build", "HipError", "SCALAR_KEYS"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "lib", "libfargo3d_amd.so")
+_lib = None
+
+SCALAR_KEYS = ["PhysicalTime", "OMEGAFRAME", "OMEGAFRAME0", "Domega", "dtemp",
+               "StepTime", "XAxisRotationAngle", "Timestepcount"]
+FIELDS_2D = {"Density0", "Energy0", "Vx0", "Vy0", "Vz0", "Reduction2D", "VxMed"}
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def library_path():
+    return _LIB
+
+
+def build(force=True):
+    from . import buildlib as _b
+    return _b.build(force=force)
+
+
+def lib():
+    """The loaded product library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            raise HipError("fargo3d_amd: %s is missing; run fargo3d_amd.build() first" % _LIB)
+        L = ctypes.CDLL(_LIB, mode=ctypes.RTLD_GLOBAL)
+        vp, d, i, cp = ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_char_p
+        L.f3d_sim_create.restype = vp
+        L.f3d_sim_create.argtypes = [cp, cp, cp, i]
+        L.f3d_sim_destroy.argtypes = [vp]
+        L.f3d_sim_error.restype = cp
+        L.f3d_sim_error.argtypes = [vp]
+        L.f3d_sim_step.argtypes = [vp, ctypes.POINTER(d)]
+        L.f3d_sim_run.argtypes = [vp, i]
+        L.f3d_sim_set_arch.argtypes = [vp, cp, cp]
+        L.f3d_sim_read_arch.argtypes = [vp, cp]
+        L.f3d_sim_field_size.restype = ctypes.c_long
+        L.f3d_sim_field_size.argtypes = [vp, cp]
+        L.f3d_sim_get_field.argtypes = [vp, cp, vp]
+        L.f3d_sim_set_field.argtypes = [vp, cp, vp]
+        L.f3d_sim_get_nshift.argtypes = [vp, vp]
+        L.f3d_sim_set_nshift.argtypes = [vp, vp]
+        L.f3d_sim_set_planets.argtypes = [vp, vp, i]
+        L.f3d_sim_device_field.restype = vp
+        L.f3d_sim_device_field.argtypes = [vp, cp]
+        L.f3d_sim_get_scalars.argtypes = [vp, vp]
+        L.f3d_sim_set_scalars.argtypes = [vp, vp]
+        L.f3d_sim_get_planets.argtypes = [vp, vp, i]
+        L.f3d_sim_shape.argtypes = [vp, vp]
+        L.f3d_sim_ctx.restype = vp
+        L.f3d_sim_ctx.argtypes = [vp]
+        L.f3d_sim_push_phys.argtypes = [vp]
+        L.f3d_sim_write_field.argtypes = [vp, cp, cp]
+        L.f3d_hip_last_error.restype = cp
+        L.f3d_hip_last_error.argtypes = [vp]
+        L.f3d_hip_sync.argtypes = [vp]
+        L.f3d_hip_device_count.restype = i
+        _lib = L
+    return _lib
+
+
+class Simulation:
+    """One device-resident run of the hydro step (a FARGO3D run on MI355X)."""
+
+    def __init__(self, parfile, overrides="", arch=None):
+        L = lib()
+        if L.f3d_hip_device_count() < 1:
+            raise HipError("fargo3d_amd: no HIP device is visible")
+        err = ctypes.create_string_buffer(512)
+        self._s = L.f3d_sim_create(parfile.encode(), overrides.encode(), err, 512)
+        if not self._s:
+            raise HipError("fargo3d_amd: " + err.value.decode())
+        self.L = L
+        shp = (ctypes.c_int * 4)()
+        L.f3d_sim_shape(self._s, shp)
+        self.nx, self.ny, self.nz, self.dimz = list(shp)
+        sy = self.ny + 6
+        sz = self.nz + 6 if self.dimz else 1
+        self.shape = (sz, sy, self.nx)
+        self.shape2d = (sz, sy)
+        for g, m in (arch or {}).items():
+            self.set_arch(g, m)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise HipError(self.L.f3d_sim_error(self._s).decode())
+
+    def close(self):
+        if getattr(self, "_s", None):
+            self.L.f3d_sim_destroy(self._s)
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ctx(self):
+        return self.L.f3d_sim_ctx(self._s)
+
+    def set_arch(self, group, mode):
+        self._check(self.L.f3d_sim_set_arch(self._s, group.encode(), mode.encode()))
+
+    def read_arch(self, path):
+        self._check(self.L.f3d_sim_read_arch(self._s, path.encode()))
+
+    def step(self):
+        dt = ctypes.c_double()
+        self._check(self.L.f3d_sim_step(self._s, ctypes.byref(dt)))
+        return dt.value
+
+    def run(self, nsteps):
+        self._check(self.L.f3d_sim_run(self._s, int(nsteps)))
+
+    def sync(self):
+        if self.L.f3d_hip_sync(self.ctx) != 0:
+            raise HipError(self.L.f3d_hip_last_error(self.ctx).decode())
+
+    def field(self, name):
+        shp = self.shape2d if name in FIELDS_2D else self.shape
+        out = np.empty(shp, dtype=np.float64)
+        self._check(self.L.f3d_sim_get_field(self._s, name.encode(), out.ctypes.data))
+        return out
+
+    def set_field(self, name, arr):
+        shp = self.shape2d if name in FIELDS_2D else self.shape
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(shp)
+        self._check(self.L.f3d_sim_set_field(self._s, name.encode(), a.ctypes.data))
+
+    def device_ptr(self, name):
+        p = self.L.f3d_sim_device_field(self._s, name.encode())
+        if not p:
+            raise KeyError(name)
+        return p
+
+    def nshift(self):
+        out = np.empty(self.shape2d, dtype=np.int32)
+        self._check(self.L.f3d_sim_get_nshift(self._s, out.ctypes.data))
+        return out
+
+    def set_nshift(self, arr):
+        a = np.ascontiguousarray(arr, dtype=np.int32).reshape(self.shape2d)
+        self._check(self.L.f3d_sim_set_nshift(self._s, a.ctypes.data))
+
+    def set_planets(self, arr):
+        a = np.zeros((len(arr), 8))
+        a[:, :7] = arr
+        self._check(self.L.f3d_sim_set_planets(self._s, a.ctypes.data, len(arr)))
+
+    def scalars(self):
+        out = (ctypes.c_double * 8)()
+        self.L.f3d_sim_get_scalars(self._s, out)
+        return dict(zip(SCALAR_KEYS, list(out)))
+
+    def set_scalars(self, d):
+        cur = self.scalars()
+        cur.update(d)
+        arr = (ctypes.c_double * 8)(*[cur[k] for k in SCALAR_KEYS])
+        self.L.f3d_sim_set_scalars(self._s, arr)
+        self._check(self.L.f3d_sim_push_phys(self._s))
+
+    def push_phys(self):
+        self._check(self.L.f3d_sim_push_phys(self._s))
+
+    def planets(self, maxn=16):
+        out = (ctypes.c_double * (8 * maxn))()
+        self.L.f3d_sim_get_planets(self._s, out, maxn)
+        a = np.array(out).reshape(maxn, 8)
+        nb = int(a[0, 7]) if a[0, 7] > 0 else 0
+        return a[:nb, :7]
+
+    def write_field(self, name, path):
+        self._check(self.L.f3d_sim_write_field(self._s, name.encode(), path.encode()))
+
+    def call(self, routine, *args):
+        """Call one C-ABI entry point f3d_<routine>(ctx, ...) by its reference
+        name.  str arguments name a bound field (its device pointer)."""
+        f = getattr(self.L, "f3d_" + routine)
+        f.restype = ctypes.c_int
+        cargs = [ctypes.c_void_p(self.ctx)]
+        for a in args:
+            if isinstance(a, float):
+                cargs.append(ctypes.c_double(a))
+            elif isinstance(a, bool) or isinstance(a, int):
+                cargs.append(ctypes.c_int(int(a)))
+            elif isinstance(a, str):
+                cargs.append(ctypes.c_void_p(self.device_ptr(a)))
+            else:
+                cargs.append(a)
+        rc = f(*cargs)
+        if rc != 0:
+            raise HipError("%s: %s" % (routine, self.L.f3d_hip_last_error(self.ctx).decode()))
+        return rc

@@ -1,0 +1,155 @@
+// f3d_runtime.hip -- handle lifetime, mesh upload, field binding, memory helpers.
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "f3d_internal.h"
+
+static thread_local char g_create_err[512];
+
+extern "C" {
+
+int f3d_hip_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int f3d_hip_create(const f3d_mesh_desc* d, f3d_hip_ctx** out) {
+  *out = nullptr;
+  if (!d || d->nx < 1 || d->ny < 1 || d->nz < 1) {
+    snprintf(g_create_err, sizeof(g_create_err), "f3d_hip_create: bad mesh description");
+    return -2;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    snprintf(g_create_err, sizeof(g_create_err), "f3d_hip_create: no HIP device is visible");
+    return -3;
+  }
+  f3d_hip_ctx* c = new f3d_hip_ctx();
+  hipGetDevice(&c->device);
+  const int sx = d->nx, sy = d->ny + 2 * d->nghy, sz = d->nz + 2 * d->nghz;
+  f3d::dmesh& m = c->m;
+  m.nx = d->nx; m.ny = d->ny; m.nz = d->nz; m.nghy = d->nghy; m.nghz = d->nghz;
+  m.dimz = d->dimz; m.geom = d->geometry;
+  m.sx = sx; m.sy = sy; m.sz = sz; m.pitch = d->pitch; m.stride = d->stride;
+  m.XMIN = d->XMIN; m.XMAX = d->XMAX; m.mindx = d->mindx;
+  if (m.pitch < sx || m.stride < m.pitch * sy) {
+    snprintf(g_create_err, sizeof(g_create_err), "f3d_hip_create: pitch/stride smaller than the mesh");
+    delete c;
+    return -2;
+  }
+  c->n3 = (size_t)m.stride * sz;
+  c->n2 = (size_t)sy * sz;
+  struct Arr { const double* src; size_t n; const double** dst; };
+  Arr arrs[] = {
+      {d->xmin, (size_t)sx + 1, &m.xmin}, {d->ymin, (size_t)sy + 1, &m.ymin}, {d->zmin, (size_t)sz + 1, &m.zmin},
+      {d->xmed, (size_t)sx, &m.xmed}, {d->ymed, (size_t)sy, &m.ymed}, {d->zmed, (size_t)sz, &m.zmed},
+      {d->sxi, (size_t)sx, &m.sxi}, {d->sxj, (size_t)sy, &m.sxj}, {d->syj, (size_t)sy, &m.syj},
+      {d->szj, (size_t)sy, &m.szj}, {d->sxk, (size_t)sz, &m.sxk}, {d->syk, (size_t)sz, &m.syk},
+      {d->szk, (size_t)sz, &m.szk}, {d->invvj, (size_t)sy, &m.invvj}, {d->invdiffxmed, (size_t)sx, &m.invdiffxmed},
+      {d->cosxmed, (size_t)sx, &m.cosxmed}, {d->sinxmed, (size_t)sx, &m.sinxmed},
+      {d->sinzmed, (size_t)sz, &m.sinzmed}, {d->coszmed, (size_t)sz, &m.coszmed},
+      {d->sinzmin, (size_t)sz + 1, &m.sinzmin}, {d->coszmin, (size_t)sz + 1, &m.coszmin}};
+  const int na = sizeof(arrs) / sizeof(arrs[0]);
+  size_t total = 0;
+  std::vector<size_t> off(na);
+  for (int a = 0; a < na; a++) {
+    off[a] = total;
+    total += (arrs[a].n + 7) & ~(size_t)7;  // 64-byte aligned slices
+  }
+  std::vector<double> host(total, 0.0);
+  for (int a = 0; a < na; a++)
+    if (arrs[a].src) memcpy(&host[off[a]], arrs[a].src, arrs[a].n * sizeof(double));
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&c->meshbuf, total * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpy(c->meshbuf, host.data(), total * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&c->dred, (c->n2 + 64) * sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&c->hres, 64 * sizeof(double), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    snprintf(g_create_err, sizeof(g_create_err), "f3d_hip_create: %s", hipGetErrorString(e));
+    f3d_hip_destroy(c);
+    return -1;
+  }
+  for (int a = 0; a < na; a++) *arrs[a].dst = c->meshbuf + off[a];
+  *out = c;
+  return 0;
+}
+
+void f3d_hip_destroy(f3d_hip_ctx* c) {
+  if (!c) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->meshbuf) hipFree(c->meshbuf);
+  if (c->bctab) hipFree(c->bctab);
+  if (c->dred) hipFree(c->dred);
+  if (c->hres) hipHostFree(c->hres);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int f3d_hip_bind(f3d_hip_ctx* c, const f3d_fields* f) {
+  if (!c || !f) return -2;
+  c->f = *f;
+  return 0;
+}
+
+int f3d_hip_set_phys(f3d_hip_ctx* c, const f3d_phys* p) {
+  if (!c || !p) return -2;
+  const bool had_tables = c->bctab != nullptr;
+  c->ph = *p;
+  c->ph.ymin_dexp = c->ph.ymax_dexp = c->ph.zmin_dens = c->ph.zmax_dens = nullptr;
+  // the per-(m,k) / per-(m,j) boundary factors go to the device once
+  if (!had_tables && p->ymin_dexp && p->ymax_dexp && p->zmin_dens && p->zmax_dens) {
+    const size_t ny3 = 3 * (size_t)c->m.sz, nz3 = 3 * (size_t)c->m.sy;
+    std::vector<double> h(2 * ny3 + 2 * nz3);
+    memcpy(&h[0], p->ymin_dexp, ny3 * sizeof(double));
+    memcpy(&h[ny3], p->ymax_dexp, ny3 * sizeof(double));
+    memcpy(&h[2 * ny3], p->zmin_dens, nz3 * sizeof(double));
+    memcpy(&h[2 * ny3 + nz3], p->zmax_dens, nz3 * sizeof(double));
+    F3D_CHECK(c, hipMalloc(&c->bctab, h.size() * sizeof(double)));
+    F3D_CHECK(c, hipMemcpy(c->bctab, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->d_ymin_dexp = c->bctab;
+    c->d_ymax_dexp = c->bctab + ny3;
+    c->d_zmin_dens = c->bctab + 2 * ny3;
+    c->d_zmax_dens = c->bctab + 2 * ny3 + nz3;
+  }
+  return 0;
+}
+
+int f3d_hip_sync(f3d_hip_ctx* c) {
+  F3D_CHECK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+const char* f3d_hip_last_error(f3d_hip_ctx* c) { return c ? c->err : g_create_err; }
+
+int f3d_hip_malloc(void** ptr, size_t bytes) {
+  if (hipMalloc(ptr, bytes ? bytes : 8) != hipSuccess) return -1;
+  return 0;
+}
+
+int f3d_hip_free(void* ptr) { return hipFree(ptr) == hipSuccess ? 0 : -1; }
+
+int f3d_hip_memcpy_h2d(f3d_hip_ctx* c, void* dst, const void* src, size_t bytes) {
+  F3D_CHECK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  F3D_CHECK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int f3d_hip_memcpy_d2h(f3d_hip_ctx* c, void* dst, const void* src, size_t bytes) {
+  F3D_CHECK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  F3D_CHECK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int f3d_hip_memcpy_d2d(f3d_hip_ctx* c, void* dst, const void* src, size_t bytes) {
+  F3D_CHECK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+  return 0;
+}
+
+int f3d_hip_memset(f3d_hip_ctx* c, void* dst, int value, size_t bytes) {
+  F3D_CHECK(c, hipMemsetAsync(dst, value, bytes, c->stream));
+  return 0;
+}
+
+}  // extern "C"

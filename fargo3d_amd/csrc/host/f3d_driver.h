@@ -1,0 +1,61 @@
+/*
+ * f3d_driver.h -- C step driver of the MI355X build (the product's host side).
+ *
+ * Keeps FARGO3D's host structure: the .par-driven set-up (src/main.c:176-316),
+ * the DT loop (src/main.c:361-448), Sources/Transport (src/algogas.c:30-159,
+ * src/transport.c:69-193) and the per-routine function-pointer table selected
+ * by a func_arch.cfg-style file (src/change_arch.c:146-570).  Every routine
+ * in the table runs on the GPU through the C ABI of include/fargo3d_hip.h;
+ * fields are device-resident for the whole run.
+ */
+#ifndef F3D_DRIVER_H
+#define F3D_DRIVER_H
+
+#include "fargo3d_hip.h"
+#include "f3d_host.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct f3d_sim f3d_sim;
+
+f3d_sim *f3d_sim_create(const char *parfile, const char *overrides, char *err, int errlen);
+void f3d_sim_destroy(f3d_sim *S);
+const char *f3d_sim_error(f3d_sim *S);
+
+/* One pass of the DT loop body; *dt_out receives the time step used. */
+int f3d_sim_step(f3d_sim *S, double *dt_out);
+int f3d_sim_run(f3d_sim *S, int nsteps);
+
+/* func_arch table: group names as std/func_arch.cfg; modes "GPU"/"HIP"
+ * (one kernel per routine) or, for the stage groups "FusedCfl",
+ * "FusedSources", "FusedTransport", "yes"/"no".  "CPU" is rejected. */
+int f3d_sim_set_arch(f3d_sim *S, const char *group, const char *mode);
+int f3d_sim_read_arch(f3d_sim *S, const char *path);
+
+/* field access (host copies of the device arrays) */
+long f3d_sim_field_size(f3d_sim *S, const char *name);
+int f3d_sim_get_field(f3d_sim *S, const char *name, double *host);
+int f3d_sim_set_field(f3d_sim *S, const char *name, const double *host);
+int f3d_sim_get_nshift(f3d_sim *S, int *host);
+int f3d_sim_set_nshift(f3d_sim *S, const int *host);
+/* planets as 8 doubles each: x y z vx vy vz mass (unused) */
+int f3d_sim_set_planets(f3d_sim *S, const double *in, int n);
+double *f3d_sim_device_field(f3d_sim *S, const char *name);
+void f3d_sim_get_scalars(f3d_sim *S, double out[8]);
+void f3d_sim_set_scalars(f3d_sim *S, const double in[8]);
+void f3d_sim_get_planets(f3d_sim *S, double *out, int maxn);
+void f3d_sim_shape(f3d_sim *S, int out[4]); /* nx, ny, nz, dimz */
+f3d_hip_ctx *f3d_sim_ctx(f3d_sim *S);
+/* refresh the kernels' copy of the run parameters and host scalars */
+int f3d_sim_push_phys(f3d_sim *S);
+
+/* Output in FARGO3D's merged raw format (src/output.c:283-315): active cells,
+ * x fastest, native-endian doubles, no header; file <dir>/<prefix><name><n>.dat. */
+int f3d_sim_write_field(f3d_sim *S, const char *name, const char *path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,196 @@
+/*
+ * fargo3d_hip.h -- C ABI of the MI355X (gfx950) implementation of FARGO3D's
+ * hydro time step.
+ *
+ * Drop-in model.  FARGO3D calls every routine of its step through a global
+ * function pointer (src/global.h:303-416) that ChangeArch binds to a *_cpu
+ * or *_gpu implementation from std/func_arch.cfg (src/change_arch.c:146-570).
+ * Those implementations read the selected fluid's fields, the mesh and the
+ * run parameters from globals.  This ABI keeps the same routine names and
+ * argument lists and makes the globals explicit in one handle:
+ *   - f3d_hip_create   uploads the mesh (the LightGlobalDev step,
+ *                      src/light_global_dev.c:5-23);
+ *   - f3d_hip_bind     names the device buffers of the selected fluid and the
+ *                      shared scratch fields (SelectFluid, src/LowTasks.c:565);
+ *   - f3d_hip_set_phys sets the run parameters and host scalars the kernels
+ *                      read (.par values, OMEGAFRAME, planets).
+ * A Field* argument of the reference becomes the device pointer of that field
+ * (Field->field_gpu); everything else is plain C.  Buffers are x-fastest
+ * arrays with the pitch/stride given at creation.
+ *
+ * Errors: every entry point returns 0 on success or a negative code; the
+ * message is available from f3d_hip_last_error().  (The reference instead
+ * exits the process from check_errors, src/lowtasks.cu:6-15.)
+ * Launches are asynchronous on the handle's stream unless a routine returns a
+ * host value (the reductions and cfl), which synchronises.
+ */
+#ifndef FARGO3D_HIP_H
+#define FARGO3D_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct f3d_hip_ctx f3d_hip_ctx;
+
+/* Mesh description (host arrays; sizes: x* nx+1 / nx, y* sy+1 / sy, z* sz+1 / sz
+ * with sy = ny+2*nghy, sz = nz+2*nghz).  Fields mirror src/global.h:210-242. */
+typedef struct f3d_mesh_desc {
+  int nx, ny, nz, nghy, nghz, dimz, geometry; /* geometry: 1 cylindrical, 2 spherical */
+  int pitch, stride;                          /* in elements */
+  double XMIN, XMAX, mindx;
+  const double *xmin, *ymin, *zmin, *xmed, *ymed, *zmed;
+  const double *sxi, *sxj, *syj, *szj, *sxk, *syk, *szk, *invvj, *invdiffxmed;
+  const double *cosxmed, *sinxmed;            /* cos/sin(Xmed), nx */
+  const double *sinzmed, *coszmed;            /* sin/cos(Zmed), sz */
+  const double *sinzmin, *coszmin;            /* sin/cos(Zmin), sz+1 */
+} f3d_mesh_desc;
+
+/* Device buffers of the selected fluid and the shared scratch fields
+ * (src/LowTasks.c:565-642).  NULL where a field does not exist (e.g. Vz in 2-D). */
+typedef struct f3d_fields {
+  double *Density, *Energy, *Vx, *Vy, *Vz, *Vx_temp, *Vy_temp, *Vz_temp;
+  double *Mpx, *Mmx, *Mpy, *Mmy, *Mpz, *Mmz;
+  double *Pot, *Slope, *DivRho, *DensStar, *Qs, *Pressure, *Total_Density, *QL, *QR;
+  double *Density0, *Energy0, *Vx0, *Vy0, *Vz0; /* Stockholm references, YZ arrays */
+  double *Reduction2D, *VxMed;                   /* YZ arrays */
+  int *Nshift;                                   /* YZ int array */
+} f3d_fields;
+
+/* Run parameters and host scalars the kernels read. */
+typedef struct f3d_phys {
+  int fluidtype;       /* 1 gas, 2 dust (Fluidtype) */
+  int eos;             /* 1 isothermal, 2 adiabatic */
+  int visc;            /* 0 none, 1 NU, 2 ALPHA */
+  int potential, fargo, stockholm, indirectterm, periodicz, periodicy;
+  int bc;              /* 1: 2-D Keplerian disk boundaries, 2: 3-D spherical disk boundaries */
+  double nu, alpha, gamma, cfl;
+  double OMEGAFRAME, OMEGAFRAME0, PhysicalTime;
+  double ymin_par, ymax_par, zmin_par, zmax_par, taudamp;
+  double Y_inf, Y_sup, Z_inf, Z_sup;               /* damping zones (src/stockholm.c:76-90) */
+  /* boundary factors (host glibc evaluation of std/boundaries.txt formulas) */
+  double ymin_dens[3], ymax_dens[3], ymin_vsq[3], ymax_vsq[3], ymin_vfac[3], ymax_vfac[3];
+  double zmin_vfac[3], zmax_vfac[3];
+  const double *ymin_dexp, *ymax_dexp, *zmin_dens, *zmax_dens; /* host arrays, 3*sz / 3*sy */
+  /* planets for Potential (positions at the start of the step) */
+  int nplanets;
+  double px[16], py[16], pz[16], pmass[16]; /* mass already multiplied by the taper */
+  double psmooth2[16];                      /* smoothing^2 (src/potential.c:120-131) */
+} f3d_phys;
+
+/* ---- runtime ---- */
+int f3d_hip_create(const f3d_mesh_desc *mesh, f3d_hip_ctx **out);
+void f3d_hip_destroy(f3d_hip_ctx *ctx);
+int f3d_hip_bind(f3d_hip_ctx *ctx, const f3d_fields *fields);
+int f3d_hip_set_phys(f3d_hip_ctx *ctx, const f3d_phys *phys);
+int f3d_hip_sync(f3d_hip_ctx *ctx);
+const char *f3d_hip_last_error(f3d_hip_ctx *ctx);
+/* device memory helpers for callers without their own allocator */
+int f3d_hip_malloc(void **ptr, size_t bytes);
+int f3d_hip_free(void *ptr);
+int f3d_hip_memcpy_h2d(f3d_hip_ctx *ctx, void *dst, const void *src, size_t bytes);
+int f3d_hip_memcpy_d2h(f3d_hip_ctx *ctx, void *dst, const void *src, size_t bytes);
+int f3d_hip_memcpy_d2d(f3d_hip_ctx *ctx, void *dst, const void *src, size_t bytes);
+int f3d_hip_memset(f3d_hip_ctx *ctx, void *dst, int value, size_t bytes);
+int f3d_hip_device_count(void);
+
+/* ---- per-routine entry points (reference pointer table, src/global.h:303-416) ---- */
+/* ComputePressureFieldIso/Ad: src/CompPresIso.c:10, src/CompPresAd.c:10 */
+int f3d_ComputePressureFieldIso(f3d_hip_ctx *ctx);
+int f3d_ComputePressureFieldAd(f3d_hip_ctx *ctx);
+/* ComputeTotalDensity: src/totaldens.c:10; Reset_field: src/resetfield.c:10 */
+int f3d_ComputeTotalDensity(f3d_hip_ctx *ctx);
+int f3d_Reset_field(f3d_hip_ctx *ctx, double *F);
+/* SubStep1_x/y/z: src/substep1_x.c:10, src/substep1_y.c:10, src/substep1_z.c:10 */
+int f3d_SubStep1_x(f3d_hip_ctx *ctx, double dt);
+int f3d_SubStep1_y(f3d_hip_ctx *ctx, double dt);
+int f3d_SubStep1_z(f3d_hip_ctx *ctx, double dt);
+/* SubStep2_a/b: src/substep2_a.c:10, src/substep2_b.c:10; SubStep3: src/substep3.c:10 */
+int f3d_SubStep2_a(f3d_hip_ctx *ctx, double dt);
+int f3d_SubStep2_b(f3d_hip_ctx *ctx, double dt);
+int f3d_SubStep3(f3d_hip_ctx *ctx, double dt);
+/* visctensor_cyl/sph + addviscosity_cyl/sph: src/visctensor_cyl.c:10,
+ * src/addviscosity_cyl.c:10, src/visctensor_sph.c:10, src/addviscosity_sph.c:10 */
+int f3d_visctensor_cyl(f3d_hip_ctx *ctx);
+int f3d_addviscosity_cyl(f3d_hip_ctx *ctx, double dt);
+int f3d_visctensor_sph(f3d_hip_ctx *ctx);
+int f3d_addviscosity_sph(f3d_hip_ctx *ctx, double dt);
+/* Potential: src/potential.c:51; CorrectVtheta: src/correct_vtheta.c:10 */
+int f3d_Potential(f3d_hip_ctx *ctx);
+int f3d_CorrectVtheta(f3d_hip_ctx *ctx, double domega);
+/* _ComputeForce: src/compute_force.c:16 (fills the 12 force sums, host values) */
+int f3d_ComputeForce(f3d_hip_ctx *ctx, double x, double y, double z, double rsmoothing, double mass,
+                     double localforce_out[12]);
+/* copy_velocities: src/cpvel.c:3 (option 1 = VTEMP2V, 2 = V2VTEMP) */
+int f3d_copy_velocities(f3d_hip_ctx *ctx, int option);
+/* boundaries: src/boundary.c:2 (zmin, zmax, ymin, ymax of the bound family) */
+int f3d_boundaries(f3d_hip_ctx *ctx);
+/* momenta_x/y/z: src/momenta_x.c:10, src/momenta_y.c:10, src/momenta_z.c:10 */
+int f3d_momenta_x(f3d_hip_ctx *ctx);
+int f3d_momenta_y(f3d_hip_ctx *ctx);
+int f3d_momenta_z(f3d_hip_ctx *ctx);
+/* DivideByRho: src/divrho.c:10 */
+int f3d_DivideByRho(f3d_hip_ctx *ctx, const double *Q);
+/* VanLeerX/Y/Z_a/b: src/vanleer_x_a.c:10, src/vanleer_x_b.c:10, src/vanleer_y_a.c:10,
+ * src/vanleer_y_b.c:10, src/vanleer_z_a.c:10, src/vanleer_z_b.c:10 */
+int f3d_VanLeerX_a(f3d_hip_ctx *ctx, const double *Q);
+int f3d_VanLeerX_b(f3d_hip_ctx *ctx, double dt, const double *Q, double *Qs, const double *Vx_t);
+int f3d_VanLeerY_a(f3d_hip_ctx *ctx, const double *Q);
+int f3d_VanLeerY_b(f3d_hip_ctx *ctx, double dt, const double *Q, double *Qs);
+int f3d_VanLeerZ_a(f3d_hip_ctx *ctx, const double *Q);
+int f3d_VanLeerZ_b(f3d_hip_ctx *ctx, double dt, const double *Q, double *Qs);
+/* VanLeerX_PPA_a/b/c/d: src/fargo_ppa_a.c:10, src/fargo_ppa_b.c:10, src/fargo_ppa_c.c:10,
+ * src/fargo_ppa_d.c:10 */
+int f3d_VanLeerX_PPA_a(f3d_hip_ctx *ctx, const double *Q);
+int f3d_VanLeerX_PPA_b(f3d_hip_ctx *ctx, const double *Q);
+int f3d_VanLeerX_PPA_c(f3d_hip_ctx *ctx, const double *Q);
+int f3d_VanLeerX_PPA_d(f3d_hip_ctx *ctx, double dt, const double *Q, double *Qs, const double *Vx_t);
+/* UpdateX/Y/Z: src/update_x.c:10, src/update_y.c:10, src/update_z.c;
+ * UpdateDensityX/Y/Z: src/update_density_x.c:10, src/update_density_y.c:10, src/update_density_z.c */
+int f3d_UpdateX(f3d_hip_ctx *ctx, double dt, double *Q, const double *Qs, const double *Vx_t);
+int f3d_UpdateY(f3d_hip_ctx *ctx, double dt, double *Q, const double *Qs);
+int f3d_UpdateZ(f3d_hip_ctx *ctx, double dt, double *Q, const double *Qs);
+int f3d_UpdateDensityX(f3d_hip_ctx *ctx, double dt, double *Q, const double *Vx_t);
+int f3d_UpdateDensityY(f3d_hip_ctx *ctx, double dt, double *Q);
+int f3d_UpdateDensityZ(f3d_hip_ctx *ctx, double dt, double *Q);
+/* NewVelocity_x/y/z: src/newvel_x.c:10, src/newvel_y.c:10, src/newvel_z.c:10 */
+int f3d_NewVelocity_x(f3d_hip_ctx *ctx);
+int f3d_NewVelocity_y(f3d_hip_ctx *ctx);
+int f3d_NewVelocity_z(f3d_hip_ctx *ctx);
+/* ComputeVweight: src/compute_vmed.c:42; ComputeVmed: src/compute_vmed.c:10 */
+int f3d_ComputeVweight(f3d_hip_ctx *ctx, const double *V, double *Q);
+int f3d_ComputeVmed(f3d_hip_ctx *ctx, const double *V);
+/* ComputeResidual: src/compute_residual.c:10; AdvectSHIFT: src/advect_shift.c:9 */
+int f3d_ComputeResidual(f3d_hip_ctx *ctx, double dt);
+int f3d_AdvectSHIFT(f3d_hip_ctx *ctx, double *F, const int *Nshift);
+/* reduction_SUM/MIN: src/reduction_generic.c:5 (per-ring, into Reduction2D, ring sums in
+ * sequential i order); reduction_full_*: src/reduction_full_generic.c:1 (host value) */
+int f3d_reduction_SUM(f3d_hip_ctx *ctx, const double *F, int ymin, int ymax, int zmin, int zmax);
+int f3d_reduction_MIN(f3d_hip_ctx *ctx, const double *F, int ymin, int ymax, int zmin, int zmax);
+int f3d_reduction_full_SUM(f3d_hip_ctx *ctx, const double *F, int ymin, int ymax, int zmin, int zmax,
+                           double *result);
+int f3d_reduction_full_MIN(f3d_hip_ctx *ctx, const double *F, int ymin, int ymax, int zmin, int zmax,
+                           double *result);
+/* cfl: src/cfl.c:10 + cfl_b: src/cfl_b.c:3 (per-fluid Min, host value) */
+int f3d_cfl(f3d_hip_ctx *ctx, double *fluid_min);
+/* StockholmBoundary: src/stockholm.c:10; init_stockholm: src/init_stockholm.c:3 */
+int f3d_StockholmBoundary(f3d_hip_ctx *ctx, double dt);
+int f3d_init_stockholm(f3d_hip_ctx *ctx);
+
+/* ---- fused stages (no counterpart in the reference: the same arithmetic as
+ * the routine sequences they replace, in fewer HBM passes) ---- */
+/* ComputeVmed(Vx) + cfl + cfl_b in two launches; writes the fluid's Min to *fluid_min. */
+int f3d_fused_cfl(f3d_hip_ctx *ctx, double *fluid_min);
+/* Sources from the pressure field to the mid-step ghost fill (src/algogas.c:34-118),
+ * after Potential; domega is the CorrectVtheta argument (0 when not corotating). */
+int f3d_fused_sources(f3d_hip_ctx *ctx, double dt, double domega);
+/* Transport (src/algogas.c:135-159) and, when stockholm != 0, StockholmBoundary and
+ * the end-of-step ghost fill. */
+int f3d_fused_transport(f3d_hip_ctx *ctx, double dt, int end_of_step);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,247 @@
+"""GPU parity of the HIP path against the CPU restatement (oracle).
+
+Two kinds of check, as FARGO3D's own debugging harness does (FARGO_DEBUG,
+src/define.h:523-535): the whole step from the same initial conditions, and
+each routine separately from an identical state.  The bar is 1e-12 relative
+(max |a-b| / max |b| per field) for floating point and exact equality for the
+integer shift counts; the number of bit-level differences is reported too.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+C1 = "fargo3d_amd/setups/c1_fargo2d.par"
+C2 = "fargo3d_amd/setups/c2_p3d.par"
+C2_SMALL = "Nx=64 Ny=24 Nz=12"
+RTOL = 1e-12
+
+STATE3D = ["Density", "Energy", "Vx", "Vy", "Vx_temp", "Vy_temp", "Mpx", "Mmx", "Mpy", "Mmy",
+           "Pot", "Slope", "DivRho", "DensStar", "Qs", "Pressure", "Total_Density", "QL", "QR"]
+STATE3D_Z = ["Vz", "Vz_temp", "Mpz", "Mmz"]
+STATE2D = ["Density0", "Energy0", "Vx0", "Vy0", "Vz0", "Reduction2D", "VxMed"]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def relerr(a, b):
+    den = float(np.max(np.abs(b))) if b.size else 0.0
+    d = float(np.max(np.abs(a - b))) if a.size else 0.0
+    if den == 0.0:
+        return d
+    return d / den
+
+
+def nbits(a, b):
+    return int(np.count_nonzero(a.view(np.int64) != b.view(np.int64)))
+
+
+def active(x, dimz):
+    if dimz:
+        return x[3:-3, 3:-3, :]
+    return x[:, 3:-3, :]
+
+
+@pytest.fixture(scope="module")
+def fa():
+    import fargo3d_amd
+    return fargo3d_amd
+
+
+@pytest.fixture(scope="module")
+def orc():
+    import oracle
+    return oracle
+
+
+def make_pair(fa, orc, par, ov="", arch=None):
+    o = orc.Oracle(par, ov)
+    s = fa.Simulation(par, ov, arch=arch or {})
+    o.set_shape(s.nx, s.ny, s.nz, s.dimz)
+    return o, s
+
+
+def copy_state(o, s):
+    names = STATE3D + (STATE3D_Z if s.dimz else [])
+    for n in names:
+        s.set_field(n, o.field(n))
+    for n in STATE2D:
+        s.set_field(n, o.field(n))
+    s.set_nshift(o.nshift())
+    s.set_planets(o.planets())
+    s.set_scalars(o.scalars())
+
+
+def compare_all(o, s, tag, exact=False):
+    names = STATE3D + (STATE3D_Z if s.dimz else [])
+    worst = 0.0
+    for n in names + STATE2D:
+        a, b = s.field(n), o.field(n)
+        e = relerr(a, b)
+        if e > RTOL or (exact and nbits(a, b)):
+            raise AssertionError("%s: field %s rel err %.3e, %d bits differ" % (tag, n, e, nbits(a, b)))
+        worst = max(worst, e)
+    ns, no = s.nshift(), o.nshift()
+    assert np.array_equal(ns, no), "%s: Nshift differs" % tag
+    return worst
+
+
+@pytest.mark.parametrize("par,ov", [(C1, ""), (C2, C2_SMALL)], ids=["c1_2d", "c2_3d_small"])
+def test_initial_state(fa, orc, par, ov):
+    """Initial conditions + first ghost fill + Stockholm references agree everywhere."""
+    say("initial state", par, ov)
+    o, s = make_pair(fa, orc, par, ov)
+    for n in ["Density", "Energy", "Vx", "Vy"] + (["Vz"] if s.dimz else []):
+        a, b = s.field(n), o.field(n)
+        assert nbits(a, b) == 0, "%s differs after the initial ghost fill" % n
+    for n in ["Density0", "Vx0", "Vy0"]:
+        assert nbits(s.field(n), o.field(n)) == 0, n
+
+
+ROUTINES_2D = [
+    ("ComputePressureFieldIso", ()),
+    ("Potential", ()),
+    ("CorrectVtheta", (1.0e-3,)),
+    ("SubStep1_x", ("dt",)),
+    ("SubStep1_y", ("dt",)),
+    ("visctensor_cyl", ()),
+    ("addviscosity_cyl", ("dt",)),
+    ("SubStep2_a", ("dt",)),
+    ("SubStep2_b", ("dt",)),
+    ("boundaries", ()),
+    ("momenta_x", ()),
+    ("momenta_y", ()),
+    ("DivideByRho", ("Mpx",)),
+    ("VanLeerY_a", ("DivRho",)),
+    ("VanLeerY_b", ("dt", "DivRho", "Qs")),
+    ("UpdateY", ("dt", "Mpx", "Qs")),
+    ("UpdateDensityY", ("dt", "Density")),
+    ("ComputeVmed", ("Vx_temp",)),
+    ("ComputeResidual", ("dt",)),
+    ("VanLeerX_a", ("Density",)),
+    ("VanLeerX_b", ("dt", "Density", "DensStar", "Vx")),
+    ("VanLeerX_PPA_a", ("DivRho",)),
+    ("VanLeerX_PPA_b", ("DivRho",)),
+    ("VanLeerX_PPA_c", ("DivRho",)),
+    ("VanLeerX_PPA_d", ("dt", "DivRho", "Qs", "Vx_temp")),
+    ("UpdateX", ("dt", "Mpx", "Qs", "Vx")),
+    ("UpdateDensityX", ("dt", "Density", "Vx")),
+    ("AdvectSHIFT", ("Mpx", "Nshift")),
+    ("NewVelocity_x", ()),
+    ("NewVelocity_y", ()),
+    ("StockholmBoundary", ("dt",)),
+    ("ComputeTotalDensity", ()),
+]
+
+ROUTINES_3D = [
+    ("SubStep1_z", ("dt",)),
+    ("visctensor_sph", ()),
+    ("addviscosity_sph", ("dt",)),
+    ("momenta_z", ()),
+    ("VanLeerZ_a", ("Density",)),
+    ("VanLeerZ_b", ("dt", "Density", "DensStar")),
+    ("UpdateZ", ("dt", "Mpz", "Qs")),
+    ("UpdateDensityZ", ("dt", "Density")),
+    ("NewVelocity_z", ()),
+]
+
+
+def _args(o, s, args, dt):
+    oa, sa = [], []
+    for a in args:
+        if a == "dt":
+            oa.append(dt)
+            sa.append(dt)
+        elif a == "Nshift":
+            oa.append(o.L.orc_nshift(o._s))
+            sa.append(ctypes.c_void_p(s.device_ptr("Nshift")))
+        else:
+            oa.append(a)
+            sa.append(a)
+    return oa, sa
+
+
+def _routine_sweep(fa, orc, par, ov, routines, nwarm):
+    o, s = make_pair(fa, orc, par, ov)
+    dt = 0.0
+    for _ in range(nwarm):
+        dt = o.step()
+    failures = []
+    for name, args in routines:
+        copy_state(o, s)
+        oa, sa = _args(o, s, args, dt)
+        say("routine", name, args)
+        o.call(name, *oa)
+        s.call(name, *sa)
+        s.sync()
+        try:
+            compare_all(o, s, name, exact=False)
+        except AssertionError as e:
+            failures.append(str(e))
+    assert not failures, "\n".join(failures)
+
+
+def test_routines_2d(fa, orc):
+    _routine_sweep(fa, orc, C1, "", ROUTINES_2D, nwarm=3)
+
+
+def test_routines_3d(fa, orc):
+    _routine_sweep(fa, orc, C2, C2_SMALL, ROUTINES_2D[:5] + [("SubStep2_a", ("dt",)), ("SubStep2_b", ("dt",)),
+                                                            ("boundaries", ()), ("momenta_x", ()),
+                                                            ("ComputeResidual", ("dt",)),
+                                                            ("AdvectSHIFT", ("Mpx", "Nshift")),
+                                                            ("NewVelocity_x", ()), ("NewVelocity_y", ()),
+                                                            ("StockholmBoundary", ("dt",))] + ROUTINES_3D,
+                   nwarm=2)
+
+
+def test_reductions_and_cfl(fa, orc):
+    o, s = make_pair(fa, orc, C1)
+    for _ in range(2):
+        o.step()
+    copy_state(o, s)
+    # cfl (per-cell dt + MIN + shear limit): order-independent MIN, exact
+    o.call("ComputeVmed", "Vx")
+    s.call("ComputeVmed", "Vx")
+    o.call("cfl")
+    m = ctypes.c_double()
+    s.call("cfl", ctypes.byref(m))
+    omin = o.L.orc_field  # noqa: F841 (keeps the symbol table warm)
+    # the oracle stores its per-fluid Min in the state; recompute via StepTime
+    o.L.orc_CflFluidsMin.restype = ctypes.c_int
+    o.L.orc_CflFluidsMin.argtypes = [ctypes.c_void_p]
+    o.L.orc_CflFluidsMin(o._s)
+    assert m.value == o.scalars()["StepTime"], (m.value, o.scalars()["StepTime"])
+    # full reductions in sequential ring order: bit-identical
+    o.L.orc_reduction_full_SUM.restype = ctypes.c_double
+    o.L.orc_reduction_full_SUM.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)] + [ctypes.c_int] * 4
+    ref = o.L.orc_reduction_full_SUM(o._s, o.L.orc_field(o._s, b"Density"), 3, s.ny + 3, 0, 1)
+    got = ctypes.c_double()
+    s.call("reduction_full_SUM", "Density", 3, s.ny + 3, 0, 1, ctypes.byref(got))
+    assert got.value == ref
+
+
+@pytest.mark.parametrize("par,ov,nsteps", [(C1, "", 20), (C2, C2_SMALL, 6)], ids=["c1_2d", "c2_3d_small"])
+def test_whole_step_routines(fa, orc, par, ov, nsteps):
+    """The C driver over the per-routine kernels reproduces the CPU run step by step."""
+    o, s = make_pair(fa, orc, par, ov, arch={"FusedCfl": "no", "FusedSources": "no", "FusedTransport": "no"})
+    for n in range(1, nsteps + 1):
+        dto, dts = o.step(), s.step()
+        assert dto == dts, "step %d: dt %r vs %r" % (n, dts, dto)
+        if n in (1, 2, 5, nsteps):
+            worst = 0.0
+            for f in ["Density", "Vx", "Vy"] + (["Vz"] if s.dimz else []):
+                a, b = active(s.field(f), s.dimz), active(o.field(f), s.dimz)
+                e = relerr(a, b)
+                say("step %d %s rel %.3e bits %d" % (n, f, e, nbits(a, b)))
+                assert e <= RTOL, (n, f, e)
+                worst = max(worst, e)
+            assert np.array_equal(s.nshift(), o.nshift())
+    po, ps = o.planets(), s.planets()
+    assert np.array_equal(po, ps)
