@@ -1,0 +1,156 @@
+"""Benchmark of FARGO3D's hydro time step on one MI355X.
+
+Workload: the 3-D spherical isothermal disk with an embedded planet
+(256 azimuth x 96 radius x 32 colatitude cells, FP64, FARGO orbital
+advection, alpha viscosity, Stockholm damping), synthetic initial conditions
+from the setup (vertical hydrostatic equilibrium), the fused HIP path.
+
+One "step" is one pass of the DT loop body (src/main.c:361-448): CFL time
+step, sources, transport, damping and ghost fill, including the one device->
+host read of dt the host needs for the planet integrator.  Fields are device
+resident before the timed region starts.
+
+Prints one JSON line (see DESIGN.md for the figures it reports).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+os.chdir(ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PAR = {"c1": "fargo3d_amd/setups/c1_fargo2d.par", "c2": "fargo3d_amd/setups/c2_p3d.par"}
+WORKLOAD = {
+    "c1": "2-D polar isothermal disk 384x128, FP64, FARGO, nu=1e-5, 1e-3 Mstar planet",
+    "c2": "3-D spherical isothermal disk 256x96x32, FP64, FARGO, alpha=1e-3, 3e-4 Mstar planet",
+}
+# Algorithmic bytes per active cell and launch of each fused stage: every
+# array the stage must read once and write once at 8 bytes (FP64).
+# 2-D / 3-D counts: reads + writes.
+STAGE_BYTES = {
+    "cfl": (3 * 8, 4 * 8),                 # cs, vx, vy (, vz)
+    "sources_a": ((3 + 5) * 8, (5 + 9) * 8),  # rho, v (, cs) -> stresses + q
+    "sources_b": ((6 + 5 + 2) * 8, (7 + 9 + 3) * 8),
+    "sweep_z": (0, (5 + 7) * 8),           # rho, v_temp -> rho + 6 momenta
+    "sweep_y": ((3 + 5) * 8, (1 + 7 + 7) * 8),
+    "sweep_x": ((6 + 4) * 8, (8 + 6) * 8),  # rho, momenta, vx_temp -> rho, vx, momenta
+    "finish": ((4 + 2 + 3) * 8, (6 + 3 + 4) * 8),
+    "potential": (1 * 8, 1 * 8),
+}
+
+
+def cpu_baseline(parfile, budget_s=12.0):
+    """The CPU restatement (oracle) on one host core, a bounded sample."""
+    import oracle
+    o = oracle.Oracle(parfile)
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        o.step()
+        n += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or n >= 200:
+            break
+    return n, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="c2", choices=sorted(PAR))
+    ap.add_argument("--routines", action="store_true", help="one kernel per routine instead of the fused path")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; the multi-GPU decomposition is a design only "
+              "(radial bands with halo exchange), so --gpus %d is not supported." % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import fargo3d_amd as fa
+    L = fa.lib()
+    L.f3d_hip_timing_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.f3d_hip_timing_read.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.f3d_hip_stage_name.restype = ctypes.c_char_p
+    nst = L.f3d_hip_stage_count()
+    names = [L.f3d_hip_stage_name(s).decode() for s in range(nst)]
+
+    parfile = PAR[args.config]
+    arch = {}
+    if args.routines:
+        arch = {"FusedCfl": "no", "FusedSources": "no", "FusedTransport": "no"}
+    sim = fa.Simulation(parfile, arch=arch)
+    cells = sim.nx * sim.ny * (sim.nz if sim.dimz else 1)
+    for _ in range(args.warmup):
+        sim.step()
+    sim.sync()
+    L.f3d_hip_timing_enable(sim.ctx, 1)
+    sim.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.step()
+    sim.sync()
+    elapsed = time.perf_counter() - t0
+    ms = (ctypes.c_double * nst)()
+    cnt = (ctypes.c_long * nst)()
+    L.f3d_hip_timing_read(sim.ctx, ms, cnt)
+    L.f3d_hip_timing_enable(sim.ctx, 0)
+
+    value = cells * args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    b_alg = 72 if sim.dimz else 56  # bytes per cell-update of the state (FP64)
+    stages = {}
+    for s in range(nst):
+        if cnt[s]:
+            stages[names[s]] = {"avg_us": round(1e3 * ms[s] / cnt[s], 3), "launches": int(cnt[s])}
+    roofline = None
+    if stages:
+        dom = max(stages, key=lambda n: stages[n]["avg_us"] * stages[n]["launches"])
+        per_cell = STAGE_BYTES.get(dom, (0, 0))[1 if sim.dimz else 0]
+        avg_s = stages[dom]["avg_us"] * 1e-6
+        achieved = per_cell * cells / avg_s / 1e9 if avg_s > 0 else 0.0
+        roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "bytes_per_cell": per_cell}
+    step_gbs = value * b_alg / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        n, el = cpu_baseline(parfile, args.cpu_budget)
+        cpu = {"value": round(cells * n / el, 1), "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": "%d steps of the same workload by the CPU restatement (oracle/), one core, %.1f s"
+                         % (n, el)}
+
+    line = {
+        "metric": "cell-updates/s on 1 MI355X (FP64 vs FP32+FP64-fix vs batched), % HBM roofline",
+        "value": round(value, 1),
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (setup initial conditions)",
+        "config": {"workload": WORKLOAD[args.config], "form": "fp64 routines" if args.routines else "fp64 fused",
+                   "cells": cells, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "step_roofline": {"bytes_per_cell_update": b_alg, "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+        "stages": stages,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -426,16 +426,15 @@ struct CflPar {
 template <int G, int DZ>
 __device__ __forceinline__ double cfl_cell(const dmesh& m, int i, int j, int k, int l, const double* cs,
                                            const double* rho, const double* vx, const double* vy, const double* vz,
-                                           const double* vxmed, const CflPar& cp) {
+                                           double vxmed, const CflPar& cp) {
   const int lxp = lxp_of(l, i, m.nx), lyp = l + m.pitch, lzp = l + m.stride;
-  const int l2 = j + k * m.sy;
   double vxx, vxxp;
   if (!cp.fargo) {
     vxx = vx[l];
     vxxp = vx[lxp];
   } else {
-    vxx = vx[l] - vxmed[l2];
-    vxxp = vx[lxp] - vxmed[l2];
+    vxx = vx[l] - vxmed;
+    vxxp = vx[lxp] - vxmed;
   }
   double soundspeed2, viscosity;
   if (cp.iso) soundspeed2 = cs[l] * cs[l];

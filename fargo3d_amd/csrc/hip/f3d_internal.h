@@ -8,6 +8,21 @@
 #include "../../../include/fargo3d_hip.h"
 #include "f3d_device.h"
 
+// stages of the fused step, in launch order (names in f3d_runtime.hip)
+enum {
+  F3D_ST_CFL = 0,
+  F3D_ST_SRC_A,
+  F3D_ST_SRC_B,
+  F3D_ST_BC_MID,
+  F3D_ST_SWEEP_Z,
+  F3D_ST_SWEEP_Y,
+  F3D_ST_SWEEP_X,
+  F3D_ST_FINISH,
+  F3D_ST_BC_END,
+  F3D_ST_POTENTIAL,
+  F3D_NSTAGES
+};
+
 struct f3d_hip_ctx {
   hipStream_t stream = nullptr;
   int device = 0;
@@ -24,7 +39,34 @@ struct f3d_hip_ctx {
   double* hres = nullptr;           // pinned host result slots (64 doubles)
   size_t n3 = 0, n2 = 0;
   char err[512] = {0};
+  // per-stage event timing of the fused kernels (f3d_hip_timing_*)
+  int timing = 0;
+  hipEvent_t ev[F3D_NSTAGES][2] = {};
+  int pending[F3D_NSTAGES] = {};
+  double stage_ms[F3D_NSTAGES] = {};
+  long stage_n[F3D_NSTAGES] = {};
 };
+
+// Event pair around one launch of stage s when timing is on; the previous
+// pair of the same stage is complete by then (every step synchronises on dt).
+inline void f3d_stage_begin(f3d_hip_ctx* c, int s) {
+  if (!c->timing) return;
+  if (c->pending[s]) {
+    float ms = 0.f;
+    hipEventSynchronize(c->ev[s][1]);
+    if (hipEventElapsedTime(&ms, c->ev[s][0], c->ev[s][1]) == hipSuccess) {
+      c->stage_ms[s] += ms;
+      c->stage_n[s] += 1;
+    }
+    c->pending[s] = 0;
+  }
+  hipEventRecord(c->ev[s][0], c->stream);
+}
+inline void f3d_stage_end(f3d_hip_ctx* c, int s) {
+  if (!c->timing) return;
+  hipEventRecord(c->ev[s][1], c->stream);
+  c->pending[s] = 1;
+}
 
 #define F3D_CHECK(ctx, call)                                                               \
   do {                                                                                     \

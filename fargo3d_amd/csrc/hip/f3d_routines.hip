@@ -463,7 +463,7 @@ template <int G, int DZ>
 __global__ void k_cfl(dmesh m, Range r, const double* cs, const double* rho, const double* vx, const double* vy,
                       const double* vz, const double* vxmed, double* dtime, cell::CflPar cp) {
   CELL_INDEX;
-  dtime[l] = cell::cfl_cell<G, DZ>(m, i, j, k, l, cs, rho, vx, vy, vz, vxmed, cp);
+  dtime[l] = cell::cfl_cell<G, DZ>(m, i, j, k, l, cs, rho, vx, vy, vz, vxmed[j + k * m.sy], cp);
 }
 
 // ---- reductions (src/reduction_generic.c:5-23, sequential i order per ring)
@@ -699,7 +699,9 @@ int f3d_Potential(f3d_hip_ctx* ctx) {
   NEED(ctx->f.Pot, "Pot");
   F3D_REQUIRE(ctx, ctx->ph.nplanets <= cell::kMaxPlanets, "Potential: too many planets");
   Range r = rng(ctx, 0, ctx->m.sy, 0, ctx->m.sz);
+  f3d_stage_begin(ctx, F3D_ST_POTENTIAL);
   DISPATCH(ctx, k_potential, r, ctx->f.Pot, planet_set(ctx));
+  f3d_stage_end(ctx, F3D_ST_POTENTIAL);
   return 0;
 }
 
@@ -739,8 +741,17 @@ static cell::BcPar bc_par(const f3d_hip_ctx* ctx, int side) {
 }
 
 int f3d_boundaries(f3d_hip_ctx* ctx) {
+  return f3d_boundaries_on(ctx, ctx->f.Density, ctx->f.Vx, ctx->f.Vy, ctx->f.Vz);
+}
+
+// The ghost fill on explicit buffers (the fused path fills V_temp directly).
+int f3d_boundaries_on(f3d_hip_ctx* ctx, double* dens, double* vx, double* vy, double* vz) {
   const dmesh& m = ctx->m;
-  const f3d_fields& f = ctx->f;
+  f3d_fields f = ctx->f;
+  f.Density = dens;
+  f.Vx = vx;
+  f.Vy = vy;
+  f.Vz = vz;
   dim3 blk(BX);
   if (!ctx->ph.periodicz && m.dimz) {
     F3D_REQUIRE(ctx, ctx->ph.bc == 2, "3-D boundaries need the spherical-disk family");

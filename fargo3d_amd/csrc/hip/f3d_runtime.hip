@@ -76,9 +76,50 @@ int f3d_hip_create(const f3d_mesh_desc* d, f3d_hip_ctx** out) {
   return 0;
 }
 
+static const char* kStageNames[F3D_NSTAGES] = {"cfl", "sources_a", "sources_b", "ghosts_mid", "sweep_z",
+                                                "sweep_y", "sweep_x", "finish", "ghosts_end", "potential"};
+
+int f3d_hip_stage_count(void) { return F3D_NSTAGES; }
+const char* f3d_hip_stage_name(int s) { return (s >= 0 && s < F3D_NSTAGES) ? kStageNames[s] : ""; }
+
+int f3d_hip_timing_enable(f3d_hip_ctx* c, int on) {
+  if (on && !c->ev[0][0]) {
+    for (int s = 0; s < F3D_NSTAGES; s++)
+      for (int e = 0; e < 2; e++) F3D_CHECK(c, hipEventCreate(&c->ev[s][e]));
+  }
+  F3D_CHECK(c, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < F3D_NSTAGES; s++) {
+    c->pending[s] = 0;
+    c->stage_ms[s] = 0.0;
+    c->stage_n[s] = 0;
+  }
+  c->timing = on;
+  return 0;
+}
+
+int f3d_hip_timing_read(f3d_hip_ctx* c, double* ms, long* counts) {
+  F3D_CHECK(c, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < F3D_NSTAGES; s++) {
+    if (c->pending[s]) {
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, c->ev[s][0], c->ev[s][1]) == hipSuccess) {
+        c->stage_ms[s] += t;
+        c->stage_n[s] += 1;
+      }
+      c->pending[s] = 0;
+    }
+    ms[s] = c->stage_ms[s];
+    counts[s] = c->stage_n[s];
+  }
+  return 0;
+}
+
 void f3d_hip_destroy(f3d_hip_ctx* c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
+  for (int s = 0; s < F3D_NSTAGES; s++)
+    for (int e = 0; e < 2; e++)
+      if (c->ev[s][e]) hipEventDestroy(c->ev[s][e]);
   if (c->meshbuf) hipFree(c->meshbuf);
   if (c->bctab) hipFree(c->bctab);
   if (c->dred) hipFree(c->dred);

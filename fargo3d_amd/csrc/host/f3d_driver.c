@@ -143,7 +143,7 @@ static int apply_overrides(f3d_params *p, const char *ov, char *err, int errlen)
 f3d_sim *f3d_sim_create(const char *parfile, const char *overrides, char *err, int errlen) {
   f3d_sim *S = (f3d_sim *)calloc(1, sizeof(f3d_sim));
   f3d_params_defaults(&S->p);
-  S->arch.fused_cfl = S->arch.fused_sources = S->arch.fused_transport = 0;
+  S->arch.fused_cfl = S->arch.fused_sources = S->arch.fused_transport = 1;
   if (parfile && parfile[0] && f3d_params_read(&S->p, parfile, err, (size_t)errlen) != 0) goto fail;
   if (apply_overrides(&S->p, overrides, err, errlen) != 0) goto fail;
   f3d_params_finalize(&S->p);
@@ -520,14 +520,17 @@ int f3d_sim_step(f3d_sim *S, double *dt_out) {
     if (sources_routines(S, dt)) return -1;
   }
   if (S->arch.fused_transport) {
-    CALL(S, f3d_fused_transport(c, dt, 0));
+    /* transport, NewVelocity, Stockholm damping and the ghost fill */
+    CALL(S, f3d_fused_transport(c, dt, 1));
+    S->clk.PhysicalTime += dt;
+    S->clk.Timestepcount++;
   } else {
     if (transport_routines(S, dt)) return -1;
+    S->clk.PhysicalTime += dt;
+    S->clk.Timestepcount++;
+    if (S->p.stockholm) CALL(S, f3d_StockholmBoundary(c, dt));
+    if (fill_ghosts(S)) return -1;
   }
-  S->clk.PhysicalTime += dt;
-  S->clk.Timestepcount++;
-  if (S->p.stockholm) CALL(S, f3d_StockholmBoundary(c, dt));
-  if (fill_ghosts(S)) return -1;
   S->dt_last = dt;
   if (dt_out) *dt_out = dt;
   return 0;

@@ -95,6 +95,12 @@ int f3d_hip_memcpy_d2h(f3d_hip_ctx *ctx, void *dst, const void *src, size_t byte
 int f3d_hip_memcpy_d2d(f3d_hip_ctx *ctx, void *dst, const void *src, size_t bytes);
 int f3d_hip_memset(f3d_hip_ctx *ctx, void *dst, int value, size_t bytes);
 int f3d_hip_device_count(void);
+/* HIP-event timing of the fused stages and Potential on the handle's stream
+ * (no counterpart in the reference; used by the benchmark) */
+int f3d_hip_stage_count(void);
+const char *f3d_hip_stage_name(int stage);
+int f3d_hip_timing_enable(f3d_hip_ctx *ctx, int on);
+int f3d_hip_timing_read(f3d_hip_ctx *ctx, double *ms, long *counts);
 
 /* ---- per-routine entry points (reference pointer table, src/global.h:303-416) ---- */
 /* ComputePressureFieldIso/Ad: src/CompPresIso.c:10, src/CompPresAd.c:10 */
@@ -127,6 +133,8 @@ int f3d_ComputeForce(f3d_hip_ctx *ctx, double x, double y, double z, double rsmo
 int f3d_copy_velocities(f3d_hip_ctx *ctx, int option);
 /* boundaries: src/boundary.c:2 (zmin, zmax, ymin, ymax of the bound family) */
 int f3d_boundaries(f3d_hip_ctx *ctx);
+/* the same ghost fill on explicit buffers (e.g. V_temp during the mid-step fill) */
+int f3d_boundaries_on(f3d_hip_ctx *ctx, double *Density, double *Vx, double *Vy, double *Vz);
 /* momenta_x/y/z: src/momenta_x.c:10, src/momenta_y.c:10, src/momenta_z.c:10 */
 int f3d_momenta_x(f3d_hip_ctx *ctx);
 int f3d_momenta_y(f3d_hip_ctx *ctx);
