@@ -127,6 +127,7 @@ int *orc_nshift(orc_state *S);
 void orc_get_scalars(orc_state *S, double *out8);
 void orc_set_scalars(orc_state *S, const double *in8);
 void orc_get_planets(orc_state *S, double *out, int maxn);
+void orc_psys_integrate(orc_state *S, double t, int nsub);
 
 #ifdef __cplusplus
 }

@@ -400,3 +400,9 @@ void orc_get_planets(orc_state *S, double *out, int maxn) {
     out[8 * i + 6] = S->sys.mass[i]; out[8 * i + 7] = (double)S->sys.nb;
   }
 }
+
+/* Planet-only integration for the orbit tests: advance the planetary system
+ * by time t in nsub RK5 substeps (src/runge_kutta.c), no disk, no frame change. */
+void orc_psys_integrate(orc_state *S, double t, int nsub) {
+  for (int s = 0; s < nsub; s++) f3d_psys_rk5(&S->sys, &S->p, &S->clk, t / nsub);
+}
