@@ -6,15 +6,15 @@
 // values with the same per-cell arithmetic (f3d_cell.h) but keep
 // intermediates on chip:
 //
-//   kf_cfl         ring-owning: VxMed (sequential ring sum), per-cell CFL,
-//                  ring minima, and in the last workgroup the global MIN and
-//                  shear limit -> dt (one launch, one 8-byte copy to host)
-//   kf_src_a       viscous stress, artificial pressure and potential at
-//                  every cell the next kernel reads
+//   kf_cfl         ring-owning: VxMed of the ring and of the next ring
+//                  (sequential sums, in parallel), per-cell CFL, shear limit,
+//                  one atomic MIN per ring -> dt (one launch, 8 bytes to host)
+//   kf_src_a       viscous stress and artificial pressure at every cell the
+//                  next kernel reads
 //   kf_src_b       SubStep1 + addviscosity + SubStep2_b on active cells
 //   (boundaries)   mid-step ghost fill on V_temp
 //   kf_sweep_z     3-D: momenta + Z sweep, one thread per (i,j) column
-//                  marching in k with a register window
+//                  marching in k over a register window
 //   kf_sweep_y     momenta (2-D) + Y sweep, columns marching in j
 //   kf_sweep_x     ring-owning: VxMed, residual, van Leer and PPA X sweeps,
 //                  integer shift and NewVelocity_x in LDS
@@ -43,13 +43,14 @@ enum { FS_RHOY = 0, FS_RHOX, FS_QX, FS_QY, FS_QZ, FS_Z0, FS_COUNT = FS_Z0 + 7 };
 
 struct Scratch {
   double* f[FS_COUNT];
-  double* ringmin;
-  unsigned* counter;
-  double* result;
+  unsigned long long* result;  // two slots, alternated per step
+  int slot;
 };
 
 Scratch g_scr[16];
 void* g_owner[16];
+
+constexpr unsigned long long kInfBits = 0x7FF0000000000000ull;
 
 int scratch_of(f3d_hip_ctx* ctx, Scratch** out) {
   int slot = -1;
@@ -65,10 +66,9 @@ int scratch_of(f3d_hip_ctx* ctx, Scratch** out) {
       if (hipMalloc(&S.f[a], ctx->n3 * sizeof(double)) != hipSuccess) return -1;
       if (hipMemset(S.f[a], 0, ctx->n3 * sizeof(double)) != hipSuccess) return -1;
     }
-    if (hipMalloc(&S.ringmin, (ctx->n2 + 64) * sizeof(double)) != hipSuccess) return -1;
-    if (hipMalloc(&S.counter, 64 * sizeof(unsigned)) != hipSuccess) return -1;
-    if (hipMemset(S.counter, 0, 64 * sizeof(unsigned)) != hipSuccess) return -1;
-    if (hipMalloc(&S.result, 64 * sizeof(double)) != hipSuccess) return -1;
+    if (hipMalloc(&S.result, 2 * sizeof(unsigned long long)) != hipSuccess) return -1;
+    unsigned long long init[2] = {kInfBits, kInfBits};
+    if (hipMemcpy(S.result, init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess) return -1;
     g_owner[slot] = ctx;
   }
   *out = &g_scr[slot];
@@ -100,70 +100,71 @@ __device__ double block_min(double v, double* sh) {
   return r;
 }
 
+// Sequential sum in index order (the reduction order of src/reduction_generic.c
+// :18-20); eight LDS loads are issued ahead of their dependent additions.
+__device__ __forceinline__ double seq_sum(const double* w, int n) {
+  double acc = 0.0;
+  int i = 0;
+  for (; i + 8 <= n; i += 8) {
+    double t0 = w[i], t1 = w[i + 1], t2 = w[i + 2], t3 = w[i + 3];
+    double t4 = w[i + 4], t5 = w[i + 5], t6 = w[i + 6], t7 = w[i + 7];
+    acc = acc + t0;
+    acc = acc + t1;
+    acc = acc + t2;
+    acc = acc + t3;
+    acc = acc + t4;
+    acc = acc + t5;
+    acc = acc + t6;
+    acc = acc + t7;
+  }
+  for (; i < n; i++) acc = acc + w[i];
+  return acc;
+}
+
 }  // namespace
 
 // ----------------------------------------------------------------- kf_cfl
-// One workgroup per ring (j,k), j in [nghy, ny+nghy] (the extra ring feeds the
-// shear limit of the last active ring), k active.
+// One workgroup per active ring (j,k).  The MIN over cells and rings is
+// order-independent, so one atomic MIN per ring on the bit pattern of the
+// (positive) time steps gives the value of reduction_full_MIN + cfl_b
+// (src/cfl_b.c:6-37) exactly.
 template <int G, int DZ>
 __global__ void __launch_bounds__(WG) kf_cfl(dmesh m, const double* cs, const double* rho, const double* vx,
-                                             const double* vy, const double* vz, double* vxmed, double* ringmin,
-                                             unsigned* counter, double* result, cell::CflPar cp) {
-  extern __shared__ double lds[];  // nx weights + 8 for reductions
-  double* w = lds;
-  double* sh = lds + m.nx;
-  const int nrj = m.ny + 1;
-  const int j = m.nghy + (int)(blockIdx.x % nrj);
-  const int k = DZ ? m.nghz + (int)(blockIdx.x / nrj) : 0;
+                                             const double* vy, const double* vz, double* vxmed,
+                                             unsigned long long* result, int slot, cell::CflPar cp) {
+  extern __shared__ double lds[];  // 2 nx weights + 16
+  double* w0 = lds;
+  double* w1 = lds + m.nx;
+  double* sh = lds + 2 * m.nx;
+  const int j = m.nghy + (int)(blockIdx.x % m.ny);
+  const int k = DZ ? m.nghz + (int)(blockIdx.x / m.ny) : 0;
   const int base = j * m.pitch + k * m.stride;
   const int l2 = j + k * m.sy;
-  // ComputeVweight + reduction_SUM: Qs = v*(xmin(i+1)-xmin(i)), summed in i order
-  for (int i = threadIdx.x; i < m.nx; i += blockDim.x) w[i] = vx[base + i] * (m.xmin[i + 1] - m.xmin[i]);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double acc = 0.0;
-    for (int i = 0; i < m.nx; i++) acc = acc + w[i];
-    double vm = acc / (m.XMAX - m.XMIN);
-    vxmed[l2] = vm;
-    sh[8] = vm;
+  if (blockIdx.x == 0 && threadIdx.x == 0) result[slot ^ 1] = kInfBits;  // the next step's slot
+  // ComputeVweight + reduction_SUM (src/compute_vmed.c:14-30) of rings j and j+1
+  for (int i = threadIdx.x; i < m.nx; i += blockDim.x) {
+    const double dx = m.xmin[i + 1] - m.xmin[i];
+    w0[i] = vx[base + i] * dx;
+    if (cp.fargo) w1[i] = vx[base + m.pitch + i] * dx;
   }
   __syncthreads();
+  if (threadIdx.x == 0) sh[8] = seq_sum(w0, m.nx) / (m.XMAX - m.XMIN);
+  if (threadIdx.x == 64 && cp.fargo) sh[9] = seq_sum(w1, m.nx) / (m.XMAX - m.XMIN);
+  __syncthreads();
+  const double vm0 = sh[8];
   double mn = 1e20;
-  if (j < m.ny + m.nghy) {
-    for (int i = threadIdx.x; i < m.nx; i += blockDim.x) {
-      double d = cell::cfl_cell<G, DZ>(m, i, j, k, base + i, cs, rho, vx, vy, vz, sh[8], cp);
-      mn = (d < mn) ? d : mn;
-    }
+  for (int i = threadIdx.x; i < m.nx; i += blockDim.x) {
+    double d = cell::cfl_cell<G, DZ>(m, i, j, k, base + i, cs, rho, vx, vy, vz, vm0, cp);
+    mn = (d < mn) ? d : mn;
   }
   mn = block_min(mn, sh);
-  if (threadIdx.x == 0) ringmin[blockIdx.x] = mn;
-  // last workgroup: global MIN + shear limit (src/cfl_b.c:6-37)
-  __shared__ int last;
-  __threadfence();
-  if (threadIdx.x == 0) last = (atomicAdd(counter, 1u) == gridDim.x - 1);
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double r = 1e20;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-    double a = __hip_atomic_load(&ringmin[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    r = (a < r) ? a : r;
-  }
-  if (cp.fargo) {
-    const int nk = DZ ? m.nz : 1;
-    for (int t = threadIdx.x; t < m.ny * nk; t += blockDim.x) {
-      int jj = m.nghy + t % m.ny, kk = DZ ? m.nghz + t / m.ny : 0;
-      int q2 = jj + kk * m.sy;
-      double a = __hip_atomic_load(&vxmed[q2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      double b = __hip_atomic_load(&vxmed[q2 + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      double shearlimit = cp.cfl * m.mindx / fabs(a / m.ymed[jj] - b / m.ymed[jj + 1]);
-      r = (shearlimit < r) ? shearlimit : r;
-    }
-  }
-  r = block_min(r, sh);
   if (threadIdx.x == 0) {
-    result[0] = r;
-    *counter = 0u;  // ready for the next call, no memset needed
+    vxmed[l2] = vm0;
+    if (cp.fargo) {
+      const double shearlimit = cp.cfl * m.mindx / fabs(vm0 / m.ymed[j] - sh[9] / m.ymed[j + 1]);
+      if (shearlimit < mn) mn = shearlimit;
+    }
+    atomicMin(&result[slot], (unsigned long long)__double_as_longlong(mn));
   }
 }
 
@@ -202,14 +203,13 @@ __device__ __forceinline__ VxcLoader<G> vxc(const double* vx, const dmesh& m, in
   return VxcLoader<G>{cell::gl(vx, m, l, i), m.ymed, m.sinzmed, j, k, dom};
 }
 
-// kf_src_a: stresses (visctensor ranges j,k in [1, s-2]), artificial pressure
-// (SubStep2_a) and the potential, at every cell the active update reads.
+// kf_src_a: stresses (visctensor ranges j,k in [1, s-2]) and artificial
+// pressure (SubStep2_a) at every cell the active update reads.
 template <int G, int DZ>
 __global__ void __launch_bounds__(WG) kf_src_a(dmesh m, const double* rho, const double* energy, const double* vx,
                                                const double* vy, const double* vz, double* txx, double* tyy,
                                                double* tzz, double* txz, double* tyx, double* tzy, double* qx,
-                                               double* qy, double* qz, double* pot, cell::ViscPar vp, int visc,
-                                               cell::PlanetSet ps, int potential, double domega) {
+                                               double* qy, double* qz, cell::ViscPar vp, int visc, double domega) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int j = 1 + (int)blockIdx.y;       // 1 .. sy-2
@@ -236,7 +236,6 @@ __global__ void __launch_bounds__(WG) kf_src_a(dmesh m, const double* rho, const
   qx[l] = cell::art_q(r0, VX(1, 0, 0) - VX(0, 0, 0));
   qy[l] = cell::art_q(r0, VY(0, 1, 0) - VY(0, 0, 0));
   if (DZ) qz[l] = cell::art_q(r0, VZ(0, 0, 1) - VZ(0, 0, 0));
-  if (potential) pot[l] = cell::potential<G>(m, i, j, k, ps);
 }
 
 // kf_src_b: the velocity update of SubStep1 + addviscosity + SubStep2_b on
@@ -279,360 +278,452 @@ __global__ void __launch_bounds__(WG) kf_src_b(dmesh m, double dt, const double*
 }
 
 // --------------------------------------------------------------- transport
-// Y sweep (2-D, with momenta): thread per column i, rows [jb, je) active.
-// Q order: Mpx, Mmx, Mpy, Mmy; the density is updated last with DensStar.
-template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_sweep_y(dmesh m, double dt, const double* rho_in, const double* vxt,
-                                                 const double* vyt, const double* qin0, const double* qin1,
-                                                 const double* qin2, const double* qin3, const double* qin4,
-                                                 const double* qin5, double* rho_out, double* q0, double* q1,
-                                                 double* q2, double* q3, double* q4, double* q5, int rows_per,
-                                                 double omegaframe, int with_momenta) {
-  constexpr int NQ = DZ ? 6 : 4;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m.nx) return;
-  const int k = DZ ? m.nghz + (int)blockIdx.z : 0;
-  const int jb = m.nghy + (int)blockIdx.y * rows_per;
-  const int je = min(jb + rows_per, m.ny + m.nghy);
-  const int ixp = ixp_of(i, m.nx);
-  const int kofs = k * m.stride;
-  const double* qin[6] = {qin0, qin1, qin2, qin3, qin4, qin5};
-  double* qout[6] = {q0, q1, q2, q3, q4, q5};
-  // value of quantity c (or the density, c = -1) at row jj
-  auto load = [&](int c, int jj) -> double {
-    const int l = i + jj * m.pitch + kofs;
-    if (c < 0) return rho_in[l];
-    if (!with_momenta) return qin[c][l];
-    const double r = rho_in[l];
-    const double ym = m.ymed[jj];
-    switch (c) {
-      case 0: {  // Mpx (src/momenta_x.c:77-78)
-        double mm, mp;
-        cell::momenta_x<G>(m, i, jj, k, r, vxt[l], vxt[i + (ixp - i) + jj * m.pitch + kofs], omegaframe, mm, mp);
-        (void)ym;
-        return mp;
-      }
-      case 1: {  // Mmx
-        double mm, mp;
-        cell::momenta_x<G>(m, i, jj, k, r, vxt[l], vxt[ixp + jj * m.pitch + kofs], omegaframe, mm, mp);
-        return mm;
-      }
-      case 2:  // Mpy (src/momenta_y.c:62)
-        return vyt[l + m.pitch] * r;
-      default:  // Mmy
-        return vyt[l] * r;
-    }
-  };
-  double Qv[NQ + 1][5];  // [NQ] is the density
-  for (int o = 0; o < 4; o++) {
-    const int jj = jb - 2 + o;
-    Qv[NQ][o] = load(-1, jj);
-    for (int c = 0; c < NQ; c++) Qv[c][o] = load(c, jj);
-  }
-  // face values at the face of row jb (needs rows jb-2..jb+1): density first
-  auto face = [&](int c, int jj, const double* qq /*q at jj-2..jj+1*/, const double* rr) -> double {
-    // q = Q/rho for momenta, q = rho for the density (DensStar)
-    double a0, a1, a2, a3;
-    if (c < 0) {
-      a0 = qq[0]; a1 = qq[1]; a2 = qq[2]; a3 = qq[3];
-    } else {
-      a0 = qq[0] / rr[0]; a1 = qq[1] / rr[1]; a2 = qq[2] / rr[2]; a3 = qq[3] / rr[3];
-    }
-    // slopes at rows jj-1 and jj (src/vanleer_y_a.c:63-68)
-    double sm = vanleer_slope((a1 - a0) / zone_size_y(m, jj - 1), (a2 - a1) / zone_size_y(m, jj));
-    double s0 = vanleer_slope((a2 - a1) / zone_size_y(m, jj), (a3 - a2) / zone_size_y(m, jj + 1));
-    const double v = vyt[i + jj * m.pitch + kofs];
-    return vanleer_face(v, dt, a1, sm, zone_size_y(m, jj - 1), a2, s0, zone_size_y(m, jj));
-  };
-  double fprev[NQ + 1], dsprev;
+// One directional van Leer sweep of one column (the Y or Z transport of
+// src/transport.c:81-127): NQ conserved quantities plus the density
+// (index NQ), rows r in [r0, r1).  Per row the quantity q = Q/rho
+// (src/divrho.c:56), the gradient at the lower face (dqm of
+// src/vanleer_y_a.c:63, equal to dqp of the row below), the slope and the
+// upwind face value (src/vanleer_y_b.c:60-65) are each computed once and
+// kept in a window of registers.
+//   col.rho(r)      density at row r (not modified by the sweep)
+//   col.Q(c, r)     conserved quantity c at row r
+//   col.zs(r)       zone size of row r in the sweep direction
+//   col.v(r)        velocity at the lower face of row r
+//   col.flux(r, a, b) conservative increment of row r from face fluxes a, b
+//   col.out(c, r, value) store of the updated quantity (c == NQ: density)
+template <int NQ, class Col>
+__device__ __forceinline__ void column_sweep(Col& col, int r0, int r1, double dt) {
+  constexpr int N = NQ + 1;
+  double A1[N], A2[N], G2[N], S1[N], F0[N], F1[N];
   {
-    double rr[4] = {Qv[NQ][0], Qv[NQ][1], Qv[NQ][2], Qv[NQ][3]};
-    dsprev = face(-1, jb, rr, rr);
-    for (int c = 0; c < NQ; c++) {
-      double qq[4] = {Qv[c][0], Qv[c][1], Qv[c][2], Qv[c][3]};
-      fprev[c] = face(c, jb, qq, rr);
+    double a[N][5];
+#pragma unroll
+    for (int t = 0; t < 5; t++) {
+      const int r = r0 - 2 + t;
+      const double rr = col.rho(r);
+#pragma unroll
+      for (int c = 0; c < NQ; c++) a[c][t] = col.Q(c, r) / rr;
+      a[NQ][t] = rr;
+    }
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      const double g0 = (a[c][1] - a[c][0]) / col.zs(r0 - 1);
+      const double g1 = (a[c][2] - a[c][1]) / col.zs(r0);
+      const double g2 = (a[c][3] - a[c][2]) / col.zs(r0 + 1);
+      const double g3 = (a[c][4] - a[c][3]) / col.zs(r0 + 2);
+      const double sm = vanleer_slope(g0, g1);
+      const double s0 = vanleer_slope(g1, g2);
+      const double s1 = vanleer_slope(g2, g3);
+      F0[c] = vanleer_face(col.v(r0), dt, a[c][1], sm, col.zs(r0 - 1), a[c][2], s0, col.zs(r0));
+      F1[c] = vanleer_face(col.v(r0 + 1), dt, a[c][2], s0, col.zs(r0), a[c][3], s1, col.zs(r0 + 1));
+      A1[c] = a[c][3];
+      A2[c] = a[c][4];
+      G2[c] = g3;
+      S1[c] = s1;
     }
   }
-  for (int jj = jb; jj < je; jj++) {
-    // shift in row jj+2
-    const int jn = jj + 2;
-    Qv[NQ][4] = load(-1, jn);
-    for (int c = 0; c < NQ; c++) Qv[c][4] = load(c, jn);
-    double rr[4] = {Qv[NQ][1], Qv[NQ][2], Qv[NQ][3], Qv[NQ][4]};
-    const double dsn = face(-1, jj + 1, rr, rr);
-    const double vy0 = vyt[i + jj * m.pitch + kofs], vy1 = vyt[i + (jj + 1) * m.pitch + kofs];
-    const int l = i + jj * m.pitch + kofs;
-    for (int c = 0; c < NQ; c++) {
-      double qq[4] = {Qv[c][1], Qv[c][2], Qv[c][3], Qv[c][4]};
-      const double fn = face(c, jj + 1, qq, rr);
-      // UpdateY (src/update_y.c:68-70)
-      qout[c][l] = Qv[c][2] + cell::flux_y(m, i, jj, k, dt, vy0 * fprev[c] * dsprev, vy1 * fn * dsn);
-      fprev[c] = fn;
-    }
-    // UpdateDensityY (src/update_density_y.c:66-68)
-    rho_out[l] = Qv[NQ][2] + cell::flux_y(m, i, jj, k, dt, vy0 * dsprev, vy1 * dsn);
-    dsprev = dsn;
-    for (int o = 0; o < 4; o++) {
-      Qv[NQ][o] = Qv[NQ][o + 1];
-      for (int c = 0; c < NQ; c++) Qv[c][o] = Qv[c][o + 1];
+  for (int r = r0; r < r1; r++) {
+    const double v0 = col.v(r), v1 = col.v(r + 1);
+    // UpdateY/Z (src/update_y.c:68-70), then the density last (src/update_density_y.c:66-68)
+#pragma unroll
+    for (int c = 0; c < NQ; c++)
+      col.out(c, r, col.Q(c, r) + col.flux(r, v0 * F0[c] * F0[NQ], v1 * F1[c] * F1[NQ]));
+    col.out(NQ, r, col.rho(r) + col.flux(r, v0 * F0[NQ], v1 * F1[NQ]));
+    if (r + 1 >= r1) break;
+    // advance the window by one row: row r+3 enters
+    const int rn = r + 3;
+    const double rr = col.rho(rn);
+    const double zsn = col.zs(rn), zs2 = col.zs(r + 2), zs1 = col.zs(r + 1), v2 = col.v(r + 2);
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      const double an = (c < NQ) ? col.Q(c, rn) / rr : rr;
+      const double g3 = (an - A2[c]) / zsn;
+      const double s2 = vanleer_slope(G2[c], g3);
+      const double f2 = vanleer_face(v2, dt, A1[c], S1[c], zs1, A2[c], s2, zs2);
+      F0[c] = F1[c];
+      F1[c] = f2;
+      A1[c] = A2[c];
+      A2[c] = an;
+      G2[c] = g3;
+      S1[c] = s2;
     }
   }
 }
 
-// Z sweep (3-D) with momenta: thread per (i, j) column, j in [1, sy-2], k
-// active in strips; writes the Z-updated density and six momenta.
-template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_sweep_z(dmesh m, double dt, const double* rho_in, const double* vxt,
-                                                 const double* vyt, const double* vzt, double* rho_out, double* q0,
-                                                 double* q1, double* q2, double* q3, double* q4, double* q5,
-                                                 int planes_per, double omegaframe) {
-  constexpr int NQ = 6;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m.nx) return;
-  const int j = 1 + (int)blockIdx.y;
-  const int kb = m.nghz + (int)blockIdx.z * planes_per;
-  const int ke = min(kb + planes_per, m.nz + m.nghz);
-  const int ixp = ixp_of(i, m.nx);
-  double* qout[6] = {q0, q1, q2, q3, q4, q5};
-  auto load = [&](int c, int kk) -> double {
-    const int l = i + j * m.pitch + kk * m.stride;
-    const double r = rho_in[l];
-    if (c < 0) return r;
-    double mm, mp;
+// Y column of the 2-D step: momenta computed from rho and V_temp on the fly
+// (src/momenta_x.c:76-79, src/momenta_y.c:62-63).
+template <int G>
+struct ColY2 {
+  const dmesh* m;
+  const double *rho_in, *vxt, *vyt;
+  double *rho_out, *q0, *q1, *q2, *q3;
+  int i, ixp;
+  double of, dt;
+  __device__ __forceinline__ int L(int r) const { return i + r * m->pitch; }
+  __device__ __forceinline__ double rho(int r) const { return rho_in[L(r)]; }
+  __device__ __forceinline__ double Q(int c, int r) const {
+    const int l = L(r);
+    const double rr = rho_in[l];
+    if (c < 2) {
+      double mm, mp;
+      cell::momenta_x<G>(*m, i, r, 0, rr, vxt[l], vxt[ixp + r * m->pitch], of, mm, mp);
+      return c == 0 ? mp : mm;
+    }
+    return c == 2 ? vyt[l + m->pitch] * rr : vyt[l] * rr;
+  }
+  __device__ __forceinline__ double zs(int r) const { return zone_size_y(*m, r); }
+  __device__ __forceinline__ double v(int r) const { return vyt[L(r)]; }
+  __device__ __forceinline__ double flux(int r, double a, double b) const { return cell::flux_y(*m, i, r, 0, dt, a, b); }
+  __device__ __forceinline__ void out(int c, int r, double val) const {
+    double* o = c == 0 ? q0 : c == 1 ? q1 : c == 2 ? q2 : c == 3 ? q3 : rho_out;
+    o[L(r)] = val;
+  }
+};
+
+// Z column of the 3-D step: momenta from rho and V_temp on the fly
+// (src/momenta_x.c:80-84, src/momenta_y.c:62-63, src/momenta_z.c:66-71).
+template <int G>
+struct ColZ {
+  const dmesh* m;
+  const double *rho_in, *vxt, *vyt, *vzt;
+  double* out_[7];
+  int i, j, ixp;
+  double of, dt;
+  __device__ __forceinline__ int L(int r) const { return i + j * m->pitch + r * m->stride; }
+  __device__ __forceinline__ double rho(int r) const { return rho_in[L(r)]; }
+  __device__ __forceinline__ double Q(int c, int r) const {
+    const int l = L(r);
+    const double rr = rho_in[l];
     switch (c) {
       case 0:
-        cell::momenta_x<G>(m, i, j, kk, r, vxt[l], vxt[ixp + j * m.pitch + kk * m.stride], omegaframe, mm, mp);
-        return mp;
-      case 1:
-        cell::momenta_x<G>(m, i, j, kk, r, vxt[l], vxt[ixp + j * m.pitch + kk * m.stride], omegaframe, mm, mp);
-        return mm;
+      case 1: {
+        double mm, mp;
+        cell::momenta_x<G>(*m, i, j, r, rr, vxt[l], vxt[ixp + j * m->pitch + r * m->stride], of, mm, mp);
+        return c == 0 ? mp : mm;
+      }
       case 2:
-        return vyt[l + m.pitch] * r;
+        return vyt[l + m->pitch] * rr;
       case 3:
-        return vyt[l] * r;
-      case 4: {  // Mpz (src/momenta_z.c:66-71)
-        double p = vzt[l + m.stride] * r;
-        if (G == SPH) p *= m.ymed[j];
+        return vyt[l] * rr;
+      case 4: {
+        double p = vzt[l + m->stride] * rr;
+        if (G == SPH) p *= m->ymed[j];
         return p;
       }
-      default: {  // Mmz
-        double q = vzt[l] * r;
-        if (G == SPH) q *= m.ymed[j];
+      default: {
+        double q = vzt[l] * rr;
+        if (G == SPH) q *= m->ymed[j];
         return q;
       }
     }
-  };
-  double Qv[NQ + 1][5];
-  for (int o = 0; o < 4; o++) {
-    const int kk = kb - 2 + o;
-    Qv[NQ][o] = load(-1, kk);
-    for (int c = 0; c < NQ; c++) Qv[c][o] = load(c, kk);
   }
-  auto face = [&](int c, int kk, const double* qq, const double* rr) -> double {
-    double a0, a1, a2, a3;
-    if (c < 0) {
-      a0 = qq[0]; a1 = qq[1]; a2 = qq[2]; a3 = qq[3];
-    } else {
-      a0 = qq[0] / rr[0]; a1 = qq[1] / rr[1]; a2 = qq[2] / rr[2]; a3 = qq[3] / rr[3];
-    }
-    // src/vanleer_z_a.c:63-68 and src/vanleer_z_b.c:60-65
-    double sm = vanleer_slope((a1 - a0) / zone_size_z<G>(m, j, kk - 1), (a2 - a1) / zone_size_z<G>(m, j, kk));
-    double s0 = vanleer_slope((a2 - a1) / zone_size_z<G>(m, j, kk), (a3 - a2) / zone_size_z<G>(m, j, kk + 1));
-    const double v = vzt[i + j * m.pitch + kk * m.stride];
-    return vanleer_face(v, dt, a1, sm, zone_size_z<G>(m, j, kk - 1), a2, s0, zone_size_z<G>(m, j, kk));
-  };
-  double fprev[NQ], dsprev;
-  {
-    double rr[4] = {Qv[NQ][0], Qv[NQ][1], Qv[NQ][2], Qv[NQ][3]};
-    dsprev = face(-1, kb, rr, rr);
-    for (int c = 0; c < NQ; c++) {
-      double qq[4] = {Qv[c][0], Qv[c][1], Qv[c][2], Qv[c][3]};
-      fprev[c] = face(c, kb, qq, rr);
-    }
-  }
-  for (int kk = kb; kk < ke; kk++) {
-    const int kn = kk + 2;
-    Qv[NQ][4] = load(-1, kn);
-    for (int c = 0; c < NQ; c++) Qv[c][4] = load(c, kn);
-    double rr[4] = {Qv[NQ][1], Qv[NQ][2], Qv[NQ][3], Qv[NQ][4]};
-    const double dsn = face(-1, kk + 1, rr, rr);
-    const double vz0 = vzt[i + j * m.pitch + kk * m.stride], vz1 = vzt[i + j * m.pitch + (kk + 1) * m.stride];
-    const int l = i + j * m.pitch + kk * m.stride;
-    for (int c = 0; c < NQ; c++) {
-      double qq[4] = {Qv[c][1], Qv[c][2], Qv[c][3], Qv[c][4]};
-      const double fn = face(c, kk + 1, qq, rr);
-      qout[c][l] = Qv[c][2] + cell::flux_z(m, i, j, kk, dt, vz0 * fprev[c] * dsprev, vz1 * fn * dsn);
-      fprev[c] = fn;
-    }
-    rho_out[l] = Qv[NQ][2] + cell::flux_z(m, i, j, kk, dt, vz0 * dsprev, vz1 * dsn);
-    dsprev = dsn;
-    for (int o = 0; o < 4; o++) {
-      Qv[NQ][o] = Qv[NQ][o + 1];
-      for (int c = 0; c < NQ; c++) Qv[c][o] = Qv[c][o + 1];
-    }
-  }
+  __device__ __forceinline__ double zs(int r) const { return zone_size_z<G>(*m, j, r); }
+  __device__ __forceinline__ double v(int r) const { return vzt[L(r)]; }
+  __device__ __forceinline__ double flux(int r, double a, double b) const { return cell::flux_z(*m, i, j, r, dt, a, b); }
+  __device__ __forceinline__ void out(int c, int r, double val) const { out_[c][L(r)] = val; }
+};
+
+// Y column of the 3-D step: reads the Z-swept density and momenta.
+struct ColY3 {
+  const dmesh* m;
+  const double* rho_in;
+  const double* q_in[6];
+  const double* vyt;
+  double* out_[7];
+  int i, k;
+  double dt;
+  __device__ __forceinline__ int L(int r) const { return i + r * m->pitch + k * m->stride; }
+  __device__ __forceinline__ double rho(int r) const { return rho_in[L(r)]; }
+  __device__ __forceinline__ double Q(int c, int r) const { return q_in[c][L(r)]; }
+  __device__ __forceinline__ double zs(int r) const { return zone_size_y(*m, r); }
+  __device__ __forceinline__ double v(int r) const { return vyt[L(r)]; }
+  __device__ __forceinline__ double flux(int r, double a, double b) const { return cell::flux_y(*m, i, r, k, dt, a, b); }
+  __device__ __forceinline__ void out(int c, int r, double val) const { out_[c][L(r)] = val; }
+};
+
+// 2-D: momenta + Y sweep; rows [nghy + s*rows_per, ...) of the active range
+template <int G, int DZ>
+__global__ void __launch_bounds__(BXF) kf_sweep_y2(dmesh m, double dt, const double* rho_in, const double* vxt,
+                                                   const double* vyt, double* rho_out, double* q0, double* q1,
+                                                   double* q2, double* q3, int rows_per, double of) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int r0 = m.nghy + (int)blockIdx.y * rows_per;
+  const int r1 = min(r0 + rows_per, m.ny + m.nghy);
+  ColY2<G> col{&m, rho_in, vxt, vyt, rho_out, q0, q1, q2, q3, i, ixp_of(i, m.nx), of, dt};
+  column_sweep<4>(col, r0, r1, dt);
+}
+
+// 3-D: Y sweep of the Z-swept fields
+template <int G, int DZ>
+__global__ void __launch_bounds__(BXF) kf_sweep_y3(dmesh m, double dt, const double* rho_in, const double* z0,
+                                                   const double* z1, const double* z2, const double* z3,
+                                                   const double* z4, const double* z5, const double* vyt,
+                                                   double* rho_out, double* q0, double* q1, double* q2, double* q3,
+                                                   double* q4, double* q5, int rows_per) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int k = m.nghz + (int)blockIdx.z;
+  const int r0 = m.nghy + (int)blockIdx.y * rows_per;
+  const int r1 = min(r0 + rows_per, m.ny + m.nghy);
+  ColY3 col{&m, rho_in, {z0, z1, z2, z3, z4, z5}, vyt, {q0, q1, q2, q3, q4, q5, rho_out}, i, k, dt};
+  column_sweep<6>(col, r0, r1, dt);
+}
+
+// 3-D: momenta + Z sweep for rows j in [1, sy-2] (the Y sweep's stencil)
+template <int G, int DZ>
+__global__ void __launch_bounds__(BXF) kf_sweep_z(dmesh m, double dt, const double* rho_in, const double* vxt,
+                                                  const double* vyt, const double* vzt, double* rho_out, double* q0,
+                                                  double* q1, double* q2, double* q3, double* q4, double* q5,
+                                                  int planes_per, double of) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int j = 1 + (int)blockIdx.y;
+  const int r0 = m.nghz + (int)blockIdx.z * planes_per;
+  const int r1 = min(r0 + planes_per, m.nz + m.nghz);
+  ColZ<G> col{&m, rho_in, vxt, vyt, vzt, {q0, q1, q2, q3, q4, q5, rho_out}, i, j, ixp_of(i, m.nx), of, dt};
+  column_sweep<6>(col, r0, r1, dt);
 }
 
 // X sweep, ring-owning: VxMed of Vx_temp, residual split, van Leer pass with
 // the cell residual, PPA pass with the uniform residual, integer shift and
-// NewVelocity_x (src/transport.c:129-192).  LDS holds the ring.
-template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_sweep_x(dmesh m, double dt, const double* rho_in, const double* vxt,
-                                                 double* mpx, double* mmx, double* mpy, double* mmy, double* mpz,
-                                                 double* mmz, double* rho_out, double* vx_out, double* vxmed_out,
-                                                 int* nshift_out, double omegaframe, int fargo) {
+// NewVelocity_x (src/transport.c:129-192).  LDS holds the ring; every
+// quantity of a pass goes through the same phase between two barriers.
+// Thread t owns cells i = t + s*blockDim.x, s < CPT.
+template <int G, int DZ, int CPT>
+__global__ void __launch_bounds__(256) kf_sweep_x(dmesh m, double dt, const double* rho_in, const double* vxt,
+                                                   double* mpx, double* mmx, double* mpy, double* mmy, double* mpz,
+                                                   double* mmz, double* rho_out, double* vx_out, double* vxmed_out,
+                                                   int* nshift_out, double omegaframe, int fargo) {
   constexpr int NQ = DZ ? 6 : 4;
+  constexpr int N = NQ + 1;  // quantities + density (index NQ)
   extern __shared__ double lds[];
   const int nx = m.nx;
-  double* R = lds;                 // density
-  double* Qa = R + nx;             // NQ quantities
-  double* VR = Qa + NQ * nx;       // cell residual / total velocity
-  double* VU = VR + nx;            // uniform residual
-  double* DS = VU + nx;            // DensStar
-  double* QQ = DS + nx;            // q = Q/rho
-  double* SL = QQ + nx;            // slope / qL
-  double* QR = SL + nx;            // qR
-  double* QS = QR + nx;            // interface value
-  double* sh = QS + nx;            // 8 doubles
+  double* ST = lds;               // state: N * nx (quantities, then density)
+  double* AQ = ST + N * nx;       // q = Q/rho (or rho): N * nx
+  double* SL = AQ + N * nx;       // slopes, then limited qL: N * nx
+  double* FC = SL + N * nx;       // faces, then face fluxes: N * nx
+  double* QL2 = FC + N * nx;      // limited qR of the PPA pass: N * nx
+  double* V = QL2 + N * nx;       // advection velocity: nx
+  double* sh = V + nx;            // 8
   const int j = m.nghy + (int)(blockIdx.x % m.ny);
   const int k = DZ ? m.nghz + (int)(blockIdx.x / m.ny) : 0;
   const int base = j * m.pitch + k * m.stride;
   double* qg[6] = {mpx, mmx, mpy, mmy, mpz, mmz};
   const int t = threadIdx.x, nt = blockDim.x;
-  for (int i = t; i < nx; i += nt) {
-    R[i] = rho_in[base + i];
-    for (int c = 0; c < NQ; c++) Qa[c * nx + i] = qg[c][base + i];
-    QS[i] = vxt[base + i] * (m.xmin[i + 1] - m.xmin[i]);
+  double vt[CPT], vu[CPT];
+#pragma unroll
+  for (int s = 0; s < CPT; s++) {
+    const int i = t + s * nt;
+    if (i < nx) {
+      ST[NQ * nx + i] = rho_in[base + i];
+      #pragma unroll
+      for (int c = 0; c < NQ; c++) ST[c * nx + i] = qg[c][base + i];
+      vt[s] = vxt[base + i];
+      V[i] = vt[s] * (m.xmin[i + 1] - m.xmin[i]);
+    }
   }
   __syncthreads();
-  // ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30): sequential ring sum
-  if (t == 0) {
-    double acc = 0.0;
-    for (int i = 0; i < nx; i++) acc = acc + QS[i];
-    sh[0] = acc / (m.XMAX - m.XMIN);
-  }
+  // ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30)
+  if (t == 0) sh[0] = seq_sum(V, nx) / (m.XMAX - m.XMIN);
   __syncthreads();
   const double vmed = sh[0];
-  if (t == 0) vxmed_out[j + k * m.sy] = vmed;
-  int ns = 0;
-  // ComputeResidual (src/compute_residual.c:68-73)
-  for (int i = t; i < nx; i += nt) {
-    const double v = vxt[base + i];
-    if (fargo) {
-      const double zs = zone_size_x<G>(m, i, j, k);
-      double ntilde = vmed * dt / zs;
-      double nround = floor(ntilde + 0.5);
-      if (i == 0) sh[1] = nround;
-      VR[i] = v - vmed;
-      VU[i] = (ntilde - nround) * zs / dt;
-    } else {
-      VR[i] = v;
+  // ComputeResidual (src/compute_residual.c:68-73): cell residual into V
+#pragma unroll
+  for (int s = 0; s < CPT; s++) {
+    const int i = t + s * nt;
+    if (i < nx) {
+      if (fargo) {
+        const double zs = zone_size_x<G>(m, i, j, k);
+        const double ntilde = vmed * dt / zs;
+        const double nround = floor(ntilde + 0.5);
+        if (i == 0) sh[1] = nround;
+        V[i] = vt[s] - vmed;
+        vu[s] = (ntilde - nround) * zs / dt;
+      } else {
+        V[i] = vt[s];
+      }
     }
   }
   __syncthreads();
+  int ns = 0;
   if (fargo) {
     ns = (int)sh[1];
-    if (t == 0) nshift_out[j + k * m.sy] = ns;
+    if (t == 0) {
+      nshift_out[j + k * m.sy] = ns;
+      vxmed_out[j + k * m.sy] = vmed;
+    }
   }
-  // one X advection pass (X_advection, src/transport.c:208-226)
+  // X_advection (src/transport.c:208-226), twice with FARGO
   for (int pass = 0; pass < (fargo ? 2 : 1); pass++) {
-    const double* V = pass ? VU : VR;
     const bool ppa = (pass == 1);
-    // quantities in order: density (DensStar), NQ momenta, density update
-    for (int c = -1; c <= NQ; c++) {
-      if (c == NQ) {
-        // UpdateDensityX (src/update_density_x.c:66-68)
-        for (int i = t; i < nx; i += nt) {
-          const int ip = ixp_of(i, nx);
-          QQ[i] = R[i] + cell::flux_x(m, i, j, k, dt, V[i] * DS[i], V[ip] * DS[ip]);
-        }
-        __syncthreads();
-        for (int i = t; i < nx; i += nt) R[i] = QQ[i];
-        __syncthreads();
-        break;
-      }
-      // q: the density itself for DensStar, Q/rho otherwise (DivideByRho)
-      for (int i = t; i < nx; i += nt) QQ[i] = (c < 0) ? R[i] : Qa[c * nx + i] / R[i];
+    if (ppa) {
       __syncthreads();
-      if (!ppa) {
-        // VanLeerX_a (src/vanleer_x_a.c:60-68)
-        for (int i = t; i < nx; i += nt) {
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) V[i] = vu[s];
+      }
+    }
+    // q = Q/rho (DivideByRho) and, for the density, q = rho
+#pragma unroll
+    for (int s = 0; s < CPT; s++) {
+      const int i = t + s * nt;
+      if (i < nx) {
+        const double r = ST[NQ * nx + i];
+        #pragma unroll
+        for (int c = 0; c < NQ; c++) AQ[c * nx + i] = ST[c * nx + i] / r;
+        AQ[NQ * nx + i] = r;
+      }
+    }
+    __syncthreads();
+    if (!ppa) {
+      // VanLeerX_a (src/vanleer_x_a.c:60-68)
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
           const int im = ixm_of(i, nx), ip = ixp_of(i, nx);
-          double dqm = (QQ[i] - QQ[im]) / zone_size_x<G>(m, i, j, k);
-          double dqp = (QQ[ip] - QQ[i]) / zone_size_x<G>(m, ip, j, k);
-          SL[i] = vanleer_slope(dqm, dqp);
+          const double zi = zone_size_x<G>(m, i, j, k), zp = zone_size_x<G>(m, ip, j, k);
+          #pragma unroll
+          for (int c = 0; c < N; c++) {
+            const double* a = AQ + c * nx;
+            SL[c * nx + i] = vanleer_slope((a[i] - a[im]) / zi, (a[ip] - a[i]) / zp);
+          }
         }
-        __syncthreads();
-        // VanLeerX_b (src/vanleer_x_b.c:60-65)
-        double* out = (c < 0) ? DS : QS;
-        for (int i = t; i < nx; i += nt) {
-          const int im = ixm_of(i, nx);
-          out[i] = vanleer_face(V[i], dt, QQ[im], SL[im], zone_size_x<G>(m, im, j, k), QQ[i], SL[i],
-                                zone_size_x<G>(m, i, j, k));
-        }
-        __syncthreads();
-      } else {
-        // VanLeerX_PPA_a/b/c/d (src/fargo_ppa_a.c .. src/fargo_ppa_d.c)
-        for (int i = t; i < nx; i += nt) SL[i] = cell::ppa_slope(QQ[ixm_of(i, nx)], QQ[i], QQ[ixp_of(i, nx)]);
-        __syncthreads();
-        // QR[i] = face i+1/2; qL of cell i is face i-1/2 = QR[i-1]
-        for (int i = t; i < nx; i += nt) {
-          const int ip = ixp_of(i, nx);
-          QR[i] = cell::ppa_face(QQ[i], QQ[ip], SL[i], SL[ip]);
-        }
-        __syncthreads();
-        double* out = (c < 0) ? DS : QS;
-        // limited (qL, qR) of each cell: store qL in SL, qR in a register pass
-        for (int i = t; i < nx; i += nt) {
-          double qL = QR[ixm_of(i, nx)], qR = QR[i];
-          cell::ppa_limit(QQ[i], qL, qR);
-          SL[i] = qL;
-          out[i] = qR;  // temporarily the limited qR
-        }
-        __syncthreads();
-        // copy limited qR out of the face buffer before computing faces
-        for (int i = t; i < nx; i += nt) QR[i] = out[i];
-        __syncthreads();
-        for (int i = t; i < nx; i += nt) {
-          const int im = ixm_of(i, nx);
-          out[i] = cell::ppa_flux_value(V[i], dt, zone_size_x<G>(m, i, j, k), QQ[im], SL[im], QR[im], QQ[i],
-                                        SL[i], QR[i]);
-        }
-        __syncthreads();
       }
-      if (c >= 0) {
-        // UpdateX (src/update_x.c:69-71)
-        double* Qc = Qa + c * nx;
-        for (int i = t; i < nx; i += nt) {
-          const int ip = ixp_of(i, nx);
-          QQ[i] = Qc[i] + cell::flux_x(m, i, j, k, dt, V[i] * QS[i] * DS[i], V[ip] * QS[ip] * DS[ip]);
+      __syncthreads();
+      // VanLeerX_b (src/vanleer_x_b.c:60-65)
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+          const int im = ixm_of(i, nx);
+          const double zi = zone_size_x<G>(m, i, j, k), zm = zone_size_x<G>(m, im, j, k);
+          const double v = V[i];
+          #pragma unroll
+          for (int c = 0; c < N; c++) {
+            const double* a = AQ + c * nx;
+            const double* sl = SL + c * nx;
+            FC[c * nx + i] = vanleer_face(v, dt, a[im], sl[im], zm, a[i], sl[i], zi);
+          }
         }
-        __syncthreads();
-        for (int i = t; i < nx; i += nt) Qc[i] = QQ[i];
-        __syncthreads();
+      }
+      __syncthreads();
+    } else {
+      // VanLeerX_PPA_a: monotonised-centred slope (src/fargo_ppa_a.c:58-69)
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+          const int im = ixm_of(i, nx), ip = ixp_of(i, nx);
+          #pragma unroll
+          for (int c = 0; c < N; c++) {
+            const double* a = AQ + c * nx;
+            SL[c * nx + i] = cell::ppa_slope(a[im], a[i], a[ip]);
+          }
+        }
+      }
+      __syncthreads();
+      // VanLeerX_PPA_b: face value q_{i+1/2} = qR of cell i = qL of cell i+1
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+          const int ip = ixp_of(i, nx);
+          #pragma unroll
+          for (int c = 0; c < N; c++) {
+            const double* a = AQ + c * nx;
+            const double* sl = SL + c * nx;
+            FC[c * nx + i] = cell::ppa_face(a[i], a[ip], sl[i], sl[ip]);
+          }
+        }
+      }
+      __syncthreads();
+      // VanLeerX_PPA_c (src/fargo_ppa_c.c): limited (qL, qR) of each cell into
+      // SL (slopes are no longer read) and QL2; the faces stay in FC
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+          const int im = ixm_of(i, nx);
+#pragma unroll
+          for (int c = 0; c < N; c++) {
+            double qL = FC[c * nx + im], qR = FC[c * nx + i];
+            cell::ppa_limit(AQ[c * nx + i], qL, qR);
+            SL[c * nx + i] = qL;
+            QL2[c * nx + i] = qR;
+          }
+        }
+      }
+      __syncthreads();
+      // VanLeerX_PPA_d: parabolic average over the swept zone (src/fargo_ppa_d.c:62-70)
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+          const int im = ixm_of(i, nx);
+          const double zi = zone_size_x<G>(m, i, j, k);
+          const double v = V[i];
+#pragma unroll
+          for (int c = 0; c < N; c++) {
+            const double* a = AQ + c * nx;
+            const double* qL = SL + c * nx;
+            const double* qR = QL2 + c * nx;
+            FC[c * nx + i] = cell::ppa_flux_value(v, dt, zi, a[im], qL[im], qR[im], a[i], qL[i], qR[i]);
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // UpdateX for every quantity and UpdateDensityX last (src/update_x.c:69-71,
+    // src/update_density_x.c:66-68); a cell reads only faces, so in place
+#pragma unroll
+    for (int s = 0; s < CPT; s++) {
+      const int i = t + s * nt;
+      if (i < nx) {
+        const int ip = ixp_of(i, nx);
+        const double v0 = V[i], v1 = V[ip];
+        const double* ds = FC + NQ * nx;
+        #pragma unroll
+        for (int c = 0; c < NQ; c++) {
+          const double* f = FC + c * nx;
+          ST[c * nx + i] += cell::flux_x(m, i, j, k, dt, v0 * f[i] * ds[i], v1 * f[ip] * ds[ip]);
+        }
+        ST[NQ * nx + i] += cell::flux_x(m, i, j, k, dt, v0 * ds[i], v1 * ds[ip]);
       }
     }
+    __syncthreads();
   }
-  // AdvectSHIFT (src/advect_shift.c:45-58) and write-back
+  // AdvectSHIFT (src/advect_shift.c:45-58): cell i moves to i + ns (mod nx);
+  // the shifted ring goes to AQ, which is free after the last update
+  const double* OUT = ST;
   if (fargo) {
-    for (int c = -1; c < NQ; c++) {
-      double* A = (c < 0) ? R : Qa + c * nx;
-      for (int i = t; i < nx; i += nt) {
-        int it = (i - ns) % nx;
+#pragma unroll
+    for (int s = 0; s < CPT; s++) {
+      const int i = t + s * nt;
+      if (i < nx) {
+        int it = (i + ns) % nx;
         if (it < 0) it += nx;
-        QQ[i] = A[it];
+#pragma unroll
+        for (int c = 0; c < N; c++) AQ[c * nx + it] = ST[c * nx + i];
       }
-      __syncthreads();
-      for (int i = t; i < nx; i += nt) A[i] = QQ[i];
-      __syncthreads();
     }
+    __syncthreads();
+    OUT = AQ;
   }
   // NewVelocity_x (src/newvel_x.c:78-86) and outputs
-  for (int i = t; i < nx; i += nt) {
-    const int im = ixm_of(i, nx);
-    vx_out[base + i] = cell::newvel_x<G>(m, i, j, k, R[i], R[im], Qa[1 * nx + i], Qa[0 * nx + im], omegaframe);
-    rho_out[base + i] = R[i];
-    for (int c = 2; c < NQ; c++) qg[c][base + i] = Qa[c * nx + i];
+#pragma unroll
+  for (int s = 0; s < CPT; s++) {
+    const int i = t + s * nt;
+    if (i < nx) {
+      const int im = ixm_of(i, nx);
+      const double* R = OUT + NQ * nx;
+      vx_out[base + i] = cell::newvel_x<G>(m, i, j, k, R[i], R[im], OUT[1 * nx + i], OUT[0 * nx + im], omegaframe);
+      rho_out[base + i] = R[i];
+#pragma unroll
+      for (int c = 2; c < NQ; c++) qg[c][base + i] = OUT[c * nx + i];
+    }
   }
 }
 
@@ -694,21 +785,33 @@ static cell::CflPar cfl_par_f(const f3d_hip_ctx* ctx) {
 
 extern "C" int f3d_boundaries_on(f3d_hip_ctx* ctx, double* dens, double* vx, double* vy, double* vz);
 
+// tuning knobs of the column sweeps (rows per thread); set by the tuner
+static int g_rows_y = 8, g_rows_z = 8;
+
 extern "C" {
+
+int f3d_fused_set_tiling(int rows_y, int rows_z) {
+  if (rows_y < 1 || rows_z < 1) return -2;
+  g_rows_y = rows_y;
+  g_rows_z = rows_z;
+  return 0;
+}
 
 int f3d_fused_cfl(f3d_hip_ctx* ctx, double* fluid_min) {
   Scratch* S;
   F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
   F3D_REQUIRE(ctx, ctx->ph.eos == 1, "the fused path covers the isothermal equation of state");
   const dmesh& m = ctx->m;
-  const int nrings = (m.ny + 1) * (m.dimz ? m.nz : 1);
-  const size_t shmem = (m.nx + 16) * sizeof(double);
+  const int nrings = m.ny * (m.dimz ? m.nz : 1);
+  const size_t shmem = (2 * m.nx + 16) * sizeof(double);
+  const int slot = S->slot;
   f3d_stage_begin(ctx, F3D_ST_CFL);
   FDISPATCH(ctx, kf_cfl, dim3(nrings), dim3(WG), shmem, ctx->f.Energy, ctx->f.Density, ctx->f.Vx, ctx->f.Vy,
-            ctx->f.Vz, ctx->f.VxMed, S->ringmin, S->counter, S->result, cfl_par_f(ctx));
+            ctx->f.Vz, ctx->f.VxMed, S->result, slot, cfl_par_f(ctx));
   f3d_stage_end(ctx, F3D_ST_CFL);
-  F3D_CHECK(ctx, hipMemcpyAsync(ctx->hres, S->result, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  F3D_CHECK(ctx, hipMemcpyAsync(ctx->hres, S->result + slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   F3D_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  S->slot ^= 1;
   *fluid_min = ctx->hres[0];
   return 0;
 }
@@ -725,22 +828,12 @@ int f3d_fused_sources(f3d_hip_ctx* ctx, double dt, double domega) {
   vp.alpha = ctx->ph.alpha;
   vp.gamma = ctx->ph.gamma;
   vp.iso = 1;
-  cell::PlanetSet ps;
-  ps.n = ctx->ph.nplanets < cell::kMaxPlanets ? ctx->ph.nplanets : cell::kMaxPlanets;
-  ps.indirect = ctx->ph.indirectterm;
-  for (int n = 0; n < ps.n; n++) {
-    ps.x[n] = ctx->ph.px[n];
-    ps.y[n] = ctx->ph.py[n];
-    ps.z[n] = ctx->ph.pz[n];
-    ps.mp[n] = ctx->ph.pmass[n];
-    ps.smooth2[n] = ctx->ph.psmooth2[n];
-  }
   const int visc = ctx->ph.visc != 0 && ctx->ph.fluidtype == 1;
   const int pot = ctx->ph.potential;
   dim3 ga((m.nx + BXF - 1) / BXF, m.sy - 2, m.dimz ? m.sz - 2 : 1);
   f3d_stage_begin(ctx, F3D_ST_SRC_A);
   FDISPATCH(ctx, kf_src_a, ga, dim3(BXF), 0, f.Density, f.Energy, f.Vx, f.Vy, f.Vz, f.Mmx, f.Mmy, f.Mmz, f.Mpx,
-            f.Mpy, f.Mpz, S->f[FS_QX], S->f[FS_QY], S->f[FS_QZ], f.Pot, vp, visc, ps, 0, domega);
+            f.Mpy, f.Mpz, S->f[FS_QX], S->f[FS_QY], S->f[FS_QZ], vp, visc, domega);
   f3d_stage_end(ctx, F3D_ST_SRC_A);
   dim3 gb((m.nx + BXF - 1) / BXF, m.ny, m.dimz ? m.nz : 1);
   f3d_stage_begin(ctx, F3D_ST_SRC_B);
@@ -762,34 +855,50 @@ int f3d_fused_transport(f3d_hip_ctx* ctx, double dt, int end_of_step) {
   const dmesh& m = ctx->m;
   const f3d_fields& f = ctx->f;
   const double of = ctx->ph.OMEGAFRAME;
-  const int rows_per = 16;
+  const int ry = g_rows_y, rz = g_rows_z;
   if (m.dimz) {
     // seven separate buffers: six Z-swept momenta and the Z-swept density
     double* const* Z = &S->f[FS_Z0];
-    dim3 gz((m.nx + BXF - 1) / BXF, m.sy - 2, (m.nz + rows_per - 1) / rows_per);
+    dim3 gz((m.nx + BXF - 1) / BXF, m.sy - 2, (m.nz + rz - 1) / rz);
     f3d_stage_begin(ctx, F3D_ST_SWEEP_Z);
     FDISPATCH(ctx, kf_sweep_z, gz, dim3(BXF), 0, dt, f.Density, f.Vx_temp, f.Vy_temp, f.Vz_temp, Z[6], Z[0], Z[1],
-              Z[2], Z[3], Z[4], Z[5], rows_per, of);
+              Z[2], Z[3], Z[4], Z[5], rz, of);
     f3d_stage_end(ctx, F3D_ST_SWEEP_Z);
-    dim3 gy((m.nx + BXF - 1) / BXF, (m.ny + rows_per - 1) / rows_per, m.nz);
+    dim3 gy((m.nx + BXF - 1) / BXF, (m.ny + ry - 1) / ry, m.nz);
     f3d_stage_begin(ctx, F3D_ST_SWEEP_Y);
-    FDISPATCH(ctx, kf_sweep_y, gy, dim3(BXF), 0, dt, Z[6], f.Vx_temp, f.Vy_temp, Z[0], Z[1], Z[2], Z[3], Z[4], Z[5],
-              S->f[FS_RHOY], f.Mpx, f.Mmx, f.Mpy, f.Mmy, f.Mpz, f.Mmz, rows_per, of, 0);
+    FDISPATCH(ctx, kf_sweep_y3, gy, dim3(BXF), 0, dt, Z[6], Z[0], Z[1], Z[2], Z[3], Z[4], Z[5], f.Vy_temp,
+              S->f[FS_RHOY], f.Mpx, f.Mmx, f.Mpy, f.Mmy, f.Mpz, f.Mmz, ry);
     f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
   } else {
-    dim3 gy((m.nx + BXF - 1) / BXF, (m.ny + rows_per - 1) / rows_per, 1);
+    dim3 gy((m.nx + BXF - 1) / BXF, (m.ny + ry - 1) / ry, 1);
     f3d_stage_begin(ctx, F3D_ST_SWEEP_Y);
-    FDISPATCH(ctx, kf_sweep_y, gy, dim3(BXF), 0, dt, f.Density, f.Vx_temp, f.Vy_temp, nullptr, nullptr, nullptr,
-              nullptr, nullptr, nullptr, S->f[FS_RHOY], f.Mpx, f.Mmx, f.Mpy, f.Mmy, nullptr, nullptr, rows_per, of, 1);
+    FDISPATCH(ctx, kf_sweep_y2, gy, dim3(BXF), 0, dt, f.Density, f.Vx_temp, f.Vy_temp, S->f[FS_RHOY], f.Mpx, f.Mmx,
+              f.Mpy, f.Mmy, ry, of);
     f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
   }
-  const int nq = m.dimz ? 6 : 4;
-  const size_t shx = ((size_t)(nq + 8) * m.nx + 8) * sizeof(double);
-  F3D_REQUIRE(ctx, shx <= 160 * 1024, "ring too long for the LDS X sweep");
+  // X sweep: one workgroup of at most 256 threads per active ring, up to 4
+  // cells per thread
+  const int nq1 = (m.dimz ? 6 : 4) + 1;
+  const size_t shx = ((size_t)5 * nq1 * m.nx + m.nx + 8) * sizeof(double);
+  F3D_REQUIRE(ctx, shx <= 160 * 1024 && m.nx <= 1024, "ring too long for the LDS X sweep");
+  const int cpt = (m.nx <= 256) ? 1 : (m.nx <= 512) ? 2 : 4;
+  const int nthreads = (((m.nx + cpt - 1) / cpt + 63) / 64) * 64;
   dim3 gx(m.ny * (m.dimz ? m.nz : 1));
   f3d_stage_begin(ctx, F3D_ST_SWEEP_X);
-  FDISPATCH(ctx, kf_sweep_x, gx, dim3(WG), shx, dt, S->f[FS_RHOY], f.Vx_temp, f.Mpx, f.Mmx, f.Mpy, f.Mmy, f.Mpz,
-            f.Mmz, S->f[FS_RHOX], f.Vx, f.VxMed, f.Nshift, of, ctx->ph.fargo);
+#define XSWEEP(G_, DZ_, CPT_)                                                                                  \
+  hipLaunchKernelGGL((kf_sweep_x<G_, DZ_, CPT_>), gx, dim3(nthreads), shx, ctx->stream, m, dt, S->f[FS_RHOY], \
+                     f.Vx_temp, f.Mpx, f.Mmx, f.Mpy, f.Mmy, f.Mpz, f.Mmz, S->f[FS_RHOX], f.Vx, f.VxMed, f.Nshift, \
+                     of, ctx->ph.fargo)
+#define XSWEEP_G(G_, DZ_) \
+  do { if (cpt == 1) XSWEEP(G_, DZ_, 1); else if (cpt == 2) XSWEEP(G_, DZ_, 2); else XSWEEP(G_, DZ_, 4); } while (0)
+  if (m.geom == SPH) {
+    if (m.dimz) XSWEEP_G(SPH, 1); else XSWEEP_G(SPH, 0);
+  } else {
+    if (m.dimz) XSWEEP_G(CYL, 1); else XSWEEP_G(CYL, 0);
+  }
+#undef XSWEEP_G
+#undef XSWEEP
+  F3D_LAUNCH_CHECK(ctx);
   f3d_stage_end(ctx, F3D_ST_SWEEP_X);
   cell::DampPar dp;
   dp.Y_inf = ctx->ph.Y_inf; dp.Y_sup = ctx->ph.Y_sup; dp.Z_inf = ctx->ph.Z_inf; dp.Z_sup = ctx->ph.Z_sup;

@@ -53,18 +53,18 @@ inline void f3d_stage_begin(f3d_hip_ctx* c, int s) {
   if (!c->timing) return;
   if (c->pending[s]) {
     float ms = 0.f;
-    hipEventSynchronize(c->ev[s][1]);
-    if (hipEventElapsedTime(&ms, c->ev[s][0], c->ev[s][1]) == hipSuccess) {
+    if (hipEventSynchronize(c->ev[s][1]) == hipSuccess &&
+        hipEventElapsedTime(&ms, c->ev[s][0], c->ev[s][1]) == hipSuccess) {
       c->stage_ms[s] += ms;
       c->stage_n[s] += 1;
     }
     c->pending[s] = 0;
   }
-  hipEventRecord(c->ev[s][0], c->stream);
+  (void)hipEventRecord(c->ev[s][0], c->stream);
 }
 inline void f3d_stage_end(f3d_hip_ctx* c, int s) {
   if (!c->timing) return;
-  hipEventRecord(c->ev[s][1], c->stream);
+  (void)hipEventRecord(c->ev[s][1], c->stream);
   c->pending[s] = 1;
 }
 

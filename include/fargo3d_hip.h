@@ -189,7 +189,8 @@ int f3d_init_stockholm(f3d_hip_ctx *ctx);
 
 /* ---- fused stages (no counterpart in the reference: the same arithmetic as
  * the routine sequences they replace, in fewer HBM passes) ---- */
-/* ComputeVmed(Vx) + cfl + cfl_b in two launches; writes the fluid's Min to *fluid_min. */
+/* ComputeVmed(Vx) + cfl + cfl_b in one launch (one workgroup per ring, one
+ * atomic MIN per ring); writes the fluid's Min to *fluid_min. */
 int f3d_fused_cfl(f3d_hip_ctx *ctx, double *fluid_min);
 /* Sources from the pressure field to the mid-step ghost fill (src/algogas.c:34-118),
  * after Potential; domega is the CorrectVtheta argument (0 when not corotating). */
@@ -197,6 +198,9 @@ int f3d_fused_sources(f3d_hip_ctx *ctx, double dt, double domega);
 /* Transport (src/algogas.c:135-159) and, when stockholm != 0, StockholmBoundary and
  * the end-of-step ghost fill. */
 int f3d_fused_transport(f3d_hip_ctx *ctx, double dt, int end_of_step);
+/* Rows (Y sweep) and planes (Z sweep) one thread marches through; a tuning
+ * knob that does not change results.  Returns -2 for values below 1. */
+int f3d_fused_set_tiling(int rows_y, int rows_z);
 
 #ifdef __cplusplus
 }
