@@ -59,6 +59,24 @@ def cpu_baseline(parfile, budget_s=12.0):
     return n, el
 
 
+def copy_calibration(L, sim, nbytes):
+    """Three copies of nbytes between two device buffers (outside the timed region)."""
+    L.f3d_hip_malloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    L.f3d_hip_free.argtypes = [ctypes.c_void_p]
+    L.f3d_hip_memcpy_d2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    L.f3d_hip_memset.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if L.f3d_hip_malloc(ctypes.byref(a), nbytes) or L.f3d_hip_malloc(ctypes.byref(b), nbytes):
+        raise RuntimeError("copy calibration: allocation failed")
+    L.f3d_hip_memset(sim.ctx, a, 1, nbytes)
+    for _ in range(3):
+        if L.f3d_hip_memcpy_d2d(sim.ctx, b, a, nbytes):
+            raise RuntimeError("copy calibration: copy failed")
+    sim.sync()
+    L.f3d_hip_free(a)
+    L.f3d_hip_free(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +86,9 @@ def main():
     ap.add_argument("--routines", action="store_true", help="one kernel per routine instead of the fused path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run three device-to-device copies of MIB MiB on the library stream (a copy "
+                         "kernel of known size that scales the hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU decomposition is a design only "
@@ -102,6 +123,8 @@ def main():
     cnt = (ctypes.c_long * nst)()
     L.f3d_hip_timing_read(sim.ctx, ms, cnt)
     L.f3d_hip_timing_enable(sim.ctx, 0)
+    if args.copy_calibration > 0:
+        copy_calibration(L, sim, args.copy_calibration << 20)
 
     value = cells * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps

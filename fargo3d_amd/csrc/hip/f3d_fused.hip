@@ -786,7 +786,7 @@ static cell::CflPar cfl_par_f(const f3d_hip_ctx* ctx) {
 extern "C" int f3d_boundaries_on(f3d_hip_ctx* ctx, double* dens, double* vx, double* vy, double* vz);
 
 // tuning knobs of the column sweeps (rows per thread); set by the tuner
-static int g_rows_y = 8, g_rows_z = 8;
+static int g_rows_y = 4, g_rows_z = 3;
 
 extern "C" {
 

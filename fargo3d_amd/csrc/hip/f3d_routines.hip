@@ -525,20 +525,27 @@ __global__ void k_vmed_div(dmesh m, const double* red2d, double* vxmed, int n2) 
 }
 
 // ---- boundaries (std/boundary_template.c via scripts/boundparser.py:237-363)
-__global__ void k_bound_y(dmesh m, int side, double* dens, double* vx, double* vy, double* vz, cell::BcPar bp,
-                          const double* dexp) {
+// Y ghost fill of side side0 (0: ymin) and, when gridDim.y = 2 nghy, also of
+// side 1 in the same launch.  With ny > nghy the two sides write disjoint rows
+// and neither reads a row the other writes, so the reference's order (ymin
+// first, src/boundary.c:2-23) does not change any value.
+__global__ void k_bound_y(dmesh m, int side0, double* dens, double* vx, double* vy, double* vz, cell::BcPar bp0,
+                          cell::BcPar bp1, const double* dexp0, const double* dexp1) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
-  const int mm = blockIdx.y, k = blockIdx.z;
-  cell::bound_y_cell(m, side, i, mm, k, dens, vx, vy, vz, bp, dexp);
+  const int side = side0 + ((int)blockIdx.y >= m.nghy);
+  const int mm = (int)blockIdx.y - (side - side0) * m.nghy, k = blockIdx.z;
+  cell::bound_y_cell(m, side, i, mm, k, dens, vx, vy, vz, side ? bp1 : bp0, side ? dexp1 : dexp0);
 }
 
-__global__ void k_bound_z(dmesh m, int side, double* dens, double* vx, double* vy, double* vz, cell::BcPar bp,
-                          const double* zdens) {
+// Z ghost fill, the same arrangement as k_bound_y (both sides when nz > nghz).
+__global__ void k_bound_z(dmesh m, int side0, double* dens, double* vx, double* vy, double* vz, cell::BcPar bp0,
+                          cell::BcPar bp1, const double* zdens0, const double* zdens1) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
-  const int mm = blockIdx.z, j = blockIdx.y;
-  cell::bound_z_cell(m, side, i, mm, j, dens, vx, vy, vz, bp, zdens);
+  const int side = side0 + ((int)blockIdx.z >= m.nghz);
+  const int mm = (int)blockIdx.z - (side - side0) * m.nghz, j = blockIdx.y;
+  cell::bound_z_cell(m, side, i, mm, j, dens, vx, vy, vz, side ? bp1 : bp0, side ? zdens1 : zdens0);
 }
 
 __global__ void k_init_stockholm(dmesh m, const double* rho, const double* vx, const double* vy, const double* vz,
@@ -753,21 +760,41 @@ int f3d_boundaries_on(f3d_hip_ctx* ctx, double* dens, double* vx, double* vy, do
   f.Vy = vy;
   f.Vz = vz;
   dim3 blk(BX);
+  // zmin, zmax, then ymin, ymax (src/boundary.c:2-23); the Y fill covers the
+  // Z ghost planes, so it runs after the Z fill
   if (!ctx->ph.periodicz && m.dimz) {
     F3D_REQUIRE(ctx, ctx->ph.bc == 2, "3-D boundaries need the spherical-disk family");
-    for (int side = 0; side < 2; side++) {
-      dim3 g((m.nx + BX - 1) / BX, m.sy, m.nghz);
-      hipLaunchKernelGGL(k_bound_z, g, blk, 0, ctx->stream, m, side, f.Density, f.Vx, f.Vy, f.Vz, bc_par(ctx, side),
-                         side ? ctx->d_zmax_dens : ctx->d_zmin_dens);
+    const cell::BcPar b0 = bc_par(ctx, 0), b1 = bc_par(ctx, 1);
+    if (m.nz > m.nghz) {
+      dim3 g((m.nx + BX - 1) / BX, m.sy, 2 * m.nghz);
+      hipLaunchKernelGGL(k_bound_z, g, blk, 0, ctx->stream, m, 0, f.Density, f.Vx, f.Vy, f.Vz, b0, b1,
+                         ctx->d_zmin_dens, ctx->d_zmax_dens);
       F3D_LAUNCH_CHECK(ctx);
+    } else {
+      for (int side = 0; side < 2; side++) {
+        dim3 g((m.nx + BX - 1) / BX, m.sy, m.nghz);
+        const cell::BcPar b = side ? b1 : b0;
+        const double* t = side ? ctx->d_zmax_dens : ctx->d_zmin_dens;
+        hipLaunchKernelGGL(k_bound_z, g, blk, 0, ctx->stream, m, side, f.Density, f.Vx, f.Vy, f.Vz, b, b, t, t);
+        F3D_LAUNCH_CHECK(ctx);
+      }
     }
   }
   if (!ctx->ph.periodicy) {
-    for (int side = 0; side < 2; side++) {
-      dim3 g((m.nx + BX - 1) / BX, m.nghy, m.dimz ? m.sz : 1);
-      hipLaunchKernelGGL(k_bound_y, g, blk, 0, ctx->stream, m, side, f.Density, f.Vx, f.Vy, f.Vz, bc_par(ctx, side),
-                         side ? ctx->d_ymax_dexp : ctx->d_ymin_dexp);
+    const cell::BcPar b0 = bc_par(ctx, 0), b1 = bc_par(ctx, 1);
+    if (m.ny > m.nghy) {
+      dim3 g((m.nx + BX - 1) / BX, 2 * m.nghy, m.dimz ? m.sz : 1);
+      hipLaunchKernelGGL(k_bound_y, g, blk, 0, ctx->stream, m, 0, f.Density, f.Vx, f.Vy, f.Vz, b0, b1,
+                         ctx->d_ymin_dexp, ctx->d_ymax_dexp);
       F3D_LAUNCH_CHECK(ctx);
+    } else {
+      for (int side = 0; side < 2; side++) {
+        dim3 g((m.nx + BX - 1) / BX, m.nghy, m.dimz ? m.sz : 1);
+        const cell::BcPar b = side ? b1 : b0;
+        const double* t = side ? ctx->d_ymax_dexp : ctx->d_ymin_dexp;
+        hipLaunchKernelGGL(k_bound_y, g, blk, 0, ctx->stream, m, side, f.Density, f.Vx, f.Vy, f.Vz, b, b, t, t);
+        F3D_LAUNCH_CHECK(ctx);
+      }
     }
   }
   return 0;

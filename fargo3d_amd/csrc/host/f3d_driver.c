@@ -508,8 +508,14 @@ int f3d_sim_step(f3d_sim *S, double *dt_out) {
   double dt = S->StepTime;
   S->dtemp += dt;
   if (S->dtemp > DT) dt = DT - (S->dtemp - dt);
-  CALL(S, f3d_Reset_field(c, S->f.Total_Density));
-  CALL(S, f3d_ComputeTotalDensity(c));
+  /* Total_Density (src/main.c:399-403) feeds only the disk force; the fused
+     path skips it when no planet feels the disk */
+  int feel = 0;
+  for (int k = 0; k < S->sys.nb; k++) feel |= S->sys.feeldisk[k];
+  if (!S->arch.fused_sources || (S->p.potential && feel)) {
+    CALL(S, f3d_Reset_field(c, S->f.Total_Density));
+    CALL(S, f3d_ComputeTotalDensity(c));
+  }
   if (S->arch.fused_sources) {
     if (S->p.potential) {
       if (compute_potential(S, dt)) return -1;
