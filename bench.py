@@ -29,19 +29,39 @@ WORKLOAD = {
     "c1": "2-D polar isothermal disk 384x128, FP64, FARGO, nu=1e-5, 1e-3 Mstar planet",
     "c2": "3-D spherical isothermal disk 256x96x32, FP64, FARGO, alpha=1e-3, 3e-4 Mstar planet",
 }
-# Algorithmic bytes per active cell and launch of each fused stage: every
-# array the stage must read once and write once at 8 bytes (FP64).
-# 2-D / 3-D counts: reads + writes.
+# Algorithmic bytes per active cell and launch of each fused stage, as
+# (2-D, 3-D): every 3-D array the stage must read once and write once, at
+# 8 bytes (FP64); 1-D/2-D tables and ghost cells are not counted.
 STAGE_BYTES = {
-    "cfl": (3 * 8, 4 * 8),                 # cs, vx, vy (, vz)
-    "sources_a": ((3 + 5) * 8, (5 + 9) * 8),  # rho, v (, cs) -> stresses + q
-    "sources_b": ((6 + 5 + 2) * 8, (7 + 9 + 3) * 8),
-    "sweep_z": (0, (5 + 7) * 8),           # rho, v_temp -> rho + 6 momenta
-    "sweep_y": ((3 + 5) * 8, (1 + 7 + 7) * 8),
-    "sweep_x": ((6 + 4) * 8, (8 + 6) * 8),  # rho, momenta, vx_temp -> rho, vx, momenta
-    "finish": ((4 + 2 + 3) * 8, (6 + 3 + 4) * 8),
-    "potential": (1 * 8, 1 * 8),
+    "cfl": (3 * 8, 4 * 8),                        # cs, vx, vy (, vz)
+    "sources_a": ((3 + 5) * 8, (5 + 9) * 8),      # rho, v (, cs) -> stresses + q
+    "sources_b": ((10 + 2) * 8, (15 + 3) * 8),    # rho, cs, pot, v, stresses, q -> v_temp
+    "sweep_z": (0, (4 + 7) * 8),                  # rho, v_temp -> rho + 6 momenta
+    "sweep_y": ((3 + 5) * 8, (8 + 7) * 8),        # (Z-swept) rho + momenta, vy_temp -> rho + momenta
+    "sweep_x": ((6 + 4) * 8, (8 + 6) * 8),        # rho, momenta, vx_temp -> rho, vx, momenta
+    "finish": ((3 + 2) * 8, (5 + 3) * 8),         # rho, y/z momenta -> rho, vy (, vz)
+    "potential": (1 * 8, 1 * 8),                  # -> pot
 }
+# kernel behind each stage (names in the rocprofv3 summaries)
+STAGE_KERNEL = {"cfl": "kf_cfl", "sources_a": "kf_src_a", "sources_b": "kf_src_b", "sweep_z": "kf_sweep_z",
+                "sweep_y": "kf_sweep_y", "sweep_x": "kf_sweep_x", "finish": "kf_finish", "potential": "k_potential"}
+# rocprofv3 summary with the measured HBM traffic of each kernel of this workload
+PROFILE = {"c2": "profiles/fused_step_c2_3d_256x96x32.md", "c1": "profiles/fused_step_c1_2d_384x128.md"}
+
+
+def measured_traffic(config, stage):
+    """HBM bytes per launch of the stage's kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (scaled by a copy of known size), or None."""
+    path = os.path.join(ROOT, PROFILE.get(config, ""))
+    if not os.path.isfile(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch = "):
+            table = json.loads(line.split("=", 1)[1])
+            for name, v in table.items():
+                if name.startswith(STAGE_KERNEL.get(stage, "?") + "<") or name == STAGE_KERNEL.get(stage):
+                    return v
+    return None
 
 
 def cpu_baseline(parfile, budget_s=12.0):
@@ -140,8 +160,9 @@ def main():
         avg_s = stages[dom]["avg_us"] * 1e-6
         achieved = per_cell * cells / avg_s / 1e9 if avg_s > 0 else 0.0
         roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                    "bytes_per_cell": per_cell}
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic(args.config, dom), "bytes_per_cell": per_cell,
+                    "bytes_per_launch": per_cell * cells}
     step_gbs = value * b_alg / 1e9
 
     cpu = None
