@@ -66,6 +66,7 @@ def lib():
         L.f3d_sim_set_scalars.argtypes = [vp, vp]
         L.f3d_sim_get_planets.argtypes = [vp, vp, i]
         L.f3d_sim_shape.argtypes = [vp, vp]
+        L.f3d_sim_eos.argtypes = [vp]
         L.f3d_sim_ctx.restype = vp
         L.f3d_sim_ctx.argtypes = [vp]
         L.f3d_sim_push_phys.argtypes = [vp]
@@ -93,6 +94,7 @@ class Simulation:
         shp = (ctypes.c_int * 4)()
         L.f3d_sim_shape(self._s, shp)
         self.nx, self.ny, self.nz, self.dimz = list(shp)
+        self.adiabatic = (L.f3d_sim_eos(self._s) == 2)
         sy = self.ny + 6
         sz = self.nz + 6 if self.dimz else 1
         self.shape = (sz, sy, self.nx)

@@ -534,6 +534,7 @@ __device__ __forceinline__ void stockholm(const dmesh& m, int j, int k, int l, d
 // ---------------------------------------------------------------- boundaries
 struct BcPar {
   int p3d;
+  int energy;  // Energy ghosts SYMMETRIC (centered |a|a|, std/boundaries.txt:55-57)
   double of;
   double dens[3], vsq[3], vfac[3], zvfac[3];
 };
@@ -542,7 +543,7 @@ struct BcPar {
 // 62-66, 81-85 formulas; field lists setups/fargo/fargo.bound.0,
 // setups/p3disof/p3disof.bound.0)
 __device__ __forceinline__ void bound_y_cell(const dmesh& m, int side, int i, int mm, int k, double* dens, double* vx,
-                                             double* vy, double* vz, const BcPar& b, const double* dexp) {
+                                             double* vy, double* vz, double* e, const BcPar& b, const double* dexp) {
   const int pitch = m.pitch, stride = m.stride, ny = m.ny, nghy = m.nghy;
   int lgh, lghs, lact, lacts, lnull, jgh, jact;
   if (!side) {
@@ -571,11 +572,12 @@ __device__ __forceinline__ void bound_y_cell(const dmesh& m, int side, int i, in
   }
   if (!side || mm < nghy - 1) vy[lghs] = -vy[lacts];
   vy[lnull] = 0;
+  if (b.energy) e[lgh] = e[lact];
 }
 
 // Z sides (scripts/boundparser.py:341-358; std/boundaries.txt:31-40, 87-91)
 __device__ __forceinline__ void bound_z_cell(const dmesh& m, int side, int i, int mm, int j, double* dens, double* vx,
-                                             double* vy, double* vz, const BcPar& b, const double* zdens) {
+                                             double* vy, double* vz, double* e, const BcPar& b, const double* zdens) {
   const int pitch = m.pitch, stride = m.stride, nz = m.nz, nghz = m.nghz;
   int lgh, lghs, lact, lacts, lnull, kgh, kact;
   if (!side) {
@@ -599,6 +601,7 @@ __device__ __forceinline__ void bound_z_cell(const dmesh& m, int side, int i, in
   vy[lgh] = vy[lact];
   if (!side || mm < nghz - 1) vz[lghs] = -vz[lacts];
   vz[lnull] = 0;
+  if (b.energy) e[lgh] = e[lact];
 }
 
 }  // namespace cell

@@ -38,8 +38,11 @@ namespace {
 constexpr int WG = 256;
 constexpr int BXF = 64;
 
-// fused scratch buffers (allocated on first use, each n3 doubles)
-enum { FS_RHOY = 0, FS_RHOX, FS_QX, FS_QY, FS_QZ, FS_Z0, FS_COUNT = FS_Z0 + 7 };
+// Scratch of the fused stages, n3 doubles each, allocated on a handle's
+// first fused call and released with the handle: Y- and X-swept density,
+// artificial pressure (3), energy after SubStep2_b and after the Y sweep, and
+// (3-D) the Z-swept density, six momenta and energy.
+enum { FS_RHOY = 0, FS_RHOX, FS_QX, FS_QY, FS_QZ, FS_E2, FS_EY, FS_Z0, FS_COUNT = FS_Z0 + 8 };
 
 struct Scratch {
   double* f[FS_COUNT];
@@ -47,31 +50,32 @@ struct Scratch {
   int slot;
 };
 
-Scratch g_scr[16];
-void* g_owner[16];
-
 constexpr unsigned long long kInfBits = 0x7FF0000000000000ull;
 
+void scratch_free(void* p) {
+  Scratch* S = static_cast<Scratch*>(p);
+  for (int a = 0; a < FS_COUNT; a++)
+    if (S->f[a]) (void)hipFree(S->f[a]);
+  if (S->result) (void)hipFree(S->result);
+  delete S;
+}
+
 int scratch_of(f3d_hip_ctx* ctx, Scratch** out) {
-  int slot = -1;
-  for (int s = 0; s < 16; s++)
-    if (g_owner[s] == ctx) slot = s;
-  if (slot < 0) {
-    for (int s = 0; s < 16 && slot < 0; s++)
-      if (!g_owner[s]) slot = s;
-    if (slot < 0) return -1;
-    Scratch& S = g_scr[slot];
-    memset(&S, 0, sizeof(S));
-    for (int a = 0; a < FS_COUNT; a++) {
-      if (hipMalloc(&S.f[a], ctx->n3 * sizeof(double)) != hipSuccess) return -1;
-      if (hipMemset(S.f[a], 0, ctx->n3 * sizeof(double)) != hipSuccess) return -1;
+  if (!ctx->fused) {
+    Scratch* S = new Scratch();
+    memset(S, 0, sizeof(*S));
+    ctx->fused = S;
+    ctx->fused_free = scratch_free;
+    const int nbuf = ctx->m.dimz ? FS_COUNT : FS_Z0;
+    for (int a = 0; a < nbuf; a++) {
+      if (hipMalloc(&S->f[a], ctx->n3 * sizeof(double)) != hipSuccess) return -1;
+      if (hipMemset(S->f[a], 0, ctx->n3 * sizeof(double)) != hipSuccess) return -1;
     }
-    if (hipMalloc(&S.result, 2 * sizeof(unsigned long long)) != hipSuccess) return -1;
+    if (hipMalloc(&S->result, 2 * sizeof(unsigned long long)) != hipSuccess) return -1;
     unsigned long long init[2] = {kInfBits, kInfBits};
-    if (hipMemcpy(S.result, init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess) return -1;
-    g_owner[slot] = ctx;
+    if (hipMemcpy(S->result, init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess) return -1;
   }
-  *out = &g_scr[slot];
+  *out = static_cast<Scratch*>(ctx->fused);
   return 0;
 }
 
@@ -246,8 +250,8 @@ __global__ void __launch_bounds__(WG) kf_src_b(dmesh m, double dt, const double*
                                                const double* pot, const double* txx, const double* tyy,
                                                const double* tzz, const double* txz, const double* tyx,
                                                const double* tzy, const double* qx, const double* qy,
-                                               const double* qz, double* vxt, double* vyt, double* vzt, int visc,
-                                               int potential, int iso, double gm1, double omegaframe,
+                                               const double* qz, double* vxt, double* vyt, double* vzt, double* e2,
+                                               int visc, int potential, int iso, double gm1, double omegaframe,
                                                double domega) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
@@ -269,12 +273,28 @@ __global__ void __launch_bounds__(WG) kf_src_b(dmesh m, double dt, const double*
     cell::addviscosity<G, DZ>(m, i, j, k, dt, R, cell::gl(txx, m, l, i), cell::gl(tyy, m, l, i), TZZ, TXZ,
                               cell::gl(tyx, m, l, i), TZY, ax, ay, az, ax, ay, az);
   }
-  double e_unused;
+  double e = iso ? 0.0 : energy[l];
   cell::substep2_b<G, DZ>(m, i, j, k, dt, R, VX, VY, VZ, cell::gl(qx, m, l, i), cell::gl(qy, m, l, i),
-                          cell::gl(DZ ? qz : qy, m, l, i), 0, ax, ay, az, 0.0, ax, ay, az, e_unused);
+                          cell::gl(DZ ? qz : qy, m, l, i), !iso, ax, ay, az, e, ax, ay, az, e);
   vxt[l] = ax;
   vyt[l] = ay;
   if (DZ) vzt[l] = az;
+  if (!iso) e2[l] = e;
+}
+
+// SubStep3 (src/substep3.c:98-110) on active cells: compression work with the
+// divergence of V_temp; reads the energy after SubStep2_b.
+template <int G, int DZ>
+__global__ void __launch_bounds__(WG) kf_substep3(dmesh m, double dt, const double* e2, const double* vxt,
+                                                  const double* vyt, const double* vzt, double* energy,
+                                                  double gamma) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int j = m.nghy + (int)blockIdx.y;
+  const int k = DZ ? m.nghz + (int)blockIdx.z : 0;
+  const int l = i + j * m.pitch + k * m.stride;
+  energy[l] = cell::substep3<DZ>(m, i, j, k, dt, e2[l], cell::gl(vxt, m, l, i), cell::gl(vyt, m, l, i),
+                                 cell::gl(DZ ? vzt : vyt, m, l, i), gamma);
 }
 
 // --------------------------------------------------------------- transport
@@ -351,12 +371,13 @@ __device__ __forceinline__ void column_sweep(Col& col, int r0, int r1, double dt
 }
 
 // Y column of the 2-D step: momenta computed from rho and V_temp on the fly
-// (src/momenta_x.c:76-79, src/momenta_y.c:62-63).
+// (src/momenta_x.c:76-79, src/momenta_y.c:62-63); quantity 4 is the energy
+// (adiabatic).  out_[c] receives quantity c, out_[NQ] the density.
 template <int G>
 struct ColY2 {
   const dmesh* m;
-  const double *rho_in, *vxt, *vyt;
-  double *rho_out, *q0, *q1, *q2, *q3;
+  const double *rho_in, *vxt, *vyt, *e_in;
+  double* out_[6];
   int i, ixp;
   double of, dt;
   __device__ __forceinline__ int L(int r) const { return i + r * m->pitch; }
@@ -369,24 +390,24 @@ struct ColY2 {
       cell::momenta_x<G>(*m, i, r, 0, rr, vxt[l], vxt[ixp + r * m->pitch], of, mm, mp);
       return c == 0 ? mp : mm;
     }
-    return c == 2 ? vyt[l + m->pitch] * rr : vyt[l] * rr;
+    if (c == 2) return vyt[l + m->pitch] * rr;
+    if (c == 3) return vyt[l] * rr;
+    return e_in[l];
   }
   __device__ __forceinline__ double zs(int r) const { return zone_size_y(*m, r); }
   __device__ __forceinline__ double v(int r) const { return vyt[L(r)]; }
   __device__ __forceinline__ double flux(int r, double a, double b) const { return cell::flux_y(*m, i, r, 0, dt, a, b); }
-  __device__ __forceinline__ void out(int c, int r, double val) const {
-    double* o = c == 0 ? q0 : c == 1 ? q1 : c == 2 ? q2 : c == 3 ? q3 : rho_out;
-    o[L(r)] = val;
-  }
+  __device__ __forceinline__ void out(int c, int r, double val) const { out_[c][L(r)] = val; }
 };
 
 // Z column of the 3-D step: momenta from rho and V_temp on the fly
-// (src/momenta_x.c:80-84, src/momenta_y.c:62-63, src/momenta_z.c:66-71).
+// (src/momenta_x.c:80-84, src/momenta_y.c:62-63, src/momenta_z.c:66-71);
+// quantity 6 is the energy (adiabatic).
 template <int G>
 struct ColZ {
   const dmesh* m;
-  const double *rho_in, *vxt, *vyt, *vzt;
-  double* out_[7];
+  const double *rho_in, *vxt, *vyt, *vzt, *e_in;
+  double* out_[8];
   int i, j, ixp;
   double of, dt;
   __device__ __forceinline__ int L(int r) const { return i + j * m->pitch + r * m->stride; }
@@ -410,11 +431,13 @@ struct ColZ {
         if (G == SPH) p *= m->ymed[j];
         return p;
       }
-      default: {
+      case 5: {
         double q = vzt[l] * rr;
         if (G == SPH) q *= m->ymed[j];
         return q;
       }
+      default:
+        return e_in[l];
     }
   }
   __device__ __forceinline__ double zs(int r) const { return zone_size_z<G>(*m, j, r); }
@@ -423,13 +446,13 @@ struct ColZ {
   __device__ __forceinline__ void out(int c, int r, double val) const { out_[c][L(r)] = val; }
 };
 
-// Y column of the 3-D step: reads the Z-swept density and momenta.
+// Y column of the 3-D step: reads the Z-swept density, momenta (and energy).
 struct ColY3 {
   const dmesh* m;
   const double* rho_in;
-  const double* q_in[6];
+  const double* q_in[7];
   const double* vyt;
-  double* out_[7];
+  double* out_[8];
   int i, k;
   double dt;
   __device__ __forceinline__ int L(int r) const { return i + r * m->pitch + k * m->stride; }
@@ -441,61 +464,71 @@ struct ColY3 {
   __device__ __forceinline__ void out(int c, int r, double val) const { out_[c][L(r)] = val; }
 };
 
-// 2-D: momenta + Y sweep; rows [nghy + s*rows_per, ...) of the active range
-template <int G, int DZ>
+// 2-D: momenta + Y sweep; rows [nghy + s*rows_per, ...) of the active range.
+// AD = 1 also transports the energy (e_in -> e_out).
+template <int G, int DZ, int AD>
 __global__ void __launch_bounds__(BXF) kf_sweep_y2(dmesh m, double dt, const double* rho_in, const double* vxt,
-                                                   const double* vyt, double* rho_out, double* q0, double* q1,
-                                                   double* q2, double* q3, int rows_per, double of) {
+                                                   const double* vyt, const double* e_in, double* rho_out,
+                                                   double* q0, double* q1, double* q2, double* q3, double* e_out,
+                                                   int rows_per, double of) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int r0 = m.nghy + (int)blockIdx.y * rows_per;
   const int r1 = min(r0 + rows_per, m.ny + m.nghy);
-  ColY2<G> col{&m, rho_in, vxt, vyt, rho_out, q0, q1, q2, q3, i, ixp_of(i, m.nx), of, dt};
-  column_sweep<4>(col, r0, r1, dt);
+  ColY2<G> col{&m, rho_in, vxt, vyt, e_in, {q0, q1, q2, q3, AD ? e_out : rho_out, rho_out}, i, ixp_of(i, m.nx), of,
+               dt};
+  column_sweep<4 + AD>(col, r0, r1, dt);
 }
 
 // 3-D: Y sweep of the Z-swept fields
-template <int G, int DZ>
+template <int G, int DZ, int AD>
 __global__ void __launch_bounds__(BXF) kf_sweep_y3(dmesh m, double dt, const double* rho_in, const double* z0,
                                                    const double* z1, const double* z2, const double* z3,
-                                                   const double* z4, const double* z5, const double* vyt,
-                                                   double* rho_out, double* q0, double* q1, double* q2, double* q3,
-                                                   double* q4, double* q5, int rows_per) {
+                                                   const double* z4, const double* z5, const double* ze,
+                                                   const double* vyt, double* rho_out, double* q0, double* q1,
+                                                   double* q2, double* q3, double* q4, double* q5, double* e_out,
+                                                   int rows_per) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int k = m.nghz + (int)blockIdx.z;
   const int r0 = m.nghy + (int)blockIdx.y * rows_per;
   const int r1 = min(r0 + rows_per, m.ny + m.nghy);
-  ColY3 col{&m, rho_in, {z0, z1, z2, z3, z4, z5}, vyt, {q0, q1, q2, q3, q4, q5, rho_out}, i, k, dt};
-  column_sweep<6>(col, r0, r1, dt);
+  ColY3 col{&m, rho_in, {z0, z1, z2, z3, z4, z5, ze}, vyt,
+            {q0, q1, q2, q3, q4, q5, AD ? e_out : rho_out, rho_out}, i, k, dt};
+  column_sweep<6 + AD>(col, r0, r1, dt);
 }
 
 // 3-D: momenta + Z sweep for rows j in [1, sy-2] (the Y sweep's stencil)
-template <int G, int DZ>
+template <int G, int DZ, int AD>
 __global__ void __launch_bounds__(BXF) kf_sweep_z(dmesh m, double dt, const double* rho_in, const double* vxt,
-                                                  const double* vyt, const double* vzt, double* rho_out, double* q0,
-                                                  double* q1, double* q2, double* q3, double* q4, double* q5,
-                                                  int planes_per, double of) {
+                                                  const double* vyt, const double* vzt, const double* e_in,
+                                                  double* rho_out, double* q0, double* q1, double* q2, double* q3,
+                                                  double* q4, double* q5, double* e_out, int planes_per, double of) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int j = 1 + (int)blockIdx.y;
   const int r0 = m.nghz + (int)blockIdx.z * planes_per;
   const int r1 = min(r0 + planes_per, m.nz + m.nghz);
-  ColZ<G> col{&m, rho_in, vxt, vyt, vzt, {q0, q1, q2, q3, q4, q5, rho_out}, i, j, ixp_of(i, m.nx), of, dt};
-  column_sweep<6>(col, r0, r1, dt);
+  ColZ<G> col{&m, rho_in, vxt, vyt, vzt, e_in, {q0, q1, q2, q3, q4, q5, AD ? e_out : rho_out, rho_out}, i, j,
+              ixp_of(i, m.nx), of, dt};
+  column_sweep<6 + AD>(col, r0, r1, dt);
 }
 
 // X sweep, ring-owning: VxMed of Vx_temp, residual split, van Leer pass with
 // the cell residual, PPA pass with the uniform residual, integer shift and
 // NewVelocity_x (src/transport.c:129-192).  LDS holds the ring; every
 // quantity of a pass goes through the same phase between two barriers.
-// Thread t owns cells i = t + s*blockDim.x, s < CPT.
-template <int G, int DZ, int CPT>
+// Thread t owns cells i = t + s*blockDim.x, s < CPT.  Quantities: the
+// momenta in the order of src/transport.c (Mpx, Mmx, Mpy, Mmy[, Mpz, Mmz]),
+// then with AD the energy (e_in -> e_out), then the density.
+template <int G, int DZ, int AD, int CPT>
 __global__ void __launch_bounds__(256) kf_sweep_x(dmesh m, double dt, const double* rho_in, const double* vxt,
                                                    double* mpx, double* mmx, double* mpy, double* mmy, double* mpz,
-                                                   double* mmz, double* rho_out, double* vx_out, double* vxmed_out,
-                                                   int* nshift_out, double omegaframe, int fargo) {
-  constexpr int NQ = DZ ? 6 : 4;
+                                                   double* mmz, const double* e_in, double* e_out, double* rho_out,
+                                                   double* vx_out, double* vxmed_out, int* nshift_out,
+                                                   double omegaframe, int fargo) {
+  constexpr int NM = DZ ? 6 : 4;  // momenta
+  constexpr int NQ = NM + AD;
   constexpr int N = NQ + 1;  // quantities + density (index NQ)
   extern __shared__ double lds[];
   const int nx = m.nx;
@@ -518,7 +551,7 @@ __global__ void __launch_bounds__(256) kf_sweep_x(dmesh m, double dt, const doub
     if (i < nx) {
       ST[NQ * nx + i] = rho_in[base + i];
       #pragma unroll
-      for (int c = 0; c < NQ; c++) ST[c * nx + i] = qg[c][base + i];
+      for (int c = 0; c < NQ; c++) ST[c * nx + i] = (c < NM) ? qg[c][base + i] : e_in[base + i];
       vt[s] = vxt[base + i];
       V[i] = vt[s] * (m.xmin[i + 1] - m.xmin[i]);
     }
@@ -722,7 +755,10 @@ __global__ void __launch_bounds__(256) kf_sweep_x(dmesh m, double dt, const doub
       vx_out[base + i] = cell::newvel_x<G>(m, i, j, k, R[i], R[im], OUT[1 * nx + i], OUT[0 * nx + im], omegaframe);
       rho_out[base + i] = R[i];
 #pragma unroll
-      for (int c = 2; c < NQ; c++) qg[c][base + i] = OUT[c * nx + i];
+      for (int c = 2; c < NQ; c++) {
+        if (c < NM) qg[c][base + i] = OUT[c * nx + i];
+        else e_out[base + i] = OUT[c * nx + i];
+      }
     }
   }
 }
@@ -771,6 +807,30 @@ __global__ void __launch_bounds__(WG) kf_finish(dmesh m, double dt, const double
     F3D_LAUNCH_CHECK(ctx);                                                                              \
   } while (0)
 
+// the same with a third template argument AD (1: adiabatic, the energy is transported)
+#define FDISPATCH_AD(ctx, KERN, grid, block, shmem, ...)                                                 \
+  do {                                                                                                    \
+    const bool ad_ = ((ctx)->ph.eos == 2);                                                                \
+    if ((ctx)->m.geom == SPH) {                                                                           \
+      if ((ctx)->m.dimz) {                                                                                \
+        if (ad_) hipLaunchKernelGGL((KERN<SPH, 1, 1>), grid, block, shmem, (ctx)->stream, (ctx)->m, __VA_ARGS__); \
+        else hipLaunchKernelGGL((KERN<SPH, 1, 0>), grid, block, shmem, (ctx)->stream, (ctx)->m, __VA_ARGS__);     \
+      } else {                                                                                            \
+        if (ad_) hipLaunchKernelGGL((KERN<SPH, 0, 1>), grid, block, shmem, (ctx)->stream, (ctx)->m, __VA_ARGS__); \
+        else hipLaunchKernelGGL((KERN<SPH, 0, 0>), grid, block, shmem, (ctx)->stream, (ctx)->m, __VA_ARGS__);     \
+      }                                                                                                   \
+    } else {                                                                                              \
+      if ((ctx)->m.dimz) {                                                                                \
+        if (ad_) hipLaunchKernelGGL((KERN<CYL, 1, 1>), grid, block, shmem, (ctx)->stream, (ctx)->m, __VA_ARGS__); \
+        else hipLaunchKernelGGL((KERN<CYL, 1, 0>), grid, block, shmem, (ctx)->stream, (ctx)->m, __VA_ARGS__);     \
+      } else {                                                                                            \
+        if (ad_) hipLaunchKernelGGL((KERN<CYL, 0, 1>), grid, block, shmem, (ctx)->stream, (ctx)->m, __VA_ARGS__); \
+        else hipLaunchKernelGGL((KERN<CYL, 0, 0>), grid, block, shmem, (ctx)->stream, (ctx)->m, __VA_ARGS__);     \
+      }                                                                                                   \
+    }                                                                                                     \
+    F3D_LAUNCH_CHECK(ctx);                                                                                \
+  } while (0)
+
 static cell::CflPar cfl_par_f(const f3d_hip_ctx* ctx) {
   cell::CflPar c;
   c.cfl = ctx->ph.cfl;
@@ -800,7 +860,6 @@ int f3d_fused_set_tiling(int rows_y, int rows_z) {
 int f3d_fused_cfl(f3d_hip_ctx* ctx, double* fluid_min) {
   Scratch* S;
   F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
-  F3D_REQUIRE(ctx, ctx->ph.eos == 1, "the fused path covers the isothermal equation of state");
   const dmesh& m = ctx->m;
   const int nrings = m.ny * (m.dimz ? m.nz : 1);
   const size_t shmem = (2 * m.nx + 16) * sizeof(double);
@@ -819,7 +878,6 @@ int f3d_fused_cfl(f3d_hip_ctx* ctx, double* fluid_min) {
 int f3d_fused_sources(f3d_hip_ctx* ctx, double dt, double domega) {
   Scratch* S;
   F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
-  F3D_REQUIRE(ctx, ctx->ph.eos == 1, "the fused path covers the isothermal equation of state");
   const dmesh& m = ctx->m;
   const f3d_fields& f = ctx->f;
   cell::ViscPar vp;
@@ -827,7 +885,8 @@ int f3d_fused_sources(f3d_hip_ctx* ctx, double dt, double domega) {
   vp.nu = ctx->ph.nu;
   vp.alpha = ctx->ph.alpha;
   vp.gamma = ctx->ph.gamma;
-  vp.iso = 1;
+  const int iso = (ctx->ph.eos != 2);
+  vp.iso = iso;
   const int visc = ctx->ph.visc != 0 && ctx->ph.fluidtype == 1;
   const int pot = ctx->ph.potential;
   dim3 ga((m.nx + BXF - 1) / BXF, m.sy - 2, m.dimz ? m.sz - 2 : 1);
@@ -835,12 +894,24 @@ int f3d_fused_sources(f3d_hip_ctx* ctx, double dt, double domega) {
   FDISPATCH(ctx, kf_src_a, ga, dim3(BXF), 0, f.Density, f.Energy, f.Vx, f.Vy, f.Vz, f.Mmx, f.Mmy, f.Mmz, f.Mpx,
             f.Mpy, f.Mpz, S->f[FS_QX], S->f[FS_QY], S->f[FS_QZ], vp, visc, domega);
   f3d_stage_end(ctx, F3D_ST_SRC_A);
-  dim3 gb((m.nx + BXF - 1) / BXF, m.ny, m.dimz ? m.nz : 1);
+  // adiabatic: SubStep3 reads V_temp one row (and plane) past the active
+  // range, which the reference computes there with the same formulas
+  // (src/substep1_y.c:78-80, src/substep2_b.c:84-86), so those are included
+  const int ext = iso ? 0 : 1;
+  dim3 gb((m.nx + BXF - 1) / BXF, m.ny + ext, m.dimz ? m.nz + ext : 1);
   f3d_stage_begin(ctx, F3D_ST_SRC_B);
   FDISPATCH(ctx, kf_src_b, gb, dim3(BXF), 0, dt, f.Density, f.Energy, f.Vx, f.Vy, f.Vz, f.Pot, f.Mmx, f.Mmy, f.Mmz,
-            f.Mpx, f.Mpy, f.Mpz, S->f[FS_QX], S->f[FS_QY], S->f[FS_QZ], f.Vx_temp, f.Vy_temp, f.Vz_temp, visc, pot,
-            1, ctx->ph.gamma - 1.0, ctx->ph.OMEGAFRAME, domega);
+            f.Mpx, f.Mpy, f.Mpz, S->f[FS_QX], S->f[FS_QY], S->f[FS_QZ], f.Vx_temp, f.Vy_temp, f.Vz_temp,
+            S->f[FS_E2], visc, pot, iso, ctx->ph.gamma - 1.0, ctx->ph.OMEGAFRAME, domega);
   f3d_stage_end(ctx, F3D_ST_SRC_B);
+  if (!iso && ctx->ph.fluidtype == 1) {
+    // SubStep3 (src/algogas.c:110-112)
+    dim3 g3((m.nx + BXF - 1) / BXF, m.ny, m.dimz ? m.nz : 1);
+    f3d_stage_begin(ctx, F3D_ST_SUBSTEP3);
+    FDISPATCH(ctx, kf_substep3, g3, dim3(BXF), 0, dt, S->f[FS_E2], f.Vx_temp, f.Vy_temp, f.Vz_temp, f.Energy,
+              ctx->ph.gamma);
+    f3d_stage_end(ctx, F3D_ST_SUBSTEP3);
+  }
   // mid-step ghost fill (src/algogas.c:116-118) applied to V_temp directly
   f3d_stage_begin(ctx, F3D_ST_BC_MID);
   if (f3d_boundaries_on(ctx, f.Density, f.Vx_temp, f.Vy_temp, f.Vz_temp)) return -1;
@@ -851,7 +922,6 @@ int f3d_fused_sources(f3d_hip_ctx* ctx, double dt, double domega) {
 int f3d_fused_transport(f3d_hip_ctx* ctx, double dt, int end_of_step) {
   Scratch* S;
   F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
-  F3D_REQUIRE(ctx, ctx->ph.eos == 1, "the fused path covers the isothermal equation of state");
   const dmesh& m = ctx->m;
   const f3d_fields& f = ctx->f;
   const double of = ctx->ph.OMEGAFRAME;
@@ -861,34 +931,37 @@ int f3d_fused_transport(f3d_hip_ctx* ctx, double dt, int end_of_step) {
     double* const* Z = &S->f[FS_Z0];
     dim3 gz((m.nx + BXF - 1) / BXF, m.sy - 2, (m.nz + rz - 1) / rz);
     f3d_stage_begin(ctx, F3D_ST_SWEEP_Z);
-    FDISPATCH(ctx, kf_sweep_z, gz, dim3(BXF), 0, dt, f.Density, f.Vx_temp, f.Vy_temp, f.Vz_temp, Z[6], Z[0], Z[1],
-              Z[2], Z[3], Z[4], Z[5], rz, of);
+    FDISPATCH_AD(ctx, kf_sweep_z, gz, dim3(BXF), 0, dt, f.Density, f.Vx_temp, f.Vy_temp, f.Vz_temp, f.Energy, Z[6],
+                 Z[0], Z[1], Z[2], Z[3], Z[4], Z[5], Z[7], rz, of);
     f3d_stage_end(ctx, F3D_ST_SWEEP_Z);
     dim3 gy((m.nx + BXF - 1) / BXF, (m.ny + ry - 1) / ry, m.nz);
     f3d_stage_begin(ctx, F3D_ST_SWEEP_Y);
-    FDISPATCH(ctx, kf_sweep_y3, gy, dim3(BXF), 0, dt, Z[6], Z[0], Z[1], Z[2], Z[3], Z[4], Z[5], f.Vy_temp,
-              S->f[FS_RHOY], f.Mpx, f.Mmx, f.Mpy, f.Mmy, f.Mpz, f.Mmz, ry);
+    FDISPATCH_AD(ctx, kf_sweep_y3, gy, dim3(BXF), 0, dt, Z[6], Z[0], Z[1], Z[2], Z[3], Z[4], Z[5], Z[7], f.Vy_temp,
+                 S->f[FS_RHOY], f.Mpx, f.Mmx, f.Mpy, f.Mmy, f.Mpz, f.Mmz, S->f[FS_EY], ry);
     f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
   } else {
     dim3 gy((m.nx + BXF - 1) / BXF, (m.ny + ry - 1) / ry, 1);
     f3d_stage_begin(ctx, F3D_ST_SWEEP_Y);
-    FDISPATCH(ctx, kf_sweep_y2, gy, dim3(BXF), 0, dt, f.Density, f.Vx_temp, f.Vy_temp, S->f[FS_RHOY], f.Mpx, f.Mmx,
-              f.Mpy, f.Mmy, ry, of);
+    FDISPATCH_AD(ctx, kf_sweep_y2, gy, dim3(BXF), 0, dt, f.Density, f.Vx_temp, f.Vy_temp, f.Energy, S->f[FS_RHOY],
+                 f.Mpx, f.Mmx, f.Mpy, f.Mmy, S->f[FS_EY], ry, of);
     f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
   }
   // X sweep: one workgroup of at most 256 threads per active ring, up to 4
   // cells per thread
-  const int nq1 = (m.dimz ? 6 : 4) + 1;
+  const int ad = (ctx->ph.eos == 2);
+  const int nq1 = (m.dimz ? 6 : 4) + ad + 1;
   const size_t shx = ((size_t)5 * nq1 * m.nx + m.nx + 8) * sizeof(double);
   F3D_REQUIRE(ctx, shx <= 160 * 1024 && m.nx <= 1024, "ring too long for the LDS X sweep");
   const int cpt = (m.nx <= 256) ? 1 : (m.nx <= 512) ? 2 : 4;
   const int nthreads = (((m.nx + cpt - 1) / cpt + 63) / 64) * 64;
   dim3 gx(m.ny * (m.dimz ? m.nz : 1));
   f3d_stage_begin(ctx, F3D_ST_SWEEP_X);
-#define XSWEEP(G_, DZ_, CPT_)                                                                                  \
-  hipLaunchKernelGGL((kf_sweep_x<G_, DZ_, CPT_>), gx, dim3(nthreads), shx, ctx->stream, m, dt, S->f[FS_RHOY], \
-                     f.Vx_temp, f.Mpx, f.Mmx, f.Mpy, f.Mmy, f.Mpz, f.Mmz, S->f[FS_RHOX], f.Vx, f.VxMed, f.Nshift, \
-                     of, ctx->ph.fargo)
+#define XSWEEP1(G_, DZ_, AD_, CPT_)                                                                           \
+  hipLaunchKernelGGL((kf_sweep_x<G_, DZ_, AD_, CPT_>), gx, dim3(nthreads), shx, ctx->stream, m, dt,              \
+                     S->f[FS_RHOY], f.Vx_temp, f.Mpx, f.Mmx, f.Mpy, f.Mmy, f.Mpz, f.Mmz, S->f[FS_EY], f.Energy,   \
+                     S->f[FS_RHOX], f.Vx, f.VxMed, f.Nshift, of, ctx->ph.fargo)
+#define XSWEEP(G_, DZ_, CPT_) \
+  do { if (ad) XSWEEP1(G_, DZ_, 1, CPT_); else XSWEEP1(G_, DZ_, 0, CPT_); } while (0)
 #define XSWEEP_G(G_, DZ_) \
   do { if (cpt == 1) XSWEEP(G_, DZ_, 1); else if (cpt == 2) XSWEEP(G_, DZ_, 2); else XSWEEP(G_, DZ_, 4); } while (0)
   if (m.geom == SPH) {
@@ -897,6 +970,7 @@ int f3d_fused_transport(f3d_hip_ctx* ctx, double dt, int end_of_step) {
     if (m.dimz) XSWEEP_G(CYL, 1); else XSWEEP_G(CYL, 0);
   }
 #undef XSWEEP_G
+#undef XSWEEP1
 #undef XSWEEP
   F3D_LAUNCH_CHECK(ctx);
   f3d_stage_end(ctx, F3D_ST_SWEEP_X);

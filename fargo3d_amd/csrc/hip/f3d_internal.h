@@ -20,6 +20,7 @@ enum {
   F3D_ST_FINISH,
   F3D_ST_BC_END,
   F3D_ST_POTENTIAL,
+  F3D_ST_SUBSTEP3,
   F3D_NSTAGES
 };
 
@@ -39,6 +40,8 @@ struct f3d_hip_ctx {
   double* hres = nullptr;           // pinned host result slots (64 doubles)
   size_t n3 = 0, n2 = 0;
   char err[512] = {0};
+  void* fused = nullptr;             // scratch of the fused stages, released with the handle
+  void (*fused_free)(void*) = nullptr;
   // per-stage event timing of the fused kernels (f3d_hip_timing_*)
   int timing = 0;
   hipEvent_t ev[F3D_NSTAGES][2] = {};

@@ -529,23 +529,23 @@ __global__ void k_vmed_div(dmesh m, const double* red2d, double* vxmed, int n2) 
 // side 1 in the same launch.  With ny > nghy the two sides write disjoint rows
 // and neither reads a row the other writes, so the reference's order (ymin
 // first, src/boundary.c:2-23) does not change any value.
-__global__ void k_bound_y(dmesh m, int side0, double* dens, double* vx, double* vy, double* vz, cell::BcPar bp0,
+__global__ void k_bound_y(dmesh m, int side0, double* dens, double* vx, double* vy, double* vz, double* e, cell::BcPar bp0,
                           cell::BcPar bp1, const double* dexp0, const double* dexp1) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int side = side0 + ((int)blockIdx.y >= m.nghy);
   const int mm = (int)blockIdx.y - (side - side0) * m.nghy, k = blockIdx.z;
-  cell::bound_y_cell(m, side, i, mm, k, dens, vx, vy, vz, side ? bp1 : bp0, side ? dexp1 : dexp0);
+  cell::bound_y_cell(m, side, i, mm, k, dens, vx, vy, vz, e, side ? bp1 : bp0, side ? dexp1 : dexp0);
 }
 
 // Z ghost fill, the same arrangement as k_bound_y (both sides when nz > nghz).
-__global__ void k_bound_z(dmesh m, int side0, double* dens, double* vx, double* vy, double* vz, cell::BcPar bp0,
+__global__ void k_bound_z(dmesh m, int side0, double* dens, double* vx, double* vy, double* vz, double* e, cell::BcPar bp0,
                           cell::BcPar bp1, const double* zdens0, const double* zdens1) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int side = side0 + ((int)blockIdx.z >= m.nghz);
   const int mm = (int)blockIdx.z - (side - side0) * m.nghz, j = blockIdx.y;
-  cell::bound_z_cell(m, side, i, mm, j, dens, vx, vy, vz, side ? bp1 : bp0, side ? zdens1 : zdens0);
+  cell::bound_z_cell(m, side, i, mm, j, dens, vx, vy, vz, e, side ? bp1 : bp0, side ? zdens1 : zdens0);
 }
 
 __global__ void k_init_stockholm(dmesh m, const double* rho, const double* vx, const double* vy, const double* vz,
@@ -737,6 +737,7 @@ int f3d_copy_velocities(f3d_hip_ctx* ctx, int option) {
 static cell::BcPar bc_par(const f3d_hip_ctx* ctx, int side) {
   cell::BcPar b;
   b.p3d = (ctx->ph.bc == 2);
+  b.energy = ctx->ph.energy_bc && ctx->f.Energy != nullptr;
   b.of = ctx->ph.OMEGAFRAME;
   for (int q = 0; q < 3; q++) {
     b.dens[q] = side ? ctx->ph.ymax_dens[q] : ctx->ph.ymin_dens[q];
@@ -767,7 +768,7 @@ int f3d_boundaries_on(f3d_hip_ctx* ctx, double* dens, double* vx, double* vy, do
     const cell::BcPar b0 = bc_par(ctx, 0), b1 = bc_par(ctx, 1);
     if (m.nz > m.nghz) {
       dim3 g((m.nx + BX - 1) / BX, m.sy, 2 * m.nghz);
-      hipLaunchKernelGGL(k_bound_z, g, blk, 0, ctx->stream, m, 0, f.Density, f.Vx, f.Vy, f.Vz, b0, b1,
+      hipLaunchKernelGGL(k_bound_z, g, blk, 0, ctx->stream, m, 0, f.Density, f.Vx, f.Vy, f.Vz, f.Energy, b0, b1,
                          ctx->d_zmin_dens, ctx->d_zmax_dens);
       F3D_LAUNCH_CHECK(ctx);
     } else {
@@ -775,7 +776,7 @@ int f3d_boundaries_on(f3d_hip_ctx* ctx, double* dens, double* vx, double* vy, do
         dim3 g((m.nx + BX - 1) / BX, m.sy, m.nghz);
         const cell::BcPar b = side ? b1 : b0;
         const double* t = side ? ctx->d_zmax_dens : ctx->d_zmin_dens;
-        hipLaunchKernelGGL(k_bound_z, g, blk, 0, ctx->stream, m, side, f.Density, f.Vx, f.Vy, f.Vz, b, b, t, t);
+        hipLaunchKernelGGL(k_bound_z, g, blk, 0, ctx->stream, m, side, f.Density, f.Vx, f.Vy, f.Vz, f.Energy, b, b, t, t);
         F3D_LAUNCH_CHECK(ctx);
       }
     }
@@ -784,7 +785,7 @@ int f3d_boundaries_on(f3d_hip_ctx* ctx, double* dens, double* vx, double* vy, do
     const cell::BcPar b0 = bc_par(ctx, 0), b1 = bc_par(ctx, 1);
     if (m.ny > m.nghy) {
       dim3 g((m.nx + BX - 1) / BX, 2 * m.nghy, m.dimz ? m.sz : 1);
-      hipLaunchKernelGGL(k_bound_y, g, blk, 0, ctx->stream, m, 0, f.Density, f.Vx, f.Vy, f.Vz, b0, b1,
+      hipLaunchKernelGGL(k_bound_y, g, blk, 0, ctx->stream, m, 0, f.Density, f.Vx, f.Vy, f.Vz, f.Energy, b0, b1,
                          ctx->d_ymin_dexp, ctx->d_ymax_dexp);
       F3D_LAUNCH_CHECK(ctx);
     } else {
@@ -792,7 +793,7 @@ int f3d_boundaries_on(f3d_hip_ctx* ctx, double* dens, double* vx, double* vy, do
         dim3 g((m.nx + BX - 1) / BX, m.nghy, m.dimz ? m.sz : 1);
         const cell::BcPar b = side ? b1 : b0;
         const double* t = side ? ctx->d_ymax_dexp : ctx->d_ymin_dexp;
-        hipLaunchKernelGGL(k_bound_y, g, blk, 0, ctx->stream, m, side, f.Density, f.Vx, f.Vy, f.Vz, b, b, t, t);
+        hipLaunchKernelGGL(k_bound_y, g, blk, 0, ctx->stream, m, side, f.Density, f.Vx, f.Vy, f.Vz, f.Energy, b, b, t, t);
         F3D_LAUNCH_CHECK(ctx);
       }
     }

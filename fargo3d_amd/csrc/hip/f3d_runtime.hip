@@ -77,7 +77,8 @@ int f3d_hip_create(const f3d_mesh_desc* d, f3d_hip_ctx** out) {
 }
 
 static const char* kStageNames[F3D_NSTAGES] = {"cfl", "sources_a", "sources_b", "ghosts_mid", "sweep_z",
-                                                "sweep_y", "sweep_x", "finish", "ghosts_end", "potential"};
+                                                "sweep_y", "sweep_x", "finish", "ghosts_end", "potential",
+                                                "substep3"};
 
 int f3d_hip_stage_count(void) { return F3D_NSTAGES; }
 const char* f3d_hip_stage_name(int s) { return (s >= 0 && s < F3D_NSTAGES) ? kStageNames[s] : ""; }
@@ -120,6 +121,7 @@ void f3d_hip_destroy(f3d_hip_ctx* c) {
   for (int s = 0; s < F3D_NSTAGES; s++)
     for (int e = 0; e < 2; e++)
       if (c->ev[s][e]) hipEventDestroy(c->ev[s][e]);
+  if (c->fused && c->fused_free) c->fused_free(c->fused);
   if (c->meshbuf) hipFree(c->meshbuf);
   if (c->bctab) hipFree(c->bctab);
   if (c->dred) hipFree(c->dred);

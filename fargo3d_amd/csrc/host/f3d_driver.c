@@ -50,6 +50,8 @@ struct f3d_sim {
 const char *f3d_sim_error(f3d_sim *S) { return S ? S->err : "no simulation"; }
 f3d_hip_ctx *f3d_sim_ctx(f3d_sim *S) { return S->ctx; }
 
+int f3d_sim_eos(f3d_sim *S) { return S->p.eos; }
+
 void f3d_sim_shape(f3d_sim *S, int out[4]) {
   out[0] = S->g.nx;
   out[1] = S->g.ny;
@@ -80,6 +82,7 @@ int f3d_sim_push_phys(f3d_sim *S) {
   ph->periodicz = S->p.periodicz;
   ph->periodicy = S->p.periodicy;
   ph->bc = S->p.bc;
+  ph->energy_bc = S->p.energy_bc;
   ph->nu = S->p.nu;
   ph->alpha = S->p.alpha;
   ph->gamma = S->p.gamma;

@@ -47,6 +47,7 @@ void f3d_sim_get_scalars(f3d_sim *S, double out[8]);
 void f3d_sim_set_scalars(f3d_sim *S, const double in[8]);
 void f3d_sim_get_planets(f3d_sim *S, double *out, int maxn);
 void f3d_sim_shape(f3d_sim *S, int out[4]); /* nx, ny, nz, dimz */
+int f3d_sim_eos(f3d_sim *S);                 /* 1 isothermal, 2 adiabatic */
 f3d_hip_ctx *f3d_sim_ctx(f3d_sim *S);
 /* refresh the kernels' copy of the run parameters and host scalars */
 int f3d_sim_push_phys(f3d_sim *S);

@@ -139,6 +139,7 @@ void f3d_params_defaults(f3d_params *p) {
   p->stockholm = 1;
   p->fargo = 1;
   p->bc = F3D_BC_FARGO2D;
+  p->energy_bc = 0;
   p->nfluids = 1;
   p->ic_keplerian = 0;
   p->seed = 0;
@@ -157,6 +158,8 @@ int f3d_params_setup_options(f3d_params *p, const char *setup) {
     p->geometry = F3D_CYLINDRICAL; p->dimz = 0; p->eos = F3D_ADIABATIC;
     p->visc = F3D_VISC_NU; p->potential = 1; p->stockholm = 1; p->fargo = 1;
     p->bc = F3D_BC_FARGO2D;
+    /* a setup-local .bound entry "Energy: Ymin/Ymax: SYMMETRIC" */
+    p->energy_bc = 1;
   } else if (strcmp(setup, "p3disof") == 0 || strcmp(setup, "p3disof_alpha") == 0) {
     /* setups/p3disof/p3disof.opt:15-33 (ALPHAVISCOSITY variant on request) */
     p->geometry = F3D_SPHERICAL; p->dimz = 1; p->eos = F3D_ISOTHERMAL;
@@ -207,6 +210,7 @@ int f3d_params_set(f3d_params *p, const char *name_in, const char *value) {
   if (strcmp(name, "STOCKHOLM") == 0) { p->stockholm = (toupper((unsigned char)value[0]) == 'Y'); return 0; }
   if (strcmp(name, "FARGOADVECTION") == 0) { p->fargo = (toupper((unsigned char)value[0]) == 'Y'); return 0; }
   if (strcmp(name, "POTENTIALTERM") == 0) { p->potential = (toupper((unsigned char)value[0]) == 'Y'); return 0; }
+  if (strcmp(name, "ENERGYBOUNDARY") == 0) { p->energy_bc = (toupper((unsigned char)value[0]) == 'S'); return 0; }
   if (strcmp(name, "SEED") == 0) { p->seed = strtoull(value, NULL, 10); return 0; }
   for (int i = 0; IGNORED[i]; i++)
     if (strcmp(name, IGNORED[i]) == 0) return 0;

@@ -66,6 +66,7 @@ typedef struct f3d_phys {
   int visc;            /* 0 none, 1 NU, 2 ALPHA */
   int potential, fargo, stockholm, indirectterm, periodicz, periodicy;
   int bc;              /* 1: 2-D Keplerian disk boundaries, 2: 3-D spherical disk boundaries */
+  int energy_bc;       /* 1: Energy ghosts SYMMETRIC on the non-periodic sides (setup .bound entry) */
   double nu, alpha, gamma, cfl;
   double OMEGAFRAME, OMEGAFRAME0, PhysicalTime;
   double ymin_par, ymax_par, zmin_par, zmax_par, taudamp;

@@ -772,6 +772,7 @@ static void bound_z(orc_state *S, int side) {
         vy[lgh] = vy[lact];
         if (!side || m < NGHZ - 1) vz[lghs] = -vz[lacts];
         vz[lnull] = 0;
+        if (S->p.energy_bc) S->Energy[lgh] = S->Energy[lact];
       }
 }
 
@@ -816,6 +817,8 @@ static void bound_y(orc_state *S, int side) {
         }
         if (!side || m < NGHY - 1) vy[lghs] = -vy[lacts];
         vy[lnull] = 0;
+        /* SYMMETRIC, centered |a|a| (std/boundaries.txt:55-57) */
+        if (S->p.energy_bc) S->Energy[lgh] = S->Energy[lact];
       }
 }
 

@@ -17,6 +17,10 @@ pytestmark = pytest.mark.gpu
 C1 = "fargo3d_amd/setups/c1_fargo2d.par"
 C2 = "fargo3d_amd/setups/c2_p3d.par"
 C2_SMALL = "Nx=64 Ny=24 Nz=12"
+C3 = "fargo3d_amd/setups/c3_adiabatic2d.par"
+C3_SMALL = "Nx=256 Ny=96"
+# the 3-D disk with the adiabatic equation of state (energy transported)
+C2_AD = C2_SMALL + " Eos=adiabatic EnergyBoundary=symmetric Gamma=1.4"
 RTOL = 1e-12
 
 STATE3D = ["Density", "Energy", "Vx", "Vy", "Vx_temp", "Vy_temp", "Mpx", "Mmx", "Mpy", "Mmy",
@@ -40,6 +44,14 @@ def relerr(a, b):
 
 def nbits(a, b):
     return int(np.count_nonzero(a.view(np.int64) != b.view(np.int64)))
+
+
+def fields_of(s):
+    """The primitive fields that carry the state (the energy only when it evolves)."""
+    f = ["Density", "Vx", "Vy"] + (["Vz"] if s.dimz else [])
+    if s.adiabatic:
+        f.append("Energy")
+    return f
 
 
 def active(x, dimz):
@@ -92,7 +104,8 @@ def compare_all(o, s, tag, exact=False):
     return worst
 
 
-@pytest.mark.parametrize("par,ov", [(C1, ""), (C2, C2_SMALL)], ids=["c1_2d", "c2_3d_small"])
+@pytest.mark.parametrize("par,ov", [(C1, ""), (C2, C2_SMALL), (C3, C3_SMALL)],
+                         ids=["c1_2d", "c2_3d_small", "c3_2d_adiabatic_small"])
 def test_initial_state(fa, orc, par, ov):
     """Initial conditions + first ghost fill + Stockholm references agree everywhere."""
     say("initial state", par, ov)
@@ -201,6 +214,26 @@ def test_routines_3d(fa, orc):
                    nwarm=2)
 
 
+ROUTINES_ADIABATIC = [
+    ("ComputePressureFieldAd", ()),
+    ("SubStep1_x", ("dt",)),
+    ("SubStep1_y", ("dt",)),
+    ("SubStep2_a", ("dt",)),
+    ("SubStep2_b", ("dt",)),
+    ("SubStep3", ("dt",)),
+    ("boundaries", ()),
+    ("DivideByRho", ("Energy",)),
+    ("UpdateY", ("dt", "Energy", "Qs")),
+    ("AdvectSHIFT", ("Energy", "Nshift")),
+    ("StockholmBoundary", ("dt",)),
+]
+
+
+def test_routines_adiabatic(fa, orc):
+    """The energy-equation routines (pressure, heating, compression work, energy ghosts)."""
+    _routine_sweep(fa, orc, C3, C3_SMALL, ROUTINES_ADIABATIC, nwarm=2)
+
+
 def test_reductions_and_cfl(fa, orc):
     o, s = make_pair(fa, orc, C1)
     for _ in range(2):
@@ -227,7 +260,8 @@ def test_reductions_and_cfl(fa, orc):
     assert got.value == ref
 
 
-@pytest.mark.parametrize("par,ov,nsteps", [(C1, "", 20), (C2, C2_SMALL, 6)], ids=["c1_2d", "c2_3d_small"])
+@pytest.mark.parametrize("par,ov,nsteps", [(C1, "", 20), (C2, C2_SMALL, 6), (C3, C3_SMALL, 10)],
+                         ids=["c1_2d", "c2_3d_small", "c3_2d_adiabatic_small"])
 def test_whole_step_routines(fa, orc, par, ov, nsteps):
     """The C driver over the per-routine kernels reproduces the CPU run step by step."""
     o, s = make_pair(fa, orc, par, ov, arch={"FusedCfl": "no", "FusedSources": "no", "FusedTransport": "no"})
@@ -236,7 +270,7 @@ def test_whole_step_routines(fa, orc, par, ov, nsteps):
         assert dto == dts, "step %d: dt %r vs %r" % (n, dts, dto)
         if n in (1, 2, 5, nsteps):
             worst = 0.0
-            for f in ["Density", "Vx", "Vy"] + (["Vz"] if s.dimz else []):
+            for f in fields_of(s):
                 a, b = active(s.field(f), s.dimz), active(o.field(f), s.dimz)
                 e = relerr(a, b)
                 say("step %d %s rel %.3e bits %d" % (n, f, e, nbits(a, b)))
@@ -247,8 +281,9 @@ def test_whole_step_routines(fa, orc, par, ov, nsteps):
     assert np.array_equal(po, ps)
 
 
-@pytest.mark.parametrize("par,ov,nsteps", [(C1, "", 30), (C2, C2_SMALL, 8), (C2, "", 3)],
-                         ids=["c1_2d", "c2_3d_small", "c2_3d_full"])
+@pytest.mark.parametrize("par,ov,nsteps", [(C1, "", 30), (C2, C2_SMALL, 8), (C2, "", 3), (C3, C3_SMALL, 20),
+                                           (C2, C2_AD, 8)],
+                         ids=["c1_2d", "c2_3d_small", "c2_3d_full", "c3_2d_adiabatic_small", "c2_3d_adiabatic_small"])
 def test_whole_step_fused(fa, orc, par, ov, nsteps):
     """The fused kernels reproduce the CPU run; the active cells carry the state."""
     o, s = make_pair(fa, orc, par, ov)
@@ -256,12 +291,12 @@ def test_whole_step_fused(fa, orc, par, ov, nsteps):
         dto, dts = o.step(), s.step()
         assert dto == dts, "step %d: dt %r vs %r" % (n, dts, dto)
         if n in (1, 2, nsteps):
-            for f in ["Density", "Vx", "Vy"] + (["Vz"] if s.dimz else []):
+            for f in fields_of(s):
                 a, b = active(s.field(f), s.dimz), active(o.field(f), s.dimz)
                 e = relerr(a, b)
                 say("fused step %d %s rel %.3e bits %d" % (n, f, e, nbits(a, b)))
                 assert e <= RTOL, (n, f, e)
             # ghost cells are refilled from active ones at the end of a step
-            for f in ["Density", "Vx", "Vy"]:
+            for f in fields_of(s):
                 e = relerr(s.field(f), o.field(f))
                 assert e <= RTOL, ("ghosts", n, f, e)
