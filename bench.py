@@ -24,29 +24,43 @@ sys.path.insert(0, ROOT)
 os.chdir(ROOT)
 
 HBM_PEAK_GBS = 8000.0
-PAR = {"c1": "fargo3d_amd/setups/c1_fargo2d.par", "c2": "fargo3d_amd/setups/c2_p3d.par"}
+PAR = {"c1": "fargo3d_amd/setups/c1_fargo2d.par", "c2": "fargo3d_amd/setups/c2_p3d.par",
+       "c3": "fargo3d_amd/setups/c3_adiabatic2d.par"}
 WORKLOAD = {
     "c1": "2-D polar isothermal disk 384x128, FP64, FARGO, nu=1e-5, 1e-3 Mstar planet",
     "c2": "3-D spherical isothermal disk 256x96x32, FP64, FARGO, alpha=1e-3, 3e-4 Mstar planet",
+    "c3": "2-D polar adiabatic disk 4096x1024, FP64, FARGO, gamma=1.4, nu=1e-5, 1e-3 Mstar planet",
 }
-# Algorithmic bytes per active cell and launch of each fused stage, as
-# (2-D, 3-D): every 3-D array the stage must read once and write once, at
-# 8 bytes (FP64); 1-D/2-D tables and ghost cells are not counted.
-STAGE_BYTES = {
-    "cfl": (3 * 8, 4 * 8),                        # cs, vx, vy (, vz)
-    "sources_a": ((3 + 5) * 8, (5 + 9) * 8),      # rho, v (, cs) -> stresses + q
-    "sources_b": ((10 + 2) * 8, (15 + 3) * 8),    # rho, cs, pot, v, stresses, q -> v_temp
-    "sweep_z": (0, (4 + 7) * 8),                  # rho, v_temp -> rho + 6 momenta
-    "sweep_y": ((3 + 5) * 8, (8 + 7) * 8),        # (Z-swept) rho + momenta, vy_temp -> rho + momenta
-    "sweep_x": ((6 + 4) * 8, (8 + 6) * 8),        # rho, momenta, vx_temp -> rho, vx, momenta
-    "finish": ((3 + 2) * 8, (5 + 3) * 8),         # rho, y/z momenta -> rho, vy (, vz)
-    "potential": (1 * 8, 1 * 8),                  # -> pot
-}
+# Algorithmic bytes per active cell and launch of each fused stage: every
+# 3-D array the stage must read once and write once, at 8 bytes (FP64);
+# 1-D/2-D tables and ghost cells are not counted.  dz: 3-D, ad: adiabatic
+# (the energy is one more transported array), nu: constant viscosity (alpha
+# viscosity also reads the sound speed in the stress stage).
+def stage_arrays(stage, dz, ad, alpha):
+    nv = 3 if dz else 2                  # velocity components
+    nm = 2 * nv                          # staggered momenta
+    ns = 6 if dz else 3                  # stress components
+    table = {
+        "cfl": 1 + nv + ad,                                  # cs or e, v (, rho)
+        "sources_a": (1 + nv + alpha) + (ns + nv),           # rho, v (, cs) -> stresses + q
+        "sources_b": (3 + nv + ns + nv) + (nv + ad),         # rho, cs/e, pot, v, stresses, q -> v_temp (, e)
+        "substep3": (1 + nv) + 1,                            # e, v_temp -> e
+        "sweep_z": (1 + nv + ad) + (1 + nm + ad),            # rho, v_temp (, e) -> rho, momenta (, e)
+        "sweep_y": ((1 + nm + ad + 1) if dz else (1 + nv + ad)) + (1 + nm + ad),
+        "sweep_x": (1 + nm + 1 + ad) + (2 + nm - 2 + ad),    # rho, momenta, vx_temp (, e) -> rho, vx, y/z momenta (, e)
+        "finish": (1 + nm - 2) + nv,                         # rho, y/z momenta -> rho, vy (, vz)
+        "potential": 1,                                      # -> pot
+    }
+    return table.get(stage, 0)
+
+
 # kernel behind each stage (names in the rocprofv3 summaries)
 STAGE_KERNEL = {"cfl": "kf_cfl", "sources_a": "kf_src_a", "sources_b": "kf_src_b", "sweep_z": "kf_sweep_z",
-                "sweep_y": "kf_sweep_y", "sweep_x": "kf_sweep_x", "finish": "kf_finish", "potential": "k_potential"}
+                "sweep_y": "kf_sweep_y", "sweep_x": "kf_sweep_x", "finish": "kf_finish", "potential": "k_potential",
+                "substep3": "kf_substep3"}
 # rocprofv3 summary with the measured HBM traffic of each kernel of this workload
-PROFILE = {"c2": "profiles/fused_step_c2_3d_256x96x32.md", "c1": "profiles/fused_step_c1_2d_384x128.md"}
+PROFILE = {"c2": "profiles/fused_step_c2_3d_256x96x32.md", "c1": "profiles/fused_step_c1_2d_384x128.md",
+           "c3": "profiles/fused_step_c3_2d_adiabatic_4096x1024.md"}
 
 
 def measured_traffic(config, stage):
@@ -129,6 +143,8 @@ def main():
         arch = {"FusedCfl": "no", "FusedSources": "no", "FusedTransport": "no"}
     sim = fa.Simulation(parfile, arch=arch)
     cells = sim.nx * sim.ny * (sim.nz if sim.dimz else 1)
+    ad = 1 if sim.adiabatic else 0
+    alpha = 1 if args.config == "c2" else 0
     for _ in range(args.warmup):
         sim.step()
     sim.sync()
@@ -148,7 +164,10 @@ def main():
 
     value = cells * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
-    b_alg = 72 if sim.dimz else 56  # bytes per cell-update of the state (FP64)
+    # bytes per cell-update of the state (FP64): rho, v (, e) read and written,
+    # the sound speed read (isothermal)
+    nv = 3 if sim.dimz else 2
+    b_alg = 8 * (2 * (1 + nv + ad) + (0 if ad else 1))
     stages = {}
     for s in range(nst):
         if cnt[s]:
@@ -156,7 +175,7 @@ def main():
     roofline = None
     if stages:
         dom = max(stages, key=lambda n: stages[n]["avg_us"] * stages[n]["launches"])
-        per_cell = STAGE_BYTES.get(dom, (0, 0))[1 if sim.dimz else 0]
+        per_cell = 8 * stage_arrays(dom, sim.dimz, ad, alpha)
         avg_s = stages[dom]["avg_us"] * 1e-6
         achieved = per_cell * cells / avg_s / 1e9 if avg_s > 0 else 0.0
         roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

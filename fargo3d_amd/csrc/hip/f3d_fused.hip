@@ -40,9 +40,9 @@ constexpr int BXF = 64;
 
 // Scratch of the fused stages, n3 doubles each, allocated on a handle's
 // first fused call and released with the handle: Y- and X-swept density,
-// artificial pressure (3), energy after SubStep2_b and after the Y sweep, and
-// (3-D) the Z-swept density, six momenta and energy.
-enum { FS_RHOY = 0, FS_RHOX, FS_QX, FS_QY, FS_QZ, FS_E2, FS_EY, FS_Z0, FS_COUNT = FS_Z0 + 8 };
+// artificial pressure (3), energy after SubStep2_b and after the Y sweep, the
+// X-swept Y/Z momenta (4) and (3-D) the Z-swept density, six momenta and energy.
+enum { FS_RHOY = 0, FS_RHOX, FS_QX, FS_QY, FS_QZ, FS_E2, FS_EY, FS_XM0, FS_Z0 = FS_XM0 + 4, FS_COUNT = FS_Z0 + 8 };
 
 struct Scratch {
   double* f[FS_COUNT];
@@ -525,7 +525,8 @@ template <int G, int DZ, int AD, int CPT>
 __global__ void __launch_bounds__(256) kf_sweep_x(dmesh m, double dt, const double* rho_in, const double* vxt,
                                                    double* mpx, double* mmx, double* mpy, double* mmy, double* mpz,
                                                    double* mmz, const double* e_in, double* e_out, double* rho_out,
-                                                   double* vx_out, double* vxmed_out, int* nshift_out,
+                                                   double* vx_out, double* mpy_o, double* mmy_o, double* mpz_o,
+                                                   double* mmz_o, double* vxmed_out, int* nshift_out,
                                                    double omegaframe, int fargo) {
   constexpr int NM = DZ ? 6 : 4;  // momenta
   constexpr int NQ = NM + AD;
@@ -756,9 +757,181 @@ __global__ void __launch_bounds__(256) kf_sweep_x(dmesh m, double dt, const doub
       rho_out[base + i] = R[i];
 #pragma unroll
       for (int c = 2; c < NQ; c++) {
-        if (c < NM) qg[c][base + i] = OUT[c * nx + i];
+        if (c < NM) (c == 2 ? mpy_o : c == 3 ? mmy_o : c == 4 ? mpz_o : mmz_o)[base + i] = OUT[c * nx + i];
         else e_out[base + i] = OUT[c * nx + i];
       }
+    }
+  }
+}
+
+// ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30) for the tiled X sweep: one
+// workgroup per active ring, the ring's weights in LDS, one sequential sum.
+template <int G, int DZ>
+__global__ void __launch_bounds__(WG) kf_vmed(dmesh m, const double* vxt, double* vxmed) {
+  extern __shared__ double w[];
+  const int j = m.nghy + (int)(blockIdx.x % m.ny);
+  const int k = DZ ? m.nghz + (int)(blockIdx.x / m.ny) : 0;
+  const int base = j * m.pitch + k * m.stride;
+  for (int i = threadIdx.x; i < m.nx; i += blockDim.x) w[i] = vxt[base + i] * (m.xmin[i + 1] - m.xmin[i]);
+  __syncthreads();
+  if (threadIdx.x == 0) vxmed[j + k * m.sy] = seq_sum(w, m.nx) / (m.XMAX - m.XMIN);
+}
+
+// X sweep of rings too long for LDS: workgroups own tiles of T cells and
+// load H = 6 halo cells on each side (the van Leer pass needs 2, the PPA pass
+// 3 more of the van Leer result, NewVelocity_x the left neighbour), so no
+// tile waits for another.  VxMed comes
+// from kf_vmed.  The outputs go to their shifted positions directly
+// (AdvectSHIFT, src/advect_shift.c:45-58) with NewVelocity_x computed at the
+// destination cell from the source cell and its left neighbour.
+template <int G, int DZ, int AD, int T>
+__global__ void __launch_bounds__(T) kf_sweep_x_tile(dmesh m, double dt, const double* rho_in, const double* vxt,
+                                                      const double* mpx, const double* mmx, const double* mpy,
+                                                      const double* mmy, const double* mpz, const double* mmz,
+                                                      const double* e_in, double* e_out, double* rho_out,
+                                                      double* vx_out, double* mpy_o, double* mmy_o, double* mpz_o,
+                                                      double* mmz_o, const double* vxmed, int* nshift_out,
+                                                      double omegaframe, int fargo) {
+  constexpr int H = 6, W = T + 2 * H;
+  constexpr int NM = DZ ? 6 : 4;
+  constexpr int NQ = NM + AD;
+  constexpr int N = NQ + 1;
+  __shared__ double ST[N * W], AQ[N * W], SL[N * W], FC[N * W], QR2[N * W];
+  __shared__ double V[W], VU[W], ZS[W];
+  const int nx = m.nx;
+  const int ring = blockIdx.y;
+  const int j = m.nghy + ring % m.ny;
+  const int k = DZ ? m.nghz + ring / m.ny : 0;
+  const int base = j * m.pitch + k * m.stride;
+  const int i0 = blockIdx.x * T;
+  const int t = threadIdx.x;
+  const double vmed = vxmed[j + k * m.sy];
+  const double* qin[7] = {mpx, mmx, mpy, mmy, DZ ? mpz : e_in, DZ ? mmz : e_in, e_in};
+  auto gidx = [&](int p) {
+    int g = (i0 - H + p) % nx;
+    return g < 0 ? g + nx : g;
+  };
+  // the integer shift of the ring: ComputeResidual at i = 0 (src/compute_residual.c:68-71)
+  const int ns = fargo ? (int)floor(vmed * dt / zone_size_x<G>(m, 0, j, k) + 0.5) : 0;
+  if (fargo && blockIdx.x == 0 && t == 0) nshift_out[j + k * m.sy] = ns;
+  for (int p = t; p < W; p += T) {
+    const int g = gidx(p);
+    ST[NQ * W + p] = rho_in[base + g];
+#pragma unroll
+    for (int c = 0; c < NQ; c++) ST[c * W + p] = (c < NM) ? qin[c][base + g] : e_in[base + g];
+    const double zs = zone_size_x<G>(m, g, j, k);
+    const double v = vxt[base + g];
+    ZS[p] = zs;
+    if (fargo) {
+      const double ntilde = vmed * dt / zs;
+      const double nround = floor(ntilde + 0.5);
+      V[p] = v - vmed;
+      VU[p] = (ntilde - nround) * zs / dt;
+    } else {
+      V[p] = v;
+    }
+  }
+  __syncthreads();
+  for (int pass = 0; pass < (fargo ? 2 : 1); pass++) {
+    const double* VV = pass ? VU : V;
+    // valid range of the state entering this pass: [lo, hi)
+    const int lo = pass ? 2 : 0, hi = pass ? W - 2 : W;
+    for (int p = lo + t; p < hi; p += T) {
+      const double r = ST[NQ * W + p];
+#pragma unroll
+      for (int c = 0; c < NQ; c++) AQ[c * W + p] = ST[c * W + p] / r;
+      AQ[NQ * W + p] = r;
+    }
+    __syncthreads();
+    int ulo, uhi;  // cells updated by this pass
+    if (!pass) {
+      for (int p = 1 + t; p < W - 1; p += T) {
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+          const double* a = AQ + c * W;
+          SL[c * W + p] = vanleer_slope((a[p] - a[p - 1]) / ZS[p], (a[p + 1] - a[p]) / ZS[p + 1]);
+        }
+      }
+      __syncthreads();
+      for (int p = 2 + t; p < W - 1; p += T) {
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+          const double* a = AQ + c * W;
+          const double* sl = SL + c * W;
+          FC[c * W + p] = vanleer_face(VV[p], dt, a[p - 1], sl[p - 1], ZS[p - 1], a[p], sl[p], ZS[p]);
+        }
+      }
+      __syncthreads();
+      ulo = 2;
+      uhi = W - 2;
+    } else {
+      for (int p = 3 + t; p < W - 3; p += T) {
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+          const double* a = AQ + c * W;
+          SL[c * W + p] = cell::ppa_slope(a[p - 1], a[p], a[p + 1]);
+        }
+      }
+      __syncthreads();
+      for (int p = 3 + t; p < W - 4; p += T) {
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+          const double* a = AQ + c * W;
+          const double* sl = SL + c * W;
+          FC[c * W + p] = cell::ppa_face(a[p], a[p + 1], sl[p], sl[p + 1]);
+        }
+      }
+      __syncthreads();
+      for (int p = 4 + t; p < W - 4; p += T) {
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+          double qL = FC[c * W + p - 1], qR = FC[c * W + p];
+          cell::ppa_limit(AQ[c * W + p], qL, qR);
+          SL[c * W + p] = qL;
+          QR2[c * W + p] = qR;
+        }
+      }
+      __syncthreads();
+      for (int p = 5 + t; p < W - 4; p += T) {
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+          const double* a = AQ + c * W;
+          const double* qL = SL + c * W;
+          const double* qR = QR2 + c * W;
+          FC[c * W + p] = cell::ppa_flux_value(VV[p], dt, ZS[p], a[p - 1], qL[p - 1], qR[p - 1], a[p], qL[p], qR[p]);
+        }
+      }
+      __syncthreads();
+      ulo = 5;
+      uhi = W - 5;
+    }
+    // UpdateX / UpdateDensityX (src/update_x.c:69-71, src/update_density_x.c:66-68)
+    for (int p = ulo + t; p < uhi; p += T) {
+      const int g = gidx(p);
+      const double v0 = VV[p], v1 = VV[p + 1];
+      const double* ds = FC + NQ * W;
+#pragma unroll
+      for (int c = 0; c < NQ; c++) {
+        const double* f = FC + c * W;
+        ST[c * W + p] += cell::flux_x(m, g, j, k, dt, v0 * f[p] * ds[p], v1 * f[p + 1] * ds[p + 1]);
+      }
+      ST[NQ * W + p] += cell::flux_x(m, g, j, k, dt, v0 * ds[p], v1 * ds[p + 1]);
+    }
+    __syncthreads();
+  }
+  // outputs of the tile's cells at their shifted positions
+  const int p = H + t;
+  const int i = i0 + t;
+  if (i < nx) {
+    int d = (i + ns) % nx;
+    if (d < 0) d += nx;
+    const double* R = ST + NQ * W;
+    vx_out[base + d] = cell::newvel_x<G>(m, d, j, k, R[p], R[p - 1], ST[1 * W + p], ST[0 * W + p - 1], omegaframe);
+    rho_out[base + d] = R[p];
+#pragma unroll
+    for (int c = 2; c < NQ; c++) {
+      if (c < NM) (c == 2 ? mpy_o : c == 3 ? mmy_o : c == 4 ? mpz_o : mmz_o)[base + d] = ST[c * W + p];
+      else e_out[base + d] = ST[c * W + p];
     }
   }
 }
@@ -848,7 +1021,73 @@ extern "C" int f3d_boundaries_on(f3d_hip_ctx* ctx, double* dens, double* vx, dou
 // tuning knobs of the column sweeps (rows per thread); set by the tuner
 static int g_rows_y = 4, g_rows_z = 3;
 
+// NewVelocity_y/z, Stockholm damping and the end-of-step ghost fill
+static int finish_step(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step) {
+  const dmesh& m = ctx->m;
+  const f3d_fields& f = ctx->f;
+  const double of = ctx->ph.OMEGAFRAME;
+  double* const* XM = &S->f[FS_XM0];
+  cell::DampPar dp;
+  dp.Y_inf = ctx->ph.Y_inf; dp.Y_sup = ctx->ph.Y_sup; dp.Z_inf = ctx->ph.Z_inf; dp.Z_sup = ctx->ph.Z_sup;
+  dp.y_min = ctx->ph.ymin_par; dp.y_max = ctx->ph.ymax_par; dp.z_min = ctx->ph.zmin_par; dp.z_max = ctx->ph.zmax_par;
+  dp.ds = ctx->ph.taudamp; dp.of = of; dp.of0 = ctx->ph.OMEGAFRAME0; dp.periodic_z = ctx->ph.periodicz;
+  dim3 gf((m.nx + BXF - 1) / BXF, m.ny, m.dimz ? m.nz : 1);
+  f3d_stage_begin(ctx, F3D_ST_FINISH);
+  FDISPATCH(ctx, kf_finish, gf, dim3(BXF), 0, dt, S->f[FS_RHOX], XM[1], XM[0], XM[3], XM[2], f.Density, f.Vx, f.Vy,
+            f.Vz, f.Density0, f.Vx0, f.Vy0, f.Vz0, dp, ctx->ph.stockholm);
+  f3d_stage_end(ctx, F3D_ST_FINISH);
+  if (end_of_step) {
+    // end-of-step ghost fill (src/main.c:431)
+    f3d_stage_begin(ctx, F3D_ST_BC_END);
+    if (f3d_boundaries_on(ctx, f.Density, f.Vx, f.Vy, f.Vz)) return -1;
+    f3d_stage_end(ctx, F3D_ST_BC_END);
+  }
+  return 0;
+}
+
+// X sweep by tiles (rings longer than the LDS ring kernel holds): VxMed of
+// each ring, then the tiles.
+static int x_sweep_tiled(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step) {
+  const dmesh& m = ctx->m;
+  const f3d_fields& f = ctx->f;
+  const double of = ctx->ph.OMEGAFRAME;
+  double* const* XM = &S->f[FS_XM0];
+  const int nrings = m.ny * (m.dimz ? m.nz : 1);
+  const int ad = (ctx->ph.eos == 2);
+  constexpr int T = 128;
+  F3D_REQUIRE(ctx, m.nx >= 12, "the tiled X sweep needs at least 12 cells per ring");
+  F3D_REQUIRE(ctx, (size_t)m.nx * sizeof(double) <= 160 * 1024, "ring too long for the VxMed pass");
+  f3d_stage_begin(ctx, F3D_ST_SWEEP_X);
+  if (ctx->ph.fargo)
+    FDISPATCH(ctx, kf_vmed, dim3(nrings), dim3(WG), m.nx * sizeof(double), f.Vx_temp, f.VxMed);
+  dim3 g((m.nx + T - 1) / T, nrings);
+#define XTILE(G_, DZ_, AD_)                                                                                     \
+  hipLaunchKernelGGL((kf_sweep_x_tile<G_, DZ_, AD_, T>), g, dim3(T), 0, ctx->stream, m, dt, S->f[FS_RHOY],      \
+                     f.Vx_temp, f.Mpx, f.Mmx, f.Mpy, f.Mmy, f.Mpz, f.Mmz, S->f[FS_EY], f.Energy, S->f[FS_RHOX], \
+                     f.Vx, XM[0], XM[1], XM[2], XM[3], f.VxMed, f.Nshift, of, ctx->ph.fargo)
+  if (m.geom == SPH) {
+    if (m.dimz) { if (ad) XTILE(SPH, 1, 1); else XTILE(SPH, 1, 0); }
+    else { if (ad) XTILE(SPH, 0, 1); else XTILE(SPH, 0, 0); }
+  } else {
+    if (m.dimz) { if (ad) XTILE(CYL, 1, 1); else XTILE(CYL, 1, 0); }
+    else { if (ad) XTILE(CYL, 0, 1); else XTILE(CYL, 0, 0); }
+  }
+#undef XTILE
+  F3D_LAUNCH_CHECK(ctx);
+  f3d_stage_end(ctx, F3D_ST_SWEEP_X);
+  return finish_step(ctx, S, dt, end_of_step);
+}
+
+// X-sweep kernel choice: 0 automatic, 1 ring in LDS, 2 tiles
+static int g_xmode = 0;
+
 extern "C" {
+
+int f3d_fused_set_xmode(int mode) {
+  if (mode < 0 || mode > 2) return -2;
+  g_xmode = mode;
+  return 0;
+}
 
 int f3d_fused_set_tiling(int rows_y, int rows_z) {
   if (rows_y < 1 || rows_z < 1) return -2;
@@ -926,6 +1165,7 @@ int f3d_fused_transport(f3d_hip_ctx* ctx, double dt, int end_of_step) {
   const f3d_fields& f = ctx->f;
   const double of = ctx->ph.OMEGAFRAME;
   const int ry = g_rows_y, rz = g_rows_z;
+  double* const* XM = &S->f[FS_XM0];
   if (m.dimz) {
     // seven separate buffers: six Z-swept momenta and the Z-swept density
     double* const* Z = &S->f[FS_Z0];
@@ -946,12 +1186,14 @@ int f3d_fused_transport(f3d_hip_ctx* ctx, double dt, int end_of_step) {
                  f.Mpx, f.Mmx, f.Mpy, f.Mmy, S->f[FS_EY], ry, of);
     f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
   }
-  // X sweep: one workgroup of at most 256 threads per active ring, up to 4
-  // cells per thread
+  // X sweep: one workgroup of at most 256 threads per active ring (up to 4
+  // cells per thread) when the ring fits in LDS, else tiles with halos
   const int ad = (ctx->ph.eos == 2);
   const int nq1 = (m.dimz ? 6 : 4) + ad + 1;
   const size_t shx = ((size_t)5 * nq1 * m.nx + m.nx + 8) * sizeof(double);
-  F3D_REQUIRE(ctx, shx <= 160 * 1024 && m.nx <= 1024, "ring too long for the LDS X sweep");
+  const bool ring_fits = shx <= 160 * 1024 && m.nx <= 1024;
+  if (g_xmode == 2 || (g_xmode == 0 && !ring_fits)) return x_sweep_tiled(ctx, S, dt, end_of_step);
+  F3D_REQUIRE(ctx, ring_fits, "ring too long for the LDS X sweep");
   const int cpt = (m.nx <= 256) ? 1 : (m.nx <= 512) ? 2 : 4;
   const int nthreads = (((m.nx + cpt - 1) / cpt + 63) / 64) * 64;
   dim3 gx(m.ny * (m.dimz ? m.nz : 1));
@@ -959,7 +1201,7 @@ int f3d_fused_transport(f3d_hip_ctx* ctx, double dt, int end_of_step) {
 #define XSWEEP1(G_, DZ_, AD_, CPT_)                                                                           \
   hipLaunchKernelGGL((kf_sweep_x<G_, DZ_, AD_, CPT_>), gx, dim3(nthreads), shx, ctx->stream, m, dt,              \
                      S->f[FS_RHOY], f.Vx_temp, f.Mpx, f.Mmx, f.Mpy, f.Mmy, f.Mpz, f.Mmz, S->f[FS_EY], f.Energy,   \
-                     S->f[FS_RHOX], f.Vx, f.VxMed, f.Nshift, of, ctx->ph.fargo)
+                     S->f[FS_RHOX], f.Vx, XM[0], XM[1], XM[2], XM[3], f.VxMed, f.Nshift, of, ctx->ph.fargo)
 #define XSWEEP(G_, DZ_, CPT_) \
   do { if (ad) XSWEEP1(G_, DZ_, 1, CPT_); else XSWEEP1(G_, DZ_, 0, CPT_); } while (0)
 #define XSWEEP_G(G_, DZ_) \
@@ -974,22 +1216,7 @@ int f3d_fused_transport(f3d_hip_ctx* ctx, double dt, int end_of_step) {
 #undef XSWEEP
   F3D_LAUNCH_CHECK(ctx);
   f3d_stage_end(ctx, F3D_ST_SWEEP_X);
-  cell::DampPar dp;
-  dp.Y_inf = ctx->ph.Y_inf; dp.Y_sup = ctx->ph.Y_sup; dp.Z_inf = ctx->ph.Z_inf; dp.Z_sup = ctx->ph.Z_sup;
-  dp.y_min = ctx->ph.ymin_par; dp.y_max = ctx->ph.ymax_par; dp.z_min = ctx->ph.zmin_par; dp.z_max = ctx->ph.zmax_par;
-  dp.ds = ctx->ph.taudamp; dp.of = of; dp.of0 = ctx->ph.OMEGAFRAME0; dp.periodic_z = ctx->ph.periodicz;
-  dim3 gf((m.nx + BXF - 1) / BXF, m.ny, m.dimz ? m.nz : 1);
-  f3d_stage_begin(ctx, F3D_ST_FINISH);
-  FDISPATCH(ctx, kf_finish, gf, dim3(BXF), 0, dt, S->f[FS_RHOX], f.Mmy, f.Mpy, f.Mmz, f.Mpz, f.Density, f.Vx, f.Vy,
-            f.Vz, f.Density0, f.Vx0, f.Vy0, f.Vz0, dp, ctx->ph.stockholm);
-  f3d_stage_end(ctx, F3D_ST_FINISH);
-  if (end_of_step) {
-    // end-of-step ghost fill (src/main.c:431)
-    f3d_stage_begin(ctx, F3D_ST_BC_END);
-    if (f3d_boundaries_on(ctx, f.Density, f.Vx, f.Vy, f.Vz)) return -1;
-    f3d_stage_end(ctx, F3D_ST_BC_END);
-  }
-  return 0;
+  return finish_step(ctx, S, dt, end_of_step);
 }
 
 }  // extern "C"

@@ -202,6 +202,10 @@ int f3d_fused_transport(f3d_hip_ctx *ctx, double dt, int end_of_step);
 /* Rows (Y sweep) and planes (Z sweep) one thread marches through; a tuning
  * knob that does not change results.  Returns -2 for values below 1. */
 int f3d_fused_set_tiling(int rows_y, int rows_z);
+/* X-sweep kernel: 0 automatic (the ring in LDS when it fits, else tiles of
+ * 128 cells with halos), 1 ring, 2 tiles; results do not depend on it.
+ * Returns -2 for other values. */
+int f3d_fused_set_xmode(int mode);
 
 #ifdef __cplusplus
 }

@@ -54,6 +54,11 @@ def fields_of(s):
     return f
 
 
+def active_rings(x2d, dimz):
+    """Active (j, k) rings of a YZ array of shape (sz, sy)."""
+    return x2d[3:-3, 3:-3] if dimz else x2d[:, 3:-3]
+
+
 def active(x, dimz):
     if dimz:
         return x[3:-3, 3:-3, :]
@@ -281,11 +286,7 @@ def test_whole_step_routines(fa, orc, par, ov, nsteps):
     assert np.array_equal(po, ps)
 
 
-@pytest.mark.parametrize("par,ov,nsteps", [(C1, "", 30), (C2, C2_SMALL, 8), (C2, "", 3), (C3, C3_SMALL, 20),
-                                           (C2, C2_AD, 8)],
-                         ids=["c1_2d", "c2_3d_small", "c2_3d_full", "c3_2d_adiabatic_small", "c2_3d_adiabatic_small"])
-def test_whole_step_fused(fa, orc, par, ov, nsteps):
-    """The fused kernels reproduce the CPU run; the active cells carry the state."""
+def _fused_run(fa, orc, par, ov, nsteps):
     o, s = make_pair(fa, orc, par, ov)
     for n in range(1, nsteps + 1):
         dto, dts = o.step(), s.step()
@@ -300,3 +301,31 @@ def test_whole_step_fused(fa, orc, par, ov, nsteps):
             for f in fields_of(s):
                 e = relerr(s.field(f), o.field(f))
                 assert e <= RTOL, ("ghosts", n, f, e)
+            # the shift counts of the active rings (ghost rings are never shifted
+            # into active cells) are integers: exactly equal
+            assert np.array_equal(active_rings(s.nshift(), s.dimz), active_rings(o.nshift(), s.dimz)), \
+                "step %d: Nshift differs" % n
+    return o, s
+
+
+@pytest.mark.parametrize("par,ov,nsteps", [(C1, "", 30), (C2, C2_SMALL, 8), (C2, "", 3), (C3, C3_SMALL, 20),
+                                           (C2, C2_AD, 8), (C3, "", 2)],
+                         ids=["c1_2d", "c2_3d_small", "c2_3d_full", "c3_2d_adiabatic_small", "c2_3d_adiabatic_small",
+                              "c3_2d_adiabatic_full"])
+def test_whole_step_fused(fa, orc, par, ov, nsteps):
+    """The fused kernels reproduce the CPU run; the active cells carry the state.
+    The full 4096-cell rings take the tiled X sweep."""
+    _fused_run(fa, orc, par, ov, nsteps)
+
+
+@pytest.mark.parametrize("par,ov,nsteps", [(C1, "", 12), (C2, C2_SMALL, 6), (C3, C3_SMALL, 10), (C2, C2_AD, 4)],
+                         ids=["c1_2d", "c2_3d_small", "c3_2d_adiabatic_small", "c2_3d_adiabatic_small"])
+def test_whole_step_fused_tiled_x(fa, orc, par, ov, nsteps):
+    """The tiled X sweep (tiles of 128 cells with halos, shifted writes) on
+    rings the LDS ring kernel also handles gives the same results."""
+    L = fa.lib()
+    assert L.f3d_fused_set_xmode(2) == 0
+    try:
+        _fused_run(fa, orc, par, ov, nsteps)
+    finally:
+        L.f3d_fused_set_xmode(0)
