@@ -112,6 +112,67 @@ def copy_calibration(L, sim, nbytes):
     L.f3d_hip_free(b)
 
 
+def timed_run(fa, L, names, parfile, overrides, arch, steps, warmup):
+    """Warm up, then time exactly `steps` steps (synchronised on both sides);
+    returns the wall time, the per-stage HIP-event averages and the simulation."""
+    nst = len(names)
+    sim = fa.Simulation(parfile, overrides, arch=arch)
+    for _ in range(warmup):
+        sim.step()
+    sim.sync()
+    L.f3d_hip_timing_enable(sim.ctx, 1)
+    sim.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    sim.sync()
+    elapsed = time.perf_counter() - t0
+    ms = (ctypes.c_double * nst)()
+    cnt = (ctypes.c_long * nst)()
+    L.f3d_hip_timing_read(sim.ctx, ms, cnt)
+    L.f3d_hip_timing_enable(sim.ctx, 0)
+    stages = {}
+    for st in range(nst):
+        if cnt[st]:
+            stages[names[st]] = {"avg_us": round(1e3 * ms[st] / cnt[st], 3), "launches": int(cnt[st])}
+    return elapsed, stages, sim
+
+
+def roofline_of(stages, cells, dimz, ad, alpha, esize, config):
+    """Roofline figures of the stage with the largest total time."""
+    if not stages:
+        return None
+    dom = max(stages, key=lambda n: stages[n]["avg_us"] * stages[n]["launches"])
+    per_cell = esize * stage_arrays(dom, dimz, ad, alpha)
+    avg_s = stages[dom]["avg_us"] * 1e-6
+    achieved = per_cell * cells / avg_s / 1e9 if avg_s > 0 else 0.0
+    return {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(config, dom) if esize == 8 else None,
+            "bytes_per_cell": per_cell, "bytes_per_launch": per_cell * cells}
+
+
+def fp32_errors(fa, parfile, steps):
+    """Relative L1 and L-infinity errors of every field of the FP32 form against
+    the FP64 path after `steps` steps on one time-step sequence."""
+    import numpy as np
+    s64 = fa.Simulation(parfile)
+    s32 = fa.Simulation(parfile, "Precision=32")
+    for _ in range(steps):
+        dt = s64.step()
+        s32.force_dt(dt)
+        s32.step()
+    fields = ["Density", "Vx", "Vy"] + (["Vz"] if s64.dimz else []) + (["Energy"] if s64.adiabatic else [])
+    out = {"steps": steps, "time": round(s64.scalars()["PhysicalTime"], 6)}
+    for f in fields:
+        a, b = s32.field(f), s64.field(f)
+        a = a[3:-3, 3:-3, :] if s64.dimz else a[:, 3:-3, :]
+        b = b[3:-3, 3:-3, :] if s64.dimz else b[:, 3:-3, :]
+        d = np.abs(a - b)
+        out[f] = {"L1": float("%.3e" % (d.sum() / np.abs(b).sum())), "Linf": float("%.3e" % (d.max() / np.abs(b).max()))}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +182,9 @@ def main():
     ap.add_argument("--routines", action="store_true", help="one kernel per routine instead of the fused path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-fp32", action="store_true", help="skip the reduced-precision form")
+    ap.add_argument("--error-steps", type=int, default=200,
+                    help="steps of the paired FP64/FP32 run that measures the FP32 form's error")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run three device-to-device copies of MIB MiB on the library stream (a copy "
                          "kernel of known size that scales the hardware byte counters)")
@@ -142,48 +206,40 @@ def main():
     arch = {}
     if args.routines:
         arch = {"FusedCfl": "no", "FusedSources": "no", "FusedTransport": "no"}
-    sim = fa.Simulation(parfile, arch=arch)
+    elapsed, stages, sim = timed_run(fa, L, names, parfile, "", arch, args.steps, args.warmup)
+    if args.copy_calibration > 0:
+        copy_calibration(L, sim, args.copy_calibration << 20)
     cells = sim.nx * sim.ny * (sim.nz if sim.dimz else 1)
     ad = 1 if sim.adiabatic else 0
     alpha = 1 if args.config == "c2" else 0
-    for _ in range(args.warmup):
-        sim.step()
-    sim.sync()
-    L.f3d_hip_timing_enable(sim.ctx, 1)
-    sim.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sim.step()
-    sim.sync()
-    elapsed = time.perf_counter() - t0
-    ms = (ctypes.c_double * nst)()
-    cnt = (ctypes.c_long * nst)()
-    L.f3d_hip_timing_read(sim.ctx, ms, cnt)
-    L.f3d_hip_timing_enable(sim.ctx, 0)
-    if args.copy_calibration > 0:
-        copy_calibration(L, sim, args.copy_calibration << 20)
+    dimz = sim.dimz
+    sim.close()
 
     value = cells * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
-    # bytes per cell-update of the state (FP64): rho, v (, e) read and written,
-    # the sound speed read (isothermal)
-    nv = 3 if sim.dimz else 2
-    b_alg = 8 * (2 * (1 + nv + ad) + (0 if ad else 1))
-    stages = {}
-    for s in range(nst):
-        if cnt[s]:
-            stages[names[s]] = {"avg_us": round(1e3 * ms[s] / cnt[s], 3), "launches": int(cnt[s])}
-    roofline = None
-    if stages:
-        dom = max(stages, key=lambda n: stages[n]["avg_us"] * stages[n]["launches"])
-        per_cell = 8 * stage_arrays(dom, sim.dimz, ad, alpha)
-        avg_s = stages[dom]["avg_us"] * 1e-6
-        achieved = per_cell * cells / avg_s / 1e9 if avg_s > 0 else 0.0
-        roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": measured_traffic(args.config, dom), "bytes_per_cell": per_cell,
-                    "bytes_per_launch": per_cell * cells}
+    # bytes per cell-update of the state: rho, v (, e) read and written, the
+    # sound speed read (isothermal)
+    nv = 3 if dimz else 2
+    n_arrays = 2 * (1 + nv + ad) + (0 if ad else 1)
+    b_alg = 8 * n_arrays
+    roofline = roofline_of(stages, cells, dimz, ad, alpha, 8, args.config)
     step_gbs = value * b_alg / 1e9
+
+    # the reduced-precision form on the same workload (FP32 fields, FP64 sums,
+    # residual/shift split and CFL) and its error against the FP64 path
+    forms = None
+    if not args.routines and not args.no_fp32:
+        e32, st32, s32 = timed_run(fa, L, names, parfile, "Precision=32", {}, args.steps, args.warmup)
+        s32.close()
+        v32 = cells * args.steps / e32
+        forms = {"fp32": {"value": round(v32, 1), "ms_per_step": round(1e3 * e32 / args.steps, 4),
+                          "speedup_vs_fp64": round(v32 / value, 3), "dtype": "f32 (f64 sums, residual, cfl)",
+                          "step_roofline": {"bytes_per_cell_update": 4 * n_arrays,
+                                            "achieved": round(v32 * 4 * n_arrays / 1e9, 1), "peak": HBM_PEAK_GBS,
+                                            "unit": "GB/s", "frac": round(v32 * 4 * n_arrays / 1e9 / HBM_PEAK_GBS, 4)},
+                          "roofline": roofline_of(st32, cells, dimz, ad, alpha, 4, args.config),
+                          "stages": st32,
+                          "error_vs_fp64": fp32_errors(fa, parfile, args.error_steps)}}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -211,6 +267,7 @@ def main():
         "step_roofline": {"bytes_per_cell_update": b_alg, "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "stages": stages,
+        "forms": forms,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -67,6 +67,7 @@ def lib():
         L.f3d_sim_get_planets.argtypes = [vp, vp, i]
         L.f3d_sim_shape.argtypes = [vp, vp]
         L.f3d_sim_eos.argtypes = [vp]
+        L.f3d_sim_force_dt.argtypes = [vp, d]
         L.f3d_sim_ctx.restype = vp
         L.f3d_sim_ctx.argtypes = [vp]
         L.f3d_sim_push_phys.argtypes = [vp]
@@ -131,6 +132,10 @@ class Simulation:
         dt = ctypes.c_double()
         self._check(self.L.f3d_sim_step(self._s, ctypes.byref(dt)))
         return dt.value
+
+    def force_dt(self, dt):
+        """Use dt as the next step's time step (paired-run measurements)."""
+        self._check(self.L.f3d_sim_force_dt(self._s, float(dt)))
 
     def run(self, nsteps):
         self._check(self.L.f3d_sim_run(self._s, int(nsteps)))

@@ -26,13 +26,16 @@ ARCH = os.environ.get("F3D_OFFLOAD_ARCH", "gfx950")
 
 HOST_SRCS = ["host/f3d_mesh.c", "host/f3d_param.c", "host/f3d_planets.c", "host/f3d_setup.c",
              "host/f3d_driver.c"]
-HIP_SRCS = ["hip/f3d_runtime.hip", "hip/f3d_routines.hip", "hip/f3d_fused.hip"]
+HIP_SRCS = ["hip/f3d_runtime.hip", "hip/f3d_routines.hip", "hip/f3d_fused.hip", "hip/f3d_fused32.hip"]
 
 CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-Wall",
           "-Wno-unused-function"]
 HIPFLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contract=off",
             "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-result", "-I" + os.path.join(ROOT, "include")]
+# The reduced-precision build is not bit-matched to anything: it may fuse
+# multiply-adds and divide through the hardware reciprocal (about 1 ulp).
+EXTRA = {"hip/f3d_fused32.hip": ["-ffp-contract=fast", "-freciprocal-math", "-fapprox-func"]}
 
 
 def _run(cmd):
@@ -70,7 +73,7 @@ def build(force=True, verbose=False):
         src = os.path.join(CSRC, s)
         obj = os.path.join(OBJDIR, os.path.basename(s) + ".o")
         if force or _stale(obj, [src] + headers):
-            _run([HIPCC] + HIPFLAGS + ["-c", src, "-o", obj])
+            _run([HIPCC] + HIPFLAGS + EXTRA.get(s, []) + ["-c", src, "-o", obj])
         objs.append(obj)
     lib = os.path.join(LIBDIR, "libfargo3d_amd.so")
     tmp = lib + ".tmp"

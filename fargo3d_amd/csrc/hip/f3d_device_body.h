@@ -1,0 +1,115 @@
+// f3d_device_body.h -- device-side mesh view and metric forms, written once
+// for the storage/arithmetic type `real` of a build (included by
+// f3d_device.h with real = double in namespace f3d, and by f3d_device32.h
+// with real = float in namespace f3d32; no include guard on purpose).
+//
+// Layout: every 3-D field is a dense x-fastest array of sx*sy*sz reals with
+// l = i + j*pitch + k*stride (src/define.h:414-420, no X ghosts); 2-D (YZ)
+// fields use l2 = j + k*sy.  Coordinates and metric factors live in one
+// device buffer described by f3d_dmesh (the device copy FARGO3D makes in
+// LightGlobalDev, src/light_global_dev.c:5-23), plus host-evaluated glibc
+// tables of cos/sin of the coordinates so that no kernel evaluates a
+// transcendental of a coordinate.
+//
+// Association: FARGO3D's CPU metric macros have no outer parentheses
+// (src/define.h:228-236), so "x*dt*InvVol(i,j,k)" rounds as
+// ((x*dt)*InvVj[j])/(Syk[k]*Sxi[i]).  The helpers below take the left
+// operand and reproduce that order exactly; kernels are compiled with
+// -ffp-contract=off so no multiply-add is fused.
+
+namespace F3D_NS {
+
+typedef F3D_REAL real;
+#ifdef F3D_MESH_REAL
+typedef F3D_MESH_REAL mreal;
+#else
+typedef F3D_REAL mreal;
+#endif
+
+enum { CYL = 1, SPH = 2 };
+
+struct dmesh {
+  int nx, ny, nz, nghy, nghz, dimz, geom;
+  int sx, sy, sz, pitch, stride;
+  double XMIN, XMAX, mindx;
+  const mreal *xmin, *ymin, *zmin, *xmed, *ymed, *zmed;
+  const mreal *sxi, *sxj, *syj, *szj, *sxk, *syk, *szk, *invvj, *invdiffxmed;
+  const mreal *cosxmed, *sinxmed, *sinzmed, *coszmed, *sinzmin, *coszmin;
+  // FP64 copies for the parts that stay FP64 in the reduced-precision form
+  // (ring sums, residual/shift split, CFL); the same arrays in the FP64 build
+  const double *xmin64, *ymin64, *zmin64, *ymed64, *sinzmed64, *sxi64;
+};
+
+__device__ __forceinline__ int lxp_of(int l, int i, int nx) { return (i < nx - 1) ? l + 1 : l - (nx - 1); }
+__device__ __forceinline__ int lxm_of(int l, int i, int nx) { return (i > 0) ? l - 1 : l + (nx - 1); }
+__device__ __forceinline__ int ixm_of(int i, int nx) { return i > 0 ? i - 1 : nx - 1; }
+__device__ __forceinline__ int ixp_of(int i, int nx) { return i < nx - 1 ? i + 1 : 0; }
+
+// x * InvVol(i,j,k)  ->  (x*InvVj[j]) / (Syk[k]*Sxi[i])
+__device__ __forceinline__ real mul_invvol(const dmesh& m, real x, int i, int j, int k) {
+  return (x * m.invvj[j]) / (m.syk[k] * m.sxi[i]);
+}
+// x * Vol(i,j,k)  ->  ((x*Syk[k])*Sxi[i]) / InvVj[j]
+__device__ __forceinline__ real mul_vol(const dmesh& m, real x, int i, int j, int k) {
+  return ((x * m.syk[k]) * m.sxi[i]) / m.invvj[j];
+}
+// Vol(i,j,k) standing alone  ->  (Syk[k]*Sxi[i]) / InvVj[j]
+__device__ __forceinline__ real vol(const dmesh& m, int i, int j, int k) {
+  return (m.syk[k] * m.sxi[i]) / m.invvj[j];
+}
+// x * SurfX(j,k)
+__device__ __forceinline__ real mul_surfx(const dmesh& m, real x, int j, int k) {
+  return (x * m.sxj[j]) * m.sxk[k];
+}
+// x * SurfY(i,j,k)
+__device__ __forceinline__ real mul_surfy(const dmesh& m, real x, int i, int j, int k) {
+  return ((x * m.syj[j]) * m.syk[k]) * m.sxi[i];
+}
+// x * SurfZ(i,j,k)
+__device__ __forceinline__ real mul_surfz(const dmesh& m, real x, int i, int j, int k) {
+  return ((x * m.szj[j]) * m.szk[k]) * m.sxi[i];
+}
+template <int G>
+__device__ __forceinline__ real zone_size_x(const dmesh& m, int i, int j, int k) {
+  if (G == SPH) return (m.sxi[i] * m.ymed[j]) * m.sinzmed[k];
+  return m.sxi[i] * m.ymed[j];
+}
+template <int G>
+__device__ __forceinline__ real inv_zone_size_xmed(const dmesh& m, int i, int j, int k) {
+  if (G == SPH) return (m.invdiffxmed[i] / m.ymed[j]) / m.sinzmed[k];
+  return m.invdiffxmed[i] / m.ymed[j];
+}
+__device__ __forceinline__ real zone_size_y(const dmesh& m, int j) { return m.ymin[j + 1] - m.ymin[j]; }
+template <int G>
+__device__ __forceinline__ real zone_size_z(const dmesh& m, int j, int k) {
+  if (G == SPH) return (m.zmin[k + 1] - m.zmin[k]) * m.ymed[j];
+  return m.zmin[k + 1] - m.zmin[k];
+}
+
+// FP64 zone sizes from the FP64 copies (the same operations as above)
+template <int G>
+__device__ __forceinline__ double zone_size_x64(const dmesh& m, int i, int j, int k) {
+  if (G == SPH) return (m.sxi64[i] * m.ymed64[j]) * m.sinzmed64[k];
+  return m.sxi64[i] * m.ymed64[j];
+}
+__device__ __forceinline__ double zone_size_y64(const dmesh& m, int j) { return m.ymin64[j + 1] - m.ymin64[j]; }
+template <int G>
+__device__ __forceinline__ double zone_size_z64(const dmesh& m, int j, int k) {
+  if (G == SPH) return (m.zmin64[k + 1] - m.zmin64[k]) * m.ymed64[j];
+  return m.zmin64[k + 1] - m.zmin64[k];
+}
+
+// van Leer harmonic slope (src/vanleer_y_a.c:61-68): 0 at extrema.
+__device__ __forceinline__ real vanleer_slope(real dqm, real dqp) {
+  if (dqp * dqm <= real(0.0)) return real(0.0);
+  return real(2.) * dqp * dqm / (dqm + dqp);
+}
+// upwind interface value (src/vanleer_y_b.c:60-65)
+__device__ __forceinline__ real vanleer_face(real v, real dt, real qm, real sm, real dzm,
+                                               real q, real s, real dz) {
+  if (v > real(0.)) return qm + real(0.5) * (dzm - v * dt) * sm;
+  return q - real(0.5) * (dz + v * dt) * s;
+}
+
+}  // namespace F3D_NS
+

@@ -1,0 +1,1463 @@
+// f3d_fused_body.h -- the fused step, written once for the type `real` of a
+// build: included by f3d_fused.hip (real = double, namespace f3d, entry points
+// f3d_fused_*) and f3d_fused32.hip (real = float, namespace f3d32, entry points
+// f3d32_fused_*).  The parts that stay FP64 in every build are written with
+// `double`: ring sums (VxMed), the residual/shift split, the CFL time step.
+// Requires F3D_NS, F3D_PREC (64 or 32) and F3D_API(name) from the includer.
+//
+// The per-routine entry points (f3d_routines.hip) move every intermediate
+// array through HBM, as the reference's one-launch-per-routine GPU build
+// does (about 370 array passes per step).  The kernels here compute the same
+// values with the same per-cell arithmetic (f3d_cell.h) but keep
+// intermediates on chip:
+//
+//   kf_cfl         ring-owning: VxMed of the ring and of the next ring
+//                  (sequential sums, in parallel), per-cell CFL, shear limit,
+//                  one atomic MIN per ring -> dt (one launch, 8 bytes to host)
+//   kf_src_a       viscous stress and artificial pressure at every cell the
+//                  next kernel reads
+//   kf_src_b       SubStep1 + addviscosity + SubStep2_b on active cells
+//   (boundaries)   mid-step ghost fill on V_temp
+//   kf_sweep_z     3-D: momenta + Z sweep, one thread per (i,j) column
+//                  marching in k over a register window
+//   kf_sweep_y     momenta (2-D) + Y sweep, columns marching in j
+//   kf_sweep_x     ring-owning: VxMed, residual, van Leer and PPA X sweeps,
+//                  integer shift and NewVelocity_x in LDS
+//   kf_finish      NewVelocity_y/z and Stockholm damping
+//   (boundaries)   end-of-step ghost fill
+//
+// Only the active cells of the primitive fields carry information from one
+// stage to the next: every ghost value is rewritten by the ghost fill from
+// active values (std/boundaries.txt formulas read mirror rows only), so the
+// fused kernels evaluate ghost rows only where an active cell's stencil
+// reads them, and there with the reference's formulas.
+namespace F3D_NS {
+namespace fused {
+
+namespace {
+
+constexpr int WG = 256;
+constexpr int BXF = 64;
+
+// Scratch of the fused stages, n3 doubles each, allocated on a handle's
+// first fused call and released with the handle: Y- and X-swept density,
+// artificial pressure (3), energy after SubStep2_b and after the Y sweep, the
+// X-swept Y/Z momenta (4) and (3-D) the Z-swept density, six momenta and energy.
+enum { FS_RHOY = 0, FS_RHOX, FS_QX, FS_QY, FS_QZ, FS_E2, FS_EY, FS_XM0, FS_Z0 = FS_XM0 + 4, FS_COUNT = FS_Z0 + 8 };
+
+struct Scratch {
+  real* f[FS_COUNT];
+  unsigned long long* result;  // two slots, alternated per step
+  int slot;
+  dmesh m;                     // the build's mesh view
+  mreal* mesh_r;               // FP32 build: converted mesh arrays
+  real* bctab_r;               // FP32 build: converted boundary tables
+  const real *ymin_dexp, *ymax_dexp, *zmin_dens, *zmax_dens;
+};
+
+constexpr unsigned long long kInfBits = 0x7FF0000000000000ull;
+
+void scratch_free(void* p) {
+  Scratch* S = static_cast<Scratch*>(p);
+  for (int a = 0; a < FS_COUNT; a++)
+    if (S->f[a]) (void)hipFree(S->f[a]);
+  if (S->result) (void)hipFree(S->result);
+  if (S->mesh_r) (void)hipFree(S->mesh_r);
+  if (S->bctab_r) (void)hipFree(S->bctab_r);
+  delete S;
+}
+
+template <class T>
+__global__ void k_convert(const double* a, T* b, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = (T)a[t];
+}
+
+// The build's view of the handle's mesh: the FP64 build uses the handle's
+// arrays; the FP32 build converts them once (the FP64 copies stay for the
+// parts that remain FP64).
+int make_mesh(f3d_hip_ctx* ctx, Scratch* S) {
+#if F3D_PREC == 64
+  S->m = ctx->m;
+  S->ymin_dexp = ctx->d_ymin_dexp;
+  S->ymax_dexp = ctx->d_ymax_dexp;
+  S->zmin_dens = ctx->d_zmin_dens;
+  S->zmax_dens = ctx->d_zmax_dens;
+#else
+  const f3d::dmesh& d = ctx->m;
+  const size_t n = ctx->meshlen;
+  if (hipMalloc(&S->mesh_r, n * sizeof(mreal)) != hipSuccess) return -1;
+  hipLaunchKernelGGL(k_convert<mreal>, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, ctx->meshbuf, S->mesh_r, n);
+  dmesh& m = S->m;
+  m.nx = d.nx; m.ny = d.ny; m.nz = d.nz; m.nghy = d.nghy; m.nghz = d.nghz; m.dimz = d.dimz; m.geom = d.geom;
+  m.sx = d.sx; m.sy = d.sy; m.sz = d.sz; m.pitch = d.pitch; m.stride = d.stride;
+  m.XMIN = d.XMIN; m.XMAX = d.XMAX; m.mindx = d.mindx;
+  auto conv = [&](const double* p) -> const mreal* { return p ? S->mesh_r + (p - ctx->meshbuf) : nullptr; };
+  m.xmin = conv(d.xmin); m.ymin = conv(d.ymin); m.zmin = conv(d.zmin);
+  m.xmed = conv(d.xmed); m.ymed = conv(d.ymed); m.zmed = conv(d.zmed);
+  m.sxi = conv(d.sxi); m.sxj = conv(d.sxj); m.syj = conv(d.syj); m.szj = conv(d.szj);
+  m.sxk = conv(d.sxk); m.syk = conv(d.syk); m.szk = conv(d.szk); m.invvj = conv(d.invvj);
+  m.invdiffxmed = conv(d.invdiffxmed); m.cosxmed = conv(d.cosxmed); m.sinxmed = conv(d.sinxmed);
+  m.sinzmed = conv(d.sinzmed); m.coszmed = conv(d.coszmed); m.sinzmin = conv(d.sinzmin); m.coszmin = conv(d.coszmin);
+  m.xmin64 = d.xmin; m.ymin64 = d.ymin; m.zmin64 = d.zmin; m.ymed64 = d.ymed; m.sinzmed64 = d.sinzmed;
+  m.sxi64 = d.sxi;
+  if (ctx->bctab) {
+    const size_t ny3 = 3 * (size_t)d.sz, nz3 = 3 * (size_t)d.sy, nb = 2 * ny3 + 2 * nz3;
+    if (hipMalloc(&S->bctab_r, nb * sizeof(real)) != hipSuccess) return -1;
+    hipLaunchKernelGGL(k_convert<real>, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream, ctx->bctab, S->bctab_r, nb);
+    S->ymin_dexp = S->bctab_r;
+    S->ymax_dexp = S->bctab_r + ny3;
+    S->zmin_dens = S->bctab_r + 2 * ny3;
+    S->zmax_dens = S->bctab_r + 2 * ny3 + nz3;
+  }
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return -1;
+#endif
+  return 0;
+}
+
+constexpr int kPrec = (F3D_PREC == 32) ? 1 : 0;
+
+int scratch_of(f3d_hip_ctx* ctx, Scratch** out) {
+  if (!ctx->fused[kPrec]) {
+    Scratch* S = new Scratch();
+    memset(S, 0, sizeof(*S));
+    ctx->fused[kPrec] = S;
+    ctx->fused_free[kPrec] = scratch_free;
+    if (make_mesh(ctx, S)) return -1;
+    const int nbuf = ctx->m.dimz ? FS_COUNT : FS_Z0;
+    for (int a = 0; a < nbuf; a++) {
+      if (hipMalloc(&S->f[a], ctx->n3 * sizeof(real)) != hipSuccess) return -1;
+      if (hipMemset(S->f[a], 0, ctx->n3 * sizeof(real)) != hipSuccess) return -1;
+    }
+    if (hipMalloc(&S->result, 2 * sizeof(unsigned long long)) != hipSuccess) return -1;
+    unsigned long long init[2] = {kInfBits, kInfBits};
+    if (hipMemcpy(S->result, init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess) return -1;
+  }
+  *out = static_cast<Scratch*>(ctx->fused[kPrec]);
+  return 0;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    double w = __shfl_down(v, o, 64);
+    v = (w > v) ? w : v;
+  }
+  return v;
+}
+
+// Block-wide max; every thread gets the result.
+__device__ double block_max(double v, double* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  v = wave_max(v);
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < nw; w++) r = (sh[w] > r) ? sh[w] : r;
+    sh[0] = r;
+  }
+  __syncthreads();
+  r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// Sequential sum in index order (the reduction order of src/reduction_generic.c
+// :18-20), in FP64; eight LDS loads are issued ahead of their dependent additions.
+__device__ __forceinline__ double seq_sum(const double* w, int n) {
+  double acc = 0.0;
+  int i = 0;
+  for (; i + 8 <= n; i += 8) {
+    double t0 = w[i], t1 = w[i + 1], t2 = w[i + 2], t3 = w[i + 3];
+    double t4 = w[i + 4], t5 = w[i + 5], t6 = w[i + 6], t7 = w[i + 7];
+    acc = acc + t0;
+    acc = acc + t1;
+    acc = acc + t2;
+    acc = acc + t3;
+    acc = acc + t4;
+    acc = acc + t5;
+    acc = acc + t6;
+    acc = acc + t7;
+  }
+  for (; i < n; i++) acc = acc + w[i];
+  return acc;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- kf_cfl
+// One workgroup per active ring (j,k), in FP64 in every build.  The cell time
+// step CFL/sqrt(s) is monotone non-increasing in s and IEEE sqrt/division
+// round monotonically, so the MIN over a ring's cells is CFL/sqrt(max s)
+// exactly; the MIN over rings is order independent, so one atomic MIN per
+// ring on the bit pattern of the positive double gives the value of
+// reduction_full_MIN + cfl_b (src/cfl_b.c:6-37) exactly.
+template <int G, int DZ>
+__global__ void __launch_bounds__(WG) kf_cfl(dmesh m, const real* cs, const real* rho, const real* vx,
+                                             const real* vy, const real* vz, double* vxmed,
+                                             unsigned long long* result, int slot, cell::CflPar cp) {
+  extern __shared__ double lds[];  // 2 nx weights + 16
+  double* w0 = lds;
+  double* w1 = lds + m.nx;
+  double* sh = lds + 2 * m.nx;
+  const int j = m.nghy + (int)(blockIdx.x % m.ny);
+  const int k = DZ ? m.nghz + (int)(blockIdx.x / m.ny) : 0;
+  const int base = j * m.pitch + k * m.stride;
+  const int l2 = j + k * m.sy;
+  if (blockIdx.x == 0 && threadIdx.x == 0) result[slot ^ 1] = kInfBits;  // the next step's slot
+  // ComputeVweight + reduction_SUM (src/compute_vmed.c:14-30) of rings j and j+1
+  for (int i = threadIdx.x; i < m.nx; i += blockDim.x) {
+    const double dx = m.xmin64[i + 1] - m.xmin64[i];
+    w0[i] = (double)vx[base + i] * dx;
+    if (cp.fargo) w1[i] = (double)vx[base + m.pitch + i] * dx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) sh[8] = seq_sum(w0, m.nx) / (m.XMAX - m.XMIN);
+  if (threadIdx.x == 64 && cp.fargo) sh[9] = seq_sum(w1, m.nx) / (m.XMAX - m.XMIN);
+  __syncthreads();
+  const double vm0 = sh[8];
+  double mx = 0.0;
+  for (int i = threadIdx.x; i < m.nx; i += blockDim.x) {
+    const double s = cell::cfl_sumsq<G, DZ>(m, i, j, k, base + i, cs, rho, vx, vy, vz, vm0, cp);
+    mx = (s > mx) ? s : mx;
+  }
+  mx = block_max(mx, sh);
+  if (threadIdx.x == 0) {
+    vxmed[l2] = vm0;
+    double mn = cp.cfl / sqrt(mx);
+    if (cp.fargo) {
+      const double shearlimit = cp.cfl * m.mindx / fabs(vm0 / m.ymed64[j] - sh[9] / m.ymed64[j + 1]);
+      if (shearlimit < mn) mn = shearlimit;
+    }
+    atomicMin(&result[slot], (unsigned long long)__double_as_longlong(mn));
+  }
+}
+
+// ------------------------------------------------------------------ sources
+// Loader of the pressure field computed on the fly (src/CompPresIso.c:56,
+// src/CompPresAd.c:58).
+struct PresLoader {
+  cell::GL rho, e;
+  real gm1;
+  int iso;
+  __device__ __forceinline__ real operator()(int di, int dj, int dk) const {
+    if (iso) {
+      real c = e(di, dj, dk);
+      return rho(di, dj, dk) * c * c;
+    }
+    return gm1 * e(di, dj, dk);
+  }
+};
+
+// Loader of Vx after CorrectVtheta (src/correct_vtheta.c:51-56) applied on the fly.
+template <int G>
+struct VxcLoader {
+  cell::GL v;
+  const mreal* ymed;
+  const mreal* sinzmed;
+  int j, k;
+  real domega;
+  __device__ __forceinline__ real operator()(int di, int dj, int dk) const {
+    if (G == SPH) return v(di, dj, dk) - domega * ymed[j + dj] * sinzmed[k + dk];
+    return v(di, dj, dk) - domega * ymed[j + dj];
+  }
+};
+
+template <int G>
+__device__ __forceinline__ VxcLoader<G> vxc(const real* vx, const dmesh& m, int l, int i, int j, int k, real dom) {
+  return VxcLoader<G>{cell::gl(vx, m, l, i), m.ymed, m.sinzmed, j, k, dom};
+}
+
+// kf_src_a: stresses (visctensor ranges j,k in [1, s-2]) and artificial
+// pressure (SubStep2_a) at every cell the active update reads.
+template <int G, int DZ>
+__global__ void __launch_bounds__(WG) kf_src_a(dmesh m, const real* rho, const real* energy, const real* vx,
+                                               const real* vy, const real* vz, real* txx, real* tyy,
+                                               real* tzz, real* txz, real* tyx, real* tzy, real* qx,
+                                               real* qy, real* qz, cell::ViscPar vp, int visc, real domega) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int j = 1 + (int)blockIdx.y;       // 1 .. sy-2
+  const int k = DZ ? 1 + (int)blockIdx.z : 0;
+  const int l = i + j * m.pitch + k * m.stride;
+  const auto VX = vxc<G>(vx, m, l, i, j, k, domega);
+  const cell::GL VY = cell::gl(vy, m, l, i);
+  const cell::GL VZ = cell::gl(DZ ? vz : vy, m, l, i);
+  const cell::GL R = cell::gl(rho, m, l, i);
+  if (visc) {
+    cell::Taus t = cell::visctensor<G, DZ>(m, i, j, k, R, cell::gl(energy, m, l, i), VX, VY, VZ, vp);
+    txx[l] = t.xx;
+    tyy[l] = t.yy;
+    tyx[l] = t.yx;
+    if (DZ) {
+      tzz[l] = t.zz;
+      txz[l] = t.xz;
+      tzy[l] = t.zy;
+    }
+  }
+  // SubStep2_a (src/substep2_a.c:103-134) on j in [1, sy-2] (a subset of its
+  // range [0, sy-2] that covers every read of the active update)
+  const real r0 = R(0, 0, 0);
+  qx[l] = cell::art_q(r0, VX(1, 0, 0) - VX(0, 0, 0));
+  qy[l] = cell::art_q(r0, VY(0, 1, 0) - VY(0, 0, 0));
+  if (DZ) qz[l] = cell::art_q(r0, VZ(0, 0, 1) - VZ(0, 0, 0));
+}
+
+// kf_src_b: the velocity update of SubStep1 + addviscosity + SubStep2_b on
+// active cells, in the order the reference applies them.
+template <int G, int DZ>
+__global__ void __launch_bounds__(WG) kf_src_b(dmesh m, real dt, const real* rho, const real* energy,
+                                               const real* vx, const real* vy, const real* vz,
+                                               const real* pot, const real* txx, const real* tyy,
+                                               const real* tzz, const real* txz, const real* tyx,
+                                               const real* tzy, const real* qx, const real* qy,
+                                               const real* qz, real* vxt, real* vyt, real* vzt, real* e2,
+                                               int visc, int potential, int iso, real gm1, real omegaframe,
+                                               real domega) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int j = m.nghy + (int)blockIdx.y;
+  const int k = DZ ? m.nghz + (int)blockIdx.z : 0;
+  const int l = i + j * m.pitch + k * m.stride;
+  const cell::GL R = cell::gl(rho, m, l, i);
+  const PresLoader P{R, cell::gl(energy, m, l, i), gm1, iso};
+  const cell::GL PT = cell::gl(potential ? pot : rho, m, l, i);
+  const auto VX = vxc<G>(vx, m, l, i, j, k, domega);
+  const cell::GL VY = cell::gl(vy, m, l, i);
+  const cell::GL VZ = cell::gl(DZ ? vz : vy, m, l, i);
+  real ax = cell::substep1_x<G>(m, i, j, k, dt, R, P, PT, VX, 0, potential);
+  real ay = cell::substep1_y<G, DZ>(m, i, j, k, dt, R, P, PT, VX, VY, VZ, 0, potential, omegaframe);
+  real az = DZ ? cell::substep1_z<G>(m, i, j, k, dt, R, P, PT, VX, VZ, 0, potential, omegaframe) : real(0.0);
+  if (visc) {
+    const cell::GL TZZ = cell::gl(DZ ? tzz : txx, m, l, i), TXZ = cell::gl(DZ ? txz : txx, m, l, i),
+                   TZY = cell::gl(DZ ? tzy : txx, m, l, i);
+    cell::addviscosity<G, DZ>(m, i, j, k, dt, R, cell::gl(txx, m, l, i), cell::gl(tyy, m, l, i), TZZ, TXZ,
+                              cell::gl(tyx, m, l, i), TZY, ax, ay, az, ax, ay, az);
+  }
+  real e = iso ? real(0.0) : energy[l];
+  cell::substep2_b<G, DZ>(m, i, j, k, dt, R, VX, VY, VZ, cell::gl(qx, m, l, i), cell::gl(qy, m, l, i),
+                          cell::gl(DZ ? qz : qy, m, l, i), !iso, ax, ay, az, e, ax, ay, az, e);
+  vxt[l] = ax;
+  vyt[l] = ay;
+  if (DZ) vzt[l] = az;
+  if (!iso) e2[l] = e;
+}
+
+// SubStep3 (src/substep3.c:98-110) on active cells: compression work with the
+// divergence of V_temp; reads the energy after SubStep2_b.
+template <int G, int DZ>
+__global__ void __launch_bounds__(WG) kf_substep3(dmesh m, real dt, const real* e2, const real* vxt,
+                                                  const real* vyt, const real* vzt, real* energy,
+                                                  real gamma) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int j = m.nghy + (int)blockIdx.y;
+  const int k = DZ ? m.nghz + (int)blockIdx.z : 0;
+  const int l = i + j * m.pitch + k * m.stride;
+  energy[l] = cell::substep3<DZ>(m, i, j, k, dt, e2[l], cell::gl(vxt, m, l, i), cell::gl(vyt, m, l, i),
+                                 cell::gl(DZ ? vzt : vyt, m, l, i), gamma);
+}
+
+// --------------------------------------------------------------- transport
+// One directional van Leer sweep of one column (the Y or Z transport of
+// src/transport.c:81-127): NQ conserved quantities plus the density
+// (index NQ), rows r in [r0, r1).  Per row the quantity q = Q/rho
+// (src/divrho.c:56), the gradient at the lower face (dqm of
+// src/vanleer_y_a.c:63, equal to dqp of the row below), the slope and the
+// upwind face value (src/vanleer_y_b.c:60-65) are each computed once and
+// kept in a window of registers.
+//   col.rho(r)      density at row r (not modified by the sweep)
+//   col.Q(c, r)     conserved quantity c at row r
+//   col.zs(r)       zone size of row r in the sweep direction
+//   col.v(r)        velocity at the lower face of row r
+//   col.flux(r, a, b) conservative increment of row r from face fluxes a, b
+//   col.out(c, r, value) store of the updated quantity (c == NQ: density)
+template <int NQ, class Col>
+__device__ __forceinline__ void column_sweep(Col& col, int r0, int r1, real dt) {
+  constexpr int N = NQ + 1;
+  real A1[N], A2[N], G2[N], S1[N], F0[N], F1[N];
+  {
+    real a[N][5];
+#pragma unroll
+    for (int t = 0; t < 5; t++) {
+      const int r = r0 - 2 + t;
+      const real rr = col.rho(r);
+#pragma unroll
+      for (int c = 0; c < NQ; c++) a[c][t] = col.Q(c, r) / rr;
+      a[NQ][t] = rr;
+    }
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      const real g0 = (a[c][1] - a[c][0]) / col.zs(r0 - 1);
+      const real g1 = (a[c][2] - a[c][1]) / col.zs(r0);
+      const real g2 = (a[c][3] - a[c][2]) / col.zs(r0 + 1);
+      const real g3 = (a[c][4] - a[c][3]) / col.zs(r0 + 2);
+      const real sm = vanleer_slope(g0, g1);
+      const real s0 = vanleer_slope(g1, g2);
+      const real s1 = vanleer_slope(g2, g3);
+      F0[c] = vanleer_face(col.v(r0), dt, a[c][1], sm, col.zs(r0 - 1), a[c][2], s0, col.zs(r0));
+      F1[c] = vanleer_face(col.v(r0 + 1), dt, a[c][2], s0, col.zs(r0), a[c][3], s1, col.zs(r0 + 1));
+      A1[c] = a[c][3];
+      A2[c] = a[c][4];
+      G2[c] = g3;
+      S1[c] = s1;
+    }
+  }
+  for (int r = r0; r < r1; r++) {
+    const real v0 = col.v(r), v1 = col.v(r + 1);
+    // UpdateY/Z (src/update_y.c:68-70), then the density last (src/update_density_y.c:66-68)
+#pragma unroll
+    for (int c = 0; c < NQ; c++)
+      col.out(c, r, col.Q(c, r) + col.flux(r, v0 * F0[c] * F0[NQ], v1 * F1[c] * F1[NQ]));
+    col.out(NQ, r, col.rho(r) + col.flux(r, v0 * F0[NQ], v1 * F1[NQ]));
+    if (r + 1 >= r1) break;
+    // advance the window by one row: row r+3 enters
+    const int rn = r + 3;
+    const real rr = col.rho(rn);
+    const real zsn = col.zs(rn), zs2 = col.zs(r + 2), zs1 = col.zs(r + 1), v2 = col.v(r + 2);
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      const real an = (c < NQ) ? col.Q(c, rn) / rr : rr;
+      const real g3 = (an - A2[c]) / zsn;
+      const real s2 = vanleer_slope(G2[c], g3);
+      const real f2 = vanleer_face(v2, dt, A1[c], S1[c], zs1, A2[c], s2, zs2);
+      F0[c] = F1[c];
+      F1[c] = f2;
+      A1[c] = A2[c];
+      A2[c] = an;
+      G2[c] = g3;
+      S1[c] = s2;
+    }
+  }
+}
+
+// Y column of the 2-D step: momenta computed from rho and V_temp on the fly
+// (src/momenta_x.c:76-79, src/momenta_y.c:62-63); quantity 4 is the energy
+// (adiabatic).  out_[c] receives quantity c, out_[NQ] the density.
+template <int G>
+struct ColY2 {
+  const dmesh* m;
+  const real *rho_in, *vxt, *vyt, *e_in;
+  real* out_[6];
+  int i, ixp;
+  real of, dt;
+  __device__ __forceinline__ int L(int r) const { return i + r * m->pitch; }
+  __device__ __forceinline__ real rho(int r) const { return rho_in[L(r)]; }
+  __device__ __forceinline__ real Q(int c, int r) const {
+    const int l = L(r);
+    const real rr = rho_in[l];
+    if (c < 2) {
+      real mm, mp;
+      cell::momenta_x<G>(*m, i, r, 0, rr, vxt[l], vxt[ixp + r * m->pitch], of, mm, mp);
+      return c == 0 ? mp : mm;
+    }
+    if (c == 2) return vyt[l + m->pitch] * rr;
+    if (c == 3) return vyt[l] * rr;
+    return e_in[l];
+  }
+  __device__ __forceinline__ real zs(int r) const { return zone_size_y(*m, r); }
+  __device__ __forceinline__ real v(int r) const { return vyt[L(r)]; }
+  __device__ __forceinline__ real flux(int r, real a, real b) const { return cell::flux_y(*m, i, r, 0, dt, a, b); }
+  __device__ __forceinline__ void out(int c, int r, real val) const { out_[c][L(r)] = val; }
+};
+
+// Z column of the 3-D step: momenta from rho and V_temp on the fly
+// (src/momenta_x.c:80-84, src/momenta_y.c:62-63, src/momenta_z.c:66-71);
+// quantity 6 is the energy (adiabatic).
+template <int G>
+struct ColZ {
+  const dmesh* m;
+  const real *rho_in, *vxt, *vyt, *vzt, *e_in;
+  real* out_[8];
+  int i, j, ixp;
+  real of, dt;
+  __device__ __forceinline__ int L(int r) const { return i + j * m->pitch + r * m->stride; }
+  __device__ __forceinline__ real rho(int r) const { return rho_in[L(r)]; }
+  __device__ __forceinline__ real Q(int c, int r) const {
+    const int l = L(r);
+    const real rr = rho_in[l];
+    switch (c) {
+      case 0:
+      case 1: {
+        real mm, mp;
+        cell::momenta_x<G>(*m, i, j, r, rr, vxt[l], vxt[ixp + j * m->pitch + r * m->stride], of, mm, mp);
+        return c == 0 ? mp : mm;
+      }
+      case 2:
+        return vyt[l + m->pitch] * rr;
+      case 3:
+        return vyt[l] * rr;
+      case 4: {
+        real p = vzt[l + m->stride] * rr;
+        if (G == SPH) p *= m->ymed[j];
+        return p;
+      }
+      case 5: {
+        real q = vzt[l] * rr;
+        if (G == SPH) q *= m->ymed[j];
+        return q;
+      }
+      default:
+        return e_in[l];
+    }
+  }
+  __device__ __forceinline__ real zs(int r) const { return zone_size_z<G>(*m, j, r); }
+  __device__ __forceinline__ real v(int r) const { return vzt[L(r)]; }
+  __device__ __forceinline__ real flux(int r, real a, real b) const { return cell::flux_z(*m, i, j, r, dt, a, b); }
+  __device__ __forceinline__ void out(int c, int r, real val) const { out_[c][L(r)] = val; }
+};
+
+// Y column of the 3-D step: reads the Z-swept density, momenta (and energy).
+struct ColY3 {
+  const dmesh* m;
+  const real* rho_in;
+  const real* q_in[7];
+  const real* vyt;
+  real* out_[8];
+  int i, k;
+  real dt;
+  __device__ __forceinline__ int L(int r) const { return i + r * m->pitch + k * m->stride; }
+  __device__ __forceinline__ real rho(int r) const { return rho_in[L(r)]; }
+  __device__ __forceinline__ real Q(int c, int r) const { return q_in[c][L(r)]; }
+  __device__ __forceinline__ real zs(int r) const { return zone_size_y(*m, r); }
+  __device__ __forceinline__ real v(int r) const { return vyt[L(r)]; }
+  __device__ __forceinline__ real flux(int r, real a, real b) const { return cell::flux_y(*m, i, r, k, dt, a, b); }
+  __device__ __forceinline__ void out(int c, int r, real val) const { out_[c][L(r)] = val; }
+};
+
+// 2-D: momenta + Y sweep; rows [nghy + s*rows_per, ...) of the active range.
+// AD = 1 also transports the energy (e_in -> e_out).
+template <int G, int DZ, int AD>
+__global__ void __launch_bounds__(BXF) kf_sweep_y2(dmesh m, real dt, const real* rho_in, const real* vxt,
+                                                   const real* vyt, const real* e_in, real* rho_out,
+                                                   real* q0, real* q1, real* q2, real* q3, real* e_out,
+                                                   int rows_per, real of) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int r0 = m.nghy + (int)blockIdx.y * rows_per;
+  const int r1 = min(r0 + rows_per, m.ny + m.nghy);
+  ColY2<G> col{&m, rho_in, vxt, vyt, e_in, {q0, q1, q2, q3, AD ? e_out : rho_out, rho_out}, i, ixp_of(i, m.nx), of,
+               dt};
+  column_sweep<4 + AD>(col, r0, r1, dt);
+}
+
+// 3-D: Y sweep of the Z-swept fields
+template <int G, int DZ, int AD>
+__global__ void __launch_bounds__(BXF) kf_sweep_y3(dmesh m, real dt, const real* rho_in, const real* z0,
+                                                   const real* z1, const real* z2, const real* z3,
+                                                   const real* z4, const real* z5, const real* ze,
+                                                   const real* vyt, real* rho_out, real* q0, real* q1,
+                                                   real* q2, real* q3, real* q4, real* q5, real* e_out,
+                                                   int rows_per) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int k = m.nghz + (int)blockIdx.z;
+  const int r0 = m.nghy + (int)blockIdx.y * rows_per;
+  const int r1 = min(r0 + rows_per, m.ny + m.nghy);
+  ColY3 col{&m, rho_in, {z0, z1, z2, z3, z4, z5, ze}, vyt,
+            {q0, q1, q2, q3, q4, q5, AD ? e_out : rho_out, rho_out}, i, k, dt};
+  column_sweep<6 + AD>(col, r0, r1, dt);
+}
+
+// 3-D: momenta + Z sweep for rows j in [1, sy-2] (the Y sweep's stencil)
+template <int G, int DZ, int AD>
+__global__ void __launch_bounds__(BXF) kf_sweep_z(dmesh m, real dt, const real* rho_in, const real* vxt,
+                                                  const real* vyt, const real* vzt, const real* e_in,
+                                                  real* rho_out, real* q0, real* q1, real* q2, real* q3,
+                                                  real* q4, real* q5, real* e_out, int planes_per, real of) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int j = 1 + (int)blockIdx.y;
+  const int r0 = m.nghz + (int)blockIdx.z * planes_per;
+  const int r1 = min(r0 + planes_per, m.nz + m.nghz);
+  ColZ<G> col{&m, rho_in, vxt, vyt, vzt, e_in, {q0, q1, q2, q3, q4, q5, AD ? e_out : rho_out, rho_out}, i, j,
+              ixp_of(i, m.nx), of, dt};
+  column_sweep<6 + AD>(col, r0, r1, dt);
+}
+
+// X sweep, ring-owning: VxMed of Vx_temp, residual split, van Leer pass with
+// the cell residual, PPA pass with the uniform residual, integer shift and
+// NewVelocity_x (src/transport.c:129-192).  LDS holds the ring; every
+// quantity of a pass goes through the same phase between two barriers.
+// Thread t owns cells i = t + s*blockDim.x, s < CPT.  Quantities: the
+// momenta in the order of src/transport.c (Mpx, Mmx, Mpy, Mmy[, Mpz, Mmz]),
+// then with AD the energy (e_in -> e_out), then the density.
+template <int G, int DZ, int AD, int CPT>
+__global__ void __launch_bounds__(256) kf_sweep_x(dmesh m, double dt64, const real* rho_in, const real* vxt,
+                                                   real* mpx, real* mmx, real* mpy, real* mmy, real* mpz,
+                                                   real* mmz, const real* e_in, real* e_out, real* rho_out,
+                                                   real* vx_out, real* mpy_o, real* mmy_o, real* mpz_o,
+                                                   real* mmz_o, double* vxmed_out, int* nshift_out,
+                                                   real omegaframe, int fargo) {
+  constexpr int NM = DZ ? 6 : 4;  // momenta
+  constexpr int NQ = NM + AD;
+  constexpr int N = NQ + 1;  // quantities + density (index NQ)
+  extern __shared__ double lds_d[];
+  const int nx = m.nx;
+  const real dt = (real)dt64;
+  double* Wt = lds_d;           // FP64 ring weights: nx
+  double* sh = Wt + nx;         // 8
+  real* ST = (real*)(sh + 8);   // state: N * nx (quantities, then density)
+  real* AQ = ST + N * nx;       // q = Q/rho (or rho): N * nx
+  real* SL = AQ + N * nx;       // slopes, then limited qL: N * nx
+  real* FC = SL + N * nx;       // faces, then face fluxes: N * nx
+  real* QL2 = FC + N * nx;      // limited qR of the PPA pass: N * nx
+  real* V = QL2 + N * nx;       // advection velocity: nx
+  const int j = m.nghy + (int)(blockIdx.x % m.ny);
+  const int k = DZ ? m.nghz + (int)(blockIdx.x / m.ny) : 0;
+  const int base = j * m.pitch + k * m.stride;
+  real* qg[6] = {mpx, mmx, mpy, mmy, mpz, mmz};
+  const int t = threadIdx.x, nt = blockDim.x;
+  double vt[CPT];
+  real vu[CPT];
+#pragma unroll
+  for (int s = 0; s < CPT; s++) {
+    const int i = t + s * nt;
+    if (i < nx) {
+      ST[NQ * nx + i] = rho_in[base + i];
+      #pragma unroll
+      for (int c = 0; c < NQ; c++) ST[c * nx + i] = (c < NM) ? qg[c][base + i] : e_in[base + i];
+      vt[s] = vxt[base + i];
+      Wt[i] = vt[s] * (m.xmin64[i + 1] - m.xmin64[i]);
+    }
+  }
+  __syncthreads();
+  // ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30), FP64
+  if (t == 0) sh[0] = seq_sum(Wt, nx) / (m.XMAX - m.XMIN);
+  __syncthreads();
+  const double vmed = sh[0];
+  // ComputeResidual (src/compute_residual.c:68-73), FP64: cell residual into V
+#pragma unroll
+  for (int s = 0; s < CPT; s++) {
+    const int i = t + s * nt;
+    if (i < nx) {
+      if (fargo) {
+        const double zs = zone_size_x64<G>(m, i, j, k);
+        const double ntilde = vmed * dt64 / zs;
+        const double nround = floor(ntilde + 0.5);
+        if (i == 0) sh[1] = nround;
+        V[i] = (real)(vt[s] - vmed);
+        vu[s] = (real)((ntilde - nround) * zs / dt64);
+      } else {
+        V[i] = (real)vt[s];
+      }
+    }
+  }
+  __syncthreads();
+  int ns = 0;
+  if (fargo) {
+    ns = (int)sh[1];
+    if (t == 0) {
+      nshift_out[j + k * m.sy] = ns;
+      vxmed_out[j + k * m.sy] = vmed;
+    }
+  }
+  // X_advection (src/transport.c:208-226), twice with FARGO
+  for (int pass = 0; pass < (fargo ? 2 : 1); pass++) {
+    const bool ppa = (pass == 1);
+    if (ppa) {
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) V[i] = vu[s];
+      }
+    }
+    // q = Q/rho (DivideByRho) and, for the density, q = rho
+#pragma unroll
+    for (int s = 0; s < CPT; s++) {
+      const int i = t + s * nt;
+      if (i < nx) {
+        const real r = ST[NQ * nx + i];
+        #pragma unroll
+        for (int c = 0; c < NQ; c++) AQ[c * nx + i] = ST[c * nx + i] / r;
+        AQ[NQ * nx + i] = r;
+      }
+    }
+    __syncthreads();
+    if (!ppa) {
+      // VanLeerX_a (src/vanleer_x_a.c:60-68)
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+          const int im = ixm_of(i, nx), ip = ixp_of(i, nx);
+          const real zi = zone_size_x<G>(m, i, j, k), zp = zone_size_x<G>(m, ip, j, k);
+          #pragma unroll
+          for (int c = 0; c < N; c++) {
+            const real* a = AQ + c * nx;
+            SL[c * nx + i] = vanleer_slope((a[i] - a[im]) / zi, (a[ip] - a[i]) / zp);
+          }
+        }
+      }
+      __syncthreads();
+      // VanLeerX_b (src/vanleer_x_b.c:60-65)
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+          const int im = ixm_of(i, nx);
+          const real zi = zone_size_x<G>(m, i, j, k), zm = zone_size_x<G>(m, im, j, k);
+          const real v = V[i];
+          #pragma unroll
+          for (int c = 0; c < N; c++) {
+            const real* a = AQ + c * nx;
+            const real* sl = SL + c * nx;
+            FC[c * nx + i] = vanleer_face(v, dt, a[im], sl[im], zm, a[i], sl[i], zi);
+          }
+        }
+      }
+      __syncthreads();
+    } else {
+      // VanLeerX_PPA_a: monotonised-centred slope (src/fargo_ppa_a.c:58-69)
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+          const int im = ixm_of(i, nx), ip = ixp_of(i, nx);
+          #pragma unroll
+          for (int c = 0; c < N; c++) {
+            const real* a = AQ + c * nx;
+            SL[c * nx + i] = cell::ppa_slope(a[im], a[i], a[ip]);
+          }
+        }
+      }
+      __syncthreads();
+      // VanLeerX_PPA_b: face value q_{i+1/2} = qR of cell i = qL of cell i+1
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+          const int ip = ixp_of(i, nx);
+          #pragma unroll
+          for (int c = 0; c < N; c++) {
+            const real* a = AQ + c * nx;
+            const real* sl = SL + c * nx;
+            FC[c * nx + i] = cell::ppa_face(a[i], a[ip], sl[i], sl[ip]);
+          }
+        }
+      }
+      __syncthreads();
+      // VanLeerX_PPA_c (src/fargo_ppa_c.c): limited (qL, qR) of each cell into
+      // SL (slopes are no longer read) and QL2; the faces stay in FC
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+          const int im = ixm_of(i, nx);
+#pragma unroll
+          for (int c = 0; c < N; c++) {
+            real qL = FC[c * nx + im], qR = FC[c * nx + i];
+            cell::ppa_limit(AQ[c * nx + i], qL, qR);
+            SL[c * nx + i] = qL;
+            QL2[c * nx + i] = qR;
+          }
+        }
+      }
+      __syncthreads();
+      // VanLeerX_PPA_d: parabolic average over the swept zone (src/fargo_ppa_d.c:62-70)
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+          const int im = ixm_of(i, nx);
+          const real zi = zone_size_x<G>(m, i, j, k);
+          const real v = V[i];
+#pragma unroll
+          for (int c = 0; c < N; c++) {
+            const real* a = AQ + c * nx;
+            const real* qL = SL + c * nx;
+            const real* qR = QL2 + c * nx;
+            FC[c * nx + i] = cell::ppa_flux_value(v, dt, zi, a[im], qL[im], qR[im], a[i], qL[i], qR[i]);
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // UpdateX for every quantity and UpdateDensityX last (src/update_x.c:69-71,
+    // src/update_density_x.c:66-68); a cell reads only faces, so in place
+#pragma unroll
+    for (int s = 0; s < CPT; s++) {
+      const int i = t + s * nt;
+      if (i < nx) {
+        const int ip = ixp_of(i, nx);
+        const real v0 = V[i], v1 = V[ip];
+        const real* ds = FC + NQ * nx;
+        #pragma unroll
+        for (int c = 0; c < NQ; c++) {
+          const real* f = FC + c * nx;
+          ST[c * nx + i] += cell::flux_x(m, i, j, k, dt, v0 * f[i] * ds[i], v1 * f[ip] * ds[ip]);
+        }
+        ST[NQ * nx + i] += cell::flux_x(m, i, j, k, dt, v0 * ds[i], v1 * ds[ip]);
+      }
+    }
+    __syncthreads();
+  }
+  // AdvectSHIFT (src/advect_shift.c:45-58): cell i moves to i + ns (mod nx);
+  // the shifted ring goes to AQ, which is free after the last update
+  const real* OUT = ST;
+  if (fargo) {
+#pragma unroll
+    for (int s = 0; s < CPT; s++) {
+      const int i = t + s * nt;
+      if (i < nx) {
+        int it = (i + ns) % nx;
+        if (it < 0) it += nx;
+#pragma unroll
+        for (int c = 0; c < N; c++) AQ[c * nx + it] = ST[c * nx + i];
+      }
+    }
+    __syncthreads();
+    OUT = AQ;
+  }
+  // NewVelocity_x (src/newvel_x.c:78-86) and outputs
+#pragma unroll
+  for (int s = 0; s < CPT; s++) {
+    const int i = t + s * nt;
+    if (i < nx) {
+      const int im = ixm_of(i, nx);
+      const real* R = OUT + NQ * nx;
+      vx_out[base + i] = cell::newvel_x<G>(m, i, j, k, R[i], R[im], OUT[1 * nx + i], OUT[0 * nx + im], omegaframe);
+      rho_out[base + i] = R[i];
+#pragma unroll
+      for (int c = 2; c < NQ; c++) {
+        if (c < NM) (c == 2 ? mpy_o : c == 3 ? mmy_o : c == 4 ? mpz_o : mmz_o)[base + i] = OUT[c * nx + i];
+        else e_out[base + i] = OUT[c * nx + i];
+      }
+    }
+  }
+}
+
+// ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30) for the tiled X sweep: one
+// workgroup per active ring, the ring's weights in LDS, one sequential sum.
+template <int G, int DZ>
+__global__ void __launch_bounds__(WG) kf_vmed(dmesh m, const real* vxt, double* vxmed) {
+  extern __shared__ double w[];
+  const int j = m.nghy + (int)(blockIdx.x % m.ny);
+  const int k = DZ ? m.nghz + (int)(blockIdx.x / m.ny) : 0;
+  const int base = j * m.pitch + k * m.stride;
+  for (int i = threadIdx.x; i < m.nx; i += blockDim.x)
+    w[i] = (double)vxt[base + i] * (m.xmin64[i + 1] - m.xmin64[i]);
+  __syncthreads();
+  if (threadIdx.x == 0) vxmed[j + k * m.sy] = seq_sum(w, m.nx) / (m.XMAX - m.XMIN);
+}
+
+// X sweep of rings too long for LDS: workgroups own tiles of T cells and
+// load H = 6 halo cells on each side (the van Leer pass needs 2, the PPA pass
+// 3 more of the van Leer result, NewVelocity_x the left neighbour), so no
+// tile waits for another.  VxMed comes
+// from kf_vmed.  The outputs go to their shifted positions directly
+// (AdvectSHIFT, src/advect_shift.c:45-58) with NewVelocity_x computed at the
+// destination cell from the source cell and its left neighbour.
+template <int G, int DZ, int AD, int T>
+__global__ void __launch_bounds__(T) kf_sweep_x_tile(dmesh m, double dt64, const real* rho_in, const real* vxt,
+                                                      const real* mpx, const real* mmx, const real* mpy,
+                                                      const real* mmy, const real* mpz, const real* mmz,
+                                                      const real* e_in, real* e_out, real* rho_out,
+                                                      real* vx_out, real* mpy_o, real* mmy_o, real* mpz_o,
+                                                      real* mmz_o, const double* vxmed, int* nshift_out,
+                                                      real omegaframe, int fargo) {
+  constexpr int H = 6, W = T + 2 * H;
+  const real dt = (real)dt64;
+  constexpr int NM = DZ ? 6 : 4;
+  constexpr int NQ = NM + AD;
+  constexpr int N = NQ + 1;
+  __shared__ real ST[N * W], AQ[N * W], SL[N * W], FC[N * W], QR2[N * W];
+  __shared__ real V[W], VU[W], ZS[W];
+  const int nx = m.nx;
+  const int ring = blockIdx.y;
+  const int j = m.nghy + ring % m.ny;
+  const int k = DZ ? m.nghz + ring / m.ny : 0;
+  const int base = j * m.pitch + k * m.stride;
+  const int i0 = blockIdx.x * T;
+  const int t = threadIdx.x;
+  const double vmed = vxmed[j + k * m.sy];
+  const real* qin[7] = {mpx, mmx, mpy, mmy, DZ ? mpz : e_in, DZ ? mmz : e_in, e_in};
+  auto gidx = [&](int p) {
+    int g = (i0 - H + p) % nx;
+    return g < 0 ? g + nx : g;
+  };
+  // the integer shift of the ring: ComputeResidual at i = 0 (src/compute_residual.c:68-71)
+  const int ns = fargo ? (int)floor(vmed * dt64 / zone_size_x64<G>(m, 0, j, k) + 0.5) : 0;
+  if (fargo && blockIdx.x == 0 && t == 0) nshift_out[j + k * m.sy] = ns;
+  for (int p = t; p < W; p += T) {
+    const int g = gidx(p);
+    ST[NQ * W + p] = rho_in[base + g];
+#pragma unroll
+    for (int c = 0; c < NQ; c++) ST[c * W + p] = (c < NM) ? qin[c][base + g] : e_in[base + g];
+    ZS[p] = zone_size_x<G>(m, g, j, k);
+    const double v = vxt[base + g];
+    if (fargo) {
+      // the residual split in FP64 (src/compute_residual.c:68-73)
+      const double zs = zone_size_x64<G>(m, g, j, k);
+      const double ntilde = vmed * dt64 / zs;
+      const double nround = floor(ntilde + 0.5);
+      V[p] = (real)(v - vmed);
+      VU[p] = (real)((ntilde - nround) * zs / dt64);
+    } else {
+      V[p] = (real)v;
+    }
+  }
+  __syncthreads();
+  for (int pass = 0; pass < (fargo ? 2 : 1); pass++) {
+    const real* VV = pass ? VU : V;
+    // valid range of the state entering this pass: [lo, hi)
+    const int lo = pass ? 2 : 0, hi = pass ? W - 2 : W;
+    for (int p = lo + t; p < hi; p += T) {
+      const real r = ST[NQ * W + p];
+#pragma unroll
+      for (int c = 0; c < NQ; c++) AQ[c * W + p] = ST[c * W + p] / r;
+      AQ[NQ * W + p] = r;
+    }
+    __syncthreads();
+    int ulo, uhi;  // cells updated by this pass
+    if (!pass) {
+      for (int p = 1 + t; p < W - 1; p += T) {
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+          const real* a = AQ + c * W;
+          SL[c * W + p] = vanleer_slope((a[p] - a[p - 1]) / ZS[p], (a[p + 1] - a[p]) / ZS[p + 1]);
+        }
+      }
+      __syncthreads();
+      for (int p = 2 + t; p < W - 1; p += T) {
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+          const real* a = AQ + c * W;
+          const real* sl = SL + c * W;
+          FC[c * W + p] = vanleer_face(VV[p], dt, a[p - 1], sl[p - 1], ZS[p - 1], a[p], sl[p], ZS[p]);
+        }
+      }
+      __syncthreads();
+      ulo = 2;
+      uhi = W - 2;
+    } else {
+      for (int p = 3 + t; p < W - 3; p += T) {
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+          const real* a = AQ + c * W;
+          SL[c * W + p] = cell::ppa_slope(a[p - 1], a[p], a[p + 1]);
+        }
+      }
+      __syncthreads();
+      for (int p = 3 + t; p < W - 4; p += T) {
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+          const real* a = AQ + c * W;
+          const real* sl = SL + c * W;
+          FC[c * W + p] = cell::ppa_face(a[p], a[p + 1], sl[p], sl[p + 1]);
+        }
+      }
+      __syncthreads();
+      for (int p = 4 + t; p < W - 4; p += T) {
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+          real qL = FC[c * W + p - 1], qR = FC[c * W + p];
+          cell::ppa_limit(AQ[c * W + p], qL, qR);
+          SL[c * W + p] = qL;
+          QR2[c * W + p] = qR;
+        }
+      }
+      __syncthreads();
+      for (int p = 5 + t; p < W - 4; p += T) {
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+          const real* a = AQ + c * W;
+          const real* qL = SL + c * W;
+          const real* qR = QR2 + c * W;
+          FC[c * W + p] = cell::ppa_flux_value(VV[p], dt, ZS[p], a[p - 1], qL[p - 1], qR[p - 1], a[p], qL[p], qR[p]);
+        }
+      }
+      __syncthreads();
+      ulo = 5;
+      uhi = W - 5;
+    }
+    // UpdateX / UpdateDensityX (src/update_x.c:69-71, src/update_density_x.c:66-68)
+    for (int p = ulo + t; p < uhi; p += T) {
+      const int g = gidx(p);
+      const real v0 = VV[p], v1 = VV[p + 1];
+      const real* ds = FC + NQ * W;
+#pragma unroll
+      for (int c = 0; c < NQ; c++) {
+        const real* f = FC + c * W;
+        ST[c * W + p] += cell::flux_x(m, g, j, k, dt, v0 * f[p] * ds[p], v1 * f[p + 1] * ds[p + 1]);
+      }
+      ST[NQ * W + p] += cell::flux_x(m, g, j, k, dt, v0 * ds[p], v1 * ds[p + 1]);
+    }
+    __syncthreads();
+  }
+  // outputs of the tile's cells at their shifted positions
+  const int p = H + t;
+  const int i = i0 + t;
+  if (i < nx) {
+    int d = (i + ns) % nx;
+    if (d < 0) d += nx;
+    const real* R = ST + NQ * W;
+    vx_out[base + d] = cell::newvel_x<G>(m, d, j, k, R[p], R[p - 1], ST[1 * W + p], ST[0 * W + p - 1], omegaframe);
+    rho_out[base + d] = R[p];
+#pragma unroll
+    for (int c = 2; c < NQ; c++) {
+      if (c < NM) (c == 2 ? mpy_o : c == 3 ? mmy_o : c == 4 ? mpz_o : mmz_o)[base + d] = ST[c * W + p];
+      else e_out[base + d] = ST[c * W + p];
+    }
+  }
+}
+
+// NewVelocity_y/z on active faces and Stockholm damping on active cells.
+template <int G, int DZ>
+__global__ void __launch_bounds__(WG) kf_finish(dmesh m, real dt, const real* rho_x, const real* mmy,
+                                                const real* mpy, const real* mmz, const real* mpz,
+                                                real* rho, real* vx, real* vy, real* vz, const real* rho0,
+                                                const real* vx0, const real* vy0, const real* vz0,
+                                                cell::DampPar dp, int stockholm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int j = m.nghy + (int)blockIdx.y;
+  const int k = DZ ? m.nghz + (int)blockIdx.z : 0;
+  const int l = i + j * m.pitch + k * m.stride;
+  const real r = rho_x[l];
+  // the faces j = nghy and k = nghz lie on the mesh boundary; the ghost fill
+  // sets them to zero, so they are not computed here
+  if (j > m.nghy) vy[l] = cell::newvel_y(m, i, j, k, r, rho_x[l - m.pitch], mmy[l], mpy[l - m.pitch]);
+  if (DZ && k > m.nghz) {
+    real v = (mul_vol(m, mmz[l], i, j, k) + mul_vol(m, mpz[l - m.stride], i, j, k - 1)) /
+               (mul_vol(m, r, i, j, k) + mul_vol(m, rho_x[l - m.stride], i, j, k - 1));
+    if (G == SPH) v /= m.ymed[j];
+    vz[l] = v;
+  }
+  rho[l] = r;
+  if (stockholm) cell::stockholm<G, DZ>(m, j, k, l, dt, rho, vx, vy, vz, rho0, vx0, vy0, vz0, dp);
+}
+
+// ------------------------------------------------------------------ ghost fill, potential, damping references
+// The same per-cell formulas as the per-routine entry points (f3d_cell.h),
+// in this build's type.  Y: both sides in one launch when ny > nghy (the
+// sides then write disjoint rows and read none the other writes), Z likewise.
+__global__ void kf_bound_y(dmesh m, int side0, real* dens, real* vx, real* vy, real* vz, real* e, cell::BcPar bp0,
+                           cell::BcPar bp1, const real* dexp0, const real* dexp1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int side = side0 + ((int)blockIdx.y >= m.nghy);
+  const int mm = (int)blockIdx.y - (side - side0) * m.nghy, k = blockIdx.z;
+  cell::bound_y_cell(m, side, i, mm, k, dens, vx, vy, vz, e, side ? bp1 : bp0, side ? dexp1 : dexp0);
+}
+
+__global__ void kf_bound_z(dmesh m, int side0, real* dens, real* vx, real* vy, real* vz, real* e, cell::BcPar bp0,
+                           cell::BcPar bp1, const real* zdens0, const real* zdens1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int side = side0 + ((int)blockIdx.z >= m.nghz);
+  const int mm = (int)blockIdx.z - (side - side0) * m.nghz, j = blockIdx.y;
+  cell::bound_z_cell(m, side, i, mm, j, dens, vx, vy, vz, e, side ? bp1 : bp0, side ? zdens1 : zdens0);
+}
+
+// Potential (src/potential.c:51-208) on every cell
+template <int G, int DZ>
+__global__ void __launch_bounds__(WG) kf_potential(dmesh m, real* pot, cell::PlanetSet ps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int j = blockIdx.y, k = DZ ? (int)blockIdx.z : 0;
+  pot[i + j * m.pitch + k * m.stride] = cell::potential<G>(m, i, j, k, ps);
+}
+
+// init_stockholm (src/init_stockholm.c:3-126): the damping references are the
+// i = 0 column of the initial fields
+__global__ void kf_init_stockholm(dmesh m, const real* rho, const real* vx, const real* vy, const real* vz,
+                                  const real* e, real* rho0, real* vx0, real* vy0, real* vz0, real* e0,
+                                  int adiabatic) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nk = m.dimz ? m.sz : 1;
+  if (t >= m.sy * nk) return;
+  const int j = t % m.sy, k = t / m.sy;
+  const int l = j * m.pitch + k * m.stride, l2 = j + k * m.sy;
+  if (adiabatic) e0[l2] = e[l];
+  vx0[l2] = vx[l];
+  vy0[l2] = vy[l];
+  if (m.dimz) vz0[l2] = vz[l];
+  rho0[l2] = rho[l];
+}
+
+// ------------------------------------------------------------------ C ABI
+
+#define FDISPATCH(ctx, KERN, grid, block, shmem, ...)                                                  \
+  do {                                                                                                  \
+    if (m.geom == SPH) {                                                                         \
+      if (m.dimz)                                                                                \
+        hipLaunchKernelGGL((KERN<SPH, 1>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);   \
+      else                                                                                              \
+        hipLaunchKernelGGL((KERN<SPH, 0>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);   \
+    } else {                                                                                            \
+      if (m.dimz)                                                                                \
+        hipLaunchKernelGGL((KERN<CYL, 1>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);   \
+      else                                                                                              \
+        hipLaunchKernelGGL((KERN<CYL, 0>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);   \
+    }                                                                                                   \
+    F3D_LAUNCH_CHECK(ctx);                                                                              \
+  } while (0)
+
+// the same with a third template argument AD (1: adiabatic, the energy is transported)
+#define FDISPATCH_AD(ctx, KERN, grid, block, shmem, ...)                                                 \
+  do {                                                                                                    \
+    const bool ad_ = ((ctx)->ph.eos == 2);                                                                \
+    if (m.geom == SPH) {                                                                           \
+      if (m.dimz) {                                                                                \
+        if (ad_) hipLaunchKernelGGL((KERN<SPH, 1, 1>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__); \
+        else hipLaunchKernelGGL((KERN<SPH, 1, 0>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);     \
+      } else {                                                                                            \
+        if (ad_) hipLaunchKernelGGL((KERN<SPH, 0, 1>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__); \
+        else hipLaunchKernelGGL((KERN<SPH, 0, 0>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);     \
+      }                                                                                                   \
+    } else {                                                                                              \
+      if (m.dimz) {                                                                                \
+        if (ad_) hipLaunchKernelGGL((KERN<CYL, 1, 1>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__); \
+        else hipLaunchKernelGGL((KERN<CYL, 1, 0>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);     \
+      } else {                                                                                            \
+        if (ad_) hipLaunchKernelGGL((KERN<CYL, 0, 1>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__); \
+        else hipLaunchKernelGGL((KERN<CYL, 0, 0>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);     \
+      }                                                                                                   \
+    }                                                                                                     \
+    F3D_LAUNCH_CHECK(ctx);                                                                                \
+  } while (0)
+
+static cell::CflPar cfl_par_f(const f3d_hip_ctx* ctx) {
+  cell::CflPar c;
+  c.cfl = ctx->ph.cfl;
+  c.nu = ctx->ph.nu;
+  c.alpha = ctx->ph.alpha;
+  c.gamma = ctx->ph.gamma;
+  c.iso = (ctx->ph.eos != 2);
+  c.alpha_visc = (ctx->ph.visc == 2);
+  c.fargo = ctx->ph.fargo;
+  return c;
+}
+
+// The handle's fields as this build's type (the FP32 form allocates FP32
+// arrays behind the same pointers; VxMed stays FP64, Nshift int).
+struct RF {
+  real *Density, *Energy, *Vx, *Vy, *Vz, *Vx_temp, *Vy_temp, *Vz_temp;
+  real *Mpx, *Mmx, *Mpy, *Mmy, *Mpz, *Mmz, *Pot;
+  real *Density0, *Energy0, *Vx0, *Vy0, *Vz0;
+  double* VxMed;
+  int* Nshift;
+};
+
+static RF rfields(const f3d_hip_ctx* ctx) {
+  const f3d_fields& f = ctx->f;
+  auto R = [](double* p) { return reinterpret_cast<real*>(p); };
+  RF r;
+  r.Density = R(f.Density); r.Energy = R(f.Energy); r.Vx = R(f.Vx); r.Vy = R(f.Vy); r.Vz = R(f.Vz);
+  r.Vx_temp = R(f.Vx_temp); r.Vy_temp = R(f.Vy_temp); r.Vz_temp = R(f.Vz_temp);
+  r.Mpx = R(f.Mpx); r.Mmx = R(f.Mmx); r.Mpy = R(f.Mpy); r.Mmy = R(f.Mmy); r.Mpz = R(f.Mpz); r.Mmz = R(f.Mmz);
+  r.Pot = R(f.Pot);
+  r.Density0 = R(f.Density0); r.Energy0 = R(f.Energy0); r.Vx0 = R(f.Vx0); r.Vy0 = R(f.Vy0); r.Vz0 = R(f.Vz0);
+  r.VxMed = f.VxMed;
+  r.Nshift = f.Nshift;
+  return r;
+}
+
+static cell::BcPar bc_par_f(const f3d_hip_ctx* ctx, int side) {
+  cell::BcPar b;
+  b.p3d = (ctx->ph.bc == 2);
+  b.energy = ctx->ph.energy_bc && ctx->f.Energy != nullptr;
+  b.of = (real)ctx->ph.OMEGAFRAME;
+  for (int q = 0; q < 3; q++) {
+    b.dens[q] = (real)(side ? ctx->ph.ymax_dens[q] : ctx->ph.ymin_dens[q]);
+    b.vsq[q] = (real)(side ? ctx->ph.ymax_vsq[q] : ctx->ph.ymin_vsq[q]);
+    b.vfac[q] = (real)(side ? ctx->ph.ymax_vfac[q] : ctx->ph.ymin_vfac[q]);
+    b.zvfac[q] = (real)(side ? ctx->ph.zmax_vfac[q] : ctx->ph.zmin_vfac[q]);
+  }
+  return b;
+}
+
+// The ghost fill (src/boundary.c:2-23: zmin, zmax, then ymin, ymax) on the
+// given density/velocity buffers and the energy.
+static int fused_boundaries(f3d_hip_ctx* ctx, Scratch* S, real* dens, real* vx, real* vy, real* vz, real* e) {
+  const dmesh& m = S->m;
+  constexpr int BX = 64;
+  dim3 blk(BX);
+  const cell::BcPar b0 = bc_par_f(ctx, 0), b1 = bc_par_f(ctx, 1);
+  if (!ctx->ph.periodicz && m.dimz) {
+    F3D_REQUIRE(ctx, ctx->ph.bc == 2, "3-D boundaries need the spherical-disk family");
+    if (m.nz > m.nghz) {
+      hipLaunchKernelGGL(kf_bound_z, dim3((m.nx + BX - 1) / BX, m.sy, 2 * m.nghz), blk, 0, ctx->stream, m, 0, dens, vx,
+                         vy, vz, e, b0, b1, S->zmin_dens, S->zmax_dens);
+      F3D_LAUNCH_CHECK(ctx);
+    } else {
+      for (int side = 0; side < 2; side++) {
+        const cell::BcPar b = side ? b1 : b0;
+        const real* t = side ? S->zmax_dens : S->zmin_dens;
+        hipLaunchKernelGGL(kf_bound_z, dim3((m.nx + BX - 1) / BX, m.sy, m.nghz), blk, 0, ctx->stream, m, side, dens,
+                           vx, vy, vz, e, b, b, t, t);
+        F3D_LAUNCH_CHECK(ctx);
+      }
+    }
+  }
+  if (!ctx->ph.periodicy) {
+    if (m.ny > m.nghy) {
+      hipLaunchKernelGGL(kf_bound_y, dim3((m.nx + BX - 1) / BX, 2 * m.nghy, m.dimz ? m.sz : 1), blk, 0, ctx->stream,
+                         m, 0, dens, vx, vy, vz, e, b0, b1, S->ymin_dexp, S->ymax_dexp);
+      F3D_LAUNCH_CHECK(ctx);
+    } else {
+      for (int side = 0; side < 2; side++) {
+        const cell::BcPar b = side ? b1 : b0;
+        const real* t = side ? S->ymax_dexp : S->ymin_dexp;
+        hipLaunchKernelGGL(kf_bound_y, dim3((m.nx + BX - 1) / BX, m.nghy, m.dimz ? m.sz : 1), blk, 0, ctx->stream,
+                           m, side, dens, vx, vy, vz, e, b, b, t, t);
+        F3D_LAUNCH_CHECK(ctx);
+      }
+    }
+  }
+  return 0;
+}
+
+// tuning knobs of the column sweeps (rows per thread); set by the tuner
+static int g_rows_y = 4, g_rows_z = 3;
+// X-sweep kernel choice: 0 automatic, 1 ring in LDS, 2 tiles
+static int g_xmode = 0;
+
+// NewVelocity_y/z, Stockholm damping and the end-of-step ghost fill
+static int finish_step(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step) {
+  const dmesh& m = S->m;
+  const RF f = rfields(ctx);
+  real* const* XM = &S->f[FS_XM0];
+  cell::DampPar dp;
+  dp.Y_inf = (real)ctx->ph.Y_inf; dp.Y_sup = (real)ctx->ph.Y_sup;
+  dp.Z_inf = (real)ctx->ph.Z_inf; dp.Z_sup = (real)ctx->ph.Z_sup;
+  dp.y_min = (real)ctx->ph.ymin_par; dp.y_max = (real)ctx->ph.ymax_par;
+  dp.z_min = (real)ctx->ph.zmin_par; dp.z_max = (real)ctx->ph.zmax_par;
+  dp.ds = (real)ctx->ph.taudamp; dp.of = (real)ctx->ph.OMEGAFRAME; dp.of0 = (real)ctx->ph.OMEGAFRAME0;
+  dp.periodic_z = ctx->ph.periodicz;
+  dim3 gf((m.nx + BXF - 1) / BXF, m.ny, m.dimz ? m.nz : 1);
+  f3d_stage_begin(ctx, F3D_ST_FINISH);
+  FDISPATCH(ctx, kf_finish, gf, dim3(BXF), 0, (real)dt, S->f[FS_RHOX], XM[1], XM[0], XM[3], XM[2], f.Density, f.Vx,
+            f.Vy, f.Vz, f.Density0, f.Vx0, f.Vy0, f.Vz0, dp, ctx->ph.stockholm);
+  f3d_stage_end(ctx, F3D_ST_FINISH);
+  if (end_of_step) {
+    // end-of-step ghost fill (src/main.c:431)
+    f3d_stage_begin(ctx, F3D_ST_BC_END);
+    if (fused_boundaries(ctx, S, f.Density, f.Vx, f.Vy, f.Vz, f.Energy)) return -1;
+    f3d_stage_end(ctx, F3D_ST_BC_END);
+  }
+  return 0;
+}
+
+// X sweep by tiles (rings longer than the LDS ring kernel holds): VxMed of
+// each ring, then the tiles.
+static int x_sweep_tiled(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step) {
+  const dmesh& m = S->m;
+  const RF f = rfields(ctx);
+  const real of = (real)ctx->ph.OMEGAFRAME;
+  real* const* XM = &S->f[FS_XM0];
+  const int nrings = m.ny * (m.dimz ? m.nz : 1);
+  const int ad = (ctx->ph.eos == 2);
+  constexpr int T = 128;
+  F3D_REQUIRE(ctx, m.nx >= 12, "the tiled X sweep needs at least 12 cells per ring");
+  F3D_REQUIRE(ctx, (size_t)m.nx * sizeof(double) <= 160 * 1024, "ring too long for the VxMed pass");
+  f3d_stage_begin(ctx, F3D_ST_SWEEP_X);
+  if (ctx->ph.fargo)
+    FDISPATCH(ctx, kf_vmed, dim3(nrings), dim3(WG), m.nx * sizeof(double), f.Vx_temp, f.VxMed);
+  dim3 g((m.nx + T - 1) / T, nrings);
+#define XTILE(G_, DZ_, AD_)                                                                                     \
+  hipLaunchKernelGGL((kf_sweep_x_tile<G_, DZ_, AD_, T>), g, dim3(T), 0, ctx->stream, m, dt, S->f[FS_RHOY],      \
+                     f.Vx_temp, f.Mpx, f.Mmx, f.Mpy, f.Mmy, f.Mpz, f.Mmz, S->f[FS_EY], f.Energy, S->f[FS_RHOX], \
+                     f.Vx, XM[0], XM[1], XM[2], XM[3], f.VxMed, f.Nshift, of, ctx->ph.fargo)
+  if (m.geom == SPH) {
+    if (m.dimz) { if (ad) XTILE(SPH, 1, 1); else XTILE(SPH, 1, 0); }
+    else { if (ad) XTILE(SPH, 0, 1); else XTILE(SPH, 0, 0); }
+  } else {
+    if (m.dimz) { if (ad) XTILE(CYL, 1, 1); else XTILE(CYL, 1, 0); }
+    else { if (ad) XTILE(CYL, 0, 1); else XTILE(CYL, 0, 0); }
+  }
+#undef XTILE
+  F3D_LAUNCH_CHECK(ctx);
+  f3d_stage_end(ctx, F3D_ST_SWEEP_X);
+  return finish_step(ctx, S, dt, end_of_step);
+}
+
+extern "C" {
+
+int F3D_API(fused_set_xmode)(int mode) {
+  if (mode < 0 || mode > 2) return -2;
+  g_xmode = mode;
+  return 0;
+}
+
+int F3D_API(fused_set_tiling)(int rows_y, int rows_z) {
+  if (rows_y < 1 || rows_z < 1) return -2;
+  g_rows_y = rows_y;
+  g_rows_z = rows_z;
+  return 0;
+}
+
+int F3D_API(fused_cfl)(f3d_hip_ctx* ctx, double* fluid_min) {
+  Scratch* S;
+  F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
+  const dmesh& m = S->m;
+  const RF f = rfields(ctx);
+  const int nrings = m.ny * (m.dimz ? m.nz : 1);
+  const size_t shmem = (2 * m.nx + 16) * sizeof(double);
+  const int slot = S->slot;
+  f3d_stage_begin(ctx, F3D_ST_CFL);
+  FDISPATCH(ctx, kf_cfl, dim3(nrings), dim3(WG), shmem, f.Energy, f.Density, f.Vx, f.Vy, f.Vz, f.VxMed, S->result,
+            slot, cfl_par_f(ctx));
+  f3d_stage_end(ctx, F3D_ST_CFL);
+  F3D_CHECK(ctx, hipMemcpyAsync(ctx->hres, S->result + slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  F3D_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  S->slot ^= 1;
+  *fluid_min = ctx->hres[0];
+  return 0;
+}
+
+int F3D_API(fused_potential)(f3d_hip_ctx* ctx) {
+  Scratch* S;
+  F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
+  F3D_REQUIRE(ctx, ctx->ph.nplanets <= cell::kMaxPlanets, "Potential: too many planets");
+  const dmesh& m = S->m;
+  const RF f = rfields(ctx);
+  cell::PlanetSet ps;
+  ps.n = ctx->ph.nplanets;
+  ps.indirect = ctx->ph.indirectterm;
+  for (int n = 0; n < ps.n; n++) {
+    ps.x[n] = (real)ctx->ph.px[n];
+    ps.y[n] = (real)ctx->ph.py[n];
+    ps.z[n] = (real)ctx->ph.pz[n];
+    ps.mp[n] = (real)ctx->ph.pmass[n];
+    ps.smooth2[n] = (real)ctx->ph.psmooth2[n];
+  }
+  dim3 g((m.nx + BXF - 1) / BXF, m.sy, m.dimz ? m.sz : 1);
+  f3d_stage_begin(ctx, F3D_ST_POTENTIAL);
+  FDISPATCH(ctx, kf_potential, g, dim3(BXF), 0, f.Pot, ps);
+  f3d_stage_end(ctx, F3D_ST_POTENTIAL);
+  return 0;
+}
+
+int F3D_API(fused_init_stockholm)(f3d_hip_ctx* ctx) {
+  Scratch* S;
+  F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
+  const dmesh& m = S->m;
+  const RF f = rfields(ctx);
+  const int n = m.sy * (m.dimz ? m.sz : 1);
+  hipLaunchKernelGGL(kf_init_stockholm, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, m, f.Density, f.Vx, f.Vy,
+                     f.Vz, f.Energy, f.Density0, f.Vx0, f.Vy0, f.Vz0, f.Energy0, (int)(ctx->ph.eos == 2));
+  F3D_LAUNCH_CHECK(ctx);
+  return 0;
+}
+
+int F3D_API(fused_boundaries)(f3d_hip_ctx* ctx) {
+  Scratch* S;
+  F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
+  const RF f = rfields(ctx);
+  return fused_boundaries(ctx, S, f.Density, f.Vx, f.Vy, f.Vz, f.Energy);
+}
+
+int F3D_API(fused_sources)(f3d_hip_ctx* ctx, double dt, double domega) {
+  Scratch* S;
+  F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
+  const dmesh& m = S->m;
+  const RF f = rfields(ctx);
+  cell::ViscPar vp;
+  vp.model = ctx->ph.visc;
+  vp.nu = (real)ctx->ph.nu;
+  vp.alpha = (real)ctx->ph.alpha;
+  vp.gamma = (real)ctx->ph.gamma;
+  const int iso = (ctx->ph.eos != 2);
+  vp.iso = iso;
+  const int visc = ctx->ph.visc != 0 && ctx->ph.fluidtype == 1;
+  const int pot = ctx->ph.potential;
+  const real rdt = (real)dt, rdom = (real)domega;
+  dim3 ga((m.nx + BXF - 1) / BXF, m.sy - 2, m.dimz ? m.sz - 2 : 1);
+  f3d_stage_begin(ctx, F3D_ST_SRC_A);
+  FDISPATCH(ctx, kf_src_a, ga, dim3(BXF), 0, f.Density, f.Energy, f.Vx, f.Vy, f.Vz, f.Mmx, f.Mmy, f.Mmz, f.Mpx,
+            f.Mpy, f.Mpz, S->f[FS_QX], S->f[FS_QY], S->f[FS_QZ], vp, visc, rdom);
+  f3d_stage_end(ctx, F3D_ST_SRC_A);
+  // adiabatic: SubStep3 reads V_temp one row (and plane) past the active
+  // range, which the reference computes there with the same formulas
+  // (src/substep1_y.c:78-80, src/substep2_b.c:84-86), so those are included
+  const int ext = iso ? 0 : 1;
+  dim3 gb((m.nx + BXF - 1) / BXF, m.ny + ext, m.dimz ? m.nz + ext : 1);
+  f3d_stage_begin(ctx, F3D_ST_SRC_B);
+  FDISPATCH(ctx, kf_src_b, gb, dim3(BXF), 0, rdt, f.Density, f.Energy, f.Vx, f.Vy, f.Vz, f.Pot, f.Mmx, f.Mmy, f.Mmz,
+            f.Mpx, f.Mpy, f.Mpz, S->f[FS_QX], S->f[FS_QY], S->f[FS_QZ], f.Vx_temp, f.Vy_temp, f.Vz_temp,
+            S->f[FS_E2], visc, pot, iso, (real)(ctx->ph.gamma - 1.0), (real)ctx->ph.OMEGAFRAME, rdom);
+  f3d_stage_end(ctx, F3D_ST_SRC_B);
+  if (!iso && ctx->ph.fluidtype == 1) {
+    // SubStep3 (src/algogas.c:110-112)
+    dim3 g3((m.nx + BXF - 1) / BXF, m.ny, m.dimz ? m.nz : 1);
+    f3d_stage_begin(ctx, F3D_ST_SUBSTEP3);
+    FDISPATCH(ctx, kf_substep3, g3, dim3(BXF), 0, rdt, S->f[FS_E2], f.Vx_temp, f.Vy_temp, f.Vz_temp, f.Energy,
+              (real)ctx->ph.gamma);
+    f3d_stage_end(ctx, F3D_ST_SUBSTEP3);
+  }
+  // mid-step ghost fill (src/algogas.c:116-118) applied to V_temp directly
+  f3d_stage_begin(ctx, F3D_ST_BC_MID);
+  if (fused_boundaries(ctx, S, f.Density, f.Vx_temp, f.Vy_temp, f.Vz_temp, f.Energy)) return -1;
+  f3d_stage_end(ctx, F3D_ST_BC_MID);
+  return 0;
+}
+
+int F3D_API(fused_transport)(f3d_hip_ctx* ctx, double dt, int end_of_step) {
+  Scratch* S;
+  F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
+  const dmesh& m = S->m;
+  const RF f = rfields(ctx);
+  const real of = (real)ctx->ph.OMEGAFRAME;
+  const real rdt = (real)dt;
+  const int ry = g_rows_y, rz = g_rows_z;
+  real* const* XM = &S->f[FS_XM0];
+  if (m.dimz) {
+    // eight separate buffers: six Z-swept momenta, the Z-swept density and energy
+    real* const* Z = &S->f[FS_Z0];
+    dim3 gz((m.nx + BXF - 1) / BXF, m.sy - 2, (m.nz + rz - 1) / rz);
+    f3d_stage_begin(ctx, F3D_ST_SWEEP_Z);
+    FDISPATCH_AD(ctx, kf_sweep_z, gz, dim3(BXF), 0, rdt, f.Density, f.Vx_temp, f.Vy_temp, f.Vz_temp, f.Energy, Z[6],
+                 Z[0], Z[1], Z[2], Z[3], Z[4], Z[5], Z[7], rz, of);
+    f3d_stage_end(ctx, F3D_ST_SWEEP_Z);
+    dim3 gy((m.nx + BXF - 1) / BXF, (m.ny + ry - 1) / ry, m.nz);
+    f3d_stage_begin(ctx, F3D_ST_SWEEP_Y);
+    FDISPATCH_AD(ctx, kf_sweep_y3, gy, dim3(BXF), 0, rdt, Z[6], Z[0], Z[1], Z[2], Z[3], Z[4], Z[5], Z[7], f.Vy_temp,
+                 S->f[FS_RHOY], f.Mpx, f.Mmx, f.Mpy, f.Mmy, f.Mpz, f.Mmz, S->f[FS_EY], ry);
+    f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
+  } else {
+    dim3 gy((m.nx + BXF - 1) / BXF, (m.ny + ry - 1) / ry, 1);
+    f3d_stage_begin(ctx, F3D_ST_SWEEP_Y);
+    FDISPATCH_AD(ctx, kf_sweep_y2, gy, dim3(BXF), 0, rdt, f.Density, f.Vx_temp, f.Vy_temp, f.Energy, S->f[FS_RHOY],
+                 f.Mpx, f.Mmx, f.Mpy, f.Mmy, S->f[FS_EY], ry, of);
+    f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
+  }
+  // X sweep: one workgroup of at most 256 threads per active ring (up to 4
+  // cells per thread) when the ring fits in LDS, else tiles with halos
+  const int ad = (ctx->ph.eos == 2);
+  const int nq1 = (m.dimz ? 6 : 4) + ad + 1;
+  const size_t shx = ((size_t)m.nx + 8) * sizeof(double) + ((size_t)5 * nq1 * m.nx + m.nx) * sizeof(real);
+  const bool ring_fits = shx <= 160 * 1024 && m.nx <= 1024;
+  if (g_xmode == 2 || (g_xmode == 0 && !ring_fits)) return x_sweep_tiled(ctx, S, dt, end_of_step);
+  F3D_REQUIRE(ctx, ring_fits, "ring too long for the LDS X sweep");
+  const int cpt = (m.nx <= 256) ? 1 : (m.nx <= 512) ? 2 : 4;
+  const int nthreads = (((m.nx + cpt - 1) / cpt + 63) / 64) * 64;
+  dim3 gx(m.ny * (m.dimz ? m.nz : 1));
+  f3d_stage_begin(ctx, F3D_ST_SWEEP_X);
+#define XSWEEP1(G_, DZ_, AD_, CPT_)                                                                           \
+  hipLaunchKernelGGL((kf_sweep_x<G_, DZ_, AD_, CPT_>), gx, dim3(nthreads), shx, ctx->stream, m, dt,              \
+                     S->f[FS_RHOY], f.Vx_temp, f.Mpx, f.Mmx, f.Mpy, f.Mmy, f.Mpz, f.Mmz, S->f[FS_EY], f.Energy,   \
+                     S->f[FS_RHOX], f.Vx, XM[0], XM[1], XM[2], XM[3], f.VxMed, f.Nshift, of, ctx->ph.fargo)
+#define XSWEEP(G_, DZ_, CPT_) \
+  do { if (ad) XSWEEP1(G_, DZ_, 1, CPT_); else XSWEEP1(G_, DZ_, 0, CPT_); } while (0)
+#define XSWEEP_G(G_, DZ_) \
+  do { if (cpt == 1) XSWEEP(G_, DZ_, 1); else if (cpt == 2) XSWEEP(G_, DZ_, 2); else XSWEEP(G_, DZ_, 4); } while (0)
+  if (m.geom == SPH) {
+    if (m.dimz) XSWEEP_G(SPH, 1); else XSWEEP_G(SPH, 0);
+  } else {
+    if (m.dimz) XSWEEP_G(CYL, 1); else XSWEEP_G(CYL, 0);
+  }
+#undef XSWEEP_G
+#undef XSWEEP1
+#undef XSWEEP
+  F3D_LAUNCH_CHECK(ctx);
+  f3d_stage_end(ctx, F3D_ST_SWEEP_X);
+  return finish_step(ctx, S, dt, end_of_step);
+}
+
+}  // extern "C"
+
+}  // namespace fused
+}  // namespace F3D_NS

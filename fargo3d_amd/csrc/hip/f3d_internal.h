@@ -39,9 +39,12 @@ struct f3d_hip_ctx {
   double* dred = nullptr;           // device reduction workspace (n2d + 64 doubles)
   double* hres = nullptr;           // pinned host result slots (64 doubles)
   size_t n3 = 0, n2 = 0;
+  size_t meshlen = 0;                // doubles in meshbuf
   char err[512] = {0};
-  void* fused = nullptr;             // scratch of the fused stages, released with the handle
-  void (*fused_free)(void*) = nullptr;
+  // scratch of the fused stages of the FP64 [0] and FP32 [1] builds, released
+  // with the handle
+  void* fused[2] = {nullptr, nullptr};
+  void (*fused_free[2])(void*) = {nullptr, nullptr};
   // per-stage event timing of the fused kernels (f3d_hip_timing_*)
   int timing = 0;
   hipEvent_t ev[F3D_NSTAGES][2] = {};

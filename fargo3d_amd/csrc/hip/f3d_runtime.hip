@@ -72,6 +72,10 @@ int f3d_hip_create(const f3d_mesh_desc* d, f3d_hip_ctx** out) {
     return -1;
   }
   for (int a = 0; a < na; a++) *arrs[a].dst = c->meshbuf + off[a];
+  // the FP64 parts read these; in the FP64 build they are the same arrays
+  m.xmin64 = m.xmin; m.ymin64 = m.ymin; m.zmin64 = m.zmin;
+  m.ymed64 = m.ymed; m.sinzmed64 = m.sinzmed; m.sxi64 = m.sxi;
+  c->meshlen = total;
   *out = c;
   return 0;
 }
@@ -121,7 +125,8 @@ void f3d_hip_destroy(f3d_hip_ctx* c) {
   for (int s = 0; s < F3D_NSTAGES; s++)
     for (int e = 0; e < 2; e++)
       if (c->ev[s][e]) hipEventDestroy(c->ev[s][e]);
-  if (c->fused && c->fused_free) c->fused_free(c->fused);
+  for (int b = 0; b < 2; b++)
+    if (c->fused[b] && c->fused_free[b]) c->fused_free[b](c->fused[b]);
   if (c->meshbuf) hipFree(c->meshbuf);
   if (c->bctab) hipFree(c->bctab);
   if (c->dred) hipFree(c->dred);

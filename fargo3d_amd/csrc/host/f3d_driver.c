@@ -34,6 +34,9 @@ struct f3d_sim {
   double dtemp, StepTime, dt_last;
   double DiskOnPrimaryAcceleration[3];
   long alreadycalculated;
+  int prec32;    /* FP32 fields and the f3d32_fused_* stages */
+  double dt_force; /* > 0: the next step's time step (measurement of two runs on one dt sequence) */
+  size_t es;     /* bytes per element of the 3-D fields and the damping references */
   char err[512];
 };
 
@@ -50,6 +53,11 @@ struct f3d_sim {
 const char *f3d_sim_error(f3d_sim *S) { return S ? S->err : "no simulation"; }
 f3d_hip_ctx *f3d_sim_ctx(f3d_sim *S) { return S->ctx; }
 
+int f3d_sim_force_dt(f3d_sim *S, double dt) {
+  S->dt_force = dt;
+  return 0;
+}
+
 int f3d_sim_eos(f3d_sim *S) { return S->p.eos; }
 
 void f3d_sim_shape(f3d_sim *S, int out[4]) {
@@ -59,13 +67,37 @@ void f3d_sim_shape(f3d_sim *S, int out[4]) {
   out[3] = S->g.dimz;
 }
 
-static double *dalloc(size_t n, int *fail) {
+static double *dalloc_bytes(size_t bytes, int *fail) {
   void *p = NULL;
-  if (f3d_hip_malloc(&p, n * sizeof(double)) != 0) {
+  if (f3d_hip_malloc(&p, bytes) != 0) {
     *fail = 1;
     return NULL;
   }
   return (double *)p;
+}
+#define dalloc(n, fail) dalloc_bytes((n) * S->es, fail)
+
+/* host buffer of n doubles -> device array of the simulation's element type */
+static int upload(f3d_sim *S, void *dev, const double *h, size_t n) {
+  if (!S->prec32) return f3d_hip_memcpy_h2d(S->ctx, dev, h, n * sizeof(double));
+  float *t = (float *)malloc(n * sizeof(float));
+  if (!t) return -1;
+  for (size_t i = 0; i < n; i++) t[i] = (float)h[i];
+  int rc = f3d_hip_memcpy_h2d(S->ctx, dev, t, n * sizeof(float));
+  if (rc == 0) rc = f3d_hip_sync(S->ctx);
+  free(t);
+  return rc;
+}
+
+static int download(f3d_sim *S, double *h, const void *dev, size_t n) {
+  if (!S->prec32) return f3d_hip_memcpy_d2h(S->ctx, h, dev, n * sizeof(double));
+  float *t = (float *)malloc(n * sizeof(float));
+  if (!t) return -1;
+  int rc = f3d_hip_memcpy_d2h(S->ctx, t, dev, n * sizeof(float));
+  if (rc == 0) rc = f3d_hip_sync(S->ctx);
+  for (size_t i = 0; rc == 0 && i < n; i++) h[i] = t[i];
+  free(t);
+  return rc;
 }
 
 /* run parameters and host scalars as the kernels see them */
@@ -158,9 +190,20 @@ f3d_sim *f3d_sim_create(const char *parfile, const char *overrides, char *err, i
     if (err) snprintf(err, errlen, "this build runs one fluid per simulation (NFLUIDS=%d)", S->p.nfluids);
     goto fail;
   }
+  S->prec32 = (S->p.precision == 32);
+  S->es = S->prec32 ? sizeof(float) : sizeof(double);
   f3d_bctab_init(&S->bct, &S->g, &S->p);
   f3d_damp_init(&S->damp, &S->p);
   if (f3d_psys_init(&S->sys, &S->p, err, (size_t)errlen) != 0) goto fail;
+  if (S->prec32) {
+    /* the FP32 form runs the fused stages only; the disk force on planets
+       (FP64 per-routine reductions) is not part of it */
+    for (int k = 0; k < S->sys.nb; k++)
+      if (S->sys.feeldisk[k]) {
+        if (err) snprintf(err, errlen, "the FP32 form has no disk force on planets (FeelDisk)");
+        goto fail;
+      }
+  }
   S->clk.OMEGAFRAME = S->p.omegaframe;
   if (S->p.corotating) S->clk.OMEGAFRAME = f3d_psys_info(&S->sys, &S->p, F3D_PSYS_FREQUENCY);
   S->clk.OMEGAFRAME0 = S->clk.OMEGAFRAME;
@@ -198,7 +241,8 @@ f3d_sim *f3d_sim_create(const char *parfile, const char *overrides, char *err, i
     f->Total_Density = dalloc(n, &bad); f->QL = dalloc(n, &bad); f->QR = dalloc(n, &bad);
     f->Density0 = dalloc(n2, &bad); f->Energy0 = dalloc(n2, &bad); f->Vx0 = dalloc(n2, &bad);
     f->Vy0 = dalloc(n2, &bad); f->Vz0 = dalloc(n2, &bad);
-    f->Reduction2D = dalloc(n2, &bad); f->VxMed = dalloc(n2, &bad);
+    /* ring sums stay FP64 in both forms */
+    f->Reduction2D = dalloc_bytes(n2 * sizeof(double), &bad); f->VxMed = dalloc_bytes(n2 * sizeof(double), &bad);
     {
       void *pn = NULL;
       if (f3d_hip_malloc(&pn, n2 * sizeof(int)) != 0) bad = 1;
@@ -213,9 +257,11 @@ f3d_sim *f3d_sim_create(const char *parfile, const char *overrides, char *err, i
                       &f->Mpx, &f->Mmx, &f->Mpy, &f->Mmy, &f->Mpz, &f->Mmz, &f->Pot, &f->Slope, &f->DivRho,
                       &f->DensStar, &f->Qs, &f->Pressure, &f->Total_Density, &f->QL, &f->QR};
     for (size_t a = 0; a < sizeof(all) / sizeof(all[0]); a++)
-      if (*all[a]) f3d_hip_memset(S->ctx, *all[a], 0, n * sizeof(double));
-    double **all2[] = {&f->Density0, &f->Energy0, &f->Vx0, &f->Vy0, &f->Vz0, &f->Reduction2D, &f->VxMed};
-    for (size_t a = 0; a < sizeof(all2) / sizeof(all2[0]); a++) f3d_hip_memset(S->ctx, *all2[a], 0, n2 * sizeof(double));
+      if (*all[a]) f3d_hip_memset(S->ctx, *all[a], 0, n * S->es);
+    double **all2[] = {&f->Density0, &f->Energy0, &f->Vx0, &f->Vy0, &f->Vz0};
+    for (size_t a = 0; a < sizeof(all2) / sizeof(all2[0]); a++) f3d_hip_memset(S->ctx, *all2[a], 0, n2 * S->es);
+    f3d_hip_memset(S->ctx, f->Reduction2D, 0, n2 * sizeof(double));
+    f3d_hip_memset(S->ctx, f->VxMed, 0, n2 * sizeof(double));
     f3d_hip_memset(S->ctx, f->Nshift, 0, n2 * sizeof(int));
     f3d_hip_bind(S->ctx, f);
   }
@@ -224,19 +270,23 @@ f3d_sim *f3d_sim_create(const char *parfile, const char *overrides, char *err, i
     size_t n = S->g.ncell;
     double *h = (double *)calloc(5 * n, sizeof(double));
     f3d_initial_conditions(&S->g, &S->p, &S->clk, h, h + n, h + 2 * n, h + 3 * n, S->g.dimz ? h + 4 * n : NULL);
-    f3d_hip_memcpy_h2d(S->ctx, S->f.Density, h, n * sizeof(double));
-    f3d_hip_memcpy_h2d(S->ctx, S->f.Energy, h + n, n * sizeof(double));
-    f3d_hip_memcpy_h2d(S->ctx, S->f.Vx, h + 2 * n, n * sizeof(double));
-    f3d_hip_memcpy_h2d(S->ctx, S->f.Vy, h + 3 * n, n * sizeof(double));
-    if (S->g.dimz) f3d_hip_memcpy_h2d(S->ctx, S->f.Vz, h + 4 * n, n * sizeof(double));
+    int bad = upload(S, S->f.Density, h, n) | upload(S, S->f.Energy, h + n, n) |
+              upload(S, S->f.Vx, h + 2 * n, n) | upload(S, S->f.Vy, h + 3 * n, n);
+    if (S->g.dimz) bad |= upload(S, S->f.Vz, h + 4 * n, n);
     free(h);
+    if (bad) {
+      if (err) snprintf(err, errlen, "upload of the initial conditions failed");
+      goto fail;
+    }
   }
   if (f3d_sim_push_phys(S) != 0) {
     if (err) snprintf(err, errlen, "%s", S->err);
     goto fail;
   }
   /* FillGhosts then init_stockholm (src/main.c:311-316) */
-  if (f3d_boundaries(S->ctx) != 0 || (S->p.stockholm && f3d_init_stockholm(S->ctx) != 0) ||
+  if ((S->prec32 ? f3d32_fused_boundaries(S->ctx) : f3d_boundaries(S->ctx)) != 0 ||
+      (S->p.stockholm &&
+       (S->prec32 ? f3d32_fused_init_stockholm(S->ctx) : f3d_init_stockholm(S->ctx)) != 0) ||
       f3d_hip_sync(S->ctx) != 0) {
     if (err) snprintf(err, errlen, "%s", f3d_hip_last_error(S->ctx));
     goto fail;
@@ -318,9 +368,10 @@ static int compute_potential(f3d_sim *S, double dt) {
      gas indirect term, which no supported setup enables; it is evaluated when
      a planet feels the disk (AdvanceSystemFromDisk), where it is consumed. */
   CALL(S, f3d_sim_push_phys(S));
-  CALL(S, f3d_Potential(S->ctx));
+  CALL(S, S->prec32 ? f3d32_fused_potential(S->ctx) : f3d_Potential(S->ctx));
   for (int k = 0; k < S->sys.nb; k++) {
     if (!S->sys.feeldisk[k]) continue;
+    if (S->prec32) ERR(S, "the FP32 form has no disk force on planets (FeelDisk)");
     double lf[12];
     double m = S->sys.mass[k], x = S->sys.x[k], y = S->sys.y[k], z = S->sys.z[k];
     double r = sqrt(x * x + y * y + z * z), smoothing;
@@ -496,7 +547,11 @@ int f3d_sim_step(f3d_sim *S, double *dt_out) {
   if (S->dtemp >= DT) S->dtemp = 0.0;
   if (f3d_sim_push_phys(S)) return -1;
   double fluid_min;
-  if (S->arch.fused_cfl) {
+  if (S->prec32 && !(S->arch.fused_cfl && S->arch.fused_sources && S->arch.fused_transport))
+    ERR(S, "the FP32 form runs the fused stages only");
+  if (S->prec32) {
+    CALL(S, f3d32_fused_cfl(c, &fluid_min));
+  } else if (S->arch.fused_cfl) {
     CALL(S, f3d_fused_cfl(c, &fluid_min));
   } else {
     if (S->p.fargo) CALL(S, f3d_ComputeVmed(c, S->f.Vx));
@@ -505,6 +560,10 @@ int f3d_sim_step(f3d_sim *S, double *dt_out) {
   /* CflFluidsMin (src/cfl_fluids_min.c:3-22) */
   double step = 1e30;
   if (step > fluid_min) step = fluid_min;
+  if (S->dt_force > 0.0) {
+    step = S->dt_force;
+    S->dt_force = 0.0;
+  }
   S->StepTime = step;
   if (S->StepTime <= (1e-10 * sqrt(F3D_R0 * F3D_R0 * F3D_R0 / (F3D_G * F3D_MSTAR))))
     ERR(S, "Error!!!--> Null dt");
@@ -515,7 +574,7 @@ int f3d_sim_step(f3d_sim *S, double *dt_out) {
      path skips it when no planet feels the disk */
   int feel = 0;
   for (int k = 0; k < S->sys.nb; k++) feel |= S->sys.feeldisk[k];
-  if (!S->arch.fused_sources || (S->p.potential && feel)) {
+  if (!S->prec32 && (!S->arch.fused_sources || (S->p.potential && feel))) {
     CALL(S, f3d_Reset_field(c, S->f.Total_Density));
     CALL(S, f3d_ComputeTotalDensity(c));
   }
@@ -524,13 +583,14 @@ int f3d_sim_step(f3d_sim *S, double *dt_out) {
       if (compute_potential(S, dt)) return -1;
     }
     if (f3d_sim_push_phys(S)) return -1;
-    CALL(S, f3d_fused_sources(c, dt, (S->p.potential && S->p.corotating) ? S->clk.Domega : 0.0));
+    const double dom = (S->p.potential && S->p.corotating) ? S->clk.Domega : 0.0;
+    CALL(S, S->prec32 ? f3d32_fused_sources(c, dt, dom) : f3d_fused_sources(c, dt, dom));
   } else {
     if (sources_routines(S, dt)) return -1;
   }
   if (S->arch.fused_transport) {
     /* transport, NewVelocity, Stockholm damping and the ghost fill */
-    CALL(S, f3d_fused_transport(c, dt, 1));
+    CALL(S, S->prec32 ? f3d32_fused_transport(c, dt, 1) : f3d_fused_transport(c, dt, 1));
     S->clk.PhysicalTime += dt;
     S->clk.Timestepcount++;
   } else {
@@ -553,6 +613,8 @@ int f3d_sim_run(f3d_sim *S, int nsteps) {
 }
 
 /* ---- field access ---- */
+/* is2d: 1 for YZ arrays; the element type is the simulation's except for the
+   FP64 ring sums (Reduction2D, VxMed) */
 static double *field_ptr(f3d_sim *S, const char *name, int *is2d) {
   f3d_fields *f = &S->f;
   struct { const char *n; double *p; int d2; } t[] = {
@@ -598,12 +660,15 @@ long f3d_sim_field_size(f3d_sim *S, const char *name) {
   return d2 ? (long)S->g.n2d : (long)S->g.ncell;
 }
 
+static int is_fp64_array(f3d_sim *S, const double *p) { return p == S->f.Reduction2D || p == S->f.VxMed; }
+
 int f3d_sim_get_field(f3d_sim *S, const char *name, double *host) {
   int d2 = 0;
   double *p = field_ptr(S, name, &d2);
   if (!p) ERR(S, "no field %s", name);
   size_t n = d2 ? S->g.n2d : S->g.ncell;
-  CALL(S, f3d_hip_memcpy_d2h(S->ctx, host, p, n * sizeof(double)));
+  if (is_fp64_array(S, p)) CALL(S, f3d_hip_memcpy_d2h(S->ctx, host, p, n * sizeof(double)));
+  else CALL(S, download(S, host, p, n));
   return 0;
 }
 
@@ -612,7 +677,8 @@ int f3d_sim_set_field(f3d_sim *S, const char *name, const double *host) {
   double *p = field_ptr(S, name, &d2);
   if (!p) ERR(S, "no field %s", name);
   size_t n = d2 ? S->g.n2d : S->g.ncell;
-  CALL(S, f3d_hip_memcpy_h2d(S->ctx, p, host, n * sizeof(double)));
+  if (is_fp64_array(S, p)) CALL(S, f3d_hip_memcpy_h2d(S->ctx, p, host, n * sizeof(double)));
+  else CALL(S, upload(S, p, host, n));
   return 0;
 }
 
@@ -647,7 +713,7 @@ int f3d_sim_write_field(f3d_sim *S, const char *name, const char *path) {
   if (!p || d2) ERR(S, "no 3-D field %s", name);
   size_t n = S->g.ncell;
   double *h = (double *)malloc(n * sizeof(double));
-  if (f3d_hip_memcpy_d2h(S->ctx, h, p, n * sizeof(double)) != 0) {
+  if (download(S, h, p, n) != 0) {
     free(h);
     ERR(S, "copy of %s failed", name);
   }

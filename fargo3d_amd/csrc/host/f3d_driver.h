@@ -26,6 +26,10 @@ const char *f3d_sim_error(f3d_sim *S);
 
 /* One pass of the DT loop body; *dt_out receives the time step used. */
 int f3d_sim_step(f3d_sim *S, double *dt_out);
+/* Use dt (> 0) as the next step's CFL time step instead of the computed one
+ * (no counterpart in the reference: lets two runs, e.g. FP64 and FP32, share
+ * one dt sequence so that their difference measures arithmetic alone). */
+int f3d_sim_force_dt(f3d_sim *S, double dt);
 int f3d_sim_run(f3d_sim *S, int nsteps);
 
 /* func_arch table: group names as std/func_arch.cfg; modes "GPU"/"HIP"

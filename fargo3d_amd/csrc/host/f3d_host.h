@@ -78,6 +78,7 @@ typedef struct f3d_params {
   int fargo;                   /* 1 = FARGO orbital advection, 0 = -DSTANDARD */
   int bc;                      /* boundary family */
   int energy_bc;               /* 1: Energy ghosts SYMMETRIC (|a|a|) on the Y (and Z) sides */
+  int precision;               /* 64: FP64 fields (the drop-in); 32: FP32 fields, FP64 sums/residual/CFL */
   int nfluids;
   int ic_keplerian;            /* 1: v_phi = r^-1/2 - Omega_f r, v_r = 0, noise on density only */
   uint64_t seed;               /* srand48 seed for the noise (0 = glibc default state) */

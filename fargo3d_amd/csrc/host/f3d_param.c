@@ -140,6 +140,7 @@ void f3d_params_defaults(f3d_params *p) {
   p->fargo = 1;
   p->bc = F3D_BC_FARGO2D;
   p->energy_bc = 0;
+  p->precision = 64;
   p->nfluids = 1;
   p->ic_keplerian = 0;
   p->seed = 0;
@@ -210,6 +211,10 @@ int f3d_params_set(f3d_params *p, const char *name_in, const char *value) {
   if (strcmp(name, "STOCKHOLM") == 0) { p->stockholm = (toupper((unsigned char)value[0]) == 'Y'); return 0; }
   if (strcmp(name, "FARGOADVECTION") == 0) { p->fargo = (toupper((unsigned char)value[0]) == 'Y'); return 0; }
   if (strcmp(name, "POTENTIALTERM") == 0) { p->potential = (toupper((unsigned char)value[0]) == 'Y'); return 0; }
+  if (strcmp(name, "PRECISION") == 0) {
+    p->precision = (strstr(value, "32") != NULL) ? 32 : 64;
+    return 0;
+  }
   if (strcmp(name, "ENERGYBOUNDARY") == 0) { p->energy_bc = (toupper((unsigned char)value[0]) == 'S'); return 0; }
   if (strcmp(name, "SEED") == 0) { p->seed = strtoull(value, NULL, 10); return 0; }
   for (int i = 0; IGNORED[i]; i++)

@@ -206,6 +206,26 @@ int f3d_fused_set_tiling(int rows_y, int rows_z);
  * 128 cells with halos), 1 ring, 2 tiles; results do not depend on it.
  * Returns -2 for other values. */
 int f3d_fused_set_xmode(int mode);
+/* Potential, the ghost fill and the damping references on the fused stages'
+ * scratch-free kernels (the same formulas as f3d_Potential, f3d_boundaries,
+ * f3d_init_stockholm) */
+int f3d_fused_potential(f3d_hip_ctx *ctx);
+int f3d_fused_boundaries(f3d_hip_ctx *ctx);
+int f3d_fused_init_stockholm(f3d_hip_ctx *ctx);
+
+/* ---- reduced-precision form (no counterpart in the reference): the fused
+ * stages on FP32 fields (every 3-D field and the damping references bound to
+ * the handle are FP32 arrays behind the same pointers), with FP32 stencil
+ * arithmetic and FP64 kept for the ring sums (VxMed stays an FP64 array), the
+ * residual/shift split (Nshift stays int) and the CFL time step. ---- */
+int f3d32_fused_cfl(f3d_hip_ctx *ctx, double *fluid_min);
+int f3d32_fused_sources(f3d_hip_ctx *ctx, double dt, double domega);
+int f3d32_fused_transport(f3d_hip_ctx *ctx, double dt, int end_of_step);
+int f3d32_fused_potential(f3d_hip_ctx *ctx);
+int f3d32_fused_boundaries(f3d_hip_ctx *ctx);
+int f3d32_fused_init_stockholm(f3d_hip_ctx *ctx);
+int f3d32_fused_set_tiling(int rows_y, int rows_z);
+int f3d32_fused_set_xmode(int mode);
 
 #ifdef __cplusplus
 }
