@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-__all__ = ["Simulation", "lib", "library_path", "build", "HipError", "SCALAR_KEYS"]
+__all__ = ["Simulation", "Batch", "lib", "library_path", "build", "HipError", "SCALAR_KEYS"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "lib", "libfargo3d_amd.so")
@@ -83,14 +83,18 @@ def lib():
 class Simulation:
     """One device-resident run of the hydro step (a FARGO3D run on MI355X)."""
 
-    def __init__(self, parfile, overrides="", arch=None):
+    def __init__(self, parfile, overrides="", arch=None, _handle=None):
         L = lib()
         if L.f3d_hip_device_count() < 1:
             raise HipError("fargo3d_amd: no HIP device is visible")
-        err = ctypes.create_string_buffer(512)
-        self._s = L.f3d_sim_create(parfile.encode(), overrides.encode(), err, 512)
-        if not self._s:
-            raise HipError("fargo3d_amd: " + err.value.decode())
+        self._owned = _handle is None
+        if _handle is None:
+            err = ctypes.create_string_buffer(512)
+            self._s = L.f3d_sim_create(parfile.encode(), overrides.encode(), err, 512)
+            if not self._s:
+                raise HipError("fargo3d_amd: " + err.value.decode())
+        else:
+            self._s = _handle
         self.L = L
         shp = (ctypes.c_int * 4)()
         L.f3d_sim_shape(self._s, shp)
@@ -109,7 +113,8 @@ class Simulation:
 
     def close(self):
         if getattr(self, "_s", None):
-            self.L.f3d_sim_destroy(self._s)
+            if self._owned:
+                self.L.f3d_sim_destroy(self._s)
             self._s = None
 
     def __del__(self):
@@ -219,3 +224,73 @@ class Simulation:
         if rc != 0:
             raise HipError("%s: %s" % (routine, self.L.f3d_hip_last_error(self.ctx).decode()))
         return rc
+
+
+class Batch:
+    """Many independent 2-D isothermal disks advanced together in the FP32
+    form: each stage of the step is one launch over every instance, and the
+    step (time steps included) is one hipGraph replay."""
+
+    def __init__(self, specs):
+        L = lib()
+        vp, cp, i = ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int
+        L.f3d_batch_create.restype = vp
+        L.f3d_batch_create.argtypes = [i, ctypes.POINTER(cp), ctypes.POINTER(cp), cp, i]
+        L.f3d_batch_destroy.argtypes = [vp]
+        L.f3d_batch_error.restype = cp
+        L.f3d_batch_error.argtypes = [vp]
+        L.f3d_batch_instance.restype = vp
+        L.f3d_batch_instance.argtypes = [vp, i]
+        L.f3d_batch_step.argtypes = [vp, i]
+        L.f3d_batch_sync.argtypes = [vp]
+        L.f3d_batch_clock.argtypes = [vp, i, vp]
+        L.f3d_batch_set_tiling.argtypes = [vp, i, i]
+        L.f3d_batch_tune.argtypes = [vp, i, vp, vp]
+        self.L = L
+        n = len(specs)
+        pars = (cp * n)(*[p.encode() for p, _ in specs])
+        ovs = (cp * n)(*[o.encode() for _, o in specs])
+        err = ctypes.create_string_buffer(512)
+        self._b = L.f3d_batch_create(n, pars, ovs, err, 512)
+        if not self._b:
+            raise HipError("fargo3d_amd: " + err.value.decode())
+        self.n = n
+        self.instances = [Simulation(None, _handle=L.f3d_batch_instance(self._b, k)) for k in range(n)]
+        self.cells = [s.nx * s.ny for s in self.instances]
+
+    def _check(self, rc):
+        if rc != 0:
+            raise HipError(self.L.f3d_batch_error(self._b).decode())
+
+    def step(self, nsteps=1):
+        self._check(self.L.f3d_batch_step(self._b, int(nsteps)))
+
+    def sync(self):
+        self._check(self.L.f3d_batch_sync(self._b))
+
+    def clock(self, k):
+        out = (ctypes.c_double * 3)()
+        self._check(self.L.f3d_batch_clock(self._b, k, out))
+        return {"dt": out[0], "dtemp": out[1], "PhysicalTime": out[2]}
+
+    def set_tiling(self, rows_y, tile):
+        self._check(self.L.f3d_batch_set_tiling(self._b, int(rows_y), int(tile)))
+
+    def tune(self, reps=5):
+        r, t = ctypes.c_int(), ctypes.c_int()
+        self._check(self.L.f3d_batch_tune(self._b, int(reps), ctypes.byref(r), ctypes.byref(t)))
+        return r.value, t.value
+
+    def close(self):
+        if getattr(self, "_b", None):
+            for s in self.instances:
+                s._s = None
+            self.L.f3d_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -1,0 +1,438 @@
+// f3d_batch_body.h -- the batched form: many independent 2-D isothermal disks
+// (uneven grids, per-instance parameters and time step) advanced together.
+// Every stage of the step is one launch over all instances: a workgroup finds
+// its instance in a prefix table of per-instance workgroup counts and runs
+// the same device function the single-disk kernels run.  The time step of
+// each instance is computed and clamped on the device (src/main.c:383-395),
+// so a step needs no host round trip and is captured once in a hipGraph.
+// Included by f3d_fused_body.h in the FP32 build only.
+#include "../host/f3d_driver.h"
+
+namespace F3D_NS {
+namespace fused {
+
+// one instance as the batched kernels see it (device memory)
+struct BInst {
+  dmesh m;
+  real *rho, *cs, *vx, *vy, *vxt, *vyt, *mpx, *mmx, *mpy, *mmy, *pot, *rho0, *vx0, *vy0;
+  real *rhoy, *rhox, *qx, *qy, *e2, *ey, *xm0, *xm1, *xm2, *xm3;
+  double* vxmed;
+  int* nshift;
+  const real *ydexp0, *ydexp1;
+  cell::CflPar cp;
+  cell::ViscPar vp;
+  cell::DampPar dp;
+  cell::BcPar b0, b1;
+  real of;
+  int visc, potential, stockholm, fargo;
+  double* clk;              // [0] dt, [1] dtemp, [2] PhysicalTime, [3] DT, [4] null-dt flag
+  unsigned long long* res;  // the CFL atomic MIN (two slots; kd_cfl resets the other)
+};
+
+// the stages, each with its own per-instance workgroup counts
+enum { BK_CFL = 0, BK_SRCA, BK_SRCB, BK_BND, BK_SWY, BK_VMED, BK_XT, BK_FIN, BK_N };
+
+__device__ __forceinline__ int find_inst(const int* off, int n, int b) {
+  int lo = 0, hi = n;  // off[lo] <= b < off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// local 2-D workgroup index of a grid gx wide
+__device__ __forceinline__ dim3 local2(int lb, int gx) { return dim3(lb % gx, lb / gx, 0); }
+
+__global__ void __launch_bounds__(WG) kb_cfl(const BInst* I, const int* off, int n) {
+  const int ii = find_inst(off, n, blockIdx.x);
+  const BInst& in = I[ii];
+  kd_cfl<CYL, 0>(dim3(blockIdx.x - off[ii]), in.m, in.cs, in.rho, in.vx, in.vy, in.vy, in.vxmed, in.res, 0, in.cp);
+}
+
+// CflFluidsMin and the output-time clamp (src/cfl_fluids_min.c:3-22,
+// src/main.c:383-395, 423), one thread per instance; clears the MIN slot.
+__global__ void kb_dt(const BInst* I, int n) {
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ii >= n) return;
+  const BInst& in = I[ii];
+  double* c = in.clk;
+  const double DT = c[3];
+  if (c[1] >= DT) c[1] = 0.0;
+  const double fluid_min = __longlong_as_double((long long)in.res[0]);
+  in.res[0] = kInfBits;
+  double step = 1e30;
+  if (step > fluid_min) step = fluid_min;
+  if (step <= 1e-10) c[4] = 1.0;
+  double dt = step;
+  c[1] += dt;
+  if (c[1] > DT) dt = DT - (c[1] - dt);
+  c[0] = dt;
+  c[2] += dt;
+}
+
+__global__ void __launch_bounds__(BXF) kb_src_a(const BInst* I, const int* off, int n) {
+  const int ii = find_inst(off, n, blockIdx.x);
+  const BInst& in = I[ii];
+  const int gx = (in.m.nx + BXF - 1) / BXF;
+  kd_src_a<CYL, 0>(local2(blockIdx.x - off[ii], gx), in.m, in.rho, in.cs, in.vx, in.vy, nullptr, in.mmx, in.mmy,
+                   nullptr, in.mpx, in.mpy, nullptr, in.qx, in.qy, nullptr, in.vp, in.visc, (real)0);
+}
+
+__global__ void __launch_bounds__(BXF) kb_src_b(const BInst* I, const int* off, int n) {
+  const int ii = find_inst(off, n, blockIdx.x);
+  const BInst& in = I[ii];
+  const int gx = (in.m.nx + BXF - 1) / BXF;
+  kd_src_b<CYL, 0>(local2(blockIdx.x - off[ii], gx), in.m, (real)in.clk[0], in.rho, in.cs, in.vx, in.vy, nullptr,
+                   in.pot, in.mmx, in.mmy, nullptr, in.mpx, in.mpy, nullptr, in.qx, in.qy, nullptr, in.vxt, in.vyt,
+                   nullptr, in.e2, in.visc, in.potential, 1, (real)0, in.of, (real)0);
+}
+
+// the ghost fill in Y (both sides in one launch): mid = 1 fills V_temp
+__global__ void __launch_bounds__(BXF) kb_bound(const BInst* I, const int* off, int n, int mid) {
+  const int ii = find_inst(off, n, blockIdx.x);
+  const BInst& in = I[ii];
+  const int gx = (in.m.nx + BXF - 1) / BXF;
+  kd_bound_y(local2(blockIdx.x - off[ii], gx), in.m, 0, in.rho, mid ? in.vxt : in.vx, mid ? in.vyt : in.vy, nullptr,
+             in.cs, in.b0, in.b1, in.ydexp0, in.ydexp1);
+}
+
+__global__ void __launch_bounds__(BXF) kb_sweep_y(const BInst* I, const int* off, int n, int rows_per) {
+  const int ii = find_inst(off, n, blockIdx.x);
+  const BInst& in = I[ii];
+  const int gx = (in.m.nx + BXF - 1) / BXF;
+  kd_sweep_y2<CYL, 0, 0>(local2(blockIdx.x - off[ii], gx), in.m, (real)in.clk[0], in.rho, in.vxt, in.vyt, in.cs,
+                         in.rhoy, in.mpx, in.mmx, in.mpy, in.mmy, in.ey, rows_per, in.of);
+}
+
+__global__ void __launch_bounds__(WG) kb_vmed(const BInst* I, const int* off, int n) {
+  const int ii = find_inst(off, n, blockIdx.x);
+  const BInst& in = I[ii];
+  kd_vmed<CYL, 0>(dim3(blockIdx.x - off[ii]), in.m, in.vxt, in.vxmed);
+}
+
+template <int T>
+__global__ void __launch_bounds__(T) kb_sweep_x(const BInst* I, const int* off, int n) {
+  const int ii = find_inst(off, n, blockIdx.x);
+  const BInst& in = I[ii];
+  const int tx = (in.m.nx + T - 1) / T;
+  kd_sweep_x_tile<CYL, 0, 0, T>(local2(blockIdx.x - off[ii], tx), in.m, in.clk[0], in.rhoy, in.vxt, in.mpx, in.mmx,
+                                in.mpy, in.mmy, nullptr, nullptr, in.ey, in.e2, in.rhox, in.vx, in.xm0, in.xm1,
+                                in.xm2, in.xm3, in.vxmed, in.nshift, in.of, in.fargo);
+}
+
+__global__ void __launch_bounds__(BXF) kb_finish(const BInst* I, const int* off, int n) {
+  const int ii = find_inst(off, n, blockIdx.x);
+  const BInst& in = I[ii];
+  const int gx = (in.m.nx + BXF - 1) / BXF;
+  kd_finish<CYL, 0>(local2(blockIdx.x - off[ii], gx), in.m, (real)in.clk[0], in.rhox, in.xm1, in.xm0, in.xm3, in.xm2,
+                    in.rho, in.vx, in.vy, nullptr, in.rho0, in.vx0, in.vy0, nullptr, in.dp, in.stockholm);
+}
+
+}  // namespace fused
+}  // namespace F3D_NS
+
+// ------------------------------------------------------------------ host side
+struct f3d_batch {
+  int n = 0;
+  f3d_sim** sims = nullptr;
+  hipStream_t stream = nullptr;
+  F3D_NS::fused::BInst* dI = nullptr;
+  double* dclk = nullptr;
+  unsigned long long* dres = nullptr;
+  int* doff[F3D_NS::fused::BK_N] = {};
+  int total[F3D_NS::fused::BK_N] = {};
+  int* dswy = nullptr;  // workgroup offsets of the Y sweep for the chosen rows per thread
+  int swy_total = 0;
+  int* dxt = nullptr;   // and of the X tiles for the chosen tile
+  int xt_total = 0;
+  int nxmax = 0;
+  int rows_y = 4, tile = 128;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<double*> save;  // state copies used by the tuner
+  char err[512] = {0};
+};
+
+namespace F3D_NS {
+namespace fused {
+
+#define BCHECK(B, expr)                                                                           \
+  do {                                                                                            \
+    hipError_t e_ = (expr);                                                                       \
+    if (e_ != hipSuccess) {                                                                       \
+      snprintf((B)->err, sizeof((B)->err), "%s: %s", #expr, hipGetErrorString(e_));               \
+      return -1;                                                                                  \
+    }                                                                                             \
+  } while (0)
+
+static int batch_offsets(f3d_batch* B, const std::vector<int>& counts, int** dst, int* total) {
+  std::vector<int> off(B->n + 1, 0);
+  for (int i = 0; i < B->n; i++) off[i + 1] = off[i] + counts[i];
+  if (*dst) (void)hipFree(*dst);
+  *dst = nullptr;
+  BCHECK(B, hipMalloc(dst, off.size() * sizeof(int)));
+  BCHECK(B, hipMemcpy(*dst, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+  *total = off[B->n];
+  return 0;
+}
+
+static int batch_layout(f3d_batch* B) {
+  std::vector<std::vector<int>> c(BK_N, std::vector<int>(B->n));
+  std::vector<int> swy(B->n), xt(B->n);
+  for (int i = 0; i < B->n; i++) {
+    const f3d::dmesh& m = f3d_sim_ctx(B->sims[i])->m;
+    const int gx = (m.nx + BXF - 1) / BXF;
+    c[BK_CFL][i] = m.ny;
+    c[BK_SRCA][i] = gx * (m.sy - 2);
+    c[BK_SRCB][i] = gx * m.ny;
+    c[BK_BND][i] = gx * 2 * m.nghy;
+    c[BK_VMED][i] = m.ny;
+    c[BK_FIN][i] = gx * m.ny;
+    swy[i] = gx * ((m.ny + B->rows_y - 1) / B->rows_y);
+    xt[i] = ((m.nx + B->tile - 1) / B->tile) * m.ny;
+  }
+  for (int k = 0; k < BK_N; k++)
+    if (k != BK_SWY && k != BK_XT && batch_offsets(B, c[k], &B->doff[k], &B->total[k])) return -1;
+  if (batch_offsets(B, swy, &B->dswy, &B->swy_total)) return -1;
+  if (batch_offsets(B, xt, &B->dxt, &B->xt_total)) return -1;
+  return 0;
+}
+
+// one step: every stage one launch over all instances
+static void batch_launch(f3d_batch* B) {
+  const int n = B->n;
+  hipStream_t s = B->stream;
+  const size_t cfl_lds = (2 * (size_t)B->nxmax + 16) * sizeof(double);
+  hipLaunchKernelGGL(kb_cfl, dim3(B->total[BK_CFL]), dim3(WG), cfl_lds, s, B->dI, B->doff[BK_CFL], n);
+  hipLaunchKernelGGL(kb_dt, dim3((n + 63) / 64), dim3(64), 0, s, B->dI, n);
+  hipLaunchKernelGGL(kb_src_a, dim3(B->total[BK_SRCA]), dim3(BXF), 0, s, B->dI, B->doff[BK_SRCA], n);
+  hipLaunchKernelGGL(kb_src_b, dim3(B->total[BK_SRCB]), dim3(BXF), 0, s, B->dI, B->doff[BK_SRCB], n);
+  hipLaunchKernelGGL(kb_bound, dim3(B->total[BK_BND]), dim3(BXF), 0, s, B->dI, B->doff[BK_BND], n, 1);
+  hipLaunchKernelGGL(kb_sweep_y, dim3(B->swy_total), dim3(BXF), 0, s, B->dI, B->dswy, n, B->rows_y);
+  hipLaunchKernelGGL(kb_vmed, dim3(B->total[BK_VMED]), dim3(WG), (size_t)B->nxmax * sizeof(double), s, B->dI,
+                     B->doff[BK_VMED], n);
+  if (B->tile == 64) hipLaunchKernelGGL(kb_sweep_x<64>, dim3(B->xt_total), dim3(64), 0, s, B->dI, B->dxt, n);
+  else if (B->tile == 256) hipLaunchKernelGGL(kb_sweep_x<256>, dim3(B->xt_total), dim3(256), 0, s, B->dI, B->dxt, n);
+  else hipLaunchKernelGGL(kb_sweep_x<128>, dim3(B->xt_total), dim3(128), 0, s, B->dI, B->dxt, n);
+  hipLaunchKernelGGL(kb_finish, dim3(B->total[BK_FIN]), dim3(BXF), 0, s, B->dI, B->doff[BK_FIN], n);
+  hipLaunchKernelGGL(kb_bound, dim3(B->total[BK_BND]), dim3(BXF), 0, s, B->dI, B->doff[BK_BND], n, 0);
+}
+
+static int batch_capture(f3d_batch* B) {
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  B->exec = nullptr;
+  B->graph = nullptr;
+  BCHECK(B, hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+  batch_launch(B);
+  BCHECK(B, hipStreamEndCapture(B->stream, &B->graph));
+  BCHECK(B, hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+// copies of the evolving state (rho, vx, vy and the clocks) for the tuner
+static int batch_state(f3d_batch* B, int restore) {
+  size_t k = 0;
+  for (int i = 0; i < B->n; i++) {
+    f3d_hip_ctx* c = f3d_sim_ctx(B->sims[i]);
+    double* f[3] = {c->f.Density, c->f.Vx, c->f.Vy};
+    const size_t bytes = c->n3 * sizeof(real);
+    for (int a = 0; a < 3; a++, k++) {
+      if (!restore && k >= B->save.size()) {
+        double* p = nullptr;
+        BCHECK(B, hipMalloc(&p, bytes));
+        B->save.push_back(p);
+      }
+      if (restore) BCHECK(B, hipMemcpyAsync(f[a], B->save[k], bytes, hipMemcpyDeviceToDevice, B->stream));
+      else BCHECK(B, hipMemcpyAsync(B->save[k], f[a], bytes, hipMemcpyDeviceToDevice, B->stream));
+    }
+  }
+  const size_t cb = (size_t)B->n * 5 * sizeof(double);
+  if (!restore && k >= B->save.size()) {
+    double* p = nullptr;
+    BCHECK(B, hipMalloc(&p, cb));
+    B->save.push_back(p);
+  }
+  if (restore) BCHECK(B, hipMemcpyAsync(B->dclk, B->save[k], cb, hipMemcpyDeviceToDevice, B->stream));
+  else BCHECK(B, hipMemcpyAsync(B->save[k], B->dclk, cb, hipMemcpyDeviceToDevice, B->stream));
+  BCHECK(B, hipStreamSynchronize(B->stream));
+  return 0;
+}
+
+}  // namespace fused
+}  // namespace F3D_NS
+
+extern "C" {
+
+void f3d_batch_destroy(f3d_batch* B) {
+  if (!B) return;
+  if (B->stream) (void)hipStreamSynchronize(B->stream);
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  for (int k = 0; k < F3D_NS::fused::BK_N; k++)
+    if (B->doff[k]) (void)hipFree(B->doff[k]);
+  if (B->dswy) (void)hipFree(B->dswy);
+  if (B->dxt) (void)hipFree(B->dxt);
+  for (double* p : B->save) (void)hipFree(p);
+  if (B->dI) (void)hipFree(B->dI);
+  if (B->dclk) (void)hipFree(B->dclk);
+  if (B->dres) (void)hipFree(B->dres);
+  if (B->sims)
+    for (int i = 0; i < B->n; i++)
+      if (B->sims[i]) f3d_sim_destroy(B->sims[i]);
+  free(B->sims);
+  if (B->stream) (void)hipStreamDestroy(B->stream);
+  delete B;
+}
+
+f3d_batch* f3d_batch_create(int n, const char* const* parfiles, const char* const* overrides, char* err,
+                            int errlen) {
+  using namespace F3D_NS;
+  using namespace F3D_NS::fused;
+  f3d_batch* B = new f3d_batch();
+  auto fail = [&](const char* msg) -> f3d_batch* {
+    if (err) snprintf(err, errlen, "%s", msg);
+    f3d_batch_destroy(B);
+    return nullptr;
+  };
+  if (n < 1) return fail("f3d_batch_create: no instances");
+  B->n = n;
+  B->sims = (f3d_sim**)calloc(n, sizeof(f3d_sim*));
+  std::vector<BInst> h(n);
+  std::vector<double> clk(5 * (size_t)n, 0.0);
+  if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) return fail("stream creation failed");
+  if (hipMalloc(&B->dclk, clk.size() * sizeof(double)) != hipSuccess ||
+      hipMalloc(&B->dres, 2 * (size_t)n * sizeof(unsigned long long)) != hipSuccess)
+    return fail("device allocation failed");
+  for (int i = 0; i < n; i++) {
+    char ov[1024];
+    snprintf(ov, sizeof(ov), "%s Precision=32", overrides && overrides[i] ? overrides[i] : "");
+    char e2[512] = {0};
+    B->sims[i] = f3d_sim_create(parfiles[i], ov, e2, sizeof(e2));
+    if (!B->sims[i]) return fail(e2);
+    f3d_hip_ctx* c = f3d_sim_ctx(B->sims[i]);
+    if (c->m.dimz || c->m.geom != CYL || c->ph.eos != 1 || c->ph.nplanets > cell::kMaxPlanets ||
+        c->m.ny <= c->m.nghy || c->m.nx < 12)
+      return fail("the batched form runs 2-D cylindrical isothermal disks (ny > 3, nx >= 12)");
+    Scratch* S;
+    if (scratch_of(c, &S) || F3D_API(fused_potential)(c) || hipStreamSynchronize(c->stream) != hipSuccess)
+      return fail("instance set-up failed");
+    const RF f = rfields(c);
+    BInst& b = h[i];
+    b.m = S->m;
+    b.rho = f.Density; b.cs = f.Energy; b.vx = f.Vx; b.vy = f.Vy; b.vxt = f.Vx_temp; b.vyt = f.Vy_temp;
+    b.mpx = f.Mpx; b.mmx = f.Mmx; b.mpy = f.Mpy; b.mmy = f.Mmy; b.pot = f.Pot;
+    b.rho0 = f.Density0; b.vx0 = f.Vx0; b.vy0 = f.Vy0;
+    b.rhoy = S->f[FS_RHOY]; b.rhox = S->f[FS_RHOX]; b.qx = S->f[FS_QX]; b.qy = S->f[FS_QY];
+    b.e2 = S->f[FS_E2]; b.ey = S->f[FS_EY];
+    b.xm0 = S->f[FS_XM0]; b.xm1 = S->f[FS_XM0 + 1]; b.xm2 = S->f[FS_XM0 + 2]; b.xm3 = S->f[FS_XM0 + 3];
+    b.vxmed = f.VxMed; b.nshift = f.Nshift;
+    b.ydexp0 = S->ymin_dexp; b.ydexp1 = S->ymax_dexp;
+    b.cp = cfl_par_f(c);
+    b.vp.model = c->ph.visc; b.vp.nu = (real)c->ph.nu; b.vp.alpha = (real)c->ph.alpha;
+    b.vp.gamma = (real)c->ph.gamma; b.vp.iso = 1;
+    b.dp.Y_inf = (real)c->ph.Y_inf; b.dp.Y_sup = (real)c->ph.Y_sup; b.dp.Z_inf = (real)c->ph.Z_inf;
+    b.dp.Z_sup = (real)c->ph.Z_sup; b.dp.y_min = (real)c->ph.ymin_par; b.dp.y_max = (real)c->ph.ymax_par;
+    b.dp.z_min = (real)c->ph.zmin_par; b.dp.z_max = (real)c->ph.zmax_par; b.dp.ds = (real)c->ph.taudamp;
+    b.dp.of = (real)c->ph.OMEGAFRAME; b.dp.of0 = (real)c->ph.OMEGAFRAME0; b.dp.periodic_z = c->ph.periodicz;
+    b.b0 = bc_par_f(c, 0);
+    b.b1 = bc_par_f(c, 1);
+    b.of = (real)c->ph.OMEGAFRAME;
+    b.visc = c->ph.visc != 0 && c->ph.fluidtype == 1;
+    b.potential = c->ph.potential;
+    b.stockholm = c->ph.stockholm;
+    b.fargo = c->ph.fargo;
+    b.clk = B->dclk + 5 * (size_t)i;
+    b.res = B->dres + 2 * (size_t)i;
+    clk[5 * (size_t)i + 3] = f3d_sim_dt_out(B->sims[i]);
+    if (c->m.nx > B->nxmax) B->nxmax = c->m.nx;
+  }
+  std::vector<unsigned long long> res(2 * (size_t)n, kInfBits);
+  if (hipMalloc(&B->dI, n * sizeof(BInst)) != hipSuccess ||
+      hipMemcpy(B->dI, h.data(), n * sizeof(BInst), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(B->dclk, clk.data(), clk.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(B->dres, res.data(), res.size() * sizeof(unsigned long long), hipMemcpyHostToDevice) != hipSuccess)
+    return fail("device upload failed");
+  if (batch_layout(B) || batch_capture(B)) return fail(B->err);
+  return B;
+}
+
+const char* f3d_batch_error(f3d_batch* B) { return B ? B->err : "no batch"; }
+
+int f3d_batch_size(f3d_batch* B) { return B->n; }
+
+f3d_sim* f3d_batch_instance(f3d_batch* B, int i) { return (i >= 0 && i < B->n) ? B->sims[i] : nullptr; }
+
+int f3d_batch_step(f3d_batch* B, int nsteps) {
+  for (int s = 0; s < nsteps; s++) BCHECK(B, hipGraphLaunch(B->exec, B->stream));
+  return 0;
+}
+
+int f3d_batch_sync(f3d_batch* B) {
+  BCHECK(B, hipStreamSynchronize(B->stream));
+  std::vector<double> clk(5 * (size_t)B->n);
+  BCHECK(B, hipMemcpy(clk.data(), B->dclk, clk.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int i = 0; i < B->n; i++)
+    if (clk[5 * (size_t)i + 4] != 0.0) {
+      snprintf(B->err, sizeof(B->err), "instance %d: null time step", i);
+      return -1;
+    }
+  return 0;
+}
+
+int f3d_batch_clock(f3d_batch* B, int i, double out[3]) {
+  if (i < 0 || i >= B->n) return -2;
+  BCHECK(B, hipStreamSynchronize(B->stream));
+  BCHECK(B, hipMemcpy(out, B->dclk + 5 * (size_t)i, 3 * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int f3d_batch_set_tiling(f3d_batch* B, int rows_y, int tile) {
+  using namespace F3D_NS;
+  using namespace F3D_NS::fused;
+  if (rows_y < 1 || (tile != 64 && tile != 128 && tile != 256)) return -2;
+  B->rows_y = rows_y;
+  B->tile = tile;
+  if (batch_layout(B) || batch_capture(B)) return -1;
+  return 0;
+}
+
+int f3d_batch_tune(f3d_batch* B, int reps, int* best_rows_y, int* best_tile) {
+  using namespace F3D_NS;
+  using namespace F3D_NS::fused;
+  if (reps < 1) reps = 5;
+  if (batch_state(B, 0)) return -1;
+  hipEvent_t e0, e1;
+  BCHECK(B, hipEventCreate(&e0));
+  BCHECK(B, hipEventCreate(&e1));
+  float best = 1e30f;
+  int br = B->rows_y, bt = B->tile;
+  const int rows[] = {1, 2, 4, 8}, tiles[] = {64, 128, 256};
+  for (int r : rows)
+    for (int t : tiles) {
+      if (f3d_batch_set_tiling(B, r, t)) return -1;
+      if (f3d_batch_step(B, 2)) return -1;
+      BCHECK(B, hipEventRecord(e0, B->stream));
+      if (f3d_batch_step(B, reps)) return -1;
+      BCHECK(B, hipEventRecord(e1, B->stream));
+      BCHECK(B, hipEventSynchronize(e1));
+      float ms = 0.f;
+      BCHECK(B, hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        br = r;
+        bt = t;
+      }
+      if (batch_state(B, 1)) return -1;
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (f3d_batch_set_tiling(B, br, bt)) return -1;
+  if (best_rows_y) *best_rows_y = br;
+  if (best_tile) *best_tile = bt;
+  return 0;
+}
+
+}  // extern "C"

@@ -193,18 +193,16 @@ __device__ __forceinline__ double seq_sum(const double* w, int n) {
 // ring on the bit pattern of the positive double gives the value of
 // reduction_full_MIN + cfl_b (src/cfl_b.c:6-37) exactly.
 template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_cfl(dmesh m, const real* cs, const real* rho, const real* vx,
-                                             const real* vy, const real* vz, double* vxmed,
-                                             unsigned long long* result, int slot, cell::CflPar cp) {
+__device__ __forceinline__ void kd_cfl(const dim3 bid, const dmesh& m, const real* cs, const real* rho, const real* vx, const real* vy, const real* vz, double* vxmed, unsigned long long* result, int slot, cell::CflPar cp) {
   extern __shared__ double lds[];  // 2 nx weights + 16
   double* w0 = lds;
   double* w1 = lds + m.nx;
   double* sh = lds + 2 * m.nx;
-  const int j = m.nghy + (int)(blockIdx.x % m.ny);
-  const int k = DZ ? m.nghz + (int)(blockIdx.x / m.ny) : 0;
+  const int j = m.nghy + (int)(bid.x % m.ny);
+  const int k = DZ ? m.nghz + (int)(bid.x / m.ny) : 0;
   const int base = j * m.pitch + k * m.stride;
   const int l2 = j + k * m.sy;
-  if (blockIdx.x == 0 && threadIdx.x == 0) result[slot ^ 1] = kInfBits;  // the next step's slot
+  if (bid.x == 0 && threadIdx.x == 0) result[slot ^ 1] = kInfBits;  // the next step's slot
   // ComputeVweight + reduction_SUM (src/compute_vmed.c:14-30) of rings j and j+1
   for (int i = threadIdx.x; i < m.nx; i += blockDim.x) {
     const double dx = m.xmin64[i + 1] - m.xmin64[i];
@@ -231,6 +229,12 @@ __global__ void __launch_bounds__(WG) kf_cfl(dmesh m, const real* cs, const real
     }
     atomicMin(&result[slot], (unsigned long long)__double_as_longlong(mn));
   }
+}
+template <int G, int DZ>
+__global__ void __launch_bounds__(WG) kf_cfl(dmesh m, const real* cs, const real* rho, const real* vx,
+                                             const real* vy, const real* vz, double* vxmed,
+                                             unsigned long long* result, int slot, cell::CflPar cp) {
+  kd_cfl<G, DZ>(blockIdx, m, cs, rho, vx, vy, vz, vxmed, result, slot, cp);
 }
 
 // ------------------------------------------------------------------ sources
@@ -271,14 +275,11 @@ __device__ __forceinline__ VxcLoader<G> vxc(const real* vx, const dmesh& m, int 
 // kf_src_a: stresses (visctensor ranges j,k in [1, s-2]) and artificial
 // pressure (SubStep2_a) at every cell the active update reads.
 template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_src_a(dmesh m, const real* rho, const real* energy, const real* vx,
-                                               const real* vy, const real* vz, real* txx, real* tyy,
-                                               real* tzz, real* txz, real* tyx, real* tzy, real* qx,
-                                               real* qy, real* qz, cell::ViscPar vp, int visc, real domega) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void kd_src_a(const dim3 bid, const dmesh& m, const real* rho, const real* energy, const real* vx, const real* vy, const real* vz, real* txx, real* tyy, real* tzz, real* txz, real* tyx, real* tzy, real* qx, real* qy, real* qz, cell::ViscPar vp, int visc, real domega) {
+  const int i = bid.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
-  const int j = 1 + (int)blockIdx.y;       // 1 .. sy-2
-  const int k = DZ ? 1 + (int)blockIdx.z : 0;
+  const int j = 1 + (int)bid.y;       // 1 .. sy-2
+  const int k = DZ ? 1 + (int)bid.z : 0;
   const int l = i + j * m.pitch + k * m.stride;
   const auto VX = vxc<G>(vx, m, l, i, j, k, domega);
   const cell::GL VY = cell::gl(vy, m, l, i);
@@ -302,22 +303,22 @@ __global__ void __launch_bounds__(WG) kf_src_a(dmesh m, const real* rho, const r
   qy[l] = cell::art_q(r0, VY(0, 1, 0) - VY(0, 0, 0));
   if (DZ) qz[l] = cell::art_q(r0, VZ(0, 0, 1) - VZ(0, 0, 0));
 }
+template <int G, int DZ>
+__global__ void __launch_bounds__(WG) kf_src_a(dmesh m, const real* rho, const real* energy, const real* vx,
+                                               const real* vy, const real* vz, real* txx, real* tyy,
+                                               real* tzz, real* txz, real* tyx, real* tzy, real* qx,
+                                               real* qy, real* qz, cell::ViscPar vp, int visc, real domega) {
+  kd_src_a<G, DZ>(blockIdx, m, rho, energy, vx, vy, vz, txx, tyy, tzz, txz, tyx, tzy, qx, qy, qz, vp, visc, domega);
+}
 
 // kf_src_b: the velocity update of SubStep1 + addviscosity + SubStep2_b on
 // active cells, in the order the reference applies them.
 template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_src_b(dmesh m, real dt, const real* rho, const real* energy,
-                                               const real* vx, const real* vy, const real* vz,
-                                               const real* pot, const real* txx, const real* tyy,
-                                               const real* tzz, const real* txz, const real* tyx,
-                                               const real* tzy, const real* qx, const real* qy,
-                                               const real* qz, real* vxt, real* vyt, real* vzt, real* e2,
-                                               int visc, int potential, int iso, real gm1, real omegaframe,
-                                               real domega) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt, const real* rho, const real* energy, const real* vx, const real* vy, const real* vz, const real* pot, const real* txx, const real* tyy, const real* tzz, const real* txz, const real* tyx, const real* tzy, const real* qx, const real* qy, const real* qz, real* vxt, real* vyt, real* vzt, real* e2, int visc, int potential, int iso, real gm1, real omegaframe, real domega) {
+  const int i = bid.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
-  const int j = m.nghy + (int)blockIdx.y;
-  const int k = DZ ? m.nghz + (int)blockIdx.z : 0;
+  const int j = m.nghy + (int)bid.y;
+  const int k = DZ ? m.nghz + (int)bid.z : 0;
   const int l = i + j * m.pitch + k * m.stride;
   const cell::GL R = cell::gl(rho, m, l, i);
   const PresLoader P{R, cell::gl(energy, m, l, i), gm1, iso};
@@ -341,6 +342,17 @@ __global__ void __launch_bounds__(WG) kf_src_b(dmesh m, real dt, const real* rho
   vyt[l] = ay;
   if (DZ) vzt[l] = az;
   if (!iso) e2[l] = e;
+}
+template <int G, int DZ>
+__global__ void __launch_bounds__(WG) kf_src_b(dmesh m, real dt, const real* rho, const real* energy,
+                                               const real* vx, const real* vy, const real* vz,
+                                               const real* pot, const real* txx, const real* tyy,
+                                               const real* tzz, const real* txz, const real* tyx,
+                                               const real* tzy, const real* qx, const real* qy,
+                                               const real* qz, real* vxt, real* vyt, real* vzt, real* e2,
+                                               int visc, int potential, int iso, real gm1, real omegaframe,
+                                               real domega) {
+  kd_src_b<G, DZ>(blockIdx, m, dt, rho, energy, vx, vy, vz, pot, txx, tyy, tzz, txz, tyx, tzy, qx, qy, qz, vxt, vyt, vzt, e2, visc, potential, iso, gm1, omegaframe, domega);
 }
 
 // SubStep3 (src/substep3.c:98-110) on active cells: compression work with the
@@ -528,17 +540,21 @@ struct ColY3 {
 // 2-D: momenta + Y sweep; rows [nghy + s*rows_per, ...) of the active range.
 // AD = 1 also transports the energy (e_in -> e_out).
 template <int G, int DZ, int AD>
-__global__ void __launch_bounds__(BXF) kf_sweep_y2(dmesh m, real dt, const real* rho_in, const real* vxt,
-                                                   const real* vyt, const real* e_in, real* rho_out,
-                                                   real* q0, real* q1, real* q2, real* q3, real* e_out,
-                                                   int rows_per, real of) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void kd_sweep_y2(const dim3 bid, const dmesh& m, real dt, const real* rho_in, const real* vxt, const real* vyt, const real* e_in, real* rho_out, real* q0, real* q1, real* q2, real* q3, real* e_out, int rows_per, real of) {
+  const int i = bid.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
-  const int r0 = m.nghy + (int)blockIdx.y * rows_per;
+  const int r0 = m.nghy + (int)bid.y * rows_per;
   const int r1 = min(r0 + rows_per, m.ny + m.nghy);
   ColY2<G> col{&m, rho_in, vxt, vyt, e_in, {q0, q1, q2, q3, AD ? e_out : rho_out, rho_out}, i, ixp_of(i, m.nx), of,
                dt};
   column_sweep<4 + AD>(col, r0, r1, dt);
+}
+template <int G, int DZ, int AD>
+__global__ void __launch_bounds__(BXF) kf_sweep_y2(dmesh m, real dt, const real* rho_in, const real* vxt,
+                                                   const real* vyt, const real* e_in, real* rho_out,
+                                                   real* q0, real* q1, real* q2, real* q3, real* e_out,
+                                                   int rows_per, real of) {
+  kd_sweep_y2<G, DZ, AD>(blockIdx, m, dt, rho_in, vxt, vyt, e_in, rho_out, q0, q1, q2, q3, e_out, rows_per, of);
 }
 
 // 3-D: Y sweep of the Z-swept fields
@@ -831,15 +847,19 @@ __global__ void __launch_bounds__(256) kf_sweep_x(dmesh m, double dt64, const re
 // ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30) for the tiled X sweep: one
 // workgroup per active ring, the ring's weights in LDS, one sequential sum.
 template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_vmed(dmesh m, const real* vxt, double* vxmed) {
+__device__ __forceinline__ void kd_vmed(const dim3 bid, const dmesh& m, const real* vxt, double* vxmed) {
   extern __shared__ double w[];
-  const int j = m.nghy + (int)(blockIdx.x % m.ny);
-  const int k = DZ ? m.nghz + (int)(blockIdx.x / m.ny) : 0;
+  const int j = m.nghy + (int)(bid.x % m.ny);
+  const int k = DZ ? m.nghz + (int)(bid.x / m.ny) : 0;
   const int base = j * m.pitch + k * m.stride;
   for (int i = threadIdx.x; i < m.nx; i += blockDim.x)
     w[i] = (double)vxt[base + i] * (m.xmin64[i + 1] - m.xmin64[i]);
   __syncthreads();
   if (threadIdx.x == 0) vxmed[j + k * m.sy] = seq_sum(w, m.nx) / (m.XMAX - m.XMIN);
+}
+template <int G, int DZ>
+__global__ void __launch_bounds__(WG) kf_vmed(dmesh m, const real* vxt, double* vxmed) {
+  kd_vmed<G, DZ>(blockIdx, m, vxt, vxmed);
 }
 
 // X sweep of rings too long for LDS: workgroups own tiles of T cells and
@@ -850,13 +870,7 @@ __global__ void __launch_bounds__(WG) kf_vmed(dmesh m, const real* vxt, double* 
 // (AdvectSHIFT, src/advect_shift.c:45-58) with NewVelocity_x computed at the
 // destination cell from the source cell and its left neighbour.
 template <int G, int DZ, int AD, int T>
-__global__ void __launch_bounds__(T) kf_sweep_x_tile(dmesh m, double dt64, const real* rho_in, const real* vxt,
-                                                      const real* mpx, const real* mmx, const real* mpy,
-                                                      const real* mmy, const real* mpz, const real* mmz,
-                                                      const real* e_in, real* e_out, real* rho_out,
-                                                      real* vx_out, real* mpy_o, real* mmy_o, real* mpz_o,
-                                                      real* mmz_o, const double* vxmed, int* nshift_out,
-                                                      real omegaframe, int fargo) {
+__device__ __forceinline__ void kd_sweep_x_tile(const dim3 bid, const dmesh& m, double dt64, const real* rho_in, const real* vxt, const real* mpx, const real* mmx, const real* mpy, const real* mmy, const real* mpz, const real* mmz, const real* e_in, real* e_out, real* rho_out, real* vx_out, real* mpy_o, real* mmy_o, real* mpz_o, real* mmz_o, const double* vxmed, int* nshift_out, real omegaframe, int fargo) {
   constexpr int H = 6, W = T + 2 * H;
   const real dt = (real)dt64;
   constexpr int NM = DZ ? 6 : 4;
@@ -865,11 +879,11 @@ __global__ void __launch_bounds__(T) kf_sweep_x_tile(dmesh m, double dt64, const
   __shared__ real ST[N * W], AQ[N * W], SL[N * W], FC[N * W], QR2[N * W];
   __shared__ real V[W], VU[W], ZS[W];
   const int nx = m.nx;
-  const int ring = blockIdx.y;
+  const int ring = bid.y;
   const int j = m.nghy + ring % m.ny;
   const int k = DZ ? m.nghz + ring / m.ny : 0;
   const int base = j * m.pitch + k * m.stride;
-  const int i0 = blockIdx.x * T;
+  const int i0 = bid.x * T;
   const int t = threadIdx.x;
   const double vmed = vxmed[j + k * m.sy];
   const real* qin[7] = {mpx, mmx, mpy, mmy, DZ ? mpz : e_in, DZ ? mmz : e_in, e_in};
@@ -879,7 +893,7 @@ __global__ void __launch_bounds__(T) kf_sweep_x_tile(dmesh m, double dt64, const
   };
   // the integer shift of the ring: ComputeResidual at i = 0 (src/compute_residual.c:68-71)
   const int ns = fargo ? (int)floor(vmed * dt64 / zone_size_x64<G>(m, 0, j, k) + 0.5) : 0;
-  if (fargo && blockIdx.x == 0 && t == 0) nshift_out[j + k * m.sy] = ns;
+  if (fargo && bid.x == 0 && t == 0) nshift_out[j + k * m.sy] = ns;
   for (int p = t; p < W; p += T) {
     const int g = gidx(p);
     ST[NQ * W + p] = rho_in[base + g];
@@ -1002,18 +1016,24 @@ __global__ void __launch_bounds__(T) kf_sweep_x_tile(dmesh m, double dt64, const
     }
   }
 }
+template <int G, int DZ, int AD, int T>
+__global__ void __launch_bounds__(T) kf_sweep_x_tile(dmesh m, double dt64, const real* rho_in, const real* vxt,
+                                                      const real* mpx, const real* mmx, const real* mpy,
+                                                      const real* mmy, const real* mpz, const real* mmz,
+                                                      const real* e_in, real* e_out, real* rho_out,
+                                                      real* vx_out, real* mpy_o, real* mmy_o, real* mpz_o,
+                                                      real* mmz_o, const double* vxmed, int* nshift_out,
+                                                      real omegaframe, int fargo) {
+  kd_sweep_x_tile<G, DZ, AD, T>(blockIdx, m, dt64, rho_in, vxt, mpx, mmx, mpy, mmy, mpz, mmz, e_in, e_out, rho_out, vx_out, mpy_o, mmy_o, mpz_o, mmz_o, vxmed, nshift_out, omegaframe, fargo);
+}
 
 // NewVelocity_y/z on active faces and Stockholm damping on active cells.
 template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_finish(dmesh m, real dt, const real* rho_x, const real* mmy,
-                                                const real* mpy, const real* mmz, const real* mpz,
-                                                real* rho, real* vx, real* vy, real* vz, const real* rho0,
-                                                const real* vx0, const real* vy0, const real* vz0,
-                                                cell::DampPar dp, int stockholm) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void kd_finish(const dim3 bid, const dmesh& m, real dt, const real* rho_x, const real* mmy, const real* mpy, const real* mmz, const real* mpz, real* rho, real* vx, real* vy, real* vz, const real* rho0, const real* vx0, const real* vy0, const real* vz0, cell::DampPar dp, int stockholm) {
+  const int i = bid.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
-  const int j = m.nghy + (int)blockIdx.y;
-  const int k = DZ ? m.nghz + (int)blockIdx.z : 0;
+  const int j = m.nghy + (int)bid.y;
+  const int k = DZ ? m.nghz + (int)bid.z : 0;
   const int l = i + j * m.pitch + k * m.stride;
   const real r = rho_x[l];
   // the faces j = nghy and k = nghz lie on the mesh boundary; the ghost fill
@@ -1028,18 +1048,29 @@ __global__ void __launch_bounds__(WG) kf_finish(dmesh m, real dt, const real* rh
   rho[l] = r;
   if (stockholm) cell::stockholm<G, DZ>(m, j, k, l, dt, rho, vx, vy, vz, rho0, vx0, vy0, vz0, dp);
 }
+template <int G, int DZ>
+__global__ void __launch_bounds__(WG) kf_finish(dmesh m, real dt, const real* rho_x, const real* mmy,
+                                                const real* mpy, const real* mmz, const real* mpz,
+                                                real* rho, real* vx, real* vy, real* vz, const real* rho0,
+                                                const real* vx0, const real* vy0, const real* vz0,
+                                                cell::DampPar dp, int stockholm) {
+  kd_finish<G, DZ>(blockIdx, m, dt, rho_x, mmy, mpy, mmz, mpz, rho, vx, vy, vz, rho0, vx0, vy0, vz0, dp, stockholm);
+}
 
 // ------------------------------------------------------------------ ghost fill, potential, damping references
 // The same per-cell formulas as the per-routine entry points (f3d_cell.h),
 // in this build's type.  Y: both sides in one launch when ny > nghy (the
 // sides then write disjoint rows and read none the other writes), Z likewise.
+__device__ __forceinline__ void kd_bound_y(const dim3 bid, const dmesh& m, int side0, real* dens, real* vx, real* vy, real* vz, real* e, cell::BcPar bp0, cell::BcPar bp1, const real* dexp0, const real* dexp1) {
+  const int i = bid.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int side = side0 + ((int)bid.y >= m.nghy);
+  const int mm = (int)bid.y - (side - side0) * m.nghy, k = bid.z;
+  cell::bound_y_cell(m, side, i, mm, k, dens, vx, vy, vz, e, side ? bp1 : bp0, side ? dexp1 : dexp0);
+}
 __global__ void kf_bound_y(dmesh m, int side0, real* dens, real* vx, real* vy, real* vz, real* e, cell::BcPar bp0,
                            cell::BcPar bp1, const real* dexp0, const real* dexp1) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m.nx) return;
-  const int side = side0 + ((int)blockIdx.y >= m.nghy);
-  const int mm = (int)blockIdx.y - (side - side0) * m.nghy, k = blockIdx.z;
-  cell::bound_y_cell(m, side, i, mm, k, dens, vx, vy, vz, e, side ? bp1 : bp0, side ? dexp1 : dexp0);
+  kd_bound_y(blockIdx, m, side0, dens, vx, vy, vz, e, bp0, bp1, dexp0, dexp1);
 }
 
 __global__ void kf_bound_z(dmesh m, int side0, real* dens, real* vx, real* vy, real* vz, real* e, cell::BcPar bp0,
@@ -1461,3 +1492,7 @@ int F3D_API(fused_transport)(f3d_hip_ctx* ctx, double dt, int end_of_step) {
 
 }  // namespace fused
 }  // namespace F3D_NS
+
+#if F3D_PREC == 32
+#include "f3d_batch_body.h"
+#endif

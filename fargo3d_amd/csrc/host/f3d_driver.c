@@ -58,6 +58,8 @@ int f3d_sim_force_dt(f3d_sim *S, double dt) {
   return 0;
 }
 
+double f3d_sim_dt_out(f3d_sim *S) { return S->p.dt_out; }
+
 int f3d_sim_eos(f3d_sim *S) { return S->p.eos; }
 
 void f3d_sim_shape(f3d_sim *S, int out[4]) {

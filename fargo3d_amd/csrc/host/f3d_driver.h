@@ -52,6 +52,7 @@ void f3d_sim_set_scalars(f3d_sim *S, const double in[8]);
 void f3d_sim_get_planets(f3d_sim *S, double *out, int maxn);
 void f3d_sim_shape(f3d_sim *S, int out[4]); /* nx, ny, nz, dimz */
 int f3d_sim_eos(f3d_sim *S);                 /* 1 isothermal, 2 adiabatic */
+double f3d_sim_dt_out(f3d_sim *S);           /* DT, the output interval the step clamps to */
 f3d_hip_ctx *f3d_sim_ctx(f3d_sim *S);
 /* refresh the kernels' copy of the run parameters and host scalars */
 int f3d_sim_push_phys(f3d_sim *S);
@@ -59,6 +60,28 @@ int f3d_sim_push_phys(f3d_sim *S);
 /* Output in FARGO3D's merged raw format (src/output.c:283-315): active cells,
  * x fastest, native-endian doubles, no header; file <dir>/<prefix><name><n>.dat. */
 int f3d_sim_write_field(f3d_sim *S, const char *name, const char *path);
+
+/* ---- batched form (no counterpart in the reference): n independent 2-D
+ * isothermal disks, each its own .par set-up, run in the FP32 form with
+ * their planets held on their orbits (potential computed once); every stage
+ * of the step is one launch over all instances and one step is one hipGraph
+ * replay, time steps included (per-instance CFL and output-time clamp on the
+ * device).  Instances are read through f3d_batch_instance + f3d_sim_get_field. */
+typedef struct f3d_batch f3d_batch;
+f3d_batch *f3d_batch_create(int n, const char *const *parfiles, const char *const *overrides, char *err,
+                            int errlen);
+void f3d_batch_destroy(f3d_batch *B);
+const char *f3d_batch_error(f3d_batch *B);
+int f3d_batch_size(f3d_batch *B);
+f3d_sim *f3d_batch_instance(f3d_batch *B, int i);
+int f3d_batch_step(f3d_batch *B, int nsteps);
+int f3d_batch_sync(f3d_batch *B);                        /* also reports a null time step */
+int f3d_batch_clock(f3d_batch *B, int i, double out[3]); /* dt, dtemp, PhysicalTime */
+/* tuning knobs (rows per thread of the Y sweep; 64/128/256-cell X tiles) and
+ * the autotuner that times every combination on the batch itself, restoring
+ * the state afterwards */
+int f3d_batch_set_tiling(f3d_batch *B, int rows_y, int tile);
+int f3d_batch_tune(f3d_batch *B, int reps, int *best_rows_y, int *best_tile);
 
 #ifdef __cplusplus
 }
