@@ -173,6 +173,67 @@ def fp32_errors(fa, parfile, steps):
     return out
 
 
+def batch_specs(n, seed=20240605):
+    """n 2-D isothermal disk set-ups with uneven grids and seeded parameters:
+    Nx in {256,384,768,1536}, Ny in {128,192,384}, planet mass log-uniform in
+    [1e-5, 3e-3], viscosity log-uniform in [1e-6, 1e-4], aspect ratio uniform
+    in [0.03, 0.08]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(n):
+        nx = int(rng.choice([256, 384, 768, 1536]))
+        ny = int(rng.choice([128, 192, 384]))
+        mp = 10 ** rng.uniform(np.log10(1e-5), np.log10(3e-3))
+        nu = 10 ** rng.uniform(-6, -4)
+        h = rng.uniform(0.03, 0.08)
+        specs.append((PAR["c1"], "Nx=%d Ny=%d PlanetMass=%.6e Nu=%.6e AspectRatio=%.6f" % (nx, ny, mp, nu, h)))
+    return specs
+
+
+def batched_form(fa, n, steps, loop_steps):
+    """Aggregate cell-updates/s of n disks advanced together (FP32, one graph
+    replay per step, autotuned tiles) and of looping the plain FP64 path over
+    the same disks."""
+    specs = batch_specs(n)
+    b = fa.Batch(specs)
+    cells = sum(b.cells)
+    b.step(5)
+    b.sync()
+    rows_y, tile = b.tune(reps=3)
+    b.step(5)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    el = time.perf_counter() - t0
+    b.close()
+    v = cells * steps / el
+    # the plain FP64 path, one disk after the other
+    t_loop = 0.0
+    for par, ov in specs:
+        s = fa.Simulation(par, ov)
+        for _ in range(3):
+            s.step()
+        s.sync()
+        t1 = time.perf_counter()
+        for _ in range(loop_steps):
+            s.step()
+        s.sync()
+        t_loop += time.perf_counter() - t1
+        s.close()
+    v_loop = cells * loop_steps / t_loop
+    return {"value": round(v, 1), "unit": "cell-updates/s", "instances": n, "cells": cells, "steps": steps,
+            "ms_per_step": round(1e3 * el / steps, 4), "dtype": "f32 (f64 sums, residual, cfl)",
+            "tuned": {"rows_y": rows_y, "x_tile": tile},
+            "step_roofline": {"bytes_per_cell_update": 28, "achieved": round(v * 28 / 1e9, 1), "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": round(v * 28 / 1e9 / HBM_PEAK_GBS, 4)},
+            "fp64_loop": {"value": round(v_loop, 1), "steps_per_instance": loop_steps},
+            "speedup_vs_fp64_loop": round(v / v_loop, 2),
+            "workload": "%d 2-D isothermal disks, Nx in {256,384,768,1536} x Ny in {128,192,384}, seeded planet "
+                        "mass, viscosity and aspect ratio; planets held on their orbits" % n}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,6 +244,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-fp32", action="store_true", help="skip the reduced-precision form")
+    ap.add_argument("--batch", type=int, default=64, help="instances of the batched form (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--error-steps", type=int, default=200,
                     help="steps of the paired FP64/FP32 run that measures the FP32 form's error")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -240,6 +303,9 @@ def main():
                           "roofline": roofline_of(st32, cells, dimz, ad, alpha, 4, args.config),
                           "stages": st32,
                           "error_vs_fp64": fp32_errors(fa, parfile, args.error_steps)}}
+    if not args.routines and args.batch > 0:
+        forms = forms or {}
+        forms["batched"] = batched_form(fa, args.batch, args.batch_steps, 10)
 
     cpu = None
     if not args.no_cpu_baseline:
