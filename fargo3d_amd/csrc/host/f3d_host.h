@@ -79,7 +79,12 @@ typedef struct f3d_params {
   int bc;                      /* boundary family */
   int energy_bc;               /* 1: Energy ghosts SYMMETRIC (|a|a|) on the Y (and Z) sides */
   int precision;               /* 64: FP64 fields (the drop-in); 32: FP32 fields, FP64 sums/residual/CFL */
-  int nfluids;
+  int nfluids;                 /* fluid 0 is the gas, the others dust (setups/fargo_multifluid) */
+  double invstokes[F3D_MAX_FLUIDS]; /* Invstokes1.. : inverse Stokes number of dust fluid k */
+  double epsilon;              /* dust-to-gas density ratio of each dust fluid */
+  int dragforce;               /* -DDRAGFORCE with -DSTOKESNUMBER (implicit drag, src/collisions.c) */
+  int dustdiffusion;           /* -DDUSTDIFFUSION (src/dust_diffusion_main.c) */
+  int feedback;                /* the gas feels the drag of the dust (ColRate feedback, src/colrate.c) */
   int ic_keplerian;            /* 1: v_phi = r^-1/2 - Omega_f r, v_r = 0, noise on density only */
   uint64_t seed;               /* srand48 seed for the noise (0 = glibc default state) */
   /* derived (src/param.c:236-240) */
@@ -169,6 +174,9 @@ void f3d_host_advance_planets(f3d_psys *sys, const f3d_params *p, f3d_clock *c, 
  * as the setup's condinit.c does. */
 int f3d_initial_conditions(const f3d_grid *g, const f3d_params *p, const f3d_clock *c,
                            double *dens, double *energy, double *vx, double *vy, double *vz);
+/* the same for fluid type F3D_GAS or F3D_DUST of a multifluid setup */
+int f3d_initial_conditions_fluid(const f3d_grid *g, const f3d_params *p, const f3d_clock *c, int fluidtype,
+                                 double *dens, double *energy, double *vx, double *vy, double *vz);
 
 /* ---- boundary tables ----
  * The generated boundary kernels (std/boundaries.txt, scripts/boundparser.py)

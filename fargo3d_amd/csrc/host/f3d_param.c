@@ -73,6 +73,17 @@ static const var_t VARS[] = {
     {"OUTPUTDIR", T_STR, OFF(outputdir), SLEN(outputdir)},
     /* .opt switches, settable at run time */
     {"NFLUIDS", T_INT, OFF(nfluids), 0},
+    {"INVSTOKES1", T_REAL, OFF(invstokes[1]), 0},
+    {"INVSTOKES2", T_REAL, OFF(invstokes[2]), 0},
+    {"INVSTOKES3", T_REAL, OFF(invstokes[3]), 0},
+    {"INVSTOKES4", T_REAL, OFF(invstokes[4]), 0},
+    {"INVSTOKES5", T_REAL, OFF(invstokes[5]), 0},
+    {"INVSTOKES6", T_REAL, OFF(invstokes[6]), 0},
+    {"INVSTOKES7", T_REAL, OFF(invstokes[7]), 0},
+    {"EPSILON", T_REAL, OFF(epsilon), 0},
+    {"DRAGFORCE", T_BOOL, OFF(dragforce), 0},
+    {"DUSTDIFFUSION", T_BOOL, OFF(dustdiffusion), 0},
+    {"FEEDBACK", T_BOOL, OFF(feedback), 0},
     {"KEPLERIANIC", T_BOOL, OFF(ic_keplerian), 0},
 };
 
@@ -140,6 +151,9 @@ void f3d_params_defaults(f3d_params *p) {
   p->fargo = 1;
   p->bc = F3D_BC_FARGO2D;
   p->energy_bc = 0;
+  p->epsilon = 0.0;
+  p->dragforce = p->dustdiffusion = 0;
+  p->feedback = 1;
   p->precision = 64;
   p->nfluids = 1;
   p->ic_keplerian = 0;
@@ -161,6 +175,15 @@ int f3d_params_setup_options(f3d_params *p, const char *setup) {
     p->bc = F3D_BC_FARGO2D;
     /* a setup-local .bound entry "Energy: Ymin/Ymax: SYMMETRIC" */
     p->energy_bc = 1;
+  } else if (strcmp(setup, "fargo_multifluid") == 0) {
+    /* setups/fargo_multifluid/fargo_multifluid.opt:13-44: one gas and
+       NFLUIDS-1 dust fluids, implicit drag with Stokes numbers, dust diffusion */
+    p->geometry = F3D_CYLINDRICAL; p->dimz = 0; p->eos = F3D_ISOTHERMAL;
+    p->visc = F3D_VISC_NU; p->potential = 1; p->stockholm = 1; p->fargo = 1;
+    p->bc = F3D_BC_FARGO2D;
+    p->nfluids = 4;
+    p->dragforce = 1;
+    p->dustdiffusion = 1;
   } else if (strcmp(setup, "p3disof") == 0 || strcmp(setup, "p3disof_alpha") == 0) {
     /* setups/p3disof/p3disof.opt:15-33 (ALPHAVISCOSITY variant on request) */
     p->geometry = F3D_SPHERICAL; p->dimz = 1; p->eos = F3D_ISOTHERMAL;

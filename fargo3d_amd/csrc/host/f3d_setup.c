@@ -30,6 +30,40 @@ static double gauss48(void) {
 
 int f3d_initial_conditions(const f3d_grid *g, const f3d_params *p, const f3d_clock *c,
                            double *dens, double *energy, double *vx, double *vy, double *vz) {
+  return f3d_initial_conditions_fluid(g, p, c, F3D_GAS, dens, energy, vx, vy, vz);
+}
+
+/* setups/fargo_multifluid/condinit.c:8-52: gas in pressure-supported
+   rotation, each dust fluid Keplerian and pressureless with EPSILON times the
+   gas density; no noise */
+static void multifluid_ic(const f3d_grid *g, const f3d_params *p, double OMEGAFRAME, int fluidtype, double *dens,
+                          double *energy, double *vx, double *vy) {
+  for (int j = 0; j < g->sy; j++) {
+    for (int i = 0; i < g->sx; i++) {
+      size_t l = (size_t)i + (size_t)j * g->pitch;
+      double r = g->ymed[j];
+      double omega = sqrt(G * MSTAR / r / r / r);
+      double rhog = p->sigma0 * pow(r / R0, -p->sigmaslope);
+      double rhod = rhog * p->epsilon;
+      if (fluidtype == F3D_GAS) {
+        dens[l] = rhog;
+        vx[l] = omega * r * sqrt(1.0 + pow(p->aspectratio, 2) * pow(r / R0, 2 * p->flaringindex) *
+                                           (2.0 * p->flaringindex - 1.0 - p->sigmaslope));
+        vy[l] = 0.0;
+        energy[l] = p->aspectratio * pow(r / R0, p->flaringindex) * omega * r;
+      } else {
+        dens[l] = rhod;
+        vx[l] = omega * r;
+        vy[l] = 0.0;
+        energy[l] = 0.0;
+      }
+      vx[l] -= OMEGAFRAME * r;
+    }
+  }
+}
+
+int f3d_initial_conditions_fluid(const f3d_grid *g, const f3d_params *p, const f3d_clock *c, int fluidtype,
+                                 double *dens, double *energy, double *vx, double *vy, double *vz) {
   const double OMEGAFRAME = c->OMEGAFRAME;
   f3d_seed(p->seed);
   size_t n = g->ncell;
@@ -38,6 +72,11 @@ int f3d_initial_conditions(const f3d_grid *g, const f3d_params *p, const f3d_clo
   memset(vx, 0, n * sizeof(double));
   memset(vy, 0, n * sizeof(double));
   if (vz) memset(vz, 0, n * sizeof(double));
+  if (strcmp(p->setup, "fargo_multifluid") == 0) {
+    if (g->dimz) return -1;
+    multifluid_ic(g, p, OMEGAFRAME, fluidtype, dens, energy, vx, vy);
+    return 0;
+  }
   if (!g->dimz) {
     /* 2-D cylindrical disks: setups/fargo/condinit.c:25-50 */
     for (int j = 0; j < g->sy; j++) {

@@ -101,6 +101,11 @@ class Oracle:
         self.shape = (sz, sy, nx)
         self.shape2d = (sz, sy)
 
+    def select_fluid(self, n):
+        """SelectFluid (src/LowTasks.c:565-593): field() then reads fluid n."""
+        self.L.orc_select_fluid.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        self.L.orc_select_fluid(self._s, int(n))
+
     def field(self, name):
         p = self.L.orc_field(self._s, name.encode())
         if not p:

@@ -52,6 +52,11 @@ typedef struct orc_state {
   double localforce[12];
   long alreadycalculated;
   int compute_disk_force; /* 1: evaluate ComputeAccel every step as the reference does */
+  /* multifluid: collision rates (Alpha, src/colrate.c:10-23), dust diffusion
+     coefficients (Sdiffyczc/Sdiffyfzc), density-only ghost fill flag */
+  double alpha[F3D_MAX_FLUIDS * F3D_MAX_FLUIDS];
+  double *Sdiffyczc, *Sdiffyfzc;
+  int fill_density_only;
 } orc_state;
 
 /* lifetime */
@@ -119,6 +124,10 @@ void orc_NewVelocity_z(orc_state *S);
 void orc_StockholmBoundary(orc_state *S, double dt);
 void orc_init_stockholm(orc_state *S);
 void orc_Sources(orc_state *S, double dt);
+void orc_Collisions(orc_state *S, double dt);
+void orc_DustDiffusion_Coefficients(orc_state *S);
+void orc_DustDiffusion_Core(orc_state *S, double dt);
+void orc_DustDiffusion_Main(orc_state *S, double dt);
 void orc_Transport(orc_state *S, double dt);
 
 /* field access for the Python side */

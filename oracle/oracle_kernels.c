@@ -803,6 +803,12 @@ static void bound_y(orc_state *S, int side) {
         }
         const double dfac = side ? S->bct.ymax_dens[m] : S->bct.ymin_dens[m];
         const double vsq = side ? S->bct.ymax_vsq[m] : S->bct.ymin_vsq[m];
+        if (S->fill_density_only) {
+          /* FillGhosts(DENS) (src/dust_diffusion_main.c:25) */
+          dens[lgh] = p3d ? dens[lact] * dfac * (side ? S->bct.ymax_dexp : S->bct.ymin_dexp)[m * sz + k]
+                          : dens[lact] * dfac;
+          continue;
+        }
         if (!p3d) {
           /* KEPLERIAN2DDENS, KEPLERIAN2DVAZIM, ANTISYMMETRIC */
           dens[lgh] = dens[lact] * dfac;
@@ -1349,4 +1355,170 @@ void orc_init_stockholm(orc_state *S) {
       if (DIMZ) S->Vz0[l2] = S->Vz[l];
       S->Density0[l2] = S->Density[l];
     }
+}
+
+/* ------------------------------------------------------------------ multifluid */
+
+/* _collisions_cpu (src/collisions.c:10-154) with the STOKESNUMBER collision
+   matrix (src/collision_kernel.h:14-94) and the partial-pivot Gauss solver
+   (src/gauss.h:1-45), option 1 (V_temp in and out), one direction
+   (id1, id2, id3) per call. */
+static void collisions_dir(orc_state *S, double dt, int id1, int id2, int id3) {
+  ORC_MESH(S);
+  const int NF = S->nfluids;
+  double *rho[F3D_MAX_FLUIDS], *vel[F3D_MAX_FLUIDS];
+  for (int f = 0; f < NF; f++) {
+    rho[f] = S->fl[f].Density;
+    vel[f] = id1 ? S->fl[f].Vx_temp : id2 ? S->fl[f].Vy_temp : S->fl[f].Vz_temp;
+  }
+  const double *alpha = S->alpha;
+  const int size_y = Ny + 2 * NGHY, size_z = Nz + 2 * NGHZ;
+  KRANGE(1, size_z);
+  double b[F3D_MAX_FLUIDS], m[F3D_MAX_FLUIDS * F3D_MAX_FLUIDS];
+  for (int k = kbeg; k < kend; k++)
+    for (int j = 1; j < size_y; j++)
+      for (int i = 0; i < Nx; i++) {
+        const int l = lidx(i, j, k);
+        const int idm = LXM(l, i) * id1 + LYM(l) * id2 + LZM(l) * id3;
+        const double omega = (id1 + id3) * sqrt(G * MSTAR / (ymed(j) * ymed(j) * ymed(j))) +
+                             id2 * sqrt(G * MSTAR / (ymin(j) * ymin(j) * ymin(j)));
+        for (int o = 0; o < NF; o++) {
+          for (int p = 0; p < NF; p++) {
+            const double rho_p = 0.5 * (rho[p][l] + rho[p][idm]);
+            if (p != o) {
+              const double rho_o = 0.5 * (rho[o][l] + rho[o][idm]);
+              if (p > o) m[p + o * NF] = -dt * omega * alpha[p + o * NF] * rho_p / rho_o;
+              else m[p + o * NF] = -dt * omega * alpha[p + o * NF];
+            } else {
+              double sum = 0.0;
+              for (int q = 0; q < NF; q++) {
+                if (q != p) {
+                  const double rho_q = 0.5 * (rho[q][l] + rho[q][idm]);
+                  if (q > p) sum += alpha[q + p * NF] * rho_q / rho_p;
+                  else sum += alpha[q + p * NF];
+                }
+              }
+              m[p + p * NF] = 1.0 + dt * omega * sum;
+            }
+          }
+          b[o] = vel[o][l];
+        }
+        for (int ir = 0; ir < NF - 1; ir++) {
+          /* pivot: as src/gauss.h:8-14 (the stored maximum is the signed value) */
+          double max_value = -1;
+          int ir_max = ir;
+          for (int ir2 = ir; ir2 < NF; ir2++) {
+            if (fabs(m[ir + ir2 * NF]) >= max_value) {
+              max_value = m[ir + ir2 * NF];
+              ir_max = ir2;
+            }
+          }
+          if (ir_max != ir) {
+            for (int ic = 0; ic < NF; ic++) {
+              double temp = m[ic + ir * NF];
+              m[ic + ir * NF] = m[ic + ir_max * NF];
+              m[ic + ir_max * NF] = temp;
+            }
+            double temp = b[ir];
+            b[ir] = b[ir_max];
+            b[ir_max] = temp;
+          }
+          for (int ir2 = ir + 1; ir2 < NF; ir2++) {
+            const double factor = m[ir + ir2 * NF] / m[ir + ir * NF];
+            for (int ic = 0; ic < NF; ic++) m[ic + ir2 * NF] -= m[ic + ir * NF] * factor;
+            b[ir2] -= b[ir] * factor;
+          }
+        }
+        for (int ir = NF - 1; ir > -1; ir--) {
+          for (int ic = NF - 1; ic > ir; ic--) b[ir] -= m[ic + ir * NF] * b[ic];
+          b[ir] /= m[ir + ir * NF];
+        }
+        for (int o = 0; o < NF; o++) vel[o][l] = b[o];
+      }
+}
+
+/* Collisions(dt, 1) (src/collisions.c:156-189): X, then Y, then Z */
+void orc_Collisions(orc_state *S, double dt) {
+  collisions_dir(S, dt, 1, 0, 0);
+  collisions_dir(S, dt, 0, 1, 0);
+  if (S->g.dimz) collisions_dir(S, dt, 0, 0, 1);
+}
+
+/* DustDiffusion_Coefficients_cpu (src/dust_diffusion_coefficients.c:10-143):
+   the diffusion coefficient at cell centres and at lower Y faces, from the
+   gas (Fluids[0]); 2-D isothermal forms */
+void orc_DustDiffusion_Coefficients(orc_state *S) {
+  ORC_MESH(S);
+  const double *cs = S->fl[0].Energy;
+  double *yczc = S->Sdiffyczc, *yfzc = S->Sdiffyfzc;
+  const int size_y = Ny + 2 * NGHY, size_z = Nz + 2 * NGHZ;
+  KRANGE(0, size_z);
+  for (int k = kbeg; k < kend; k++)
+    for (int j = 0; j < size_y; j++)
+      for (int i = 0; i < Nx; i++) {
+        const int ll = lidx(i, j, k), llym = LYM(ll);
+        if (S->p.visc == F3D_VISC_ALPHA) {
+          const double soundspeed2 = cs[ll] * cs[ll];
+          double soundspeedf2;
+          if (j == 0) soundspeedf2 = soundspeed2;
+          else soundspeedf2 = 0.5 * (cs[ll] + cs[llym]) * 0.5 * (cs[ll] + cs[llym]);
+          const double r3yczc = ymed(j) * ymed(j) * ymed(j);
+          const double r3yfzc = ymin(j) * ymin(j) * ymin(j);
+          yczc[ll] = S->p.alpha * soundspeed2 / sqrt(G * MSTAR / r3yczc);
+          yfzc[ll] = S->p.alpha * soundspeedf2 / sqrt(G * MSTAR / r3yfzc);
+        } else {
+          yczc[ll] = S->p.nu;
+          yfzc[ll] = S->p.nu;
+        }
+      }
+}
+
+/* DustDiffusion_Core_cpu (src/dust_diffusion_core.c:10-192), CYLINDRICAL 2-D:
+   the selected (dust) fluid's updated density into Pressure */
+void orc_DustDiffusion_Core(orc_state *S, double dt) {
+  ORC_MESH(S);
+  const double *yfzc = S->Sdiffyfzc, *yczc = S->Sdiffyczc;
+  const double *rhod = S->Density, *rhog = S->fl[0].Density;
+  double *temp = S->Pressure;
+  const int size_y = Ny + 2 * NGHY - 1, size_z = Nz + 2 * NGHZ - 1;
+  KRANGE(1, size_z);
+  for (int k = kbeg; k < kend; k++)
+    for (int j = 1; j < size_y; j++)
+      for (int i = 0; i < Nx; i++) {
+        const int ll = lidx(i, j, k), llxm = LXM(ll, i), llxp = LXP(ll, i), llyp = LYP(ll), llym = LYM(ll);
+        double update = 0.0;
+        const double c = rhod[ll] / (rhod[ll] + rhog[ll]);
+        const double dxmin = xmin(i + 1) - xmin(i);
+        const double dxmedp = 1.0 / InvDiffXmed[IXP(i)];
+        const double dxmedm = 1.0 / InvDiffXmed[i];
+        const double d1 = 0.25 * (rhod[ll] + rhog[ll] + rhod[llxp] + rhog[llxp]) * (yczc[llxp] + yczc[ll]);
+        const double d2 = 0.25 * (rhod[ll] + rhog[ll] + rhod[llxm] + rhog[llxm]) * (yczc[llxm] + yczc[ll]);
+        const double cxp = rhod[llxp] / (rhod[llxp] + rhog[llxp]);
+        const double cxm = rhod[llxm] / (rhod[llxm] + rhog[llxm]);
+        update += 1.0 / ymed(j) / ymed(j) / (dxmin) * (d1 * (cxp - c) / (dxmedp) - (d2 * (c - cxm)) / (dxmedm));
+        const double d3 = 0.5 * (rhod[ll] + rhog[ll] + rhod[llyp] + rhog[llyp]) * yfzc[llyp];
+        const double d4 = 0.5 * (rhod[ll] + rhog[ll] + rhod[llym] + rhog[llym]) * yfzc[ll];
+        const double cyp = rhod[llyp] / (rhod[llyp] + rhog[llyp]);
+        const double cym = rhod[llym] / (rhod[llym] + rhog[llym]);
+        update += 1.0 / ymed(j) / (ymin(j + 1) - ymin(j)) *
+                  (ymin(j + 1) * d3 * (cyp - c) / (ymed(j + 1) - ymed(j)) -
+                   (ymin(j) * d4 * (c - cym)) / (ymed(j) - ymed(j - 1)));
+        temp[ll] = rhod[ll] + dt * update;
+      }
+}
+
+/* DustDiffusion_Main (src/dust_diffusion_main.c:3-28): coefficients, then for
+   each dust fluid the core, copy_field(Density, Pressure) (src/copy_field.c:3-12)
+   and the density ghost fill */
+void orc_DustDiffusion_Main(orc_state *S, double dt) {
+  orc_DustDiffusion_Coefficients(S);
+  for (int f = 0; f < S->nfluids; f++) {
+    orc_select_fluid(S, f);
+    if (S->Fluidtype != F3D_DUST) continue;
+    orc_DustDiffusion_Core(S, dt);
+    memcpy(S->Density, S->Pressure, S->g.ncell * sizeof(double));
+    S->fill_density_only = 1;
+    orc_FillGhosts(S);
+    S->fill_density_only = 0;
+  }
 }

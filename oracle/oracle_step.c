@@ -61,10 +61,15 @@ orc_state *orc_create(const char *parfile, const char *overrides, char *err, int
     return NULL;
   }
   size_t n = S->g.ncell, n2 = S->g.n2d;
-  S->nfluids = 1;
+  S->nfluids = S->p.nfluids;
+  if (S->nfluids > F3D_MAX_FLUIDS) {
+    if (err) snprintf(err, errlen, "too many fluids");
+    free(S);
+    return NULL;
+  }
   for (int f = 0; f < S->nfluids; f++) {
     orc_fluid *F = &S->fl[f];
-    F->type = F3D_GAS;
+    F->type = f == 0 ? F3D_GAS : F3D_DUST; /* setups/fargo_multifluid/condinit.c:58-76 */
     F->Density = fld(n); F->Energy = fld(n);
     F->Vx = fld(n); F->Vy = fld(n); F->Vz = fld(n);
     F->Vx_temp = fld(n); F->Vy_temp = fld(n); F->Vz_temp = fld(n);
@@ -76,6 +81,12 @@ orc_state *orc_create(const char *parfile, const char *overrides, char *err, int
   S->Pressure = fld(n); S->Total_Density = fld(n); S->QL = fld(n); S->QR = fld(n);
   S->Reduction2D = fld(n2); S->VxMed = fld(n2);
   S->Nshift = (int *)calloc(n2, sizeof(int));
+  S->Sdiffyczc = fld(n); S->Sdiffyfzc = fld(n);
+  /* ColRate(INVSTOKESk, gas, k, feedback) (src/colrate.c:10-23) */
+  for (int f = 1; f < S->nfluids; f++) {
+    S->alpha[0 + f * S->nfluids] = S->p.invstokes[f];
+    S->alpha[f + 0 * S->nfluids] = S->p.feedback ? S->p.invstokes[f] : 0.0;
+  }
   S->alreadycalculated = -1;
   S->compute_disk_force = 1;
   f3d_bctab_init(&S->bct, &S->g, &S->p);
@@ -91,8 +102,8 @@ orc_state *orc_create(const char *parfile, const char *overrides, char *err, int
   /* CondInit, FillGhosts, init_stockholm (src/main.c:265, 311, 315) */
   for (int f = 0; f < S->nfluids; f++) {
     orc_select_fluid(S, f);
-    f3d_initial_conditions(&S->g, &S->p, &S->clk, S->Density, S->Energy, S->Vx, S->Vy,
-                           S->g.dimz ? S->Vz : NULL);
+    f3d_initial_conditions_fluid(&S->g, &S->p, &S->clk, S->Fluidtype, S->Density, S->Energy, S->Vx, S->Vy,
+                                 S->g.dimz ? S->Vz : NULL);
   }
   for (int f = 0; f < S->nfluids; f++) {
     orc_select_fluid(S, f);
@@ -116,7 +127,8 @@ void orc_destroy(orc_state *S) {
     for (size_t i = 0; i < sizeof(a) / sizeof(a[0]); i++) free(a[i]);
   }
   double *b[] = {S->Mpx, S->Mmx, S->Mpy, S->Mmy, S->Mpz, S->Mmz, S->Pot, S->Slope, S->DivRho,
-                 S->DensStar, S->Qs, S->Pressure, S->Total_Density, S->QL, S->QR, S->Reduction2D, S->VxMed};
+                 S->DensStar, S->Qs, S->Pressure, S->Total_Density, S->QL, S->QR, S->Reduction2D, S->VxMed,
+                 S->Sdiffyczc, S->Sdiffyfzc};
   for (size_t i = 0; i < sizeof(b) / sizeof(b[0]); i++) free(b[i]);
   free(S->Nshift);
   f3d_bctab_free(&S->bct);
@@ -345,6 +357,8 @@ double orc_step(orc_state *S) {
     orc_select_fluid(S, f);
     orc_Sources(S, dt);
   }
+  if (S->p.dragforce) orc_Collisions(S, dt); /* src/main.c:412-414 */
+  if (S->p.dustdiffusion) orc_DustDiffusion_Main(S, dt); /* src/main.c:416-418 */
   for (int f = 0; f < S->nfluids; f++) {
     orc_select_fluid(S, f);
     orc_Transport(S, dt);
