@@ -68,6 +68,8 @@ def lib():
         L.f3d_sim_shape.argtypes = [vp, vp]
         L.f3d_sim_eos.argtypes = [vp]
         L.f3d_sim_force_dt.argtypes = [vp, d]
+        L.f3d_sim_nfluids.argtypes = [vp]
+        L.f3d_sim_select_fluid.argtypes = [vp, i]
         L.f3d_sim_ctx.restype = vp
         L.f3d_sim_ctx.argtypes = [vp]
         L.f3d_sim_push_phys.argtypes = [vp]
@@ -100,6 +102,7 @@ class Simulation:
         L.f3d_sim_shape(self._s, shp)
         self.nx, self.ny, self.nz, self.dimz = list(shp)
         self.adiabatic = (L.f3d_sim_eos(self._s) == 2)
+        self.nfluids = L.f3d_sim_nfluids(self._s)
         sy = self.ny + 6
         sz = self.nz + 6 if self.dimz else 1
         self.shape = (sz, sy, self.nx)
@@ -122,6 +125,10 @@ class Simulation:
             self.close()
         except Exception:
             pass
+
+    def select_fluid(self, n):
+        """SelectFluid (src/LowTasks.c:565): field access then refers to fluid n (0 = gas)."""
+        self._check(self.L.f3d_sim_select_fluid(self._s, int(n)))
 
     @property
     def ctx(self):

@@ -29,7 +29,7 @@ HOST_SRCS = ["host/f3d_mesh.c", "host/f3d_param.c", "host/f3d_planets.c", "host/
 HIP_SRCS = ["hip/f3d_runtime.hip", "hip/f3d_routines.hip", "hip/f3d_fused.hip", "hip/f3d_fused32.hip"]
 
 CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-Wall",
-          "-Wno-unused-function"]
+          "-Wno-unused-function", "-Werror=implicit-function-declaration"]
 HIPFLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contract=off",
             "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-result", "-I" + os.path.join(ROOT, "include")]

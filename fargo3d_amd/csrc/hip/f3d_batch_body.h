@@ -48,7 +48,8 @@ __device__ __forceinline__ dim3 local2(int lb, int gx) { return dim3(lb % gx, lb
 __global__ void __launch_bounds__(WG) kb_cfl(const BInst* I, const int* off, int n) {
   const int ii = find_inst(off, n, blockIdx.x);
   const BInst& in = I[ii];
-  kd_cfl<CYL, 0>(dim3(blockIdx.x - off[ii]), in.m, in.cs, in.rho, in.vx, in.vy, in.vy, in.vxmed, in.res, 0, in.cp);
+  kd_cfl<CYL, 0>(dim3(blockIdx.x - off[ii]), in.m, in.cs, in.rho, in.vx, in.vy, in.vy, in.vx, in.vxmed, in.res, 0,
+                 in.cp);
 }
 
 // CflFluidsMin and the output-time clamp (src/cfl_fluids_min.c:3-22,
@@ -86,7 +87,7 @@ __global__ void __launch_bounds__(BXF) kb_src_b(const BInst* I, const int* off, 
   const int gx = (in.m.nx + BXF - 1) / BXF;
   kd_src_b<CYL, 0>(local2(blockIdx.x - off[ii], gx), in.m, (real)in.clk[0], in.rho, in.cs, in.vx, in.vy, nullptr,
                    in.pot, in.mmx, in.mmy, nullptr, in.mpx, in.mpy, nullptr, in.qx, in.qy, nullptr, in.vxt, in.vyt,
-                   nullptr, in.e2, in.visc, in.potential, 1, (real)0, in.of, (real)0);
+                   nullptr, in.e2, in.visc, in.potential, 0, 1, (real)0, in.of, (real)0);
 }
 
 // the ghost fill in Y (both sides in one launch): mid = 1 fills V_temp

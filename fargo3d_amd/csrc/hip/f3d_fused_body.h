@@ -193,7 +193,7 @@ __device__ __forceinline__ double seq_sum(const double* w, int n) {
 // ring on the bit pattern of the positive double gives the value of
 // reduction_full_MIN + cfl_b (src/cfl_b.c:6-37) exactly.
 template <int G, int DZ>
-__device__ __forceinline__ void kd_cfl(const dim3 bid, const dmesh& m, const real* cs, const real* rho, const real* vx, const real* vy, const real* vz, double* vxmed, unsigned long long* result, int slot, cell::CflPar cp) {
+__device__ __forceinline__ void kd_cfl(const dim3 bid, const dmesh& m, const real* cs, const real* rho, const real* vx, const real* vy, const real* vz, const real* vxm, double* vxmed, unsigned long long* result, int slot, cell::CflPar cp) {
   extern __shared__ double lds[];  // 2 nx weights + 16
   double* w0 = lds;
   double* w1 = lds + m.nx;
@@ -204,10 +204,12 @@ __device__ __forceinline__ void kd_cfl(const dim3 bid, const dmesh& m, const rea
   const int l2 = j + k * m.sy;
   if (bid.x == 0 && threadIdx.x == 0) result[slot ^ 1] = kInfBits;  // the next step's slot
   // ComputeVweight + reduction_SUM (src/compute_vmed.c:14-30) of rings j and j+1
+  // of vxm (the velocity whose VxMed the reference's cfl reads: with several
+  // fluids, the last fluid's, src/main.c:366 then :381)
   for (int i = threadIdx.x; i < m.nx; i += blockDim.x) {
     const double dx = m.xmin64[i + 1] - m.xmin64[i];
-    w0[i] = (double)vx[base + i] * dx;
-    if (cp.fargo) w1[i] = (double)vx[base + m.pitch + i] * dx;
+    w0[i] = (double)vxm[base + i] * dx;
+    if (cp.fargo) w1[i] = (double)vxm[base + m.pitch + i] * dx;
   }
   __syncthreads();
   if (threadIdx.x == 0) sh[8] = seq_sum(w0, m.nx) / (m.XMAX - m.XMIN);
@@ -232,9 +234,9 @@ __device__ __forceinline__ void kd_cfl(const dim3 bid, const dmesh& m, const rea
 }
 template <int G, int DZ>
 __global__ void __launch_bounds__(WG) kf_cfl(dmesh m, const real* cs, const real* rho, const real* vx,
-                                             const real* vy, const real* vz, double* vxmed,
+                                             const real* vy, const real* vz, const real* vxm, double* vxmed,
                                              unsigned long long* result, int slot, cell::CflPar cp) {
-  kd_cfl<G, DZ>(blockIdx, m, cs, rho, vx, vy, vz, vxmed, result, slot, cp);
+  kd_cfl<G, DZ>(blockIdx, m, cs, rho, vx, vy, vz, vxm, vxmed, result, slot, cp);
 }
 
 // ------------------------------------------------------------------ sources
@@ -314,7 +316,7 @@ __global__ void __launch_bounds__(WG) kf_src_a(dmesh m, const real* rho, const r
 // kf_src_b: the velocity update of SubStep1 + addviscosity + SubStep2_b on
 // active cells, in the order the reference applies them.
 template <int G, int DZ>
-__device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt, const real* rho, const real* energy, const real* vx, const real* vy, const real* vz, const real* pot, const real* txx, const real* tyy, const real* tzz, const real* txz, const real* tyx, const real* tzy, const real* qx, const real* qy, const real* qz, real* vxt, real* vyt, real* vzt, real* e2, int visc, int potential, int iso, real gm1, real omegaframe, real domega) {
+__device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt, const real* rho, const real* energy, const real* vx, const real* vy, const real* vz, const real* pot, const real* txx, const real* tyy, const real* tzz, const real* txz, const real* tyx, const real* tzy, const real* qx, const real* qy, const real* qz, real* vxt, real* vyt, real* vzt, real* e2, int visc, int potential, int dust, int iso, real gm1, real omegaframe, real domega) {
   const int i = bid.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int j = m.nghy + (int)bid.y;
@@ -326,9 +328,9 @@ __device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt
   const auto VX = vxc<G>(vx, m, l, i, j, k, domega);
   const cell::GL VY = cell::gl(vy, m, l, i);
   const cell::GL VZ = cell::gl(DZ ? vz : vy, m, l, i);
-  real ax = cell::substep1_x<G>(m, i, j, k, dt, R, P, PT, VX, 0, potential);
-  real ay = cell::substep1_y<G, DZ>(m, i, j, k, dt, R, P, PT, VX, VY, VZ, 0, potential, omegaframe);
-  real az = DZ ? cell::substep1_z<G>(m, i, j, k, dt, R, P, PT, VX, VZ, 0, potential, omegaframe) : real(0.0);
+  real ax = cell::substep1_x<G>(m, i, j, k, dt, R, P, PT, VX, dust, potential);
+  real ay = cell::substep1_y<G, DZ>(m, i, j, k, dt, R, P, PT, VX, VY, VZ, dust, potential, omegaframe);
+  real az = DZ ? cell::substep1_z<G>(m, i, j, k, dt, R, P, PT, VX, VZ, dust, potential, omegaframe) : real(0.0);
   if (visc) {
     const cell::GL TZZ = cell::gl(DZ ? tzz : txx, m, l, i), TXZ = cell::gl(DZ ? txz : txx, m, l, i),
                    TZY = cell::gl(DZ ? tzy : txx, m, l, i);
@@ -350,9 +352,9 @@ __global__ void __launch_bounds__(WG) kf_src_b(dmesh m, real dt, const real* rho
                                                const real* tzz, const real* txz, const real* tyx,
                                                const real* tzy, const real* qx, const real* qy,
                                                const real* qz, real* vxt, real* vyt, real* vzt, real* e2,
-                                               int visc, int potential, int iso, real gm1, real omegaframe,
+                                               int visc, int potential, int dust, int iso, real gm1, real omegaframe,
                                                real domega) {
-  kd_src_b<G, DZ>(blockIdx, m, dt, rho, energy, vx, vy, vz, pot, txx, tyy, tzz, txz, tyx, tzy, qx, qy, qz, vxt, vyt, vzt, e2, visc, potential, iso, gm1, omegaframe, domega);
+  kd_src_b<G, DZ>(blockIdx, m, dt, rho, energy, vx, vy, vz, pot, txx, tyy, tzz, txz, tyx, tzy, qx, qy, qz, vxt, vyt, vzt, e2, visc, potential, dust, iso, gm1, omegaframe, domega);
 }
 
 // SubStep3 (src/substep3.c:98-110) on active cells: compression work with the
@@ -1108,6 +1110,178 @@ __global__ void kf_init_stockholm(dmesh m, const real* rho, const real* vx, cons
   rho0[l2] = rho[l];
 }
 
+// ------------------------------------------------------------------ multifluid
+// The fluids of a multifluid run, as the drag and dust-diffusion kernels see
+// them (one struct by value; at most F3D_MAX_FLUIDS fluids).
+struct FluidSet {
+  const real* rho[8];
+  real* vx[8];
+  real* vy[8];
+  real* vz[8];
+  double alpha[64];  // collision rates Alpha (src/colrate.c:10-23), column + row*NF
+  int nf;
+};
+
+// One direction of the implicit drag (src/collision_kernel.h:14-94, src/gauss.h
+// :1-45) at one cell; v[] holds that direction's velocity of each fluid.
+template <int NF>
+__device__ __forceinline__ void drag_solve(const FluidSet& F, real* const* v, int l, int idm, double dt,
+                                           double omega) {
+  real m[NF * NF], b[NF];
+#pragma unroll
+  for (int o = 0; o < NF; o++) {
+#pragma unroll
+    for (int p = 0; p < NF; p++) {
+      const real rho_p = real(0.5) * (F.rho[p][l] + F.rho[p][idm]);
+      if (p != o) {
+        const real rho_o = real(0.5) * (F.rho[o][l] + F.rho[o][idm]);
+        if (p > o) m[p + o * NF] = -(real)dt * (real)omega * (real)F.alpha[p + o * NF] * rho_p / rho_o;
+        else m[p + o * NF] = -(real)dt * (real)omega * (real)F.alpha[p + o * NF];
+      } else {
+        real sum = real(0.0);
+#pragma unroll
+        for (int q = 0; q < NF; q++) {
+          if (q != p) {
+            const real rho_q = real(0.5) * (F.rho[q][l] + F.rho[q][idm]);
+            if (q > p) sum += (real)F.alpha[q + p * NF] * rho_q / rho_p;
+            else sum += (real)F.alpha[q + p * NF];
+          }
+        }
+        m[p + p * NF] = real(1.0) + (real)dt * (real)omega * sum;
+      }
+    }
+    b[o] = v[o][l];
+  }
+#pragma unroll
+  for (int ir = 0; ir < NF - 1; ir++) {
+    // pivot as src/gauss.h:8-14: the comparison uses |m|, the stored maximum
+    // the signed value
+    real max_value = real(-1);
+    int ir_max = ir;
+#pragma unroll
+    for (int ir2 = ir; ir2 < NF; ir2++) {
+      if (fabs(m[ir + ir2 * NF]) >= max_value) {
+        max_value = m[ir + ir2 * NF];
+        ir_max = ir2;
+      }
+    }
+#pragma unroll
+    for (int r = ir + 1; r < NF; r++) {
+      if (ir_max == r) {
+#pragma unroll
+        for (int ic = 0; ic < NF; ic++) {
+          const real temp = m[ic + ir * NF];
+          m[ic + ir * NF] = m[ic + r * NF];
+          m[ic + r * NF] = temp;
+        }
+        const real temp = b[ir];
+        b[ir] = b[r];
+        b[r] = temp;
+      }
+    }
+#pragma unroll
+    for (int ir2 = ir + 1; ir2 < NF; ir2++) {
+      const real factor = m[ir + ir2 * NF] / m[ir + ir * NF];
+#pragma unroll
+      for (int ic = 0; ic < NF; ic++) m[ic + ir2 * NF] -= m[ic + ir * NF] * factor;
+      b[ir2] -= b[ir] * factor;
+    }
+  }
+#pragma unroll
+  for (int ir = NF - 1; ir > -1; ir--) {
+#pragma unroll
+    for (int ic = NF - 1; ic > ir; ic--) b[ir] -= m[ic + ir * NF] * b[ic];
+    b[ir] /= m[ir + ir * NF];
+  }
+#pragma unroll
+  for (int o = 0; o < NF; o++) v[o][l] = b[o];
+}
+
+// Collisions(dt, 1) (src/collisions.c:156-189) on V_temp in place over the
+// reference's range (k, j from 1, ghosts included): the X, Y and Z solves of a
+// cell touch different velocity components, so one launch does the
+// directions dirs (bit 0: x, 1: y, 2: z) in the reference's order.
+template <int G, int DZ, int NF>
+__global__ void __launch_bounds__(BXF) kf_collisions(dmesh m, FluidSet F, double dt, int dirs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int j = 1 + (int)blockIdx.y;
+  const int k = DZ ? 1 + (int)blockIdx.z : 0;
+  const int l = i + j * m.pitch + k * m.stride;
+  const double om_c = sqrt(1.0 * 1.0 / (m.ymed64[j] * m.ymed64[j] * m.ymed64[j]));
+  const double om_f = sqrt(1.0 * 1.0 / (m.ymin64[j] * m.ymin64[j] * m.ymin64[j]));
+  if (dirs & 1) drag_solve<NF>(F, F.vx, l, lxm_of(l, i, m.nx), dt, om_c);
+  if (dirs & 2) drag_solve<NF>(F, F.vy, l, l - m.pitch, dt, om_f);
+  if (DZ && (dirs & 4)) drag_solve<NF>(F, F.vz, l, l - m.stride, dt, om_c);
+}
+
+// DustDiffusion_Coefficients (src/dust_diffusion_coefficients.c:10-143),
+// 2-D isothermal forms, every cell: yczc at cell centres, yfzc at lower Y faces
+template <int G, int DZ>
+__global__ void __launch_bounds__(BXF) kf_ddiff_coef(dmesh m, const real* cs, real* yczc, real* yfzc, int alpha_visc,
+                                                     real alpha, real nu) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int j = blockIdx.y;
+  const int ll = i + j * m.pitch;
+  if (alpha_visc) {
+    const real c0 = cs[ll];
+    const real soundspeed2 = c0 * c0;
+    real soundspeedf2;
+    if (j == 0) soundspeedf2 = soundspeed2;
+    else soundspeedf2 = real(0.5) * (c0 + cs[ll - m.pitch]) * real(0.5) * (c0 + cs[ll - m.pitch]);
+    const real r3yczc = m.ymed[j] * m.ymed[j] * m.ymed[j];
+    const real r3yfzc = m.ymin[j] * m.ymin[j] * m.ymin[j];
+    yczc[ll] = alpha * soundspeed2 / sqrt(real(1.0) * real(1.0) / r3yczc);
+    yfzc[ll] = alpha * soundspeedf2 / sqrt(real(1.0) * real(1.0) / r3yfzc);
+  } else {
+    yczc[ll] = nu;
+    yfzc[ll] = nu;
+  }
+}
+
+// DustDiffusion_Core (src/dust_diffusion_core.c:10-192), CYLINDRICAL 2-D, on
+// rows j in [1, sy-2]: the dust fluid's updated density into temp
+template <int G, int DZ>
+__global__ void __launch_bounds__(BXF) kf_ddiff_core(dmesh m, real dt, const real* rhod, const real* rhog,
+                                                     const real* yczc, const real* yfzc, real* temp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int j = 1 + (int)blockIdx.y;
+  const int ll = i + j * m.pitch;
+  const int llxm = lxm_of(ll, i, m.nx), llxp = lxp_of(ll, i, m.nx), llyp = ll + m.pitch, llym = ll - m.pitch;
+  real update = real(0.0);
+  const real c = rhod[ll] / (rhod[ll] + rhog[ll]);
+  const real dxmin = m.xmin[i + 1] - m.xmin[i];
+  const real dxmedp = real(1.0) / m.invdiffxmed[ixp_of(i, m.nx)];
+  const real dxmedm = real(1.0) / m.invdiffxmed[i];
+  const real d1 = real(0.25) * (rhod[ll] + rhog[ll] + rhod[llxp] + rhog[llxp]) * (yczc[llxp] + yczc[ll]);
+  const real d2 = real(0.25) * (rhod[ll] + rhog[ll] + rhod[llxm] + rhog[llxm]) * (yczc[llxm] + yczc[ll]);
+  const real cxp = rhod[llxp] / (rhod[llxp] + rhog[llxp]);
+  const real cxm = rhod[llxm] / (rhod[llxm] + rhog[llxm]);
+  const real ym = m.ymed[j];
+  update += real(1.0) / ym / ym / (dxmin) * (d1 * (cxp - c) / (dxmedp) - (d2 * (c - cxm)) / (dxmedm));
+  const real d3 = real(0.5) * (rhod[ll] + rhog[ll] + rhod[llyp] + rhog[llyp]) * yfzc[llyp];
+  const real d4 = real(0.5) * (rhod[ll] + rhog[ll] + rhod[llym] + rhog[llym]) * yfzc[ll];
+  const real cyp = rhod[llyp] / (rhod[llyp] + rhog[llyp]);
+  const real cym = rhod[llym] / (rhod[llym] + rhog[llym]);
+  update += real(1.0) / ym / (m.ymin[j + 1] - m.ymin[j]) *
+            (m.ymin[j + 1] * d3 * (cyp - c) / (m.ymed[j + 1] - ym) -
+             (m.ymin[j] * d4 * (c - cym)) / (ym - m.ymed[j - 1]));
+  temp[ll] = rhod[ll] + dt * update;
+}
+
+// Density-only Y ghost fill (FillGhosts(DENS), src/dust_diffusion_main.c:25)
+__global__ void kf_bound_y_dens(dmesh m, real* dens, cell::BcPar bp0, cell::BcPar bp1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int side = (int)blockIdx.y >= m.nghy;
+  const int mm = (int)blockIdx.y - side * m.nghy;
+  const int lgh = side ? i + (m.ny + m.nghy + mm) * m.pitch : i + mm * m.pitch;
+  const int lact = side ? i + (m.ny + m.nghy - 1 - mm) * m.pitch : i + (2 * m.nghy - mm - 1) * m.pitch;
+  dens[lgh] = dens[lact] * (side ? bp1 : bp0).dens[mm];
+}
+
 // ------------------------------------------------------------------ C ABI
 
 #define FDISPATCH(ctx, KERN, grid, block, shmem, ...)                                                  \
@@ -1320,7 +1494,12 @@ int F3D_API(fused_set_tiling)(int rows_y, int rows_z) {
   return 0;
 }
 
-int F3D_API(fused_cfl)(f3d_hip_ctx* ctx, double* fluid_min) {
+int F3D_API(fused_cfl_vm)(f3d_hip_ctx* ctx, const double* vxm_in, double* fluid_min);
+
+int F3D_API(fused_cfl)(f3d_hip_ctx* ctx, double* fluid_min) { return F3D_API(fused_cfl_vm)(ctx, nullptr, fluid_min); }
+
+int F3D_API(fused_cfl_vm)(f3d_hip_ctx* ctx, const double* vxm_in, double* fluid_min) {
+  const real* vxm = reinterpret_cast<const real*>(vxm_in);
   Scratch* S;
   F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
   const dmesh& m = S->m;
@@ -1329,7 +1508,7 @@ int F3D_API(fused_cfl)(f3d_hip_ctx* ctx, double* fluid_min) {
   const size_t shmem = (2 * m.nx + 16) * sizeof(double);
   const int slot = S->slot;
   f3d_stage_begin(ctx, F3D_ST_CFL);
-  FDISPATCH(ctx, kf_cfl, dim3(nrings), dim3(WG), shmem, f.Energy, f.Density, f.Vx, f.Vy, f.Vz, f.VxMed, S->result,
+  FDISPATCH(ctx, kf_cfl, dim3(nrings), dim3(WG), shmem, f.Energy, f.Density, f.Vx, f.Vy, f.Vz, vxm ? vxm : f.Vx, f.VxMed, S->result,
             slot, cfl_par_f(ctx));
   f3d_stage_end(ctx, F3D_ST_CFL);
   F3D_CHECK(ctx, hipMemcpyAsync(ctx->hres, S->result + slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1381,6 +1560,79 @@ int F3D_API(fused_boundaries)(f3d_hip_ctx* ctx) {
   return fused_boundaries(ctx, S, f.Density, f.Vx, f.Vy, f.Vz, f.Energy);
 }
 
+// Implicit drag of all fluids (Collisions(dt, 1), src/collisions.c:156-189),
+// V_temp in place; the arrays hold each fluid's device pointers.
+int F3D_API(fused_collisions)(f3d_hip_ctx* ctx, double dt, int nf, double* const* rho, double* const* vxt,
+                              double* const* vyt, double* const* vzt, const double* alpha) {
+  Scratch* S;
+  F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
+  F3D_REQUIRE(ctx, nf >= 2 && nf <= 8, "collisions: 2 to 8 fluids");
+  const dmesh& m = S->m;
+  FluidSet F;
+  memset(&F, 0, sizeof(F));
+  F.nf = nf;
+  for (int f = 0; f < nf; f++) {
+    F.rho[f] = reinterpret_cast<const real*>(rho[f]);
+    F.vx[f] = reinterpret_cast<real*>(vxt[f]);
+    F.vy[f] = reinterpret_cast<real*>(vyt[f]);
+    F.vz[f] = reinterpret_cast<real*>(m.dimz ? vzt[f] : vyt[f]);
+  }
+  for (int a = 0; a < nf * nf; a++) F.alpha[a] = alpha[a];
+  dim3 g((m.nx + BXF - 1) / BXF, m.sy - 1, m.dimz ? m.sz - 1 : 1);
+  const int dirs = m.dimz ? 7 : 3;
+#define COLL(NF_)                                                                                  \
+  case NF_:                                                                                        \
+    if (m.geom == SPH) {                                                                           \
+      if (m.dimz) hipLaunchKernelGGL((kf_collisions<SPH, 1, NF_>), g, dim3(BXF), 0, ctx->stream, m, F, dt, dirs); \
+      else hipLaunchKernelGGL((kf_collisions<SPH, 0, NF_>), g, dim3(BXF), 0, ctx->stream, m, F, dt, dirs);        \
+    } else {                                                                                       \
+      if (m.dimz) hipLaunchKernelGGL((kf_collisions<CYL, 1, NF_>), g, dim3(BXF), 0, ctx->stream, m, F, dt, dirs); \
+      else hipLaunchKernelGGL((kf_collisions<CYL, 0, NF_>), g, dim3(BXF), 0, ctx->stream, m, F, dt, dirs);        \
+    }                                                                                              \
+    break;
+  switch (nf) {
+    COLL(2) COLL(3) COLL(4) COLL(5) COLL(6) COLL(7) COLL(8)
+  }
+#undef COLL
+  F3D_LAUNCH_CHECK(ctx);
+  return 0;
+}
+
+// DustDiffusion_Main (src/dust_diffusion_main.c:3-28): coefficients from the
+// gas (fluid 0), then for each dust fluid the core, the copy of the result
+// into its density (copy_field, src/copy_field.c:3-12) and its density ghost
+// fill.  2-D cylindrical, isothermal gas.
+int F3D_API(fused_dust_diffusion)(f3d_hip_ctx* ctx, double dt, int nf, double* const* rho, const int* type,
+                                  const double* cs_gas) {
+  Scratch* S;
+  F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
+  const dmesh& m = S->m;
+  F3D_REQUIRE(ctx, !m.dimz && m.geom == CYL, "dust diffusion: 2-D cylindrical meshes");
+  F3D_REQUIRE(ctx, type[0] == 1, "dust diffusion: fluid 0 must be the gas");
+  real* yczc = S->f[FS_QY];
+  real* yfzc = S->f[FS_QZ];
+  real* temp = S->f[FS_QX];
+  hipLaunchKernelGGL((kf_ddiff_coef<CYL, 0>), dim3((m.nx + BXF - 1) / BXF, m.sy), dim3(BXF), 0, ctx->stream, m,
+                     reinterpret_cast<const real*>(cs_gas), yczc, yfzc, (int)(ctx->ph.visc == 2),
+                     (real)ctx->ph.alpha, (real)ctx->ph.nu);
+  F3D_LAUNCH_CHECK(ctx);
+  const cell::BcPar b0 = bc_par_f(ctx, 0), b1 = bc_par_f(ctx, 1);
+  for (int f = 1; f < nf; f++) {
+    if (type[f] != 2) continue;
+    real* rd = reinterpret_cast<real*>(rho[f]);
+    hipLaunchKernelGGL((kf_ddiff_core<CYL, 0>), dim3((m.nx + BXF - 1) / BXF, m.sy - 2), dim3(BXF), 0, ctx->stream, m,
+                       (real)dt, rd, reinterpret_cast<const real*>(rho[0]), yczc, yfzc, temp);
+    F3D_LAUNCH_CHECK(ctx);
+    F3D_CHECK(ctx, hipMemcpyAsync(rd, temp, ctx->n3 * sizeof(real), hipMemcpyDeviceToDevice, ctx->stream));
+    if (!ctx->ph.periodicy) {
+      hipLaunchKernelGGL(kf_bound_y_dens, dim3((m.nx + BXF - 1) / BXF, 2 * m.nghy), dim3(BXF), 0, ctx->stream, m, rd,
+                         b0, b1);
+      F3D_LAUNCH_CHECK(ctx);
+    }
+  }
+  return 0;
+}
+
 int F3D_API(fused_sources)(f3d_hip_ctx* ctx, double dt, double domega) {
   Scratch* S;
   F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
@@ -1409,7 +1661,8 @@ int F3D_API(fused_sources)(f3d_hip_ctx* ctx, double dt, double domega) {
   f3d_stage_begin(ctx, F3D_ST_SRC_B);
   FDISPATCH(ctx, kf_src_b, gb, dim3(BXF), 0, rdt, f.Density, f.Energy, f.Vx, f.Vy, f.Vz, f.Pot, f.Mmx, f.Mmy, f.Mmz,
             f.Mpx, f.Mpy, f.Mpz, S->f[FS_QX], S->f[FS_QY], S->f[FS_QZ], f.Vx_temp, f.Vy_temp, f.Vz_temp,
-            S->f[FS_E2], visc, pot, iso, (real)(ctx->ph.gamma - 1.0), (real)ctx->ph.OMEGAFRAME, rdom);
+            S->f[FS_E2], visc, pot, (int)(ctx->ph.fluidtype == 2), iso, (real)(ctx->ph.gamma - 1.0),
+            (real)ctx->ph.OMEGAFRAME, rdom);
   f3d_stage_end(ctx, F3D_ST_SRC_B);
   if (!iso && ctx->ph.fluidtype == 1) {
     // SubStep3 (src/algogas.c:110-112)

@@ -37,8 +37,30 @@ struct f3d_sim {
   int prec32;    /* FP32 fields and the f3d32_fused_* stages */
   double dt_force; /* > 0: the next step's time step (measurement of two runs on one dt sequence) */
   size_t es;     /* bytes per element of the 3-D fields and the damping references */
+  /* fluids (src/allocate.c:45 CreateFluid: each fluid owns its primitive fields and damping
+     references; the scratch fields are shared) */
+  int nfl, cur;
+  int ftype[F3D_MAX_FLUIDS];
+  double alpha[F3D_MAX_FLUIDS * F3D_MAX_FLUIDS]; /* ColRate matrix, [i + j*nfl] (src/colrate.c:10-23) */
+  f3d_fields fl[F3D_MAX_FLUIDS];                 /* the per-fluid members of each fluid */
   char err[512];
 };
+
+/* the members of f3d_fields each fluid owns */
+#define F3D_FLUID_MEMBERS(X) \
+  X(Density) X(Energy) X(Vx) X(Vy) X(Vz) X(Vx_temp) X(Vy_temp) X(Vz_temp) X(Density0) X(Energy0) X(Vx0) X(Vy0) X(Vz0)
+
+/* SelectFluid (src/LowTasks.c:565): point the bound field set at fluid n */
+static void select_fluid(f3d_sim *S, int n) {
+#define SEL(m) S->f.m = S->fl[n].m;
+  F3D_FLUID_MEMBERS(SEL)
+#undef SEL
+  S->cur = n;
+  S->ph.fluidtype = S->ftype[n];
+  f3d_hip_bind(S->ctx, &S->f);
+  f3d_hip_set_phys(S->ctx, &S->ph);
+}
+
 
 #define ERR(S, ...)                                        \
   do {                                                     \
@@ -51,6 +73,13 @@ struct f3d_sim {
   } while (0)
 
 const char *f3d_sim_error(f3d_sim *S) { return S ? S->err : "no simulation"; }
+int f3d_sim_nfluids(f3d_sim *S) { return S->nfl; }
+
+int f3d_sim_select_fluid(f3d_sim *S, int n) {
+  if (n < 0 || n >= S->nfl) ERR(S, "no fluid %d (%d fluids)", n, S->nfl);
+  select_fluid(S, n);
+  return 0;
+}
 f3d_hip_ctx *f3d_sim_ctx(f3d_sim *S) { return S->ctx; }
 
 int f3d_sim_force_dt(f3d_sim *S, double dt) {
@@ -106,7 +135,7 @@ static int download(f3d_sim *S, double *h, const void *dev, size_t n) {
 int f3d_sim_push_phys(f3d_sim *S) {
   f3d_phys *ph = &S->ph;
   memset(ph, 0, sizeof(*ph));
-  ph->fluidtype = F3D_GAS;
+  ph->fluidtype = S->ftype[S->cur];
   ph->eos = S->p.eos;
   ph->visc = S->p.visc;
   ph->potential = S->p.potential;
@@ -188,9 +217,17 @@ f3d_sim *f3d_sim_create(const char *parfile, const char *overrides, char *err, i
     if (err) snprintf(err, errlen, "bad mesh parameters");
     goto fail;
   }
-  if (S->p.nfluids != 1) {
-    if (err) snprintf(err, errlen, "this build runs one fluid per simulation (NFLUIDS=%d)", S->p.nfluids);
+  if (S->p.nfluids < 1 || S->p.nfluids > F3D_MAX_FLUIDS) {
+    if (err) snprintf(err, errlen, "NFLUIDS=%d: 1 to %d fluids", S->p.nfluids, F3D_MAX_FLUIDS);
     goto fail;
+  }
+  S->nfl = S->p.nfluids;
+  /* fluid 0 is the gas, the others dust (setups/fargo_multifluid/condinit.c:58-76) */
+  for (int k = 0; k < S->nfl; k++) S->ftype[k] = k == 0 ? F3D_GAS : F3D_DUST;
+  /* ColRate(INVSTOKESk, 0, k, feedback) (src/colrate.c:10-23) */
+  for (int k = 1; k < S->nfl; k++) {
+    S->alpha[0 + k * S->nfl] = S->p.invstokes[k];
+    S->alpha[k + 0 * S->nfl] = S->p.feedback ? S->p.invstokes[k] : 0.0;
   }
   S->prec32 = (S->p.precision == 32);
   S->es = S->prec32 ? sizeof(float) : sizeof(double);
@@ -232,17 +269,20 @@ f3d_sim *f3d_sim_create(const char *parfile, const char *overrides, char *err, i
     size_t n = S->g.ncell, n2 = S->g.n2d;
     int bad = 0;
     f3d_fields *f = &S->f;
-    f->Density = dalloc(n, &bad); f->Energy = dalloc(n, &bad);
-    f->Vx = dalloc(n, &bad); f->Vy = dalloc(n, &bad);
-    f->Vx_temp = dalloc(n, &bad); f->Vy_temp = dalloc(n, &bad);
-    if (S->g.dimz) { f->Vz = dalloc(n, &bad); f->Vz_temp = dalloc(n, &bad); }
+    for (int k = 0; k < S->nfl; k++) {
+      f3d_fields *F = &S->fl[k];
+      F->Density = dalloc(n, &bad); F->Energy = dalloc(n, &bad);
+      F->Vx = dalloc(n, &bad); F->Vy = dalloc(n, &bad);
+      F->Vx_temp = dalloc(n, &bad); F->Vy_temp = dalloc(n, &bad);
+      if (S->g.dimz) { F->Vz = dalloc(n, &bad); F->Vz_temp = dalloc(n, &bad); }
+      F->Density0 = dalloc(n2, &bad); F->Energy0 = dalloc(n2, &bad); F->Vx0 = dalloc(n2, &bad);
+      F->Vy0 = dalloc(n2, &bad); F->Vz0 = dalloc(n2, &bad);
+    }
     f->Mpx = dalloc(n, &bad); f->Mmx = dalloc(n, &bad); f->Mpy = dalloc(n, &bad); f->Mmy = dalloc(n, &bad);
     if (S->g.dimz) { f->Mpz = dalloc(n, &bad); f->Mmz = dalloc(n, &bad); }
     f->Pot = dalloc(n, &bad); f->Slope = dalloc(n, &bad); f->DivRho = dalloc(n, &bad);
     f->DensStar = dalloc(n, &bad); f->Qs = dalloc(n, &bad); f->Pressure = dalloc(n, &bad);
     f->Total_Density = dalloc(n, &bad); f->QL = dalloc(n, &bad); f->QR = dalloc(n, &bad);
-    f->Density0 = dalloc(n2, &bad); f->Energy0 = dalloc(n2, &bad); f->Vx0 = dalloc(n2, &bad);
-    f->Vy0 = dalloc(n2, &bad); f->Vz0 = dalloc(n2, &bad);
     /* ring sums stay FP64 in both forms */
     f->Reduction2D = dalloc_bytes(n2 * sizeof(double), &bad); f->VxMed = dalloc_bytes(n2 * sizeof(double), &bad);
     {
@@ -255,42 +295,62 @@ f3d_sim *f3d_sim_create(const char *parfile, const char *overrides, char *err, i
       goto fail;
     }
     /* fields start at zero (src/allocate.c:140-158) */
-    double **all[] = {&f->Density, &f->Energy, &f->Vx, &f->Vy, &f->Vz, &f->Vx_temp, &f->Vy_temp, &f->Vz_temp,
-                      &f->Mpx, &f->Mmx, &f->Mpy, &f->Mmy, &f->Mpz, &f->Mmz, &f->Pot, &f->Slope, &f->DivRho,
+    for (int k = 0; k < S->nfl; k++) {
+      f3d_fields *F = &S->fl[k];
+      double *own3[] = {F->Density, F->Energy, F->Vx, F->Vy, F->Vz, F->Vx_temp, F->Vy_temp, F->Vz_temp};
+      for (size_t a = 0; a < sizeof(own3) / sizeof(own3[0]); a++)
+        if (own3[a]) f3d_hip_memset(S->ctx, own3[a], 0, n * S->es);
+      double *own2[] = {F->Density0, F->Energy0, F->Vx0, F->Vy0, F->Vz0};
+      for (size_t a = 0; a < sizeof(own2) / sizeof(own2[0]); a++) f3d_hip_memset(S->ctx, own2[a], 0, n2 * S->es);
+    }
+    double **all[] = {&f->Mpx, &f->Mmx, &f->Mpy, &f->Mmy, &f->Mpz, &f->Mmz, &f->Pot, &f->Slope, &f->DivRho,
                       &f->DensStar, &f->Qs, &f->Pressure, &f->Total_Density, &f->QL, &f->QR};
     for (size_t a = 0; a < sizeof(all) / sizeof(all[0]); a++)
       if (*all[a]) f3d_hip_memset(S->ctx, *all[a], 0, n * S->es);
-    double **all2[] = {&f->Density0, &f->Energy0, &f->Vx0, &f->Vy0, &f->Vz0};
-    for (size_t a = 0; a < sizeof(all2) / sizeof(all2[0]); a++) f3d_hip_memset(S->ctx, *all2[a], 0, n2 * S->es);
     f3d_hip_memset(S->ctx, f->Reduction2D, 0, n2 * sizeof(double));
     f3d_hip_memset(S->ctx, f->VxMed, 0, n2 * sizeof(double));
     f3d_hip_memset(S->ctx, f->Nshift, 0, n2 * sizeof(int));
-    f3d_hip_bind(S->ctx, f);
+    select_fluid(S, 0);
   }
   {
-    /* CondInit on the host, then upload (src/main.c:265) */
+    /* CondInit of every fluid on the host, then upload (src/main.c:265) */
     size_t n = S->g.ncell;
     double *h = (double *)calloc(5 * n, sizeof(double));
-    f3d_initial_conditions(&S->g, &S->p, &S->clk, h, h + n, h + 2 * n, h + 3 * n, S->g.dimz ? h + 4 * n : NULL);
-    int bad = upload(S, S->f.Density, h, n) | upload(S, S->f.Energy, h + n, n) |
-              upload(S, S->f.Vx, h + 2 * n, n) | upload(S, S->f.Vy, h + 3 * n, n);
-    if (S->g.dimz) bad |= upload(S, S->f.Vz, h + 4 * n, n);
+    int bad = 0;
+    for (int k = 0; k < S->nfl && !bad; k++) {
+      const f3d_fields *F = &S->fl[k];
+      f3d_initial_conditions_fluid(&S->g, &S->p, &S->clk, S->ftype[k], h, h + n, h + 2 * n, h + 3 * n,
+                                   S->g.dimz ? h + 4 * n : NULL);
+      bad = upload(S, F->Density, h, n) | upload(S, F->Energy, h + n, n) | upload(S, F->Vx, h + 2 * n, n) |
+            upload(S, F->Vy, h + 3 * n, n);
+      if (S->g.dimz) bad |= upload(S, F->Vz, h + 4 * n, n);
+    }
     free(h);
     if (bad) {
       if (err) snprintf(err, errlen, "upload of the initial conditions failed");
       goto fail;
     }
   }
-  if (f3d_sim_push_phys(S) != 0) {
-    if (err) snprintf(err, errlen, "%s", S->err);
-    goto fail;
+  /* FillGhosts of every fluid, then init_stockholm of every fluid (src/main.c:311-316) */
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1 && !S->p.stockholm) break;
+    for (int k = 0; k < S->nfl; k++) {
+      select_fluid(S, k);
+      if (f3d_sim_push_phys(S) != 0) {
+        if (err) snprintf(err, errlen, "%s", S->err);
+        goto fail;
+      }
+      int rc = pass == 0 ? (S->prec32 ? f3d32_fused_boundaries(S->ctx) : f3d_boundaries(S->ctx))
+                         : (S->prec32 ? f3d32_fused_init_stockholm(S->ctx) : f3d_init_stockholm(S->ctx));
+      if (rc != 0) {
+        if (err) snprintf(err, errlen, "%s", f3d_hip_last_error(S->ctx));
+        goto fail;
+      }
+    }
   }
-  /* FillGhosts then init_stockholm (src/main.c:311-316) */
-  if ((S->prec32 ? f3d32_fused_boundaries(S->ctx) : f3d_boundaries(S->ctx)) != 0 ||
-      (S->p.stockholm &&
-       (S->prec32 ? f3d32_fused_init_stockholm(S->ctx) : f3d_init_stockholm(S->ctx)) != 0) ||
-      f3d_hip_sync(S->ctx) != 0) {
-    if (err) snprintf(err, errlen, "%s", f3d_hip_last_error(S->ctx));
+  select_fluid(S, 0);
+  if (f3d_sim_push_phys(S) != 0 || f3d_hip_sync(S->ctx) != 0) {
+    if (err) snprintf(err, errlen, "%s", S->err[0] ? S->err : f3d_hip_last_error(S->ctx));
     goto fail;
   }
   return S;
@@ -302,11 +362,15 @@ fail:
 void f3d_sim_destroy(f3d_sim *S) {
   if (!S) return;
   if (S->ctx) f3d_hip_sync(S->ctx);
+  for (int k = 0; k < S->nfl; k++) {
+#define FREE1(m) \
+  if (S->fl[k].m) f3d_hip_free(S->fl[k].m);
+    F3D_FLUID_MEMBERS(FREE1)
+#undef FREE1
+  }
   f3d_fields *f = &S->f;
-  void *all[] = {f->Density, f->Energy, f->Vx, f->Vy, f->Vz, f->Vx_temp, f->Vy_temp, f->Vz_temp, f->Mpx, f->Mmx,
-                 f->Mpy, f->Mmy, f->Mpz, f->Mmz, f->Pot, f->Slope, f->DivRho, f->DensStar, f->Qs, f->Pressure,
-                 f->Total_Density, f->QL, f->QR, f->Density0, f->Energy0, f->Vx0, f->Vy0, f->Vz0, f->Reduction2D,
-                 f->VxMed, f->Nshift};
+  void *all[] = {f->Mpx, f->Mmx, f->Mpy, f->Mmy, f->Mpz, f->Mmz, f->Pot, f->Slope, f->DivRho, f->DensStar, f->Qs,
+                 f->Pressure, f->Total_Density, f->QL, f->QR, f->Reduction2D, f->VxMed, f->Nshift};
   for (size_t a = 0; a < sizeof(all) / sizeof(all[0]); a++)
     if (all[a]) f3d_hip_free(all[a]);
   if (S->ctx) f3d_hip_destroy(S->ctx);
@@ -543,25 +607,54 @@ static int transport_routines(f3d_sim *S, double dt) {
   return 0;
 }
 
+/* Collisions(dt, 1) and DustDiffusion_Main(dt) (src/main.c:412-418) */
+static int fluid_coupling(f3d_sim *S, double dt) {
+  if (S->nfl < 2) return 0;
+  f3d_hip_ctx *c = S->ctx;
+  double *rho[F3D_MAX_FLUIDS], *vxt[F3D_MAX_FLUIDS], *vyt[F3D_MAX_FLUIDS], *vzt[F3D_MAX_FLUIDS];
+  for (int k = 0; k < S->nfl; k++) {
+    rho[k] = S->fl[k].Density;
+    vxt[k] = S->fl[k].Vx_temp;
+    vyt[k] = S->fl[k].Vy_temp;
+    vzt[k] = S->fl[k].Vz_temp;
+  }
+  if (S->p.dragforce)
+    CALL(S, S->prec32 ? f3d32_fused_collisions(c, dt, S->nfl, rho, vxt, vyt, vzt, S->alpha)
+                      : f3d_fused_collisions(c, dt, S->nfl, rho, vxt, vyt, vzt, S->alpha));
+  if (S->p.dustdiffusion)
+    CALL(S, S->prec32 ? f3d32_fused_dust_diffusion(c, dt, S->nfl, rho, S->ftype, S->fl[0].Energy)
+                      : f3d_fused_dust_diffusion(c, dt, S->nfl, rho, S->ftype, S->fl[0].Energy));
+  return 0;
+}
+
 int f3d_sim_step(f3d_sim *S, double *dt_out) {
   f3d_hip_ctx *c = S->ctx;
   const double DT = S->p.dt_out;
+  const int nfl = S->nfl;
   if (S->dtemp >= DT) S->dtemp = 0.0;
-  if (f3d_sim_push_phys(S)) return -1;
-  double fluid_min;
   if (S->prec32 && !(S->arch.fused_cfl && S->arch.fused_sources && S->arch.fused_transport))
     ERR(S, "the FP32 form runs the fused stages only");
-  if (S->prec32) {
-    CALL(S, f3d32_fused_cfl(c, &fluid_min));
-  } else if (S->arch.fused_cfl) {
-    CALL(S, f3d_fused_cfl(c, &fluid_min));
-  } else {
-    if (S->p.fargo) CALL(S, f3d_ComputeVmed(c, S->f.Vx));
-    CALL(S, f3d_cfl(c, &fluid_min));
-  }
-  /* CflFluidsMin (src/cfl_fluids_min.c:3-22) */
+  /* MULTIFLUID(ComputeVmed(Vx)); MULTIFLUID(cfl()) (src/main.c:366-381): every
+     fluid's CFL reads the VxMed of the last fluid */
   double step = 1e30;
-  if (step > fluid_min) step = fluid_min;
+  if (!S->arch.fused_cfl && S->p.fargo)
+    for (int k = 0; k < nfl; k++) {
+      select_fluid(S, k);
+      CALL(S, f3d_ComputeVmed(c, S->f.Vx));
+    }
+  for (int k = 0; k < nfl; k++) {
+    select_fluid(S, k);
+    if (f3d_sim_push_phys(S)) return -1;
+    double fluid_min;
+    if (S->prec32)
+      CALL(S, f3d32_fused_cfl_vm(c, S->fl[nfl - 1].Vx, &fluid_min));
+    else if (S->arch.fused_cfl)
+      CALL(S, f3d_fused_cfl_vm(c, S->fl[nfl - 1].Vx, &fluid_min));
+    else
+      CALL(S, f3d_cfl(c, &fluid_min));
+    /* CflFluidsMin (src/cfl_fluids_min.c:3-22) */
+    if (step > fluid_min) step = fluid_min;
+  }
   if (S->dt_force > 0.0) {
     step = S->dt_force;
     S->dt_force = 0.0;
@@ -578,30 +671,52 @@ int f3d_sim_step(f3d_sim *S, double *dt_out) {
   for (int k = 0; k < S->sys.nb; k++) feel |= S->sys.feeldisk[k];
   if (!S->prec32 && (!S->arch.fused_sources || (S->p.potential && feel))) {
     CALL(S, f3d_Reset_field(c, S->f.Total_Density));
-    CALL(S, f3d_ComputeTotalDensity(c));
-  }
-  if (S->arch.fused_sources) {
-    if (S->p.potential) {
-      if (compute_potential(S, dt)) return -1;
+    for (int k = 0; k < nfl; k++) {
+      select_fluid(S, k);
+      CALL(S, f3d_ComputeTotalDensity(c));
     }
+  }
+  /* MULTIFLUID(Sources(dt)) (src/main.c:410) */
+  for (int k = 0; k < nfl; k++) {
+    select_fluid(S, k);
+    if (S->arch.fused_sources) {
+      if (S->p.potential) {
+        if (compute_potential(S, dt)) return -1;
+      }
+      if (f3d_sim_push_phys(S)) return -1;
+      const double dom = (S->p.potential && S->p.corotating) ? S->clk.Domega : 0.0;
+      CALL(S, S->prec32 ? f3d32_fused_sources(c, dt, dom) : f3d_fused_sources(c, dt, dom));
+    } else {
+      if (f3d_sim_push_phys(S)) return -1;
+      if (sources_routines(S, dt)) return -1;
+    }
+  }
+  if (fluid_coupling(S, dt)) return -1;
+  /* MULTIFLUID(Transport(dt)), then MULTIFLUID(StockholmBoundary(dt)) and
+     MULTIFLUID(FillGhosts) (src/main.c:420-434); the fused transport ends with
+     the damping and the ghost fill of its own fluid */
+  for (int k = 0; k < nfl; k++) {
+    select_fluid(S, k);
+    if (S->arch.fused_transport) {
+      CALL(S, S->prec32 ? f3d32_fused_transport(c, dt, 1) : f3d_fused_transport(c, dt, 1));
+    } else {
+      if (transport_routines(S, dt)) return -1;
+    }
+  }
+  S->clk.PhysicalTime += dt;
+  S->clk.Timestepcount++;
+  if (!S->arch.fused_transport) {
     if (f3d_sim_push_phys(S)) return -1;
-    const double dom = (S->p.potential && S->p.corotating) ? S->clk.Domega : 0.0;
-    CALL(S, S->prec32 ? f3d32_fused_sources(c, dt, dom) : f3d_fused_sources(c, dt, dom));
-  } else {
-    if (sources_routines(S, dt)) return -1;
+    for (int k = 0; k < nfl; k++) {
+      select_fluid(S, k);
+      if (S->p.stockholm) CALL(S, f3d_StockholmBoundary(c, dt));
+    }
+    for (int k = 0; k < nfl; k++) {
+      select_fluid(S, k);
+      if (fill_ghosts(S)) return -1;
+    }
   }
-  if (S->arch.fused_transport) {
-    /* transport, NewVelocity, Stockholm damping and the ghost fill */
-    CALL(S, S->prec32 ? f3d32_fused_transport(c, dt, 1) : f3d_fused_transport(c, dt, 1));
-    S->clk.PhysicalTime += dt;
-    S->clk.Timestepcount++;
-  } else {
-    if (transport_routines(S, dt)) return -1;
-    S->clk.PhysicalTime += dt;
-    S->clk.Timestepcount++;
-    if (S->p.stockholm) CALL(S, f3d_StockholmBoundary(c, dt));
-    if (fill_ghosts(S)) return -1;
-  }
+  select_fluid(S, 0);
   S->dt_last = dt;
   if (dt_out) *dt_out = dt;
   return 0;
@@ -689,13 +804,13 @@ int f3d_sim_get_nshift(f3d_sim *S, int *host) {
   return 0;
 }
 
-void f3d_sim_get_scalars(f3d_sim *S, double *o) {
+void f3d_sim_get_scalars(f3d_sim *S, double o[8]) {
   o[0] = S->clk.PhysicalTime; o[1] = S->clk.OMEGAFRAME; o[2] = S->clk.OMEGAFRAME0;
   o[3] = S->clk.Domega; o[4] = S->dtemp; o[5] = S->StepTime; o[6] = S->clk.XAxisRotationAngle;
   o[7] = (double)S->clk.Timestepcount;
 }
 
-void f3d_sim_set_scalars(f3d_sim *S, const double *in) {
+void f3d_sim_set_scalars(f3d_sim *S, const double in[8]) {
   S->clk.PhysicalTime = in[0]; S->clk.OMEGAFRAME = in[1]; S->clk.OMEGAFRAME0 = in[2];
   S->clk.Domega = in[3]; S->dtemp = in[4]; S->StepTime = in[5]; S->clk.XAxisRotationAngle = in[6];
   S->clk.Timestepcount = (long)in[7];

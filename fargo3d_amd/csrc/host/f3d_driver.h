@@ -52,6 +52,10 @@ void f3d_sim_set_scalars(f3d_sim *S, const double in[8]);
 void f3d_sim_get_planets(f3d_sim *S, double *out, int maxn);
 void f3d_sim_shape(f3d_sim *S, int out[4]); /* nx, ny, nz, dimz */
 int f3d_sim_eos(f3d_sim *S);                 /* 1 isothermal, 2 adiabatic */
+/* SelectFluid (src/LowTasks.c:565): field access, set_arch-independent calls and
+ * f3d_sim_ctx's bound fields then refer to fluid n (0 = gas). */
+int f3d_sim_nfluids(f3d_sim *S);
+int f3d_sim_select_fluid(f3d_sim *S, int n);
 double f3d_sim_dt_out(f3d_sim *S);           /* DT, the output interval the step clamps to */
 f3d_hip_ctx *f3d_sim_ctx(f3d_sim *S);
 /* refresh the kernels' copy of the run parameters and host scalars */

@@ -212,6 +212,29 @@ int f3d_fused_set_xmode(int mode);
 int f3d_fused_potential(f3d_hip_ctx *ctx);
 int f3d_fused_boundaries(f3d_hip_ctx *ctx);
 int f3d_fused_init_stockholm(f3d_hip_ctx *ctx);
+/* f3d_fused_cfl with the ring sums (ComputeVmed, src/compute_vmed.c:14-30)
+ * taken over vxm instead of the bound fluid's Vx: with several fluids the
+ * reference runs MULTIFLUID(ComputeVmed(Vx)) before MULTIFLUID(cfl())
+ * (src/main.c:366-381), so every fluid's cfl reads the last fluid's VxMed.
+ * vxm NULL means the bound Vx. */
+int f3d_fused_cfl_vm(f3d_hip_ctx *ctx, const double *vxm, double *fluid_min);
+
+/* ---- coupling of several fluids (src/main.c:412-418) ---- */
+/* Collisions(dt, 1): src/collisions.c:156 (+ _collisions_cpu :10, ColRate
+ * src/colrate.c:10).  Implicit drag on the V_temp fields of nf (2..8) fluids,
+ * one direction after the other; alpha is the ColRate matrix, alpha[i + j*nf]
+ * the rate fluid i feels from fluid j (the STOKESNUMBER form: multiplied by
+ * the local Keplerian frequency).  rho/vxt/vyt/vzt hold each fluid's device
+ * fields (vzt ignored in 2-D). */
+int f3d_fused_collisions(f3d_hip_ctx *ctx, double dt, int nf, double *const *rho, double *const *vxt,
+                         double *const *vyt, double *const *vzt, const double *alpha);
+/* DustDiffusion_Main(dt): src/dust_diffusion_main.c:3 (+ the coefficients,
+ * src/dust_diffusion_coefficients.c:10, and the core,
+ * src/dust_diffusion_core.c:10).  Diffuses the density of every fluid whose
+ * type is 2 (dust) against the gas density rho[0]; cs_gas is the gas
+ * Energy (sound speed) field.  2-D cylindrical meshes. */
+int f3d_fused_dust_diffusion(f3d_hip_ctx *ctx, double dt, int nf, double *const *rho, const int *type,
+                             const double *cs_gas);
 
 /* ---- reduced-precision form (no counterpart in the reference): the fused
  * stages on FP32 fields (every 3-D field and the damping references bound to
@@ -226,6 +249,11 @@ int f3d32_fused_boundaries(f3d_hip_ctx *ctx);
 int f3d32_fused_init_stockholm(f3d_hip_ctx *ctx);
 int f3d32_fused_set_tiling(int rows_y, int rows_z);
 int f3d32_fused_set_xmode(int mode);
+int f3d32_fused_cfl_vm(f3d_hip_ctx *ctx, const double *vxm, double *fluid_min);
+int f3d32_fused_collisions(f3d_hip_ctx *ctx, double dt, int nf, double *const *rho, double *const *vxt,
+                           double *const *vyt, double *const *vzt, const double *alpha);
+int f3d32_fused_dust_diffusion(f3d_hip_ctx *ctx, double dt, int nf, double *const *rho, const int *type,
+                               const double *cs_gas);
 
 #ifdef __cplusplus
 }
