@@ -26,6 +26,7 @@ os.chdir(ROOT)
 HBM_PEAK_GBS = 8000.0
 PAR = {"c1": "fargo3d_amd/setups/c1_fargo2d.par", "c2": "fargo3d_amd/setups/c2_p3d.par",
        "c3": "fargo3d_amd/setups/c3_adiabatic2d.par"}
+PAR_MULTIFLUID = "fargo3d_amd/setups/c4_multifluid2d.par"
 WORKLOAD = {
     "c1": "2-D polar isothermal disk 384x128, FP64, FARGO, nu=1e-5, 1e-3 Mstar planet",
     "c2": "3-D spherical isothermal disk 256x96x32, FP64, FARGO, alpha=1e-3, 3e-4 Mstar planet",
@@ -234,6 +235,31 @@ def batched_form(fa, n, steps, loop_steps):
                         "mass, viscosity and aspect ratio; planets held on their orbits" % n}
 
 
+def multifluid_form(fa, L, names, steps, warmup):
+    """The 2-D disk of one gas and four dust fluids (1024x512, implicit drag,
+    dust diffusion), FP64 and FP32: fluid-cell-updates/s (active cells x
+    fluids x steps / wall time).  Bytes per cell: the gas reads and writes
+    rho, vx, vy and reads cs (7 arrays), each dust fluid reads and writes rho,
+    vx, vy (6 arrays)."""
+    out = {"workload": "2-D polar disk, gas + 4 dust fluids (St 1e-3..1), 1024x512, FARGO, implicit drag with "
+                       "feedback, dust diffusion (alpha 1e-4), 1e-3 Mstar planet"}
+    for tag, ov, es in (("fp64", "", 8), ("fp32", "Precision=32", 4)):
+        el, st, sim = timed_run(fa, L, names, PAR_MULTIFLUID, ov, {}, steps, warmup)
+        nf = sim.nfluids
+        cells = sim.nx * sim.ny
+        sim.close()
+        v = cells * nf * steps / el
+        b_cell = es * (7 + 6 * (nf - 1))
+        gbs = cells * steps / el * b_cell / 1e9
+        out[tag] = {"value": round(v, 1), "unit": "fluid-cell-updates/s", "fluids": nf, "cells": cells,
+                    "steps": steps, "ms_per_step": round(1e3 * el / steps, 4),
+                    "step_roofline": {"bytes_per_cell": b_cell, "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)},
+                    "stages": st}
+    out["fp32"]["speedup_vs_fp64"] = round(out["fp32"]["value"] / out["fp64"]["value"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,6 +272,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the reduced-precision form")
     ap.add_argument("--batch", type=int, default=64, help="instances of the batched form (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--multifluid-steps", type=int, default=100,
+                    help="timed steps of the multifluid form (0: skip it)")
     ap.add_argument("--error-steps", type=int, default=200,
                     help="steps of the paired FP64/FP32 run that measures the FP32 form's error")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -306,6 +334,9 @@ def main():
     if not args.routines and args.batch > 0:
         forms = forms or {}
         forms["batched"] = batched_form(fa, args.batch, args.batch_steps, 10)
+    if not args.routines and args.multifluid_steps > 0:
+        forms = forms or {}
+        forms["multifluid"] = multifluid_form(fa, L, names, args.multifluid_steps, 10)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -49,6 +49,7 @@ struct Scratch {
   real* f[FS_COUNT];
   unsigned long long* result;  // two slots, alternated per step
   int slot;
+  double* ringmin;             // per-ring CFL minima of up to 8 fluids (two-pass MIN)
   dmesh m;                     // the build's mesh view
   mreal* mesh_r;               // FP32 build: converted mesh arrays
   real* bctab_r;               // FP32 build: converted boundary tables
@@ -62,6 +63,7 @@ void scratch_free(void* p) {
   for (int a = 0; a < FS_COUNT; a++)
     if (S->f[a]) (void)hipFree(S->f[a]);
   if (S->result) (void)hipFree(S->result);
+  if (S->ringmin) (void)hipFree(S->ringmin);
   if (S->mesh_r) (void)hipFree(S->mesh_r);
   if (S->bctab_r) (void)hipFree(S->bctab_r);
   delete S;
@@ -130,6 +132,8 @@ int scratch_of(f3d_hip_ctx* ctx, Scratch** out) {
       if (hipMemset(S->f[a], 0, ctx->n3 * sizeof(real)) != hipSuccess) return -1;
     }
     if (hipMalloc(&S->result, 2 * sizeof(unsigned long long)) != hipSuccess) return -1;
+    const size_t nrings = (size_t)ctx->m.ny * (ctx->m.dimz ? ctx->m.nz : 1);
+    if (hipMalloc(&S->ringmin, 8 * nrings * sizeof(double)) != hipSuccess) return -1;
     unsigned long long init[2] = {kInfBits, kInfBits};
     if (hipMemcpy(S->result, init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess) return -1;
   }
@@ -193,7 +197,7 @@ __device__ __forceinline__ double seq_sum(const double* w, int n) {
 // ring on the bit pattern of the positive double gives the value of
 // reduction_full_MIN + cfl_b (src/cfl_b.c:6-37) exactly.
 template <int G, int DZ>
-__device__ __forceinline__ void kd_cfl(const dim3 bid, const dmesh& m, const real* cs, const real* rho, const real* vx, const real* vy, const real* vz, const real* vxm, double* vxmed, unsigned long long* result, int slot, cell::CflPar cp) {
+__device__ __forceinline__ void kd_cfl(const dim3 bid, const dmesh& m, const real* cs, const real* rho, const real* vx, const real* vy, const real* vz, const real* vxm, double* vxmed, unsigned long long* result, int slot, cell::CflPar cp, double* ringmin) {
   extern __shared__ double lds[];  // 2 nx weights + 16
   double* w0 = lds;
   double* w1 = lds + m.nx;
@@ -229,14 +233,38 @@ __device__ __forceinline__ void kd_cfl(const dim3 bid, const dmesh& m, const rea
       const double shearlimit = cp.cfl * m.mindx / fabs(vm0 / m.ymed64[j] - sh[9] / m.ymed64[j + 1]);
       if (shearlimit < mn) mn = shearlimit;
     }
-    atomicMin(&result[slot], (unsigned long long)__double_as_longlong(mn));
+    if (ringmin) ringmin[bid.x] = mn;
+    else atomicMin(&result[slot], (unsigned long long)__double_as_longlong(mn));
   }
 }
 template <int G, int DZ>
 __global__ void __launch_bounds__(WG) kf_cfl(dmesh m, const real* cs, const real* rho, const real* vx,
                                              const real* vy, const real* vz, const real* vxm, double* vxmed,
-                                             unsigned long long* result, int slot, cell::CflPar cp) {
-  kd_cfl<G, DZ>(blockIdx, m, cs, rho, vx, vy, vz, vxm, vxmed, result, slot, cp);
+                                             unsigned long long* result, int slot, cell::CflPar cp,
+                                             double* ringmin) {
+  kd_cfl<G, DZ>(blockIdx, m, cs, rho, vx, vy, vz, vxm, vxmed, result, slot, cp, ringmin);
+}
+
+// Second pass of the two-pass MIN: one workgroup takes the MIN of the n ring
+// minima (order independent, so equal to the atomic form) into result[slot].
+__global__ void __launch_bounds__(1024) kf_cfl_fin(const double* ringmin, int n, unsigned long long* result,
+                                                   int slot) {
+  __shared__ double sh[16];
+  double v = __longlong_as_double((long long)kInfBits);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double x = ringmin[i];
+    v = (x < v) ? x : v;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double w = __shfl_down(v, o, 64);
+    v = (w < v) ? w : v;
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) v = (sh[w] < v) ? sh[w] : v;
+    result[slot] = (unsigned long long)__double_as_longlong(v);
+  }
 }
 
 // ------------------------------------------------------------------ sources
@@ -1419,6 +1447,9 @@ static int fused_boundaries(f3d_hip_ctx* ctx, Scratch* S, real* dens, real* vx, 
 static int g_rows_y = 4, g_rows_z = 3;
 // X-sweep kernel choice: 0 automatic, 1 ring in LDS, 2 tiles
 static int g_xmode = 0;
+// CFL MIN: 0 one atomic MIN per ring, 1 ring minima to an array and a
+// one-workgroup second pass
+static int g_cflmode = 0;
 
 // NewVelocity_y/z, Stockholm damping and the end-of-step ghost fill
 static int finish_step(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step) {
@@ -1481,6 +1512,12 @@ static int x_sweep_tiled(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_ste
 
 extern "C" {
 
+int F3D_API(fused_set_cflmode)(int mode) {
+  if (mode < 0 || mode > 1) return -2;
+  g_cflmode = mode;
+  return 0;
+}
+
 int F3D_API(fused_set_xmode)(int mode) {
   if (mode < 0 || mode > 2) return -2;
   g_xmode = mode;
@@ -1508,13 +1545,51 @@ int F3D_API(fused_cfl_vm)(f3d_hip_ctx* ctx, const double* vxm_in, double* fluid_
   const size_t shmem = (2 * m.nx + 16) * sizeof(double);
   const int slot = S->slot;
   f3d_stage_begin(ctx, F3D_ST_CFL);
-  FDISPATCH(ctx, kf_cfl, dim3(nrings), dim3(WG), shmem, f.Energy, f.Density, f.Vx, f.Vy, f.Vz, vxm ? vxm : f.Vx, f.VxMed, S->result,
-            slot, cfl_par_f(ctx));
+  double* rm = g_cflmode ? S->ringmin : nullptr;
+  FDISPATCH(ctx, kf_cfl, dim3(nrings), dim3(WG), shmem, f.Energy, f.Density, f.Vx, f.Vy, f.Vz, vxm ? vxm : f.Vx,
+            f.VxMed, S->result, slot, cfl_par_f(ctx), rm);
+  if (rm) {
+    hipLaunchKernelGGL(kf_cfl_fin, dim3(1), dim3(1024), 0, ctx->stream, rm, nrings, S->result, slot);
+    F3D_LAUNCH_CHECK(ctx);
+  }
   f3d_stage_end(ctx, F3D_ST_CFL);
   F3D_CHECK(ctx, hipMemcpyAsync(ctx->hres, S->result + slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   F3D_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   S->slot ^= 1;
   *fluid_min = ctx->hres[0];
+  return 0;
+}
+
+// cfl of nf fluids with one read-back: every launch takes its MIN into the
+// same slot (MIN is order independent, so the result is CflFluidsMin's)
+int F3D_API(fused_cfl_fluids)(f3d_hip_ctx* ctx, int nf, double* const* cs, double* const* rho, double* const* vx,
+                              double* const* vy, double* const* vz, const double* vxm_in, double* step_min) {
+  Scratch* S;
+  F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
+  F3D_REQUIRE(ctx, nf >= 1 && nf <= 8, "cfl: 1 to 8 fluids");
+  const dmesh& m = S->m;
+  const RF f = rfields(ctx);
+  const int nrings = m.ny * (m.dimz ? m.nz : 1);
+  const size_t shmem = (2 * m.nx + 16) * sizeof(double);
+  const int slot = S->slot;
+  const real* vxm = reinterpret_cast<const real*>(vxm_in);
+  auto R = [](double* p) { return reinterpret_cast<real*>(p); };
+  f3d_stage_begin(ctx, F3D_ST_CFL);
+  for (int k = 0; k < nf; k++) {
+    real* vzk = m.dimz ? R(vz[k]) : R(vy[k]);
+    double* rm = g_cflmode ? S->ringmin + (size_t)k * nrings : nullptr;
+    FDISPATCH(ctx, kf_cfl, dim3(nrings), dim3(WG), shmem, R(cs[k]), R(rho[k]), R(vx[k]), R(vy[k]), vzk, vxm, f.VxMed,
+              S->result, slot, cfl_par_f(ctx), rm);
+  }
+  if (g_cflmode) {
+    hipLaunchKernelGGL(kf_cfl_fin, dim3(1), dim3(1024), 0, ctx->stream, S->ringmin, nf * nrings, S->result, slot);
+    F3D_LAUNCH_CHECK(ctx);
+  }
+  f3d_stage_end(ctx, F3D_ST_CFL);
+  F3D_CHECK(ctx, hipMemcpyAsync(ctx->hres, S->result + slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  F3D_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  S->slot ^= 1;
+  *step_min = ctx->hres[0];
   return 0;
 }
 

@@ -642,18 +642,32 @@ int f3d_sim_step(f3d_sim *S, double *dt_out) {
       select_fluid(S, k);
       CALL(S, f3d_ComputeVmed(c, S->f.Vx));
     }
-  for (int k = 0; k < nfl; k++) {
-    select_fluid(S, k);
+  if (S->arch.fused_cfl || S->prec32) {
+    /* every fluid's cfl into one device MIN, one read-back */
+    double *cs[F3D_MAX_FLUIDS], *rho[F3D_MAX_FLUIDS], *vx[F3D_MAX_FLUIDS], *vy[F3D_MAX_FLUIDS], *vz[F3D_MAX_FLUIDS];
+    for (int k = 0; k < nfl; k++) {
+      cs[k] = S->fl[k].Energy;
+      rho[k] = S->fl[k].Density;
+      vx[k] = S->fl[k].Vx;
+      vy[k] = S->fl[k].Vy;
+      vz[k] = S->fl[k].Vz;
+    }
     if (f3d_sim_push_phys(S)) return -1;
-    double fluid_min;
+    double fluids_min;
     if (S->prec32)
-      CALL(S, f3d32_fused_cfl_vm(c, S->fl[nfl - 1].Vx, &fluid_min));
-    else if (S->arch.fused_cfl)
-      CALL(S, f3d_fused_cfl_vm(c, S->fl[nfl - 1].Vx, &fluid_min));
+      CALL(S, f3d32_fused_cfl_fluids(c, nfl, cs, rho, vx, vy, vz, S->fl[nfl - 1].Vx, &fluids_min));
     else
+      CALL(S, f3d_fused_cfl_fluids(c, nfl, cs, rho, vx, vy, vz, S->fl[nfl - 1].Vx, &fluids_min));
+    step = fluids_min;
+  } else {
+    for (int k = 0; k < nfl; k++) {
+      select_fluid(S, k);
+      if (f3d_sim_push_phys(S)) return -1;
+      double fluid_min;
       CALL(S, f3d_cfl(c, &fluid_min));
-    /* CflFluidsMin (src/cfl_fluids_min.c:3-22) */
-    if (step > fluid_min) step = fluid_min;
+      /* CflFluidsMin (src/cfl_fluids_min.c:3-22) */
+      if (step > fluid_min) step = fluid_min;
+    }
   }
   if (S->dt_force > 0.0) {
     step = S->dt_force;

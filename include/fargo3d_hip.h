@@ -206,6 +206,10 @@ int f3d_fused_set_tiling(int rows_y, int rows_z);
  * 128 cells with halos), 1 ring, 2 tiles; results do not depend on it.
  * Returns -2 for other values. */
 int f3d_fused_set_xmode(int mode);
+/* CFL MIN over rings: 0 one 64-bit atomic MIN per ring, 1 ring minima to an
+ * array and a one-workgroup second pass; the result is the same value.
+ * Returns -2 for other values. */
+int f3d_fused_set_cflmode(int mode);
 /* Potential, the ghost fill and the damping references on the fused stages'
  * scratch-free kernels (the same formulas as f3d_Potential, f3d_boundaries,
  * f3d_init_stockholm) */
@@ -218,6 +222,11 @@ int f3d_fused_init_stockholm(f3d_hip_ctx *ctx);
  * (src/main.c:366-381), so every fluid's cfl reads the last fluid's VxMed.
  * vxm NULL means the bound Vx. */
 int f3d_fused_cfl_vm(f3d_hip_ctx *ctx, const double *vxm, double *fluid_min);
+/* cfl of nf fluids (cs = each fluid's Energy) and CflFluidsMin
+ * (src/cfl_fluids_min.c:3-22) with one device MIN and one read-back; the ring
+ * sums are taken over vxm as in f3d_fused_cfl_vm (vz ignored in 2-D). */
+int f3d_fused_cfl_fluids(f3d_hip_ctx *ctx, int nf, double *const *cs, double *const *rho, double *const *vx,
+                         double *const *vy, double *const *vz, const double *vxm, double *step_min);
 
 /* ---- coupling of several fluids (src/main.c:412-418) ---- */
 /* Collisions(dt, 1): src/collisions.c:156 (+ _collisions_cpu :10, ColRate
@@ -249,7 +258,10 @@ int f3d32_fused_boundaries(f3d_hip_ctx *ctx);
 int f3d32_fused_init_stockholm(f3d_hip_ctx *ctx);
 int f3d32_fused_set_tiling(int rows_y, int rows_z);
 int f3d32_fused_set_xmode(int mode);
+int f3d32_fused_set_cflmode(int mode);
 int f3d32_fused_cfl_vm(f3d_hip_ctx *ctx, const double *vxm, double *fluid_min);
+int f3d32_fused_cfl_fluids(f3d_hip_ctx *ctx, int nf, double *const *cs, double *const *rho, double *const *vx,
+                           double *const *vy, double *const *vz, const double *vxm, double *step_min);
 int f3d32_fused_collisions(f3d_hip_ctx *ctx, double dt, int nf, double *const *rho, double *const *vxt,
                            double *const *vyt, double *const *vzt, const double *alpha);
 int f3d32_fused_dust_diffusion(f3d_hip_ctx *ctx, double dt, int nf, double *const *rho, const int *type,
