@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(WG) kb_cfl(const BInst* I, const int* off, int
   const int ii = find_inst(off, n, blockIdx.x);
   const BInst& in = I[ii];
   kd_cfl<CYL, 0>(dim3(blockIdx.x - off[ii]), in.m, in.cs, in.rho, in.vx, in.vy, in.vy, in.vx, in.vxmed, in.res, 0,
-                 in.cp, nullptr);
+                 in.cp, nullptr, 0);
 }
 
 // CflFluidsMin and the output-time clamp (src/cfl_fluids_min.c:3-22,

@@ -167,21 +167,34 @@ __device__ double block_max(double v, double* sh) {
 }
 
 // Sequential sum in index order (the reduction order of src/reduction_generic.c
-// :18-20), in FP64; eight LDS loads are issued ahead of their dependent additions.
+// :18-20), in FP64.  Software-pipelined: the LDS loads of the next 16 terms
+// are issued before the 16 dependent additions of the current ones, so the
+// chain of additions, not the LDS latency, sets the pace.
 __device__ __forceinline__ double seq_sum(const double* w, int n) {
+  constexpr int B = 16;
   double acc = 0.0;
   int i = 0;
-  for (; i + 8 <= n; i += 8) {
-    double t0 = w[i], t1 = w[i + 1], t2 = w[i + 2], t3 = w[i + 3];
-    double t4 = w[i + 4], t5 = w[i + 5], t6 = w[i + 6], t7 = w[i + 7];
-    acc = acc + t0;
-    acc = acc + t1;
-    acc = acc + t2;
-    acc = acc + t3;
-    acc = acc + t4;
-    acc = acc + t5;
-    acc = acc + t6;
-    acc = acc + t7;
+  if (n >= 2 * B) {
+    // ping-pong between two register batches: a holds terms [i-B, i)
+    double a[B], b[B];
+#pragma unroll
+    for (int q = 0; q < B; q++) a[q] = w[q];
+    for (i = B; i + 2 * B <= n; i += 2 * B) {
+      // the scheduling barriers keep each batch's loads ahead of the other
+      // batch's additions
+#pragma unroll
+      for (int q = 0; q < B; q++) b[q] = w[i + q];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < B; q++) acc = acc + a[q];
+#pragma unroll
+      for (int q = 0; q < B; q++) a[q] = w[i + B + q];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < B; q++) acc = acc + b[q];
+    }
+#pragma unroll
+    for (int q = 0; q < B; q++) acc = acc + a[q];
   }
   for (; i < n; i++) acc = acc + w[i];
   return acc;
@@ -197,27 +210,36 @@ __device__ __forceinline__ double seq_sum(const double* w, int n) {
 // ring on the bit pattern of the positive double gives the value of
 // reduction_full_MIN + cfl_b (src/cfl_b.c:6-37) exactly.
 template <int G, int DZ>
-__device__ __forceinline__ void kd_cfl(const dim3 bid, const dmesh& m, const real* cs, const real* rho, const real* vx, const real* vy, const real* vz, const real* vxm, double* vxmed, unsigned long long* result, int slot, cell::CflPar cp, double* ringmin) {
-  extern __shared__ double lds[];  // 2 nx weights + 16
+__device__ __forceinline__ void kd_cfl(const dim3 bid, const dmesh& m, const real* cs, const real* rho, const real* vx, const real* vy, const real* vz, const real* vxm, double* vxmed, unsigned long long* result, int slot, cell::CflPar cp, double* ringmin,
+    int pre) {
+  extern __shared__ double lds[];  // 2 nx weights + 16 (16 with pre)
   double* w0 = lds;
   double* w1 = lds + m.nx;
-  double* sh = lds + 2 * m.nx;
+  double* sh = pre ? lds : lds + 2 * m.nx;
   const int j = m.nghy + (int)(bid.x % m.ny);
   const int k = DZ ? m.nghz + (int)(bid.x / m.ny) : 0;
   const int base = j * m.pitch + k * m.stride;
   const int l2 = j + k * m.sy;
   if (bid.x == 0 && threadIdx.x == 0) result[slot ^ 1] = kInfBits;  // the next step's slot
-  // ComputeVweight + reduction_SUM (src/compute_vmed.c:14-30) of rings j and j+1
-  // of vxm (the velocity whose VxMed the reference's cfl reads: with several
-  // fluids, the last fluid's, src/main.c:366 then :381)
-  for (int i = threadIdx.x; i < m.nx; i += blockDim.x) {
-    const double dx = m.xmin64[i + 1] - m.xmin64[i];
-    w0[i] = (double)vxm[base + i] * dx;
-    if (cp.fargo) w1[i] = (double)vxm[base + m.pitch + i] * dx;
+  if (pre) {
+    // VxMed of rings j and j+1 from the pass before (kf_vmed_rings)
+    if (threadIdx.x == 0) {
+      sh[8] = vxmed[l2];
+      sh[9] = vxmed[l2 + 1];
+    }
+  } else {
+    // ComputeVweight + reduction_SUM (src/compute_vmed.c:14-30) of rings j and j+1
+    // of vxm (the velocity whose VxMed the reference's cfl reads: with several
+    // fluids, the last fluid's, src/main.c:366 then :381)
+    for (int i = threadIdx.x; i < m.nx; i += blockDim.x) {
+      const double dx = m.xmin64[i + 1] - m.xmin64[i];
+      w0[i] = (double)vxm[base + i] * dx;
+      if (cp.fargo) w1[i] = (double)vxm[base + m.pitch + i] * dx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) sh[8] = seq_sum(w0, m.nx) / (m.XMAX - m.XMIN);
+    if (threadIdx.x == 64 && cp.fargo) sh[9] = seq_sum(w1, m.nx) / (m.XMAX - m.XMIN);
   }
-  __syncthreads();
-  if (threadIdx.x == 0) sh[8] = seq_sum(w0, m.nx) / (m.XMAX - m.XMIN);
-  if (threadIdx.x == 64 && cp.fargo) sh[9] = seq_sum(w1, m.nx) / (m.XMAX - m.XMIN);
   __syncthreads();
   const double vm0 = sh[8];
   double mx = 0.0;
@@ -241,8 +263,23 @@ template <int G, int DZ>
 __global__ void __launch_bounds__(WG) kf_cfl(dmesh m, const real* cs, const real* rho, const real* vx,
                                              const real* vy, const real* vz, const real* vxm, double* vxmed,
                                              unsigned long long* result, int slot, cell::CflPar cp,
-                                             double* ringmin) {
-  kd_cfl<G, DZ>(blockIdx, m, cs, rho, vx, vy, vz, vxm, vxmed, result, slot, cp, ringmin);
+                                             double* ringmin, int pre) {
+  kd_cfl<G, DZ>(blockIdx, m, cs, rho, vx, vy, vz, vxm, vxmed, result, slot, cp, ringmin, pre);
+}
+
+// ComputeVmed (src/compute_vmed.c:14-30) of rings j = NGHY .. NGHY+nry-1 of
+// every active plane: one workgroup per ring, the weights in LDS, one
+// sequential sum in i order.
+template <int G, int DZ>
+__global__ void __launch_bounds__(WG) kf_vmed_rings(dmesh m, const real* v, double* vxmed, int nry) {
+  extern __shared__ double w[];
+  const int j = m.nghy + (int)(blockIdx.x % nry);
+  const int k = DZ ? m.nghz + (int)(blockIdx.x / nry) : 0;
+  const int base = j * m.pitch + k * m.stride;
+  for (int i = threadIdx.x; i < m.nx; i += blockDim.x)
+    w[i] = (double)v[base + i] * (m.xmin64[i + 1] - m.xmin64[i]);
+  __syncthreads();
+  if (threadIdx.x == 0) vxmed[j + k * m.sy] = seq_sum(w, m.nx) / (m.XMAX - m.XMIN);
 }
 
 // Second pass of the two-pass MIN: one workgroup takes the MIN of the n ring
@@ -1447,9 +1484,27 @@ static int fused_boundaries(f3d_hip_ctx* ctx, Scratch* S, real* dens, real* vx, 
 static int g_rows_y = 4, g_rows_z = 3;
 // X-sweep kernel choice: 0 automatic, 1 ring in LDS, 2 tiles
 static int g_xmode = 0;
-// CFL MIN: 0 one atomic MIN per ring, 1 ring minima to an array and a
-// one-workgroup second pass
-static int g_cflmode = 0;
+// CFL kernel form, bit 0: ring minima to an array and a one-workgroup
+// second MIN pass (else one atomic MIN per ring); bit 1: the ring sums of
+// VxMed in a pass of their own (else two sums inside each ring's workgroup).
+// -1: chosen from the mesh shape.
+static int g_cflmode = -1;
+
+static int cfl_mode(const dmesh& m) {
+  if (g_cflmode >= 0) return g_cflmode;
+  const int nrings = m.ny * (m.dimz ? m.nz : 1);
+  return (nrings >= 2048 ? 1 : 0) | (m.nx >= 1024 ? 2 : 0);
+}
+
+// VxMed of every active ring and of the ring above the last (the shear limit
+// reads ring j+1), for the CFL kernel's pre-pass form
+static int vmed_prepass(f3d_hip_ctx* ctx, Scratch* S, const real* v) {
+  const dmesh& m = S->m;
+  const int nry = m.ny + (ctx->ph.fargo ? 1 : 0);
+  const int nr = nry * (m.dimz ? m.nz : 1);
+  FDISPATCH(ctx, kf_vmed_rings, dim3(nr), dim3(WG), m.nx * sizeof(double), v, ctx->f.VxMed, nry);
+  return 0;
+}
 
 // NewVelocity_y/z, Stockholm damping and the end-of-step ghost fill
 static int finish_step(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step) {
@@ -1513,7 +1568,7 @@ static int x_sweep_tiled(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_ste
 extern "C" {
 
 int F3D_API(fused_set_cflmode)(int mode) {
-  if (mode < 0 || mode > 1) return -2;
+  if (mode < -1 || mode > 3) return -2;
   g_cflmode = mode;
   return 0;
 }
@@ -1545,9 +1600,12 @@ int F3D_API(fused_cfl_vm)(f3d_hip_ctx* ctx, const double* vxm_in, double* fluid_
   const size_t shmem = (2 * m.nx + 16) * sizeof(double);
   const int slot = S->slot;
   f3d_stage_begin(ctx, F3D_ST_CFL);
-  double* rm = g_cflmode ? S->ringmin : nullptr;
-  FDISPATCH(ctx, kf_cfl, dim3(nrings), dim3(WG), shmem, f.Energy, f.Density, f.Vx, f.Vy, f.Vz, vxm ? vxm : f.Vx,
-            f.VxMed, S->result, slot, cfl_par_f(ctx), rm);
+  const int mode = cfl_mode(m);
+  double* rm = (mode & 1) ? S->ringmin : nullptr;
+  const int pre = (mode & 2) ? 1 : 0;
+  if (pre && vmed_prepass(ctx, S, vxm ? vxm : f.Vx)) return -1;
+  FDISPATCH(ctx, kf_cfl, dim3(nrings), dim3(WG), pre ? 16 * sizeof(double) : shmem, f.Energy, f.Density, f.Vx, f.Vy,
+            f.Vz, vxm ? vxm : f.Vx, f.VxMed, S->result, slot, cfl_par_f(ctx), rm, pre);
   if (rm) {
     hipLaunchKernelGGL(kf_cfl_fin, dim3(1), dim3(1024), 0, ctx->stream, rm, nrings, S->result, slot);
     F3D_LAUNCH_CHECK(ctx);
@@ -1575,13 +1633,16 @@ int F3D_API(fused_cfl_fluids)(f3d_hip_ctx* ctx, int nf, double* const* cs, doubl
   const real* vxm = reinterpret_cast<const real*>(vxm_in);
   auto R = [](double* p) { return reinterpret_cast<real*>(p); };
   f3d_stage_begin(ctx, F3D_ST_CFL);
+  const int mode = cfl_mode(m);
+  const int pre = (mode & 2) ? 1 : 0;
+  if (pre && vmed_prepass(ctx, S, vxm)) return -1;
   for (int k = 0; k < nf; k++) {
     real* vzk = m.dimz ? R(vz[k]) : R(vy[k]);
-    double* rm = g_cflmode ? S->ringmin + (size_t)k * nrings : nullptr;
-    FDISPATCH(ctx, kf_cfl, dim3(nrings), dim3(WG), shmem, R(cs[k]), R(rho[k]), R(vx[k]), R(vy[k]), vzk, vxm, f.VxMed,
-              S->result, slot, cfl_par_f(ctx), rm);
+    double* rm = (mode & 1) ? S->ringmin + (size_t)k * nrings : nullptr;
+    FDISPATCH(ctx, kf_cfl, dim3(nrings), dim3(WG), pre ? 16 * sizeof(double) : shmem, R(cs[k]), R(rho[k]), R(vx[k]),
+              R(vy[k]), vzk, vxm, f.VxMed, S->result, slot, cfl_par_f(ctx), rm, pre);
   }
-  if (g_cflmode) {
+  if (mode & 1) {
     hipLaunchKernelGGL(kf_cfl_fin, dim3(1), dim3(1024), 0, ctx->stream, S->ringmin, nf * nrings, S->result, slot);
     F3D_LAUNCH_CHECK(ctx);
   }

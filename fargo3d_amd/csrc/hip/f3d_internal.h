@@ -46,7 +46,7 @@ struct f3d_hip_ctx {
   void* fused[2] = {nullptr, nullptr};
   void (*fused_free[2])(void*) = {nullptr, nullptr};
   // per-stage event timing of the fused kernels (f3d_hip_timing_*)
-  int timing = 0;
+  unsigned timing = 0;  // bit s: time stage s
   hipEvent_t ev[F3D_NSTAGES][2] = {};
   int pending[F3D_NSTAGES] = {};
   double stage_ms[F3D_NSTAGES] = {};
@@ -56,7 +56,7 @@ struct f3d_hip_ctx {
 // Event pair around one launch of stage s when timing is on; the previous
 // pair of the same stage is complete by then (every step synchronises on dt).
 inline void f3d_stage_begin(f3d_hip_ctx* c, int s) {
-  if (!c->timing) return;
+  if (!((c->timing >> s) & 1u)) return;
   if (c->pending[s]) {
     float ms = 0.f;
     if (hipEventSynchronize(c->ev[s][1]) == hipSuccess &&
@@ -69,7 +69,7 @@ inline void f3d_stage_begin(f3d_hip_ctx* c, int s) {
   (void)hipEventRecord(c->ev[s][0], c->stream);
 }
 inline void f3d_stage_end(f3d_hip_ctx* c, int s) {
-  if (!c->timing) return;
+  if (!((c->timing >> s) & 1u)) return;
   (void)hipEventRecord(c->ev[s][1], c->stream);
   c->pending[s] = 1;
 }

@@ -87,8 +87,10 @@ static const char* kStageNames[F3D_NSTAGES] = {"cfl", "sources_a", "sources_b", 
 int f3d_hip_stage_count(void) { return F3D_NSTAGES; }
 const char* f3d_hip_stage_name(int s) { return (s >= 0 && s < F3D_NSTAGES) ? kStageNames[s] : ""; }
 
-int f3d_hip_timing_enable(f3d_hip_ctx* c, int on) {
-  if (on && !c->ev[0][0]) {
+int f3d_hip_timing_enable(f3d_hip_ctx* c, int on) { return f3d_hip_timing_mask(c, on ? ~0u : 0u); }
+
+int f3d_hip_timing_mask(f3d_hip_ctx* c, unsigned mask) {
+  if (mask && !c->ev[0][0]) {
     for (int s = 0; s < F3D_NSTAGES; s++)
       for (int e = 0; e < 2; e++) F3D_CHECK(c, hipEventCreate(&c->ev[s][e]));
   }
@@ -98,7 +100,7 @@ int f3d_hip_timing_enable(f3d_hip_ctx* c, int on) {
     c->stage_ms[s] = 0.0;
     c->stage_n[s] = 0;
   }
-  c->timing = on;
+  c->timing = mask & ((1u << F3D_NSTAGES) - 1u);
   return 0;
 }
 

@@ -101,6 +101,10 @@ int f3d_hip_device_count(void);
 int f3d_hip_stage_count(void);
 const char *f3d_hip_stage_name(int stage);
 int f3d_hip_timing_enable(f3d_hip_ctx *ctx, int on);
+/* time only the stages whose bit is set (an event pair around a stage puts
+ * a few microseconds between launches, so the benchmark times one stage in
+ * its timed region); enable(on) is mask ~0 or 0 */
+int f3d_hip_timing_mask(f3d_hip_ctx *ctx, unsigned mask);
 int f3d_hip_timing_read(f3d_hip_ctx *ctx, double *ms, long *counts);
 
 /* ---- per-routine entry points (reference pointer table, src/global.h:303-416) ---- */

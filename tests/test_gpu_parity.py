@@ -329,3 +329,22 @@ def test_whole_step_fused_tiled_x(fa, orc, par, ov, nsteps):
         _fused_run(fa, orc, par, ov, nsteps)
     finally:
         L.f3d_fused_set_xmode(0)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3], ids=["atomic", "two_pass", "vmed_pass_atomic", "vmed_pass_two_pass"])
+def test_cfl_forms(fa, orc, mode):
+    """Every form of the fused CFL kernel (atomic or two-pass MIN, VxMed summed
+    inside the ring's workgroup or in a pass of its own) gives the oracle's
+    time step bit for bit."""
+    L = fa.lib()
+    assert L.f3d_fused_set_cflmode(mode) == 0
+    try:
+        for par, ov in [(C1, ""), (C2, C2_SMALL), (C3, C3_SMALL)]:
+            o, s = make_pair(fa, orc, par, ov)
+            for n in range(3):
+                dto, dts = o.step(), s.step()
+                say("cfl form %d %s step %d dt %r" % (mode, par, n, dts))
+                assert dto == dts, (mode, par, n, dts, dto)
+            s.close()
+    finally:
+        L.f3d_fused_set_cflmode(-1)
