@@ -571,17 +571,27 @@ __device__ __forceinline__ void bound_y_cell(const dmesh& m, int side, int i, in
     jact = ny + nghy - 1 - mm;
   }
   const real of = b.of;
+  // every load before the first store: a thread's stores never touch what it
+  // reads, and without this order each load would wait for the stores before it
+  const bool wvy = !side || mm < nghy - 1;
+  const real d0 = dens[lact], x0 = vx[lact];
+  const real z0 = b.p3d ? vz[lact] : real(0.0);
+  const real y0 = wvy ? vy[lacts] : real(0.0);
+  const real e0 = b.energy ? e[lact] : real(0.0);
+  real dg, xg;
   if (!b.p3d) {
-    dens[lgh] = dens[lact] * b.dens[mm];
-    vx[lgh] = (vx[lact] + m.ymed[jact] * of) * b.vsq[mm] - m.ymed[jgh] * of;
+    dg = d0 * b.dens[mm];
+    xg = (x0 + m.ymed[jact] * of) * b.vsq[mm] - m.ymed[jgh] * of;
   } else {
-    dens[lgh] = dens[lact] * b.dens[mm] * dexp[mm * m.sz + k];
-    vx[lgh] = (vx[lact] + of * m.ymed[jact] * m.sinzmed[k]) * b.vsq[mm] * b.vfac[mm] - m.ymed[jgh] * of * m.sinzmed[k];
-    vz[lgh] = vz[lact];
+    dg = d0 * b.dens[mm] * dexp[mm * m.sz + k];
+    xg = (x0 + of * m.ymed[jact] * m.sinzmed[k]) * b.vsq[mm] * b.vfac[mm] - m.ymed[jgh] * of * m.sinzmed[k];
   }
-  if (!side || mm < nghy - 1) vy[lghs] = -vy[lacts];
+  dens[lgh] = dg;
+  vx[lgh] = xg;
+  if (b.p3d) vz[lgh] = z0;
+  if (wvy) vy[lghs] = -y0;
   vy[lnull] = 0;
-  if (b.energy) e[lgh] = e[lact];
+  if (b.energy) e[lgh] = e0;
 }
 
 // Z sides (scripts/boundparser.py:341-358; std/boundaries.txt:31-40, 87-91)
@@ -605,12 +615,19 @@ __device__ __forceinline__ void bound_z_cell(const dmesh& m, int side, int i, in
     kact = nz + nghz - 1 - mm;
   }
   const real of = b.of;
-  dens[lgh] = dens[lact] * zdens[mm * m.sy + j];
-  vx[lgh] = (vx[lact] + of * m.ymed[j] * m.sinzmed[kact]) * b.zvfac[mm] - m.ymed[j] * of * m.sinzmed[kgh];
-  vy[lgh] = vy[lact];
-  if (!side || mm < nghz - 1) vz[lghs] = -vz[lacts];
+  // every load before the first store (see bound_y_cell)
+  const bool wvz = !side || mm < nghz - 1;
+  const real d0 = dens[lact], x0 = vx[lact], y0 = vy[lact];
+  const real z0 = wvz ? vz[lacts] : real(0.0);
+  const real e0 = b.energy ? e[lact] : real(0.0);
+  const real dg = d0 * zdens[mm * m.sy + j];
+  const real xg = (x0 + of * m.ymed[j] * m.sinzmed[kact]) * b.zvfac[mm] - m.ymed[j] * of * m.sinzmed[kgh];
+  dens[lgh] = dg;
+  vx[lgh] = xg;
+  vy[lgh] = y0;
+  if (wvz) vz[lghs] = -z0;
   vz[lnull] = 0;
-  if (b.energy) e[lgh] = e[lact];
+  if (b.energy) e[lgh] = e0;
 }
 
 }  // namespace cell

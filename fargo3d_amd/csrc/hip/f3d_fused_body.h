@@ -260,10 +260,10 @@ __device__ __forceinline__ void kd_cfl(const dim3 bid, const dmesh& m, const rea
   }
 }
 template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_cfl(dmesh m, const real* cs, const real* rho, const real* vx,
-                                             const real* vy, const real* vz, const real* vxm, double* vxmed,
-                                             unsigned long long* result, int slot, cell::CflPar cp,
-                                             double* ringmin, int pre) {
+__global__ void __launch_bounds__(WG) kf_cfl(dmesh m, const real* __restrict__ cs, const real* __restrict__ rho, const real* __restrict__ vx,
+                                             const real* __restrict__ vy, const real* __restrict__ vz, const real* __restrict__ vxm, double* __restrict__ vxmed,
+                                             unsigned long long* __restrict__ result, int slot, cell::CflPar cp,
+                                             double* __restrict__ ringmin, int pre) {
   kd_cfl<G, DZ>(blockIdx, m, cs, rho, vx, vy, vz, vxm, vxmed, result, slot, cp, ringmin, pre);
 }
 
@@ -271,7 +271,7 @@ __global__ void __launch_bounds__(WG) kf_cfl(dmesh m, const real* cs, const real
 // every active plane: one workgroup per ring, the weights in LDS, one
 // sequential sum in i order.
 template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_vmed_rings(dmesh m, const real* v, double* vxmed, int nry) {
+__global__ void __launch_bounds__(WG) kf_vmed_rings(dmesh m, const real* __restrict__ v, double* __restrict__ vxmed, int nry) {
   extern __shared__ double w[];
   const int j = m.nghy + (int)(blockIdx.x % nry);
   const int k = DZ ? m.nghz + (int)(blockIdx.x / nry) : 0;
@@ -371,10 +371,10 @@ __device__ __forceinline__ void kd_src_a(const dim3 bid, const dmesh& m, const r
   if (DZ) qz[l] = cell::art_q(r0, VZ(0, 0, 1) - VZ(0, 0, 0));
 }
 template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_src_a(dmesh m, const real* rho, const real* energy, const real* vx,
-                                               const real* vy, const real* vz, real* txx, real* tyy,
-                                               real* tzz, real* txz, real* tyx, real* tzy, real* qx,
-                                               real* qy, real* qz, cell::ViscPar vp, int visc, real domega) {
+__global__ void __launch_bounds__(WG) kf_src_a(dmesh m, const real* __restrict__ rho, const real* __restrict__ energy, const real* __restrict__ vx,
+                                               const real* __restrict__ vy, const real* __restrict__ vz, real* __restrict__ txx, real* __restrict__ tyy,
+                                               real* __restrict__ tzz, real* __restrict__ txz, real* __restrict__ tyx, real* __restrict__ tzy, real* __restrict__ qx,
+                                               real* __restrict__ qy, real* __restrict__ qz, cell::ViscPar vp, int visc, real domega) {
   kd_src_a<G, DZ>(blockIdx, m, rho, energy, vx, vy, vz, txx, tyy, tzz, txz, tyx, tzy, qx, qy, qz, vp, visc, domega);
 }
 
@@ -411,12 +411,12 @@ __device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt
   if (!iso) e2[l] = e;
 }
 template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_src_b(dmesh m, real dt, const real* rho, const real* energy,
-                                               const real* vx, const real* vy, const real* vz,
-                                               const real* pot, const real* txx, const real* tyy,
-                                               const real* tzz, const real* txz, const real* tyx,
-                                               const real* tzy, const real* qx, const real* qy,
-                                               const real* qz, real* vxt, real* vyt, real* vzt, real* e2,
+__global__ void __launch_bounds__(WG) kf_src_b(dmesh m, real dt, const real* __restrict__ rho, const real* __restrict__ energy,
+                                               const real* __restrict__ vx, const real* __restrict__ vy, const real* __restrict__ vz,
+                                               const real* __restrict__ pot, const real* __restrict__ txx, const real* __restrict__ tyy,
+                                               const real* __restrict__ tzz, const real* __restrict__ txz, const real* __restrict__ tyx,
+                                               const real* __restrict__ tzy, const real* __restrict__ qx, const real* __restrict__ qy,
+                                               const real* __restrict__ qz, real* __restrict__ vxt, real* __restrict__ vyt, real* __restrict__ vzt, real* __restrict__ e2,
                                                int visc, int potential, int dust, int iso, real gm1, real omegaframe,
                                                real domega) {
   kd_src_b<G, DZ>(blockIdx, m, dt, rho, energy, vx, vy, vz, pot, txx, tyy, tzz, txz, tyx, tzy, qx, qy, qz, vxt, vyt, vzt, e2, visc, potential, dust, iso, gm1, omegaframe, domega);
@@ -425,8 +425,8 @@ __global__ void __launch_bounds__(WG) kf_src_b(dmesh m, real dt, const real* rho
 // SubStep3 (src/substep3.c:98-110) on active cells: compression work with the
 // divergence of V_temp; reads the energy after SubStep2_b.
 template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_substep3(dmesh m, real dt, const real* e2, const real* vxt,
-                                                  const real* vyt, const real* vzt, real* energy,
+__global__ void __launch_bounds__(WG) kf_substep3(dmesh m, real dt, const real* __restrict__ e2, const real* __restrict__ vxt,
+                                                  const real* __restrict__ vyt, const real* __restrict__ vzt, real* __restrict__ energy,
                                                   real gamma) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
@@ -617,20 +617,20 @@ __device__ __forceinline__ void kd_sweep_y2(const dim3 bid, const dmesh& m, real
   column_sweep<4 + AD>(col, r0, r1, dt);
 }
 template <int G, int DZ, int AD>
-__global__ void __launch_bounds__(BXF) kf_sweep_y2(dmesh m, real dt, const real* rho_in, const real* vxt,
-                                                   const real* vyt, const real* e_in, real* rho_out,
-                                                   real* q0, real* q1, real* q2, real* q3, real* e_out,
+__global__ void __launch_bounds__(BXF) kf_sweep_y2(dmesh m, real dt, const real* __restrict__ rho_in, const real* __restrict__ vxt,
+                                                   const real* __restrict__ vyt, const real* __restrict__ e_in, real* __restrict__ rho_out,
+                                                   real* __restrict__ q0, real* __restrict__ q1, real* __restrict__ q2, real* __restrict__ q3, real* __restrict__ e_out,
                                                    int rows_per, real of) {
   kd_sweep_y2<G, DZ, AD>(blockIdx, m, dt, rho_in, vxt, vyt, e_in, rho_out, q0, q1, q2, q3, e_out, rows_per, of);
 }
 
 // 3-D: Y sweep of the Z-swept fields
 template <int G, int DZ, int AD>
-__global__ void __launch_bounds__(BXF) kf_sweep_y3(dmesh m, real dt, const real* rho_in, const real* z0,
-                                                   const real* z1, const real* z2, const real* z3,
-                                                   const real* z4, const real* z5, const real* ze,
-                                                   const real* vyt, real* rho_out, real* q0, real* q1,
-                                                   real* q2, real* q3, real* q4, real* q5, real* e_out,
+__global__ void __launch_bounds__(BXF) kf_sweep_y3(dmesh m, real dt, const real* __restrict__ rho_in, const real* __restrict__ z0,
+                                                   const real* __restrict__ z1, const real* __restrict__ z2, const real* __restrict__ z3,
+                                                   const real* __restrict__ z4, const real* __restrict__ z5, const real* __restrict__ ze,
+                                                   const real* __restrict__ vyt, real* __restrict__ rho_out, real* __restrict__ q0, real* __restrict__ q1,
+                                                   real* __restrict__ q2, real* __restrict__ q3, real* __restrict__ q4, real* __restrict__ q5, real* __restrict__ e_out,
                                                    int rows_per) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
@@ -644,10 +644,10 @@ __global__ void __launch_bounds__(BXF) kf_sweep_y3(dmesh m, real dt, const real*
 
 // 3-D: momenta + Z sweep for rows j in [1, sy-2] (the Y sweep's stencil)
 template <int G, int DZ, int AD>
-__global__ void __launch_bounds__(BXF) kf_sweep_z(dmesh m, real dt, const real* rho_in, const real* vxt,
-                                                  const real* vyt, const real* vzt, const real* e_in,
-                                                  real* rho_out, real* q0, real* q1, real* q2, real* q3,
-                                                  real* q4, real* q5, real* e_out, int planes_per, real of) {
+__global__ void __launch_bounds__(BXF) kf_sweep_z(dmesh m, real dt, const real* __restrict__ rho_in, const real* __restrict__ vxt,
+                                                  const real* __restrict__ vyt, const real* __restrict__ vzt, const real* __restrict__ e_in,
+                                                  real* __restrict__ rho_out, real* __restrict__ q0, real* __restrict__ q1, real* __restrict__ q2, real* __restrict__ q3,
+                                                  real* __restrict__ q4, real* __restrict__ q5, real* __restrict__ e_out, int planes_per, real of) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int j = 1 + (int)blockIdx.y;
@@ -666,11 +666,11 @@ __global__ void __launch_bounds__(BXF) kf_sweep_z(dmesh m, real dt, const real* 
 // momenta in the order of src/transport.c (Mpx, Mmx, Mpy, Mmy[, Mpz, Mmz]),
 // then with AD the energy (e_in -> e_out), then the density.
 template <int G, int DZ, int AD, int CPT>
-__global__ void __launch_bounds__(256) kf_sweep_x(dmesh m, double dt64, const real* rho_in, const real* vxt,
-                                                   real* mpx, real* mmx, real* mpy, real* mmy, real* mpz,
-                                                   real* mmz, const real* e_in, real* e_out, real* rho_out,
-                                                   real* vx_out, real* mpy_o, real* mmy_o, real* mpz_o,
-                                                   real* mmz_o, double* vxmed_out, int* nshift_out,
+__global__ void __launch_bounds__(256) kf_sweep_x(dmesh m, double dt64, const real* __restrict__ rho_in, const real* __restrict__ vxt,
+                                                   real* __restrict__ mpx, real* __restrict__ mmx, real* __restrict__ mpy, real* __restrict__ mmy, real* __restrict__ mpz,
+                                                   real* __restrict__ mmz, const real* __restrict__ e_in, real* __restrict__ e_out, real* __restrict__ rho_out,
+                                                   real* __restrict__ vx_out, real* __restrict__ mpy_o, real* __restrict__ mmy_o, real* __restrict__ mpz_o,
+                                                   real* __restrict__ mmz_o, double* __restrict__ vxmed_out, int* __restrict__ nshift_out,
                                                    real omegaframe, int fargo) {
   constexpr int NM = DZ ? 6 : 4;  // momenta
   constexpr int NQ = NM + AD;
@@ -925,7 +925,7 @@ __device__ __forceinline__ void kd_vmed(const dim3 bid, const dmesh& m, const re
   if (threadIdx.x == 0) vxmed[j + k * m.sy] = seq_sum(w, m.nx) / (m.XMAX - m.XMIN);
 }
 template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_vmed(dmesh m, const real* vxt, double* vxmed) {
+__global__ void __launch_bounds__(WG) kf_vmed(dmesh m, const real* __restrict__ vxt, double* __restrict__ vxmed) {
   kd_vmed<G, DZ>(blockIdx, m, vxt, vxmed);
 }
 
@@ -1084,12 +1084,12 @@ __device__ __forceinline__ void kd_sweep_x_tile(const dim3 bid, const dmesh& m, 
   }
 }
 template <int G, int DZ, int AD, int T>
-__global__ void __launch_bounds__(T) kf_sweep_x_tile(dmesh m, double dt64, const real* rho_in, const real* vxt,
-                                                      const real* mpx, const real* mmx, const real* mpy,
-                                                      const real* mmy, const real* mpz, const real* mmz,
-                                                      const real* e_in, real* e_out, real* rho_out,
-                                                      real* vx_out, real* mpy_o, real* mmy_o, real* mpz_o,
-                                                      real* mmz_o, const double* vxmed, int* nshift_out,
+__global__ void __launch_bounds__(T) kf_sweep_x_tile(dmesh m, double dt64, const real* __restrict__ rho_in, const real* __restrict__ vxt,
+                                                      const real* __restrict__ mpx, const real* __restrict__ mmx, const real* __restrict__ mpy,
+                                                      const real* __restrict__ mmy, const real* __restrict__ mpz, const real* __restrict__ mmz,
+                                                      const real* __restrict__ e_in, real* __restrict__ e_out, real* __restrict__ rho_out,
+                                                      real* __restrict__ vx_out, real* __restrict__ mpy_o, real* __restrict__ mmy_o, real* __restrict__ mpz_o,
+                                                      real* __restrict__ mmz_o, const double* __restrict__ vxmed, int* __restrict__ nshift_out,
                                                       real omegaframe, int fargo) {
   kd_sweep_x_tile<G, DZ, AD, T>(blockIdx, m, dt64, rho_in, vxt, mpx, mmx, mpy, mmy, mpz, mmz, e_in, e_out, rho_out, vx_out, mpy_o, mmy_o, mpz_o, mmz_o, vxmed, nshift_out, omegaframe, fargo);
 }
@@ -1116,10 +1116,10 @@ __device__ __forceinline__ void kd_finish(const dim3 bid, const dmesh& m, real d
   if (stockholm) cell::stockholm<G, DZ>(m, j, k, l, dt, rho, vx, vy, vz, rho0, vx0, vy0, vz0, dp);
 }
 template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_finish(dmesh m, real dt, const real* rho_x, const real* mmy,
-                                                const real* mpy, const real* mmz, const real* mpz,
-                                                real* rho, real* vx, real* vy, real* vz, const real* rho0,
-                                                const real* vx0, const real* vy0, const real* vz0,
+__global__ void __launch_bounds__(WG) kf_finish(dmesh m, real dt, const real* __restrict__ rho_x, const real* __restrict__ mmy,
+                                                const real* __restrict__ mpy, const real* __restrict__ mmz, const real* __restrict__ mpz,
+                                                real* __restrict__ rho, real* __restrict__ vx, real* __restrict__ vy, real* __restrict__ vz, const real* __restrict__ rho0,
+                                                const real* __restrict__ vx0, const real* __restrict__ vy0, const real* __restrict__ vz0,
                                                 cell::DampPar dp, int stockholm) {
   kd_finish<G, DZ>(blockIdx, m, dt, rho_x, mmy, mpy, mmz, mpz, rho, vx, vy, vz, rho0, vx0, vy0, vz0, dp, stockholm);
 }
@@ -1151,7 +1151,7 @@ __global__ void kf_bound_z(dmesh m, int side0, real* dens, real* vx, real* vy, r
 
 // Potential (src/potential.c:51-208) on every cell
 template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_potential(dmesh m, real* pot, cell::PlanetSet ps) {
+__global__ void __launch_bounds__(WG) kf_potential(dmesh m, real* __restrict__ pot, cell::PlanetSet ps) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int j = blockIdx.y, k = DZ ? (int)blockIdx.z : 0;
