@@ -911,6 +911,333 @@ __global__ void __launch_bounds__(256) kf_sweep_x(dmesh m, double dt64, const re
   }
 }
 
+// The ring X sweep with the cell state in registers.  Thread t owns cells
+// i = t + s*blockDim.x (s < CPT) of the ring; LDS holds q = Q/rho, the slopes
+// (then the limited qL) and the faces (then qR, then the face fluxes) of the
+// quantities this workgroup carries, plus the advection velocity.  GRP 0: one
+// workgroup per ring carries every quantity; GRP 1/2: two workgroups per ring
+// (blockIdx.x = 2*ring + grp - 1), each with the density and half of the
+// quantities, so that each needs about half the LDS and more workgroups fit
+// on a CU.  Group 1 carries Mpx, Mmx (and Mpy in 3-D) and writes the density,
+// Vx, VxMed and Nshift; group 2 carries the other momenta and the energy.
+// Every quantity goes through the same arithmetic as in the reference's
+// routine sequence, so any split gives the same bits.
+template <int DZ, int AD, int GRP>
+struct XQuant {
+  static constexpr int NM = DZ ? 6 : 4;
+  static constexpr int NQ = NM + AD;  // quantities of the whole ring
+  static constexpr int NA = DZ ? 3 : 2;  // group 1: Mpx, Mmx (, Mpy)
+  static constexpr int NL = GRP == 0 ? NQ : GRP == 1 ? NA : NQ - NA;  // carried here
+  static constexpr int first = GRP == 2 ? NA : 0;  // global index of local quantity 0
+};
+
+template <int G, int DZ, int AD, int CPT, int GRP>
+__device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt64, const real* __restrict__ rho_in,
+                                            const real* __restrict__ vxt, const real* const* qin,
+                                            const real* __restrict__ e_in, real* __restrict__ e_out,
+                                            real* __restrict__ rho_out, real* __restrict__ vx_out,
+                                            real* const* qout, double* __restrict__ vxmed_out,
+                                            int* __restrict__ nshift_out, real omegaframe, int fargo) {
+  using XQ = XQuant<DZ, AD, GRP>;
+  constexpr int NL = XQ::NL;
+  constexpr int N = NL + 1;  // + the density (local index NL)
+  constexpr int NM = XQ::NM;
+  extern __shared__ double lds_d[];
+  const int nx = m.nx;
+  const real dt = (real)dt64;
+  const int P = (N * nx + 1) & ~1;  // N * nx rounded up to 8-byte multiples
+  real* AQ = (real*)lds_d;   // q = Q/rho (or rho): N * nx
+  real* SL = AQ + P;         // slopes, then limited qL: N * nx (FP64 ring weights before the passes)
+  real* FC = SL + P;         // faces, then qR, then face fluxes: N * nx
+  real* V = FC + P;          // advection velocity: nx
+  double* sh = (double*)(((size_t)(V + nx) + 15) & ~(size_t)15);  // 4
+  double* Wt = (double*)SL;
+  const int j = m.nghy + ring % m.ny;
+  const int k = DZ ? m.nghz + ring / m.ny : 0;
+  const int base = j * m.pitch + k * m.stride;
+  const int t = threadIdx.x, nt = blockDim.x;
+  real st[CPT][N];
+  double vt[CPT];
+  real vu[CPT];
+#pragma unroll
+  for (int s = 0; s < CPT; s++) {
+    const int i = t + s * nt;
+    if (i < nx) {
+      st[s][NL] = rho_in[base + i];
+#pragma unroll
+      for (int c = 0; c < NL; c++) {
+        const int q = XQ::first + c;
+        st[s][c] = (q < NM) ? qin[q][base + i] : e_in[base + i];
+      }
+      vt[s] = vxt[base + i];
+      Wt[i] = vt[s] * (m.xmin64[i + 1] - m.xmin64[i]);
+    }
+  }
+  __syncthreads();
+  // ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30), FP64
+  if (t == 0) sh[0] = seq_sum(Wt, nx) / (m.XMAX - m.XMIN);
+  __syncthreads();
+  const double vmed = sh[0];
+  // ComputeResidual (src/compute_residual.c:68-73), FP64: cell residual into V
+  int ns = 0;
+#pragma unroll
+  for (int s = 0; s < CPT; s++) {
+    const int i = t + s * nt;
+    if (i < nx) {
+      if (fargo) {
+        const double zs = zone_size_x64<G>(m, i, j, k);
+        const double ntilde = vmed * dt64 / zs;
+        const double nround = floor(ntilde + 0.5);
+        if (i == 0) sh[1] = nround;
+        V[i] = (real)(vt[s] - vmed);
+        vu[s] = (real)((ntilde - nround) * zs / dt64);
+      } else {
+        V[i] = (real)vt[s];
+      }
+    }
+  }
+  __syncthreads();  // V complete; the ring weights in SL are no longer read
+  if (fargo) {
+    ns = (int)sh[1];
+    if (GRP != 2 && t == 0) {
+      nshift_out[j + k * m.sy] = ns;
+      vxmed_out[j + k * m.sy] = vmed;
+    }
+  }
+  // X_advection (src/transport.c:208-226), twice with FARGO
+  for (int pass = 0; pass < (fargo ? 2 : 1); pass++) {
+    const bool ppa = (pass == 1);
+    if (ppa) {
+      __syncthreads();  // the last update read V of the neighbours
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) V[i] = vu[s];
+      }
+    }
+    // q = Q/rho (DivideByRho) and, for the density, q = rho
+#pragma unroll
+    for (int s = 0; s < CPT; s++) {
+      const int i = t + s * nt;
+      if (i < nx) {
+        const real r = st[s][NL];
+#pragma unroll
+        for (int c = 0; c < NL; c++) AQ[c * nx + i] = st[s][c] / r;
+        AQ[NL * nx + i] = r;
+      }
+    }
+    __syncthreads();
+    if (!ppa) {
+      // VanLeerX_a (src/vanleer_x_a.c:60-68)
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+          const int im = ixm_of(i, nx), ip = ixp_of(i, nx);
+          const real zi = zone_size_x<G>(m, i, j, k), zp = zone_size_x<G>(m, ip, j, k);
+#pragma unroll
+          for (int c = 0; c < N; c++) {
+            const real* a = AQ + c * nx;
+            SL[c * nx + i] = vanleer_slope((a[i] - a[im]) / zi, (a[ip] - a[i]) / zp);
+          }
+        }
+      }
+      __syncthreads();
+      // VanLeerX_b (src/vanleer_x_b.c:60-65)
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+          const int im = ixm_of(i, nx);
+          const real zi = zone_size_x<G>(m, i, j, k), zm = zone_size_x<G>(m, im, j, k);
+          const real v = V[i];
+#pragma unroll
+          for (int c = 0; c < N; c++) {
+            const real* a = AQ + c * nx;
+            const real* sl = SL + c * nx;
+            FC[c * nx + i] = vanleer_face(v, dt, a[im], sl[im], zm, a[i], sl[i], zi);
+          }
+        }
+      }
+      __syncthreads();
+    } else {
+      // VanLeerX_PPA_a: monotonised-centred slope (src/fargo_ppa_a.c:58-69)
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+          const int im = ixm_of(i, nx), ip = ixp_of(i, nx);
+#pragma unroll
+          for (int c = 0; c < N; c++) {
+            const real* a = AQ + c * nx;
+            SL[c * nx + i] = cell::ppa_slope(a[im], a[i], a[ip]);
+          }
+        }
+      }
+      __syncthreads();
+      // VanLeerX_PPA_b: face value q_{i+1/2} = qR of cell i = qL of cell i+1
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+          const int ip = ixp_of(i, nx);
+#pragma unroll
+          for (int c = 0; c < N; c++) {
+            const real* a = AQ + c * nx;
+            const real* sl = SL + c * nx;
+            FC[c * nx + i] = cell::ppa_face(a[i], a[ip], sl[i], sl[ip]);
+          }
+        }
+      }
+      __syncthreads();
+      // VanLeerX_PPA_c (src/fargo_ppa_c.c): the limited (qL, qR) of each cell;
+      // qL goes to SL (slopes are no longer read), qR to FC once every
+      // neighbour has read its face
+      real qr[CPT][N];
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+          const int im = ixm_of(i, nx);
+#pragma unroll
+          for (int c = 0; c < N; c++) {
+            real qL = FC[c * nx + im], qR = FC[c * nx + i];
+            cell::ppa_limit(AQ[c * nx + i], qL, qR);
+            SL[c * nx + i] = qL;
+            qr[s][c] = qR;
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+#pragma unroll
+          for (int c = 0; c < N; c++) FC[c * nx + i] = qr[s][c];
+        }
+      }
+      __syncthreads();
+      // VanLeerX_PPA_d: parabolic average over the swept zone (src/fargo_ppa_d.c:62-70);
+      // the fluxes replace qR in FC once every neighbour has read it
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+          const int im = ixm_of(i, nx);
+          const real zi = zone_size_x<G>(m, i, j, k);
+          const real v = V[i];
+#pragma unroll
+          for (int c = 0; c < N; c++) {
+            const real* a = AQ + c * nx;
+            const real* qL = SL + c * nx;
+            const real* qR = FC + c * nx;
+            qr[s][c] = cell::ppa_flux_value(v, dt, zi, a[im], qL[im], qR[im], a[i], qL[i], qR[i]);
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+#pragma unroll
+          for (int c = 0; c < N; c++) FC[c * nx + i] = qr[s][c];
+        }
+      }
+      __syncthreads();
+    }
+    // UpdateX for every quantity and UpdateDensityX last (src/update_x.c:69-71,
+    // src/update_density_x.c:66-68), on the registers
+#pragma unroll
+    for (int s = 0; s < CPT; s++) {
+      const int i = t + s * nt;
+      if (i < nx) {
+        const int ip = ixp_of(i, nx);
+        const real v0 = V[i], v1 = V[ip];
+        const real* ds = FC + NL * nx;
+#pragma unroll
+        for (int c = 0; c < NL; c++) {
+          const real* f = FC + c * nx;
+          st[s][c] += cell::flux_x(m, i, j, k, dt, v0 * f[i] * ds[i], v1 * f[ip] * ds[ip]);
+        }
+        st[s][NL] += cell::flux_x(m, i, j, k, dt, v0 * ds[i], v1 * ds[ip]);
+      }
+    }
+  }
+  // AdvectSHIFT (src/advect_shift.c:45-58): cell i moves to i + ns (mod nx).
+  // Quantities other than the density, Mpx and Mmx go to their shifted
+  // positions straight from the registers.
+#pragma unroll
+  for (int s = 0; s < CPT; s++) {
+    const int i = t + s * nt;
+    if (i < nx) {
+      int it = (i + ns) % nx;
+      if (it < 0) it += nx;
+#pragma unroll
+      for (int c = 0; c < NL; c++) {
+        const int q = XQ::first + c;
+        if (q >= 2) {
+          if (q < NM) qout[q - 2][base + it] = st[s][c];
+          else e_out[base + it] = st[s][c];
+        }
+      }
+      if (GRP != 2) {
+        // density, Mpx, Mmx of the shifted ring into AQ for NewVelocity_x
+        // (the AQ reads of the last pass ended before its final barrier)
+        AQ[0 * nx + it] = st[s][0];
+        AQ[1 * nx + it] = st[s][1];
+        AQ[2 * nx + it] = st[s][NL];
+        rho_out[base + it] = st[s][NL];
+      }
+    }
+  }
+  if (GRP != 2) {
+    __syncthreads();
+    // NewVelocity_x (src/newvel_x.c:78-86)
+#pragma unroll
+    for (int s = 0; s < CPT; s++) {
+      const int i = t + s * nt;
+      if (i < nx) {
+        const int im = ixm_of(i, nx);
+        const real* R = AQ + 2 * nx;
+        vx_out[base + i] = cell::newvel_x<G>(m, i, j, k, R[i], R[im], AQ[1 * nx + i], AQ[0 * nx + im], omegaframe);
+      }
+    }
+  }
+}
+
+// LDS bytes of xsweep_ring for n carried quantities + the density
+inline size_t xsweep_ring_lds(int nx, int ncarried) {
+  const size_t P = ((size_t)(ncarried + 1) * nx + 1) & ~(size_t)1;
+  return (3 * P + nx) * sizeof(real) + 64;
+}
+
+template <int G, int DZ, int AD, int CPT, int SPLIT>
+__global__ void __launch_bounds__(256) kf_sweep_xr(dmesh m, double dt64, const real* __restrict__ rho_in,
+                                                    const real* __restrict__ vxt, const real* __restrict__ mpx,
+                                                    const real* __restrict__ mmx, const real* __restrict__ mpy,
+                                                    const real* __restrict__ mmy, const real* __restrict__ mpz,
+                                                    const real* __restrict__ mmz, const real* __restrict__ e_in,
+                                                    real* __restrict__ e_out, real* __restrict__ rho_out,
+                                                    real* __restrict__ vx_out, real* __restrict__ mpy_o,
+                                                    real* __restrict__ mmy_o, real* __restrict__ mpz_o,
+                                                    real* __restrict__ mmz_o, double* __restrict__ vxmed_out,
+                                                    int* __restrict__ nshift_out, real omegaframe, int fargo) {
+  const real* qin[6] = {mpx, mmx, mpy, mmy, mpz, mmz};
+  real* qout[4] = {mpy_o, mmy_o, mpz_o, mmz_o};
+  if (!SPLIT) {
+    xsweep_ring<G, DZ, AD, CPT, 0>(m, (int)blockIdx.x, dt64, rho_in, vxt, qin, e_in, e_out, rho_out, vx_out, qout,
+                                   vxmed_out, nshift_out, omegaframe, fargo);
+  } else if ((blockIdx.x & 1) == 0) {
+    xsweep_ring<G, DZ, AD, CPT, 1>(m, (int)(blockIdx.x >> 1), dt64, rho_in, vxt, qin, e_in, e_out, rho_out, vx_out,
+                                   qout, vxmed_out, nshift_out, omegaframe, fargo);
+  } else {
+    xsweep_ring<G, DZ, AD, CPT, 2>(m, (int)(blockIdx.x >> 1), dt64, rho_in, vxt, qin, e_in, e_out, rho_out, vx_out,
+                                   qout, vxmed_out, nshift_out, omegaframe, fargo);
+  }
+}
+
 // ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30) for the tiled X sweep: one
 // workgroup per active ring, the ring's weights in LDS, one sequential sum.
 template <int G, int DZ>
@@ -1484,6 +1811,9 @@ static int fused_boundaries(f3d_hip_ctx* ctx, Scratch* S, real* dens, real* vx, 
 static int g_rows_y = 4, g_rows_z = 3;
 // X-sweep kernel choice: 0 automatic, 1 ring in LDS, 2 tiles
 static int g_xmode = 0;
+// ring X sweep: 1 two workgroups per ring (half of the quantities each), 0
+// one; -1 automatic
+static int g_xsplit = -1;
 // CFL kernel form, bit 0: ring minima to an array and a one-workgroup
 // second MIN pass (else one atomic MIN per ring); bit 1: the ring sums of
 // VxMed in a pass of their own (else two sums inside each ring's workgroup).
@@ -1570,6 +1900,12 @@ extern "C" {
 int F3D_API(fused_set_cflmode)(int mode) {
   if (mode < -1 || mode > 3) return -2;
   g_cflmode = mode;
+  return 0;
+}
+
+int F3D_API(fused_set_xsplit)(int split) {
+  if (split < -1 || split > 1) return -2;
+  g_xsplit = split;
   return 0;
 }
 
@@ -1845,23 +2181,27 @@ int F3D_API(fused_transport)(f3d_hip_ctx* ctx, double dt, int end_of_step) {
     f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
   }
   // X sweep: one workgroup of at most 256 threads per active ring (up to 4
-  // cells per thread) when the ring fits in LDS, else tiles with halos
+  // cells per thread), or two per ring each carrying half of the quantities,
+  // when the ring fits in LDS, else tiles with halos
   const int ad = (ctx->ph.eos == 2);
-  const int nq1 = (m.dimz ? 6 : 4) + ad + 1;
-  const size_t shx = ((size_t)m.nx + 8) * sizeof(double) + ((size_t)5 * nq1 * m.nx + m.nx) * sizeof(real);
+  const int nq = (m.dimz ? 6 : 4) + ad, na = m.dimz ? 3 : 2;
+  const int split = (g_xsplit < 0) ? 1 : g_xsplit;
+  const size_t shx = split ? xsweep_ring_lds(m.nx, na > nq - na ? na : nq - na) : xsweep_ring_lds(m.nx, nq);
   const bool ring_fits = shx <= 160 * 1024 && m.nx <= 1024;
   if (g_xmode == 2 || (g_xmode == 0 && !ring_fits)) return x_sweep_tiled(ctx, S, dt, end_of_step);
   F3D_REQUIRE(ctx, ring_fits, "ring too long for the LDS X sweep");
   const int cpt = (m.nx <= 256) ? 1 : (m.nx <= 512) ? 2 : 4;
   const int nthreads = (((m.nx + cpt - 1) / cpt + 63) / 64) * 64;
-  dim3 gx(m.ny * (m.dimz ? m.nz : 1));
+  dim3 gx(m.ny * (m.dimz ? m.nz : 1) * (split ? 2 : 1));
   f3d_stage_begin(ctx, F3D_ST_SWEEP_X);
-#define XSWEEP1(G_, DZ_, AD_, CPT_)                                                                           \
-  hipLaunchKernelGGL((kf_sweep_x<G_, DZ_, AD_, CPT_>), gx, dim3(nthreads), shx, ctx->stream, m, dt,              \
+#define XSWEEP1(G_, DZ_, AD_, CPT_, SP_)                                                                      \
+  hipLaunchKernelGGL((kf_sweep_xr<G_, DZ_, AD_, CPT_, SP_>), gx, dim3(nthreads), shx, ctx->stream, m, dt,      \
                      S->f[FS_RHOY], f.Vx_temp, f.Mpx, f.Mmx, f.Mpy, f.Mmy, f.Mpz, f.Mmz, S->f[FS_EY], f.Energy,   \
                      S->f[FS_RHOX], f.Vx, XM[0], XM[1], XM[2], XM[3], f.VxMed, f.Nshift, of, ctx->ph.fargo)
+#define XSWEEP2(G_, DZ_, AD_, CPT_) \
+  do { if (split) XSWEEP1(G_, DZ_, AD_, CPT_, 1); else XSWEEP1(G_, DZ_, AD_, CPT_, 0); } while (0)
 #define XSWEEP(G_, DZ_, CPT_) \
-  do { if (ad) XSWEEP1(G_, DZ_, 1, CPT_); else XSWEEP1(G_, DZ_, 0, CPT_); } while (0)
+  do { if (ad) XSWEEP2(G_, DZ_, 1, CPT_); else XSWEEP2(G_, DZ_, 0, CPT_); } while (0)
 #define XSWEEP_G(G_, DZ_) \
   do { if (cpt == 1) XSWEEP(G_, DZ_, 1); else if (cpt == 2) XSWEEP(G_, DZ_, 2); else XSWEEP(G_, DZ_, 4); } while (0)
   if (m.geom == SPH) {
@@ -1871,6 +2211,7 @@ int F3D_API(fused_transport)(f3d_hip_ctx* ctx, double dt, int end_of_step) {
   }
 #undef XSWEEP_G
 #undef XSWEEP1
+#undef XSWEEP2
 #undef XSWEEP
   F3D_LAUNCH_CHECK(ctx);
   f3d_stage_end(ctx, F3D_ST_SWEEP_X);

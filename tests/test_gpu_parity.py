@@ -348,3 +348,17 @@ def test_cfl_forms(fa, orc, mode):
             s.close()
     finally:
         L.f3d_fused_set_cflmode(-1)
+
+
+@pytest.mark.parametrize("split", [0, 1], ids=["one_workgroup_per_ring", "two_workgroups_per_ring"])
+@pytest.mark.parametrize("par,ov,nsteps", [(C1, "", 12), (C2, C2_SMALL, 6), (C3, C3_SMALL, 10), (C2, C2_AD, 4)],
+                         ids=["c1_2d", "c2_3d_small", "c3_2d_adiabatic_small", "c2_3d_adiabatic_small"])
+def test_whole_step_fused_ring_split(fa, orc, par, ov, nsteps, split):
+    """The ring X sweep with every quantity in one workgroup, or split over
+    two workgroups per ring, gives the oracle's results."""
+    L = fa.lib()
+    assert L.f3d_fused_set_xsplit(split) == 0
+    try:
+        _fused_run(fa, orc, par, ov, nsteps)
+    finally:
+        L.f3d_fused_set_xsplit(-1)
