@@ -25,12 +25,14 @@ os.chdir(ROOT)
 
 HBM_PEAK_GBS = 8000.0
 PAR = {"c1": "fargo3d_amd/setups/c1_fargo2d.par", "c2": "fargo3d_amd/setups/c2_p3d.par",
-       "c3": "fargo3d_amd/setups/c3_adiabatic2d.par"}
-PAR_MULTIFLUID = "fargo3d_amd/setups/c4_multifluid2d.par"
+       "c3": "fargo3d_amd/setups/c3_adiabatic2d.par", "c4": "fargo3d_amd/setups/c4_multifluid2d.par"}
+PAR_MULTIFLUID = PAR["c4"]
 WORKLOAD = {
     "c1": "2-D polar isothermal disk 384x128, FP64, FARGO, nu=1e-5, 1e-3 Mstar planet",
     "c2": "3-D spherical isothermal disk 256x96x32, FP64, FARGO, alpha=1e-3, 3e-4 Mstar planet",
     "c3": "2-D polar adiabatic disk 4096x1024, FP64, FARGO, gamma=1.4, nu=1e-5, 1e-3 Mstar planet",
+    "c4": "2-D polar disk, gas + 4 dust fluids 1024x512, FP64, FARGO, implicit drag, dust diffusion, 1e-3 Mstar "
+          "planet (fluid-cell-updates)",
 }
 # Algorithmic bytes per active cell and launch of each fused stage: every
 # 3-D array the stage must read once and write once, at 8 bytes (FP64);
@@ -61,7 +63,8 @@ STAGE_KERNEL = {"cfl": ["kf_cfl"], "sources_a": ["kf_src_a"], "sources_b": ["kf_
                 "finish": ["kf_finish"], "potential": ["k_potential"], "substep3": ["kf_substep3"]}
 # rocprofv3 summary with the measured HBM traffic of each kernel of this workload
 PROFILE = {"c2": "profiles/fused_step_c2_3d_256x96x32.md", "c1": "profiles/fused_step_c1_2d_384x128.md",
-           "c3": "profiles/fused_step_c3_2d_adiabatic_4096x1024.md"}
+           "c3": "profiles/fused_step_c3_2d_adiabatic_4096x1024.md",
+           "c4": "profiles/fused_step_c4_multifluid_1024x512.md"}
 
 
 def measured_traffic(config, stage):
@@ -315,18 +318,22 @@ def main():
     elapsed, stages, sim = timed_run(fa, L, names, parfile, "", arch, args.steps, args.warmup)
     if args.copy_calibration > 0:
         copy_calibration(L, sim, args.copy_calibration << 20)
-    cells = sim.nx * sim.ny * (sim.nz if sim.dimz else 1)
+    cells = sim.nx * sim.ny * (sim.nz if sim.dimz else 1)  # per fluid: what one stage launch processes
+    nfl = sim.nfluids
     ad = 1 if sim.adiabatic else 0
     alpha = 1 if args.config == "c2" else 0
     dimz = sim.dimz
     sim.close()
 
-    value = cells * args.steps / elapsed
+    value = cells * nfl * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     # bytes per cell-update of the state: rho, v (, e) read and written, the
-    # sound speed read (isothermal)
+    # sound speed read (isothermal); with several fluids, per fluid-cell
+    # update: the gas's 7 arrays and each dust fluid's 6, averaged
     nv = 3 if dimz else 2
     n_arrays = 2 * (1 + nv + ad) + (0 if ad else 1)
+    if nfl > 1:
+        n_arrays = (n_arrays + (nfl - 1) * 2 * (1 + nv)) / nfl
     b_alg = 8 * n_arrays
     roofline = roofline_of(stages, cells, dimz, ad, alpha, 8, args.config)
     step_gbs = value * b_alg / 1e9
@@ -337,7 +344,7 @@ def main():
     if not args.routines and not args.no_fp32:
         e32, st32, s32 = timed_run(fa, L, names, parfile, "Precision=32", {}, args.steps, args.warmup)
         s32.close()
-        v32 = cells * args.steps / e32
+        v32 = cells * nfl * args.steps / e32
         forms = {"fp32": {"value": round(v32, 1), "ms_per_step": round(1e3 * e32 / args.steps, 4),
                           "speedup_vs_fp64": round(v32 / value, 3), "dtype": "f32 (f64 sums, residual, cfl)",
                           "step_roofline": {"bytes_per_cell_update": 4 * n_arrays,
@@ -374,7 +381,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (setup initial conditions)",
         "config": {"workload": WORKLOAD[args.config], "form": "fp64 routines" if args.routines else "fp64 fused",
-                   "cells": cells, "parallelism": "single GPU"},
+                   "cells": cells, "fluids": nfl, "parallelism": "single GPU"},
         "roofline": roofline,
         "step_roofline": {"bytes_per_cell_update": b_alg, "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
