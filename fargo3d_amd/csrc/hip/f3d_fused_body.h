@@ -2185,7 +2185,10 @@ int F3D_API(fused_transport)(f3d_hip_ctx* ctx, double dt, int end_of_step) {
   // when the ring fits in LDS, else tiles with halos
   const int ad = (ctx->ph.eos == 2);
   const int nq = (m.dimz ? 6 : 4) + ad, na = m.dimz ? 3 : 2;
-  const int split = (g_xsplit < 0) ? 1 : g_xsplit;
+  // measured on MI355X: the split pays while a thread owns one or two cells
+  // of the ring (nx <= 512); with four, the register pressure of the split
+  // costs more than the occupancy it gains
+  const int split = (g_xsplit < 0) ? (m.nx <= 512 ? 1 : 0) : g_xsplit;
   const size_t shx = split ? xsweep_ring_lds(m.nx, na > nq - na ? na : nq - na) : xsweep_ring_lds(m.nx, nq);
   const bool ring_fits = shx <= 160 * 1024 && m.nx <= 1024;
   if (g_xmode == 2 || (g_xmode == 0 && !ring_fits)) return x_sweep_tiled(ctx, S, dt, end_of_step);
