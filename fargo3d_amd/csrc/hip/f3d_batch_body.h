@@ -14,7 +14,8 @@ namespace fused {
 // one instance as the batched kernels see it (device memory)
 struct BInst {
   dmesh m;
-  real *rho, *cs, *vx, *vy, *vxt, *vyt, *mpx, *mmx, *mpy, *mmy, *pot, *rho0, *vx0, *vy0;
+  real *rho, *cs, *vx, *vy, *vxt, *vyt, *mpx, *mmx, *mpy, *mmy, *rho0, *vx0, *vy0;
+  const double* pot;  // FP64 potential
   real *rhoy, *rhox, *qx, *qy, *e2, *ey, *xm0, *xm1, *xm2, *xm3;
   double* vxmed;
   int* nshift;
@@ -324,7 +325,7 @@ f3d_batch* f3d_batch_create(int n, const char* const* parfiles, const char* cons
     BInst& b = h[i];
     b.m = S->m;
     b.rho = f.Density; b.cs = f.Energy; b.vx = f.Vx; b.vy = f.Vy; b.vxt = f.Vx_temp; b.vyt = f.Vy_temp;
-    b.mpx = f.Mpx; b.mmx = f.Mmx; b.mpy = f.Mpy; b.mmy = f.Mmy; b.pot = f.Pot;
+    b.mpx = f.Mpx; b.mmx = f.Mmx; b.mpy = f.Mpy; b.mmy = f.Mmy; b.pot = c->f.Pot;
     b.rho0 = f.Density0; b.vx0 = f.Vx0; b.vy0 = f.Vy0;
     b.rhoy = S->f[FS_RHOY]; b.rhox = S->f[FS_RHOX]; b.qx = S->f[FS_QX]; b.qy = S->f[FS_QY];
     b.e2 = S->f[FS_E2]; b.ey = S->f[FS_EY];

@@ -17,10 +17,11 @@ constexpr real CVNR = real(1.41);  // src/define.h:9
 constexpr int kMaxPlanets = 8;
 
 // Loader over a dense global array with x wrap (src/define.h:391-392).
-struct GL {
-  const real* p;
+template <class T>
+struct GLT {
+  const T* p;
   int l, i, nx, pitch, stride;
-  __device__ __forceinline__ real operator()(int di, int dj, int dk) const {
+  __device__ __forceinline__ T operator()(int di, int dj, int dk) const {
     int off = di;
     int ii = i + di;
     if (ii < 0) off += nx;
@@ -28,10 +29,21 @@ struct GL {
     return p[l + off + dj * pitch + dk * stride];
   }
 };
+using GL = GLT<real>;
 
 __device__ __forceinline__ GL gl(const real* p, const dmesh& m, int l, int i) {
   return GL{p, l, i, m.nx, m.pitch, m.stride};
 }
+// the gravitational potential is an FP64 array in every build
+__device__ __forceinline__ GLT<double> gld(const double* p, const dmesh& m, int l, int i) {
+  return GLT<double>{p, l, i, m.nx, m.pitch, m.stride};
+}
+
+// In the FP32 build the terms that cancel in a rotating disk -- the
+// centrifugal acceleration against gravity, radially and (3-D) vertically --
+// are summed in FP64 from the FP64 potential and the FP64 mesh, and only
+// their sum (of the order of the pressure support) is rounded to FP32.
+constexpr bool kFix64 = sizeof(real) == 4;
 
 // ---------------------------------------------------------------- SubStep1
 // src/substep1_x.c:112-120
@@ -57,17 +69,34 @@ __device__ __forceinline__ real substep1_y(const dmesh& m, int i, int j, int k, 
                       (m.ymed[j] - m.ymed[j - 1]);
   real v = vy(0, 0, 0);
   if (!dust) v -= dtOVERrhom * (p(0, 0, 0) - p(0, -1, 0)) / (m.ymed[j] - m.ymed[j - 1]);
-  real vphi = real(.25) * (vx(0, 0, 0) + vx(1, 0, 0) + vx(0, -1, 0) + vx(1, -1, 0));
-  if (G == SPH) {
-    vphi += m.ymin[j] * m.sinzmed[k] * omegaframe;
-    real vzz = DZ ? real(.25) * (vz(0, 0, 0) + vz(0, 0, 1) + vz(0, -1, 0) + vz(0, -1, 1)) : real(0.0);
-    v += (vphi * vphi + vzz * vzz) / m.ymin[j] * dt;
+  if constexpr (kFix64) {
+    double vphi = 0.25 * ((double)vx(0, 0, 0) + (double)vx(1, 0, 0) + (double)vx(0, -1, 0) + (double)vx(1, -1, 0));
+    double acc;
+    if (G == SPH) {
+      vphi += m.ymin64[j] * m.sinzmed64[k] * (double)omegaframe;
+      const double vzz = DZ ? 0.25 * ((double)vz(0, 0, 0) + (double)vz(0, 0, 1) + (double)vz(0, -1, 0) +
+                                      (double)vz(0, -1, 1))
+                            : 0.0;
+      acc = (vphi * vphi + vzz * vzz) / m.ymin64[j];
+    } else {
+      vphi += m.ymin64[j] * (double)omegaframe;
+      acc = vphi * vphi / m.ymin64[j];
+    }
+    if (potential) acc -= ((double)pot(0, 0, 0) - (double)pot(0, -1, 0)) / (m.ymed64[j] - m.ymed64[j - 1]);
+    return v + (real)(acc * (double)dt);
   } else {
-    vphi += m.ymin[j] * omegaframe;
-    v += vphi * vphi / m.ymin[j] * dt;
+    real vphi = real(.25) * (vx(0, 0, 0) + vx(1, 0, 0) + vx(0, -1, 0) + vx(1, -1, 0));
+    if (G == SPH) {
+      vphi += m.ymin[j] * m.sinzmed[k] * omegaframe;
+      real vzz = DZ ? real(.25) * (vz(0, 0, 0) + vz(0, 0, 1) + vz(0, -1, 0) + vz(0, -1, 1)) : real(0.0);
+      v += (vphi * vphi + vzz * vzz) / m.ymin[j] * dt;
+    } else {
+      vphi += m.ymin[j] * omegaframe;
+      v += vphi * vphi / m.ymin[j] * dt;
+    }
+    if (potential) v -= (pot(0, 0, 0) - pot(0, -1, 0)) * dt / (m.ymed[j] - m.ymed[j - 1]);
+    return v;
   }
-  if (potential) v -= (pot(0, 0, 0) - pot(0, -1, 0)) * dt / (m.ymed[j] - m.ymed[j - 1]);
-  return v;
 }
 
 // src/substep1_z.c:120-183 (CYLINDRICAL and SPHERICAL branches)
@@ -78,7 +107,16 @@ __device__ __forceinline__ real substep1_z(const dmesh& m, int i, int j, int k, 
   real dtOVERrhom = real(2.0) * dt / (rho(0, 0, 0) * (m.zmin[k + 1] - m.zmin[k]) + rho(0, 0, -1) * (m.zmin[k] - m.zmin[k - 1])) *
                       (m.zmed[k] - m.zmed[k - 1]);
   real v;
-  if (G == SPH) {
+  if (G == SPH && kFix64) {
+    double vphi = 0.25 * ((double)vx(0, 0, 0) + (double)vx(1, 0, 0) + (double)vx(0, 0, -1) + (double)vx(1, 0, -1));
+    const double smin = m.sinzmin64[k], r = m.ymed64[j];
+    vphi += r * smin * (double)omegaframe;
+    v = vz(0, 0, 0);
+    if (!dust) v -= dtOVERrhom * (p(0, 0, 0) - p(0, 0, -1)) / (m.ymed[j] * (m.zmed[k] - m.zmed[k - 1]));
+    double acc = vphi * vphi * m.coszmin64[k] / (smin * r);
+    if (potential) acc -= ((double)pot(0, 0, 0) - (double)pot(0, 0, -1)) / (r * (m.zmed64[k] - m.zmed64[k - 1]));
+    v += (real)(acc * (double)dt);
+  } else if (G == SPH) {
     real vphi = real(.25) * (vx(0, 0, 0) + vx(1, 0, 0) + vx(0, 0, -1) + vx(1, 0, -1));
     vphi += m.ymed[j] * m.sinzmin[k] * omegaframe;
     v = vz(0, 0, 0);
@@ -331,6 +369,15 @@ __device__ __forceinline__ real potential(const dmesh& m, int i, int j, int k, c
 template <int G>
 __device__ __forceinline__ void momenta_x(const dmesh& m, int i, int j, int k, real rho, real vx, real vxp,
                                           real omegaframe, real& mmx, real& mpx) {
+  if constexpr (kFix64) {
+    // angular momentum with the FP64 radius: the FP32 radius of a ring would
+    // give a fixed relative bias between rings to every transported momentum
+    const double rcyl = (G == SPH) ? m.ymed64[j] * m.sinzmed64[k] : m.ymed64[j];
+    const double of = omegaframe;
+    mmx = (real)(((double)vx + rcyl * of) * rcyl * (double)rho);
+    mpx = (real)(((double)vxp + rcyl * of) * rcyl * (double)rho);
+    return;
+  }
   if (G == SPH) {
     real rcyl = m.ymed[j] * m.sinzmed[k];
     mmx = (vx + rcyl * omegaframe) * rcyl * rho;
@@ -346,6 +393,12 @@ template <int G>
 __device__ __forceinline__ real newvel_x(const dmesh& m, int i, int j, int k, real rho, real rhoxm, real mmx,
                                            real mpxm, real omegaframe) {
   const int im = ixm_of(i, m.nx);
+  if constexpr (kFix64) {
+    const double r = (G == SPH) ? m.ymed64[j] * m.sinzmed64[k] : m.ymed64[j];
+    return (real)(((double)mul_vol(m, mmx, i, j, k) + (double)mul_vol(m, mpxm, im, j, k)) /
+                      (((double)mul_vol(m, rho, i, j, k) + (double)mul_vol(m, rhoxm, im, j, k)) * r) -
+                  (double)omegaframe * r);
+  }
   real r = (G == SPH) ? m.ymed[j] * m.sinzmed[k] : m.ymed[j];
   return (mul_vol(m, mmx, i, j, k) + mul_vol(m, mpxm, im, j, k)) /
              ((mul_vol(m, rho, i, j, k) + mul_vol(m, rhoxm, im, j, k)) * r) -

@@ -37,7 +37,7 @@ struct dmesh {
   const mreal *cosxmed, *sinxmed, *sinzmed, *coszmed, *sinzmin, *coszmin;
   // FP64 copies for the parts that stay FP64 in the reduced-precision form
   // (ring sums, residual/shift split, CFL); the same arrays in the FP64 build
-  const double *xmin64, *ymin64, *zmin64, *ymed64, *sinzmed64, *sxi64;
+  const double *xmin64, *ymin64, *zmin64, *ymed64, *sinzmed64, *sxi64, *zmed64, *sinzmin64, *coszmin64;
 };
 
 __device__ __forceinline__ int lxp_of(int l, int i, int nx) { return (i < nx - 1) ? l + 1 : l - (nx - 1); }

@@ -102,7 +102,7 @@ int make_mesh(f3d_hip_ctx* ctx, Scratch* S) {
   m.invdiffxmed = conv(d.invdiffxmed); m.cosxmed = conv(d.cosxmed); m.sinxmed = conv(d.sinxmed);
   m.sinzmed = conv(d.sinzmed); m.coszmed = conv(d.coszmed); m.sinzmin = conv(d.sinzmin); m.coszmin = conv(d.coszmin);
   m.xmin64 = d.xmin; m.ymin64 = d.ymin; m.zmin64 = d.zmin; m.ymed64 = d.ymed; m.sinzmed64 = d.sinzmed;
-  m.sxi64 = d.sxi;
+  m.sxi64 = d.sxi; m.zmed64 = d.zmed; m.sinzmin64 = d.sinzmin; m.coszmin64 = d.coszmin;
   if (ctx->bctab) {
     const size_t ny3 = 3 * (size_t)d.sz, nz3 = 3 * (size_t)d.sy, nb = 2 * ny3 + 2 * nz3;
     if (hipMalloc(&S->bctab_r, nb * sizeof(real)) != hipSuccess) return -1;
@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(WG) kf_src_a(dmesh m, const real* __restrict__
 // kf_src_b: the velocity update of SubStep1 + addviscosity + SubStep2_b on
 // active cells, in the order the reference applies them.
 template <int G, int DZ>
-__device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt, const real* rho, const real* energy, const real* vx, const real* vy, const real* vz, const real* pot, const real* txx, const real* tyy, const real* tzz, const real* txz, const real* tyx, const real* tzy, const real* qx, const real* qy, const real* qz, real* vxt, real* vyt, real* vzt, real* e2, int visc, int potential, int dust, int iso, real gm1, real omegaframe, real domega) {
+__device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt, const real* rho, const real* energy, const real* vx, const real* vy, const real* vz, const double* pot, const real* txx, const real* tyy, const real* tzz, const real* txz, const real* tyx, const real* tzy, const real* qx, const real* qy, const real* qz, real* vxt, real* vyt, real* vzt, real* e2, int visc, int potential, int dust, int iso, real gm1, real omegaframe, real domega) {
   const int i = bid.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int j = m.nghy + (int)bid.y;
@@ -389,7 +389,7 @@ __device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt
   const int l = i + j * m.pitch + k * m.stride;
   const cell::GL R = cell::gl(rho, m, l, i);
   const PresLoader P{R, cell::gl(energy, m, l, i), gm1, iso};
-  const cell::GL PT = cell::gl(potential ? pot : rho, m, l, i);
+  const cell::GLT<double> PT = cell::gld(pot, m, l, i);  // read only when potential != 0
   const auto VX = vxc<G>(vx, m, l, i, j, k, domega);
   const cell::GL VY = cell::gl(vy, m, l, i);
   const cell::GL VZ = cell::gl(DZ ? vz : vy, m, l, i);
@@ -413,7 +413,7 @@ __device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt
 template <int G, int DZ>
 __global__ void __launch_bounds__(WG) kf_src_b(dmesh m, real dt, const real* __restrict__ rho, const real* __restrict__ energy,
                                                const real* __restrict__ vx, const real* __restrict__ vy, const real* __restrict__ vz,
-                                               const real* __restrict__ pot, const real* __restrict__ txx, const real* __restrict__ tyy,
+                                               const double* __restrict__ pot, const real* __restrict__ txx, const real* __restrict__ tyy,
                                                const real* __restrict__ tzz, const real* __restrict__ txz, const real* __restrict__ tyx,
                                                const real* __restrict__ tzy, const real* __restrict__ qx, const real* __restrict__ qy,
                                                const real* __restrict__ qz, real* __restrict__ vxt, real* __restrict__ vyt, real* __restrict__ vzt, real* __restrict__ e2,
@@ -1991,6 +1991,14 @@ int F3D_API(fused_cfl_fluids)(f3d_hip_ctx* ctx, int nf, double* const* cs, doubl
 }
 
 int F3D_API(fused_potential)(f3d_hip_ctx* ctx) {
+#if F3D_PREC == 32
+  // the potential is an FP64 array in every form: its radial and vertical
+  // differences balance the centrifugal terms, so it is computed in FP64
+  f3d_stage_begin(ctx, F3D_ST_POTENTIAL);
+  const int rc = f3d_Potential(ctx);
+  f3d_stage_end(ctx, F3D_ST_POTENTIAL);
+  return rc;
+#else
   Scratch* S;
   F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
   F3D_REQUIRE(ctx, ctx->ph.nplanets <= cell::kMaxPlanets, "Potential: too many planets");
@@ -2011,6 +2019,7 @@ int F3D_API(fused_potential)(f3d_hip_ctx* ctx) {
   FDISPATCH(ctx, kf_potential, g, dim3(BXF), 0, f.Pot, ps);
   f3d_stage_end(ctx, F3D_ST_POTENTIAL);
   return 0;
+#endif
 }
 
 int F3D_API(fused_init_stockholm)(f3d_hip_ctx* ctx) {
@@ -2131,7 +2140,7 @@ int F3D_API(fused_sources)(f3d_hip_ctx* ctx, double dt, double domega) {
   const int ext = iso ? 0 : 1;
   dim3 gb((m.nx + BXF - 1) / BXF, m.ny + ext, m.dimz ? m.nz + ext : 1);
   f3d_stage_begin(ctx, F3D_ST_SRC_B);
-  FDISPATCH(ctx, kf_src_b, gb, dim3(BXF), 0, rdt, f.Density, f.Energy, f.Vx, f.Vy, f.Vz, f.Pot, f.Mmx, f.Mmy, f.Mmz,
+  FDISPATCH(ctx, kf_src_b, gb, dim3(BXF), 0, rdt, f.Density, f.Energy, f.Vx, f.Vy, f.Vz, ctx->f.Pot, f.Mmx, f.Mmy, f.Mmz,
             f.Mpx, f.Mpy, f.Mpz, S->f[FS_QX], S->f[FS_QY], S->f[FS_QZ], f.Vx_temp, f.Vy_temp, f.Vz_temp,
             S->f[FS_E2], visc, pot, (int)(ctx->ph.fluidtype == 2), iso, (real)(ctx->ph.gamma - 1.0),
             (real)ctx->ph.OMEGAFRAME, rdom);

@@ -75,6 +75,7 @@ int f3d_hip_create(const f3d_mesh_desc* d, f3d_hip_ctx** out) {
   // the FP64 parts read these; in the FP64 build they are the same arrays
   m.xmin64 = m.xmin; m.ymin64 = m.ymin; m.zmin64 = m.zmin;
   m.ymed64 = m.ymed; m.sinzmed64 = m.sinzmed; m.sxi64 = m.sxi;
+  m.zmed64 = m.zmed; m.sinzmin64 = m.sinzmin; m.coszmin64 = m.coszmin;
   c->meshlen = total;
   *out = c;
   return 0;

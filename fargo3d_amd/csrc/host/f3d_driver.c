@@ -280,7 +280,8 @@ f3d_sim *f3d_sim_create(const char *parfile, const char *overrides, char *err, i
     }
     f->Mpx = dalloc(n, &bad); f->Mmx = dalloc(n, &bad); f->Mpy = dalloc(n, &bad); f->Mmy = dalloc(n, &bad);
     if (S->g.dimz) { f->Mpz = dalloc(n, &bad); f->Mmz = dalloc(n, &bad); }
-    f->Pot = dalloc(n, &bad); f->Slope = dalloc(n, &bad); f->DivRho = dalloc(n, &bad);
+    /* the potential is FP64 in both forms (its differences balance the centrifugal terms) */
+    f->Pot = dalloc_bytes(n * sizeof(double), &bad); f->Slope = dalloc(n, &bad); f->DivRho = dalloc(n, &bad);
     f->DensStar = dalloc(n, &bad); f->Qs = dalloc(n, &bad); f->Pressure = dalloc(n, &bad);
     f->Total_Density = dalloc(n, &bad); f->QL = dalloc(n, &bad); f->QR = dalloc(n, &bad);
     /* ring sums stay FP64 in both forms */
@@ -303,7 +304,8 @@ f3d_sim *f3d_sim_create(const char *parfile, const char *overrides, char *err, i
       double *own2[] = {F->Density0, F->Energy0, F->Vx0, F->Vy0, F->Vz0};
       for (size_t a = 0; a < sizeof(own2) / sizeof(own2[0]); a++) f3d_hip_memset(S->ctx, own2[a], 0, n2 * S->es);
     }
-    double **all[] = {&f->Mpx, &f->Mmx, &f->Mpy, &f->Mmy, &f->Mpz, &f->Mmz, &f->Pot, &f->Slope, &f->DivRho,
+    f3d_hip_memset(S->ctx, f->Pot, 0, n * sizeof(double));
+    double **all[] = {&f->Mpx, &f->Mmx, &f->Mpy, &f->Mmy, &f->Mpz, &f->Mmz, &f->Slope, &f->DivRho,
                       &f->DensStar, &f->Qs, &f->Pressure, &f->Total_Density, &f->QL, &f->QR};
     for (size_t a = 0; a < sizeof(all) / sizeof(all[0]); a++)
       if (*all[a]) f3d_hip_memset(S->ctx, *all[a], 0, n * S->es);
@@ -745,7 +747,7 @@ int f3d_sim_run(f3d_sim *S, int nsteps) {
 
 /* ---- field access ---- */
 /* is2d: 1 for YZ arrays; the element type is the simulation's except for the
-   FP64 ring sums (Reduction2D, VxMed) */
+   FP64 ring sums (Reduction2D, VxMed) and the FP64 potential */
 static double *field_ptr(f3d_sim *S, const char *name, int *is2d) {
   f3d_fields *f = &S->f;
   struct { const char *n; double *p; int d2; } t[] = {
@@ -791,7 +793,9 @@ long f3d_sim_field_size(f3d_sim *S, const char *name) {
   return d2 ? (long)S->g.n2d : (long)S->g.ncell;
 }
 
-static int is_fp64_array(f3d_sim *S, const double *p) { return p == S->f.Reduction2D || p == S->f.VxMed; }
+static int is_fp64_array(f3d_sim *S, const double *p) {
+  return p == S->f.Reduction2D || p == S->f.VxMed || p == S->f.Pot;
+}
 
 int f3d_sim_get_field(f3d_sim *S, const char *name, double *host) {
   int d2 = 0;
