@@ -12,8 +12,9 @@
 // intermediates on chip:
 //
 //   kf_cfl         ring-owning: VxMed of the ring and of the next ring
-//                  (sequential sums, in parallel), per-cell CFL, shear limit,
-//                  one atomic MIN per ring -> dt (one launch, 8 bytes to host)
+//                  (sequential sums, in parallel, or from kf_vmed_rings),
+//                  per-cell CFL, shear limit, one MIN per ring (atomic, or
+//                  into an array that kf_cfl_fin reduces) -> dt, 8 bytes to host
 //   kf_src_a       viscous stress and artificial pressure at every cell the
 //                  next kernel reads
 //   kf_src_b       SubStep1 + addviscosity + SubStep2_b on active cells
@@ -21,8 +22,10 @@
 //   kf_sweep_z     3-D: momenta + Z sweep, one thread per (i,j) column
 //                  marching in k over a register window
 //   kf_sweep_y     momenta (2-D) + Y sweep, columns marching in j
-//   kf_sweep_x     ring-owning: VxMed, residual, van Leer and PPA X sweeps,
-//                  integer shift and NewVelocity_x in LDS
+//   kf_sweep_xr_fluids ring-owning (every fluid in one launch): VxMed,
+//                  residual, van Leer and PPA X sweeps, integer shift and
+//                  NewVelocity_x, cell state in registers, stencils in LDS
+//   kf_sweep_x_tile the same X sweep on tiles with halos, for long rings
 //   kf_finish      NewVelocity_y/z and Stockholm damping
 //   (boundaries)   end-of-step ghost fill
 //
@@ -50,6 +53,9 @@ struct Scratch {
   unsigned long long* result;  // two slots, alternated per step
   int slot;
   double* ringmin;             // per-ring CFL minima of up to 8 fluids (two-pass MIN)
+  real* fset[8][13];           // transport arrays of fluids 1..7 (multifluid transport)
+  double* vxmed_f[8];          // VxMed and Nshift of the fluids whose values the
+  int* nshift_f[8];            // reference overwrites (all but the last)
   dmesh m;                     // the build's mesh view
   mreal* mesh_r;               // FP32 build: converted mesh arrays
   real* bctab_r;               // FP32 build: converted boundary tables
@@ -64,6 +70,12 @@ void scratch_free(void* p) {
     if (S->f[a]) (void)hipFree(S->f[a]);
   if (S->result) (void)hipFree(S->result);
   if (S->ringmin) (void)hipFree(S->ringmin);
+  for (int f = 0; f < 8; f++) {
+    for (int a = 0; a < 13; a++)
+      if (S->fset[f][a]) (void)hipFree(S->fset[f][a]);
+    if (S->vxmed_f[f]) (void)hipFree(S->vxmed_f[f]);
+    if (S->nshift_f[f]) (void)hipFree(S->nshift_f[f]);
+  }
   if (S->mesh_r) (void)hipFree(S->mesh_r);
   if (S->bctab_r) (void)hipFree(S->bctab_r);
   delete S;
@@ -658,259 +670,6 @@ __global__ void __launch_bounds__(BXF) kf_sweep_z(dmesh m, real dt, const real* 
   column_sweep<6 + AD>(col, r0, r1, dt);
 }
 
-// X sweep, ring-owning: VxMed of Vx_temp, residual split, van Leer pass with
-// the cell residual, PPA pass with the uniform residual, integer shift and
-// NewVelocity_x (src/transport.c:129-192).  LDS holds the ring; every
-// quantity of a pass goes through the same phase between two barriers.
-// Thread t owns cells i = t + s*blockDim.x, s < CPT.  Quantities: the
-// momenta in the order of src/transport.c (Mpx, Mmx, Mpy, Mmy[, Mpz, Mmz]),
-// then with AD the energy (e_in -> e_out), then the density.
-template <int G, int DZ, int AD, int CPT>
-__global__ void __launch_bounds__(256) kf_sweep_x(dmesh m, double dt64, const real* __restrict__ rho_in, const real* __restrict__ vxt,
-                                                   real* __restrict__ mpx, real* __restrict__ mmx, real* __restrict__ mpy, real* __restrict__ mmy, real* __restrict__ mpz,
-                                                   real* __restrict__ mmz, const real* __restrict__ e_in, real* __restrict__ e_out, real* __restrict__ rho_out,
-                                                   real* __restrict__ vx_out, real* __restrict__ mpy_o, real* __restrict__ mmy_o, real* __restrict__ mpz_o,
-                                                   real* __restrict__ mmz_o, double* __restrict__ vxmed_out, int* __restrict__ nshift_out,
-                                                   real omegaframe, int fargo) {
-  constexpr int NM = DZ ? 6 : 4;  // momenta
-  constexpr int NQ = NM + AD;
-  constexpr int N = NQ + 1;  // quantities + density (index NQ)
-  extern __shared__ double lds_d[];
-  const int nx = m.nx;
-  const real dt = (real)dt64;
-  double* Wt = lds_d;           // FP64 ring weights: nx
-  double* sh = Wt + nx;         // 8
-  real* ST = (real*)(sh + 8);   // state: N * nx (quantities, then density)
-  real* AQ = ST + N * nx;       // q = Q/rho (or rho): N * nx
-  real* SL = AQ + N * nx;       // slopes, then limited qL: N * nx
-  real* FC = SL + N * nx;       // faces, then face fluxes: N * nx
-  real* QL2 = FC + N * nx;      // limited qR of the PPA pass: N * nx
-  real* V = QL2 + N * nx;       // advection velocity: nx
-  const int j = m.nghy + (int)(blockIdx.x % m.ny);
-  const int k = DZ ? m.nghz + (int)(blockIdx.x / m.ny) : 0;
-  const int base = j * m.pitch + k * m.stride;
-  real* qg[6] = {mpx, mmx, mpy, mmy, mpz, mmz};
-  const int t = threadIdx.x, nt = blockDim.x;
-  double vt[CPT];
-  real vu[CPT];
-#pragma unroll
-  for (int s = 0; s < CPT; s++) {
-    const int i = t + s * nt;
-    if (i < nx) {
-      ST[NQ * nx + i] = rho_in[base + i];
-      #pragma unroll
-      for (int c = 0; c < NQ; c++) ST[c * nx + i] = (c < NM) ? qg[c][base + i] : e_in[base + i];
-      vt[s] = vxt[base + i];
-      Wt[i] = vt[s] * (m.xmin64[i + 1] - m.xmin64[i]);
-    }
-  }
-  __syncthreads();
-  // ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30), FP64
-  if (t == 0) sh[0] = seq_sum(Wt, nx) / (m.XMAX - m.XMIN);
-  __syncthreads();
-  const double vmed = sh[0];
-  // ComputeResidual (src/compute_residual.c:68-73), FP64: cell residual into V
-#pragma unroll
-  for (int s = 0; s < CPT; s++) {
-    const int i = t + s * nt;
-    if (i < nx) {
-      if (fargo) {
-        const double zs = zone_size_x64<G>(m, i, j, k);
-        const double ntilde = vmed * dt64 / zs;
-        const double nround = floor(ntilde + 0.5);
-        if (i == 0) sh[1] = nround;
-        V[i] = (real)(vt[s] - vmed);
-        vu[s] = (real)((ntilde - nround) * zs / dt64);
-      } else {
-        V[i] = (real)vt[s];
-      }
-    }
-  }
-  __syncthreads();
-  int ns = 0;
-  if (fargo) {
-    ns = (int)sh[1];
-    if (t == 0) {
-      nshift_out[j + k * m.sy] = ns;
-      vxmed_out[j + k * m.sy] = vmed;
-    }
-  }
-  // X_advection (src/transport.c:208-226), twice with FARGO
-  for (int pass = 0; pass < (fargo ? 2 : 1); pass++) {
-    const bool ppa = (pass == 1);
-    if (ppa) {
-      __syncthreads();
-#pragma unroll
-      for (int s = 0; s < CPT; s++) {
-        const int i = t + s * nt;
-        if (i < nx) V[i] = vu[s];
-      }
-    }
-    // q = Q/rho (DivideByRho) and, for the density, q = rho
-#pragma unroll
-    for (int s = 0; s < CPT; s++) {
-      const int i = t + s * nt;
-      if (i < nx) {
-        const real r = ST[NQ * nx + i];
-        #pragma unroll
-        for (int c = 0; c < NQ; c++) AQ[c * nx + i] = ST[c * nx + i] / r;
-        AQ[NQ * nx + i] = r;
-      }
-    }
-    __syncthreads();
-    if (!ppa) {
-      // VanLeerX_a (src/vanleer_x_a.c:60-68)
-#pragma unroll
-      for (int s = 0; s < CPT; s++) {
-        const int i = t + s * nt;
-        if (i < nx) {
-          const int im = ixm_of(i, nx), ip = ixp_of(i, nx);
-          const real zi = zone_size_x<G>(m, i, j, k), zp = zone_size_x<G>(m, ip, j, k);
-          #pragma unroll
-          for (int c = 0; c < N; c++) {
-            const real* a = AQ + c * nx;
-            SL[c * nx + i] = vanleer_slope((a[i] - a[im]) / zi, (a[ip] - a[i]) / zp);
-          }
-        }
-      }
-      __syncthreads();
-      // VanLeerX_b (src/vanleer_x_b.c:60-65)
-#pragma unroll
-      for (int s = 0; s < CPT; s++) {
-        const int i = t + s * nt;
-        if (i < nx) {
-          const int im = ixm_of(i, nx);
-          const real zi = zone_size_x<G>(m, i, j, k), zm = zone_size_x<G>(m, im, j, k);
-          const real v = V[i];
-          #pragma unroll
-          for (int c = 0; c < N; c++) {
-            const real* a = AQ + c * nx;
-            const real* sl = SL + c * nx;
-            FC[c * nx + i] = vanleer_face(v, dt, a[im], sl[im], zm, a[i], sl[i], zi);
-          }
-        }
-      }
-      __syncthreads();
-    } else {
-      // VanLeerX_PPA_a: monotonised-centred slope (src/fargo_ppa_a.c:58-69)
-#pragma unroll
-      for (int s = 0; s < CPT; s++) {
-        const int i = t + s * nt;
-        if (i < nx) {
-          const int im = ixm_of(i, nx), ip = ixp_of(i, nx);
-          #pragma unroll
-          for (int c = 0; c < N; c++) {
-            const real* a = AQ + c * nx;
-            SL[c * nx + i] = cell::ppa_slope(a[im], a[i], a[ip]);
-          }
-        }
-      }
-      __syncthreads();
-      // VanLeerX_PPA_b: face value q_{i+1/2} = qR of cell i = qL of cell i+1
-#pragma unroll
-      for (int s = 0; s < CPT; s++) {
-        const int i = t + s * nt;
-        if (i < nx) {
-          const int ip = ixp_of(i, nx);
-          #pragma unroll
-          for (int c = 0; c < N; c++) {
-            const real* a = AQ + c * nx;
-            const real* sl = SL + c * nx;
-            FC[c * nx + i] = cell::ppa_face(a[i], a[ip], sl[i], sl[ip]);
-          }
-        }
-      }
-      __syncthreads();
-      // VanLeerX_PPA_c (src/fargo_ppa_c.c): limited (qL, qR) of each cell into
-      // SL (slopes are no longer read) and QL2; the faces stay in FC
-#pragma unroll
-      for (int s = 0; s < CPT; s++) {
-        const int i = t + s * nt;
-        if (i < nx) {
-          const int im = ixm_of(i, nx);
-#pragma unroll
-          for (int c = 0; c < N; c++) {
-            real qL = FC[c * nx + im], qR = FC[c * nx + i];
-            cell::ppa_limit(AQ[c * nx + i], qL, qR);
-            SL[c * nx + i] = qL;
-            QL2[c * nx + i] = qR;
-          }
-        }
-      }
-      __syncthreads();
-      // VanLeerX_PPA_d: parabolic average over the swept zone (src/fargo_ppa_d.c:62-70)
-#pragma unroll
-      for (int s = 0; s < CPT; s++) {
-        const int i = t + s * nt;
-        if (i < nx) {
-          const int im = ixm_of(i, nx);
-          const real zi = zone_size_x<G>(m, i, j, k);
-          const real v = V[i];
-#pragma unroll
-          for (int c = 0; c < N; c++) {
-            const real* a = AQ + c * nx;
-            const real* qL = SL + c * nx;
-            const real* qR = QL2 + c * nx;
-            FC[c * nx + i] = cell::ppa_flux_value(v, dt, zi, a[im], qL[im], qR[im], a[i], qL[i], qR[i]);
-          }
-        }
-      }
-      __syncthreads();
-    }
-    // UpdateX for every quantity and UpdateDensityX last (src/update_x.c:69-71,
-    // src/update_density_x.c:66-68); a cell reads only faces, so in place
-#pragma unroll
-    for (int s = 0; s < CPT; s++) {
-      const int i = t + s * nt;
-      if (i < nx) {
-        const int ip = ixp_of(i, nx);
-        const real v0 = V[i], v1 = V[ip];
-        const real* ds = FC + NQ * nx;
-        #pragma unroll
-        for (int c = 0; c < NQ; c++) {
-          const real* f = FC + c * nx;
-          ST[c * nx + i] += cell::flux_x(m, i, j, k, dt, v0 * f[i] * ds[i], v1 * f[ip] * ds[ip]);
-        }
-        ST[NQ * nx + i] += cell::flux_x(m, i, j, k, dt, v0 * ds[i], v1 * ds[ip]);
-      }
-    }
-    __syncthreads();
-  }
-  // AdvectSHIFT (src/advect_shift.c:45-58): cell i moves to i + ns (mod nx);
-  // the shifted ring goes to AQ, which is free after the last update
-  const real* OUT = ST;
-  if (fargo) {
-#pragma unroll
-    for (int s = 0; s < CPT; s++) {
-      const int i = t + s * nt;
-      if (i < nx) {
-        int it = (i + ns) % nx;
-        if (it < 0) it += nx;
-#pragma unroll
-        for (int c = 0; c < N; c++) AQ[c * nx + it] = ST[c * nx + i];
-      }
-    }
-    __syncthreads();
-    OUT = AQ;
-  }
-  // NewVelocity_x (src/newvel_x.c:78-86) and outputs
-#pragma unroll
-  for (int s = 0; s < CPT; s++) {
-    const int i = t + s * nt;
-    if (i < nx) {
-      const int im = ixm_of(i, nx);
-      const real* R = OUT + NQ * nx;
-      vx_out[base + i] = cell::newvel_x<G>(m, i, j, k, R[i], R[im], OUT[1 * nx + i], OUT[0 * nx + im], omegaframe);
-      rho_out[base + i] = R[i];
-#pragma unroll
-      for (int c = 2; c < NQ; c++) {
-        if (c < NM) (c == 2 ? mpy_o : c == 3 ? mmy_o : c == 4 ? mpz_o : mmz_o)[base + i] = OUT[c * nx + i];
-        else e_out[base + i] = OUT[c * nx + i];
-      }
-    }
-  }
-}
-
 // The ring X sweep with the cell state in registers.  Thread t owns cells
 // i = t + s*blockDim.x (s < CPT) of the ring; LDS holds q = Q/rho, the slopes
 // (then the limited qL) and the faces (then qR, then the face fluxes) of the
@@ -1211,31 +970,6 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
 inline size_t xsweep_ring_lds(int nx, int ncarried) {
   const size_t P = ((size_t)(ncarried + 1) * nx + 1) & ~(size_t)1;
   return (3 * P + nx) * sizeof(real) + 64;
-}
-
-template <int G, int DZ, int AD, int CPT, int SPLIT>
-__global__ void __launch_bounds__(256) kf_sweep_xr(dmesh m, double dt64, const real* __restrict__ rho_in,
-                                                    const real* __restrict__ vxt, const real* __restrict__ mpx,
-                                                    const real* __restrict__ mmx, const real* __restrict__ mpy,
-                                                    const real* __restrict__ mmy, const real* __restrict__ mpz,
-                                                    const real* __restrict__ mmz, const real* __restrict__ e_in,
-                                                    real* __restrict__ e_out, real* __restrict__ rho_out,
-                                                    real* __restrict__ vx_out, real* __restrict__ mpy_o,
-                                                    real* __restrict__ mmy_o, real* __restrict__ mpz_o,
-                                                    real* __restrict__ mmz_o, double* __restrict__ vxmed_out,
-                                                    int* __restrict__ nshift_out, real omegaframe, int fargo) {
-  const real* qin[6] = {mpx, mmx, mpy, mmy, mpz, mmz};
-  real* qout[4] = {mpy_o, mmy_o, mpz_o, mmz_o};
-  if (!SPLIT) {
-    xsweep_ring<G, DZ, AD, CPT, 0>(m, (int)blockIdx.x, dt64, rho_in, vxt, qin, e_in, e_out, rho_out, vx_out, qout,
-                                   vxmed_out, nshift_out, omegaframe, fargo);
-  } else if ((blockIdx.x & 1) == 0) {
-    xsweep_ring<G, DZ, AD, CPT, 1>(m, (int)(blockIdx.x >> 1), dt64, rho_in, vxt, qin, e_in, e_out, rho_out, vx_out,
-                                   qout, vxmed_out, nshift_out, omegaframe, fargo);
-  } else {
-    xsweep_ring<G, DZ, AD, CPT, 2>(m, (int)(blockIdx.x >> 1), dt64, rho_in, vxt, qin, e_in, e_out, rho_out, vx_out,
-                                   qout, vxmed_out, nshift_out, omegaframe, fargo);
-  }
 }
 
 // ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30) for the tiled X sweep: one
@@ -1738,6 +1472,20 @@ struct RF {
   int* Nshift;
 };
 
+// one fluid's fields (f) with the handle's shared fields (sh)
+static RF rfields_of(const f3d_fields& f, const f3d_fields& sh) {
+  auto R = [](double* p) { return reinterpret_cast<real*>(p); };
+  RF r;
+  r.Density = R(f.Density); r.Energy = R(f.Energy); r.Vx = R(f.Vx); r.Vy = R(f.Vy); r.Vz = R(f.Vz);
+  r.Vx_temp = R(f.Vx_temp); r.Vy_temp = R(f.Vy_temp); r.Vz_temp = R(f.Vz_temp);
+  r.Mpx = R(sh.Mpx); r.Mmx = R(sh.Mmx); r.Mpy = R(sh.Mpy); r.Mmy = R(sh.Mmy); r.Mpz = R(sh.Mpz); r.Mmz = R(sh.Mmz);
+  r.Pot = R(sh.Pot);
+  r.Density0 = R(f.Density0); r.Energy0 = R(f.Energy0); r.Vx0 = R(f.Vx0); r.Vy0 = R(f.Vy0); r.Vz0 = R(f.Vz0);
+  r.VxMed = sh.VxMed;
+  r.Nshift = sh.Nshift;
+  return r;
+}
+
 static RF rfields(const f3d_hip_ctx* ctx) {
   const f3d_fields& f = ctx->f;
   auto R = [](double* p) { return reinterpret_cast<real*>(p); };
@@ -1836,11 +1584,59 @@ static int vmed_prepass(f3d_hip_ctx* ctx, Scratch* S, const real* v) {
   return 0;
 }
 
+// The arrays one fluid's transport writes between its kernels: the Y-swept
+// density and energy, the swept momenta (Mpx, Mmx, Mpy, Mmy, Mpz, Mmz), the
+// X-swept density and Y/Z momenta, and where VxMed and Nshift go.  Fluid 0
+// uses the handle's arrays; the other fluids of a multifluid transport get
+// arrays of their own, allocated on first use, so that every fluid's X sweep
+// can run in one launch.  As in the reference, VxMed and Nshift keep the
+// last fluid's values.
+struct TSet {
+  real *rhoy, *ey, *rhox;
+  real* m[6];
+  real* xm[4];
+  double* vxmed;
+  int* nshift;
+};
+
+static int tset_of(f3d_hip_ctx* ctx, Scratch* S, const RF& f, int fluid, int nf, TSet* t) {
+  const size_t bytes = ctx->n3 * sizeof(real);
+  if (fluid == 0) {
+    t->rhoy = S->f[FS_RHOY]; t->ey = S->f[FS_EY]; t->rhox = S->f[FS_RHOX];
+    real* mm[6] = {f.Mpx, f.Mmx, f.Mpy, f.Mmy, f.Mpz, f.Mmz};
+    for (int a = 0; a < 6; a++) t->m[a] = mm[a];
+    for (int a = 0; a < 4; a++) t->xm[a] = S->f[FS_XM0 + a];
+  } else {
+    F3D_REQUIRE(ctx, fluid < 8, "transport: at most 8 fluids");
+    real** a = S->fset[fluid];
+    const int na = ctx->m.dimz ? 13 : 11;  // 2-D: no Z momenta
+    for (int q = 0; q < na; q++)
+      if (!a[q]) {
+        F3D_CHECK(ctx, hipMalloc(&a[q], bytes));
+        F3D_CHECK(ctx, hipMemsetAsync(a[q], 0, bytes, ctx->stream));
+      }
+    t->rhoy = a[0]; t->ey = a[1]; t->rhox = a[2];
+    for (int q = 0; q < 4; q++) t->xm[q] = a[3 + q];
+    for (int q = 0; q < 6; q++) t->m[q] = (7 + q < na) ? a[7 + q] : nullptr;
+  }
+  if (fluid == nf - 1) {
+    t->vxmed = f.VxMed;
+    t->nshift = f.Nshift;
+  } else {
+    if (!S->vxmed_f[fluid]) {
+      F3D_CHECK(ctx, hipMalloc(&S->vxmed_f[fluid], ctx->n2 * sizeof(double)));
+      F3D_CHECK(ctx, hipMalloc(&S->nshift_f[fluid], ctx->n2 * sizeof(int)));
+    }
+    t->vxmed = S->vxmed_f[fluid];
+    t->nshift = S->nshift_f[fluid];
+  }
+  return 0;
+}
+
 // NewVelocity_y/z, Stockholm damping and the end-of-step ghost fill
-static int finish_step(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step) {
+static int finish_step(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step, const RF& f, const TSet& t) {
   const dmesh& m = S->m;
-  const RF f = rfields(ctx);
-  real* const* XM = &S->f[FS_XM0];
+  real* const* XM = t.xm;
   cell::DampPar dp;
   dp.Y_inf = (real)ctx->ph.Y_inf; dp.Y_sup = (real)ctx->ph.Y_sup;
   dp.Z_inf = (real)ctx->ph.Z_inf; dp.Z_sup = (real)ctx->ph.Z_sup;
@@ -1850,7 +1646,7 @@ static int finish_step(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step)
   dp.periodic_z = ctx->ph.periodicz;
   dim3 gf((m.nx + BXF - 1) / BXF, m.ny, m.dimz ? m.nz : 1);
   f3d_stage_begin(ctx, F3D_ST_FINISH);
-  FDISPATCH(ctx, kf_finish, gf, dim3(BXF), 0, (real)dt, S->f[FS_RHOX], XM[1], XM[0], XM[3], XM[2], f.Density, f.Vx,
+  FDISPATCH(ctx, kf_finish, gf, dim3(BXF), 0, (real)dt, t.rhox, XM[1], XM[0], XM[3], XM[2], f.Density, f.Vx,
             f.Vy, f.Vz, f.Density0, f.Vx0, f.Vy0, f.Vz0, dp, ctx->ph.stockholm);
   f3d_stage_end(ctx, F3D_ST_FINISH);
   if (end_of_step) {
@@ -1862,13 +1658,41 @@ static int finish_step(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step)
   return 0;
 }
 
+// momenta and the Z (3-D) and Y sweeps of one fluid into its transport set
+static int yz_sweeps(f3d_hip_ctx* ctx, Scratch* S, double dt, const RF& f, const TSet& t) {
+  const dmesh& m = S->m;
+  const real of = (real)ctx->ph.OMEGAFRAME;
+  const real rdt = (real)dt;
+  const int ry = g_rows_y, rz = g_rows_z;
+  if (m.dimz) {
+    // eight separate buffers: six Z-swept momenta, the Z-swept density and energy
+    real* const* Z = &S->f[FS_Z0];
+    dim3 gz((m.nx + BXF - 1) / BXF, m.sy - 2, (m.nz + rz - 1) / rz);
+    f3d_stage_begin(ctx, F3D_ST_SWEEP_Z);
+    FDISPATCH_AD(ctx, kf_sweep_z, gz, dim3(BXF), 0, rdt, f.Density, f.Vx_temp, f.Vy_temp, f.Vz_temp, f.Energy, Z[6],
+                 Z[0], Z[1], Z[2], Z[3], Z[4], Z[5], Z[7], rz, of);
+    f3d_stage_end(ctx, F3D_ST_SWEEP_Z);
+    dim3 gy((m.nx + BXF - 1) / BXF, (m.ny + ry - 1) / ry, m.nz);
+    f3d_stage_begin(ctx, F3D_ST_SWEEP_Y);
+    FDISPATCH_AD(ctx, kf_sweep_y3, gy, dim3(BXF), 0, rdt, Z[6], Z[0], Z[1], Z[2], Z[3], Z[4], Z[5], Z[7], f.Vy_temp,
+                 t.rhoy, t.m[0], t.m[1], t.m[2], t.m[3], t.m[4], t.m[5], t.ey, ry);
+    f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
+  } else {
+    dim3 gy((m.nx + BXF - 1) / BXF, (m.ny + ry - 1) / ry, 1);
+    f3d_stage_begin(ctx, F3D_ST_SWEEP_Y);
+    FDISPATCH_AD(ctx, kf_sweep_y2, gy, dim3(BXF), 0, rdt, f.Density, f.Vx_temp, f.Vy_temp, f.Energy, t.rhoy,
+                 t.m[0], t.m[1], t.m[2], t.m[3], t.ey, ry, of);
+    f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
+  }
+  return 0;
+}
+
 // X sweep by tiles (rings longer than the LDS ring kernel holds): VxMed of
 // each ring, then the tiles.
-static int x_sweep_tiled(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step) {
+static int x_sweep_tiled(f3d_hip_ctx* ctx, Scratch* S, double dt, const RF& f, const TSet& t) {
   const dmesh& m = S->m;
-  const RF f = rfields(ctx);
   const real of = (real)ctx->ph.OMEGAFRAME;
-  real* const* XM = &S->f[FS_XM0];
+  real* const* XM = t.xm;
   const int nrings = m.ny * (m.dimz ? m.nz : 1);
   const int ad = (ctx->ph.eos == 2);
   constexpr int T = 128;
@@ -1876,12 +1700,12 @@ static int x_sweep_tiled(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_ste
   F3D_REQUIRE(ctx, (size_t)m.nx * sizeof(double) <= 160 * 1024, "ring too long for the VxMed pass");
   f3d_stage_begin(ctx, F3D_ST_SWEEP_X);
   if (ctx->ph.fargo)
-    FDISPATCH(ctx, kf_vmed, dim3(nrings), dim3(WG), m.nx * sizeof(double), f.Vx_temp, f.VxMed);
+    FDISPATCH(ctx, kf_vmed, dim3(nrings), dim3(WG), m.nx * sizeof(double), f.Vx_temp, t.vxmed);
   dim3 g((m.nx + T - 1) / T, nrings);
 #define XTILE(G_, DZ_, AD_)                                                                                     \
-  hipLaunchKernelGGL((kf_sweep_x_tile<G_, DZ_, AD_, T>), g, dim3(T), 0, ctx->stream, m, dt, S->f[FS_RHOY],      \
-                     f.Vx_temp, f.Mpx, f.Mmx, f.Mpy, f.Mmy, f.Mpz, f.Mmz, S->f[FS_EY], f.Energy, S->f[FS_RHOX], \
-                     f.Vx, XM[0], XM[1], XM[2], XM[3], f.VxMed, f.Nshift, of, ctx->ph.fargo)
+  hipLaunchKernelGGL((kf_sweep_x_tile<G_, DZ_, AD_, T>), g, dim3(T), 0, ctx->stream, m, dt, t.rhoy, f.Vx_temp,  \
+                     t.m[0], t.m[1], t.m[2], t.m[3], t.m[4], t.m[5], t.ey, f.Energy, t.rhox, f.Vx, XM[0], XM[1],  \
+                     XM[2], XM[3], t.vxmed, t.nshift, of, ctx->ph.fargo)
   if (m.geom == SPH) {
     if (m.dimz) { if (ad) XTILE(SPH, 1, 1); else XTILE(SPH, 1, 0); }
     else { if (ad) XTILE(SPH, 0, 1); else XTILE(SPH, 0, 0); }
@@ -1892,7 +1716,119 @@ static int x_sweep_tiled(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_ste
 #undef XTILE
   F3D_LAUNCH_CHECK(ctx);
   f3d_stage_end(ctx, F3D_ST_SWEEP_X);
-  return finish_step(ctx, S, dt, end_of_step);
+  return 0;
+}
+
+// Ring X sweep of nf fluids in one launch: nf consecutive blocks of
+// per_fluid workgroups, each block reading its fluid's arrays from the table.
+struct XTab {
+  const real* rho_in[8];
+  const real* vxt[8];
+  const real* q[8][6];
+  const real* e_in[8];
+  real* e_out[8];
+  real* rho_out[8];
+  real* vx_out[8];
+  real* qo[8][4];
+  double* vxmed[8];
+  int* nshift[8];
+};
+
+template <int G, int DZ, int AD, int CPT, int SPLIT>
+__global__ void __launch_bounds__(CPT == 2 ? 512 : 256) kf_sweep_xr_fluids(dmesh m, double dt64, XTab T, int per_fluid,
+                                                           real omegaframe, int fargo) {
+  const int fl = (int)blockIdx.x / per_fluid, b = (int)blockIdx.x % per_fluid;
+  real* qout[4] = {T.qo[fl][0], T.qo[fl][1], T.qo[fl][2], T.qo[fl][3]};
+  if (!SPLIT) {
+    xsweep_ring<G, DZ, AD, CPT, 0>(m, b, dt64, T.rho_in[fl], T.vxt[fl], T.q[fl], T.e_in[fl], T.e_out[fl],
+                                   T.rho_out[fl], T.vx_out[fl], qout, T.vxmed[fl], T.nshift[fl], omegaframe, fargo);
+  } else if ((b & 1) == 0) {
+    xsweep_ring<G, DZ, AD, CPT, 1>(m, b >> 1, dt64, T.rho_in[fl], T.vxt[fl], T.q[fl], T.e_in[fl], T.e_out[fl],
+                                   T.rho_out[fl], T.vx_out[fl], qout, T.vxmed[fl], T.nshift[fl], omegaframe, fargo);
+  } else {
+    xsweep_ring<G, DZ, AD, CPT, 2>(m, b >> 1, dt64, T.rho_in[fl], T.vxt[fl], T.q[fl], T.e_in[fl], T.e_out[fl],
+                                   T.rho_out[fl], T.vx_out[fl], qout, T.vxmed[fl], T.nshift[fl], omegaframe, fargo);
+  }
+}
+
+// Ring X sweep of nf fluids (one launch), or -1 when the rings go to tiles.
+static int x_sweep_ring(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const RF* f, const TSet* t) {
+  const dmesh& m = S->m;
+  const real of = (real)ctx->ph.OMEGAFRAME;
+  const int ad = (ctx->ph.eos == 2);
+  const int nq = (m.dimz ? 6 : 4) + ad, na = m.dimz ? 3 : 2;
+  // measured on MI355X: the split pays while a thread owns one or two cells
+  // of the ring (nx <= 512); with four, the register pressure of the split
+  // costs more than the occupancy it gains
+  const int split = (g_xsplit < 0) ? (m.nx <= 512 ? 1 : 0) : g_xsplit;
+  const size_t shx = split ? xsweep_ring_lds(m.nx, na > nq - na ? na : nq - na) : xsweep_ring_lds(m.nx, nq);
+  const bool ring_fits = shx <= 160 * 1024 && m.nx <= 1024;
+  if (g_xmode == 2 || (g_xmode == 0 && !ring_fits)) return 1;
+  F3D_REQUIRE(ctx, ring_fits, "ring too long for the LDS X sweep");
+  F3D_REQUIRE(ctx, nf >= 1 && nf <= 8, "X sweep: 1 to 8 fluids");
+  // one cell per thread up to 256 cells, else two (up to 512 threads)
+  const int cpt = (m.nx <= 256) ? 1 : 2;
+  const int nthreads = (((m.nx + cpt - 1) / cpt + 63) / 64) * 64;
+  const int per = m.ny * (m.dimz ? m.nz : 1) * (split ? 2 : 1);
+  XTab T;
+  memset(&T, 0, sizeof(T));
+  for (int k = 0; k < nf; k++) {
+    T.rho_in[k] = t[k].rhoy; T.vxt[k] = f[k].Vx_temp;
+    for (int q = 0; q < 6; q++) T.q[k][q] = t[k].m[q];
+    T.e_in[k] = t[k].ey; T.e_out[k] = f[k].Energy; T.rho_out[k] = t[k].rhox; T.vx_out[k] = f[k].Vx;
+    for (int q = 0; q < 4; q++) T.qo[k][q] = t[k].xm[q];
+    T.vxmed[k] = t[k].vxmed; T.nshift[k] = t[k].nshift;
+  }
+  dim3 gx(per * nf);
+  f3d_stage_begin(ctx, F3D_ST_SWEEP_X);
+#define XSWEEP1(G_, DZ_, AD_, CPT_, SP_)                                                                      \
+  hipLaunchKernelGGL((kf_sweep_xr_fluids<G_, DZ_, AD_, CPT_, SP_>), gx, dim3(nthreads), shx, ctx->stream, m, dt, \
+                     T, per, of, ctx->ph.fargo)
+#define XSWEEP2(G_, DZ_, AD_, CPT_) \
+  do { if (split) XSWEEP1(G_, DZ_, AD_, CPT_, 1); else XSWEEP1(G_, DZ_, AD_, CPT_, 0); } while (0)
+#define XSWEEP(G_, DZ_, CPT_) \
+  do { if (ad) XSWEEP2(G_, DZ_, 1, CPT_); else XSWEEP2(G_, DZ_, 0, CPT_); } while (0)
+#define XSWEEP_G(G_, DZ_) \
+  do { if (cpt == 1) XSWEEP(G_, DZ_, 1); else XSWEEP(G_, DZ_, 2); } while (0)
+  if (m.geom == SPH) {
+    if (m.dimz) XSWEEP_G(SPH, 1); else XSWEEP_G(SPH, 0);
+  } else {
+    if (m.dimz) XSWEEP_G(CYL, 1); else XSWEEP_G(CYL, 0);
+  }
+#undef XSWEEP_G
+#undef XSWEEP1
+#undef XSWEEP2
+#undef XSWEEP
+  F3D_LAUNCH_CHECK(ctx);
+  f3d_stage_end(ctx, F3D_ST_SWEEP_X);
+  return 0;
+}
+
+// Transport of nf fluids (fields fl[k], the handle's shared fields): every
+// fluid's momenta and Z/Y sweeps, one X-sweep launch over all fluids (or each
+// fluid's tiles), then each fluid's NewVelocity_y/z, damping and ghost fill.
+// Fluids do not interact in Transport, so this equals transporting them one
+// after the other (src/main.c:420).
+static int transport_fluids(f3d_hip_ctx* ctx, double dt, int nf, const f3d_fields* fl, int end_of_step) {
+  Scratch* S;
+  F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
+  F3D_REQUIRE(ctx, nf >= 1 && nf <= 8, "transport: 1 to 8 fluids");
+  RF f[8];
+  TSet t[8];
+  for (int k = 0; k < nf; k++) {
+    f[k] = fl ? rfields_of(fl[k], ctx->f) : rfields(ctx);
+    if (tset_of(ctx, S, f[k], k, nf, &t[k])) return -1;
+  }
+  for (int k = 0; k < nf; k++)
+    if (yz_sweeps(ctx, S, dt, f[k], t[k])) return -1;
+  const int rc = x_sweep_ring(ctx, S, dt, nf, f, t);
+  if (rc < 0) return -1;
+  if (rc == 1)
+    for (int k = 0; k < nf; k++)
+      if (x_sweep_tiled(ctx, S, dt, f[k], t[k])) return -1;
+  for (int k = 0; k < nf; k++)
+    if (finish_step(ctx, S, dt, end_of_step, f[k], t[k])) return -1;
+  return 0;
 }
 
 extern "C" {
@@ -2161,73 +2097,12 @@ int F3D_API(fused_sources)(f3d_hip_ctx* ctx, double dt, double domega) {
 }
 
 int F3D_API(fused_transport)(f3d_hip_ctx* ctx, double dt, int end_of_step) {
-  Scratch* S;
-  F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
-  const dmesh& m = S->m;
-  const RF f = rfields(ctx);
-  const real of = (real)ctx->ph.OMEGAFRAME;
-  const real rdt = (real)dt;
-  const int ry = g_rows_y, rz = g_rows_z;
-  real* const* XM = &S->f[FS_XM0];
-  if (m.dimz) {
-    // eight separate buffers: six Z-swept momenta, the Z-swept density and energy
-    real* const* Z = &S->f[FS_Z0];
-    dim3 gz((m.nx + BXF - 1) / BXF, m.sy - 2, (m.nz + rz - 1) / rz);
-    f3d_stage_begin(ctx, F3D_ST_SWEEP_Z);
-    FDISPATCH_AD(ctx, kf_sweep_z, gz, dim3(BXF), 0, rdt, f.Density, f.Vx_temp, f.Vy_temp, f.Vz_temp, f.Energy, Z[6],
-                 Z[0], Z[1], Z[2], Z[3], Z[4], Z[5], Z[7], rz, of);
-    f3d_stage_end(ctx, F3D_ST_SWEEP_Z);
-    dim3 gy((m.nx + BXF - 1) / BXF, (m.ny + ry - 1) / ry, m.nz);
-    f3d_stage_begin(ctx, F3D_ST_SWEEP_Y);
-    FDISPATCH_AD(ctx, kf_sweep_y3, gy, dim3(BXF), 0, rdt, Z[6], Z[0], Z[1], Z[2], Z[3], Z[4], Z[5], Z[7], f.Vy_temp,
-                 S->f[FS_RHOY], f.Mpx, f.Mmx, f.Mpy, f.Mmy, f.Mpz, f.Mmz, S->f[FS_EY], ry);
-    f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
-  } else {
-    dim3 gy((m.nx + BXF - 1) / BXF, (m.ny + ry - 1) / ry, 1);
-    f3d_stage_begin(ctx, F3D_ST_SWEEP_Y);
-    FDISPATCH_AD(ctx, kf_sweep_y2, gy, dim3(BXF), 0, rdt, f.Density, f.Vx_temp, f.Vy_temp, f.Energy, S->f[FS_RHOY],
-                 f.Mpx, f.Mmx, f.Mpy, f.Mmy, S->f[FS_EY], ry, of);
-    f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
-  }
-  // X sweep: one workgroup of at most 256 threads per active ring (up to 4
-  // cells per thread), or two per ring each carrying half of the quantities,
-  // when the ring fits in LDS, else tiles with halos
-  const int ad = (ctx->ph.eos == 2);
-  const int nq = (m.dimz ? 6 : 4) + ad, na = m.dimz ? 3 : 2;
-  // measured on MI355X: the split pays while a thread owns one or two cells
-  // of the ring (nx <= 512); with four, the register pressure of the split
-  // costs more than the occupancy it gains
-  const int split = (g_xsplit < 0) ? (m.nx <= 512 ? 1 : 0) : g_xsplit;
-  const size_t shx = split ? xsweep_ring_lds(m.nx, na > nq - na ? na : nq - na) : xsweep_ring_lds(m.nx, nq);
-  const bool ring_fits = shx <= 160 * 1024 && m.nx <= 1024;
-  if (g_xmode == 2 || (g_xmode == 0 && !ring_fits)) return x_sweep_tiled(ctx, S, dt, end_of_step);
-  F3D_REQUIRE(ctx, ring_fits, "ring too long for the LDS X sweep");
-  const int cpt = (m.nx <= 256) ? 1 : (m.nx <= 512) ? 2 : 4;
-  const int nthreads = (((m.nx + cpt - 1) / cpt + 63) / 64) * 64;
-  dim3 gx(m.ny * (m.dimz ? m.nz : 1) * (split ? 2 : 1));
-  f3d_stage_begin(ctx, F3D_ST_SWEEP_X);
-#define XSWEEP1(G_, DZ_, AD_, CPT_, SP_)                                                                      \
-  hipLaunchKernelGGL((kf_sweep_xr<G_, DZ_, AD_, CPT_, SP_>), gx, dim3(nthreads), shx, ctx->stream, m, dt,      \
-                     S->f[FS_RHOY], f.Vx_temp, f.Mpx, f.Mmx, f.Mpy, f.Mmy, f.Mpz, f.Mmz, S->f[FS_EY], f.Energy,   \
-                     S->f[FS_RHOX], f.Vx, XM[0], XM[1], XM[2], XM[3], f.VxMed, f.Nshift, of, ctx->ph.fargo)
-#define XSWEEP2(G_, DZ_, AD_, CPT_) \
-  do { if (split) XSWEEP1(G_, DZ_, AD_, CPT_, 1); else XSWEEP1(G_, DZ_, AD_, CPT_, 0); } while (0)
-#define XSWEEP(G_, DZ_, CPT_) \
-  do { if (ad) XSWEEP2(G_, DZ_, 1, CPT_); else XSWEEP2(G_, DZ_, 0, CPT_); } while (0)
-#define XSWEEP_G(G_, DZ_) \
-  do { if (cpt == 1) XSWEEP(G_, DZ_, 1); else if (cpt == 2) XSWEEP(G_, DZ_, 2); else XSWEEP(G_, DZ_, 4); } while (0)
-  if (m.geom == SPH) {
-    if (m.dimz) XSWEEP_G(SPH, 1); else XSWEEP_G(SPH, 0);
-  } else {
-    if (m.dimz) XSWEEP_G(CYL, 1); else XSWEEP_G(CYL, 0);
-  }
-#undef XSWEEP_G
-#undef XSWEEP1
-#undef XSWEEP2
-#undef XSWEEP
-  F3D_LAUNCH_CHECK(ctx);
-  f3d_stage_end(ctx, F3D_ST_SWEEP_X);
-  return finish_step(ctx, S, dt, end_of_step);
+  return transport_fluids(ctx, dt, 1, nullptr, end_of_step);
+}
+
+int F3D_API(fused_transport_fluids)(f3d_hip_ctx* ctx, double dt, int nf, const f3d_fields* fl) {
+  F3D_REQUIRE(ctx, fl != nullptr, "transport: no fluid fields");
+  return transport_fluids(ctx, dt, nf, fl, 1);
 }
 
 }  // extern "C"

@@ -711,12 +711,25 @@ int f3d_sim_step(f3d_sim *S, double *dt_out) {
   /* MULTIFLUID(Transport(dt)), then MULTIFLUID(StockholmBoundary(dt)) and
      MULTIFLUID(FillGhosts) (src/main.c:420-434); the fused transport ends with
      the damping and the ghost fill of its own fluid */
-  for (int k = 0; k < nfl; k++) {
-    select_fluid(S, k);
-    if (S->arch.fused_transport) {
-      CALL(S, S->prec32 ? f3d32_fused_transport(c, dt, 1) : f3d_fused_transport(c, dt, 1));
-    } else {
-      if (transport_routines(S, dt)) return -1;
+  if (S->arch.fused_transport && nfl > 1) {
+    /* every fluid's sweeps, one X-sweep launch over all fluids */
+    f3d_fields all[F3D_MAX_FLUIDS];
+    for (int k = 0; k < nfl; k++) {
+      all[k] = S->f;
+#define SEL(m) all[k].m = S->fl[k].m;
+      F3D_FLUID_MEMBERS(SEL)
+#undef SEL
+    }
+    if (f3d_sim_push_phys(S)) return -1;
+    CALL(S, S->prec32 ? f3d32_fused_transport_fluids(c, dt, nfl, all) : f3d_fused_transport_fluids(c, dt, nfl, all));
+  } else {
+    for (int k = 0; k < nfl; k++) {
+      select_fluid(S, k);
+      if (S->arch.fused_transport) {
+        CALL(S, S->prec32 ? f3d32_fused_transport(c, dt, 1) : f3d_fused_transport(c, dt, 1));
+      } else {
+        if (transport_routines(S, dt)) return -1;
+      }
     }
   }
   S->clk.PhysicalTime += dt;

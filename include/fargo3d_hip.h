@@ -203,6 +203,12 @@ int f3d_fused_sources(f3d_hip_ctx *ctx, double dt, double domega);
 /* Transport (src/algogas.c:135-159) and, when stockholm != 0, StockholmBoundary and
  * the end-of-step ghost fill. */
 int f3d_fused_transport(f3d_hip_ctx *ctx, double dt, int end_of_step);
+/* MULTIFLUID(Transport(dt)), MULTIFLUID(StockholmBoundary(dt)) and
+ * MULTIFLUID(FillGhosts) (src/main.c:420-434) for nf (1..8) fluids whose fields
+ * are fl[0..nf-1] (the scratch, potential, VxMed and Nshift pointers are taken
+ * from the handle's bound set): every fluid's sweeps, with the X sweep of all
+ * fluids in one launch.  VxMed and Nshift end with the last fluid's values. */
+int f3d_fused_transport_fluids(f3d_hip_ctx *ctx, double dt, int nf, const f3d_fields *fl);
 /* Rows (Y sweep) and planes (Z sweep) one thread marches through; a tuning
  * knob that does not change results.  Returns -2 for values below 1. */
 int f3d_fused_set_tiling(int rows_y, int rows_z);
@@ -262,6 +268,7 @@ int f3d_fused_dust_diffusion(f3d_hip_ctx *ctx, double dt, int nf, double *const 
 int f3d32_fused_cfl(f3d_hip_ctx *ctx, double *fluid_min);
 int f3d32_fused_sources(f3d_hip_ctx *ctx, double dt, double domega);
 int f3d32_fused_transport(f3d_hip_ctx *ctx, double dt, int end_of_step);
+int f3d32_fused_transport_fluids(f3d_hip_ctx *ctx, double dt, int nf, const f3d_fields *fl);
 int f3d32_fused_potential(f3d_hip_ctx *ctx);
 int f3d32_fused_boundaries(f3d_hip_ctx *ctx);
 int f3d32_fused_init_stockholm(f3d_hip_ctx *ctx);
