@@ -1000,11 +1000,15 @@ __global__ void __launch_bounds__(WG) kf_vmed(dmesh m, const real* __restrict__ 
 template <int G, int DZ, int AD, int T>
 __device__ __forceinline__ void kd_sweep_x_tile(const dim3 bid, const dmesh& m, double dt64, const real* rho_in, const real* vxt, const real* mpx, const real* mmx, const real* mpy, const real* mmy, const real* mpz, const real* mmz, const real* e_in, real* e_out, real* rho_out, real* vx_out, real* mpy_o, real* mmy_o, real* mpz_o, real* mmz_o, const double* vxmed, int* nshift_out, real omegaframe, int fargo) {
   constexpr int H = 6, W = T + 2 * H;
+  static_assert(2 * H <= T, "tile narrower than its halos");
   const real dt = (real)dt64;
   constexpr int NM = DZ ? 6 : 4;
   constexpr int NQ = NM + AD;
   constexpr int N = NQ + 1;
-  __shared__ real ST[N * W], AQ[N * W], SL[N * W], FC[N * W], QR2[N * W];
+  // q = Q/rho, slopes (then limited qL), faces (then qR, then face fluxes);
+  // the cell state stays in registers: thread t owns tile positions t and,
+  // for t < 2H, T + t
+  __shared__ real AQ[N * W], SL[N * W], FC[N * W];
   __shared__ real V[W], VU[W], ZS[W];
   const int nx = m.nx;
   const int ring = bid.y;
@@ -1022,127 +1026,162 @@ __device__ __forceinline__ void kd_sweep_x_tile(const dim3 bid, const dmesh& m, 
   // the integer shift of the ring: ComputeResidual at i = 0 (src/compute_residual.c:68-71)
   const int ns = fargo ? (int)floor(vmed * dt64 / zone_size_x64<G>(m, 0, j, k) + 0.5) : 0;
   if (fargo && bid.x == 0 && t == 0) nshift_out[j + k * m.sy] = ns;
-  for (int p = t; p < W; p += T) {
-    const int g = gidx(p);
-    ST[NQ * W + p] = rho_in[base + g];
+  real st[2][N];
 #pragma unroll
-    for (int c = 0; c < NQ; c++) ST[c * W + p] = (c < NM) ? qin[c][base + g] : e_in[base + g];
-    ZS[p] = zone_size_x<G>(m, g, j, k);
-    const double v = vxt[base + g];
-    if (fargo) {
-      // the residual split in FP64 (src/compute_residual.c:68-73)
-      const double zs = zone_size_x64<G>(m, g, j, k);
-      const double ntilde = vmed * dt64 / zs;
-      const double nround = floor(ntilde + 0.5);
-      V[p] = (real)(v - vmed);
-      VU[p] = (real)((ntilde - nround) * zs / dt64);
-    } else {
-      V[p] = (real)v;
+  for (int s = 0; s < 2; s++) {
+    const int p = t + s * T;
+    if (s == 0 || t < 2 * H) {
+      const int g = gidx(p);
+      st[s][NQ] = rho_in[base + g];
+#pragma unroll
+      for (int c = 0; c < NQ; c++) st[s][c] = (c < NM) ? qin[c][base + g] : e_in[base + g];
+      ZS[p] = zone_size_x<G>(m, g, j, k);
+      const double v = vxt[base + g];
+      if (fargo) {
+        // the residual split in FP64 (src/compute_residual.c:68-73)
+        const double zs = zone_size_x64<G>(m, g, j, k);
+        const double ntilde = vmed * dt64 / zs;
+        const double nround = floor(ntilde + 0.5);
+        V[p] = (real)(v - vmed);
+        VU[p] = (real)((ntilde - nround) * zs / dt64);
+      } else {
+        V[p] = (real)v;
+      }
     }
   }
-  __syncthreads();
+  // a phase visits the owned positions in [lo, hi)
+#define TILE_FOR(lo, hi, ...)                                      \
+  _Pragma("unroll") for (int s = 0; s < 2; s++) {                  \
+    const int p = t + s * T;                                       \
+    if ((s == 0 || t < 2 * H) && p >= (lo) && p < (hi)) __VA_ARGS__ \
+  }
   for (int pass = 0; pass < (fargo ? 2 : 1); pass++) {
     const real* VV = pass ? VU : V;
     // valid range of the state entering this pass: [lo, hi)
     const int lo = pass ? 2 : 0, hi = pass ? W - 2 : W;
-    for (int p = lo + t; p < hi; p += T) {
-      const real r = ST[NQ * W + p];
-#pragma unroll
-      for (int c = 0; c < NQ; c++) AQ[c * W + p] = ST[c * W + p] / r;
+    TILE_FOR(lo, hi, {
+      const real r = st[s][NQ];
+_Pragma("unroll")
+      for (int c = 0; c < NQ; c++) AQ[c * W + p] = st[s][c] / r;
       AQ[NQ * W + p] = r;
-    }
-    __syncthreads();
-    int ulo, uhi;  // cells updated by this pass
+    })
+    __syncthreads();  // also orders the first pass after the loads of V, VU, ZS
+    int ulo, uhi;  // positions updated by this pass
     if (!pass) {
-      for (int p = 1 + t; p < W - 1; p += T) {
-#pragma unroll
+      TILE_FOR(1, W - 1, {
+_Pragma("unroll")
         for (int c = 0; c < N; c++) {
           const real* a = AQ + c * W;
           SL[c * W + p] = vanleer_slope((a[p] - a[p - 1]) / ZS[p], (a[p + 1] - a[p]) / ZS[p + 1]);
         }
-      }
+      })
       __syncthreads();
-      for (int p = 2 + t; p < W - 1; p += T) {
-#pragma unroll
+      TILE_FOR(2, W - 1, {
+_Pragma("unroll")
         for (int c = 0; c < N; c++) {
           const real* a = AQ + c * W;
           const real* sl = SL + c * W;
           FC[c * W + p] = vanleer_face(VV[p], dt, a[p - 1], sl[p - 1], ZS[p - 1], a[p], sl[p], ZS[p]);
         }
-      }
+      })
       __syncthreads();
       ulo = 2;
       uhi = W - 2;
     } else {
-      for (int p = 3 + t; p < W - 3; p += T) {
-#pragma unroll
+      TILE_FOR(3, W - 3, {
+_Pragma("unroll")
         for (int c = 0; c < N; c++) {
           const real* a = AQ + c * W;
           SL[c * W + p] = cell::ppa_slope(a[p - 1], a[p], a[p + 1]);
         }
-      }
+      })
       __syncthreads();
-      for (int p = 3 + t; p < W - 4; p += T) {
-#pragma unroll
+      TILE_FOR(3, W - 4, {
+_Pragma("unroll")
         for (int c = 0; c < N; c++) {
           const real* a = AQ + c * W;
           const real* sl = SL + c * W;
           FC[c * W + p] = cell::ppa_face(a[p], a[p + 1], sl[p], sl[p + 1]);
         }
-      }
+      })
       __syncthreads();
-      for (int p = 4 + t; p < W - 4; p += T) {
-#pragma unroll
+      // limited (qL, qR): qL into SL (slopes are no longer read), qR into FC
+      // once every neighbour has read its face
+      real qr[2][N];
+      TILE_FOR(4, W - 4, {
+_Pragma("unroll")
         for (int c = 0; c < N; c++) {
           real qL = FC[c * W + p - 1], qR = FC[c * W + p];
           cell::ppa_limit(AQ[c * W + p], qL, qR);
           SL[c * W + p] = qL;
-          QR2[c * W + p] = qR;
+          qr[s][c] = qR;
         }
-      }
+      })
       __syncthreads();
-      for (int p = 5 + t; p < W - 4; p += T) {
-#pragma unroll
+      TILE_FOR(4, W - 4, {
+_Pragma("unroll")
+        for (int c = 0; c < N; c++) FC[c * W + p] = qr[s][c];
+      })
+      __syncthreads();
+      // face fluxes into FC once every neighbour has read qR
+      TILE_FOR(5, W - 4, {
+_Pragma("unroll")
         for (int c = 0; c < N; c++) {
           const real* a = AQ + c * W;
           const real* qL = SL + c * W;
-          const real* qR = QR2 + c * W;
-          FC[c * W + p] = cell::ppa_flux_value(VV[p], dt, ZS[p], a[p - 1], qL[p - 1], qR[p - 1], a[p], qL[p], qR[p]);
+          const real* qR = FC + c * W;
+          qr[s][c] = cell::ppa_flux_value(VV[p], dt, ZS[p], a[p - 1], qL[p - 1], qR[p - 1], a[p], qL[p], qR[p]);
         }
-      }
+      })
+      __syncthreads();
+      TILE_FOR(5, W - 4, {
+_Pragma("unroll")
+        for (int c = 0; c < N; c++) FC[c * W + p] = qr[s][c];
+      })
       __syncthreads();
       ulo = 5;
       uhi = W - 5;
     }
-    // UpdateX / UpdateDensityX (src/update_x.c:69-71, src/update_density_x.c:66-68)
-    for (int p = ulo + t; p < uhi; p += T) {
+    // UpdateX / UpdateDensityX (src/update_x.c:69-71, src/update_density_x.c:66-68), on the registers
+    TILE_FOR(ulo, uhi, {
       const int g = gidx(p);
       const real v0 = VV[p], v1 = VV[p + 1];
       const real* ds = FC + NQ * W;
-#pragma unroll
+_Pragma("unroll")
       for (int c = 0; c < NQ; c++) {
         const real* f = FC + c * W;
-        ST[c * W + p] += cell::flux_x(m, g, j, k, dt, v0 * f[p] * ds[p], v1 * f[p + 1] * ds[p + 1]);
+        st[s][c] += cell::flux_x(m, g, j, k, dt, v0 * f[p] * ds[p], v1 * f[p + 1] * ds[p + 1]);
       }
-      ST[NQ * W + p] += cell::flux_x(m, g, j, k, dt, v0 * ds[p], v1 * ds[p + 1]);
-    }
-    __syncthreads();
+      st[s][NQ] += cell::flux_x(m, g, j, k, dt, v0 * ds[p], v1 * ds[p + 1]);
+    })
+    if (pass == 0 && fargo) __syncthreads();  // the PPA pass rewrites AQ
   }
-  // outputs of the tile's cells at their shifted positions
-  const int p = H + t;
-  const int i = i0 + t;
-  if (i < nx) {
-    int d = (i + ns) % nx;
-    if (d < 0) d += nx;
-    const real* R = ST + NQ * W;
-    vx_out[base + d] = cell::newvel_x<G>(m, d, j, k, R[p], R[p - 1], ST[1 * W + p], ST[0 * W + p - 1], omegaframe);
-    rho_out[base + d] = R[p];
-#pragma unroll
-    for (int c = 2; c < NQ; c++) {
-      if (c < NM) (c == 2 ? mpy_o : c == 3 ? mmy_o : c == 4 ? mpz_o : mmz_o)[base + d] = ST[c * W + p];
-      else e_out[base + d] = ST[c * W + p];
+  // the updated density and Mpx of positions [H-1, H+T) for NewVelocity_x
+  // (AQ is free: its last reads were before the final barrier of the pass)
+  TILE_FOR(H - 1, H + T, {
+    AQ[0 * W + p] = st[s][0];
+    AQ[1 * W + p] = st[s][NQ];
+  })
+  __syncthreads();
+  // outputs of the tile's cells [H, H+T) at their shifted positions
+  // (AdvectSHIFT, src/advect_shift.c:45-58), NewVelocity_x computed at the
+  // destination from the source cell and its left neighbour
+  TILE_FOR(H, H + T, {
+    const int i = i0 + p - H;
+    if (i < nx) {
+      int d = (i + ns) % nx;
+      if (d < 0) d += nx;
+      vx_out[base + d] = cell::newvel_x<G>(m, d, j, k, st[s][NQ], AQ[1 * W + p - 1], st[s][1], AQ[0 * W + p - 1],
+                                           omegaframe);
+      rho_out[base + d] = st[s][NQ];
+_Pragma("unroll")
+      for (int c = 2; c < NQ; c++) {
+        if (c < NM) (c == 2 ? mpy_o : c == 3 ? mmy_o : c == 4 ? mpz_o : mmz_o)[base + d] = st[s][c];
+        else e_out[base + d] = st[s][c];
+      }
     }
-  }
+  })
+#undef TILE_FOR
 }
 template <int G, int DZ, int AD, int T>
 __global__ void __launch_bounds__(T) kf_sweep_x_tile(dmesh m, double dt64, const real* __restrict__ rho_in, const real* __restrict__ vxt,
@@ -1687,25 +1726,46 @@ static int yz_sweeps(f3d_hip_ctx* ctx, Scratch* S, double dt, const RF& f, const
   return 0;
 }
 
-// X sweep by tiles (rings longer than the LDS ring kernel holds): VxMed of
-// each ring, then the tiles.
+// VxMed of Vx_temp for the tiled X sweep (one sequential sum per ring), on
+// stream st
+static int x_vmed(f3d_hip_ctx* ctx, Scratch* S, const RF& f, const TSet& t, hipStream_t st) {
+  const dmesh& m = S->m;
+  const int nrings = m.ny * (m.dimz ? m.nz : 1);
+  F3D_REQUIRE(ctx, (size_t)m.nx * sizeof(double) <= 160 * 1024, "ring too long for the VxMed pass");
+#define XV(G_, DZ_) \
+  hipLaunchKernelGGL((kf_vmed<G_, DZ_>), dim3(nrings), dim3(WG), m.nx * sizeof(double), st, m, f.Vx_temp, t.vxmed)
+  if (m.geom == SPH) {
+    if (m.dimz) XV(SPH, 1); else XV(SPH, 0);
+  } else {
+    if (m.dimz) XV(CYL, 1); else XV(CYL, 0);
+  }
+#undef XV
+  F3D_LAUNCH_CHECK(ctx);
+  return 0;
+}
+
+// tile width of the tiled X sweep (cells per workgroup): 64, 128 or 256
+static int g_xtile = 256;
+
+// X sweep by tiles (rings longer than the LDS ring kernel holds), once
+// VxMed of every ring is known.
 static int x_sweep_tiled(f3d_hip_ctx* ctx, Scratch* S, double dt, const RF& f, const TSet& t) {
   const dmesh& m = S->m;
   const real of = (real)ctx->ph.OMEGAFRAME;
   real* const* XM = t.xm;
   const int nrings = m.ny * (m.dimz ? m.nz : 1);
   const int ad = (ctx->ph.eos == 2);
-  constexpr int T = 128;
+  const int T = g_xtile;
   F3D_REQUIRE(ctx, m.nx >= 12, "the tiled X sweep needs at least 12 cells per ring");
-  F3D_REQUIRE(ctx, (size_t)m.nx * sizeof(double) <= 160 * 1024, "ring too long for the VxMed pass");
   f3d_stage_begin(ctx, F3D_ST_SWEEP_X);
-  if (ctx->ph.fargo)
-    FDISPATCH(ctx, kf_vmed, dim3(nrings), dim3(WG), m.nx * sizeof(double), f.Vx_temp, t.vxmed);
   dim3 g((m.nx + T - 1) / T, nrings);
-#define XTILE(G_, DZ_, AD_)                                                                                     \
-  hipLaunchKernelGGL((kf_sweep_x_tile<G_, DZ_, AD_, T>), g, dim3(T), 0, ctx->stream, m, dt, t.rhoy, f.Vx_temp,  \
+#define XTILE1(G_, DZ_, AD_, T_)                                                                                \
+  hipLaunchKernelGGL((kf_sweep_x_tile<G_, DZ_, AD_, T_>), g, dim3(T_), 0, ctx->stream, m, dt, t.rhoy, f.Vx_temp, \
                      t.m[0], t.m[1], t.m[2], t.m[3], t.m[4], t.m[5], t.ey, f.Energy, t.rhox, f.Vx, XM[0], XM[1],  \
                      XM[2], XM[3], t.vxmed, t.nshift, of, ctx->ph.fargo)
+#define XTILE(G_, DZ_, AD_) \
+  do { if (T == 64) XTILE1(G_, DZ_, AD_, 64); else if (T == 256) XTILE1(G_, DZ_, AD_, 256); \
+       else XTILE1(G_, DZ_, AD_, 128); } while (0)
   if (m.geom == SPH) {
     if (m.dimz) { if (ad) XTILE(SPH, 1, 1); else XTILE(SPH, 1, 0); }
     else { if (ad) XTILE(SPH, 0, 1); else XTILE(SPH, 0, 0); }
@@ -1714,8 +1774,19 @@ static int x_sweep_tiled(f3d_hip_ctx* ctx, Scratch* S, double dt, const RF& f, c
     else { if (ad) XTILE(CYL, 0, 1); else XTILE(CYL, 0, 0); }
   }
 #undef XTILE
+#undef XTILE1
   F3D_LAUNCH_CHECK(ctx);
   f3d_stage_end(ctx, F3D_ST_SWEEP_X);
+  return 0;
+}
+
+// the side stream and its events, created on first use
+static int side_stream(f3d_hip_ctx* ctx) {
+  if (!ctx->side) {
+    F3D_CHECK(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+    F3D_CHECK(ctx, hipEventCreateWithFlags(&ctx->side_go, hipEventDisableTiming));
+    F3D_CHECK(ctx, hipEventCreateWithFlags(&ctx->side_done, hipEventDisableTiming));
+  }
   return 0;
 }
 
@@ -1751,20 +1822,35 @@ __global__ void __launch_bounds__(CPT == 2 ? 512 : 256) kf_sweep_xr_fluids(dmesh
   }
 }
 
-// Ring X sweep of nf fluids (one launch), or -1 when the rings go to tiles.
+// ring X sweep form: the split (two workgroups per ring) and its LDS bytes
+static void x_ring_form(const f3d_hip_ctx* ctx, const dmesh& m, int* split, size_t* shx) {
+  const int ad = (ctx->ph.eos == 2);
+  const int nq = (m.dimz ? 6 : 4) + ad, na = m.dimz ? 3 : 2;
+  // measured on MI355X: the split pays while a thread owns one cell of the
+  // ring or two of a short one (nx <= 512); beyond, its extra density work
+  // costs more than the occupancy it gains
+  *split = (g_xsplit < 0) ? (m.nx <= 512 ? 1 : 0) : g_xsplit;
+  *shx = *split ? xsweep_ring_lds(m.nx, na > nq - na ? na : nq - na) : xsweep_ring_lds(m.nx, nq);
+}
+
+// true when the ring X sweep runs (the ring fits in LDS and no tiles are asked for)
+static bool x_ring_fits(const f3d_hip_ctx* ctx, Scratch* S) {
+  int split;
+  size_t shx;
+  x_ring_form(ctx, S->m, &split, &shx);
+  const bool fits = shx <= 160 * 1024 && S->m.nx <= 1024;
+  return g_xmode == 1 || (g_xmode == 0 && fits);
+}
+
+// Ring X sweep of nf fluids in one launch.
 static int x_sweep_ring(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const RF* f, const TSet* t) {
   const dmesh& m = S->m;
   const real of = (real)ctx->ph.OMEGAFRAME;
   const int ad = (ctx->ph.eos == 2);
-  const int nq = (m.dimz ? 6 : 4) + ad, na = m.dimz ? 3 : 2;
-  // measured on MI355X: the split pays while a thread owns one or two cells
-  // of the ring (nx <= 512); with four, the register pressure of the split
-  // costs more than the occupancy it gains
-  const int split = (g_xsplit < 0) ? (m.nx <= 512 ? 1 : 0) : g_xsplit;
-  const size_t shx = split ? xsweep_ring_lds(m.nx, na > nq - na ? na : nq - na) : xsweep_ring_lds(m.nx, nq);
-  const bool ring_fits = shx <= 160 * 1024 && m.nx <= 1024;
-  if (g_xmode == 2 || (g_xmode == 0 && !ring_fits)) return 1;
-  F3D_REQUIRE(ctx, ring_fits, "ring too long for the LDS X sweep");
+  int split;
+  size_t shx;
+  x_ring_form(ctx, m, &split, &shx);
+  F3D_REQUIRE(ctx, shx <= 160 * 1024 && m.nx <= 1024, "ring too long for the LDS X sweep");
   F3D_REQUIRE(ctx, nf >= 1 && nf <= 8, "X sweep: 1 to 8 fluids");
   // one cell per thread up to 256 cells, else two (up to 512 threads)
   const int cpt = (m.nx <= 256) ? 1 : 2;
@@ -1819,13 +1905,26 @@ static int transport_fluids(f3d_hip_ctx* ctx, double dt, int nf, const f3d_field
     f[k] = fl ? rfields_of(fl[k], ctx->f) : rfields(ctx);
     if (tset_of(ctx, S, f[k], k, nf, &t[k])) return -1;
   }
+  const bool tiles = !x_ring_fits(ctx, S);
+  if (tiles && ctx->ph.fargo) {
+    // VxMed of each fluid's Vx_temp (one long sequential sum per ring) on the
+    // side stream, overlapping the Z/Y sweeps, which do not write Vx_temp
+    if (side_stream(ctx)) return -1;
+    F3D_CHECK(ctx, hipEventRecord(ctx->side_go, ctx->stream));
+    F3D_CHECK(ctx, hipStreamWaitEvent(ctx->side, ctx->side_go, 0));
+    for (int k = 0; k < nf; k++)
+      if (x_vmed(ctx, S, f[k], t[k], ctx->side)) return -1;
+    F3D_CHECK(ctx, hipEventRecord(ctx->side_done, ctx->side));
+  }
   for (int k = 0; k < nf; k++)
     if (yz_sweeps(ctx, S, dt, f[k], t[k])) return -1;
-  const int rc = x_sweep_ring(ctx, S, dt, nf, f, t);
-  if (rc < 0) return -1;
-  if (rc == 1)
+  if (!tiles) {
+    if (x_sweep_ring(ctx, S, dt, nf, f, t)) return -1;
+  } else {
+    if (ctx->ph.fargo) F3D_CHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->side_done, 0));
     for (int k = 0; k < nf; k++)
       if (x_sweep_tiled(ctx, S, dt, f[k], t[k])) return -1;
+  }
   for (int k = 0; k < nf; k++)
     if (finish_step(ctx, S, dt, end_of_step, f[k], t[k])) return -1;
   return 0;
@@ -1842,6 +1941,12 @@ int F3D_API(fused_set_cflmode)(int mode) {
 int F3D_API(fused_set_xsplit)(int split) {
   if (split < -1 || split > 1) return -2;
   g_xsplit = split;
+  return 0;
+}
+
+int F3D_API(fused_set_xtile)(int cells) {
+  if (cells != 64 && cells != 128 && cells != 256) return -2;
+  g_xtile = cells;
   return 0;
 }
 

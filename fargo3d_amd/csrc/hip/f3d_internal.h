@@ -26,6 +26,11 @@ enum {
 
 struct f3d_hip_ctx {
   hipStream_t stream = nullptr;
+  // a second stream (default priority) for work that overlaps the main
+  // stream's, with the events that order it: side_go (main -> side) and
+  // side_done (side -> main); created on first use
+  hipStream_t side = nullptr;
+  hipEvent_t side_go = nullptr, side_done = nullptr;
   int device = 0;
   f3d::dmesh m{};
   double* meshbuf = nullptr;        // one device buffer for all 1-D mesh arrays

@@ -134,6 +134,10 @@ void f3d_hip_destroy(f3d_hip_ctx* c) {
   if (c->bctab) hipFree(c->bctab);
   if (c->dred) hipFree(c->dred);
   if (c->hres) hipHostFree(c->hres);
+  if (c->side) hipStreamSynchronize(c->side);
+  if (c->side_go) hipEventDestroy(c->side_go);
+  if (c->side_done) hipEventDestroy(c->side_done);
+  if (c->side) hipStreamDestroy(c->side);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }

@@ -221,6 +221,9 @@ int f3d_fused_set_xmode(int mode);
  * workgroups per CU), 0 one workgroup per ring, -1 automatic; results do not
  * depend on it.  Returns -2 for other values. */
 int f3d_fused_set_xsplit(int split);
+/* Cells per workgroup of the tiled X sweep: 64, 128 or 256 (default);
+ * results do not depend on it.  Returns -2 for other values. */
+int f3d_fused_set_xtile(int cells);
 /* CFL MIN over rings: 0 one 64-bit atomic MIN per ring, 1 ring minima to an
  * array and a one-workgroup second pass; the result is the same value.
  * Returns -2 for other values. */
@@ -275,6 +278,7 @@ int f3d32_fused_init_stockholm(f3d_hip_ctx *ctx);
 int f3d32_fused_set_tiling(int rows_y, int rows_z);
 int f3d32_fused_set_xmode(int mode);
 int f3d32_fused_set_xsplit(int split);
+int f3d32_fused_set_xtile(int cells);
 int f3d32_fused_set_cflmode(int mode);
 int f3d32_fused_cfl_vm(f3d_hip_ctx *ctx, const double *vxm, double *fluid_min);
 int f3d32_fused_cfl_fluids(f3d_hip_ctx *ctx, int nf, double *const *cs, double *const *rho, double *const *vx,
