@@ -15,8 +15,14 @@ Parity status (how each part is pinned; see tests/test_oracle.py):
   * AdvectSHIFT                   -- brute-force permutation, bit-exact
   * ComputeResidual / ComputeVmed -- brute-force sequential sums, |n~-n| <= 1/2
   * cfl                           -- closed form per cell
+  * SubStep1_x/y                  -- closed forms of the pressure, potential,
+                                     centrifugal and frame terms
+  * SubStep2_a/b                  -- one compressed cell: q, opposite face kicks,
+                                     heating
   * SubStep3                      -- closed form e(1-t)/(1+t)
   * viscosity                     -- solid-body rotation gives tau_rphi = 0
+  * Collisions / dust diffusion   -- two-fluid closed form, direction-wise drag;
+                                     uniform dust-to-gas ratio kept, mass conserved
   * boundaries                    -- closed forms of the ghost formulas
   * planets (host code)           -- Kepler orbit closed form
   * whole step                    -- invariants (mass conservation without
@@ -26,6 +32,10 @@ Parity status (how each part is pinned; see tests/test_oracle.py):
                                      code), so no bit-level reference output pins
                                      the whole step ("parity unpinned" at that
                                      level beyond the invariants above).
+  * parity unpinned individually: SubStep1_z and the Y/Z parts of
+    SubStep2_b (pinned only through the whole-step invariants and the
+    bit-exact agreement of two independent implementations, the oracle and
+    the HIP kernels, written from the reference routines).
 """
 import ctypes
 import os
