@@ -15,7 +15,7 @@ Parity status (how each part is pinned; see tests/test_oracle.py):
   * AdvectSHIFT                   -- brute-force permutation, bit-exact
   * ComputeResidual / ComputeVmed -- brute-force sequential sums, |n~-n| <= 1/2
   * cfl                           -- closed form per cell
-  * SubStep1_x/y                  -- closed forms of the pressure, potential,
+  * SubStep1_x/y/z                -- closed forms of the pressure, potential,
                                      centrifugal and frame terms
   * SubStep2_a/b                  -- one compressed cell: q, opposite face kicks,
                                      heating
@@ -32,8 +32,7 @@ Parity status (how each part is pinned; see tests/test_oracle.py):
                                      code), so no bit-level reference output pins
                                      the whole step ("parity unpinned" at that
                                      level beyond the invariants above).
-  * parity unpinned individually: SubStep1_z and the Y/Z parts of
-    SubStep2_b (pinned only through the whole-step invariants and the
+  * parity unpinned individually: the Y/Z parts of SubStep2_b (pinned only through the whole-step invariants and the
     bit-exact agreement of two independent implementations, the oracle and
     the HIP kernels, written from the reference routines).
 """
