@@ -189,7 +189,7 @@ static int batch_layout(f3d_batch* B) {
     c[BK_CFL][i] = m.ny;
     c[BK_SRCA][i] = gx * (m.sy - 2);
     c[BK_SRCB][i] = gx * m.ny;
-    c[BK_BND][i] = gx * 2 * m.nghy;
+    c[BK_BND][i] = gx * 2;  // one block row per side, all ghost layers per thread
     c[BK_VMED][i] = m.ny;
     c[BK_FIN][i] = gx * m.ny;
     swy[i] = gx * ((m.ny + B->rows_y - 1) / B->rows_y);

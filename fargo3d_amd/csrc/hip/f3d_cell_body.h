@@ -604,83 +604,116 @@ struct BcPar {
 // Y sides (scripts/boundparser.py:323-340 indices; std/boundaries.txt:35-40,
 // 62-66, 81-85 formulas; field lists setups/fargo/fargo.bound.0,
 // setups/p3disof/p3disof.bound.0)
-__device__ __forceinline__ void bound_y_cell(const dmesh& m, int side, int i, int mm, int k, real* dens, real* vx,
-                                             real* vy, real* vz, real* e, const BcPar& b, const real* dexp) {
+// One ghost cell of one side: its indices and the values it copies.  The
+// loads are separate from the stores so that a kernel can issue the loads of
+// every layer before any store (a thread's stores never touch what it reads).
+struct GhostCell {
+  int lgh, lghs, lnull, gh, act;  // ghost, staggered ghost, null face; ghost/active row or plane
+  bool ws;                        // the staggered ghost exists on this layer
+  real d0, x0, y0, z0, e0;
+};
+
+__device__ __forceinline__ GhostCell ghost_y_load(const dmesh& m, int side, int i, int mm, int k, const real* dens,
+                                                  const real* vx, const real* vy, const real* vz, const real* e,
+                                                  const BcPar& b) {
   const int pitch = m.pitch, stride = m.stride, ny = m.ny, nghy = m.nghy;
-  int lgh, lghs, lact, lacts, lnull, jgh, jact;
+  GhostCell c;
+  int lact, lacts;
   if (!side) {
-    lgh = lghs = i + mm * pitch + k * stride;
+    c.lgh = c.lghs = i + mm * pitch + k * stride;
     lact = i + (2 * nghy - mm - 1) * pitch + k * stride;
     lacts = i + (2 * nghy - mm) * pitch + k * stride;
-    lnull = i + nghy * pitch + k * stride;
-    jgh = mm;
-    jact = 2 * nghy - mm - 1;
+    c.lnull = i + nghy * pitch + k * stride;
+    c.gh = mm;
+    c.act = 2 * nghy - mm - 1;
   } else {
-    lgh = i + (ny + nghy + mm) * pitch + k * stride;
-    lghs = i + (ny + nghy + 1 + mm) * pitch + k * stride;
+    c.lgh = i + (ny + nghy + mm) * pitch + k * stride;
+    c.lghs = i + (ny + nghy + 1 + mm) * pitch + k * stride;
     lact = lacts = i + (ny + nghy - 1 - mm) * pitch + k * stride;
-    lnull = i + (ny + nghy) * pitch + k * stride;
-    jgh = ny + nghy + mm;
-    jact = ny + nghy - 1 - mm;
+    c.lnull = i + (ny + nghy) * pitch + k * stride;
+    c.gh = ny + nghy + mm;
+    c.act = ny + nghy - 1 - mm;
   }
+  c.ws = !side || mm < nghy - 1;
+  c.d0 = dens[lact];
+  c.x0 = vx[lact];
+  c.z0 = b.p3d ? vz[lact] : real(0.0);
+  c.y0 = c.ws ? vy[lacts] : real(0.0);
+  c.e0 = b.energy ? e[lact] : real(0.0);
+  return c;
+}
+
+__device__ __forceinline__ void ghost_y_store(const dmesh& m, int mm, int k, const GhostCell& c, real* dens, real* vx,
+                                              real* vy, real* vz, real* e, const BcPar& b, const real* dexp) {
   const real of = b.of;
-  // every load before the first store: a thread's stores never touch what it
-  // reads, and without this order each load would wait for the stores before it
-  const bool wvy = !side || mm < nghy - 1;
-  const real d0 = dens[lact], x0 = vx[lact];
-  const real z0 = b.p3d ? vz[lact] : real(0.0);
-  const real y0 = wvy ? vy[lacts] : real(0.0);
-  const real e0 = b.energy ? e[lact] : real(0.0);
   real dg, xg;
   if (!b.p3d) {
-    dg = d0 * b.dens[mm];
-    xg = (x0 + m.ymed[jact] * of) * b.vsq[mm] - m.ymed[jgh] * of;
+    dg = c.d0 * b.dens[mm];
+    xg = (c.x0 + m.ymed[c.act] * of) * b.vsq[mm] - m.ymed[c.gh] * of;
   } else {
-    dg = d0 * b.dens[mm] * dexp[mm * m.sz + k];
-    xg = (x0 + of * m.ymed[jact] * m.sinzmed[k]) * b.vsq[mm] * b.vfac[mm] - m.ymed[jgh] * of * m.sinzmed[k];
+    dg = c.d0 * b.dens[mm] * dexp[mm * m.sz + k];
+    xg = (c.x0 + of * m.ymed[c.act] * m.sinzmed[k]) * b.vsq[mm] * b.vfac[mm] - m.ymed[c.gh] * of * m.sinzmed[k];
   }
-  dens[lgh] = dg;
-  vx[lgh] = xg;
-  if (b.p3d) vz[lgh] = z0;
-  if (wvy) vy[lghs] = -y0;
-  vy[lnull] = 0;
-  if (b.energy) e[lgh] = e0;
+  dens[c.lgh] = dg;
+  vx[c.lgh] = xg;
+  if (b.p3d) vz[c.lgh] = c.z0;
+  if (c.ws) vy[c.lghs] = -c.y0;
+  vy[c.lnull] = 0;
+  if (b.energy) e[c.lgh] = c.e0;
+}
+
+__device__ __forceinline__ void bound_y_cell(const dmesh& m, int side, int i, int mm, int k, real* dens, real* vx,
+                                             real* vy, real* vz, real* e, const BcPar& b, const real* dexp) {
+  ghost_y_store(m, mm, k, ghost_y_load(m, side, i, mm, k, dens, vx, vy, vz, e, b), dens, vx, vy, vz, e, b, dexp);
 }
 
 // Z sides (scripts/boundparser.py:341-358; std/boundaries.txt:31-40, 87-91)
-__device__ __forceinline__ void bound_z_cell(const dmesh& m, int side, int i, int mm, int j, real* dens, real* vx,
-                                             real* vy, real* vz, real* e, const BcPar& b, const real* zdens) {
+__device__ __forceinline__ GhostCell ghost_z_load(const dmesh& m, int side, int i, int mm, int j, const real* dens,
+                                                  const real* vx, const real* vy, const real* vz, const real* e,
+                                                  const BcPar& b) {
   const int pitch = m.pitch, stride = m.stride, nz = m.nz, nghz = m.nghz;
-  int lgh, lghs, lact, lacts, lnull, kgh, kact;
+  GhostCell c;
+  int lact, lacts;
   if (!side) {
-    lgh = lghs = i + j * pitch + mm * stride;
+    c.lgh = c.lghs = i + j * pitch + mm * stride;
     lact = i + j * pitch + (2 * nghz - mm - 1) * stride;
     lacts = i + j * pitch + (2 * nghz - mm) * stride;
-    lnull = i + j * pitch + nghz * stride;
-    kgh = mm;
-    kact = 2 * nghz - mm - 1;
+    c.lnull = i + j * pitch + nghz * stride;
+    c.gh = mm;
+    c.act = 2 * nghz - mm - 1;
   } else {
-    lgh = i + j * pitch + (nz + nghz + mm) * stride;
-    lghs = i + j * pitch + (nz + nghz + 1 + mm) * stride;
+    c.lgh = i + j * pitch + (nz + nghz + mm) * stride;
+    c.lghs = i + j * pitch + (nz + nghz + 1 + mm) * stride;
     lact = lacts = i + j * pitch + (nz + nghz - 1 - mm) * stride;
-    lnull = i + j * pitch + (nz + nghz) * stride;
-    kgh = nz + nghz + mm;
-    kact = nz + nghz - 1 - mm;
+    c.lnull = i + j * pitch + (nz + nghz) * stride;
+    c.gh = nz + nghz + mm;
+    c.act = nz + nghz - 1 - mm;
   }
+  c.ws = !side || mm < nghz - 1;
+  c.d0 = dens[lact];
+  c.x0 = vx[lact];
+  c.y0 = vy[lact];
+  c.z0 = c.ws ? vz[lacts] : real(0.0);
+  c.e0 = b.energy ? e[lact] : real(0.0);
+  return c;
+}
+
+__device__ __forceinline__ void ghost_z_store(const dmesh& m, int mm, int j, const GhostCell& c, real* dens, real* vx,
+                                              real* vy, real* vz, real* e, const BcPar& b, const real* zdens) {
   const real of = b.of;
-  // every load before the first store (see bound_y_cell)
-  const bool wvz = !side || mm < nghz - 1;
-  const real d0 = dens[lact], x0 = vx[lact], y0 = vy[lact];
-  const real z0 = wvz ? vz[lacts] : real(0.0);
-  const real e0 = b.energy ? e[lact] : real(0.0);
-  const real dg = d0 * zdens[mm * m.sy + j];
-  const real xg = (x0 + of * m.ymed[j] * m.sinzmed[kact]) * b.zvfac[mm] - m.ymed[j] * of * m.sinzmed[kgh];
-  dens[lgh] = dg;
-  vx[lgh] = xg;
-  vy[lgh] = y0;
-  if (wvz) vz[lghs] = -z0;
-  vz[lnull] = 0;
-  if (b.energy) e[lgh] = e0;
+  const real dg = c.d0 * zdens[mm * m.sy + j];
+  const real xg = (c.x0 + of * m.ymed[j] * m.sinzmed[c.act]) * b.zvfac[mm] - m.ymed[j] * of * m.sinzmed[c.gh];
+  dens[c.lgh] = dg;
+  vx[c.lgh] = xg;
+  vy[c.lgh] = c.y0;
+  if (c.ws) vz[c.lghs] = -c.z0;
+  vz[c.lnull] = 0;
+  if (b.energy) e[c.lgh] = c.e0;
+}
+
+__device__ __forceinline__ void bound_z_cell(const dmesh& m, int side, int i, int mm, int j, real* dens, real* vx,
+                                             real* vy, real* vz, real* e, const BcPar& b, const real* zdens) {
+  ghost_z_store(m, mm, j, ghost_z_load(m, side, i, mm, j, dens, vx, vy, vz, e, b), dens, vx, vy, vz, e, b, zdens);
 }
 
 }  // namespace cell

@@ -41,6 +41,7 @@ namespace {
 
 constexpr int WG = 256;
 constexpr int BXF = 64;
+constexpr int kNgh = 3;  // ghost layers in Y and Z (src/define.h:57-63)
 
 // Scratch of the fused stages, n3 doubles each, allocated on a handle's
 // first fused call and released with the handle: Y- and X-swept density,
@@ -292,6 +293,26 @@ __global__ void __launch_bounds__(WG) kf_vmed_rings(dmesh m, const real* __restr
     w[i] = (double)v[base + i] * (m.xmin64[i + 1] - m.xmin64[i]);
   __syncthreads();
   if (threadIdx.x == 0) vxmed[j + k * m.sy] = seq_sum(w, m.nx) / (m.XMAX - m.XMIN);
+}
+
+// The cfl of nf fluids in one launch: nf consecutive blocks of nrings
+// workgroups, each block reading its fluid's arrays from the table.  Every
+// fluid's workgroup of a ring computes the same VxMed of vxm and stores the
+// same value.
+struct CflTab {
+  const real* cs[8];
+  const real* rho[8];
+  const real* vx[8];
+  const real* vy[8];
+  const real* vz[8];
+  double* ringmin[8];
+};
+template <int G, int DZ>
+__global__ void __launch_bounds__(WG) kf_cfl_fluids(dmesh m, CflTab T, int nrings, const real* vxm, double* vxmed,
+                                                    unsigned long long* result, int slot, cell::CflPar cp, int pre) {
+  const int fl = (int)blockIdx.x / nrings, r = (int)blockIdx.x % nrings;
+  kd_cfl<G, DZ>(dim3(r), m, T.cs[fl], T.rho[fl], T.vx[fl], T.vy[fl], T.vz[fl], vxm, vxmed, result, slot, cp,
+                T.ringmin[fl], pre);
 }
 
 // Second pass of the two-pass MIN: one workgroup takes the MIN of the n ring
@@ -1228,12 +1249,19 @@ __global__ void __launch_bounds__(WG) kf_finish(dmesh m, real dt, const real* __
 // The same per-cell formulas as the per-routine entry points (f3d_cell.h),
 // in this build's type.  Y: both sides in one launch when ny > nghy (the
 // sides then write disjoint rows and read none the other writes), Z likewise.
+// One thread per (i, side, plane or row) fills the three ghost layers of its
+// side: the loads of every layer first, then the stores.
 __device__ __forceinline__ void kd_bound_y(const dim3 bid, const dmesh& m, int side0, real* dens, real* vx, real* vy, real* vz, real* e, cell::BcPar bp0, cell::BcPar bp1, const real* dexp0, const real* dexp1) {
   const int i = bid.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
-  const int side = side0 + ((int)bid.y >= m.nghy);
-  const int mm = (int)bid.y - (side - side0) * m.nghy, k = bid.z;
-  cell::bound_y_cell(m, side, i, mm, k, dens, vx, vy, vz, e, side ? bp1 : bp0, side ? dexp1 : dexp0);
+  const int side = side0 + (int)bid.y, k = bid.z;
+  const cell::BcPar& b = side ? bp1 : bp0;
+  cell::GhostCell c[kNgh];
+#pragma unroll
+  for (int mm = 0; mm < kNgh; mm++) c[mm] = cell::ghost_y_load(m, side, i, mm, k, dens, vx, vy, vz, e, b);
+#pragma unroll
+  for (int mm = 0; mm < kNgh; mm++)
+    cell::ghost_y_store(m, mm, k, c[mm], dens, vx, vy, vz, e, b, side ? dexp1 : dexp0);
 }
 __global__ void kf_bound_y(dmesh m, int side0, real* dens, real* vx, real* vy, real* vz, real* e, cell::BcPar bp0,
                            cell::BcPar bp1, const real* dexp0, const real* dexp1) {
@@ -1244,9 +1272,14 @@ __global__ void kf_bound_z(dmesh m, int side0, real* dens, real* vx, real* vy, r
                            cell::BcPar bp1, const real* zdens0, const real* zdens1) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
-  const int side = side0 + ((int)blockIdx.z >= m.nghz);
-  const int mm = (int)blockIdx.z - (side - side0) * m.nghz, j = blockIdx.y;
-  cell::bound_z_cell(m, side, i, mm, j, dens, vx, vy, vz, e, side ? bp1 : bp0, side ? zdens1 : zdens0);
+  const int side = side0 + (int)blockIdx.z, j = blockIdx.y;
+  const cell::BcPar& b = side ? bp1 : bp0;
+  cell::GhostCell c[kNgh];
+#pragma unroll
+  for (int mm = 0; mm < kNgh; mm++) c[mm] = cell::ghost_z_load(m, side, i, mm, j, dens, vx, vy, vz, e, b);
+#pragma unroll
+  for (int mm = 0; mm < kNgh; mm++)
+    cell::ghost_z_store(m, mm, j, c[mm], dens, vx, vy, vz, e, b, side ? zdens1 : zdens0);
 }
 
 // Potential (src/potential.c:51-208) on every cell
@@ -1560,33 +1593,35 @@ static int fused_boundaries(f3d_hip_ctx* ctx, Scratch* S, real* dens, real* vx, 
   constexpr int BX = 64;
   dim3 blk(BX);
   const cell::BcPar b0 = bc_par_f(ctx, 0), b1 = bc_par_f(ctx, 1);
+  F3D_REQUIRE(ctx, m.nghy == kNgh && (!m.dimz || m.nghz == kNgh), "ghost fill: three ghost layers");
   if (!ctx->ph.periodicz && m.dimz) {
     F3D_REQUIRE(ctx, ctx->ph.bc == 2, "3-D boundaries need the spherical-disk family");
+    // both sides in one launch when they read and write disjoint planes
     if (m.nz > m.nghz) {
-      hipLaunchKernelGGL(kf_bound_z, dim3((m.nx + BX - 1) / BX, m.sy, 2 * m.nghz), blk, 0, ctx->stream, m, 0, dens, vx,
-                         vy, vz, e, b0, b1, S->zmin_dens, S->zmax_dens);
+      hipLaunchKernelGGL(kf_bound_z, dim3((m.nx + BX - 1) / BX, m.sy, 2), blk, 0, ctx->stream, m, 0, dens, vx, vy, vz,
+                         e, b0, b1, S->zmin_dens, S->zmax_dens);
       F3D_LAUNCH_CHECK(ctx);
     } else {
       for (int side = 0; side < 2; side++) {
         const cell::BcPar b = side ? b1 : b0;
         const real* t = side ? S->zmax_dens : S->zmin_dens;
-        hipLaunchKernelGGL(kf_bound_z, dim3((m.nx + BX - 1) / BX, m.sy, m.nghz), blk, 0, ctx->stream, m, side, dens,
-                           vx, vy, vz, e, b, b, t, t);
+        hipLaunchKernelGGL(kf_bound_z, dim3((m.nx + BX - 1) / BX, m.sy, 1), blk, 0, ctx->stream, m, side, dens, vx,
+                           vy, vz, e, b, b, t, t);
         F3D_LAUNCH_CHECK(ctx);
       }
     }
   }
   if (!ctx->ph.periodicy) {
     if (m.ny > m.nghy) {
-      hipLaunchKernelGGL(kf_bound_y, dim3((m.nx + BX - 1) / BX, 2 * m.nghy, m.dimz ? m.sz : 1), blk, 0, ctx->stream,
-                         m, 0, dens, vx, vy, vz, e, b0, b1, S->ymin_dexp, S->ymax_dexp);
+      hipLaunchKernelGGL(kf_bound_y, dim3((m.nx + BX - 1) / BX, 2, m.dimz ? m.sz : 1), blk, 0, ctx->stream, m, 0,
+                         dens, vx, vy, vz, e, b0, b1, S->ymin_dexp, S->ymax_dexp);
       F3D_LAUNCH_CHECK(ctx);
     } else {
       for (int side = 0; side < 2; side++) {
         const cell::BcPar b = side ? b1 : b0;
         const real* t = side ? S->ymax_dexp : S->ymin_dexp;
-        hipLaunchKernelGGL(kf_bound_y, dim3((m.nx + BX - 1) / BX, m.nghy, m.dimz ? m.sz : 1), blk, 0, ctx->stream,
-                           m, side, dens, vx, vy, vz, e, b, b, t, t);
+        hipLaunchKernelGGL(kf_bound_y, dim3((m.nx + BX - 1) / BX, 1, m.dimz ? m.sz : 1), blk, 0, ctx->stream, m,
+                           side, dens, vx, vy, vz, e, b, b, t, t);
         F3D_LAUNCH_CHECK(ctx);
       }
     }
@@ -2013,12 +2048,18 @@ int F3D_API(fused_cfl_fluids)(f3d_hip_ctx* ctx, int nf, double* const* cs, doubl
   const int mode = cfl_mode(m);
   const int pre = (mode & 2) ? 1 : 0;
   if (pre && vmed_prepass(ctx, S, vxm)) return -1;
+  CflTab T;
+  memset(&T, 0, sizeof(T));
   for (int k = 0; k < nf; k++) {
-    real* vzk = m.dimz ? R(vz[k]) : R(vy[k]);
-    double* rm = (mode & 1) ? S->ringmin + (size_t)k * nrings : nullptr;
-    FDISPATCH(ctx, kf_cfl, dim3(nrings), dim3(WG), pre ? 16 * sizeof(double) : shmem, R(cs[k]), R(rho[k]), R(vx[k]),
-              R(vy[k]), vzk, vxm, f.VxMed, S->result, slot, cfl_par_f(ctx), rm, pre);
+    T.cs[k] = R(cs[k]);
+    T.rho[k] = R(rho[k]);
+    T.vx[k] = R(vx[k]);
+    T.vy[k] = R(vy[k]);
+    T.vz[k] = m.dimz ? R(vz[k]) : R(vy[k]);
+    T.ringmin[k] = (mode & 1) ? S->ringmin + (size_t)k * nrings : nullptr;
   }
+  FDISPATCH(ctx, kf_cfl_fluids, dim3(nrings * nf), dim3(WG), pre ? 16 * sizeof(double) : shmem, T, nrings, vxm,
+            f.VxMed, S->result, slot, cfl_par_f(ctx), pre);
   if (mode & 1) {
     hipLaunchKernelGGL(kf_cfl_fin, dim3(1), dim3(1024), 0, ctx->stream, S->ringmin, nf * nrings, S->result, slot);
     F3D_LAUNCH_CHECK(ctx);
