@@ -543,6 +543,39 @@ __device__ __forceinline__ void column_sweep(Col& col, int r0, int r1, real dt) 
   }
 }
 
+// A column carrying part of the quantities: COUNT of them from FIRST on,
+// plus the density, which every part needs for q = Q/rho and the mass flux;
+// only the part with WRITE_RHO stores the density.  Splitting a column's
+// quantities over two threads doubles the threads of a sweep (more waves to
+// hide the FP64 latency) for one extra density sweep per column.
+template <class Col, int NQ, int FIRST, int COUNT, bool WRITE_RHO>
+struct ColPart {
+  const Col& c;
+  __device__ __forceinline__ real rho(int r) const { return c.rho(r); }
+  __device__ __forceinline__ real Q(int q, int r) const { return c.Q(FIRST + q, r); }
+  __device__ __forceinline__ real zs(int r) const { return c.zs(r); }
+  __device__ __forceinline__ real v(int r) const { return c.v(r); }
+  __device__ __forceinline__ real flux(int r, real a, real b) const { return c.flux(r, a, b); }
+  __device__ __forceinline__ void out(int q, int r, real val) const {
+    if (q < COUNT) c.out(FIRST + q, r, val);
+    else if (WRITE_RHO) c.out(NQ, r, val);
+  }
+};
+
+// the sweep of part grp of a column: 0 every quantity, 1 the first half and
+// the density, 2 the second half
+template <int NQ, class Col>
+__device__ __forceinline__ void column_sweep_part(const Col& col, int grp, int r0, int r1, real dt) {
+  constexpr int H = NQ / 2;
+  if (grp == 1) {
+    ColPart<Col, NQ, 0, H, true> p{col};
+    column_sweep<H>(p, r0, r1, dt);
+  } else {
+    ColPart<Col, NQ, H, NQ - H, false> p{col};
+    column_sweep<NQ - H>(p, r0, r1, dt);
+  }
+}
+
 // Y column of the 2-D step: momenta computed from rho and V_temp on the fly
 // (src/momenta_x.c:76-79, src/momenta_y.c:62-63); quantity 4 is the energy
 // (adiabatic).  out_[c] receives quantity c, out_[NQ] the density.
@@ -639,26 +672,28 @@ struct ColY3 {
 
 // 2-D: momenta + Y sweep; rows [nghy + s*rows_per, ...) of the active range.
 // AD = 1 also transports the energy (e_in -> e_out).
-template <int G, int DZ, int AD>
+template <int G, int DZ, int AD, int SPLIT = 0>
 __device__ __forceinline__ void kd_sweep_y2(const dim3 bid, const dmesh& m, real dt, const real* rho_in, const real* vxt, const real* vyt, const real* e_in, real* rho_out, real* q0, real* q1, real* q2, real* q3, real* e_out, int rows_per, real of) {
   const int i = bid.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
-  const int r0 = m.nghy + (int)bid.y * rows_per;
+  const int seg = SPLIT ? (int)bid.y >> 1 : (int)bid.y;
+  const int r0 = m.nghy + seg * rows_per;
   const int r1 = min(r0 + rows_per, m.ny + m.nghy);
   ColY2<G> col{&m, rho_in, vxt, vyt, e_in, {q0, q1, q2, q3, AD ? e_out : rho_out, rho_out}, i, ixp_of(i, m.nx), of,
                dt};
-  column_sweep<4 + AD>(col, r0, r1, dt);
+  if (SPLIT) column_sweep_part<4 + AD>(col, 1 + ((int)bid.y & 1), r0, r1, dt);
+  else column_sweep<4 + AD>(col, r0, r1, dt);
 }
-template <int G, int DZ, int AD>
+template <int G, int DZ, int AD, int SPLIT>
 __global__ void __launch_bounds__(BXF) kf_sweep_y2(dmesh m, real dt, const real* __restrict__ rho_in, const real* __restrict__ vxt,
                                                    const real* __restrict__ vyt, const real* __restrict__ e_in, real* __restrict__ rho_out,
                                                    real* __restrict__ q0, real* __restrict__ q1, real* __restrict__ q2, real* __restrict__ q3, real* __restrict__ e_out,
                                                    int rows_per, real of) {
-  kd_sweep_y2<G, DZ, AD>(blockIdx, m, dt, rho_in, vxt, vyt, e_in, rho_out, q0, q1, q2, q3, e_out, rows_per, of);
+  kd_sweep_y2<G, DZ, AD, SPLIT>(blockIdx, m, dt, rho_in, vxt, vyt, e_in, rho_out, q0, q1, q2, q3, e_out, rows_per, of);
 }
 
 // 3-D: Y sweep of the Z-swept fields
-template <int G, int DZ, int AD>
+template <int G, int DZ, int AD, int SPLIT>
 __global__ void __launch_bounds__(BXF) kf_sweep_y3(dmesh m, real dt, const real* __restrict__ rho_in, const real* __restrict__ z0,
                                                    const real* __restrict__ z1, const real* __restrict__ z2, const real* __restrict__ z3,
                                                    const real* __restrict__ z4, const real* __restrict__ z5, const real* __restrict__ ze,
@@ -668,27 +703,30 @@ __global__ void __launch_bounds__(BXF) kf_sweep_y3(dmesh m, real dt, const real*
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int k = m.nghz + (int)blockIdx.z;
-  const int r0 = m.nghy + (int)blockIdx.y * rows_per;
+  const int seg = SPLIT ? (int)blockIdx.y >> 1 : (int)blockIdx.y;
+  const int r0 = m.nghy + seg * rows_per;
   const int r1 = min(r0 + rows_per, m.ny + m.nghy);
   ColY3 col{&m, rho_in, {z0, z1, z2, z3, z4, z5, ze}, vyt,
             {q0, q1, q2, q3, q4, q5, AD ? e_out : rho_out, rho_out}, i, k, dt};
-  column_sweep<6 + AD>(col, r0, r1, dt);
+  if (SPLIT) column_sweep_part<6 + AD>(col, 1 + ((int)blockIdx.y & 1), r0, r1, dt);
+  else column_sweep<6 + AD>(col, r0, r1, dt);
 }
 
 // 3-D: momenta + Z sweep for rows j in [1, sy-2] (the Y sweep's stencil)
-template <int G, int DZ, int AD>
+template <int G, int DZ, int AD, int SPLIT>
 __global__ void __launch_bounds__(BXF) kf_sweep_z(dmesh m, real dt, const real* __restrict__ rho_in, const real* __restrict__ vxt,
                                                   const real* __restrict__ vyt, const real* __restrict__ vzt, const real* __restrict__ e_in,
                                                   real* __restrict__ rho_out, real* __restrict__ q0, real* __restrict__ q1, real* __restrict__ q2, real* __restrict__ q3,
                                                   real* __restrict__ q4, real* __restrict__ q5, real* __restrict__ e_out, int planes_per, real of) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
-  const int j = 1 + (int)blockIdx.y;
+  const int j = 1 + (SPLIT ? (int)blockIdx.y >> 1 : (int)blockIdx.y);
   const int r0 = m.nghz + (int)blockIdx.z * planes_per;
   const int r1 = min(r0 + planes_per, m.nz + m.nghz);
   ColZ<G> col{&m, rho_in, vxt, vyt, vzt, e_in, {q0, q1, q2, q3, q4, q5, AD ? e_out : rho_out, rho_out}, i, j,
               ixp_of(i, m.nx), of, dt};
-  column_sweep<6 + AD>(col, r0, r1, dt);
+  if (SPLIT) column_sweep_part<6 + AD>(col, 1 + ((int)blockIdx.y & 1), r0, r1, dt);
+  else column_sweep<6 + AD>(col, r0, r1, dt);
 }
 
 // The ring X sweep with the cell state in registers.  Thread t owns cells
@@ -1522,6 +1560,30 @@ __global__ void kf_bound_y_dens(dmesh m, real* dens, cell::BcPar bp0, cell::BcPa
     F3D_LAUNCH_CHECK(ctx);                                                                                \
   } while (0)
 
+// FDISPATCH_AD with one more template argument T_ after the AD flag
+#define FDISPATCH_AD_T(ctx, KERN, T_, grid, block, shmem, ...)                                                    \
+  do {                                                                                                            \
+    const bool ad_ = ((ctx)->ph.eos == 2);                                                                        \
+    if (m.geom == SPH) {                                                                                          \
+      if (m.dimz) {                                                                                               \
+        if (ad_) hipLaunchKernelGGL((KERN<SPH, 1, 1, T_>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);   \
+        else hipLaunchKernelGGL((KERN<SPH, 1, 0, T_>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);       \
+      } else {                                                                                                    \
+        if (ad_) hipLaunchKernelGGL((KERN<SPH, 0, 1, T_>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);   \
+        else hipLaunchKernelGGL((KERN<SPH, 0, 0, T_>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);       \
+      }                                                                                                           \
+    } else {                                                                                                      \
+      if (m.dimz) {                                                                                               \
+        if (ad_) hipLaunchKernelGGL((KERN<CYL, 1, 1, T_>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);   \
+        else hipLaunchKernelGGL((KERN<CYL, 1, 0, T_>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);       \
+      } else {                                                                                                    \
+        if (ad_) hipLaunchKernelGGL((KERN<CYL, 0, 1, T_>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);   \
+        else hipLaunchKernelGGL((KERN<CYL, 0, 0, T_>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);       \
+      }                                                                                                           \
+    }                                                                                                             \
+    F3D_LAUNCH_CHECK(ctx);                                                                                        \
+  } while (0)
+
 static cell::CflPar cfl_par_f(const f3d_hip_ctx* ctx) {
   cell::CflPar c;
   c.cfl = ctx->ph.cfl;
@@ -1631,6 +1693,10 @@ static int fused_boundaries(f3d_hip_ctx* ctx, Scratch* S, real* dens, real* vx, 
 
 // tuning knobs of the column sweeps (rows per thread); set by the tuner
 static int g_rows_y = 4, g_rows_z = 3;
+// column sweeps: 1 two threads per column (each with the density and half of
+// the quantities), 0 one, -1 automatic: measured on MI355X, the split only
+// pays when a sweep has fewer waves than the GPU has SIMDs (small 2-D grids)
+static int g_colsplit = -1;
 // X-sweep kernel choice: 0 automatic, 1 ring in LDS, 2 tiles
 static int g_xmode = 0;
 // ring X sweep: 1 two workgroups per ring (half of the quantities each), 0
@@ -1738,26 +1804,36 @@ static int yz_sweeps(f3d_hip_ctx* ctx, Scratch* S, double dt, const RF& f, const
   const real of = (real)ctx->ph.OMEGAFRAME;
   const real rdt = (real)dt;
   const int ry = g_rows_y, rz = g_rows_z;
+  const long waves = (long)((m.nx + BXF - 1) / BXF) * (m.dimz ? (long)m.nz * ((m.ny + ry - 1) / ry)
+                                                               : (long)((m.ny + ry - 1) / ry));
+  const int split = g_colsplit >= 0 ? g_colsplit : (waves < 1024 ? 1 : 0);
+  const int sp = split ? 2 : 1;  // threads per column
+#define FDISPATCH_ADS(ctx_, KERN, grid, ...)                                             \
+  do {                                                                                  \
+    if (split) FDISPATCH_AD_T(ctx_, KERN, 1, grid, dim3(BXF), 0, __VA_ARGS__);         \
+    else FDISPATCH_AD_T(ctx_, KERN, 0, grid, dim3(BXF), 0, __VA_ARGS__);               \
+  } while (0)
   if (m.dimz) {
     // eight separate buffers: six Z-swept momenta, the Z-swept density and energy
     real* const* Z = &S->f[FS_Z0];
-    dim3 gz((m.nx + BXF - 1) / BXF, m.sy - 2, (m.nz + rz - 1) / rz);
+    dim3 gz((m.nx + BXF - 1) / BXF, (m.sy - 2) * sp, (m.nz + rz - 1) / rz);
     f3d_stage_begin(ctx, F3D_ST_SWEEP_Z);
-    FDISPATCH_AD(ctx, kf_sweep_z, gz, dim3(BXF), 0, rdt, f.Density, f.Vx_temp, f.Vy_temp, f.Vz_temp, f.Energy, Z[6],
-                 Z[0], Z[1], Z[2], Z[3], Z[4], Z[5], Z[7], rz, of);
+    FDISPATCH_ADS(ctx, kf_sweep_z, gz, rdt, f.Density, f.Vx_temp, f.Vy_temp, f.Vz_temp, f.Energy, Z[6], Z[0], Z[1],
+                  Z[2], Z[3], Z[4], Z[5], Z[7], rz, of);
     f3d_stage_end(ctx, F3D_ST_SWEEP_Z);
-    dim3 gy((m.nx + BXF - 1) / BXF, (m.ny + ry - 1) / ry, m.nz);
+    dim3 gy((m.nx + BXF - 1) / BXF, ((m.ny + ry - 1) / ry) * sp, m.nz);
     f3d_stage_begin(ctx, F3D_ST_SWEEP_Y);
-    FDISPATCH_AD(ctx, kf_sweep_y3, gy, dim3(BXF), 0, rdt, Z[6], Z[0], Z[1], Z[2], Z[3], Z[4], Z[5], Z[7], f.Vy_temp,
-                 t.rhoy, t.m[0], t.m[1], t.m[2], t.m[3], t.m[4], t.m[5], t.ey, ry);
+    FDISPATCH_ADS(ctx, kf_sweep_y3, gy, rdt, Z[6], Z[0], Z[1], Z[2], Z[3], Z[4], Z[5], Z[7], f.Vy_temp, t.rhoy,
+                  t.m[0], t.m[1], t.m[2], t.m[3], t.m[4], t.m[5], t.ey, ry);
     f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
   } else {
-    dim3 gy((m.nx + BXF - 1) / BXF, (m.ny + ry - 1) / ry, 1);
+    dim3 gy((m.nx + BXF - 1) / BXF, ((m.ny + ry - 1) / ry) * sp, 1);
     f3d_stage_begin(ctx, F3D_ST_SWEEP_Y);
-    FDISPATCH_AD(ctx, kf_sweep_y2, gy, dim3(BXF), 0, rdt, f.Density, f.Vx_temp, f.Vy_temp, f.Energy, t.rhoy,
-                 t.m[0], t.m[1], t.m[2], t.m[3], t.ey, ry, of);
+    FDISPATCH_ADS(ctx, kf_sweep_y2, gy, rdt, f.Density, f.Vx_temp, f.Vy_temp, f.Energy, t.rhoy, t.m[0], t.m[1],
+                  t.m[2], t.m[3], t.ey, ry, of);
     f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
   }
+#undef FDISPATCH_ADS
   return 0;
 }
 
@@ -1976,6 +2052,12 @@ int F3D_API(fused_set_cflmode)(int mode) {
 int F3D_API(fused_set_xsplit)(int split) {
   if (split < -1 || split > 1) return -2;
   g_xsplit = split;
+  return 0;
+}
+
+int F3D_API(fused_set_colsplit)(int split) {
+  if (split < -1 || split > 1) return -2;
+  g_colsplit = split;
   return 0;
 }
 

@@ -224,6 +224,11 @@ int f3d_fused_set_xsplit(int split);
 /* Cells per workgroup of the tiled X sweep: 64, 128 or 256 (default);
  * results do not depend on it.  Returns -2 for other values. */
 int f3d_fused_set_xtile(int cells);
+/* Column (Y and Z) sweeps: 1 two threads per column, each with the density
+ * and half of the quantities, 0 one thread per column, -1 automatic (split
+ * only for sweeps with fewer waves than SIMDs); results do not depend on it.
+ * Returns -2 for other values. */
+int f3d_fused_set_colsplit(int split);
 /* CFL MIN over rings: 0 one 64-bit atomic MIN per ring, 1 ring minima to an
  * array and a one-workgroup second pass; the result is the same value.
  * Returns -2 for other values. */
@@ -279,6 +284,7 @@ int f3d32_fused_set_tiling(int rows_y, int rows_z);
 int f3d32_fused_set_xmode(int mode);
 int f3d32_fused_set_xsplit(int split);
 int f3d32_fused_set_xtile(int cells);
+int f3d32_fused_set_colsplit(int split);
 int f3d32_fused_set_cflmode(int mode);
 int f3d32_fused_cfl_vm(f3d_hip_ctx *ctx, const double *vxm, double *fluid_min);
 int f3d32_fused_cfl_fluids(f3d_hip_ctx *ctx, int nf, double *const *cs, double *const *rho, double *const *vx,

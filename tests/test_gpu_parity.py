@@ -362,3 +362,17 @@ def test_whole_step_fused_ring_split(fa, orc, par, ov, nsteps, split):
         _fused_run(fa, orc, par, ov, nsteps)
     finally:
         L.f3d_fused_set_xsplit(-1)
+
+
+@pytest.mark.parametrize("split", [0, 1], ids=["one_thread_per_column", "two_threads_per_column"])
+@pytest.mark.parametrize("par,ov,nsteps", [(C1, "", 8), (C2, C2_SMALL, 4), (C2, C2_AD, 4), (C3, C3_SMALL, 6)],
+                         ids=["c1_2d", "c2_3d_small", "c2_3d_adiabatic_small", "c3_2d_adiabatic_small"])
+def test_whole_step_fused_column_split(fa, orc, par, ov, nsteps, split):
+    """The Y/Z column sweeps with every quantity in one thread, or split over
+    two threads per column, give the oracle's results."""
+    L = fa.lib()
+    assert L.f3d_fused_set_colsplit(split) == 0
+    try:
+        _fused_run(fa, orc, par, ov, nsteps)
+    finally:
+        L.f3d_fused_set_colsplit(-1)
