@@ -132,6 +132,9 @@ def timed_run(fa, L, names, parfile, overrides, arch, steps, warmup, probe=20):
     the timed region carries one pair per step).  Returns the wall time, the
     per-stage averages (the dominant stage's from the timed region) and the
     simulation."""
+    # the handle is a pointer: every timing entry point gets its C signature
+    L.f3d_hip_timing_mask.argtypes = [ctypes.c_void_p, ctypes.c_uint]
+    L.f3d_hip_timing_read.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     sim = fa.Simulation(parfile, overrides, arch=arch)
     for _ in range(warmup):
         sim.step()

@@ -318,7 +318,7 @@ __global__ void __launch_bounds__(WG) kf_cfl_fluids(dmesh m, CflTab T, int nring
 // Second pass of the two-pass MIN: one workgroup takes the MIN of the n ring
 // minima (order independent, so equal to the atomic form) into result[slot].
 __global__ void __launch_bounds__(1024) kf_cfl_fin(const double* ringmin, int n, unsigned long long* result,
-                                                   int slot) {
+                                                   int slot, double* host_out) {
   __shared__ double sh[16];
   double v = __longlong_as_double((long long)kInfBits);
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -334,6 +334,9 @@ __global__ void __launch_bounds__(1024) kf_cfl_fin(const double* ringmin, int n,
   if (threadIdx.x == 0) {
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) v = (sh[w] < v) ? sh[w] : v;
     result[slot] = (unsigned long long)__double_as_longlong(v);
+    // straight into pinned host memory: the host reads it after the stream
+    // synchronises, with no copy launch
+    if (host_out) *host_out = v;
   }
 }
 
@@ -2101,11 +2104,12 @@ int F3D_API(fused_cfl_vm)(f3d_hip_ctx* ctx, const double* vxm_in, double* fluid_
   FDISPATCH(ctx, kf_cfl, dim3(nrings), dim3(WG), pre ? 16 * sizeof(double) : shmem, f.Energy, f.Density, f.Vx, f.Vy,
             f.Vz, vxm ? vxm : f.Vx, f.VxMed, S->result, slot, cfl_par_f(ctx), rm, pre);
   if (rm) {
-    hipLaunchKernelGGL(kf_cfl_fin, dim3(1), dim3(1024), 0, ctx->stream, rm, nrings, S->result, slot);
+    hipLaunchKernelGGL(kf_cfl_fin, dim3(1), dim3(1024), 0, ctx->stream, rm, nrings, S->result, slot, ctx->dhres);
     F3D_LAUNCH_CHECK(ctx);
   }
   f3d_stage_end(ctx, F3D_ST_CFL);
-  F3D_CHECK(ctx, hipMemcpyAsync(ctx->hres, S->result + slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (!rm)
+    F3D_CHECK(ctx, hipMemcpyAsync(ctx->hres, S->result + slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   F3D_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   S->slot ^= 1;
   *fluid_min = ctx->hres[0];
@@ -2143,11 +2147,13 @@ int F3D_API(fused_cfl_fluids)(f3d_hip_ctx* ctx, int nf, double* const* cs, doubl
   FDISPATCH(ctx, kf_cfl_fluids, dim3(nrings * nf), dim3(WG), pre ? 16 * sizeof(double) : shmem, T, nrings, vxm,
             f.VxMed, S->result, slot, cfl_par_f(ctx), pre);
   if (mode & 1) {
-    hipLaunchKernelGGL(kf_cfl_fin, dim3(1), dim3(1024), 0, ctx->stream, S->ringmin, nf * nrings, S->result, slot);
+    hipLaunchKernelGGL(kf_cfl_fin, dim3(1), dim3(1024), 0, ctx->stream, S->ringmin, nf * nrings, S->result, slot,
+                       ctx->dhres);
     F3D_LAUNCH_CHECK(ctx);
   }
   f3d_stage_end(ctx, F3D_ST_CFL);
-  F3D_CHECK(ctx, hipMemcpyAsync(ctx->hres, S->result + slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (!(mode & 1))
+    F3D_CHECK(ctx, hipMemcpyAsync(ctx->hres, S->result + slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   F3D_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   S->slot ^= 1;
   *step_min = ctx->hres[0];

@@ -43,6 +43,7 @@ struct f3d_hip_ctx {
   const double* d_zmax_dens = nullptr;
   double* dred = nullptr;           // device reduction workspace (n2d + 64 doubles)
   double* hres = nullptr;           // pinned host result slots (64 doubles)
+  double* dhres = nullptr;          // the same slots as the device addresses them
   size_t n3 = 0, n2 = 0;
   size_t meshlen = 0;                // doubles in meshbuf
   char err[512] = {0};

@@ -26,6 +26,9 @@ int f3d_hip_create(const f3d_mesh_desc* d, f3d_hip_ctx** out) {
     snprintf(g_create_err, sizeof(g_create_err), "f3d_hip_create: no HIP device is visible");
     return -3;
   }
+  // spin while the host waits on the stream: the step's one dt read-back is
+  // on the critical path (harmless if the device is already initialised)
+  if (hipSetDeviceFlags(hipDeviceScheduleSpin) != hipSuccess) (void)hipGetLastError();
   f3d_hip_ctx* c = new f3d_hip_ctx();
   hipGetDevice(&c->device);
   const int sx = d->nx, sy = d->ny + 2 * d->nghy, sz = d->nz + 2 * d->nghz;
@@ -66,6 +69,7 @@ int f3d_hip_create(const f3d_mesh_desc* d, f3d_hip_ctx** out) {
   if (e == hipSuccess) e = hipMemcpy(c->meshbuf, host.data(), total * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(&c->dred, (c->n2 + 64) * sizeof(double));
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->hres, 64 * sizeof(double), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&c->dhres, c->hres, 0);
   if (e != hipSuccess) {
     snprintf(g_create_err, sizeof(g_create_err), "f3d_hip_create: %s", hipGetErrorString(e));
     f3d_hip_destroy(c);

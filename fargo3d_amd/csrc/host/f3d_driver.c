@@ -43,6 +43,7 @@ struct f3d_sim {
   int ftype[F3D_MAX_FLUIDS];
   double alpha[F3D_MAX_FLUIDS * F3D_MAX_FLUIDS]; /* ColRate matrix, [i + j*nfl] (src/colrate.c:10-23) */
   f3d_fields fl[F3D_MAX_FLUIDS];                 /* the per-fluid members of each fluid */
+  long pot_step;                                 /* Timestepcount of the last Potential launch */
   char err[512];
 };
 
@@ -247,6 +248,7 @@ f3d_sim *f3d_sim_create(const char *parfile, const char *overrides, char *err, i
   if (S->p.corotating) S->clk.OMEGAFRAME = f3d_psys_info(&S->sys, &S->p, F3D_PSYS_FREQUENCY);
   S->clk.OMEGAFRAME0 = S->clk.OMEGAFRAME;
   S->alreadycalculated = -1;
+  S->pot_step = -1;
   {
     f3d_mesh_desc md;
     memset(&md, 0, sizeof(md));
@@ -428,6 +430,16 @@ int f3d_sim_read_arch(f3d_sim *S, const char *path) {
 
 /* ---- the step ---- */
 
+/* the Potential kernel of this step: the planets at their current positions
+   (src/potential.c:32); it does not need dt, so the step can launch it ahead
+   of the CFL read-back */
+static int launch_potential(f3d_sim *S) {
+  CALL(S, f3d_sim_push_phys(S));
+  CALL(S, S->prec32 ? f3d32_fused_potential(S->ctx) : f3d_Potential(S->ctx));
+  S->pot_step = S->clk.Timestepcount;
+  return 0;
+}
+
 /* compute_potential (src/potential.c:10-49) */
 static int compute_potential(f3d_sim *S, double dt) {
   if (S->alreadycalculated == S->clk.Timestepcount) return 0;
@@ -435,8 +447,7 @@ static int compute_potential(f3d_sim *S, double dt) {
   /* The disk's force on the star (ComputeAccel at the origin) feeds only the
      gas indirect term, which no supported setup enables; it is evaluated when
      a planet feels the disk (AdvanceSystemFromDisk), where it is consumed. */
-  CALL(S, f3d_sim_push_phys(S));
-  CALL(S, S->prec32 ? f3d32_fused_potential(S->ctx) : f3d_Potential(S->ctx));
+  if (S->pot_step != S->clk.Timestepcount && launch_potential(S)) return -1;
   for (int k = 0; k < S->sys.nb; k++) {
     if (!S->sys.feeldisk[k]) continue;
     if (S->prec32) ERR(S, "the FP32 form has no disk force on planets (FeelDisk)");
@@ -636,6 +647,13 @@ int f3d_sim_step(f3d_sim *S, double *dt_out) {
   if (S->dtemp >= DT) S->dtemp = 0.0;
   if (S->prec32 && !(S->arch.fused_cfl && S->arch.fused_sources && S->arch.fused_transport))
     ERR(S, "the FP32 form runs the fused stages only");
+  int feel = 0;
+  for (int k = 0; k < S->sys.nb; k++) feel |= S->sys.feeldisk[k];
+  /* the Potential kernel needs no dt: queue it ahead of the CFL, so that after
+     the dt read-back only the host planet update stands before the sources */
+  if (S->arch.fused_sources && S->p.potential && !feel && S->alreadycalculated != S->clk.Timestepcount &&
+      launch_potential(S))
+    return -1;
   /* MULTIFLUID(ComputeVmed(Vx)); MULTIFLUID(cfl()) (src/main.c:366-381): every
      fluid's CFL reads the VxMed of the last fluid */
   double step = 1e30;
@@ -683,8 +701,6 @@ int f3d_sim_step(f3d_sim *S, double *dt_out) {
   if (S->dtemp > DT) dt = DT - (S->dtemp - dt);
   /* Total_Density (src/main.c:399-403) feeds only the disk force; the fused
      path skips it when no planet feels the disk */
-  int feel = 0;
-  for (int k = 0; k < S->sys.nb; k++) feel |= S->sys.feeldisk[k];
   if (!S->prec32 && (!S->arch.fused_sources || (S->p.potential && feel))) {
     CALL(S, f3d_Reset_field(c, S->f.Total_Density));
     for (int k = 0; k < nfl; k++) {
