@@ -10,7 +10,11 @@ step, sources, transport, damping and ghost fill, including the one device->
 host read of dt the host needs for the planet integrator.  Fields are device
 resident before the timed region starts.
 
-Prints one JSON line (see DESIGN.md for the figures it reports).
+Prints one JSON line: the FP64 step's cell-updates/s, the HBM roofline of the
+stage with the largest time (algorithmic bytes per launch over its HIP-event
+duration, with the traffic rocprofv3 measured for it), the same for the whole
+step, the per-stage averages, the precision, batched and multifluid forms and
+the CPU baseline (the oracle on one host core).
 """
 import argparse
 import ctypes
