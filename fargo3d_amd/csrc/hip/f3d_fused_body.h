@@ -758,7 +758,7 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
                                             const real* __restrict__ e_in, real* __restrict__ e_out,
                                             real* __restrict__ rho_out, real* __restrict__ vx_out,
                                             real* const* qout, double* __restrict__ vxmed_out,
-                                            int* __restrict__ nshift_out, real omegaframe, int fargo) {
+                                            int* __restrict__ nshift_out, real omegaframe, int fargo, int vpre) {
   using XQ = XQuant<DZ, AD, GRP>;
   constexpr int NL = XQ::NL;
   constexpr int N = NL + 1;  // + the density (local index NL)
@@ -791,14 +791,20 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
         st[s][c] = (q < NM) ? qin[q][base + i] : e_in[base + i];
       }
       vt[s] = vxt[base + i];
-      Wt[i] = vt[s] * (m.xmin64[i + 1] - m.xmin64[i]);
+      if (!vpre) Wt[i] = vt[s] * (m.xmin64[i + 1] - m.xmin64[i]);
     }
   }
-  __syncthreads();
-  // ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30), FP64
-  if (t == 0) sh[0] = seq_sum(Wt, nx) / (m.XMAX - m.XMIN);
-  __syncthreads();
-  const double vmed = sh[0];
+  double vmed;
+  if (vpre) {
+    // VxMed of this ring from kf_vmed, run before this launch
+    vmed = vxmed_out[j + k * m.sy];
+  } else {
+    __syncthreads();
+    // ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30), FP64
+    if (t == 0) sh[0] = seq_sum(Wt, nx) / (m.XMAX - m.XMIN);
+    __syncthreads();
+    vmed = sh[0];
+  }
   // ComputeResidual (src/compute_residual.c:68-73), FP64: cell residual into V
   int ns = 0;
 #pragma unroll
@@ -822,7 +828,7 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
     ns = (int)sh[1];
     if (GRP != 2 && t == 0) {
       nshift_out[j + k * m.sy] = ns;
-      vxmed_out[j + k * m.sy] = vmed;
+      if (!vpre) vxmed_out[j + k * m.sy] = vmed;
     }
   }
   // X_advection (src/transport.c:208-226), twice with FARGO
@@ -1035,7 +1041,12 @@ inline size_t xsweep_ring_lds(int nx, int ncarried) {
 }
 
 // ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30) for the tiled X sweep: one
-// workgroup per active ring, the ring's weights in LDS, one sequential sum.
+// workgroup (one wave) per active ring and fluid, the ring's weights in LDS,
+// one sequential sum; blocks [f*nrings, (f+1)*nrings) take fluid f.
+struct VmTab {
+  const real* vxt[8];
+  double* vxmed[8];
+};
 template <int G, int DZ>
 __device__ __forceinline__ void kd_vmed(const dim3 bid, const dmesh& m, const real* vxt, double* vxmed) {
   extern __shared__ double w[];
@@ -1048,8 +1059,9 @@ __device__ __forceinline__ void kd_vmed(const dim3 bid, const dmesh& m, const re
   if (threadIdx.x == 0) vxmed[j + k * m.sy] = seq_sum(w, m.nx) / (m.XMAX - m.XMIN);
 }
 template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_vmed(dmesh m, const real* __restrict__ vxt, double* __restrict__ vxmed) {
-  kd_vmed<G, DZ>(blockIdx, m, vxt, vxmed);
+__global__ void __launch_bounds__(64) kf_vmed(dmesh m, VmTab T, int nrings) {
+  const int fl = (int)blockIdx.x / nrings;
+  kd_vmed<G, DZ>(dim3(blockIdx.x % nrings), m, T.vxt[fl], T.vxmed[fl]);
 }
 
 // X sweep of rings too long for LDS: workgroups own tiles of T cells and
@@ -1711,9 +1723,9 @@ static int g_xsplit = -1;
 // -1: chosen from the mesh shape.
 static int g_cflmode = -1;
 
-static int cfl_mode(const dmesh& m) {
+static int cfl_mode(const dmesh& m, int nf) {
   if (g_cflmode >= 0) return g_cflmode;
-  const int nrings = m.ny * (m.dimz ? m.nz : 1);
+  const int nrings = m.ny * (m.dimz ? m.nz : 1) * nf;
   return (nrings >= 2048 ? 1 : 0) | (m.nx >= 1024 ? 2 : 0);
 }
 
@@ -1723,7 +1735,7 @@ static int vmed_prepass(f3d_hip_ctx* ctx, Scratch* S, const real* v) {
   const dmesh& m = S->m;
   const int nry = m.ny + (ctx->ph.fargo ? 1 : 0);
   const int nr = nry * (m.dimz ? m.nz : 1);
-  FDISPATCH(ctx, kf_vmed_rings, dim3(nr), dim3(WG), m.nx * sizeof(double), v, ctx->f.VxMed, nry);
+  FDISPATCH(ctx, kf_vmed_rings, dim3(nr), dim3(64), m.nx * sizeof(double), v, ctx->f.VxMed, nry);
   return 0;
 }
 
@@ -1842,12 +1854,19 @@ static int yz_sweeps(f3d_hip_ctx* ctx, Scratch* S, double dt, const RF& f, const
 
 // VxMed of Vx_temp for the tiled X sweep (one sequential sum per ring), on
 // stream st
-static int x_vmed(f3d_hip_ctx* ctx, Scratch* S, const RF& f, const TSet& t, hipStream_t st) {
+static int x_vmed(f3d_hip_ctx* ctx, Scratch* S, int nf, const RF* f, const TSet* t, hipStream_t st) {
   const dmesh& m = S->m;
   const int nrings = m.ny * (m.dimz ? m.nz : 1);
-  F3D_REQUIRE(ctx, (size_t)m.nx * sizeof(double) <= 160 * 1024, "ring too long for the VxMed pass");
+  F3D_REQUIRE(ctx, (size_t)m.nx * sizeof(double) <= 64 * 1024, "ring too long for the VxMed pass");
+  F3D_REQUIRE(ctx, nf >= 1 && nf <= 8, "VxMed pass: 1 to 8 fluids");
+  VmTab T;
+  memset(&T, 0, sizeof(T));
+  for (int q = 0; q < nf; q++) {
+    T.vxt[q] = f[q].Vx_temp;
+    T.vxmed[q] = t[q].vxmed;
+  }
 #define XV(G_, DZ_) \
-  hipLaunchKernelGGL((kf_vmed<G_, DZ_>), dim3(nrings), dim3(WG), m.nx * sizeof(double), st, m, f.Vx_temp, t.vxmed)
+  hipLaunchKernelGGL((kf_vmed<G_, DZ_>), dim3(nrings * nf), dim3(64), m.nx * sizeof(double), st, m, T, nrings)
   if (m.geom == SPH) {
     if (m.dimz) XV(SPH, 1); else XV(SPH, 0);
   } else {
@@ -1860,6 +1879,22 @@ static int x_vmed(f3d_hip_ctx* ctx, Scratch* S, const RF& f, const TSet& t, hipS
 
 // tile width of the tiled X sweep (cells per workgroup): 64, 128 or 256
 static int g_xtile = 256;
+
+// Where the X sweep's VxMed comes from: 0 = the ring kernel's own sum,
+// 1 = kf_vmed on the side stream, overlapping the Z/Y sweeps, 2 = kf_vmed on
+// the stream just before the X sweep; -1 = chosen here.  The tiled sweep
+// always takes it from kf_vmed.  Measured on MI355X: below about four ring
+// workgroups per CU the ring kernel's own sum costs least; above, the pass
+// pays, on the stream for short rings (a sum of <= 512 terms takes under
+// 3 us) and on the side stream for long ones, whose sums it hides.
+static int g_xvmed = -1;
+static int x_vmed_form(const dmesh& m, int nf, bool tiles) {
+  const int nrings = m.ny * (m.dimz ? m.nz : 1);
+  if (tiles) return (g_xvmed == 2) ? 2 : 1;
+  if (g_xvmed >= 0) return g_xvmed;
+  if (nrings * nf < 1024) return 0;
+  return m.nx <= 512 ? 2 : 1;
+}
 
 // X sweep by tiles (rings longer than the LDS ring kernel holds), once
 // VxMed of every ring is known.
@@ -1921,18 +1956,18 @@ struct XTab {
 
 template <int G, int DZ, int AD, int CPT, int SPLIT>
 __global__ void __launch_bounds__(CPT == 2 ? 512 : 256) kf_sweep_xr_fluids(dmesh m, double dt64, XTab T, int per_fluid,
-                                                           real omegaframe, int fargo) {
+                                                           real omegaframe, int fargo, int vpre) {
   const int fl = (int)blockIdx.x / per_fluid, b = (int)blockIdx.x % per_fluid;
   real* qout[4] = {T.qo[fl][0], T.qo[fl][1], T.qo[fl][2], T.qo[fl][3]};
   if (!SPLIT) {
     xsweep_ring<G, DZ, AD, CPT, 0>(m, b, dt64, T.rho_in[fl], T.vxt[fl], T.q[fl], T.e_in[fl], T.e_out[fl],
-                                   T.rho_out[fl], T.vx_out[fl], qout, T.vxmed[fl], T.nshift[fl], omegaframe, fargo);
+                                   T.rho_out[fl], T.vx_out[fl], qout, T.vxmed[fl], T.nshift[fl], omegaframe, fargo, vpre);
   } else if ((b & 1) == 0) {
     xsweep_ring<G, DZ, AD, CPT, 1>(m, b >> 1, dt64, T.rho_in[fl], T.vxt[fl], T.q[fl], T.e_in[fl], T.e_out[fl],
-                                   T.rho_out[fl], T.vx_out[fl], qout, T.vxmed[fl], T.nshift[fl], omegaframe, fargo);
+                                   T.rho_out[fl], T.vx_out[fl], qout, T.vxmed[fl], T.nshift[fl], omegaframe, fargo, vpre);
   } else {
     xsweep_ring<G, DZ, AD, CPT, 2>(m, b >> 1, dt64, T.rho_in[fl], T.vxt[fl], T.q[fl], T.e_in[fl], T.e_out[fl],
-                                   T.rho_out[fl], T.vx_out[fl], qout, T.vxmed[fl], T.nshift[fl], omegaframe, fargo);
+                                   T.rho_out[fl], T.vx_out[fl], qout, T.vxmed[fl], T.nshift[fl], omegaframe, fargo, vpre);
   }
 }
 
@@ -1957,7 +1992,7 @@ static bool x_ring_fits(const f3d_hip_ctx* ctx, Scratch* S) {
 }
 
 // Ring X sweep of nf fluids in one launch.
-static int x_sweep_ring(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const RF* f, const TSet* t) {
+static int x_sweep_ring(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const RF* f, const TSet* t, int vpre) {
   const dmesh& m = S->m;
   const real of = (real)ctx->ph.OMEGAFRAME;
   const int ad = (ctx->ph.eos == 2);
@@ -1983,7 +2018,7 @@ static int x_sweep_ring(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const R
   f3d_stage_begin(ctx, F3D_ST_SWEEP_X);
 #define XSWEEP1(G_, DZ_, AD_, CPT_, SP_)                                                                      \
   hipLaunchKernelGGL((kf_sweep_xr_fluids<G_, DZ_, AD_, CPT_, SP_>), gx, dim3(nthreads), shx, ctx->stream, m, dt, \
-                     T, per, of, ctx->ph.fargo)
+                     T, per, of, ctx->ph.fargo, vpre)
 #define XSWEEP2(G_, DZ_, AD_, CPT_) \
   do { if (split) XSWEEP1(G_, DZ_, AD_, CPT_, 1); else XSWEEP1(G_, DZ_, AD_, CPT_, 0); } while (0)
 #define XSWEEP(G_, DZ_, CPT_) \
@@ -2020,22 +2055,23 @@ static int transport_fluids(f3d_hip_ctx* ctx, double dt, int nf, const f3d_field
     if (tset_of(ctx, S, f[k], k, nf, &t[k])) return -1;
   }
   const bool tiles = !x_ring_fits(ctx, S);
-  if (tiles && ctx->ph.fargo) {
+  const int vpre = ctx->ph.fargo ? x_vmed_form(S->m, nf, tiles) : 0;
+  if (vpre == 1) {
     // VxMed of each fluid's Vx_temp (one long sequential sum per ring) on the
     // side stream, overlapping the Z/Y sweeps, which do not write Vx_temp
     if (side_stream(ctx)) return -1;
     F3D_CHECK(ctx, hipEventRecord(ctx->side_go, ctx->stream));
     F3D_CHECK(ctx, hipStreamWaitEvent(ctx->side, ctx->side_go, 0));
-    for (int k = 0; k < nf; k++)
-      if (x_vmed(ctx, S, f[k], t[k], ctx->side)) return -1;
+    if (x_vmed(ctx, S, nf, f, t, ctx->side)) return -1;
     F3D_CHECK(ctx, hipEventRecord(ctx->side_done, ctx->side));
   }
   for (int k = 0; k < nf; k++)
     if (yz_sweeps(ctx, S, dt, f[k], t[k])) return -1;
+  if (vpre == 1) F3D_CHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->side_done, 0));
+  if (vpre == 2 && x_vmed(ctx, S, nf, f, t, ctx->stream)) return -1;
   if (!tiles) {
-    if (x_sweep_ring(ctx, S, dt, nf, f, t)) return -1;
+    if (x_sweep_ring(ctx, S, dt, nf, f, t, vpre != 0)) return -1;
   } else {
-    if (ctx->ph.fargo) F3D_CHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->side_done, 0));
     for (int k = 0; k < nf; k++)
       if (x_sweep_tiled(ctx, S, dt, f[k], t[k])) return -1;
   }
@@ -2061,6 +2097,12 @@ int F3D_API(fused_set_xsplit)(int split) {
 int F3D_API(fused_set_colsplit)(int split) {
   if (split < -1 || split > 1) return -2;
   g_colsplit = split;
+  return 0;
+}
+
+int F3D_API(fused_set_xvmed)(int form) {
+  if (form < -1 || form > 2) return -2;
+  g_xvmed = form;
   return 0;
 }
 
@@ -2097,7 +2139,7 @@ int F3D_API(fused_cfl_vm)(f3d_hip_ctx* ctx, const double* vxm_in, double* fluid_
   const size_t shmem = (2 * m.nx + 16) * sizeof(double);
   const int slot = S->slot;
   f3d_stage_begin(ctx, F3D_ST_CFL);
-  const int mode = cfl_mode(m);
+  const int mode = cfl_mode(m, 1);
   double* rm = (mode & 1) ? S->ringmin : nullptr;
   const int pre = (mode & 2) ? 1 : 0;
   if (pre && vmed_prepass(ctx, S, vxm ? vxm : f.Vx)) return -1;
@@ -2131,7 +2173,7 @@ int F3D_API(fused_cfl_fluids)(f3d_hip_ctx* ctx, int nf, double* const* cs, doubl
   const real* vxm = reinterpret_cast<const real*>(vxm_in);
   auto R = [](double* p) { return reinterpret_cast<real*>(p); };
   f3d_stage_begin(ctx, F3D_ST_CFL);
-  const int mode = cfl_mode(m);
+  const int mode = cfl_mode(m, nf);
   const int pre = (mode & 2) ? 1 : 0;
   if (pre && vmed_prepass(ctx, S, vxm)) return -1;
   CflTab T;

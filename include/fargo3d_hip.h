@@ -224,6 +224,13 @@ int f3d_fused_set_xsplit(int split);
 /* Cells per workgroup of the tiled X sweep: 64, 128 or 256 (default);
  * results do not depend on it.  Returns -2 for other values. */
 int f3d_fused_set_xtile(int cells);
+/* Where the ring X sweep takes each ring's VxMed (ComputeVmed of Vx_temp,
+ * src/compute_vmed.c:14-30) from: 0 its own sequential sum, 1 a one-wave-
+ * per-ring pass on a second stream that overlaps the Z/Y sweeps, 2 that pass
+ * on the step's stream just before the X sweep, -1 automatic (1 from 1024
+ * rings x fluids on); the tiled sweep always uses the pass.  Results do not
+ * depend on it.  Returns -2 for other values. */
+int f3d_fused_set_xvmed(int form);
 /* Column (Y and Z) sweeps: 1 two threads per column, each with the density
  * and half of the quantities, 0 one thread per column, -1 automatic (split
  * only for sweeps with fewer waves than SIMDs); results do not depend on it.
@@ -284,6 +291,7 @@ int f3d32_fused_set_tiling(int rows_y, int rows_z);
 int f3d32_fused_set_xmode(int mode);
 int f3d32_fused_set_xsplit(int split);
 int f3d32_fused_set_xtile(int cells);
+int f3d32_fused_set_xvmed(int form);
 int f3d32_fused_set_colsplit(int split);
 int f3d32_fused_set_cflmode(int mode);
 int f3d32_fused_cfl_vm(f3d_hip_ctx *ctx, const double *vxm, double *fluid_min);

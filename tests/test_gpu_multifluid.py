@@ -104,6 +104,24 @@ def test_multifluid_fused(fa, orc, ov, nsteps):
             assert np.array_equal(s.nshift()[:, 3:-3], o.nshift()[:, 3:-3])
 
 
+@pytest.mark.parametrize("form", [0, 1, 2], ids=["sum_in_ring_kernel", "pass_on_side_stream", "pass_before_sweep"])
+def test_multifluid_xvmed_forms(fa, orc, form):
+    """Every fluid's X-sweep VxMed from the ring kernel's own sum or from one
+    pass over all fluids (second stream or step stream) gives the oracle's run."""
+    L = fa.lib()
+    assert L.f3d_fused_set_xvmed(form) == 0
+    try:
+        say("multifluid xvmed form", form)
+        o, s = make_pair(fa, orc, C4_SMALL)
+        for n in range(1, 5):
+            dto, dts = o.step(), s.step()
+            assert dto == dts, "step %d: dt %r vs %r" % (n, dts, dto)
+        compare_fluids(o, s, "xvmed form %d step 4" % form)
+        assert np.array_equal(s.nshift()[:, 3:-3], o.nshift()[:, 3:-3])
+    finally:
+        L.f3d_fused_set_xvmed(-1)
+
+
 def test_multifluid_routines(fa, orc):
     """The per-routine kernels with the same drag and diffusion kernels agree as well."""
     say("multifluid routines", C4_SMALL)

@@ -364,6 +364,21 @@ def test_whole_step_fused_ring_split(fa, orc, par, ov, nsteps, split):
         L.f3d_fused_set_xsplit(-1)
 
 
+@pytest.mark.parametrize("form", [0, 1, 2], ids=["sum_in_ring_kernel", "pass_on_side_stream", "pass_before_sweep"])
+@pytest.mark.parametrize("par,ov,nsteps", [(C1, "", 8), (C2, C2_SMALL, 4), (C3, C3_SMALL, 6)],
+                         ids=["c1_2d", "c2_3d_small", "c3_2d_adiabatic_small"])
+def test_whole_step_fused_xvmed_forms(fa, orc, par, ov, nsteps, form):
+    """The X sweep's VxMed summed inside the ring workgroup, or by a pass of
+    its own on a second stream or on the step's stream, gives the oracle's
+    results."""
+    L = fa.lib()
+    assert L.f3d_fused_set_xvmed(form) == 0
+    try:
+        _fused_run(fa, orc, par, ov, nsteps)
+    finally:
+        L.f3d_fused_set_xvmed(-1)
+
+
 @pytest.mark.parametrize("split", [0, 1], ids=["one_thread_per_column", "two_threads_per_column"])
 @pytest.mark.parametrize("par,ov,nsteps", [(C1, "", 8), (C2, C2_SMALL, 4), (C2, C2_AD, 4), (C3, C3_SMALL, 6)],
                          ids=["c1_2d", "c2_3d_small", "c2_3d_adiabatic_small", "c3_2d_adiabatic_small"])
