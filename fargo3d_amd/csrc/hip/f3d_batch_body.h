@@ -49,8 +49,8 @@ __device__ __forceinline__ dim3 local2(int lb, int gx) { return dim3(lb % gx, lb
 __global__ void __launch_bounds__(WG) kb_cfl(const BInst* I, const int* off, int n) {
   const int ii = find_inst(off, n, blockIdx.x);
   const BInst& in = I[ii];
-  kd_cfl<CYL, 0>(dim3(blockIdx.x - off[ii]), in.m, in.cs, in.rho, in.vx, in.vy, in.vy, in.vx, in.vxmed, in.res, 0,
-                 in.cp, nullptr, 0);
+  kd_cfl<CYL, 0, 0>((int)blockIdx.x - off[ii], in.m, in.cs, in.rho, in.vx, in.vy, in.vy, in.vx, in.vxmed,
+                    in.res, 0, in.cp, nullptr);
 }
 
 // CflFluidsMin and the output-time clamp (src/cfl_fluids_min.c:3-22,

@@ -222,19 +222,21 @@ __device__ __forceinline__ double seq_sum(const double* w, int n) {
 // exactly; the MIN over rings is order independent, so one atomic MIN per
 // ring on the bit pattern of the positive double gives the value of
 // reduction_full_MIN + cfl_b (src/cfl_b.c:6-37) exactly.
-template <int G, int DZ>
-__device__ __forceinline__ void kd_cfl(const dim3 bid, const dmesh& m, const real* cs, const real* rho, const real* vx, const real* vy, const real* vz, const real* vxm, double* vxmed, unsigned long long* result, int slot, cell::CflPar cp, double* ringmin,
-    int pre) {
+// PRE: VxMed of the ring and the next from kf_vmed_rings; else summed here
+// (the in-kernel sum's registers would halve the occupancy of the PRE form,
+// hence a template parameter).
+template <int G, int DZ, int PRE>
+__device__ __forceinline__ void kd_cfl(int ring, const dmesh& m, const real* cs, const real* rho, const real* vx, const real* vy, const real* vz, const real* vxm, double* vxmed, unsigned long long* result, int slot, cell::CflPar cp, double* ringmin) {
   extern __shared__ double lds[];  // 2 nx weights + 16 (16 with pre)
   double* w0 = lds;
   double* w1 = lds + m.nx;
-  double* sh = pre ? lds : lds + 2 * m.nx;
-  const int j = m.nghy + (int)(bid.x % m.ny);
-  const int k = DZ ? m.nghz + (int)(bid.x / m.ny) : 0;
+  double* sh = PRE ? lds : lds + 2 * m.nx;
+  const int j = m.nghy + ring % m.ny;
+  const int k = DZ ? m.nghz + ring / m.ny : 0;
   const int base = j * m.pitch + k * m.stride;
   const int l2 = j + k * m.sy;
-  if (bid.x == 0 && threadIdx.x == 0) result[slot ^ 1] = kInfBits;  // the next step's slot
-  if (pre) {
+  if (ring == 0 && threadIdx.x == 0) result[slot ^ 1] = kInfBits;  // the next step's slot
+  if constexpr (PRE) {
     // VxMed of rings j and j+1 from the pass before (kf_vmed_rings)
     if (threadIdx.x == 0) {
       sh[8] = vxmed[l2];
@@ -256,28 +258,29 @@ __device__ __forceinline__ void kd_cfl(const dim3 bid, const dmesh& m, const rea
   __syncthreads();
   const double vm0 = sh[8];
   double mx = 0.0;
+#pragma unroll 4
   for (int i = threadIdx.x; i < m.nx; i += blockDim.x) {
     const double s = cell::cfl_sumsq<G, DZ>(m, i, j, k, base + i, cs, rho, vx, vy, vz, vm0, cp);
     mx = (s > mx) ? s : mx;
   }
   mx = block_max(mx, sh);
   if (threadIdx.x == 0) {
-    vxmed[l2] = vm0;
+    if (!PRE) vxmed[l2] = vm0;
     double mn = cp.cfl / sqrt(mx);
     if (cp.fargo) {
       const double shearlimit = cp.cfl * m.mindx / fabs(vm0 / m.ymed64[j] - sh[9] / m.ymed64[j + 1]);
       if (shearlimit < mn) mn = shearlimit;
     }
-    if (ringmin) ringmin[bid.x] = mn;
+    if (ringmin) ringmin[ring] = mn;
     else atomicMin(&result[slot], (unsigned long long)__double_as_longlong(mn));
   }
 }
-template <int G, int DZ>
+template <int G, int DZ, int PRE>
 __global__ void __launch_bounds__(WG) kf_cfl(dmesh m, const real* __restrict__ cs, const real* __restrict__ rho, const real* __restrict__ vx,
                                              const real* __restrict__ vy, const real* __restrict__ vz, const real* __restrict__ vxm, double* __restrict__ vxmed,
                                              unsigned long long* __restrict__ result, int slot, cell::CflPar cp,
-                                             double* __restrict__ ringmin, int pre) {
-  kd_cfl<G, DZ>(blockIdx, m, cs, rho, vx, vy, vz, vxm, vxmed, result, slot, cp, ringmin, pre);
+                                             double* __restrict__ ringmin) {
+  kd_cfl<G, DZ, PRE>((int)blockIdx.x, m, cs, rho, vx, vy, vz, vxm, vxmed, result, slot, cp, ringmin);
 }
 
 // ComputeVmed (src/compute_vmed.c:14-30) of rings j = NGHY .. NGHY+nry-1 of
@@ -307,12 +310,12 @@ struct CflTab {
   const real* vz[8];
   double* ringmin[8];
 };
-template <int G, int DZ>
+template <int G, int DZ, int PRE>
 __global__ void __launch_bounds__(WG) kf_cfl_fluids(dmesh m, CflTab T, int nrings, const real* vxm, double* vxmed,
-                                                    unsigned long long* result, int slot, cell::CflPar cp, int pre) {
+                                                    unsigned long long* result, int slot, cell::CflPar cp) {
   const int fl = (int)blockIdx.x / nrings, r = (int)blockIdx.x % nrings;
-  kd_cfl<G, DZ>(dim3(r), m, T.cs[fl], T.rho[fl], T.vx[fl], T.vy[fl], T.vz[fl], vxm, vxmed, result, slot, cp,
-                T.ringmin[fl], pre);
+  kd_cfl<G, DZ, PRE>(r, m, T.cs[fl], T.rho[fl], T.vx[fl], T.vy[fl], T.vz[fl], vxm, vxmed, result, slot, cp,
+                     T.ringmin[fl]);
 }
 
 // Second pass of the two-pass MIN: one workgroup takes the MIN of the n ring
@@ -855,18 +858,30 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
     }
     __syncthreads();
     if (!ppa) {
-      // VanLeerX_a (src/vanleer_x_a.c:60-68)
+      // VanLeerX_a (src/vanleer_x_a.c:60-68).  The gradient at face i,
+      // (q_i - q_{i-1}) / zone_i, is cell i's left and cell i-1's right
+      // gradient: computed once per face into FC (free until VanLeerX_b).
 #pragma unroll
       for (int s = 0; s < CPT; s++) {
         const int i = t + s * nt;
         if (i < nx) {
-          const int im = ixm_of(i, nx), ip = ixp_of(i, nx);
-          const real zi = zone_size_x<G>(m, i, j, k), zp = zone_size_x<G>(m, ip, j, k);
+          const int im = ixm_of(i, nx);
+          const real zi = zone_size_x<G>(m, i, j, k);
 #pragma unroll
           for (int c = 0; c < N; c++) {
             const real* a = AQ + c * nx;
-            SL[c * nx + i] = vanleer_slope((a[i] - a[im]) / zi, (a[ip] - a[i]) / zp);
+            FC[c * nx + i] = (a[i] - a[im]) / zi;
           }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < CPT; s++) {
+        const int i = t + s * nt;
+        if (i < nx) {
+          const int ip = ixp_of(i, nx);
+#pragma unroll
+          for (int c = 0; c < N; c++) SL[c * nx + i] = vanleer_slope(FC[c * nx + i], FC[c * nx + ip]);
         }
       }
       __syncthreads();
@@ -1142,12 +1157,19 @@ _Pragma("unroll")
     __syncthreads();  // also orders the first pass after the loads of V, VU, ZS
     int ulo, uhi;  // positions updated by this pass
     if (!pass) {
-      TILE_FOR(1, W - 1, {
+      // the gradient at face p, (q_p - q_{p-1}) / zone_p, once per face into
+      // FC: the left gradient of cell p and the right one of cell p-1
+      TILE_FOR(1, W, {
 _Pragma("unroll")
         for (int c = 0; c < N; c++) {
           const real* a = AQ + c * W;
-          SL[c * W + p] = vanleer_slope((a[p] - a[p - 1]) / ZS[p], (a[p + 1] - a[p]) / ZS[p + 1]);
+          FC[c * W + p] = (a[p] - a[p - 1]) / ZS[p];
         }
+      })
+      __syncthreads();
+      TILE_FOR(1, W - 1, {
+_Pragma("unroll")
+        for (int c = 0; c < N; c++) SL[c * W + p] = vanleer_slope(FC[c * W + p], FC[c * W + p + 1]);
       })
       __syncthreads();
       TILE_FOR(2, W - 1, {
@@ -1552,6 +1574,23 @@ __global__ void kf_bound_y_dens(dmesh m, real* dens, cell::BcPar bp0, cell::BcPa
   } while (0)
 
 // the same with a third template argument AD (1: adiabatic, the energy is transported)
+// KERN<G, DZ, T_>
+#define FDISPATCH_T(ctx, KERN, T_, grid, block, shmem, ...)                                            \
+  do {                                                                                                  \
+    if (m.geom == SPH) {                                                                                \
+      if (m.dimz)                                                                                       \
+        hipLaunchKernelGGL((KERN<SPH, 1, T_>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);      \
+      else                                                                                              \
+        hipLaunchKernelGGL((KERN<SPH, 0, T_>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);      \
+    } else {                                                                                            \
+      if (m.dimz)                                                                                       \
+        hipLaunchKernelGGL((KERN<CYL, 1, T_>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);      \
+      else                                                                                              \
+        hipLaunchKernelGGL((KERN<CYL, 0, T_>), grid, block, shmem, (ctx)->stream, m, __VA_ARGS__);      \
+    }                                                                                                   \
+    F3D_LAUNCH_CHECK(ctx);                                                                              \
+  } while (0)
+
 #define FDISPATCH_AD(ctx, KERN, grid, block, shmem, ...)                                                 \
   do {                                                                                                    \
     const bool ad_ = ((ctx)->ph.eos == 2);                                                                \
@@ -1723,10 +1762,16 @@ static int g_xsplit = -1;
 // -1: chosen from the mesh shape.
 static int g_cflmode = -1;
 
+// CFL form, bits: 1 ring minima to an array and a second pass (else one
+// atomic MIN per ring), 2 VxMed from kf_vmed_rings, 4 (with 2) one wave per
+// workgroup.  Measured on MI355X: the array pays from about 2048 rings, the
+// pass for many or long rings, one wave per ring for rings of <= 256 cells.
 static int cfl_mode(const dmesh& m, int nf) {
   if (g_cflmode >= 0) return g_cflmode;
   const int nrings = m.ny * (m.dimz ? m.nz : 1) * nf;
-  return (nrings >= 2048 ? 1 : 0) | (m.nx >= 1024 ? 2 : 0);
+  int mode = (nrings >= 2048 ? 3 : 0) | (m.nx >= 1024 ? 2 : 0);
+  if ((mode & 2) && m.nx <= 256) mode |= 4;
+  return mode;
 }
 
 // VxMed of every active ring and of the ring above the last (the shear limit
@@ -2083,7 +2128,7 @@ static int transport_fluids(f3d_hip_ctx* ctx, double dt, int nf, const f3d_field
 extern "C" {
 
 int F3D_API(fused_set_cflmode)(int mode) {
-  if (mode < -1 || mode > 3) return -2;
+  if (mode < -1 || mode > 7 || (mode >= 4 && !(mode & 2))) return -2;
   g_cflmode = mode;
   return 0;
 }
@@ -2143,8 +2188,13 @@ int F3D_API(fused_cfl_vm)(f3d_hip_ctx* ctx, const double* vxm_in, double* fluid_
   double* rm = (mode & 1) ? S->ringmin : nullptr;
   const int pre = (mode & 2) ? 1 : 0;
   if (pre && vmed_prepass(ctx, S, vxm ? vxm : f.Vx)) return -1;
-  FDISPATCH(ctx, kf_cfl, dim3(nrings), dim3(WG), pre ? 16 * sizeof(double) : shmem, f.Energy, f.Density, f.Vx, f.Vy,
-            f.Vz, vxm ? vxm : f.Vx, f.VxMed, S->result, slot, cfl_par_f(ctx), rm, pre);
+  const int wg = (mode & 4) ? 64 : WG;
+#define CFL1(P_)                                                                                                 \
+  FDISPATCH_T(ctx, kf_cfl, P_, dim3(nrings), dim3(wg), P_ ? 16 * sizeof(double) : shmem, f.Energy, f.Density, \
+              f.Vx, f.Vy, f.Vz, vxm ? vxm : f.Vx, f.VxMed, S->result, slot, cfl_par_f(ctx), rm)
+  if (pre) CFL1(1);
+  else CFL1(0);
+#undef CFL1
   if (rm) {
     hipLaunchKernelGGL(kf_cfl_fin, dim3(1), dim3(1024), 0, ctx->stream, rm, nrings, S->result, slot, ctx->dhres);
     F3D_LAUNCH_CHECK(ctx);
@@ -2186,8 +2236,13 @@ int F3D_API(fused_cfl_fluids)(f3d_hip_ctx* ctx, int nf, double* const* cs, doubl
     T.vz[k] = m.dimz ? R(vz[k]) : R(vy[k]);
     T.ringmin[k] = (mode & 1) ? S->ringmin + (size_t)k * nrings : nullptr;
   }
-  FDISPATCH(ctx, kf_cfl_fluids, dim3(nrings * nf), dim3(WG), pre ? 16 * sizeof(double) : shmem, T, nrings, vxm,
-            f.VxMed, S->result, slot, cfl_par_f(ctx), pre);
+  const int wg = (mode & 4) ? 64 : WG;
+#define CFLF(P_)                                                                                             \
+  FDISPATCH_T(ctx, kf_cfl_fluids, P_, dim3(nrings * nf), dim3(wg), P_ ? 16 * sizeof(double) : shmem, T, nrings, \
+              vxm, f.VxMed, S->result, slot, cfl_par_f(ctx))
+  if (pre) CFLF(1);
+  else CFLF(0);
+#undef CFLF
   if (mode & 1) {
     hipLaunchKernelGGL(kf_cfl_fin, dim3(1), dim3(1024), 0, ctx->stream, S->ringmin, nf * nrings, S->result, slot,
                        ctx->dhres);

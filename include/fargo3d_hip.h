@@ -236,8 +236,10 @@ int f3d_fused_set_xvmed(int form);
  * only for sweeps with fewer waves than SIMDs); results do not depend on it.
  * Returns -2 for other values. */
 int f3d_fused_set_colsplit(int split);
-/* CFL MIN over rings: 0 one 64-bit atomic MIN per ring, 1 ring minima to an
- * array and a one-workgroup second pass; the result is the same value.
+/* Form of the fused CFL, bits: 1 ring minima to an array and a
+ * one-workgroup second pass (else one 64-bit atomic MIN per ring); 2 the
+ * rings' VxMed from a pass of its own; 4 (with bit 2) one wave per
+ * workgroup.  -1 automatic.  The result is the same value in every form.
  * Returns -2 for other values. */
 int f3d_fused_set_cflmode(int mode);
 /* Potential, the ghost fill and the damping references on the fused stages'

@@ -331,15 +331,17 @@ def test_whole_step_fused_tiled_x(fa, orc, par, ov, nsteps):
         L.f3d_fused_set_xmode(0)
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2, 3], ids=["atomic", "two_pass", "vmed_pass_atomic", "vmed_pass_two_pass"])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 6, 7], ids=["atomic", "two_pass", "vmed_pass_atomic",
+                                                     "vmed_pass_two_pass", "vmed_pass_atomic_one_wave",
+                                                     "vmed_pass_two_pass_one_wave"])
 def test_cfl_forms(fa, orc, mode):
     """Every form of the fused CFL kernel (atomic or two-pass MIN, VxMed summed
-    inside the ring's workgroup or in a pass of its own) gives the oracle's
-    time step bit for bit."""
+    inside the ring's workgroup or in a pass of its own, four waves or one per
+    workgroup) gives the oracle's time step bit for bit."""
     L = fa.lib()
     assert L.f3d_fused_set_cflmode(mode) == 0
     try:
-        for par, ov in [(C1, ""), (C2, C2_SMALL), (C3, C3_SMALL)]:
+        for par, ov in [(C1, ""), (C2, C2_SMALL), (C3, C3_SMALL), (C3, "Nx=1000 Ny=24")]:
             o, s = make_pair(fa, orc, par, ov)
             for n in range(3):
                 dto, dts = o.step(), s.step()
