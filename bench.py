@@ -62,9 +62,11 @@ def stage_arrays(stage, dz, ad, alpha):
 
 
 # kernel behind each stage (names in the rocprofv3 summaries)
-STAGE_KERNEL = {"cfl": ["kf_cfl"], "sources_a": ["kf_src_a"], "sources_b": ["kf_src_b"], "sweep_z": ["kf_sweep_z"],
-                "sweep_y": ["kf_sweep_y2", "kf_sweep_y3"], "sweep_x": ["kf_sweep_x", "kf_sweep_x_tile", "kf_vmed"],
-                "finish": ["kf_finish"], "potential": ["k_potential"], "substep3": ["kf_substep3"]}
+# the kernels each stage's events bracket (names without namespace and template arguments)
+STAGE_KERNEL = {"cfl": ["kf_cfl", "kf_cfl_fluids", "kf_vmed_rings", "kf_cfl_fin"], "sources_a": ["kf_src_a"],
+                "sources_b": ["kf_src_b"], "sweep_z": ["kf_sweep_z"], "sweep_y": ["kf_sweep_y2", "kf_sweep_y3"],
+                "sweep_x": ["kf_sweep_xr_fluids", "kf_sweep_x_tile"], "finish": ["kf_finish"],
+                "potential": ["k_potential", "kf_potential"], "substep3": ["kf_substep3"]}
 # rocprofv3 summary with the measured HBM traffic of each kernel of this workload
 PROFILE = {"c2": "profiles/fused_step_c2_3d_256x96x32.md", "c1": "profiles/fused_step_c1_2d_384x128.md",
            "c3": "profiles/fused_step_c3_2d_adiabatic_4096x1024.md",
@@ -80,9 +82,9 @@ def measured_traffic(config, stage):
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch = "):
             table = json.loads(line.split("=", 1)[1])
-            # every kernel the stage launches (e.g. the VxMed pass of the tiled X sweep)
+            # every kernel the stage's events bracket (the CFL: VxMed pass, CFL, MIN pass)
             hits = [v for name, v in table.items()
-                    if name.split("<")[0] in STAGE_KERNEL.get(stage, [])]
+                    if name.split("<")[0].split("::")[-1] in STAGE_KERNEL.get(stage, [])]
             return sum(hits) if hits else None
     return None
 

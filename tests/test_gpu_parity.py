@@ -381,6 +381,29 @@ def test_whole_step_fused_xvmed_forms(fa, orc, par, ov, nsteps, form):
         L.f3d_fused_set_xvmed(-1)
 
 
+@pytest.mark.parametrize("ov", [C2_SMALL, C2_AD], ids=["c2_3d_small", "c2_3d_adiabatic_small"])
+def test_fused_ghost_fill_random_interior(fa, orc, ov):
+    """The fused ghost fill (Z then Y, src/boundary.c:2-23) on a random
+    interior: every cell, ghosts and corners included, equals the oracle's
+    boundaries() bit for bit."""
+    o, s = make_pair(fa, orc, C2, ov)
+    rng = np.random.default_rng(11)
+    names = ["Density", "Energy", "Vx", "Vy", "Vz"]
+    for n in names:
+        a = o.field(n)
+        pos = n in ("Density", "Energy")
+        r = (0.5 + rng.random(a.shape)) * np.abs(a).mean() if pos else rng.standard_normal(a.shape)
+        o.field(n)[...] = r
+        s.set_field(n, r)
+    o.call("boundaries")
+    s.call("fused_boundaries")
+    for n in names:
+        a, b = s.field(n), o.field(n)
+        say("ghost fill %s bits %d" % (n, nbits(a, b)))
+        assert nbits(a, b) == 0, n
+    s.close()
+
+
 @pytest.mark.parametrize("split", [0, 1], ids=["one_thread_per_column", "two_threads_per_column"])
 @pytest.mark.parametrize("par,ov,nsteps", [(C1, "", 8), (C2, C2_SMALL, 4), (C2, C2_AD, 4), (C3, C3_SMALL, 6)],
                          ids=["c1_2d", "c2_3d_small", "c2_3d_adiabatic_small", "c3_2d_adiabatic_small"])
