@@ -197,6 +197,29 @@ __device__ __forceinline__ dim3 xcd_bid() {
   return dim3(p % gridDim.x, (p / gridDim.x) % gridDim.y, p / (gridDim.x * gridDim.y));
 }
 
+// The weights v_i (x_{i+1} - x_i) of one ring (ComputeVweight,
+// src/compute_vmed.c:14-21) into w[0, n), in FP64.  The loads of eight cells
+// per thread are issued before any is used (indexes clamped into the ring, so
+// no branch separates them); a guarded loop would wait for each in turn.
+__device__ __forceinline__ void ring_weights(double* w, const real* v, const double* x, int n) {
+  const int t = (int)threadIdx.x, nt = (int)blockDim.x;
+  for (int i0 = 0; i0 < n; i0 += 8 * nt) {
+    double a[8], x0[8], x1[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int i = min(i0 + t + u * nt, n - 1);
+      a[u] = (double)v[i];
+      x0[u] = x[i];
+      x1[u] = x[i + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int i = i0 + t + u * nt;
+      if (i < n) w[i] = a[u] * (x1[u] - x0[u]);
+    }
+  }
+}
+
 // Sequential sum in index order (the reduction order of src/reduction_generic.c
 // :18-20), in FP64.  Software-pipelined: the LDS loads of the next 16 terms
 // are issued before the 16 dependent additions of the current ones, so the
@@ -264,22 +287,28 @@ __device__ __forceinline__ void kd_cfl(int ring, const dmesh& m, const real* cs,
     // ComputeVweight + reduction_SUM (src/compute_vmed.c:14-30) of rings j and j+1
     // of vxm (the velocity whose VxMed the reference's cfl reads: with several
     // fluids, the last fluid's, src/main.c:366 then :381)
-    for (int i = threadIdx.x; i < m.nx; i += blockDim.x) {
-      const double dx = m.xmin64[i + 1] - m.xmin64[i];
-      w0[i] = (double)vxm[base + i] * dx;
-      if (cp.fargo) w1[i] = (double)vxm[base + m.pitch + i] * dx;
-    }
+    ring_weights(w0, vxm + base, m.xmin64, m.nx);
+    if (cp.fargo) ring_weights(w1, vxm + base + m.pitch, m.xmin64, m.nx);
     __syncthreads();
     if (threadIdx.x == 0) sh[8] = seq_sum(w0, m.nx) / (m.XMAX - m.XMIN);
     if (threadIdx.x == 64 && cp.fargo) sh[9] = seq_sum(w1, m.nx) / (m.XMAX - m.XMIN);
   }
   __syncthreads();
   const double vm0 = sh[8];
+  // MAX over the ring's cells; four cells per thread at a time, their loads
+  // issued together (indexes clamped into the ring, the extra cells masked)
   double mx = 0.0;
-#pragma unroll 4
-  for (int i = threadIdx.x; i < m.nx; i += blockDim.x) {
-    const double s = cell::cfl_sumsq<G, DZ>(m, i, j, k, base + i, cs, rho, vx, vy, vz, vm0, cp);
-    mx = (s > mx) ? s : mx;
+  const int t = (int)threadIdx.x, nt = (int)blockDim.x;
+  for (int i0 = 0; i0 < m.nx; i0 += 4 * nt) {
+    double s4[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int i = min(i0 + t + u * nt, m.nx - 1);
+      s4[u] = cell::cfl_sumsq<G, DZ>(m, i, j, k, base + i, cs, rho, vx, vy, vz, vm0, cp);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (i0 + t + u * nt < m.nx) mx = (s4[u] > mx) ? s4[u] : mx;
   }
   mx = block_max(mx, sh);
   if (threadIdx.x == 0) {
@@ -310,8 +339,7 @@ __global__ void __launch_bounds__(WG) kf_vmed_rings(dmesh m, const real* __restr
   const int j = m.nghy + (int)(blockIdx.x % nry);
   const int k = DZ ? m.nghz + (int)(blockIdx.x / nry) : 0;
   const int base = j * m.pitch + k * m.stride;
-  for (int i = threadIdx.x; i < m.nx; i += blockDim.x)
-    w[i] = (double)v[base + i] * (m.xmin64[i + 1] - m.xmin64[i]);
+  ring_weights(w, v + base, m.xmin64, m.nx);
   __syncthreads();
   if (threadIdx.x == 0) vxmed[j + k * m.sy] = seq_sum(w, m.nx) / (m.XMAX - m.XMIN);
 }
@@ -1089,8 +1117,7 @@ __device__ __forceinline__ void kd_vmed(const dim3 bid, const dmesh& m, const re
   const int j = m.nghy + (int)(bid.x % m.ny);
   const int k = DZ ? m.nghz + (int)(bid.x / m.ny) : 0;
   const int base = j * m.pitch + k * m.stride;
-  for (int i = threadIdx.x; i < m.nx; i += blockDim.x)
-    w[i] = (double)vxt[base + i] * (m.xmin64[i + 1] - m.xmin64[i]);
+  ring_weights(w, vxt + base, m.xmin64, m.nx);
   __syncthreads();
   if (threadIdx.x == 0) vxmed[j + k * m.sy] = seq_sum(w, m.nx) / (m.XMAX - m.XMIN);
 }
@@ -1796,12 +1823,15 @@ static int cfl_mode(const dmesh& m, int nf) {
 }
 
 // VxMed of every active ring and of the ring above the last (the shear limit
-// reads ring j+1), for the CFL kernel's pre-pass form
+// reads ring j+1), for the CFL kernel's pre-pass form: one wave per ring, four
+// for rings of 1024 cells or more (measured on MI355X: their loads, issued
+// by one wave, would take longer than the sum)
 static int vmed_prepass(f3d_hip_ctx* ctx, Scratch* S, const real* v) {
   const dmesh& m = S->m;
   const int nry = m.ny + (ctx->ph.fargo ? 1 : 0);
   const int nr = nry * (m.dimz ? m.nz : 1);
-  FDISPATCH(ctx, kf_vmed_rings, dim3(nr), dim3(64), m.nx * sizeof(double), v, ctx->f.VxMed, nry);
+  FDISPATCH(ctx, kf_vmed_rings, dim3(nr), dim3(m.nx >= 1024 ? 256 : 64), m.nx * sizeof(double), v, ctx->f.VxMed,
+            nry);
   return 0;
 }
 
