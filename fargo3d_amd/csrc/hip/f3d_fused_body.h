@@ -2261,9 +2261,10 @@ int F3D_API(fused_cfl_vm)(f3d_hip_ctx* ctx, const double* vxm_in, double* fluid_
 }
 
 // cfl of nf fluids with one read-back: every launch takes its MIN into the
-// same slot (MIN is order independent, so the result is CflFluidsMin's)
-int F3D_API(fused_cfl_fluids)(f3d_hip_ctx* ctx, int nf, double* const* cs, double* const* rho, double* const* vx,
-                              double* const* vy, double* const* vz, const double* vxm_in, double* step_min) {
+// same slot (MIN is order independent, so the result is CflFluidsMin's).
+// The launches are queued here; fused_cfl_wait reads the step back.
+int F3D_API(fused_cfl_fluids_async)(f3d_hip_ctx* ctx, int nf, double* const* cs, double* const* rho,
+                                    double* const* vx, double* const* vy, double* const* vz, const double* vxm_in) {
   Scratch* S;
   F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
   F3D_REQUIRE(ctx, nf >= 1 && nf <= 8, "cfl: 1 to 8 fluids");
@@ -2303,10 +2304,25 @@ int F3D_API(fused_cfl_fluids)(f3d_hip_ctx* ctx, int nf, double* const* cs, doubl
   f3d_stage_end(ctx, F3D_ST_CFL);
   if (!(mode & 1))
     F3D_CHECK(ctx, hipMemcpyAsync(ctx->hres, S->result + slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  F3D_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (!ctx->cfl_done) F3D_CHECK(ctx, hipEventCreateWithFlags(&ctx->cfl_done, hipEventDisableTiming));
+  F3D_CHECK(ctx, hipEventRecord(ctx->cfl_done, ctx->stream));
   S->slot ^= 1;
+  return 0;
+}
+
+// The step queued by fused_cfl_fluids_async, once its launches are done
+// (work queued behind them may still run).
+int F3D_API(fused_cfl_wait)(f3d_hip_ctx* ctx, double* step_min) {
+  F3D_REQUIRE(ctx, ctx->cfl_done != nullptr, "cfl_wait: no CFL queued");
+  F3D_CHECK(ctx, hipEventSynchronize(ctx->cfl_done));
   *step_min = ctx->hres[0];
   return 0;
+}
+
+int F3D_API(fused_cfl_fluids)(f3d_hip_ctx* ctx, int nf, double* const* cs, double* const* rho, double* const* vx,
+                              double* const* vy, double* const* vz, const double* vxm_in, double* step_min) {
+  if (F3D_API(fused_cfl_fluids_async)(ctx, nf, cs, rho, vx, vy, vz, vxm_in)) return -1;
+  return F3D_API(fused_cfl_wait)(ctx, step_min);
 }
 
 int F3D_API(fused_potential)(f3d_hip_ctx* ctx) {

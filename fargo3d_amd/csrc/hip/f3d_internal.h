@@ -31,6 +31,9 @@ struct f3d_hip_ctx {
   // side_done (side -> main); created on first use
   hipStream_t side = nullptr;
   hipEvent_t side_go = nullptr, side_done = nullptr;
+  // recorded after the fused CFL's last launch: the dt read-back waits for
+  // it, not for work queued behind the CFL (created on first use)
+  hipEvent_t cfl_done = nullptr;
   int device = 0;
   f3d::dmesh m{};
   double* meshbuf = nullptr;        // one device buffer for all 1-D mesh arrays

@@ -141,6 +141,7 @@ void f3d_hip_destroy(f3d_hip_ctx* c) {
   if (c->side) hipStreamSynchronize(c->side);
   if (c->side_go) hipEventDestroy(c->side_go);
   if (c->side_done) hipEventDestroy(c->side_done);
+  if (c->cfl_done) hipEventDestroy(c->cfl_done);
   if (c->side) hipStreamDestroy(c->side);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;

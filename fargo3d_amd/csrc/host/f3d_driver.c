@@ -649,11 +649,10 @@ int f3d_sim_step(f3d_sim *S, double *dt_out) {
     ERR(S, "the FP32 form runs the fused stages only");
   int feel = 0;
   for (int k = 0; k < S->sys.nb; k++) feel |= S->sys.feeldisk[k];
-  /* the Potential kernel needs no dt: queue it ahead of the CFL, so that after
-     the dt read-back only the host planet update stands before the sources */
-  if (S->arch.fused_sources && S->p.potential && !feel && S->alreadycalculated != S->clk.Timestepcount &&
-      launch_potential(S))
-    return -1;
+  /* the Potential kernel needs no dt: it is queued behind the CFL and runs
+     while the host waits for and handles dt */
+  const int pot_early =
+      S->arch.fused_sources && S->p.potential && !feel && S->alreadycalculated != S->clk.Timestepcount;
   /* MULTIFLUID(ComputeVmed(Vx)); MULTIFLUID(cfl()) (src/main.c:366-381): every
      fluid's CFL reads the VxMed of the last fluid */
   double step = 1e30;
@@ -675,11 +674,14 @@ int f3d_sim_step(f3d_sim *S, double *dt_out) {
     if (f3d_sim_push_phys(S)) return -1;
     double fluids_min;
     if (S->prec32)
-      CALL(S, f3d32_fused_cfl_fluids(c, nfl, cs, rho, vx, vy, vz, S->fl[nfl - 1].Vx, &fluids_min));
+      CALL(S, f3d32_fused_cfl_fluids_async(c, nfl, cs, rho, vx, vy, vz, S->fl[nfl - 1].Vx));
     else
-      CALL(S, f3d_fused_cfl_fluids(c, nfl, cs, rho, vx, vy, vz, S->fl[nfl - 1].Vx, &fluids_min));
+      CALL(S, f3d_fused_cfl_fluids_async(c, nfl, cs, rho, vx, vy, vz, S->fl[nfl - 1].Vx));
+    if (pot_early && launch_potential(S)) return -1;
+    CALL(S, S->prec32 ? f3d32_fused_cfl_wait(c, &fluids_min) : f3d_fused_cfl_wait(c, &fluids_min));
     step = fluids_min;
   } else {
+    if (pot_early && launch_potential(S)) return -1;
     for (int k = 0; k < nfl; k++) {
       select_fluid(S, k);
       if (f3d_sim_push_phys(S)) return -1;

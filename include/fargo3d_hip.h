@@ -259,6 +259,14 @@ int f3d_fused_cfl_vm(f3d_hip_ctx *ctx, const double *vxm, double *fluid_min);
  * sums are taken over vxm as in f3d_fused_cfl_vm (vz ignored in 2-D). */
 int f3d_fused_cfl_fluids(f3d_hip_ctx *ctx, int nf, double *const *cs, double *const *rho, double *const *vx,
                          double *const *vy, double *const *vz, const double *vxm, double *step_min);
+/* f3d_fused_cfl_fluids in two halves (no counterpart in the reference, whose
+ * cfl returns after its reduction): _async queues the launches and returns;
+ * f3d_fused_cfl_wait waits for them alone and returns the step.  A host can
+ * queue work that does not need dt (the planets' potential) in between, so
+ * that it runs while the host handles dt. */
+int f3d_fused_cfl_fluids_async(f3d_hip_ctx *ctx, int nf, double *const *cs, double *const *rho,
+                               double *const *vx, double *const *vy, double *const *vz, const double *vxm);
+int f3d_fused_cfl_wait(f3d_hip_ctx *ctx, double *step_min);
 
 /* ---- coupling of several fluids (src/main.c:412-418) ---- */
 /* Collisions(dt, 1): src/collisions.c:156 (+ _collisions_cpu :10, ColRate
@@ -299,6 +307,9 @@ int f3d32_fused_set_cflmode(int mode);
 int f3d32_fused_cfl_vm(f3d_hip_ctx *ctx, const double *vxm, double *fluid_min);
 int f3d32_fused_cfl_fluids(f3d_hip_ctx *ctx, int nf, double *const *cs, double *const *rho, double *const *vx,
                            double *const *vy, double *const *vz, const double *vxm, double *step_min);
+int f3d32_fused_cfl_fluids_async(f3d_hip_ctx *ctx, int nf, double *const *cs, double *const *rho,
+                                 double *const *vx, double *const *vy, double *const *vz, const double *vxm);
+int f3d32_fused_cfl_wait(f3d_hip_ctx *ctx, double *step_min);
 int f3d32_fused_collisions(f3d_hip_ctx *ctx, double dt, int nf, double *const *rho, double *const *vxt,
                            double *const *vyt, double *const *vzt, const double *alpha);
 int f3d32_fused_dust_diffusion(f3d_hip_ctx *ctx, double dt, int nf, double *const *rho, const int *type,
