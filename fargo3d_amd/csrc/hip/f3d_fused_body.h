@@ -804,13 +804,19 @@ struct XQuant {
   static constexpr int first = GRP == 2 ? NA : 0;  // global index of local quantity 0
 };
 
-template <int G, int DZ, int AD, int CPT, int GRP>
+// An empty asm that "changes" the LDS row stride after each barrier: the
+// compiler then computes each phase's LDS addresses in that phase instead of
+// holding every phase's addresses in registers from the start (measured:
+// 118 -> 82 registers with the VPRE form, 4 -> 5 waves per SIMD).
+#define XR_REFRESH() asm volatile("" : "+s"(nxv))
+// VPRE: VxMed of the ring from kf_vmed (the kernel then carries no sum).
+template <int G, int DZ, int AD, int CPT, int GRP, int VPRE>
 __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt64, const real* __restrict__ rho_in,
                                             const real* __restrict__ vxt, const real* const* qin,
                                             const real* __restrict__ e_in, real* __restrict__ e_out,
                                             real* __restrict__ rho_out, real* __restrict__ vx_out,
                                             real* const* qout, double* __restrict__ vxmed_out,
-                                            int* __restrict__ nshift_out, real omegaframe, int fargo, int vpre) {
+                                            int* __restrict__ nshift_out, real omegaframe, int fargo) {
   using XQ = XQuant<DZ, AD, GRP>;
   constexpr int NL = XQ::NL;
   constexpr int N = NL + 1;  // + the density (local index NL)
@@ -843,11 +849,11 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
         st[s][c] = (q < NM) ? qin[q][base + i] : e_in[base + i];
       }
       vt[s] = vxt[base + i];
-      if (!vpre) Wt[i] = vt[s] * (m.xmin64[i + 1] - m.xmin64[i]);
+      if (!VPRE) Wt[i] = vt[s] * (m.xmin64[i + 1] - m.xmin64[i]);
     }
   }
   double vmed;
-  if (vpre) {
+  if constexpr (VPRE) {
     // VxMed of this ring from kf_vmed, run before this launch
     vmed = vxmed_out[j + k * m.sy];
   } else {
@@ -880,14 +886,17 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
     ns = (int)sh[1];
     if (GRP != 2 && t == 0) {
       nshift_out[j + k * m.sy] = ns;
-      if (!vpre) vxmed_out[j + k * m.sy] = vmed;
+      if (!VPRE) vxmed_out[j + k * m.sy] = vmed;
     }
   }
+  int nxv = nx;  // LDS row stride, re-read after each barrier (keeps the
+  // addresses of each phase out of the registers of the others)
   // X_advection (src/transport.c:208-226), twice with FARGO
   for (int pass = 0; pass < (fargo ? 2 : 1); pass++) {
     const bool ppa = (pass == 1);
     if (ppa) {
-      __syncthreads();  // the last update read V of the neighbours
+      __syncthreads();
+      XR_REFRESH();  // the last update read V of the neighbours
 #pragma unroll
       for (int s = 0; s < CPT; s++) {
         const int i = t + s * nt;
@@ -901,11 +910,12 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
       if (i < nx) {
         const real r = st[s][NL];
 #pragma unroll
-        for (int c = 0; c < NL; c++) AQ[c * nx + i] = st[s][c] / r;
-        AQ[NL * nx + i] = r;
+        for (int c = 0; c < NL; c++) AQ[c * nxv + i] = st[s][c] / r;
+        AQ[NL * nxv + i] = r;
       }
     }
     __syncthreads();
+      XR_REFRESH();
     if (!ppa) {
       // VanLeerX_a (src/vanleer_x_a.c:60-68).  The gradient at face i,
       // (q_i - q_{i-1}) / zone_i, is cell i's left and cell i-1's right
@@ -918,22 +928,24 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
           const real zi = zone_size_x<G>(m, i, j, k);
 #pragma unroll
           for (int c = 0; c < N; c++) {
-            const real* a = AQ + c * nx;
-            FC[c * nx + i] = (a[i] - a[im]) / zi;
+            const real* a = AQ + c * nxv;
+            FC[c * nxv + i] = (a[i] - a[im]) / zi;
           }
         }
       }
       __syncthreads();
+      XR_REFRESH();
 #pragma unroll
       for (int s = 0; s < CPT; s++) {
         const int i = t + s * nt;
         if (i < nx) {
           const int ip = ixp_of(i, nx);
 #pragma unroll
-          for (int c = 0; c < N; c++) SL[c * nx + i] = vanleer_slope(FC[c * nx + i], FC[c * nx + ip]);
+          for (int c = 0; c < N; c++) SL[c * nxv + i] = vanleer_slope(FC[c * nxv + i], FC[c * nxv + ip]);
         }
       }
       __syncthreads();
+      XR_REFRESH();
       // VanLeerX_b (src/vanleer_x_b.c:60-65)
 #pragma unroll
       for (int s = 0; s < CPT; s++) {
@@ -944,13 +956,14 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
           const real v = V[i];
 #pragma unroll
           for (int c = 0; c < N; c++) {
-            const real* a = AQ + c * nx;
-            const real* sl = SL + c * nx;
-            FC[c * nx + i] = vanleer_face(v, dt, a[im], sl[im], zm, a[i], sl[i], zi);
+            const real* a = AQ + c * nxv;
+            const real* sl = SL + c * nxv;
+            FC[c * nxv + i] = vanleer_face(v, dt, a[im], sl[im], zm, a[i], sl[i], zi);
           }
         }
       }
       __syncthreads();
+      XR_REFRESH();
     } else {
       // VanLeerX_PPA_a: monotonised-centred slope (src/fargo_ppa_a.c:58-69)
 #pragma unroll
@@ -960,12 +973,13 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
           const int im = ixm_of(i, nx), ip = ixp_of(i, nx);
 #pragma unroll
           for (int c = 0; c < N; c++) {
-            const real* a = AQ + c * nx;
-            SL[c * nx + i] = cell::ppa_slope(a[im], a[i], a[ip]);
+            const real* a = AQ + c * nxv;
+            SL[c * nxv + i] = cell::ppa_slope(a[im], a[i], a[ip]);
           }
         }
       }
       __syncthreads();
+      XR_REFRESH();
       // VanLeerX_PPA_b: face value q_{i+1/2} = qR of cell i = qL of cell i+1
 #pragma unroll
       for (int s = 0; s < CPT; s++) {
@@ -974,13 +988,14 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
           const int ip = ixp_of(i, nx);
 #pragma unroll
           for (int c = 0; c < N; c++) {
-            const real* a = AQ + c * nx;
-            const real* sl = SL + c * nx;
-            FC[c * nx + i] = cell::ppa_face(a[i], a[ip], sl[i], sl[ip]);
+            const real* a = AQ + c * nxv;
+            const real* sl = SL + c * nxv;
+            FC[c * nxv + i] = cell::ppa_face(a[i], a[ip], sl[i], sl[ip]);
           }
         }
       }
       __syncthreads();
+      XR_REFRESH();
       // VanLeerX_PPA_c (src/fargo_ppa_c.c): the limited (qL, qR) of each cell;
       // qL goes to SL (slopes are no longer read), qR to FC once every
       // neighbour has read its face
@@ -992,23 +1007,25 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
           const int im = ixm_of(i, nx);
 #pragma unroll
           for (int c = 0; c < N; c++) {
-            real qL = FC[c * nx + im], qR = FC[c * nx + i];
-            cell::ppa_limit(AQ[c * nx + i], qL, qR);
-            SL[c * nx + i] = qL;
+            real qL = FC[c * nxv + im], qR = FC[c * nxv + i];
+            cell::ppa_limit(AQ[c * nxv + i], qL, qR);
+            SL[c * nxv + i] = qL;
             qr[s][c] = qR;
           }
         }
       }
       __syncthreads();
+      XR_REFRESH();
 #pragma unroll
       for (int s = 0; s < CPT; s++) {
         const int i = t + s * nt;
         if (i < nx) {
 #pragma unroll
-          for (int c = 0; c < N; c++) FC[c * nx + i] = qr[s][c];
+          for (int c = 0; c < N; c++) FC[c * nxv + i] = qr[s][c];
         }
       }
       __syncthreads();
+      XR_REFRESH();
       // VanLeerX_PPA_d: parabolic average over the swept zone (src/fargo_ppa_d.c:62-70);
       // the fluxes replace qR in FC once every neighbour has read it
 #pragma unroll
@@ -1020,23 +1037,25 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
           const real v = V[i];
 #pragma unroll
           for (int c = 0; c < N; c++) {
-            const real* a = AQ + c * nx;
-            const real* qL = SL + c * nx;
-            const real* qR = FC + c * nx;
+            const real* a = AQ + c * nxv;
+            const real* qL = SL + c * nxv;
+            const real* qR = FC + c * nxv;
             qr[s][c] = cell::ppa_flux_value(v, dt, zi, a[im], qL[im], qR[im], a[i], qL[i], qR[i]);
           }
         }
       }
       __syncthreads();
+      XR_REFRESH();
 #pragma unroll
       for (int s = 0; s < CPT; s++) {
         const int i = t + s * nt;
         if (i < nx) {
 #pragma unroll
-          for (int c = 0; c < N; c++) FC[c * nx + i] = qr[s][c];
+          for (int c = 0; c < N; c++) FC[c * nxv + i] = qr[s][c];
         }
       }
       __syncthreads();
+      XR_REFRESH();
     }
     // UpdateX for every quantity and UpdateDensityX last (src/update_x.c:69-71,
     // src/update_density_x.c:66-68), on the registers
@@ -1046,10 +1065,10 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
       if (i < nx) {
         const int ip = ixp_of(i, nx);
         const real v0 = V[i], v1 = V[ip];
-        const real* ds = FC + NL * nx;
+        const real* ds = FC + NL * nxv;
 #pragma unroll
         for (int c = 0; c < NL; c++) {
-          const real* f = FC + c * nx;
+          const real* f = FC + c * nxv;
           st[s][c] += cell::flux_x(m, i, j, k, dt, v0 * f[i] * ds[i], v1 * f[ip] * ds[ip]);
         }
         st[s][NL] += cell::flux_x(m, i, j, k, dt, v0 * ds[i], v1 * ds[ip]);
@@ -1085,6 +1104,7 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
   }
   if (GRP != 2) {
     __syncthreads();
+      XR_REFRESH();
     // NewVelocity_x (src/newvel_x.c:78-86)
 #pragma unroll
     for (int s = 0; s < CPT; s++) {
@@ -2050,21 +2070,21 @@ struct XTab {
   int* nshift[8];
 };
 
-template <int G, int DZ, int AD, int CPT, int SPLIT>
+template <int G, int DZ, int AD, int CPT, int SPLIT, int VPRE>
 __global__ void __launch_bounds__(CPT == 2 ? 512 : 256) kf_sweep_xr_fluids(dmesh m, double dt64, XTab T, int per_fluid,
-                                                           real omegaframe, int fargo, int vpre) {
+                                                           real omegaframe, int fargo) {
   const int lin = xcd_linear();
   const int fl = lin / per_fluid, b = lin % per_fluid;
   real* qout[4] = {T.qo[fl][0], T.qo[fl][1], T.qo[fl][2], T.qo[fl][3]};
   if (!SPLIT) {
-    xsweep_ring<G, DZ, AD, CPT, 0>(m, b, dt64, T.rho_in[fl], T.vxt[fl], T.q[fl], T.e_in[fl], T.e_out[fl],
-                                   T.rho_out[fl], T.vx_out[fl], qout, T.vxmed[fl], T.nshift[fl], omegaframe, fargo, vpre);
+    xsweep_ring<G, DZ, AD, CPT, 0, VPRE>(m, b, dt64, T.rho_in[fl], T.vxt[fl], T.q[fl], T.e_in[fl], T.e_out[fl],
+                                   T.rho_out[fl], T.vx_out[fl], qout, T.vxmed[fl], T.nshift[fl], omegaframe, fargo);
   } else if ((b & 1) == 0) {
-    xsweep_ring<G, DZ, AD, CPT, 1>(m, b >> 1, dt64, T.rho_in[fl], T.vxt[fl], T.q[fl], T.e_in[fl], T.e_out[fl],
-                                   T.rho_out[fl], T.vx_out[fl], qout, T.vxmed[fl], T.nshift[fl], omegaframe, fargo, vpre);
+    xsweep_ring<G, DZ, AD, CPT, 1, VPRE>(m, b >> 1, dt64, T.rho_in[fl], T.vxt[fl], T.q[fl], T.e_in[fl], T.e_out[fl],
+                                   T.rho_out[fl], T.vx_out[fl], qout, T.vxmed[fl], T.nshift[fl], omegaframe, fargo);
   } else {
-    xsweep_ring<G, DZ, AD, CPT, 2>(m, b >> 1, dt64, T.rho_in[fl], T.vxt[fl], T.q[fl], T.e_in[fl], T.e_out[fl],
-                                   T.rho_out[fl], T.vx_out[fl], qout, T.vxmed[fl], T.nshift[fl], omegaframe, fargo, vpre);
+    xsweep_ring<G, DZ, AD, CPT, 2, VPRE>(m, b >> 1, dt64, T.rho_in[fl], T.vxt[fl], T.q[fl], T.e_in[fl], T.e_out[fl],
+                                   T.rho_out[fl], T.vx_out[fl], qout, T.vxmed[fl], T.nshift[fl], omegaframe, fargo);
   }
 }
 
@@ -2113,9 +2133,11 @@ static int x_sweep_ring(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const R
   }
   dim3 gx(per * nf);
   f3d_stage_begin(ctx, F3D_ST_SWEEP_X);
-#define XSWEEP1(G_, DZ_, AD_, CPT_, SP_)                                                                      \
-  hipLaunchKernelGGL((kf_sweep_xr_fluids<G_, DZ_, AD_, CPT_, SP_>), gx, dim3(nthreads), shx, ctx->stream, m, dt, \
-                     T, per, of, ctx->ph.fargo, vpre)
+#define XSWEEP0(G_, DZ_, AD_, CPT_, SP_, VP_)                                                                  \
+  hipLaunchKernelGGL((kf_sweep_xr_fluids<G_, DZ_, AD_, CPT_, SP_, VP_>), gx, dim3(nthreads), shx, ctx->stream, m, \
+                     dt, T, per, of, ctx->ph.fargo)
+#define XSWEEP1(G_, DZ_, AD_, CPT_, SP_) \
+  do { if (vpre) XSWEEP0(G_, DZ_, AD_, CPT_, SP_, 1); else XSWEEP0(G_, DZ_, AD_, CPT_, SP_, 0); } while (0)
 #define XSWEEP2(G_, DZ_, AD_, CPT_) \
   do { if (split) XSWEEP1(G_, DZ_, AD_, CPT_, 1); else XSWEEP1(G_, DZ_, AD_, CPT_, 0); } while (0)
 #define XSWEEP(G_, DZ_, CPT_) \
@@ -2128,6 +2150,7 @@ static int x_sweep_ring(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const R
     if (m.dimz) XSWEEP_G(CYL, 1); else XSWEEP_G(CYL, 0);
   }
 #undef XSWEEP_G
+#undef XSWEEP0
 #undef XSWEEP1
 #undef XSWEEP2
 #undef XSWEEP
