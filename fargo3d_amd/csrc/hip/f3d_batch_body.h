@@ -118,7 +118,7 @@ template <int T>
 __global__ void __launch_bounds__(T) kb_sweep_x(const BInst* I, const int* off, int n) {
   const int ii = find_inst(off, n, blockIdx.x);
   const BInst& in = I[ii];
-  const int tx = (in.m.nx + T - 1) / T;
+  const int tx = (in.m.nx + x_tile_cells(T) - 1) / x_tile_cells(T);
   kd_sweep_x_tile<CYL, 0, 0, T>(local2(blockIdx.x - off[ii], tx), in.m, in.clk[0], in.rhoy, in.vxt, in.mpx, in.mmx,
                                 in.mpy, in.mmy, nullptr, nullptr, in.ey, in.e2, in.rhox, in.vx, in.xm0, in.xm1,
                                 in.xm2, in.xm3, in.vxmed, in.nshift, in.of, in.fargo);
@@ -193,7 +193,7 @@ static int batch_layout(f3d_batch* B) {
     c[BK_VMED][i] = m.ny;
     c[BK_FIN][i] = gx * m.ny;
     swy[i] = gx * ((m.ny + B->rows_y - 1) / B->rows_y);
-    xt[i] = ((m.nx + B->tile - 1) / B->tile) * m.ny;
+    xt[i] = ((m.nx + x_tile_cells(B->tile) - 1) / x_tile_cells(B->tile)) * m.ny;
   }
   for (int k = 0; k < BK_N; k++)
     if (k != BK_SWY && k != BK_XT && batch_offsets(B, c[k], &B->doff[k], &B->total[k])) return -1;

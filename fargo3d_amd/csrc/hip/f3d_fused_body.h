@@ -1147,205 +1147,206 @@ __global__ void __launch_bounds__(64) kf_vmed(dmesh m, VmTab T, int nrings) {
   kd_vmed<G, DZ>(dim3(blockIdx.x % nrings), m, T.vxt[fl], T.vxmed[fl]);
 }
 
-// X sweep of rings too long for LDS: workgroups own tiles of T cells and
-// load H = 6 halo cells on each side (the van Leer pass needs 2, the PPA pass
-// 3 more of the van Leer result, NewVelocity_x the left neighbour), so no
-// tile waits for another.  VxMed comes
-// from kf_vmed.  The outputs go to their shifted positions directly
-// (AdvectSHIFT, src/advect_shift.c:45-58) with NewVelocity_x computed at the
-// destination cell from the source cell and its left neighbour.
+// X sweep of rings too long for LDS: a workgroup of T threads owns a tile of
+// T - 2H cells and loads H = 6 halo cells on each side (the van Leer pass
+// needs 2, the PPA pass 3 more of the van Leer result, NewVelocity_x the left
+// neighbour), so no tile waits for another; thread t owns tile position t.
+// VxMed comes from kf_vmed.  The outputs go to their shifted positions
+// directly (AdvectSHIFT, src/advect_shift.c:45-58) with NewVelocity_x computed
+// at the destination cell from the source cell and its left neighbour.
+constexpr int kXHalo = 6;
+__host__ __device__ constexpr int x_tile_cells(int threads) { return threads - 2 * kXHalo; }
 template <int G, int DZ, int AD, int T>
 __device__ __forceinline__ void kd_sweep_x_tile(const dim3 bid, const dmesh& m, double dt64, const real* rho_in, const real* vxt, const real* mpx, const real* mmx, const real* mpy, const real* mmy, const real* mpz, const real* mmz, const real* e_in, real* e_out, real* rho_out, real* vx_out, real* mpy_o, real* mmy_o, real* mpz_o, real* mmz_o, const double* vxmed, int* nshift_out, real omegaframe, int fargo) {
-  constexpr int H = 6, W = T + 2 * H;
-  static_assert(2 * H <= T, "tile narrower than its halos");
+  constexpr int H = kXHalo, W = T, TW = x_tile_cells(T);
+  static_assert(2 * H <= TW, "tile narrower than its halos");
   const real dt = (real)dt64;
   constexpr int NM = DZ ? 6 : 4;
   constexpr int NQ = NM + AD;
   constexpr int N = NQ + 1;
-  // q = Q/rho, slopes (then limited qL), faces (then qR, then face fluxes);
-  // the cell state stays in registers: thread t owns tile positions t and,
-  // for t < 2H, T + t
+  // q = Q/rho, slopes (then limited qL), faces (then qR, then face fluxes),
+  // the advection velocity of the pass (the cell residual, then the uniform
+  // one); the cell state stays in registers
   __shared__ real AQ[N * W], SL[N * W], FC[N * W];
-  __shared__ real V[W], VU[W], ZS[W];
+  __shared__ real V[W];
   const int nx = m.nx;
   const int ring = bid.y;
   const int j = m.nghy + ring % m.ny;
   const int k = DZ ? m.nghz + ring / m.ny : 0;
   const int base = j * m.pitch + k * m.stride;
-  const int i0 = bid.x * T;
-  const int t = threadIdx.x;
+  const int i0 = bid.x * TW;
+  const int p = threadIdx.x;
   const double vmed = vxmed[j + k * m.sy];
   const real* qin[7] = {mpx, mmx, mpy, mmy, DZ ? mpz : e_in, DZ ? mmz : e_in, e_in};
-  auto gidx = [&](int p) {
-    int g = (i0 - H + p) % nx;
+  auto gidx = [&](int q) {
+    int g = (i0 - H + q) % nx;
     return g < 0 ? g + nx : g;
   };
   // the integer shift of the ring: ComputeResidual at i = 0 (src/compute_residual.c:68-71)
   const int ns = fargo ? (int)floor(vmed * dt64 / zone_size_x64<G>(m, 0, j, k) + 0.5) : 0;
-  if (fargo && bid.x == 0 && t == 0) nshift_out[j + k * m.sy] = ns;
-  real st[2][N];
+  if (fargo && bid.x == 0 && p == 0) nshift_out[j + k * m.sy] = ns;
+  const int g = gidx(p);
+  real st[N];
+  st[NQ] = rho_in[base + g];
 #pragma unroll
-  for (int s = 0; s < 2; s++) {
-    const int p = t + s * T;
-    if (s == 0 || t < 2 * H) {
-      const int g = gidx(p);
-      st[s][NQ] = rho_in[base + g];
-#pragma unroll
-      for (int c = 0; c < NQ; c++) st[s][c] = (c < NM) ? qin[c][base + g] : e_in[base + g];
-      ZS[p] = zone_size_x<G>(m, g, j, k);
-      const double v = vxt[base + g];
-      if (fargo) {
-        // the residual split in FP64 (src/compute_residual.c:68-73)
-        const double zs = zone_size_x64<G>(m, g, j, k);
-        const double ntilde = vmed * dt64 / zs;
-        const double nround = floor(ntilde + 0.5);
-        V[p] = (real)(v - vmed);
-        VU[p] = (real)((ntilde - nround) * zs / dt64);
-      } else {
-        V[p] = (real)v;
-      }
+  for (int c = 0; c < NQ; c++) st[c] = (c < NM) ? qin[c][base + g] : e_in[base + g];
+  const real zs = zone_size_x<G>(m, g, j, k);
+  // the left neighbour's zone (the upwind face reads it)
+  const real zsm = zone_size_x<G>(m, gidx(p - 1), j, k);
+  real vp, vu = real(0.0);  // this cell's advection velocity in the current pass; the uniform residual
+  {
+    const double v = vxt[base + g];
+    if (fargo) {
+      // the residual split in FP64 (src/compute_residual.c:68-73)
+      const double zs64 = zone_size_x64<G>(m, g, j, k);
+      const double ntilde = vmed * dt64 / zs64;
+      const double nround = floor(ntilde + 0.5);
+      vp = (real)(v - vmed);
+      vu = (real)((ntilde - nround) * zs64 / dt64);
+    } else {
+      vp = (real)v;
     }
-  }
-  // a phase visits the owned positions in [lo, hi)
-#define TILE_FOR(lo, hi, ...)                                      \
-  _Pragma("unroll") for (int s = 0; s < 2; s++) {                  \
-    const int p = t + s * T;                                       \
-    if ((s == 0 || t < 2 * H) && p >= (lo) && p < (hi)) __VA_ARGS__ \
+    V[p] = vp;
   }
   for (int pass = 0; pass < (fargo ? 2 : 1); pass++) {
-    const real* VV = pass ? VU : V;
     // valid range of the state entering this pass: [lo, hi)
     const int lo = pass ? 2 : 0, hi = pass ? W - 2 : W;
-    TILE_FOR(lo, hi, {
-      const real r = st[s][NQ];
-_Pragma("unroll")
-      for (int c = 0; c < NQ; c++) AQ[c * W + p] = st[s][c] / r;
+    if (p >= lo && p < hi) {
+      const real r = st[NQ];
+#pragma unroll
+      for (int c = 0; c < NQ; c++) AQ[c * W + p] = st[c] / r;
       AQ[NQ * W + p] = r;
-    })
-    __syncthreads();  // also orders the first pass after the loads of V, VU, ZS
+    }
+    __syncthreads();  // also orders the first pass after the stores of V
+    if (pass) {
+      // the uniform residual: every read of the cell residual ended before
+      // the barrier above
+      vp = vu;
+      V[p] = vu;
+    }
     int ulo, uhi;  // positions updated by this pass
     if (!pass) {
       // the gradient at face p, (q_p - q_{p-1}) / zone_p, once per face into
       // FC: the left gradient of cell p and the right one of cell p-1
-      TILE_FOR(1, W, {
-_Pragma("unroll")
+      if (p >= 1) {
+#pragma unroll
         for (int c = 0; c < N; c++) {
           const real* a = AQ + c * W;
-          FC[c * W + p] = (a[p] - a[p - 1]) / ZS[p];
+          FC[c * W + p] = (a[p] - a[p - 1]) / zs;
         }
-      })
+      }
       __syncthreads();
-      TILE_FOR(1, W - 1, {
-_Pragma("unroll")
+      if (p >= 1 && p < W - 1) {
+#pragma unroll
         for (int c = 0; c < N; c++) SL[c * W + p] = vanleer_slope(FC[c * W + p], FC[c * W + p + 1]);
-      })
+      }
       __syncthreads();
-      TILE_FOR(2, W - 1, {
-_Pragma("unroll")
+      if (p >= 2 && p < W - 1) {
+#pragma unroll
         for (int c = 0; c < N; c++) {
           const real* a = AQ + c * W;
           const real* sl = SL + c * W;
-          FC[c * W + p] = vanleer_face(VV[p], dt, a[p - 1], sl[p - 1], ZS[p - 1], a[p], sl[p], ZS[p]);
+          FC[c * W + p] = vanleer_face(vp, dt, a[p - 1], sl[p - 1], zsm, a[p], sl[p], zs);
         }
-      })
+      }
       __syncthreads();
       ulo = 2;
       uhi = W - 2;
     } else {
-      TILE_FOR(3, W - 3, {
-_Pragma("unroll")
+      if (p >= 3 && p < W - 3) {
+#pragma unroll
         for (int c = 0; c < N; c++) {
           const real* a = AQ + c * W;
           SL[c * W + p] = cell::ppa_slope(a[p - 1], a[p], a[p + 1]);
         }
-      })
+      }
       __syncthreads();
-      TILE_FOR(3, W - 4, {
-_Pragma("unroll")
+      if (p >= 3 && p < W - 4) {
+#pragma unroll
         for (int c = 0; c < N; c++) {
           const real* a = AQ + c * W;
           const real* sl = SL + c * W;
           FC[c * W + p] = cell::ppa_face(a[p], a[p + 1], sl[p], sl[p + 1]);
         }
-      })
+      }
       __syncthreads();
       // limited (qL, qR): qL into SL (slopes are no longer read), qR into FC
       // once every neighbour has read its face
-      real qr[2][N];
-      TILE_FOR(4, W - 4, {
-_Pragma("unroll")
+      real qr[N];
+      const bool lim = p >= 4 && p < W - 4;
+      if (lim) {
+#pragma unroll
         for (int c = 0; c < N; c++) {
           real qL = FC[c * W + p - 1], qR = FC[c * W + p];
           cell::ppa_limit(AQ[c * W + p], qL, qR);
           SL[c * W + p] = qL;
-          qr[s][c] = qR;
+          qr[c] = qR;
         }
-      })
+      }
       __syncthreads();
-      TILE_FOR(4, W - 4, {
-_Pragma("unroll")
-        for (int c = 0; c < N; c++) FC[c * W + p] = qr[s][c];
-      })
+      if (lim) {
+#pragma unroll
+        for (int c = 0; c < N; c++) FC[c * W + p] = qr[c];
+      }
       __syncthreads();
       // face fluxes into FC once every neighbour has read qR
-      TILE_FOR(5, W - 4, {
-_Pragma("unroll")
+      const bool flx = p >= 5 && p < W - 4;
+      if (flx) {
+#pragma unroll
         for (int c = 0; c < N; c++) {
           const real* a = AQ + c * W;
           const real* qL = SL + c * W;
           const real* qR = FC + c * W;
-          qr[s][c] = cell::ppa_flux_value(VV[p], dt, ZS[p], a[p - 1], qL[p - 1], qR[p - 1], a[p], qL[p], qR[p]);
+          qr[c] = cell::ppa_flux_value(vp, dt, zs, a[p - 1], qL[p - 1], qR[p - 1], a[p], qL[p], qR[p]);
         }
-      })
+      }
       __syncthreads();
-      TILE_FOR(5, W - 4, {
-_Pragma("unroll")
-        for (int c = 0; c < N; c++) FC[c * W + p] = qr[s][c];
-      })
+      if (flx) {
+#pragma unroll
+        for (int c = 0; c < N; c++) FC[c * W + p] = qr[c];
+      }
       __syncthreads();
       ulo = 5;
       uhi = W - 5;
     }
     // UpdateX / UpdateDensityX (src/update_x.c:69-71, src/update_density_x.c:66-68), on the registers
-    TILE_FOR(ulo, uhi, {
-      const int g = gidx(p);
-      const real v0 = VV[p], v1 = VV[p + 1];
+    if (p >= ulo && p < uhi) {
+      const real v0 = vp, v1 = V[p + 1];
       const real* ds = FC + NQ * W;
-_Pragma("unroll")
+#pragma unroll
       for (int c = 0; c < NQ; c++) {
         const real* f = FC + c * W;
-        st[s][c] += cell::flux_x(m, g, j, k, dt, v0 * f[p] * ds[p], v1 * f[p + 1] * ds[p + 1]);
+        st[c] += cell::flux_x(m, g, j, k, dt, v0 * f[p] * ds[p], v1 * f[p + 1] * ds[p + 1]);
       }
-      st[s][NQ] += cell::flux_x(m, g, j, k, dt, v0 * ds[p], v1 * ds[p + 1]);
-    })
-    if (pass == 0 && fargo) __syncthreads();  // the PPA pass rewrites AQ
+      st[NQ] += cell::flux_x(m, g, j, k, dt, v0 * ds[p], v1 * ds[p + 1]);
+    }
+    if (pass == 0 && fargo) __syncthreads();  // the PPA pass rewrites AQ and V
   }
-  // the updated density and Mpx of positions [H-1, H+T) for NewVelocity_x
+  // the updated density and Mpx of positions [H-1, H+TW) for NewVelocity_x
   // (AQ is free: its last reads were before the final barrier of the pass)
-  TILE_FOR(H - 1, H + T, {
-    AQ[0 * W + p] = st[s][0];
-    AQ[1 * W + p] = st[s][NQ];
-  })
+  if (p >= H - 1 && p < H + TW) {
+    AQ[0 * W + p] = st[0];
+    AQ[1 * W + p] = st[NQ];
+  }
   __syncthreads();
-  // outputs of the tile's cells [H, H+T) at their shifted positions
+  // outputs of the tile's cells [H, H+TW) at their shifted positions
   // (AdvectSHIFT, src/advect_shift.c:45-58), NewVelocity_x computed at the
   // destination from the source cell and its left neighbour
-  TILE_FOR(H, H + T, {
+  if (p >= H && p < H + TW) {
     const int i = i0 + p - H;
     if (i < nx) {
       int d = (i + ns) % nx;
       if (d < 0) d += nx;
-      vx_out[base + d] = cell::newvel_x<G>(m, d, j, k, st[s][NQ], AQ[1 * W + p - 1], st[s][1], AQ[0 * W + p - 1],
+      vx_out[base + d] = cell::newvel_x<G>(m, d, j, k, st[NQ], AQ[1 * W + p - 1], st[1], AQ[0 * W + p - 1],
                                            omegaframe);
-      rho_out[base + d] = st[s][NQ];
-_Pragma("unroll")
+      rho_out[base + d] = st[NQ];
+#pragma unroll
       for (int c = 2; c < NQ; c++) {
-        if (c < NM) (c == 2 ? mpy_o : c == 3 ? mmy_o : c == 4 ? mpz_o : mmz_o)[base + d] = st[s][c];
-        else e_out[base + d] = st[s][c];
+        if (c < NM) (c == 2 ? mpy_o : c == 3 ? mmy_o : c == 4 ? mpz_o : mmz_o)[base + d] = st[c];
+        else e_out[base + d] = st[c];
       }
     }
-  })
-#undef TILE_FOR
+  }
 }
 template <int G, int DZ, int AD, int T>
 __global__ void __launch_bounds__(T) kf_sweep_x_tile(dmesh m, double dt64, const real* __restrict__ rho_in, const real* __restrict__ vxt,
@@ -2023,7 +2024,7 @@ static int x_sweep_tiled(f3d_hip_ctx* ctx, Scratch* S, double dt, const RF& f, c
   const int T = g_xtile;
   F3D_REQUIRE(ctx, m.nx >= 12, "the tiled X sweep needs at least 12 cells per ring");
   f3d_stage_begin(ctx, F3D_ST_SWEEP_X);
-  dim3 g((m.nx + T - 1) / T, nrings);
+  dim3 g((m.nx + x_tile_cells(T) - 1) / x_tile_cells(T), nrings);
 #define XTILE1(G_, DZ_, AD_, T_)                                                                                \
   hipLaunchKernelGGL((kf_sweep_x_tile<G_, DZ_, AD_, T_>), g, dim3(T_), 0, ctx->stream, m, dt, t.rhoy, f.Vx_temp, \
                      t.m[0], t.m[1], t.m[2], t.m[3], t.m[4], t.m[5], t.ey, f.Energy, t.rhox, f.Vx, XM[0], XM[1],  \
