@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(BXF) kb_sweep_y(const BInst* I, const int* off
 __global__ void __launch_bounds__(WG) kb_vmed(const BInst* I, const int* off, int n) {
   const int ii = find_inst(off, n, blockIdx.x);
   const BInst& in = I[ii];
-  kd_vmed<CYL, 0>(dim3(blockIdx.x - off[ii]), in.m, in.vxt, in.vxmed);
+  kd_vmed<CYL, 0, 0>(dim3(blockIdx.x - off[ii]), in.m, in.vxt, in.vxmed);
 }
 
 template <int T>

@@ -254,6 +254,50 @@ __device__ __forceinline__ double seq_sum(const double* w, int n) {
   return acc;
 }
 
+// seq_sum for the VxMed pass kernels: the same additions in the same order,
+// the LDS terms read in pairs (w 16-byte aligned), 2B terms per batch.
+// Measured on MI355X: 4096 terms 61.6 -> 51.7 us (B = 16), 256 terms
+// 11.9 -> 9.9 us (B = 8), for every ring of the grid at once.  B = 16 holds
+// 128 registers: only for the CFL's pass on long rings, which runs alone;
+// the X sweep's pass shares the CUs with the Z/Y sweeps and keeps B = 8.
+template <int B>
+__device__ __forceinline__ double seq_sum_pairs(const double* w, int n) {
+  const double2* w2 = reinterpret_cast<const double2*>(w);
+  double acc = 0.0;
+  int i = 0;  // in pairs
+  const int n2 = n / 2;
+  if (n2 >= 2 * B) {
+    double2 a[B], b[B];
+#pragma unroll
+    for (int q = 0; q < B; q++) a[q] = w2[q];
+    for (i = B; i + 2 * B <= n2; i += 2 * B) {
+#pragma unroll
+      for (int q = 0; q < B; q++) b[q] = w2[i + q];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < B; q++) {
+        acc = acc + a[q].x;
+        acc = acc + a[q].y;
+      }
+#pragma unroll
+      for (int q = 0; q < B; q++) a[q] = w2[i + B + q];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < B; q++) {
+        acc = acc + b[q].x;
+        acc = acc + b[q].y;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < B; q++) {
+      acc = acc + a[q].x;
+      acc = acc + a[q].y;
+    }
+  }
+  for (int e = 2 * i; e < n; e++) acc = acc + w[e];
+  return acc;
+}
+
 }  // namespace
 
 // ----------------------------------------------------------------- kf_cfl
@@ -333,7 +377,7 @@ __global__ void __launch_bounds__(WG) kf_cfl(dmesh m, const real* __restrict__ c
 // ComputeVmed (src/compute_vmed.c:14-30) of rings j = NGHY .. NGHY+nry-1 of
 // every active plane: one workgroup per ring, the weights in LDS, one
 // sequential sum in i order.
-template <int G, int DZ>
+template <int G, int DZ, int LONG>
 __global__ void __launch_bounds__(WG) kf_vmed_rings(dmesh m, const real* __restrict__ v, double* __restrict__ vxmed, int nry) {
   extern __shared__ double w[];
   const int j = m.nghy + (int)(blockIdx.x % nry);
@@ -341,7 +385,7 @@ __global__ void __launch_bounds__(WG) kf_vmed_rings(dmesh m, const real* __restr
   const int base = j * m.pitch + k * m.stride;
   ring_weights(w, v + base, m.xmin64, m.nx);
   __syncthreads();
-  if (threadIdx.x == 0) vxmed[j + k * m.sy] = seq_sum(w, m.nx) / (m.XMAX - m.XMIN);
+  if (threadIdx.x == 0) vxmed[j + k * m.sy] = (LONG ? seq_sum_pairs<16>(w, m.nx) : seq_sum_pairs<8>(w, m.nx)) / (m.XMAX - m.XMIN);
 }
 
 // The cfl of nf fluids in one launch: nf consecutive blocks of nrings
@@ -1131,7 +1175,7 @@ struct VmTab {
   const real* vxt[8];
   double* vxmed[8];
 };
-template <int G, int DZ>
+template <int G, int DZ, int LONG>
 __device__ __forceinline__ void kd_vmed(const dim3 bid, const dmesh& m, const real* vxt, double* vxmed) {
   extern __shared__ double w[];
   const int j = m.nghy + (int)(bid.x % m.ny);
@@ -1139,12 +1183,12 @@ __device__ __forceinline__ void kd_vmed(const dim3 bid, const dmesh& m, const re
   const int base = j * m.pitch + k * m.stride;
   ring_weights(w, vxt + base, m.xmin64, m.nx);
   __syncthreads();
-  if (threadIdx.x == 0) vxmed[j + k * m.sy] = seq_sum(w, m.nx) / (m.XMAX - m.XMIN);
+  if (threadIdx.x == 0) vxmed[j + k * m.sy] = (LONG ? seq_sum_pairs<16>(w, m.nx) : seq_sum_pairs<8>(w, m.nx)) / (m.XMAX - m.XMIN);
 }
-template <int G, int DZ>
+template <int G, int DZ, int LONG>
 __global__ void __launch_bounds__(64) kf_vmed(dmesh m, VmTab T, int nrings) {
   const int fl = (int)blockIdx.x / nrings;
-  kd_vmed<G, DZ>(dim3(blockIdx.x % nrings), m, T.vxt[fl], T.vxmed[fl]);
+  kd_vmed<G, DZ, LONG>(dim3(blockIdx.x % nrings), m, T.vxt[fl], T.vxmed[fl]);
 }
 
 // X sweep of rings too long for LDS: a workgroup of T threads owns a tile of
@@ -1851,8 +1895,10 @@ static int vmed_prepass(f3d_hip_ctx* ctx, Scratch* S, const real* v) {
   const dmesh& m = S->m;
   const int nry = m.ny + (ctx->ph.fargo ? 1 : 0);
   const int nr = nry * (m.dimz ? m.nz : 1);
-  FDISPATCH(ctx, kf_vmed_rings, dim3(nr), dim3(m.nx >= 1024 ? 256 : 64), m.nx * sizeof(double), v, ctx->f.VxMed,
-            nry);
+  if (m.nx >= 1024)
+    FDISPATCH_T(ctx, kf_vmed_rings, 1, dim3(nr), dim3(256), m.nx * sizeof(double), v, ctx->f.VxMed, nry);
+  else
+    FDISPATCH_T(ctx, kf_vmed_rings, 0, dim3(nr), dim3(64), m.nx * sizeof(double), v, ctx->f.VxMed, nry);
   return 0;
 }
 
@@ -1983,7 +2029,7 @@ static int x_vmed(f3d_hip_ctx* ctx, Scratch* S, int nf, const RF* f, const TSet*
     T.vxmed[q] = t[q].vxmed;
   }
 #define XV(G_, DZ_) \
-  hipLaunchKernelGGL((kf_vmed<G_, DZ_>), dim3(nrings * nf), dim3(64), m.nx * sizeof(double), st, m, T, nrings)
+  hipLaunchKernelGGL((kf_vmed<G_, DZ_, 0>), dim3(nrings * nf), dim3(64), m.nx * sizeof(double), st, m, T, nrings)
   if (m.geom == SPH) {
     if (m.dimz) XV(SPH, 1); else XV(SPH, 0);
   } else {
