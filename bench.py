@@ -49,8 +49,8 @@ def stage_arrays(stage, dz, ad, alpha):
     ns = 6 if dz else 3                  # stress components
     table = {
         "cfl": 1 + nv + ad,                                  # cs or e, v (, rho)
-        "sources_a": (1 + nv + alpha) + (ns + nv),           # rho, v (, cs) -> stresses + q
-        "sources_b": (3 + nv + ns + nv) + (nv + ad),         # rho, cs/e, pot, v, stresses, q -> v_temp (, e)
+        "sources_a": (1 + nv + alpha) + ns,                  # rho, v (, cs) -> stresses
+        "sources_b": (3 + nv + ns) + (nv + ad),              # rho, cs/e, pot, v, stresses -> v_temp (, e); q on the fly
         "substep3": (1 + nv) + 1,                            # e, v_temp -> e
         "sweep_z": (1 + nv + ad) + (1 + nm + ad),            # rho, v_temp (, e) -> rho, momenta (, e)
         "sweep_y": ((1 + nm + ad + 1) if dz else (1 + nv + ad)) + (1 + nm + ad),
@@ -61,7 +61,6 @@ def stage_arrays(stage, dz, ad, alpha):
     return table.get(stage, 0)
 
 
-# kernel behind each stage (names in the rocprofv3 summaries)
 # the kernels each stage's events bracket (names without namespace and template arguments)
 STAGE_KERNEL = {"cfl": ["kf_cfl", "kf_cfl_fluids", "kf_vmed_rings", "kf_cfl_fin"], "sources_a": ["kf_src_a"],
                 "sources_b": ["kf_src_b"], "sweep_z": ["kf_sweep_z"], "sweep_y": ["kf_sweep_y2", "kf_sweep_y3"],

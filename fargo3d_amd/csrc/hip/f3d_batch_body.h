@@ -16,7 +16,7 @@ struct BInst {
   dmesh m;
   real *rho, *cs, *vx, *vy, *vxt, *vyt, *mpx, *mmx, *mpy, *mmy, *rho0, *vx0, *vy0;
   const double* pot;  // FP64 potential
-  real *rhoy, *rhox, *qx, *qy, *e2, *ey, *xm0, *xm1, *xm2, *xm3;
+  real *rhoy, *rhox, *e2, *ey, *xm0, *xm1, *xm2, *xm3;
   double* vxmed;
   int* nshift;
   const real *ydexp0, *ydexp1;
@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(BXF) kb_src_a(const BInst* I, const int* off, 
   const BInst& in = I[ii];
   const int gx = (in.m.nx + BXF - 1) / BXF;
   kd_src_a<CYL, 0>(local2(blockIdx.x - off[ii], gx), in.m, in.rho, in.cs, in.vx, in.vy, nullptr, in.mmx, in.mmy,
-                   nullptr, in.mpx, in.mpy, nullptr, in.qx, in.qy, nullptr, in.vp, in.visc, (real)0);
+                   nullptr, in.mpx, in.mpy, nullptr, in.vp, in.visc, (real)0);
 }
 
 __global__ void __launch_bounds__(BXF) kb_src_b(const BInst* I, const int* off, int n) {
@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(BXF) kb_src_b(const BInst* I, const int* off, 
   const BInst& in = I[ii];
   const int gx = (in.m.nx + BXF - 1) / BXF;
   kd_src_b<CYL, 0>(local2(blockIdx.x - off[ii], gx), in.m, (real)in.clk[0], in.rho, in.cs, in.vx, in.vy, nullptr,
-                   in.pot, in.mmx, in.mmy, nullptr, in.mpx, in.mpy, nullptr, in.qx, in.qy, nullptr, in.vxt, in.vyt,
+                   in.pot, in.mmx, in.mmy, nullptr, in.mpx, in.mpy, nullptr, in.vxt, in.vyt,
                    nullptr, in.e2, in.visc, in.potential, 0, 1, (real)0, in.of, (real)0);
 }
 
@@ -327,7 +327,7 @@ f3d_batch* f3d_batch_create(int n, const char* const* parfiles, const char* cons
     b.rho = f.Density; b.cs = f.Energy; b.vx = f.Vx; b.vy = f.Vy; b.vxt = f.Vx_temp; b.vyt = f.Vy_temp;
     b.mpx = f.Mpx; b.mmx = f.Mmx; b.mpy = f.Mpy; b.mmy = f.Mmy; b.pot = c->f.Pot;
     b.rho0 = f.Density0; b.vx0 = f.Vx0; b.vy0 = f.Vy0;
-    b.rhoy = S->f[FS_RHOY]; b.rhox = S->f[FS_RHOX]; b.qx = S->f[FS_QX]; b.qy = S->f[FS_QY];
+    b.rhoy = S->f[FS_RHOY]; b.rhox = S->f[FS_RHOX];
     b.e2 = S->f[FS_E2]; b.ey = S->f[FS_EY];
     b.xm0 = S->f[FS_XM0]; b.xm1 = S->f[FS_XM0 + 1]; b.xm2 = S->f[FS_XM0 + 2]; b.xm3 = S->f[FS_XM0 + 3];
     b.vxmed = f.VxMed; b.nshift = f.Nshift;

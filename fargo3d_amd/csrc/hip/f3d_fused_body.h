@@ -45,9 +45,10 @@ constexpr int kNgh = 3;  // ghost layers in Y and Z (src/define.h:57-63)
 
 // Scratch of the fused stages, n3 doubles each, allocated on a handle's
 // first fused call and released with the handle: Y- and X-swept density,
-// artificial pressure (3), energy after SubStep2_b and after the Y sweep, the
-// X-swept Y/Z momenta (4) and (3-D) the Z-swept density, six momenta and energy.
-enum { FS_RHOY = 0, FS_RHOX, FS_QX, FS_QY, FS_QZ, FS_E2, FS_EY, FS_XM0, FS_Z0 = FS_XM0 + 4, FS_COUNT = FS_Z0 + 8 };
+// three work arrays (dust diffusion), energy after SubStep2_b and after the Y
+// sweep, the X-swept Y/Z momenta (4) and (3-D) the Z-swept density, six
+// momenta and energy.
+enum { FS_RHOY = 0, FS_RHOX, FS_W0, FS_W1, FS_W2, FS_E2, FS_EY, FS_XM0, FS_Z0 = FS_XM0 + 4, FS_COUNT = FS_Z0 + 8 };
 
 struct Scratch {
   real* f[FS_COUNT];
@@ -469,10 +470,23 @@ __device__ __forceinline__ VxcLoader<G> vxc(const real* vx, const dmesh& m, int 
   return VxcLoader<G>{cell::gl(vx, m, l, i), m.ymed, m.sinzmed, j, k, dom};
 }
 
-// kf_src_a: stresses (visctensor ranges j,k in [1, s-2]) and artificial
-// pressure (SubStep2_a) at every cell the active update reads.
+// The artificial pressure of SubStep2_a (src/substep2_a.c:103-113) at a
+// cell, computed where SubStep2_b reads it, from the same loaders and with
+// the same arithmetic as the array the reference stores: q = art_q(rho,
+// v(+1 along AX) - v).
+template <class LR, class LV, int AX>
+struct ArtQ {
+  LR r;
+  LV v;
+  __device__ __forceinline__ real operator()(int di, int dj, int dk) const {
+    return cell::art_q(r(di, dj, dk), v(di + (AX == 0), dj + (AX == 1), dk + (AX == 2)) - v(di, dj, dk));
+  }
+};
+
+// kf_src_a: stresses (visctensor ranges j,k in [1, s-2]) at every cell the
+// active update reads (launched only with viscosity).
 template <int G, int DZ>
-__device__ __forceinline__ void kd_src_a(const dim3 bid, const dmesh& m, const real* rho, const real* energy, const real* vx, const real* vy, const real* vz, real* txx, real* tyy, real* tzz, real* txz, real* tyx, real* tzy, real* qx, real* qy, real* qz, cell::ViscPar vp, int visc, real domega) {
+__device__ __forceinline__ void kd_src_a(const dim3 bid, const dmesh& m, const real* rho, const real* energy, const real* vx, const real* vy, const real* vz, real* txx, real* tyy, real* tzz, real* txz, real* tyx, real* tzy, cell::ViscPar vp, int visc, real domega) {
   const int i = bid.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int j = 1 + (int)bid.y;       // 1 .. sy-2
@@ -493,25 +507,19 @@ __device__ __forceinline__ void kd_src_a(const dim3 bid, const dmesh& m, const r
       tzy[l] = t.zy;
     }
   }
-  // SubStep2_a (src/substep2_a.c:103-134) on j in [1, sy-2] (a subset of its
-  // range [0, sy-2] that covers every read of the active update)
-  const real r0 = R(0, 0, 0);
-  qx[l] = cell::art_q(r0, VX(1, 0, 0) - VX(0, 0, 0));
-  qy[l] = cell::art_q(r0, VY(0, 1, 0) - VY(0, 0, 0));
-  if (DZ) qz[l] = cell::art_q(r0, VZ(0, 0, 1) - VZ(0, 0, 0));
 }
 template <int G, int DZ>
 __global__ void __launch_bounds__(WG) kf_src_a(dmesh m, const real* __restrict__ rho, const real* __restrict__ energy, const real* __restrict__ vx,
                                                const real* __restrict__ vy, const real* __restrict__ vz, real* __restrict__ txx, real* __restrict__ tyy,
-                                               real* __restrict__ tzz, real* __restrict__ txz, real* __restrict__ tyx, real* __restrict__ tzy, real* __restrict__ qx,
-                                               real* __restrict__ qy, real* __restrict__ qz, cell::ViscPar vp, int visc, real domega) {
-  kd_src_a<G, DZ>(xcd_bid(), m, rho, energy, vx, vy, vz, txx, tyy, tzz, txz, tyx, tzy, qx, qy, qz, vp, visc, domega);
+                                               real* __restrict__ tzz, real* __restrict__ txz, real* __restrict__ tyx, real* __restrict__ tzy,
+                                               cell::ViscPar vp, int visc, real domega) {
+  kd_src_a<G, DZ>(xcd_bid(), m, rho, energy, vx, vy, vz, txx, tyy, tzz, txz, tyx, tzy, vp, visc, domega);
 }
 
 // kf_src_b: the velocity update of SubStep1 + addviscosity + SubStep2_b on
 // active cells, in the order the reference applies them.
 template <int G, int DZ>
-__device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt, const real* rho, const real* energy, const real* vx, const real* vy, const real* vz, const double* pot, const real* txx, const real* tyy, const real* tzz, const real* txz, const real* tyx, const real* tzy, const real* qx, const real* qy, const real* qz, real* vxt, real* vyt, real* vzt, real* e2, int visc, int potential, int dust, int iso, real gm1, real omegaframe, real domega) {
+__device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt, const real* rho, const real* energy, const real* vx, const real* vy, const real* vz, const double* pot, const real* txx, const real* tyy, const real* tzz, const real* txz, const real* tyx, const real* tzy, real* vxt, real* vyt, real* vzt, real* e2, int visc, int potential, int dust, int iso, real gm1, real omegaframe, real domega) {
   const int i = bid.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int j = m.nghy + (int)bid.y;
@@ -533,8 +541,11 @@ __device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt
                               cell::gl(tyx, m, l, i), TZY, ax, ay, az, ax, ay, az);
   }
   real e = iso ? real(0.0) : energy[l];
-  cell::substep2_b<G, DZ>(m, i, j, k, dt, R, VX, VY, VZ, cell::gl(qx, m, l, i), cell::gl(qy, m, l, i),
-                          cell::gl(DZ ? qz : qy, m, l, i), !iso, ax, ay, az, e, ax, ay, az, e);
+  // SubStep2_a's q (src/substep2_a.c:103-134) on the fly
+  const ArtQ<cell::GL, decltype(VX), 0> QX{R, VX};
+  const ArtQ<cell::GL, cell::GL, 1> QY{R, VY};
+  const ArtQ<cell::GL, cell::GL, 2> QZ{R, VZ};
+  cell::substep2_b<G, DZ>(m, i, j, k, dt, R, VX, VY, VZ, QX, QY, QZ, !iso, ax, ay, az, e, ax, ay, az, e);
   vxt[l] = ax;
   vyt[l] = ay;
   if (DZ) vzt[l] = az;
@@ -545,11 +556,10 @@ __global__ void __launch_bounds__(WG) kf_src_b(dmesh m, real dt, const real* __r
                                                const real* __restrict__ vx, const real* __restrict__ vy, const real* __restrict__ vz,
                                                const double* __restrict__ pot, const real* __restrict__ txx, const real* __restrict__ tyy,
                                                const real* __restrict__ tzz, const real* __restrict__ txz, const real* __restrict__ tyx,
-                                               const real* __restrict__ tzy, const real* __restrict__ qx, const real* __restrict__ qy,
-                                               const real* __restrict__ qz, real* __restrict__ vxt, real* __restrict__ vyt, real* __restrict__ vzt, real* __restrict__ e2,
+                                               const real* __restrict__ tzy, real* __restrict__ vxt, real* __restrict__ vyt, real* __restrict__ vzt, real* __restrict__ e2,
                                                int visc, int potential, int dust, int iso, real gm1, real omegaframe,
                                                real domega) {
-  kd_src_b<G, DZ>(xcd_bid(), m, dt, rho, energy, vx, vy, vz, pot, txx, tyy, tzz, txz, tyx, tzy, qx, qy, qz, vxt, vyt, vzt, e2, visc, potential, dust, iso, gm1, omegaframe, domega);
+  kd_src_b<G, DZ>(xcd_bid(), m, dt, rho, energy, vx, vy, vz, pot, txx, tyy, tzz, txz, tyx, tzy, vxt, vyt, vzt, e2, visc, potential, dust, iso, gm1, omegaframe, domega);
 }
 
 // SubStep3 (src/substep3.c:98-110) on active cells: compression work with the
@@ -2498,9 +2508,9 @@ int F3D_API(fused_dust_diffusion)(f3d_hip_ctx* ctx, double dt, int nf, double* c
   const dmesh& m = S->m;
   F3D_REQUIRE(ctx, !m.dimz && m.geom == CYL, "dust diffusion: 2-D cylindrical meshes");
   F3D_REQUIRE(ctx, type[0] == 1, "dust diffusion: fluid 0 must be the gas");
-  real* yczc = S->f[FS_QY];
-  real* yfzc = S->f[FS_QZ];
-  real* temp = S->f[FS_QX];
+  real* yczc = S->f[FS_W1];
+  real* yfzc = S->f[FS_W2];
+  real* temp = S->f[FS_W0];
   hipLaunchKernelGGL((kf_ddiff_coef<CYL, 0>), dim3((m.nx + BXF - 1) / BXF, m.sy), dim3(BXF), 0, ctx->stream, m,
                      reinterpret_cast<const real*>(cs_gas), yczc, yfzc, (int)(ctx->ph.visc == 2),
                      (real)ctx->ph.alpha, (real)ctx->ph.nu);
@@ -2538,10 +2548,12 @@ int F3D_API(fused_sources)(f3d_hip_ctx* ctx, double dt, double domega) {
   const int pot = ctx->ph.potential;
   const real rdt = (real)dt, rdom = (real)domega;
   dim3 ga((m.nx + BXF - 1) / BXF, m.sy - 2, m.dimz ? m.sz - 2 : 1);
-  f3d_stage_begin(ctx, F3D_ST_SRC_A);
-  FDISPATCH(ctx, kf_src_a, ga, dim3(BXF), 0, f.Density, f.Energy, f.Vx, f.Vy, f.Vz, f.Mmx, f.Mmy, f.Mmz, f.Mpx,
-            f.Mpy, f.Mpz, S->f[FS_QX], S->f[FS_QY], S->f[FS_QZ], vp, visc, rdom);
-  f3d_stage_end(ctx, F3D_ST_SRC_A);
+  if (visc) {
+    f3d_stage_begin(ctx, F3D_ST_SRC_A);
+    FDISPATCH(ctx, kf_src_a, ga, dim3(BXF), 0, f.Density, f.Energy, f.Vx, f.Vy, f.Vz, f.Mmx, f.Mmy, f.Mmz, f.Mpx,
+              f.Mpy, f.Mpz, vp, visc, rdom);
+    f3d_stage_end(ctx, F3D_ST_SRC_A);
+  }
   // adiabatic: SubStep3 reads V_temp one row (and plane) past the active
   // range, which the reference computes there with the same formulas
   // (src/substep1_y.c:78-80, src/substep2_b.c:84-86), so those are included
@@ -2549,7 +2561,7 @@ int F3D_API(fused_sources)(f3d_hip_ctx* ctx, double dt, double domega) {
   dim3 gb((m.nx + BXF - 1) / BXF, m.ny + ext, m.dimz ? m.nz + ext : 1);
   f3d_stage_begin(ctx, F3D_ST_SRC_B);
   FDISPATCH(ctx, kf_src_b, gb, dim3(BXF), 0, rdt, f.Density, f.Energy, f.Vx, f.Vy, f.Vz, ctx->f.Pot, f.Mmx, f.Mmy, f.Mmz,
-            f.Mpx, f.Mpy, f.Mpz, S->f[FS_QX], S->f[FS_QY], S->f[FS_QZ], f.Vx_temp, f.Vy_temp, f.Vz_temp,
+            f.Mpx, f.Mpy, f.Mpz, f.Vx_temp, f.Vy_temp, f.Vz_temp,
             S->f[FS_E2], visc, pot, (int)(ctx->ph.fluidtype == 2), iso, (real)(ctx->ph.gamma - 1.0),
             (real)ctx->ph.OMEGAFRAME, rdom);
   f3d_stage_end(ctx, F3D_ST_SRC_B);
