@@ -780,7 +780,7 @@ struct ColY3 {
 // 2-D: momenta + Y sweep; rows [nghy + s*rows_per, ...) of the active range.
 // AD = 1 also transports the energy (e_in -> e_out).
 template <int G, int DZ, int AD, int SPLIT = 0>
-__device__ __forceinline__ void kd_sweep_y2(const dim3 bid, const dmesh& m, real dt, const real* rho_in, const real* vxt, const real* vyt, const real* e_in, real* rho_out, real* q0, real* q1, real* q2, real* q3, real* e_out, int rows_per, real of) {
+__device__ __forceinline__ void kd_sweep_y2(const dim3 bid, const dmesh& m, real dt, const real* __restrict__ rho_in, const real* __restrict__ vxt, const real* __restrict__ vyt, const real* __restrict__ e_in, real* __restrict__ rho_out, real* __restrict__ q0, real* __restrict__ q1, real* __restrict__ q2, real* __restrict__ q3, real* __restrict__ e_out, int rows_per, real of) {
   const int i = bid.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int seg = SPLIT ? (int)bid.y >> 1 : (int)bid.y;
@@ -791,12 +791,22 @@ __device__ __forceinline__ void kd_sweep_y2(const dim3 bid, const dmesh& m, real
   if (SPLIT) column_sweep_part<4 + AD>(col, 1 + ((int)bid.y & 1), r0, r1, dt);
   else column_sweep<4 + AD>(col, r0, r1, dt);
 }
+// 2-D Y sweep of every fluid in one launch: blockIdx.z is the fluid
+struct Y2Tab {
+  const real* rho_in[8];
+  const real* vxt[8];
+  const real* vyt[8];
+  const real* e_in[8];
+  real* rho_out[8];
+  real* q[8][4];
+  real* e_out[8];
+};
 template <int G, int DZ, int AD, int SPLIT>
-__global__ void __launch_bounds__(BXF) kf_sweep_y2(dmesh m, real dt, const real* __restrict__ rho_in, const real* __restrict__ vxt,
-                                                   const real* __restrict__ vyt, const real* __restrict__ e_in, real* __restrict__ rho_out,
-                                                   real* __restrict__ q0, real* __restrict__ q1, real* __restrict__ q2, real* __restrict__ q3, real* __restrict__ e_out,
-                                                   int rows_per, real of) {
-  kd_sweep_y2<G, DZ, AD, SPLIT>(xcd_bid(), m, dt, rho_in, vxt, vyt, e_in, rho_out, q0, q1, q2, q3, e_out, rows_per, of);
+__global__ void __launch_bounds__(BXF) kf_sweep_y2(dmesh m, real dt, Y2Tab T, int rows_per, real of) {
+  const dim3 b = xcd_bid();
+  const int fl = (int)b.z;
+  kd_sweep_y2<G, DZ, AD, SPLIT>(b, m, dt, T.rho_in[fl], T.vxt[fl], T.vyt[fl], T.e_in[fl], T.rho_out[fl], T.q[fl][0],
+                                T.q[fl][1], T.q[fl][2], T.q[fl][3], T.e_out[fl], rows_per, of);
 }
 
 // 3-D: Y sweep of the Z-swept fields
@@ -1415,7 +1425,7 @@ __global__ void __launch_bounds__(T) kf_sweep_x_tile(dmesh m, double dt64, const
 
 // NewVelocity_y/z on active faces and Stockholm damping on active cells.
 template <int G, int DZ>
-__device__ __forceinline__ void kd_finish(const dim3 bid, const dmesh& m, real dt, const real* rho_x, const real* mmy, const real* mpy, const real* mmz, const real* mpz, real* rho, real* vx, real* vy, real* vz, const real* rho0, const real* vx0, const real* vy0, const real* vz0, cell::DampPar dp, int stockholm) {
+__device__ __forceinline__ void kd_finish(const dim3 bid, const dmesh& m, real dt, const real* __restrict__ rho_x, const real* __restrict__ mmy, const real* __restrict__ mpy, const real* __restrict__ mmz, const real* __restrict__ mpz, real* __restrict__ rho, real* __restrict__ vx, real* __restrict__ vy, real* __restrict__ vz, const real* __restrict__ rho0, const real* __restrict__ vx0, const real* __restrict__ vy0, const real* __restrict__ vz0, cell::DampPar dp, int stockholm) {
   const int i = bid.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int j = m.nghy + (int)bid.y;
@@ -1434,13 +1444,28 @@ __device__ __forceinline__ void kd_finish(const dim3 bid, const dmesh& m, real d
   rho[l] = r;
   if (stockholm) cell::stockholm<G, DZ>(m, j, k, l, dt, rho, vx, vy, vz, rho0, vx0, vy0, vz0, dp);
 }
+// every fluid in one launch: blockIdx.z = fluid * (planes) + plane
+struct FinTab {
+  const real* rho_x[8];
+  const real* mmy[8];
+  const real* mpy[8];
+  const real* mmz[8];
+  const real* mpz[8];
+  real* rho[8];
+  real* vx[8];
+  real* vy[8];
+  real* vz[8];
+  const real* rho0[8];
+  const real* vx0[8];
+  const real* vy0[8];
+  const real* vz0[8];
+};
 template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_finish(dmesh m, real dt, const real* __restrict__ rho_x, const real* __restrict__ mmy,
-                                                const real* __restrict__ mpy, const real* __restrict__ mmz, const real* __restrict__ mpz,
-                                                real* __restrict__ rho, real* __restrict__ vx, real* __restrict__ vy, real* __restrict__ vz, const real* __restrict__ rho0,
-                                                const real* __restrict__ vx0, const real* __restrict__ vy0, const real* __restrict__ vz0,
-                                                cell::DampPar dp, int stockholm) {
-  kd_finish<G, DZ>(blockIdx, m, dt, rho_x, mmy, mpy, mmz, mpz, rho, vx, vy, vz, rho0, vx0, vy0, vz0, dp, stockholm);
+__global__ void __launch_bounds__(WG) kf_finish(dmesh m, real dt, FinTab T, int nzb, cell::DampPar dp, int stockholm) {
+  const int fl = (int)blockIdx.z / nzb;
+  const dim3 b(blockIdx.x, blockIdx.y, blockIdx.z % nzb);
+  kd_finish<G, DZ>(b, m, dt, T.rho_x[fl], T.mmy[fl], T.mpy[fl], T.mmz[fl], T.mpz[fl], T.rho[fl], T.vx[fl], T.vy[fl],
+                   T.vz[fl], T.rho0[fl], T.vx0[fl], T.vy0[fl], T.vz0[fl], dp, stockholm);
 }
 
 // ------------------------------------------------------------------ ghost fill, potential, damping references
@@ -1965,9 +1990,9 @@ static int tset_of(f3d_hip_ctx* ctx, Scratch* S, const RF& f, int fluid, int nf,
 }
 
 // NewVelocity_y/z, Stockholm damping and the end-of-step ghost fill
-static int finish_step(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step, const RF& f, const TSet& t) {
+static int finish_step(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step, int nf, const RF* f,
+                       const TSet* t) {
   const dmesh& m = S->m;
-  real* const* XM = t.xm;
   cell::DampPar dp;
   dp.Y_inf = (real)ctx->ph.Y_inf; dp.Y_sup = (real)ctx->ph.Y_sup;
   dp.Z_inf = (real)ctx->ph.Z_inf; dp.Z_sup = (real)ctx->ph.Z_sup;
@@ -1975,28 +2000,38 @@ static int finish_step(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step,
   dp.z_min = (real)ctx->ph.zmin_par; dp.z_max = (real)ctx->ph.zmax_par;
   dp.ds = (real)ctx->ph.taudamp; dp.of = (real)ctx->ph.OMEGAFRAME; dp.of0 = (real)ctx->ph.OMEGAFRAME0;
   dp.periodic_z = ctx->ph.periodicz;
-  dim3 gf((m.nx + BXF - 1) / BXF, m.ny, m.dimz ? m.nz : 1);
+  FinTab T;
+  memset(&T, 0, sizeof(T));
+  for (int q = 0; q < nf; q++) {
+    real* const* XM = t[q].xm;
+    T.rho_x[q] = t[q].rhox; T.mmy[q] = XM[1]; T.mpy[q] = XM[0]; T.mmz[q] = XM[3]; T.mpz[q] = XM[2];
+    T.rho[q] = f[q].Density; T.vx[q] = f[q].Vx; T.vy[q] = f[q].Vy; T.vz[q] = f[q].Vz;
+    T.rho0[q] = f[q].Density0; T.vx0[q] = f[q].Vx0; T.vy0[q] = f[q].Vy0; T.vz0[q] = f[q].Vz0;
+  }
+  const int nzb = m.dimz ? m.nz : 1;
+  dim3 gf((m.nx + BXF - 1) / BXF, m.ny, nzb * nf);
   f3d_stage_begin(ctx, F3D_ST_FINISH);
-  FDISPATCH(ctx, kf_finish, gf, dim3(BXF), 0, (real)dt, t.rhox, XM[1], XM[0], XM[3], XM[2], f.Density, f.Vx,
-            f.Vy, f.Vz, f.Density0, f.Vx0, f.Vy0, f.Vz0, dp, ctx->ph.stockholm);
+  FDISPATCH(ctx, kf_finish, gf, dim3(BXF), 0, (real)dt, T, nzb, dp, ctx->ph.stockholm);
   f3d_stage_end(ctx, F3D_ST_FINISH);
   if (end_of_step) {
     // end-of-step ghost fill (src/main.c:431)
     f3d_stage_begin(ctx, F3D_ST_BC_END);
-    if (fused_boundaries(ctx, S, f.Density, f.Vx, f.Vy, f.Vz, f.Energy)) return -1;
+    for (int q = 0; q < nf; q++)
+      if (fused_boundaries(ctx, S, f[q].Density, f[q].Vx, f[q].Vy, f[q].Vz, f[q].Energy)) return -1;
     f3d_stage_end(ctx, F3D_ST_BC_END);
   }
   return 0;
 }
 
-// momenta and the Z (3-D) and Y sweeps of one fluid into its transport set
-static int yz_sweeps(f3d_hip_ctx* ctx, Scratch* S, double dt, const RF& f, const TSet& t) {
+// momenta and the Z (3-D) and Y sweeps of every fluid into its transport set
+// (2-D: one Y-sweep launch over the fluids; 3-D: one pair of launches per fluid)
+static int yz_sweeps(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const RF* fl, const TSet* tl) {
   const dmesh& m = S->m;
   const real of = (real)ctx->ph.OMEGAFRAME;
   const real rdt = (real)dt;
   const int ry = g_rows_y, rz = g_rows_z;
   const long waves = (long)((m.nx + BXF - 1) / BXF) * (m.dimz ? (long)m.nz * ((m.ny + ry - 1) / ry)
-                                                               : (long)((m.ny + ry - 1) / ry));
+                                                               : (long)nf * ((m.ny + ry - 1) / ry));
   const int split = g_colsplit >= 0 ? g_colsplit : (waves < 1024 ? 1 : 0);
   const int sp = split ? 2 : 1;  // threads per column
 #define FDISPATCH_ADS(ctx_, KERN, grid, ...)                                             \
@@ -2005,23 +2040,34 @@ static int yz_sweeps(f3d_hip_ctx* ctx, Scratch* S, double dt, const RF& f, const
     else FDISPATCH_AD_T(ctx_, KERN, 0, grid, dim3(BXF), 0, __VA_ARGS__);               \
   } while (0)
   if (m.dimz) {
-    // eight separate buffers: six Z-swept momenta, the Z-swept density and energy
-    real* const* Z = &S->f[FS_Z0];
-    dim3 gz((m.nx + BXF - 1) / BXF, (m.sy - 2) * sp, (m.nz + rz - 1) / rz);
-    f3d_stage_begin(ctx, F3D_ST_SWEEP_Z);
-    FDISPATCH_ADS(ctx, kf_sweep_z, gz, rdt, f.Density, f.Vx_temp, f.Vy_temp, f.Vz_temp, f.Energy, Z[6], Z[0], Z[1],
-                  Z[2], Z[3], Z[4], Z[5], Z[7], rz, of);
-    f3d_stage_end(ctx, F3D_ST_SWEEP_Z);
-    dim3 gy((m.nx + BXF - 1) / BXF, ((m.ny + ry - 1) / ry) * sp, m.nz);
-    f3d_stage_begin(ctx, F3D_ST_SWEEP_Y);
-    FDISPATCH_ADS(ctx, kf_sweep_y3, gy, rdt, Z[6], Z[0], Z[1], Z[2], Z[3], Z[4], Z[5], Z[7], f.Vy_temp, t.rhoy,
-                  t.m[0], t.m[1], t.m[2], t.m[3], t.m[4], t.m[5], t.ey, ry);
-    f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
+    for (int q = 0; q < nf; q++) {
+      const RF& f = fl[q];
+      const TSet& t = tl[q];
+      // eight separate buffers: six Z-swept momenta, the Z-swept density and energy
+      real* const* Z = &S->f[FS_Z0];
+      dim3 gz((m.nx + BXF - 1) / BXF, (m.sy - 2) * sp, (m.nz + rz - 1) / rz);
+      f3d_stage_begin(ctx, F3D_ST_SWEEP_Z);
+      FDISPATCH_ADS(ctx, kf_sweep_z, gz, rdt, f.Density, f.Vx_temp, f.Vy_temp, f.Vz_temp, f.Energy, Z[6], Z[0], Z[1],
+                    Z[2], Z[3], Z[4], Z[5], Z[7], rz, of);
+      f3d_stage_end(ctx, F3D_ST_SWEEP_Z);
+      dim3 gy((m.nx + BXF - 1) / BXF, ((m.ny + ry - 1) / ry) * sp, m.nz);
+      f3d_stage_begin(ctx, F3D_ST_SWEEP_Y);
+      FDISPATCH_ADS(ctx, kf_sweep_y3, gy, rdt, Z[6], Z[0], Z[1], Z[2], Z[3], Z[4], Z[5], Z[7], f.Vy_temp, t.rhoy,
+                    t.m[0], t.m[1], t.m[2], t.m[3], t.m[4], t.m[5], t.ey, ry);
+      f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
+    }
   } else {
-    dim3 gy((m.nx + BXF - 1) / BXF, ((m.ny + ry - 1) / ry) * sp, 1);
+    Y2Tab T;
+    memset(&T, 0, sizeof(T));
+    for (int q = 0; q < nf; q++) {
+      T.rho_in[q] = fl[q].Density; T.vxt[q] = fl[q].Vx_temp; T.vyt[q] = fl[q].Vy_temp; T.e_in[q] = fl[q].Energy;
+      T.rho_out[q] = tl[q].rhoy;
+      for (int c = 0; c < 4; c++) T.q[q][c] = tl[q].m[c];
+      T.e_out[q] = tl[q].ey;
+    }
+    dim3 gy((m.nx + BXF - 1) / BXF, ((m.ny + ry - 1) / ry) * sp, nf);
     f3d_stage_begin(ctx, F3D_ST_SWEEP_Y);
-    FDISPATCH_ADS(ctx, kf_sweep_y2, gy, rdt, f.Density, f.Vx_temp, f.Vy_temp, f.Energy, t.rhoy, t.m[0], t.m[1],
-                  t.m[2], t.m[3], t.ey, ry, of);
+    FDISPATCH_ADS(ctx, kf_sweep_y2, gy, rdt, T, ry, of);
     f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
   }
 #undef FDISPATCH_ADS
@@ -2248,8 +2294,7 @@ static int transport_fluids(f3d_hip_ctx* ctx, double dt, int nf, const f3d_field
     if (x_vmed(ctx, S, nf, f, t, ctx->side)) return -1;
     F3D_CHECK(ctx, hipEventRecord(ctx->side_done, ctx->side));
   }
-  for (int k = 0; k < nf; k++)
-    if (yz_sweeps(ctx, S, dt, f[k], t[k])) return -1;
+  if (yz_sweeps(ctx, S, dt, nf, f, t)) return -1;
   if (vpre == 1) F3D_CHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->side_done, 0));
   if (vpre == 2 && x_vmed(ctx, S, nf, f, t, ctx->stream)) return -1;
   if (!tiles) {
@@ -2258,8 +2303,7 @@ static int transport_fluids(f3d_hip_ctx* ctx, double dt, int nf, const f3d_field
     for (int k = 0; k < nf; k++)
       if (x_sweep_tiled(ctx, S, dt, f[k], t[k])) return -1;
   }
-  for (int k = 0; k < nf; k++)
-    if (finish_step(ctx, S, dt, end_of_step, f[k], t[k])) return -1;
+  if (finish_step(ctx, S, dt, end_of_step, nf, f, t)) return -1;
   return 0;
 }
 
