@@ -58,6 +58,7 @@ struct Scratch {
   real* fset[8][13];           // transport arrays of fluids 1..7 (multifluid transport)
   double* vxmed_f[8];          // VxMed and Nshift of the fluids whose values the
   int* nshift_f[8];            // reference overwrites (all but the last)
+  real* ddtemp[8];             // dust diffusion: the new density of fluid f before its copy back
   dmesh m;                     // the build's mesh view
   mreal* mesh_r;               // FP32 build: converted mesh arrays
   real* bctab_r;               // FP32 build: converted boundary tables
@@ -77,6 +78,7 @@ void scratch_free(void* p) {
       if (S->fset[f][a]) (void)hipFree(S->fset[f][a]);
     if (S->vxmed_f[f]) (void)hipFree(S->vxmed_f[f]);
     if (S->nshift_f[f]) (void)hipFree(S->nshift_f[f]);
+    if (S->ddtemp[f]) (void)hipFree(S->ddtemp[f]);
   }
   if (S->mesh_r) (void)hipFree(S->mesh_r);
   if (S->bctab_r) (void)hipFree(S->bctab_r);
@@ -1664,11 +1666,12 @@ __global__ void __launch_bounds__(BXF) kf_ddiff_coef(dmesh m, const real* cs, re
 // DustDiffusion_Core (src/dust_diffusion_core.c:10-192), CYLINDRICAL 2-D, on
 // rows j in [1, sy-2]: the dust fluid's updated density into temp
 template <int G, int DZ>
-__global__ void __launch_bounds__(BXF) kf_ddiff_core(dmesh m, real dt, const real* rhod, const real* rhog,
-                                                     const real* yczc, const real* yfzc, real* temp) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void ddiff_core_cell(const dmesh& m, real dt, int bx, int by, const real* __restrict__ rhod,
+                                                const real* __restrict__ rhog, const real* __restrict__ yczc,
+                                                const real* __restrict__ yfzc, real* __restrict__ temp) {
+  const int i = bx * (int)blockDim.x + (int)threadIdx.x;
   if (i >= m.nx) return;
-  const int j = 1 + (int)blockIdx.y;
+  const int j = 1 + by;
   const int ll = i + j * m.pitch;
   const int llxm = lxm_of(ll, i, m.nx), llxp = lxp_of(ll, i, m.nx), llyp = ll + m.pitch, llym = ll - m.pitch;
   real update = real(0.0);
@@ -1692,16 +1695,33 @@ __global__ void __launch_bounds__(BXF) kf_ddiff_core(dmesh m, real dt, const rea
   temp[ll] = rhod[ll] + dt * update;
 }
 
-// Density-only Y ghost fill (FillGhosts(DENS), src/dust_diffusion_main.c:25)
-__global__ void kf_bound_y_dens(dmesh m, real* dens, cell::BcPar bp0, cell::BcPar bp1) {
+// The dust fluids' cores in one launch (blockIdx.z = dust fluid) and the copy
+// back with the density ghost fill in one more: Temp -> Density over the
+// whole field, then FillGhosts(DENS) (src/dust_diffusion_main.c:20-25), give
+// active rows = Temp and ghost rows = the boundary formula applied to Temp's
+// active rows, which kf_ddiff_store writes directly.
+struct DdTab {
+  real* rhod[8];
+  real* temp[8];
+};
+template <int G, int DZ>
+__global__ void __launch_bounds__(BXF) kf_ddiff_core_fluids(dmesh m, real dt, DdTab T, const real* rhog,
+                                                            const real* yczc, const real* yfzc) {
+  const int fl = (int)blockIdx.z;
+  ddiff_core_cell<G, DZ>(m, dt, (int)blockIdx.x, (int)blockIdx.y, T.rhod[fl], rhog, yczc, yfzc, T.temp[fl]);
+}
+__global__ void kf_ddiff_store(dmesh m, DdTab T, cell::BcPar bp0, cell::BcPar bp1) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
-  const int side = (int)blockIdx.y >= m.nghy;
-  const int mm = (int)blockIdx.y - side * m.nghy;
-  const int lgh = side ? i + (m.ny + m.nghy + mm) * m.pitch : i + mm * m.pitch;
-  const int lact = side ? i + (m.ny + m.nghy - 1 - mm) * m.pitch : i + (2 * m.nghy - mm - 1) * m.pitch;
-  dens[lgh] = dens[lact] * (side ? bp1 : bp0).dens[mm];
+  const int j = (int)blockIdx.y, fl = (int)blockIdx.z;
+  const real* __restrict__ tp = T.temp[fl];
+  real* __restrict__ rd = T.rhod[fl];
+  const int top = m.ny + m.nghy;
+  if (j < m.nghy) rd[i + j * m.pitch] = tp[i + (2 * m.nghy - j - 1) * m.pitch] * bp0.dens[j];
+  else if (j >= top) rd[i + j * m.pitch] = tp[i + (2 * top - 1 - j) * m.pitch] * bp1.dens[j - top];
+  else rd[i + j * m.pitch] = tp[i + j * m.pitch];
 }
+
 
 // ------------------------------------------------------------------ C ABI
 
@@ -2557,24 +2577,39 @@ int F3D_API(fused_dust_diffusion)(f3d_hip_ctx* ctx, double dt, int nf, double* c
   F3D_REQUIRE(ctx, type[0] == 1, "dust diffusion: fluid 0 must be the gas");
   real* yczc = S->f[FS_W1];
   real* yfzc = S->f[FS_W2];
-  real* temp = S->f[FS_W0];
   hipLaunchKernelGGL((kf_ddiff_coef<CYL, 0>), dim3((m.nx + BXF - 1) / BXF, m.sy), dim3(BXF), 0, ctx->stream, m,
                      reinterpret_cast<const real*>(cs_gas), yczc, yfzc, (int)(ctx->ph.visc == 2),
                      (real)ctx->ph.alpha, (real)ctx->ph.nu);
   F3D_LAUNCH_CHECK(ctx);
   const cell::BcPar b0 = bc_par_f(ctx, 0), b1 = bc_par_f(ctx, 1);
+  // every dust fluid at once, each with a Temp of its own
+  F3D_REQUIRE(ctx, nf >= 1 && nf <= 8, "dust diffusion: 1 to 8 fluids");
+  DdTab T;
+  memset(&T, 0, sizeof(T));
+  int nd = 0;
   for (int f = 1; f < nf; f++) {
     if (type[f] != 2) continue;
-    real* rd = reinterpret_cast<real*>(rho[f]);
-    hipLaunchKernelGGL((kf_ddiff_core<CYL, 0>), dim3((m.nx + BXF - 1) / BXF, m.sy - 2), dim3(BXF), 0, ctx->stream, m,
-                       (real)dt, rd, reinterpret_cast<const real*>(rho[0]), yczc, yfzc, temp);
-    F3D_LAUNCH_CHECK(ctx);
-    F3D_CHECK(ctx, hipMemcpyAsync(rd, temp, ctx->n3 * sizeof(real), hipMemcpyDeviceToDevice, ctx->stream));
-    if (!ctx->ph.periodicy) {
-      hipLaunchKernelGGL(kf_bound_y_dens, dim3((m.nx + BXF - 1) / BXF, 2 * m.nghy), dim3(BXF), 0, ctx->stream, m, rd,
-                         b0, b1);
-      F3D_LAUNCH_CHECK(ctx);
+    if (!S->ddtemp[f] && hipMalloc(&S->ddtemp[f], ctx->n3 * sizeof(real)) != hipSuccess) {
+      S->ddtemp[f] = nullptr;
+      F3D_REQUIRE(ctx, false, "dust diffusion: scratch allocation failed");
     }
+    T.rhod[nd] = reinterpret_cast<real*>(rho[f]);
+    T.temp[nd] = S->ddtemp[f];
+    nd++;
+  }
+  if (!nd) return 0;
+  hipLaunchKernelGGL((kf_ddiff_core_fluids<CYL, 0>), dim3((m.nx + BXF - 1) / BXF, m.sy - 2, nd), dim3(BXF), 0,
+                     ctx->stream, m, (real)dt, T, reinterpret_cast<const real*>(rho[0]), yczc, yfzc);
+  F3D_LAUNCH_CHECK(ctx);
+  if (!ctx->ph.periodicy) {
+    // Temp -> Density and FillGhosts(DENS) in one pass
+    hipLaunchKernelGGL(kf_ddiff_store, dim3((m.nx + BXF - 1) / BXF, m.sy, nd), dim3(BXF), 0, ctx->stream, m, T, b0,
+                       b1);
+    F3D_LAUNCH_CHECK(ctx);
+  } else {
+    for (int q = 0; q < nd; q++)
+      F3D_CHECK(ctx, hipMemcpyAsync(T.rhod[q], T.temp[q], ctx->n3 * sizeof(real), hipMemcpyDeviceToDevice,
+                                    ctx->stream));
   }
   return 0;
 }
