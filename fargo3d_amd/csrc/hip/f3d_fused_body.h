@@ -1492,6 +1492,19 @@ __global__ void kf_bound_y(dmesh m, int side0, real* dens, real* vx, real* vy, r
                            cell::BcPar bp1, const real* dexp0, const real* dexp1) {
   kd_bound_y(blockIdx, m, side0, dens, vx, vy, vz, e, bp0, bp1, dexp0, dexp1);
 }
+// 2-D: the Y fill of every fluid in one launch (blockIdx.z = fluid), both sides
+struct BTab {
+  real* dens[8];
+  real* vx[8];
+  real* vy[8];
+  real* e[8];
+};
+__global__ void kf_bound_y_fluids(dmesh m, BTab T, cell::BcPar bp0, cell::BcPar bp1, const real* dexp0,
+                                  const real* dexp1) {
+  const int fl = (int)blockIdx.z;
+  kd_bound_y(dim3(blockIdx.x, blockIdx.y, 0), m, 0, T.dens[fl], T.vx[fl], T.vy[fl], T.vy[fl], T.e[fl], bp0, bp1,
+             dexp0, dexp1);
+}
 
 __global__ void kf_bound_z(dmesh m, int side0, real* dens, real* vx, real* vy, real* vz, real* e, cell::BcPar bp0,
                            cell::BcPar bp1, const real* zdens0, const real* zdens1) {
@@ -2036,8 +2049,20 @@ static int finish_step(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step,
   if (end_of_step) {
     // end-of-step ghost fill (src/main.c:431)
     f3d_stage_begin(ctx, F3D_ST_BC_END);
-    for (int q = 0; q < nf; q++)
-      if (fused_boundaries(ctx, S, f[q].Density, f[q].Vx, f[q].Vy, f[q].Vz, f[q].Energy)) return -1;
+    if (nf > 1 && !m.dimz && !ctx->ph.periodicy && m.ny > m.nghy) {
+      // 2-D: every fluid's Y fill in one launch (the same boundary factors)
+      BTab B;
+      memset(&B, 0, sizeof(B));
+      for (int q = 0; q < nf; q++) {
+        B.dens[q] = f[q].Density; B.vx[q] = f[q].Vx; B.vy[q] = f[q].Vy; B.e[q] = f[q].Energy;
+      }
+      hipLaunchKernelGGL(kf_bound_y_fluids, dim3((m.nx + 63) / 64, 2, nf), dim3(64), 0, ctx->stream, m, B,
+                         bc_par_f(ctx, 0), bc_par_f(ctx, 1), S->ymin_dexp, S->ymax_dexp);
+      F3D_LAUNCH_CHECK(ctx);
+    } else {
+      for (int q = 0; q < nf; q++)
+        if (fused_boundaries(ctx, S, f[q].Density, f[q].Vx, f[q].Vy, f[q].Vz, f[q].Energy)) return -1;
+    }
     f3d_stage_end(ctx, F3D_ST_BC_END);
   }
   return 0;
