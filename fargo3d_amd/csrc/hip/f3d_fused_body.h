@@ -521,7 +521,7 @@ __global__ void __launch_bounds__(WG) kf_src_a(dmesh m, const real* __restrict__
 // kf_src_b: the velocity update of SubStep1 + addviscosity + SubStep2_b on
 // active cells, in the order the reference applies them.
 template <int G, int DZ>
-__device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt, const real* rho, const real* energy, const real* vx, const real* vy, const real* vz, const double* pot, const real* txx, const real* tyy, const real* tzz, const real* txz, const real* tyx, const real* tzy, real* vxt, real* vyt, real* vzt, real* e2, int visc, int potential, int dust, int iso, real gm1, real omegaframe, real domega) {
+__device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt, const real* __restrict__ rho, const real* __restrict__ energy, const real* __restrict__ vx, const real* __restrict__ vy, const real* __restrict__ vz, const double* __restrict__ pot, const real* __restrict__ txx, const real* __restrict__ tyy, const real* __restrict__ tzz, const real* __restrict__ txz, const real* __restrict__ tyx, const real* __restrict__ tzy, real* __restrict__ vxt, real* __restrict__ vyt, real* __restrict__ vzt, real* __restrict__ e2, int visc, int potential, int dust, int iso, real gm1, real omegaframe, real domega) {
   const int i = bid.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int j = m.nghy + (int)bid.y;
@@ -562,6 +562,34 @@ __global__ void __launch_bounds__(WG) kf_src_b(dmesh m, real dt, const real* __r
                                                int visc, int potential, int dust, int iso, real gm1, real omegaframe,
                                                real domega) {
   kd_src_b<G, DZ>(xcd_bid(), m, dt, rho, energy, vx, vy, vz, pot, txx, tyy, tzz, txz, tyx, tzy, vxt, vyt, vzt, e2, visc, potential, dust, iso, gm1, omegaframe, domega);
+}
+
+// kf_src_b of every fluid in one launch (isothermal runs): blockIdx.z =
+// fluid * (planes) + plane; each fluid has its own viscosity and dust flags
+struct SrcTab {
+  const real* rho[8];
+  const real* cs[8];
+  const real* vx[8];
+  const real* vy[8];
+  const real* vz[8];
+  real* vxt[8];
+  real* vyt[8];
+  real* vzt[8];
+  int visc[8];
+  int dust[8];
+};
+template <int G, int DZ>
+__global__ void __launch_bounds__(WG) kf_src_b_fluids(dmesh m, real dt, SrcTab T, int nzb, const double* __restrict__ pot,
+                                                      const real* __restrict__ txx, const real* __restrict__ tyy,
+                                                      const real* __restrict__ tzz, const real* __restrict__ txz,
+                                                      const real* __restrict__ tyx, const real* __restrict__ tzy,
+                                                      int potential, real gm1, real omegaframe, real domega) {
+  const dim3 b0 = xcd_bid();
+  const int fl = (int)b0.z / nzb;
+  const dim3 b(b0.x, b0.y, b0.z % nzb);
+  kd_src_b<G, DZ>(b, m, dt, T.rho[fl], T.cs[fl], T.vx[fl], T.vy[fl], T.vz[fl], pot, txx, tyy, tzz, txz, tyx, tzy,
+                  T.vxt[fl], T.vyt[fl], T.vzt[fl], nullptr, T.visc[fl], potential, T.dust[fl], 1, gm1, omegaframe,
+                  domega);
 }
 
 // SubStep3 (src/substep3.c:98-110) on active cells: compression work with the
@@ -2683,6 +2711,75 @@ int F3D_API(fused_sources)(f3d_hip_ctx* ctx, double dt, double domega) {
   // mid-step ghost fill (src/algogas.c:116-118) applied to V_temp directly
   f3d_stage_begin(ctx, F3D_ST_BC_MID);
   if (fused_boundaries(ctx, S, f.Density, f.Vx_temp, f.Vy_temp, f.Vz_temp, f.Energy)) return -1;
+  f3d_stage_end(ctx, F3D_ST_BC_MID);
+  return 0;
+}
+
+// Sources of every fluid (isothermal runs, at most one viscous fluid): the
+// viscous fluid's stresses, one kf_src_b launch over all fluids, the
+// mid-step ghost fill (2-D: one launch over all fluids).
+int F3D_API(fused_sources_fluids)(f3d_hip_ctx* ctx, double dt, double domega, int nf, const f3d_fields* fl,
+                                  const int* type) {
+  Scratch* S;
+  F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
+  F3D_REQUIRE(ctx, nf >= 1 && nf <= 8 && fl && type, "sources: 1 to 8 fluids");
+  F3D_REQUIRE(ctx, ctx->ph.eos != 2, "sources of several fluids in one launch: isothermal runs");
+  const dmesh& m = S->m;
+  RF f[8];
+  int nvisc = 0, vq = -1;
+  for (int q = 0; q < nf; q++) {
+    f[q] = rfields_of(fl[q], ctx->f);
+    if (ctx->ph.visc != 0 && type[q] == 1) {
+      nvisc++;
+      vq = q;
+    }
+  }
+  F3D_REQUIRE(ctx, nvisc <= 1, "sources of several fluids in one launch: one viscous fluid at most");
+  cell::ViscPar vp;
+  vp.model = ctx->ph.visc;
+  vp.nu = (real)ctx->ph.nu;
+  vp.alpha = (real)ctx->ph.alpha;
+  vp.gamma = (real)ctx->ph.gamma;
+  vp.iso = 1;
+  const real rdt = (real)dt, rdom = (real)domega;
+  // the stresses land in the shared stress arrays, which only the viscous fluid reads
+  const RF& sh = f[vq < 0 ? 0 : vq];
+  if (vq >= 0) {
+    dim3 ga((m.nx + BXF - 1) / BXF, m.sy - 2, m.dimz ? m.sz - 2 : 1);
+    f3d_stage_begin(ctx, F3D_ST_SRC_A);
+    FDISPATCH(ctx, kf_src_a, ga, dim3(BXF), 0, sh.Density, sh.Energy, sh.Vx, sh.Vy, sh.Vz, sh.Mmx, sh.Mmy, sh.Mmz,
+              sh.Mpx, sh.Mpy, sh.Mpz, vp, 1, rdom);
+    f3d_stage_end(ctx, F3D_ST_SRC_A);
+  }
+  SrcTab T;
+  memset(&T, 0, sizeof(T));
+  for (int q = 0; q < nf; q++) {
+    T.rho[q] = f[q].Density; T.cs[q] = f[q].Energy; T.vx[q] = f[q].Vx; T.vy[q] = f[q].Vy; T.vz[q] = f[q].Vz;
+    T.vxt[q] = f[q].Vx_temp; T.vyt[q] = f[q].Vy_temp; T.vzt[q] = f[q].Vz_temp;
+    T.visc[q] = (q == vq);
+    T.dust[q] = (type[q] == 2);
+  }
+  const int nzb = m.dimz ? m.nz : 1;
+  dim3 gb((m.nx + BXF - 1) / BXF, m.ny, nzb * nf);
+  f3d_stage_begin(ctx, F3D_ST_SRC_B);
+  FDISPATCH(ctx, kf_src_b_fluids, gb, dim3(BXF), 0, rdt, T, nzb, ctx->f.Pot, sh.Mmx, sh.Mmy, sh.Mmz, sh.Mpx, sh.Mpy,
+            sh.Mpz, ctx->ph.potential, (real)(ctx->ph.gamma - 1.0), (real)ctx->ph.OMEGAFRAME, rdom);
+  f3d_stage_end(ctx, F3D_ST_SRC_B);
+  // mid-step ghost fill (src/algogas.c:116-118) on every fluid's V_temp
+  f3d_stage_begin(ctx, F3D_ST_BC_MID);
+  if (nf > 1 && !m.dimz && !ctx->ph.periodicy && m.ny > m.nghy) {
+    BTab B;
+    memset(&B, 0, sizeof(B));
+    for (int q = 0; q < nf; q++) {
+      B.dens[q] = f[q].Density; B.vx[q] = f[q].Vx_temp; B.vy[q] = f[q].Vy_temp; B.e[q] = f[q].Energy;
+    }
+    hipLaunchKernelGGL(kf_bound_y_fluids, dim3((m.nx + 63) / 64, 2, nf), dim3(64), 0, ctx->stream, m, B,
+                       bc_par_f(ctx, 0), bc_par_f(ctx, 1), S->ymin_dexp, S->ymax_dexp);
+    F3D_LAUNCH_CHECK(ctx);
+  } else {
+    for (int q = 0; q < nf; q++)
+      if (fused_boundaries(ctx, S, f[q].Density, f[q].Vx_temp, f[q].Vy_temp, f[q].Vz_temp, f[q].Energy)) return -1;
+  }
   f3d_stage_end(ctx, F3D_ST_BC_MID);
   return 0;
 }

@@ -710,8 +710,27 @@ int f3d_sim_step(f3d_sim *S, double *dt_out) {
       CALL(S, f3d_ComputeTotalDensity(c));
     }
   }
-  /* MULTIFLUID(Sources(dt)) (src/main.c:410) */
-  for (int k = 0; k < nfl; k++) {
+  /* MULTIFLUID(Sources(dt)) (src/main.c:410): isothermal runs with at most
+     one viscous fluid take every fluid in one pass */
+  int nvisc = 0;
+  for (int k = 0; k < nfl; k++) nvisc += (S->p.visc != 0 && S->ftype[k] == F3D_GAS);
+  const int srcs_all = nfl > 1 && S->arch.fused_sources && S->p.eos != F3D_ADIABATIC && nvisc <= 1;
+  if (srcs_all) {
+    select_fluid(S, 0);
+    if (S->p.potential && compute_potential(S, dt)) return -1;
+    if (f3d_sim_push_phys(S)) return -1;
+    f3d_fields all[F3D_MAX_FLUIDS];
+    for (int k = 0; k < nfl; k++) {
+      all[k] = S->f;
+#define SELS(m) all[k].m = S->fl[k].m;
+      F3D_FLUID_MEMBERS(SELS)
+#undef SELS
+    }
+    const double dom = (S->p.potential && S->p.corotating) ? S->clk.Domega : 0.0;
+    CALL(S, S->prec32 ? f3d32_fused_sources_fluids(c, dt, dom, nfl, all, S->ftype)
+                      : f3d_fused_sources_fluids(c, dt, dom, nfl, all, S->ftype));
+  }
+  for (int k = 0; k < nfl && !srcs_all; k++) {
     select_fluid(S, k);
     if (S->arch.fused_sources) {
       if (S->p.potential) {

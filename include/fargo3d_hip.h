@@ -200,6 +200,12 @@ int f3d_fused_cfl(f3d_hip_ctx *ctx, double *fluid_min);
 /* Sources from the pressure field to the mid-step ghost fill (src/algogas.c:34-118),
  * after Potential; domega is the CorrectVtheta argument (0 when not corotating). */
 int f3d_fused_sources(f3d_hip_ctx *ctx, double dt, double domega);
+/* MULTIFLUID(Sources(dt)) (src/main.c:410) of nf fluids (type[q]: 1 gas,
+ * 2 dust) in one pass: the viscous fluid's stresses, one SubStep1/2 launch
+ * over every fluid, the mid-step ghost fill.  Isothermal runs with at most
+ * one viscous fluid (else -1; call f3d_fused_sources per fluid). */
+int f3d_fused_sources_fluids(f3d_hip_ctx *ctx, double dt, double domega, int nf, const f3d_fields *fl,
+                             const int *type);
 /* Transport (src/algogas.c:135-159) and, when stockholm != 0, StockholmBoundary and
  * the end-of-step ghost fill. */
 int f3d_fused_transport(f3d_hip_ctx *ctx, double dt, int end_of_step);
@@ -292,6 +298,8 @@ int f3d_fused_dust_diffusion(f3d_hip_ctx *ctx, double dt, int nf, double *const 
  * residual/shift split (Nshift stays int) and the CFL time step. ---- */
 int f3d32_fused_cfl(f3d_hip_ctx *ctx, double *fluid_min);
 int f3d32_fused_sources(f3d_hip_ctx *ctx, double dt, double domega);
+int f3d32_fused_sources_fluids(f3d_hip_ctx *ctx, double dt, double domega, int nf, const f3d_fields *fl,
+                               const int *type);
 int f3d32_fused_transport(f3d_hip_ctx *ctx, double dt, int end_of_step);
 int f3d32_fused_transport_fluids(f3d_hip_ctx *ctx, double dt, int nf, const f3d_fields *fl);
 int f3d32_fused_potential(f3d_hip_ctx *ctx);
