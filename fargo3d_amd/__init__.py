@@ -74,6 +74,13 @@ def lib():
         L.f3d_sim_ctx.argtypes = [vp]
         L.f3d_sim_push_phys.argtypes = [vp]
         L.f3d_sim_write_field.argtypes = [vp, cp, cp]
+        L.f3d_sim_write_outputs.argtypes = [vp, i, cp]
+        L.f3d_sim_write_init.argtypes = [vp, cp]
+        L.f3d_sim_restart.argtypes = [vp, i, cp]
+        L.f3d_sim_advance.argtypes = [vp]
+        L.f3d_sim_main_loop.argtypes = [vp, i, cp]
+        for fn in (L.f3d_write_active, L.f3d_read_active):
+            fn.argtypes = [vp, i, i, i, i, i, ctypes.c_long, ctypes.c_long, cp]
         L.f3d_hip_last_error.restype = cp
         L.f3d_hip_last_error.argtypes = [vp]
         L.f3d_hip_sync.argtypes = [vp]
@@ -211,6 +218,33 @@ class Simulation:
 
     def write_field(self, name, path):
         self._check(self.L.f3d_sim_write_field(self._s, name.encode(), path.encode()))
+
+    def write_outputs(self, n, outdir):
+        """Planet files and WriteOutputs(ALL) of every fluid for output n
+        (src/main.c:328-334): <fluid>dens<n>.dat, ... in merged raw format."""
+        self._check(self.L.f3d_sim_write_outputs(self._s, int(n), str(outdir).encode()))
+
+    def write_init(self, outdir):
+        """Empty planet files and the damping references' 0_2d files, as a
+        fresh run writes them (src/main.c:262-266, src/init_stockholm.c)."""
+        self._check(self.L.f3d_sim_write_init(self._s, str(outdir).encode()))
+
+    def restart(self, n, outdir):
+        """Restart from output n of outdir (src/main.c:249-260); returns
+        begin_i = n * NINTERM."""
+        rc = self.L.f3d_sim_restart(self._s, int(n), str(outdir).encode())
+        self._check(-1 if rc < 0 else 0)
+        return rc
+
+    def advance(self):
+        """One pass of the DT loop (src/main.c:361-448); returns its step count."""
+        rc = self.L.f3d_sim_advance(self._s)
+        self._check(-1 if rc < 0 else 0)
+        return rc
+
+    def main_loop(self, begin_i, outdir):
+        """The main loop from begin_i to NTOT with its outputs (src/main.c:324-356)."""
+        self._check(self.L.f3d_sim_main_loop(self._s, int(begin_i), str(outdir).encode()))
 
     def call(self, routine, *args):
         """Call one C-ABI entry point f3d_<routine>(ctx, ...) by its reference

@@ -892,27 +892,313 @@ void f3d_sim_get_planets(f3d_sim *S, double *out, int maxn) {
   }
 }
 
+/* ---- outputs and restart ---- */
+
+/* WriteMerging's loop (src/output.c:283-315) on one host array: for each
+   active k, each active j, the nx active values of the row; x fastest,
+   native-endian doubles, no header */
+int f3d_write_active(const double *h, int nx, int ny, int nz, int nghy, int nghz, long pitch, long stride,
+                     const char *path) {
+  FILE *out = fopen(path, "wb");
+  if (!out) return -1;
+  size_t rows = 0;
+  for (int k = nghz; k < nz + nghz; k++)
+    for (int j = nghy; j < ny + nghy; j++)
+      rows += fwrite(h + (size_t)j * pitch + (size_t)k * stride, sizeof(double), (size_t)nx, out) == (size_t)nx;
+  int rc = fclose(out);
+  return (rows == (size_t)ny * nz && rc == 0) ? 0 : -1;
+}
+
+/* RestartDat with Restart_Full (src/LowTasks.c:1025-1040) on one host array:
+   fills the active cells, leaves the ghosts; -1 if the file is missing or
+   does not hold exactly nx*ny*nz doubles */
+int f3d_read_active(double *h, int nx, int ny, int nz, int nghy, int nghz, long pitch, long stride,
+                    const char *path) {
+  FILE *in = fopen(path, "rb");
+  if (!in) return -1;
+  fseek(in, 0, SEEK_END);
+  long sz = ftell(in);
+  fseek(in, 0, SEEK_SET);
+  int ok = sz == (long)sizeof(double) * nx * ny * nz;
+  for (int k = nghz; ok && k < nz + nghz; k++)
+    for (int j = nghy; ok && j < ny + nghy; j++)
+      ok = fread(h + (size_t)j * pitch + (size_t)k * stride, sizeof(double), (size_t)nx, in) == (size_t)nx;
+  fclose(in);
+  return ok ? 0 : -1;
+}
+
+/* active-cell extent of the grid in the (nz, nghz) of WriteMerging: 2-D runs
+   have one k row and no Z ghosts */
+static void active_extent(const f3d_grid *g, int *nz, int *nghz) {
+  *nz = g->dimz ? g->nz : 1;
+  *nghz = g->dimz ? g->nghz : 0;
+}
+
+static int write_dev_field(f3d_sim *S, const double *dev, const char *path) {
+  const f3d_grid *g = &S->g;
+  size_t n = g->ncell;
+  double *h = (double *)malloc(n * sizeof(double));
+  if (!h) ERR(S, "out of host memory");
+  if (download(S, h, dev, n) != 0) {
+    free(h);
+    ERR(S, "copy of a field for %s failed", path);
+  }
+  int nz, nghz;
+  active_extent(g, &nz, &nghz);
+  int rc = f3d_write_active(h, g->nx, g->ny, nz, g->nghy, nghz, g->pitch, g->stride, path);
+  free(h);
+  if (rc) ERR(S, "cannot write %s", path);
+  return 0;
+}
+
 int f3d_sim_write_field(f3d_sim *S, const char *name, const char *path) {
   int d2 = 0;
   double *p = field_ptr(S, name, &d2);
   if (!p || d2) ERR(S, "no 3-D field %s", name);
-  size_t n = S->g.ncell;
-  double *h = (double *)malloc(n * sizeof(double));
-  if (download(S, h, p, n) != 0) {
-    free(h);
-    ERR(S, "copy of %s failed", name);
+  return write_dev_field(S, p, path);
+}
+
+/* fluid names of the set-ups: the gas is "gas" (setups/fargo/condinit.c:54),
+   the dust fluids "dust1", "dust2", ... (setups/fargo_multifluid/condinit.c:54-69) */
+static void fluid_name(int k, char *buf, size_t n) {
+  if (k == 0) snprintf(buf, n, "gas");
+  else snprintf(buf, n, "dust%d", k);
+}
+
+/* the primitive fields WriteOutputs(ALL) writes for fluid k, in its order
+   (src/output.c:662-690): density, energy (not for dust), vx, vy, vz (3-D) */
+static int output_fields(f3d_sim *S, int k, const char **suffix, double **dev) {
+  const f3d_fields *F = &S->fl[k];
+  int n = 0;
+  suffix[n] = "dens"; dev[n++] = F->Density;
+  if (S->ftype[k] != F3D_DUST) { suffix[n] = "energy"; dev[n++] = F->Energy; }
+  suffix[n] = "vx"; dev[n++] = F->Vx;
+  suffix[n] = "vy"; dev[n++] = F->Vy;
+  if (S->g.dimz) { suffix[n] = "vz"; dev[n++] = F->Vz; }
+  return n;
+}
+
+/* WritePlanetSystemFile's planet<k>.dat line (src/output.c:46-75, 235-250) */
+static int write_planet_files(f3d_sim *S, int n, const char *dir) {
+  char path[1024];
+  for (int i = 0; i < S->sys.nb; i++) {
+    snprintf(path, sizeof(path), "%s/planet%d.dat", dir, i);
+    FILE *out = fopen(path, "a");
+    if (!out) ERR(S, "cannot write %s", path);
+    fprintf(out, "%d\t%#.18g\t%#.18g\t%#.18g\t%#.18g\t%#.18g\t%#.18g\t%#.18g\t%#.18g\t%#.18g\n", n, S->sys.x[i],
+            S->sys.y[i], S->sys.z[i], S->sys.vx[i], S->sys.vy[i], S->sys.vz[i], S->sys.mass[i],
+            S->clk.PhysicalTime, S->clk.OMEGAFRAME);
+    if (fclose(out) != 0) ERR(S, "cannot write %s", path);
   }
-  FILE *out = fopen(path, "wb");
-  if (!out) {
-    free(h);
-    ERR(S, "cannot write %s", path);
+  return 0;
+}
+
+int f3d_sim_write_outputs(f3d_sim *S, int n, const char *dir) {
+  if (write_planet_files(S, n, dir)) return -1;
+  char name[32], path[1024];
+  for (int k = 0; k < S->nfl; k++) {
+    const char *suffix[5];
+    double *dev[5];
+    int m = output_fields(S, k, suffix, dev);
+    fluid_name(k, name, sizeof(name));
+    for (int q = 0; q < m; q++) {
+      snprintf(path, sizeof(path), "%s/%s%s%d.dat", dir, name, suffix[q], n);
+      if (write_dev_field(S, dev[q], path)) return -1;
+    }
   }
-  const f3d_grid *g = &S->g;
-  int kb = g->dimz ? g->nghz : 0, ke = g->dimz ? g->nz + g->nghz : 1;
-  for (int k = kb; k < ke; k++)
-    for (int j = g->nghy; j < g->ny + g->nghy; j++)
-      fwrite(h + (size_t)j * g->pitch + (size_t)k * g->stride, sizeof(double), (size_t)g->nx, out);
-  fclose(out);
+  return 0;
+}
+
+/* the damping references and the file names init_stockholm gives them
+   (src/init_stockholm.c:30-50, 106-124): density, vx, vy, vz (3-D), energy
+   (adiabatic) */
+static int stockholm_arrays(f3d_sim *S, int k, const char **suffix, double **dev) {
+  const f3d_fields *F = &S->fl[k];
+  int n = 0;
+  suffix[n] = "dens"; dev[n++] = F->Density0;
+  suffix[n] = "vx"; dev[n++] = F->Vx0;
+  suffix[n] = "vy"; dev[n++] = F->Vy0;
+  if (S->g.dimz) { suffix[n] = "vz"; dev[n++] = F->Vz0; }
+  if (S->p.eos == F3D_ADIABATIC) { suffix[n] = "energy"; dev[n++] = F->Energy0; }
+  return n;
+}
+
+/* Write2D with GHOSTINC of one YZ array (src/output2d.c:3-59): on one rank
+   every (j, k) of the array, ghosts and corners included, j fastest */
+static int write_stockholm_files(f3d_sim *S, const char *dir) {
+  const size_t n2 = S->g.n2d;
+  double *h = (double *)malloc(n2 * sizeof(double));
+  if (!h) ERR(S, "out of host memory");
+  char name[32], path[1024];
+  for (int k = 0; k < S->nfl; k++) {
+    const char *suffix[5];
+    double *dev[5];
+    int m = stockholm_arrays(S, k, suffix, dev);
+    fluid_name(k, name, sizeof(name));
+    for (int q = 0; q < m; q++) {
+      snprintf(path, sizeof(path), "%s/%s%s0_2d.dat", dir, name, suffix[q]);
+      FILE *out = NULL;
+      if (download(S, h, dev[q], n2) != 0 || !(out = fopen(path, "wb")) ||
+          fwrite(h, sizeof(double), n2, out) != n2) {
+        if (out) fclose(out);
+        free(h);
+        ERR(S, "cannot write %s", path);
+      }
+      fclose(out);
+    }
+  }
   free(h);
+  return 0;
+}
+
+int f3d_sim_write_init(f3d_sim *S, const char *dir) {
+  /* EmptyPlanetSystemFiles (src/output.c:33-44) */
+  char path[1024];
+  for (int i = 0; i < S->sys.nb; i++) {
+    snprintf(path, sizeof(path), "%s/planet%d.dat", dir, i);
+    FILE *out = fopen(path, "w");
+    if (!out) ERR(S, "cannot write %s", path);
+    fclose(out);
+  }
+  return S->p.stockholm ? write_stockholm_files(S, dir) : 0;
+}
+
+/* GetfromPlanetFile (src/output.c:77-110): the values of the line of
+   planet<k>.dat whose first column is n; v[0..8] = columns 2..10 */
+static int planet_line(f3d_sim *S, int k, int n, const char *dir, double v[9]) {
+  char path[1024], line[1024];
+  snprintf(path, sizeof(path), "%s/planet%d.dat", dir, k);
+  FILE *in = fopen(path, "r");
+  if (!in) ERR(S, "cannot read %s", path);
+  int found = 0;
+  while (!found && fgets(line, sizeof(line), in)) {
+    char *e;
+    long t = strtol(line, &e, 10);
+    if (e == line || t != n) continue;
+    char *q = e;
+    int c = 0;
+    for (; c < 9; c++) {
+      v[c] = strtod(q, &e);
+      if (e == q) break;
+      q = e;
+    }
+    found = c == 9;
+  }
+  fclose(in);
+  if (!found) ERR(S, "no entry %d in %s", n, path);
+  return 0;
+}
+
+int f3d_sim_restart(f3d_sim *S, int n, const char *dir) {
+  /* RestartSimulation of every fluid (src/LowTasks.c:816-881), merged files */
+  const f3d_grid *g = &S->g;
+  const size_t nc = g->ncell, n2 = g->n2d;
+  int nz, nghz;
+  active_extent(g, &nz, &nghz);
+  double *h = (double *)malloc((nc > n2 ? nc : n2) * sizeof(double));
+  if (!h) ERR(S, "out of host memory");
+  char name[32], path[1024];
+  for (int k = 0; k < S->nfl; k++) {
+    const char *suffix[5];
+    double *dev[5];
+    int m = output_fields(S, k, suffix, dev);
+    fluid_name(k, name, sizeof(name));
+    for (int q = 0; q < m; q++) {
+      snprintf(path, sizeof(path), "%s/%s%s%d.dat", dir, name, suffix[q], n);
+      /* the ghosts keep the initial condition's values until the fill below */
+      if (download(S, h, dev[q], nc) != 0 ||
+          f3d_read_active(h, g->nx, g->ny, nz, g->nghy, nghz, g->pitch, g->stride, path) != 0 ||
+          upload(S, dev[q], h, nc) != 0) {
+        free(h);
+        ERR(S, "cannot restart from %s (missing, or not %d x %d x %d doubles)", path, g->nx, g->ny, nz);
+      }
+    }
+  }
+  /* the clock and the planets from planet0.dat (src/main.c:256-260) */
+  if (S->sys.nb > 0) {
+    double v[9];
+    for (int i = 0; i < S->sys.nb; i++) {
+      if (planet_line(S, i, n, dir, v)) {
+        free(h);
+        return -1;
+      }
+      if (i == 0) {
+        S->clk.PhysicalTime = v[7];
+        S->clk.OMEGAFRAME = v[8];
+      }
+      S->sys.x[i] = v[0]; S->sys.y[i] = v[1]; S->sys.z[i] = v[2];
+      S->sys.vx[i] = v[3]; S->sys.vy[i] = v[4]; S->sys.vz[i] = v[5];
+      S->sys.mass[i] = v[6];
+    }
+  }
+  S->dtemp = 0.0;
+  /* FillGhosts, then init_stockholm of every fluid (src/main.c:311-316): a
+     restart reads each damping reference from its 0_2d file when the file is
+     there with the full size, else takes it from the restarted fields
+     (src/init_stockholm.c:30-105), and writes the files again */
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1 && !S->p.stockholm) break;
+    for (int k = 0; k < S->nfl; k++) {
+      select_fluid(S, k);
+      if (f3d_sim_push_phys(S)) {
+        free(h);
+        return -1;
+      }
+      int rc = pass == 0 ? (S->prec32 ? f3d32_fused_boundaries(S->ctx) : f3d_boundaries(S->ctx))
+                         : (S->prec32 ? f3d32_fused_init_stockholm(S->ctx) : f3d_init_stockholm(S->ctx));
+      if (rc != 0) {
+        free(h);
+        ERR(S, "%s", f3d_hip_last_error(S->ctx));
+      }
+      if (pass == 0) continue;
+      const char *suffix[5];
+      double *dev[5];
+      int m = stockholm_arrays(S, k, suffix, dev);
+      fluid_name(k, name, sizeof(name));
+      for (int q = 0; q < m; q++) {
+        snprintf(path, sizeof(path), "%s/%s%s0_2d.dat", dir, name, suffix[q]);
+        FILE *in = fopen(path, "rb");
+        if (!in) continue;
+        fseek(in, 0, SEEK_END);
+        long sz = ftell(in);
+        fseek(in, 0, SEEK_SET);
+        int ok = sz == (long)(n2 * sizeof(double)) && fread(h, sizeof(double), n2, in) == n2;
+        fclose(in);
+        if (ok && upload(S, dev[q], h, n2) != 0) {
+          free(h);
+          ERR(S, "upload of %s failed", path);
+        }
+      }
+    }
+  }
+  free(h);
+  select_fluid(S, 0);
+  if (f3d_sim_push_phys(S)) return -1;
+  if (S->p.stockholm && write_stockholm_files(S, dir)) return -1;
+  CALL(S, f3d_hip_sync(S->ctx));
+  return n * S->p.ninterm;
+}
+
+/* one pass of the DT loop (src/main.c:361-448): steps until the output
+   interval DT is reached; returns the number of steps */
+int f3d_sim_advance(f3d_sim *S) {
+  int steps = 0;
+  do {
+    if (f3d_sim_step(S, NULL)) return -1;
+    steps++;
+  } while (S->dtemp < S->p.dt_out);
+  CALL(S, f3d_hip_sync(S->ctx));
+  return steps;
+}
+
+/* the main loop with its outputs (src/main.c:324-356): from begin_i to NTOT,
+   the planet files and WriteOutputs(ALL) of every fluid every NINTERM passes */
+int f3d_sim_main_loop(f3d_sim *S, int begin_i, const char *dir) {
+  const int ninterm = S->p.ninterm > 0 ? S->p.ninterm : 1;
+  for (int i = begin_i; i <= S->p.ntot; i++) {
+    if (i % ninterm == 0 && f3d_sim_write_outputs(S, i / ninterm, dir)) return -1;
+    if (i == S->p.ntot) break;
+    if (f3d_sim_advance(S) < 0) return -1;
+  }
   return 0;
 }

@@ -61,9 +61,39 @@ f3d_hip_ctx *f3d_sim_ctx(f3d_sim *S);
 /* refresh the kernels' copy of the run parameters and host scalars */
 int f3d_sim_push_phys(f3d_sim *S);
 
-/* Output in FARGO3D's merged raw format (src/output.c:283-315): active cells,
- * x fastest, native-endian doubles, no header; file <dir>/<prefix><name><n>.dat. */
+/* ---- outputs and restart ----
+ * Files in FARGO3D's merged raw format (WriteMerging, src/output.c:283-315):
+ * active cells, x fastest, native-endian doubles, no header.  The precision
+ * form writes and reads doubles too. */
 int f3d_sim_write_field(f3d_sim *S, const char *name, const char *path);
+/* WritePlanetSystemFile + WriteOutputs(ALL) of every fluid (src/main.c:328-334,
+ * src/output.c:551-700): <dir>/planet<k>.dat gets the line of output n, and
+ * <dir>/<fluid>{dens,energy,vx,vy,vz}<n>.dat are written (no energy for dust,
+ * vz in 3-D only; fluids "gas", "dust1", ...).  tqwk<k>.dat is not written. */
+int f3d_sim_write_outputs(f3d_sim *S, int n, const char *dir);
+/* start of a run (src/main.c:262-266, src/init_stockholm.c:106-124): empties
+ * planet<k>.dat and writes the damping references <fluid><q>0_2d.dat */
+int f3d_sim_write_init(f3d_sim *S, const char *dir);
+/* restart from output n of <dir> (src/main.c:249-260, RestartSimulation
+ * src/LowTasks.c:816-881 with Restart_Full, RestartPlanetarySystem
+ * src/output.c:112-123): the active cells of every fluid's primitive fields,
+ * PhysicalTime, OMEGAFRAME and the planets from planet<k>.dat, then the ghost
+ * fill and init_stockholm (references from the 0_2d files when present).
+ * Returns begin_i = n * NINTERM, or -1. */
+int f3d_sim_restart(f3d_sim *S, int n, const char *dir);
+/* one pass of the DT loop: steps until the output interval DT is reached;
+ * returns the number of steps, or -1 */
+int f3d_sim_advance(f3d_sim *S);
+/* the main loop from begin_i to NTOT with outputs every NINTERM passes
+ * (src/main.c:324-356) */
+int f3d_sim_main_loop(f3d_sim *S, int begin_i, const char *dir);
+/* the file layout on plain host arrays (pitch, stride in elements; a 2-D
+ * grid passes nz = 1, nghz = 0); read fills the active cells only and fails
+ * unless the file holds exactly nx*ny*nz doubles */
+int f3d_write_active(const double *h, int nx, int ny, int nz, int nghy, int nghz, long pitch, long stride,
+                     const char *path);
+int f3d_read_active(double *h, int nx, int ny, int nz, int nghy, int nghz, long pitch, long stride,
+                    const char *path);
 
 /* ---- batched form (no counterpart in the reference): n independent 2-D
  * isothermal disks, each its own .par set-up, run in the FP32 form with
