@@ -285,6 +285,29 @@ def multifluid_form(fa, L, names, steps, warmup):
     return out
 
 
+def output_rate(sim, reps=3):
+    """Time one WriteOutputs(ALL) of every fluid from device state (merged
+    raw files into a temporary folder): device-to-host copy plus the file
+    writes, best of reps; not part of the timed step."""
+    import shutil
+    import tempfile
+    d = tempfile.mkdtemp(prefix="f3d_out_")
+    try:
+        best = None
+        for r in range(reps):
+            sim.sync()
+            t0 = time.perf_counter()
+            sim.write_outputs(r, d)
+            el = time.perf_counter() - t0
+            best = el if best is None else min(best, el)
+        nbytes = sum(os.path.getsize(os.path.join(d, f)) for f in os.listdir(d)
+                     if f.endswith("0.dat") and not f.startswith("planet"))
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return {"what": "planet files + every fluid's dens/energy/vx/vy(/vz) files, merged raw format",
+            "bytes": nbytes, "ms": round(1e3 * best, 3), "GB_s": round(nbytes / best / 1e9, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -331,6 +354,7 @@ def main():
     ad = 1 if sim.adiabatic else 0
     alpha = 1 if args.config == "c2" else 0
     dimz = sim.dimz
+    outputs = output_rate(sim)
     sim.close()
 
     value = cells * nfl * args.steps / elapsed
@@ -395,6 +419,7 @@ def main():
                           "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "stages": stages,
         "forms": forms,
+        "outputs": outputs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
