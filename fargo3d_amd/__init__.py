@@ -79,6 +79,7 @@ def lib():
         L.f3d_sim_restart.argtypes = [vp, i, cp]
         L.f3d_sim_advance.argtypes = [vp]
         L.f3d_sim_main_loop.argtypes = [vp, i, cp]
+        L.f3d_sim_monitor.argtypes = [vp, i, i, i, i, cp]
         for fn in (L.f3d_write_active, L.f3d_read_active):
             fn.argtypes = [vp, i, i, i, i, i, ctypes.c_long, ctypes.c_long, cp]
         L.f3d_hip_last_error.restype = cp
@@ -241,6 +242,13 @@ class Simulation:
         rc = self.L.f3d_sim_advance(self._s)
         self._check(-1 if rc < 0 else 0)
         return rc
+
+    def monitor(self, m2d, y_raw, scalar, timestep, outdir):
+        """MonitorGlobal of every fluid (src/monitor.c:236-254) for the
+        quantity bits (MASS 1, MOM_X 64, MOM_Y 128, MOM_Z 256, TORQ 512) of
+        each kind, into outdir/monitor/<fluid>/."""
+        self._check(self.L.f3d_sim_monitor(self._s, int(m2d), int(y_raw), int(scalar), int(timestep),
+                                           str(outdir).encode()))
 
     def main_loop(self, begin_i, outdir):
         """The main loop from begin_i to NTOT with its outputs (src/main.c:324-356)."""

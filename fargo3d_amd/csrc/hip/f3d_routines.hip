@@ -563,6 +563,59 @@ __global__ void k_init_stockholm(dmesh m, const double* rho, const double* vx, c
   rho0[l2] = rho[l];
 }
 
+// mon_dens, mon_momx/y/z and mon_torque (src/mon_dens.c:10-68, src/mon_momx.c,
+// src/mon_momy.c, src/mon_momz.c, src/mon_torque.c): the monitored quantity of
+// each active cell into an FP64 array, with the reference's association
+// (Vol(i,j,k) expands to Syk(k)*Sxi(i)/InvVj(j) inside the product).  T is the
+// field type: double, or float for the precision form, whose values are
+// widened before any arithmetic so that its monitors are FP64 sums.
+enum { MON_DENS, MON_MOMX, MON_MOMY, MON_MOMZ, MON_TORQ };
+
+template <int G, int DZ, int Q, typename T>
+__global__ void k_mon(dmesh m, Range r, const T* __restrict__ dens, const T* __restrict__ v,
+                      double* __restrict__ out, double omegaframe, double xp, double yp, double zp, double rsm2) {
+  CELL_INDEX;
+  const double rho = (double)dens[l];
+  double q;
+  if (Q == MON_DENS) {
+    q = mul_vol(m, rho, i, j, k);
+  } else if (Q == MON_MOMX) {
+    const double vm = .5 * ((double)v[l] + (double)v[lxp_of(l, i, m.nx)]);
+    if (G == SPH) {
+      const double rcyl = m.ymed[j] * m.sinzmed[k];
+      q = mul_vol(m, ((vm + rcyl * omegaframe) * rcyl) * rho, i, j, k);
+    } else {
+      q = mul_vol(m, ((vm + m.ymed[j] * omegaframe) * m.ymed[j]) * rho, i, j, k);
+    }
+  } else if (Q == MON_MOMY) {
+    q = mul_vol(m, (rho * .5) * ((double)v[l] + (double)v[l + m.pitch]), i, j, k);
+  } else if (Q == MON_MOMZ) {
+    q = mul_vol(m, (rho * .5) * ((double)v[l] + (double)v[l + m.stride]), i, j, k);
+    if (G == SPH) q *= m.ymed[j];
+  } else {
+    const double cellmass = vol(m, i, j, k) * rho;
+    double dx, dy, dz = 0.0;
+    if (G == SPH) {
+      dx = m.ymed[j] * m.cosxmed[i] * m.sinzmed[k] - xp;
+      dy = m.ymed[j] * m.sinxmed[i] * m.sinzmed[k] - yp;
+      if (DZ) dz = m.ymed[j] * m.coszmed[k] - zp;
+    } else {
+      dx = m.ymed[j] * m.cosxmed[i] - xp;
+      dy = m.ymed[j] * m.sinxmed[i] - yp;
+      if (DZ) dz = m.zmed[k] - zp;
+    }
+    double dist2 = dx * dx + dy * dy + dz * dz;
+    dist2 += rsm2;
+    const double distance = sqrt(dist2);
+    double InvDist3 = 1.0 / (dist2 * distance);
+    InvDist3 *= 1.0 * cellmass;
+    const double fxi = dx * InvDist3;
+    const double fyi = dy * InvDist3;
+    q = xp * fyi - yp * fxi;
+  }
+  out[l] = q;
+}
+
 // ------------------------------------------------------------------ C ABI
 
 extern "C" {
@@ -1116,6 +1169,71 @@ int f3d_init_stockholm(f3d_hip_ctx* ctx) {
                      ctx->f.Vz0, ctx->f.Energy0, (int)(ctx->ph.eos == 2));
   F3D_LAUNCH_CHECK(ctx);
   return 0;
+}
+
+
+// the monitored quantity q (MON_*) of the bound fluid over the active cells
+static int mon_launch(f3d_hip_ctx* ctx, int q, int fp32, double xp, double yp, double zp, double rsmoothing,
+                      double* out) {
+  F3D_REQUIRE(ctx, out != nullptr, "monitor: null output array");
+  NEED(ctx->f.Density, "Density");
+  const double* v = nullptr;
+  if (q == MON_MOMX) { NEED(ctx->f.Vx, "Vx"); v = ctx->f.Vx; }
+  if (q == MON_MOMY) { NEED(ctx->f.Vy, "Vy"); v = ctx->f.Vy; }
+  if (q == MON_MOMZ) {
+    F3D_REQUIRE(ctx, ctx->m.dimz, "mon_momz: the mesh has no Z dimension");
+    NEED(ctx->f.Vz, "Vz");
+    v = ctx->f.Vz;
+  }
+  const dmesh& m = ctx->m;
+  Range r = rng(ctx, m.nghy, m.nghy + m.ny, m.nghz, m.nghz + m.nz);
+  if (empty(r)) return 0;
+  const dim3 g = grid_of(ctx, r);
+  const double of = ctx->ph.OMEGAFRAME, rsm2 = rsmoothing * rsmoothing;
+#define MON_Q(G_, DZ_, Q_)                                                                                    \
+  do {                                                                                                        \
+    if (fp32)                                                                                                 \
+      hipLaunchKernelGGL((k_mon<G_, DZ_, Q_, float>), g, dim3(BX), 0, ctx->stream, m, r,                      \
+                         reinterpret_cast<const float*>(ctx->f.Density), reinterpret_cast<const float*>(v),   \
+                         out, of, xp, yp, zp, rsm2);                                                          \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_mon<G_, DZ_, Q_, double>), g, dim3(BX), 0, ctx->stream, m, r, ctx->f.Density, v, \
+                         out, of, xp, yp, zp, rsm2);                                                          \
+  } while (0)
+#define MON_G(G_, DZ_)                                  \
+  do {                                                  \
+    switch (q) {                                        \
+      case MON_DENS: MON_Q(G_, DZ_, MON_DENS); break;   \
+      case MON_MOMX: MON_Q(G_, DZ_, MON_MOMX); break;   \
+      case MON_MOMY: MON_Q(G_, DZ_, MON_MOMY); break;   \
+      case MON_MOMZ: MON_Q(G_, DZ_, MON_MOMZ); break;   \
+      default: MON_Q(G_, DZ_, MON_TORQ); break;         \
+    }                                                   \
+  } while (0)
+  if (m.geom == SPH) {
+    if (m.dimz) MON_G(SPH, 1); else MON_G(SPH, 0);
+  } else {
+    if (m.dimz) MON_G(CYL, 1); else MON_G(CYL, 0);
+  }
+#undef MON_G
+#undef MON_Q
+  F3D_LAUNCH_CHECK(ctx);
+  return 0;
+}
+
+int f3d_mon_dens(f3d_hip_ctx* ctx, double* out) { return mon_launch(ctx, MON_DENS, 0, 0, 0, 0, 0, out); }
+int f3d_mon_momx(f3d_hip_ctx* ctx, double* out) { return mon_launch(ctx, MON_MOMX, 0, 0, 0, 0, 0, out); }
+int f3d_mon_momy(f3d_hip_ctx* ctx, double* out) { return mon_launch(ctx, MON_MOMY, 0, 0, 0, 0, 0, out); }
+int f3d_mon_momz(f3d_hip_ctx* ctx, double* out) { return mon_launch(ctx, MON_MOMZ, 0, 0, 0, 0, 0, out); }
+int f3d_mon_torq(f3d_hip_ctx* ctx, double xp, double yp, double zp, double rsmoothing, double* out) {
+  return mon_launch(ctx, MON_TORQ, 0, xp, yp, zp, rsmoothing, out);
+}
+int f3d32_mon_dens(f3d_hip_ctx* ctx, double* out) { return mon_launch(ctx, MON_DENS, 1, 0, 0, 0, 0, out); }
+int f3d32_mon_momx(f3d_hip_ctx* ctx, double* out) { return mon_launch(ctx, MON_MOMX, 1, 0, 0, 0, 0, out); }
+int f3d32_mon_momy(f3d_hip_ctx* ctx, double* out) { return mon_launch(ctx, MON_MOMY, 1, 0, 0, 0, 0, out); }
+int f3d32_mon_momz(f3d_hip_ctx* ctx, double* out) { return mon_launch(ctx, MON_MOMZ, 1, 0, 0, 0, 0, out); }
+int f3d32_mon_torq(f3d_hip_ctx* ctx, double xp, double yp, double zp, double rsmoothing, double* out) {
+  return mon_launch(ctx, MON_TORQ, 1, xp, yp, zp, rsmoothing, out);
 }
 
 }  // extern "C"

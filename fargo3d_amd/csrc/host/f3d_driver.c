@@ -15,6 +15,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
+#include <errno.h>
 
 typedef struct {
   int fused_cfl, fused_sources, fused_transport;
@@ -44,6 +46,8 @@ struct f3d_sim {
   double alpha[F3D_MAX_FLUIDS * F3D_MAX_FLUIDS]; /* ColRate matrix, [i + j*nfl] (src/colrate.c:10-23) */
   f3d_fields fl[F3D_MAX_FLUIDS];                 /* the per-fluid members of each fluid */
   long pot_step;                                 /* Timestepcount of the last Potential launch */
+  double *monbuf;                                /* FP64 cell values of a monitored quantity */
+  int mon_counter;                               /* MonCounter (src/monitor.c:14) */
   char err[512];
 };
 
@@ -378,6 +382,7 @@ void f3d_sim_destroy(f3d_sim *S) {
   for (size_t a = 0; a < sizeof(all) / sizeof(all[0]); a++)
     if (all[a]) f3d_hip_free(all[a]);
   if (S->ctx) f3d_hip_destroy(S->ctx);
+  if (S->monbuf) f3d_hip_free(S->monbuf);
   f3d_bctab_free(&S->bct);
   f3d_grid_free(&S->g);
   free(S);
@@ -978,11 +983,12 @@ static int output_fields(f3d_sim *S, int k, const char **suffix, double **dev) {
   return n;
 }
 
-/* WritePlanetSystemFile's planet<k>.dat line (src/output.c:46-75, 235-250) */
-static int write_planet_files(f3d_sim *S, int n, const char *dir) {
+/* WritePlanetSystemFile's planet<k>.dat (big: bigplanet<k>.dat) line
+   (src/output.c:46-75, 235-250) */
+static int write_planet_files(f3d_sim *S, int n, const char *dir, int big) {
   char path[1024];
   for (int i = 0; i < S->sys.nb; i++) {
-    snprintf(path, sizeof(path), "%s/planet%d.dat", dir, i);
+    snprintf(path, sizeof(path), big ? "%s/bigplanet%d.dat" : "%s/planet%d.dat", dir, i);
     FILE *out = fopen(path, "a");
     if (!out) ERR(S, "cannot write %s", path);
     fprintf(out, "%d\t%#.18g\t%#.18g\t%#.18g\t%#.18g\t%#.18g\t%#.18g\t%#.18g\t%#.18g\t%#.18g\n", n, S->sys.x[i],
@@ -994,7 +1000,7 @@ static int write_planet_files(f3d_sim *S, int n, const char *dir) {
 }
 
 int f3d_sim_write_outputs(f3d_sim *S, int n, const char *dir) {
-  if (write_planet_files(S, n, dir)) return -1;
+  if (write_planet_files(S, n, dir, 0)) return -1;
   char name[32], path[1024];
   for (int k = 0; k < S->nfl; k++) {
     const char *suffix[5];
@@ -1055,10 +1061,12 @@ int f3d_sim_write_init(f3d_sim *S, const char *dir) {
   /* EmptyPlanetSystemFiles (src/output.c:33-44) */
   char path[1024];
   for (int i = 0; i < S->sys.nb; i++) {
-    snprintf(path, sizeof(path), "%s/planet%d.dat", dir, i);
-    FILE *out = fopen(path, "w");
-    if (!out) ERR(S, "cannot write %s", path);
-    fclose(out);
+    for (int big = 0; big < 2; big++) {
+      snprintf(path, sizeof(path), big ? "%s/bigplanet%d.dat" : "%s/planet%d.dat", dir, i);
+      FILE *out = fopen(path, "w");
+      if (!out) ERR(S, "cannot write %s", path);
+      fclose(out);
+    }
   }
   return S->p.stockholm ? write_stockholm_files(S, dir) : 0;
 }
@@ -1191,14 +1199,147 @@ int f3d_sim_advance(f3d_sim *S) {
   return steps;
 }
 
+/* ---- monitoring (src/monitor.c) ---- */
+
+static int make_dir(f3d_sim *S, const char *path) {
+  if (mkdir(path, 0755) != 0 && errno != EEXIST) ERR(S, "cannot create %s", path);
+  return 0;
+}
+
+/* MonitorFunction (src/monitor.c:67-211) for the quantity in Reduction2D:
+   the 2-D map (NOGHOSTINC), the raw Y profile and the scalar, each if the
+   quantity is in that kind's mask */
+static int monitor_function(f3d_sim *S, int bit, const char *mname, int plnb, int timestep, const char *fdir,
+                            int m2d, int myraw, int mscalar) {
+  const f3d_grid *g = &S->g;
+  const int sy = g->ny + 2 * g->nghy;
+  const int kb = g->dimz ? g->nghz : 0, ke = g->dimz ? g->nz + g->nghz : 1;
+  const size_t n2 = g->n2d;
+  char pl[32], path[1200];
+  if (plnb < 0) pl[0] = 0;
+  else snprintf(pl, sizeof(pl), "_planet_%d", plnb);
+  double *R = (double *)malloc(n2 * sizeof(double));
+  if (!R) ERR(S, "out of host memory");
+  if (f3d_hip_memcpy_d2h(S->ctx, R, S->f.Reduction2D, n2 * sizeof(double)) != 0) {
+    free(R);
+    ERR(S, "copy of Reduction2D failed: %s", f3d_hip_last_error(S->ctx));
+  }
+  int rc = 0;
+  FILE *out = NULL;
+  if (m2d & bit) {
+    /* Write2D NOGHOSTINC into monitor/<fluid>/FG<timestep>/ (src/output2d.c:3-59) */
+    char fg[1100];
+    snprintf(fg, sizeof(fg), "%s/FG%06d", fdir, timestep);
+    if (make_dir(S, fg) == 0) {
+      snprintf(path, sizeof(path), "%s/%s_2d_%07d%s.dat", fg, mname, S->mon_counter, pl);
+      if ((out = fopen(path, "wb")) != NULL) {
+        for (int k = kb; k < ke; k++) fwrite(R + (size_t)k * sy + g->nghy, sizeof(double), (size_t)g->ny, out);
+        fclose(out);
+      } else rc = -1;
+    } else rc = -1;
+  }
+  if (rc == 0 && (myraw & bit)) {
+    double *prof = (double *)calloc((size_t)g->ny, sizeof(double));
+    for (int j = g->nghy; prof && j < g->ny + g->nghy; j++)
+      for (int k = kb; k < ke; k++) prof[j - g->nghy] += R[(size_t)k * sy + j];
+    snprintf(path, sizeof(path), "%s/%s_1d_Y_raw%s.dat", fdir, mname, pl);
+    if (prof && (out = fopen(path, "ab")) != NULL) {
+      fwrite(prof, sizeof(double), (size_t)g->ny, out);
+      fclose(out);
+    } else rc = -1;
+    free(prof);
+  }
+  if (rc == 0 && (mscalar & bit)) {
+    double lsum = 0.0;
+    for (int k = kb; k < ke; k++)
+      for (int j = g->nghy; j < g->ny + g->nghy; j++) lsum += R[(size_t)k * sy + j];
+    snprintf(path, sizeof(path), "%s/%s%s.dat", fdir, mname, pl);
+    if ((out = fopen(path, "a")) != NULL) {
+      fprintf(out, "%.12g\t%.12g\n", S->clk.PhysicalTime, lsum);
+      fclose(out);
+    } else rc = -1;
+  }
+  free(R);
+  if (rc) ERR(S, "cannot write %s", path);
+  return 0;
+}
+
+int f3d_sim_monitor(f3d_sim *S, int m2d, int myraw, int mscalar, int timestep, const char *dir) {
+  /* MonitorGlobal of every fluid (src/monitor.c:236-254): each monitored
+     quantity in increasing bit order, per planet for the torque */
+  static const struct { int bit; const char *name; int q; int perplanet; } Q[] = {
+      {F3D_MON_MASS, "mass", 0, 0}, {F3D_MON_MOM_X, "momx", 1, 0}, {F3D_MON_MOM_Y, "momy", 2, 0},
+      {F3D_MON_MOM_Z, "momz", 3, 0}, {F3D_MON_TORQ, "torq", 4, 1}};
+  const int all = m2d | myraw | mscalar;
+  if (all & ~(F3D_MON_MASS | F3D_MON_MOM_X | F3D_MON_MOM_Y | F3D_MON_MOM_Z | F3D_MON_TORQ))
+    ERR(S, "monitor: only MASS, MOM_X, MOM_Y, MOM_Z and TORQ are monitored");
+  if ((all & F3D_MON_MOM_Z) && !S->g.dimz) ERR(S, "monitor: MOM_Z needs a Z dimension");
+  const f3d_grid *g = &S->g;
+  if (!S->monbuf) {
+    void *p = NULL;
+    if (f3d_hip_malloc(&p, g->ncell * sizeof(double)) != 0) ERR(S, "monitor: device allocation failed");
+    S->monbuf = (double *)p;
+  }
+  char mdir[1024], fdir[1100], name[32];
+  snprintf(mdir, sizeof(mdir), "%s/monitor", dir);
+  if (make_dir(S, mdir)) return -1;
+  const int kb = g->dimz ? g->nghz : 0, ke = g->dimz ? g->nz + g->nghz : 1;
+  for (int f = 0; f < S->nfl; f++) {
+    select_fluid(S, f);
+    if (f3d_sim_push_phys(S)) return -1;
+    fluid_name(f, name, sizeof(name));
+    snprintf(fdir, sizeof(fdir), "%s/%s", mdir, name);
+    if (make_dir(S, fdir)) return -1;
+    for (size_t a = 0; a < sizeof(Q) / sizeof(Q[0]); a++) {
+      if (!(all & Q[a].bit)) continue;
+      const int nb = Q[a].perplanet ? S->sys.nb : 1;
+      for (int i = 0; i < nb; i++) {
+        double xp = 0, yp = 0, zp = 0, rsm = 0;
+        if (Q[a].perplanet) {
+          /* src/mon_torque.c:15-16: the smoothing length of the planet */
+          xp = S->sys.x[i]; yp = S->sys.y[i]; zp = S->sys.z[i];
+          const double rp = sqrt(xp * xp + yp * yp + zp * zp);
+          rsm = S->p.thicknesssmoothing * S->p.aspectratio * pow(rp / F3D_R0, S->p.flaringindex) * rp;
+        }
+        double *b = S->monbuf;
+        int rc;
+        switch (Q[a].q) {
+          case 0: rc = S->prec32 ? f3d32_mon_dens(S->ctx, b) : f3d_mon_dens(S->ctx, b); break;
+          case 1: rc = S->prec32 ? f3d32_mon_momx(S->ctx, b) : f3d_mon_momx(S->ctx, b); break;
+          case 2: rc = S->prec32 ? f3d32_mon_momy(S->ctx, b) : f3d_mon_momy(S->ctx, b); break;
+          case 3: rc = S->prec32 ? f3d32_mon_momz(S->ctx, b) : f3d_mon_momz(S->ctx, b); break;
+          default:
+            rc = S->prec32 ? f3d32_mon_torq(S->ctx, xp, yp, zp, rsm, b) : f3d_mon_torq(S->ctx, xp, yp, zp, rsm, b);
+        }
+        if (rc) ERR(S, "monitor %s: %s", Q[a].name, f3d_hip_last_error(S->ctx));
+        CALL(S, f3d_reduction_SUM(S->ctx, b, g->nghy, g->ny + g->nghy, kb, ke));
+        if (monitor_function(S, Q[a].bit, Q[a].name, Q[a].perplanet ? i : -1, timestep, fdir, m2d, myraw,
+                             mscalar))
+          return -1;
+      }
+    }
+  }
+  select_fluid(S, 0);
+  if (f3d_sim_push_phys(S)) return -1;
+  S->mon_counter++;
+  return 0;
+}
+
 /* the main loop with its outputs (src/main.c:324-356): from begin_i to NTOT,
    the planet files and WriteOutputs(ALL) of every fluid every NINTERM passes */
 int f3d_sim_main_loop(f3d_sim *S, int begin_i, const char *dir) {
   const int ninterm = S->p.ninterm > 0 ? S->p.ninterm : 1;
   for (int i = begin_i; i <= S->p.ntot; i++) {
-    if (i % ninterm == 0 && f3d_sim_write_outputs(S, i / ninterm, dir)) return -1;
+    const int timestep = i / ninterm;
+    if (i % ninterm == 0 && f3d_sim_write_outputs(S, timestep, dir)) return -1;
     if (i == S->p.ntot) break;
     if (f3d_sim_advance(S) < 0) return -1;
+    /* after each DT pass: MonitorGlobal of every fluid with the set-up's
+       monitors, then the bigplanet line (src/main.c:452-461) */
+    if ((S->p.mon_2d | S->p.mon_y_raw | S->p.mon_scalar) &&
+        f3d_sim_monitor(S, S->p.mon_2d, S->p.mon_y_raw, S->p.mon_scalar, timestep, dir))
+      return -1;
+    if (write_planet_files(S, timestep, dir, 1)) return -1;
   }
   return 0;
 }

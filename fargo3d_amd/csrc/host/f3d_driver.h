@@ -84,6 +84,15 @@ int f3d_sim_restart(f3d_sim *S, int n, const char *dir);
 /* one pass of the DT loop: steps until the output interval DT is reached;
  * returns the number of steps, or -1 */
 int f3d_sim_advance(f3d_sim *S);
+/* monitored quantities: the F3D_MON_* bits of f3d_host.h */
+/* MonitorGlobal of every fluid (src/monitor.c:236-254): each quantity of
+ * m2d | myraw | mscalar, per planet for TORQ, summed on the device into
+ * Reduction2D (ring sums in sequential i order) and from there on the host
+ * in the reference's order, written under <dir>/monitor/<fluid>/: the scalar
+ * line "<time>\t<sum>" of <q>[_planet_<k>].dat, the raw Y profile appended to
+ * <q>_1d_Y_raw[_planet_<k>].dat, the 2-D map FG<timestep>/<q>_2d_<counter>.dat.
+ * The precision form computes the cell values and every sum in FP64. */
+int f3d_sim_monitor(f3d_sim *S, int m2d, int myraw, int mscalar, int timestep, const char *dir);
 /* the main loop from begin_i to NTOT with outputs every NINTERM passes
  * (src/main.c:324-356) */
 int f3d_sim_main_loop(f3d_sim *S, int begin_i, const char *dir);

@@ -36,6 +36,12 @@ enum f3d_geometry { F3D_CYLINDRICAL = 1, F3D_SPHERICAL = 2 };
 enum f3d_eos { F3D_ISOTHERMAL = 1, F3D_ADIABATIC = 2 };
 enum f3d_visc { F3D_VISC_NONE = 0, F3D_VISC_NU = 1, F3D_VISC_ALPHA = 2 };
 enum f3d_fluidtype { F3D_GAS = 1, F3D_DUST = 2 };
+/* monitored quantities (the bit values of src/define.h:121-132) */
+#define F3D_MON_MASS 1
+#define F3D_MON_MOM_X 64
+#define F3D_MON_MOM_Y 128
+#define F3D_MON_MOM_Z 256
+#define F3D_MON_TORQ 512
 /* Boundary families (std/boundaries.txt:35-102, setups/<name>/<name>.bound.0) */
 enum f3d_bc { F3D_BC_FARGO2D = 1, F3D_BC_P3D = 2 };
 
@@ -86,6 +92,7 @@ typedef struct f3d_params {
   int dustdiffusion;           /* -DDUSTDIFFUSION (src/dust_diffusion_main.c) */
   int feedback;                /* the gas feels the drag of the dust (ColRate feedback, src/colrate.c) */
   int ic_keplerian;            /* 1: v_phi = r^-1/2 - Omega_f r, v_r = 0, noise on density only */
+  int mon_2d, mon_y_raw, mon_scalar; /* MONITOR_2D, MONITOR_Y_RAW, MONITOR_SCALAR of the .opt file */
   uint64_t seed;               /* srand48 seed for the noise (0 = glibc default state) */
   /* derived (src/param.c:236-240) */
   int corotating, guidingcenter;

@@ -158,6 +158,8 @@ void f3d_params_defaults(f3d_params *p) {
   p->nfluids = 1;
   p->ic_keplerian = 0;
   p->seed = 0;
+  p->mon_2d = p->mon_y_raw = 0;
+  p->mon_scalar = F3D_MON_MASS | F3D_MON_MOM_X | F3D_MON_TORQ;
 }
 
 /* The switches each setup's .opt file sets (setups/<name>/<name>.opt). */
@@ -168,6 +170,8 @@ int f3d_params_setup_options(f3d_params *p, const char *setup) {
     p->visc = F3D_VISC_NU; p->potential = 1; p->stockholm = 1; p->fargo = 1;
     p->bc = F3D_BC_FARGO2D;
     p->ic_keplerian = (strcmp(setup, "fargo_keplerian") == 0);
+    /* setups/fargo/fargo.opt:16-17 */
+    p->mon_scalar = F3D_MON_MASS | F3D_MON_MOM_X | F3D_MON_TORQ; p->mon_y_raw = F3D_MON_TORQ; p->mon_2d = 0;
   } else if (strcmp(setup, "fargo_adiabatic") == 0) {
     /* fargo.opt with -DADIABATIC in place of -DISOTHERMAL */
     p->geometry = F3D_CYLINDRICAL; p->dimz = 0; p->eos = F3D_ADIABATIC;
@@ -175,6 +179,7 @@ int f3d_params_setup_options(f3d_params *p, const char *setup) {
     p->bc = F3D_BC_FARGO2D;
     /* a setup-local .bound entry "Energy: Ymin/Ymax: SYMMETRIC" */
     p->energy_bc = 1;
+    p->mon_scalar = F3D_MON_MASS | F3D_MON_MOM_X | F3D_MON_TORQ; p->mon_y_raw = F3D_MON_TORQ; p->mon_2d = 0;
   } else if (strcmp(setup, "fargo_multifluid") == 0) {
     /* setups/fargo_multifluid/fargo_multifluid.opt:13-44: one gas and
        NFLUIDS-1 dust fluids, implicit drag with Stokes numbers, dust diffusion */
@@ -184,12 +189,16 @@ int f3d_params_setup_options(f3d_params *p, const char *setup) {
     p->nfluids = 4;
     p->dragforce = 1;
     p->dustdiffusion = 1;
+    /* setups/fargo_multifluid/fargo_multifluid.opt:21-22 */
+    p->mon_scalar = F3D_MON_MASS | F3D_MON_MOM_X | F3D_MON_TORQ; p->mon_y_raw = F3D_MON_TORQ; p->mon_2d = 0;
   } else if (strcmp(setup, "p3disof") == 0 || strcmp(setup, "p3disof_alpha") == 0) {
     /* setups/p3disof/p3disof.opt:15-33 (ALPHAVISCOSITY variant on request) */
     p->geometry = F3D_SPHERICAL; p->dimz = 1; p->eos = F3D_ISOTHERMAL;
     p->visc = (strcmp(setup, "p3disof_alpha") == 0) ? F3D_VISC_ALPHA : F3D_VISC_NU;
     p->potential = 1; p->stockholm = 1; p->fargo = 1;
     p->bc = F3D_BC_P3D;
+    /* setups/p3disof/p3disof.opt:19-20 */
+    p->mon_2d = F3D_MON_MASS | F3D_MON_MOM_X; p->mon_scalar = F3D_MON_TORQ; p->mon_y_raw = 0;
   } else {
     return -1;
   }

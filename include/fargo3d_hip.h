@@ -184,6 +184,23 @@ int f3d_reduction_SUM(f3d_hip_ctx *ctx, const double *F, int ymin, int ymax, int
 int f3d_reduction_MIN(f3d_hip_ctx *ctx, const double *F, int ymin, int ymax, int zmin, int zmax);
 int f3d_reduction_full_SUM(f3d_hip_ctx *ctx, const double *F, int ymin, int ymax, int zmin, int zmax,
                            double *result);
+/* Monitored quantities of the bound fluid into an FP64 array `out` (the
+ * reference writes them into Slope), active cells only, the ones the monitor
+ * reductions read (src/monitor.c:232-233): mon_dens src/mon_dens.c:10,
+ * mon_momx src/mon_momx.c:10, mon_momy src/mon_momy.c:10, mon_momz
+ * src/mon_momz.c:10, mon_torq src/mon_torque.c:10 (planet position and
+ * smoothing passed in place of the Xplanet.. globals).  The f3d32_ twins read
+ * the precision form's FP32 fields and compute in FP64. */
+int f3d_mon_dens(f3d_hip_ctx *ctx, double *out);
+int f3d_mon_momx(f3d_hip_ctx *ctx, double *out);
+int f3d_mon_momy(f3d_hip_ctx *ctx, double *out);
+int f3d_mon_momz(f3d_hip_ctx *ctx, double *out);
+int f3d_mon_torq(f3d_hip_ctx *ctx, double xp, double yp, double zp, double rsmoothing, double *out);
+int f3d32_mon_dens(f3d_hip_ctx *ctx, double *out);
+int f3d32_mon_momx(f3d_hip_ctx *ctx, double *out);
+int f3d32_mon_momy(f3d_hip_ctx *ctx, double *out);
+int f3d32_mon_momz(f3d_hip_ctx *ctx, double *out);
+int f3d32_mon_torq(f3d_hip_ctx *ctx, double xp, double yp, double zp, double rsmoothing, double *out);
 int f3d_reduction_full_MIN(f3d_hip_ctx *ctx, const double *F, int ymin, int ymax, int zmin, int zmax,
                            double *result);
 /* cfl: src/cfl.c:10 + cfl_b: src/cfl_b.c:3 (per-fluid Min, host value) */

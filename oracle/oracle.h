@@ -81,6 +81,7 @@ int orc_CflFluidsMin(orc_state *S);
 void orc_ComputeTotalDensity(orc_state *S);
 void orc_Reset_field(orc_state *S, double *F);
 void orc_ComputeForce(orc_state *S, double x, double y, double z, double rsmoothing, double mass);
+void orc_monitor(orc_state *S, int q, double Xplanet, double Yplanet, double Zplanet, double rsmoothing);
 void orc_Potential(orc_state *S);
 void orc_CorrectVtheta(orc_state *S, double domega);
 void orc_SubStep1_x(orc_state *S, double dt);

@@ -1522,3 +1522,64 @@ void orc_DustDiffusion_Main(orc_state *S, double dt) {
     S->fill_density_only = 0;
   }
 }
+
+/* src/mon_dens.c:10-68, src/mon_momx.c, src/mon_momy.c, src/mon_momz.c,
+   src/mon_torque.c: the monitored quantity into Slope over the active cells,
+   the cells the monitor reductions read (src/monitor.c:232-233).
+   q: 0 mass, 1 momx, 2 momy, 3 momz, 4 torq. */
+void orc_monitor(orc_state *S, int q, double Xplanet, double Yplanet, double Zplanet, double rsmoothing) {
+  ORC_MESH(S);
+  const double OMEGAFRAME = S->clk.OMEGAFRAME;
+  const double *dens = S->Density, *vx = S->Vx, *vy = S->Vy, *vz = S->Vz;
+  double *interm = S->Slope;
+  const double rsm2 = rsmoothing * rsmoothing;
+  const int kbeg = DIMZ ? NGHZ : 0, kend = DIMZ ? Nz + NGHZ : 1;
+  for (int k = kbeg; k < kend; k++)
+    for (int j = NGHY; j < Ny + NGHY; j++)
+      for (int i = 0; i < Nx; i++) {
+        int ll = lidx(i, j, k);
+        int lxp = (i < Nx - 1) ? ll + 1 : ll - (Nx - 1);
+        int lyp = ll + pitch, lzp = ll + stride;
+        double rcyl, dx, dy, dz = 0.0;
+        switch (q) {
+          case 0:
+            interm[ll] = dens[ll] * Vol(i, j, k);
+            break;
+          case 1:
+            if (SPH) {
+              rcyl = ymed(j) * sin(zmed(k));
+              interm[ll] = (.5 * (vx[ll] + vx[lxp]) + rcyl * OMEGAFRAME) * rcyl * dens[ll] * Vol(i, j, k);
+            } else {
+              interm[ll] = (.5 * (vx[ll] + vx[lxp]) + ymed(j) * OMEGAFRAME) * ymed(j) * dens[ll] * Vol(i, j, k);
+            }
+            break;
+          case 2:
+            interm[ll] = dens[ll] * .5 * (vy[ll] + vy[lyp]) * Vol(i, j, k);
+            break;
+          case 3:
+            interm[ll] = dens[ll] * .5 * (vz[ll] + vz[lzp]) * Vol(i, j, k);
+            if (SPH) interm[ll] *= ymed(j);
+            break;
+          default: {
+            double cellmass = Vol(i, j, k) * dens[ll];
+            if (SPH) {
+              dx = ymed(j) * cos(xmed(i)) * sin(zmed(k)) - Xplanet;
+              dy = ymed(j) * sin(xmed(i)) * sin(zmed(k)) - Yplanet;
+              if (DIMZ) dz = ymed(j) * cos(zmed(k)) - Zplanet;
+            } else {
+              dx = ymed(j) * cos(xmed(i)) - Xplanet;
+              dy = ymed(j) * sin(xmed(i)) - Yplanet;
+              if (DIMZ) dz = zmed(k) - Zplanet;
+            }
+            double dist2 = dx * dx + dy * dy + dz * dz;
+            dist2 += rsm2;
+            double distance = sqrt(dist2);
+            double InvDist3 = 1.0 / (dist2 * distance);
+            InvDist3 *= G * cellmass;
+            double fxi = dx * InvDist3;
+            double fyi = dy * InvDist3;
+            interm[ll] = Xplanet * fyi - Yplanet * fxi;
+          }
+        }
+      }
+}
