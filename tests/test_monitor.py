@@ -2,7 +2,8 @@
 src/mon_momy.c, src/mon_momz.c, src/mon_torque.c) and MonitorGlobal's sums
 and files (src/monitor.c).
 
-The CPU restatement is pinned against closed forms on the cylindrical mesh:
+The CPU restatement is pinned against closed forms on the cylindrical and
+spherical meshes:
 cell volumes (r_{j+1}^2 - r_j^2) dphi / 2 from the mesh edges, the frame term
 of the angular momentum, the face average of vy, and the point-mass torque of
 each cell computed from its Cartesian position; a periodic-neighbour slip, a
@@ -72,6 +73,58 @@ def test_oracle_monitors_closed_forms():
     o.field("Density")[0, 3 + ny // 2, nx // 2 + 2] = 1.0
     t = got(4, r[3 + ny // 2], 0.0, 0.0, rs)
     assert t.sum() > 0
+    o.close()
+
+
+def test_oracle_monitors_spherical_closed_forms():
+    """Spherical branches: volumes r^3/3 dphi dcos(theta), the cylindrical
+    radius r sin(theta) of the angular momentum, the r factor of momz and the
+    torque of cells placed at (r cos phi sin theta, r sin phi sin theta, r cos theta)."""
+    nx, ny, nz = 16, 8, 6
+    o = oracle.Oracle(C2, "Nx=%d Ny=%d Nz=%d" % (nx, ny, nz))
+    o.set_shape(nx, ny, nz, 1)
+    rng = np.random.default_rng(9)
+    rho, vx, _, vz = _state(rng, (nz + 6, ny + 6, nx))
+    o.field("Density")[...] = rho
+    o.field("Vx")[...] = vx
+    o.field("Vz")[...] = vz
+    om = o.scalars()["OMEGAFRAME"]
+    dr = (2.5 - 0.4) / ny
+    rmin = 0.4 + dr * (np.arange(ny + 7) - 3)
+    th0, th1 = 1.42079632679489661922, 1.72079632679489661922
+    dth = (th1 - th0) / nz
+    tmin = th0 + dth * (np.arange(nz + 7) - 3)
+    r = 0.5 * (rmin[1:] + rmin[:-1])
+    th = 0.5 * (tmin[1:] + tmin[:-1])
+    dphi = 2 * np.pi / nx
+    phi = -np.pi + (np.arange(nx) + 0.5) * dphi
+    vol = ((rmin[1:] ** 3 - rmin[:-1] ** 3) / 3)[None, :, None] * dphi * \
+        (np.cos(tmin[:-1]) - np.cos(tmin[1:]))[:, None, None]
+    act = (slice(3, 3 + nz), slice(3, 3 + ny))
+
+    def got(q, *pl):
+        o.call("monitor", q, *(pl or (0.0, 0.0, 0.0, 0.0)))
+        return o.field("Slope")[act].copy()
+
+    def close(a, b):
+        assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+
+    close(got(0), (rho * vol)[act])
+    rc = r[None, :, None] * np.sin(th)[:, None, None]
+    vxm = 0.5 * (vx + np.roll(vx, -1, axis=2))
+    close(got(1), ((vxm + rc * om) * rc * rho * vol)[act])
+    vzm = np.zeros_like(vz)
+    vzm[:-1] = 0.5 * (vz[:-1] + vz[1:])
+    close(got(3), (rho * vzm * vol * r[None, :, None])[act])
+    xp, yp, zp, rs = 1.0, 0.03, 0.02, 0.05
+    st = np.sin(th)[:, None, None]
+    cx = r[None, :, None] * np.cos(phi)[None, None, :] * st
+    cy = r[None, :, None] * np.sin(phi)[None, None, :] * st
+    cz = r[None, :, None] * np.cos(th)[:, None, None] + 0 * phi
+    dx, dy, dz = cx - xp, cy - yp, cz - zp
+    d2 = dx * dx + dy * dy + dz * dz + rs * rs
+    f = rho * vol / (d2 * np.sqrt(d2))
+    close(got(4, xp, yp, zp, rs), (xp * dy * f - yp * dx * f)[act])
     o.close()
 
 
