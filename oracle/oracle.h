@@ -138,6 +138,10 @@ void orc_get_scalars(orc_state *S, double *out8);
 void orc_set_scalars(orc_state *S, const double *in8);
 void orc_get_planets(orc_state *S, double *out, int maxn);
 void orc_psys_integrate(orc_state *S, double t, int nsub);
+void orc_shape(orc_state *S, int *out4);
+const double *orc_array(orc_state *S, const char *name, int *len);
+void orc_get_localforce(orc_state *S, double *out12);
+void orc_set_planet(orc_state *S, int n, const double *in7);
 
 #ifdef __cplusplus
 }

@@ -387,7 +387,8 @@ double *orc_field(orc_state *S, const char *name) {
       {"Mpx", S->Mpx}, {"Mmx", S->Mmx}, {"Mpy", S->Mpy}, {"Mmy", S->Mmy}, {"Mpz", S->Mpz}, {"Mmz", S->Mmz},
       {"Pot", S->Pot}, {"Slope", S->Slope}, {"DivRho", S->DivRho}, {"DensStar", S->DensStar},
       {"Qs", S->Qs}, {"Pressure", S->Pressure}, {"Total_Density", S->Total_Density}, {"QL", S->QL},
-      {"QR", S->QR}, {"Reduction2D", S->Reduction2D}, {"VxMed", S->VxMed}};
+      {"QR", S->QR}, {"Reduction2D", S->Reduction2D}, {"VxMed", S->VxMed},
+      {"Sdiffyczc", S->Sdiffyczc}, {"Sdiffyfzc", S->Sdiffyfzc}};
   for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); i++)
     if (strcmp(t[i].n, name) == 0) return t[i].p;
   return NULL;
@@ -405,6 +406,52 @@ void orc_set_scalars(orc_state *S, const double *in) {
   S->clk.PhysicalTime = in[0]; S->clk.OMEGAFRAME = in[1]; S->clk.OMEGAFRAME0 = in[2];
   S->clk.Domega = in[3]; S->dtemp = in[4]; S->StepTime = in[5]; S->clk.XAxisRotationAngle = in[6];
   S->clk.Timestepcount = (long)in[7];
+}
+
+/* mesh shape: nx, ny, nz, dimz */
+void orc_shape(orc_state *S, int *out) {
+  out[0] = S->g.nx; out[1] = S->g.ny; out[2] = S->g.nz; out[3] = S->g.dimz;
+}
+
+/* The host-layer arrays the oracle was built with (mesh, boundary-factor
+ * tables, damping-zone limits), so the tests can pin them against closed
+ * forms; *len receives the element count, NULL for an unknown name. */
+const double *orc_array(orc_state *S, const char *name, int *len) {
+  const f3d_grid *g = &S->g;
+  const f3d_bctab *b = &S->bct;
+  int sx = g->sx, sy = g->sy, sz = g->sz;
+  struct { const char *n; const double *p; int len; } t[] = {
+      {"xmin", g->xmin, sx + 1}, {"xmed", g->xmed, sx}, {"ymin", g->ymin, sy + 1}, {"ymed", g->ymed, sy},
+      {"zmin", g->zmin, sz + 1}, {"zmed", g->zmed, sz}, {"sxi", g->sxi, sx}, {"sxj", g->sxj, sy},
+      {"syj", g->syj, sy}, {"szj", g->szj, sy}, {"sxk", g->sxk, sz}, {"syk", g->syk, sz}, {"szk", g->szk, sz},
+      {"invvj", g->invvj, sy}, {"invdiffxmed", g->invdiffxmed, sx}, {"mindx", &g->mindx, 1},
+      {"ymin_dens", b->ymin_dens, F3D_NGH}, {"ymax_dens", b->ymax_dens, F3D_NGH},
+      {"ymin_vsq", b->ymin_vsq, F3D_NGH}, {"ymax_vsq", b->ymax_vsq, F3D_NGH},
+      {"ymin_vfac", b->ymin_vfac, F3D_NGH}, {"ymax_vfac", b->ymax_vfac, F3D_NGH},
+      {"zmin_vfac", b->zmin_vfac, F3D_NGH}, {"zmax_vfac", b->zmax_vfac, F3D_NGH},
+      {"ymin_dexp", b->ymin_dexp, b->ymin_dexp ? F3D_NGH * sz : 0},
+      {"ymax_dexp", b->ymax_dexp, b->ymax_dexp ? F3D_NGH * sz : 0},
+      {"zmin_dens", b->zmin_dens, b->zmin_dens ? F3D_NGH * sy : 0},
+      {"zmax_dens", b->zmax_dens, b->zmax_dens ? F3D_NGH * sy : 0},
+      {"damp", &S->damp.Y_inf, 4}};
+  for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); i++)
+    if (strcmp(t[i].n, name) == 0) {
+      if (len) *len = t[i].len;
+      return t[i].p;
+    }
+  if (len) *len = 0;
+  return NULL;
+}
+
+/* the 12 sums _ComputeForce leaves (src/compute_force.c:160-215) */
+void orc_get_localforce(orc_state *S, double *out12) { memcpy(out12, S->localforce, 12 * sizeof(double)); }
+
+/* planet n's state: x y z vx vy vz mass */
+void orc_set_planet(orc_state *S, int n, const double *in7) {
+  if (n < 0 || n >= S->sys.nb) return;
+  S->sys.x[n] = in7[0]; S->sys.y[n] = in7[1]; S->sys.z[n] = in7[2];
+  S->sys.vx[n] = in7[3]; S->sys.vy[n] = in7[4]; S->sys.vz[n] = in7[5];
+  S->sys.mass[n] = in7[6];
 }
 
 void orc_get_planets(orc_state *S, double *out, int maxn) {
