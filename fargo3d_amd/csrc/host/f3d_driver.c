@@ -218,6 +218,14 @@ f3d_sim *f3d_sim_create(const char *parfile, const char *overrides, char *err, i
   if (parfile && parfile[0] && f3d_params_read(&S->p, parfile, err, (size_t)errlen) != 0) goto fail;
   if (apply_overrides(&S->p, overrides, err, errlen) != 0) goto fail;
   f3d_params_finalize(&S->p);
+  if (S->p.nsnap != 0) {
+    /* NSNAP > 0 makes the reference write a SPECIFIC output every NSNAP
+       steps and renumber TimeStep (src/main.c:347-353), which the main loop
+       here does not do */
+    if (err) snprintf(err, errlen, "NSNAP=%d: snapshots every NSNAP steps are not supported (use NSNAP 0)",
+                      S->p.nsnap);
+    goto fail;
+  }
   if (f3d_grid_init(&S->g, &S->p) != 0) {
     if (err) snprintf(err, errlen, "bad mesh parameters");
     goto fail;
@@ -1050,7 +1058,10 @@ static int write_stockholm_files(f3d_sim *S, const char *dir) {
         free(h);
         ERR(S, "cannot write %s", path);
       }
-      fclose(out);
+      if (fclose(out) != 0) {
+        free(h);
+        ERR(S, "cannot write %s", path);
+      }
     }
   }
   free(h);
@@ -1058,15 +1069,14 @@ static int write_stockholm_files(f3d_sim *S, const char *dir) {
 }
 
 int f3d_sim_write_init(f3d_sim *S, const char *dir) {
-  /* EmptyPlanetSystemFiles (src/output.c:33-44) */
+  /* EmptyPlanetSystemFiles (src/output.c:33-44): planet<k>.dat only;
+     bigplanet<k>.dat is appended to and never emptied */
   char path[1024];
   for (int i = 0; i < S->sys.nb; i++) {
-    for (int big = 0; big < 2; big++) {
-      snprintf(path, sizeof(path), big ? "%s/bigplanet%d.dat" : "%s/planet%d.dat", dir, i);
-      FILE *out = fopen(path, "w");
-      if (!out) ERR(S, "cannot write %s", path);
-      fclose(out);
-    }
+    snprintf(path, sizeof(path), "%s/planet%d.dat", dir, i);
+    FILE *out = fopen(path, "w");
+    if (!out) ERR(S, "cannot write %s", path);
+    if (fclose(out) != 0) ERR(S, "cannot write %s", path);
   }
   return S->p.stockholm ? write_stockholm_files(S, dir) : 0;
 }
@@ -1197,61 +1207,6 @@ int f3d_sim_advance(f3d_sim *S) {
   } while (S->dtemp < S->p.dt_out);
   CALL(S, f3d_hip_sync(S->ctx));
   return steps;
-}
-
-/* SV2OE (src/planets.c:117-195): orbital elements of a state vector about a
-   central mass m; out = e, a, M, true anomaly, argument of periapsis,
-   inclination, ascending node, perihelion azimuth */
-int f3d_orbital_elements(const double sv[6], double m, double out[8]) {
-  const double x = sv[0], y = sv[1], z = sv[2], vx = sv[3], vy = sv[4], vz = sv[5];
-  const double gm = F3D_G * m;
-  const double d = sqrt(x * x + y * y + z * z);
-  /* specific angular momentum h = r x v */
-  const double hx = y * vz - z * vy, hy = z * vx - x * vz, hz = x * vy - y * vx;
-  const double hhor = sqrt(hx * hx + hy * hy);
-  const double h = sqrt(hx * hx + hy * hy + hz * hz);
-  const double inc = asin(hhor / h);
-  /* Laplace-Runge-Lenz vector v x h - G m r / |r| */
-  const double Ax = vy * hz - vz * hy - gm * x / d;
-  const double Ay = vz * hx - vx * hz - gm * y / d;
-  const double Az = vx * hy - vy * hx - gm * z / d;
-  const double e = sqrt(Ax * Ax + Ay * Ay + Az * Az) / gm;
-  const double a = h * h / (gm * (1. - e * e));
-  /* eccentric anomaly, negative on the way in */
-  double E = (e != 0.0) ? acos((1.0 - d / a) / e) : 0.0;
-  if (x * vx + y * vy + z * vz < 0) E = -E;
-  if (isnan(E)) E = (d < a) ? 0.0 : M_PI;
-  const double M = E - e * sin(E);
-  double V = (e > 1.e-14) ? acos((a * (1.0 - e * e) / d - 1.0) / e) : 0.0;
-  if (E < 0.0) V = -V;
-  double an = 0.0;
-  if (fabs(inc) > 1e-5) {
-    an = atan2(hy, hx) + M_PI * .5;
-    if (an > 2.0 * M_PI) an -= 2.0 * M_PI;
-  }
-  double per = acos((Ax * cos(an) + Ay * sin(an)) / sqrt(Ax * Ax + Ay * Ay + Az * Az));
-  if ((-hz * sin(an) * Ax + hz * cos(an) * Ay + (hx * sin(an) - hy * cos(an)) * Az) < 0.0) per = 2.0 * M_PI - per;
-  out[0] = e; out[1] = a; out[2] = M; out[3] = V; out[4] = per; out[5] = inc; out[6] = an;
-  out[7] = (Ax * Ax + Ay * Ay > 0.0) ? atan2(Ay, Ax) : atan2(y, x);
-  return 0;
-}
-
-/* SolveOrbits (src/planets.c:197-225): the orbit<k>.dat line of each planet */
-static int write_orbit_files(f3d_sim *S, const char *dir) {
-  char path[1024];
-  for (int i = 0; i < S->sys.nb; i++) {
-    const double sv[6] = {S->sys.x[i], S->sys.y[i], S->sys.z[i], S->sys.vx[i], S->sys.vy[i], S->sys.vz[i]};
-    double o[8];
-    f3d_orbital_elements(sv, F3D_MSTAR + S->sys.mass[i], o);
-    snprintf(path, sizeof(path), "%s/orbit%d.dat", dir, i);
-    FILE *out = fopen(path, "a");
-    if (!out) ERR(S, "cannot write %s", path);
-    fprintf(out, "%.12g\t%.12g\t%.12g\t%.12g\t%.12g\t%.12g\t%.12g", S->clk.PhysicalTime, o[0], o[1], o[2], o[3],
-            o[4], S->clk.XAxisRotationAngle);
-    fprintf(out, "\t%.12g\t%.12g\t%.12g\n", o[5], o[6], o[7]);
-    fclose(out);
-  }
-  return 0;
 }
 
 /* ---- monitoring (src/monitor.c) ---- */
@@ -1394,7 +1349,7 @@ int f3d_sim_main_loop(f3d_sim *S, int begin_i, const char *dir) {
     if ((S->p.mon_2d | S->p.mon_y_raw | S->p.mon_scalar) &&
         f3d_sim_monitor(S, S->p.mon_2d, S->p.mon_y_raw, S->p.mon_scalar, timestep, dir))
       return -1;
-    if (write_planet_files(S, timestep, dir, 1) || write_orbit_files(S, dir)) return -1;
+    if (write_planet_files(S, timestep, dir, 1)) return -1;
   }
   return 0;
 }

@@ -96,11 +96,6 @@ int f3d_sim_monitor(f3d_sim *S, int m2d, int myraw, int mscalar, int timestep, c
 /* the main loop from begin_i to NTOT with outputs every NINTERM passes
  * (src/main.c:324-356) */
 int f3d_sim_main_loop(f3d_sim *S, int begin_i, const char *dir);
-/* SV2OE (src/planets.c:117-195) of a state vector sv = x, y, z, vx, vy, vz
- * about the mass m: out = e, a, M, true anomaly, argument of periapsis,
- * inclination, ascending node, perihelion azimuth (the orbit<k>.dat columns
- * that main_loop writes after each pass, src/planets.c:197-225) */
-int f3d_orbital_elements(const double sv[6], double m, double out[8]);
 /* the file layout on plain host arrays (pitch, stride in elements; a 2-D
  * grid passes nz = 1, nghz = 0); read fills the active cells only and fails
  * unless the file holds exactly nx*ny*nz doubles */

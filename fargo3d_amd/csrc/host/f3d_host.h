@@ -72,6 +72,7 @@ typedef struct f3d_params {
   /* outputs / loop */
   double dt_out;               /* DT: physical time between outputs */
   int ninterm, ntot;
+  int nsnap;                   /* NSNAP: snapshots every NSNAP steps (0: none) */
   char funcarchfile[256];
   char outputdir[256];
   /* .opt switches (compile-time in FARGO3D, run-time here) */

@@ -69,6 +69,7 @@ static const var_t VARS[] = {
     {"DT", T_REAL, OFF(dt_out), 0},
     {"NINTERM", T_INT, OFF(ninterm), 0},
     {"NTOT", T_INT, OFF(ntot), 0},
+    {"NSNAP", T_INT, OFF(nsnap), 0},
     {"FUNCARCHFILE", T_STR, OFF(funcarchfile), SLEN(funcarchfile)},
     {"OUTPUTDIR", T_STR, OFF(outputdir), SLEN(outputdir)},
     /* .opt switches, settable at run time */
@@ -93,7 +94,7 @@ static const char *IGNORED[] = {"XMA", "XMB", "XMC", "YMA", "YMB", "YMC", "YMY0"
                                 "HALLEFFECTCOEFF", "AMBIPOLARDIFFUSIONCOEFF", "RESONANCE",
                                 "BETA", "VTK", "REALTYPE", "FIELD", "CMAP", "PLOTLOG",
                                 "COLORBAR", "AUTOCOLOR", "ASPECT", "VMIN", "VMAX", "PLOTLINE",
-                                "NSNAP", "WRITEDENSITY", "WRITEENERGY", "WRITEBX", "WRITEBY",
+                                "WRITEDENSITY", "WRITEENERGY", "WRITEBX", "WRITEBY",
                                 "WRITEBZ", "WRITEVX", "WRITEVY", "WRITEVZ", "WRITEDIVERGENCE",
                                 "WRITEENERGYRAD", "WRITETAU", "DISK", "INVSTOKES1",
                                 "INVSTOKES2", "INVSTOKES3", "EPSILON", "END", NULL};

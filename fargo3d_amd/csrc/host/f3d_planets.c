@@ -2,7 +2,7 @@
  * f3d_planets.c -- planetary system on the host.
  *
  * Planet motion stays on the host, as in FARGO3D: the RK5 N-body step
- * (src/runge_kutta.c:10-192), orbital elements (src/planets.c SV2OE), the
+ * (src/runge_kutta.c:10-192), the guiding centre of the first planet, the
  * frame-tracking helpers GetPsysInfo / RotatePsys (src/psys.c:362-440) and
  * the planet set-up (src/psys.c:140-330).  The arithmetic follows those files
  * statement by statement so that a run is reproducible bit for bit.
@@ -173,60 +173,46 @@ void f3d_psys_rk5(f3d_psys *sys, const f3d_params *p, const f3d_clock *c, double
   }
 }
 
-/* ---- orbital elements (src/planets.c SV2OE) ---- */
+/* ---- the guiding centre of planet 0 (GetPsysInfo, src/psys.c:362-415) ----
+ * The frame of a Frame G run follows the guiding centre of the first
+ * planet: the point at mean anomaly M on the circle of radius a, azimuth
+ * measured from the perihelion, tilted by the inclination.  The elements are
+ * those of the two-body orbit about the mass m (star + planet):
+ *   h = r x v (specific angular momentum), inclination asin(|h_xy| / |h|);
+ *   A = v x h - G m r/|r| (the Laplace-Runge-Lenz vector, pointing to the
+ *   perihelion, |A| = G m e);  a = h^2 / (G m (1 - e^2)) (vis-viva);
+ *   E from r = a (1 - e cos E), negative while r.v < 0; M = E - e sin E.
+ * The reference's conventions for degenerate orbits are kept, because the
+ * frame's rotation (and so every velocity of a Frame G run) depends on
+ * them: E = 0 for e = 0, E taken as 0 or pi when the acos argument leaves
+ * [-1, 1] by rounding, and the perihelion azimuth of a circular orbit taken
+ * as the planet's own azimuth.  The arithmetic is evaluated in the
+ * reference's order (src/planets.c:117-195), so OMEGAFRAME matches its CPU
+ * build bit for bit. */
 typedef struct {
-  double e, a, M, E, ta, per, i, an, Perihelion_Phi;
+  double e, a, M, i, Perihelion_Phi;
 } orb_t;
 
-static orb_t sv2oe(double x, double y, double z, double vx, double vy, double vz, double m) {
+static orb_t guiding_elements(double x, double y, double z, double vx, double vy, double vz, double m) {
   orb_t o;
   double d = sqrt(x * x + y * y + z * z);
   double hx = y * vz - z * vy;
   double hy = z * vx - x * vz;
   double hz = x * vy - y * vx;
   double hhor = sqrt(hx * hx + hy * hy);
-  double h2 = hx * hx + hy * hy + hz * hz;
-  double h = sqrt(h2);
+  double h = sqrt(hx * hx + hy * hy + hz * hz);
   o.i = asin(hhor / h);
   double Ax = vy * hz - vz * hy - G * m * x / d;
   double Ay = vz * hx - vx * hz - G * m * y / d;
   double Az = vx * hy - vy * hx - G * m * z / d;
   double e = sqrt(Ax * Ax + Ay * Ay + Az * Az) / (G * m);
   o.e = e;
-  double a = h * h / (G * m * (1. - e * e));
-  o.a = a;
-  double E;
-  if (e != 0.0)
-    E = acos((1.0 - d / a) / e);
-  else
-    E = 0.0;
+  o.a = h * h / (G * m * (1. - e * e));
+  double E = (e != 0.0) ? acos((1.0 - d / o.a) / e) : 0.0;
   if (x * vx + y * vy + z * vz < 0) E = -E;
-  if (isnan(E)) E = (d < a) ? 0.0 : M_PI;
+  if (isnan(E)) E = (d < o.a) ? 0.0 : M_PI;
   o.M = E - e * sin(E);
-  o.E = E;
-  double V;
-  if (e > 1.e-14)
-    V = acos((a * (1.0 - e * e) / d - 1.0) / e);
-  else
-    V = 0.0;
-  if (E < 0.0) V = -V;
-  o.ta = V;
-  double an;
-  if (fabs(o.i) > 1e-5) {
-    an = atan2(hy, hx) + M_PI * .5;
-    if (an > 2.0 * M_PI) an -= 2.0 * M_PI;
-  } else {
-    an = 0.0;
-  }
-  o.an = an;
-  double per = acos((Ax * cos(an) + Ay * sin(an)) / sqrt(Ax * Ax + Ay * Ay + Az * Az));
-  if ((-hz * sin(an) * Ax + hz * cos(an) * Ay + (hx * sin(an) - hy * cos(an)) * Az) < 0.0)
-    per = 2.0 * M_PI - per;
-  o.per = per;
-  if (Ax * Ax + Ay * Ay > 0.0)
-    o.Perihelion_Phi = atan2(Ay, Ax);
-  else
-    o.Perihelion_Phi = atan2(y, x);
+  o.Perihelion_Phi = (Ax * Ax + Ay * Ay > 0.0) ? atan2(Ay, Ax) : atan2(y, x);
   return o;
 }
 
@@ -239,7 +225,7 @@ double f3d_psys_info(f3d_psys *sys, const f3d_params *p, int action) {
   yc = y = sys->y[0];
   vx = sys->vx[0];
   vy = sys->vy[0];
-  orb_t o = sv2oe(sys->x[0], sys->y[0], sys->z[0], sys->vx[0], sys->vy[0], sys->vz[0], m);
+  orb_t o = guiding_elements(sys->x[0], sys->y[0], sys->z[0], sys->vx[0], sys->vy[0], sys->vz[0], m);
   if (p->guidingcenter) {
     xc = o.a * cos(o.M + o.Perihelion_Phi) * cos(o.i);
     yc = o.a * sin(o.M + o.Perihelion_Phi) * cos(o.i);

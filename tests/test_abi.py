@@ -68,3 +68,13 @@ def test_par_reader_and_overrides():
         oracle.Oracle("fargo3d_amd/setups/c1_fargo2d.par", "NoSuchParameter=1")
     with pytest.raises(RuntimeError):
         oracle.Oracle("fargo3d_amd/setups/does_not_exist.par")
+
+
+def test_nsnap_is_refused(lib):
+    """NSNAP > 0 (a SPECIFIC output every NSNAP steps with TimeStep renumbered,
+    src/main.c:347-353) is not implemented by the main loop: the set-up is
+    refused with a message naming it, before any device is touched."""
+    err = ctypes.create_string_buffer(512)
+    h = lib.f3d_sim_create(b"fargo3d_amd/setups/c1_fargo2d.par", b"Nx=32 Ny=20 Nsnap=5", err, 512)
+    assert not h
+    assert b"NSNAP" in err.value

@@ -196,11 +196,6 @@ def test_monitor_files(tmp_path, par, ov):
     assert torq[-1, 0] == pytest.approx(s.scalars()["PhysicalTime"], rel=1e-11)
     big = open(os.path.join(d, "bigplanet0.dat")).read().splitlines()
     assert [int(x.split("\t")[0]) for x in big] == [0, 1]
-    # orbit0.dat: time, e, a, ... of a planet on a near-circular orbit at r = 1
-    orb = np.loadtxt(os.path.join(d, "orbit0.dat"))
-    assert orb.shape == (2, 10)
-    assert orb[-1, 0] == pytest.approx(s.scalars()["PhysicalTime"], rel=1e-11)
-    assert orb[-1, 2] == pytest.approx(1.0, rel=0.02) and 0.0 <= orb[-1, 1] < 0.05
     if s.dimz:
         # p3disof: MASS and MOM_X as 2-D maps per pass, TORQ as a scalar
         m2 = np.fromfile(os.path.join(mon, "FG000001", "mass_2d_0000001.dat"))

@@ -21,6 +21,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 KER = "oracle/oracle_kernels.c"
 MESH = "fargo3d_amd/csrc/host/f3d_mesh.c"
+PLAN = "fargo3d_amd/csrc/host/f3d_planets.c"
 SETUP = "fargo3d_amd/csrc/host/f3d_setup.c"
 
 # (file, text as in the source, replacement, which occurrence (0-based), label)
@@ -123,6 +124,11 @@ MUTANTS = [
      "viscosity = ALPHA * cs[ll] * sqrt(ymed(j) * ymed(j) * ymed(j) / (G * MSTAR));", 0, "cfl alpha viscosity cs^2"),
     (KER, "double cfl3 = (max2(fabs(vy[ll]), fabs(vy[llyp]))) / zone_size_y(j, k);",
      "double cfl3 = (max2(fabs(vy[ll]), fabs(vy[llyp]))) / zone_size_x(i, j, k);", 0, "cfl vy over dx"),
+    (PLAN, "o.a = h * h / (G * m * (1. - e * e));", "o.a = h * h / (G * m * (1. + e * e));", 0,
+     "guiding centre semi-major axis"),
+    (PLAN, "return sqrt(G * m / pow(o.a, 3.0));", "return sqrt(G * m / pow(o.a, 2.0));", 0, "frame mean motion exponent"),
+    (PLAN, "sqrt((1.0 - p->eccentricity) / (1.0 + p->eccentricity)) * cos(p->inclination);",
+     "sqrt((1.0 + p->eccentricity) / (1.0 - p->eccentricity)) * cos(p->inclination);", 0, "planet apocentre speed"),
     (MESH, "g->szj[j] = 0.5 * (g->ymin[j + 1] * g->ymin[j + 1] - g->ymin[j] * g->ymin[j]);\n      g->invvj[j] = 1.0 / g->szj[j];",
      "g->szj[j] = 0.5 * (g->ymin[j + 1] * g->ymin[j + 1] - g->ymin[j] * g->ymin[j]);\n      g->invvj[j] = 2.0 / g->szj[j];",
      0, "mesh cylindrical InvVj factor"),

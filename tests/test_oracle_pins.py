@@ -1184,3 +1184,33 @@ def test_dust_diffusion_coefficient_is_alpha_cs2_over_omega():
     assert np.allclose(o.field("Sdiffyczc")[0], 3e-3 * cs[0] ** 2 * (r ** 1.5)[:, None], rtol=1e-13)
     csf = 0.5 * (cs[0, 1:] + cs[0, :-1])
     assert np.allclose(o.field("Sdiffyfzc")[0, 1:], 3e-3 * csf ** 2 * (rf[1:] ** 1.5)[:, None], rtol=1e-13)
+
+
+@pytest.mark.parametrize("ecc", [0.0, 0.1, 0.3])
+def test_frame_rotation_rates(ecc):
+    """OMEGAFRAME at start (src/main.c:238-242 with GetPsysInfo FREQUENCY,
+    src/psys.c:362-415): a guiding-centre frame (Frame G) turns at the mean
+    motion sqrt(G (M* + m)/a^3) of the first planet whatever its
+    eccentricity; a corotating frame (Frame C) at the planet's own angular
+    velocity, here at apocentre r = a (1 + e) with the vis-viva speed."""
+    m, a = 1e-3, 1.3
+    og = orc(C1, "Nx=8 Ny=20 Frame=G OrbitalRadius=1.3 Eccentricity=%r" % ecc).scalars()["OMEGAFRAME"]
+    assert og == pytest.approx(math.sqrt((1 + m) / a ** 3), rel=1e-13)
+    oc = orc(C1, "Nx=8 Ny=20 Frame=C OrbitalRadius=1.3 Eccentricity=%r" % ecc).scalars()["OMEGAFRAME"]
+    v = math.sqrt((1 + m) / a * (1 - ecc) / (1 + ecc))
+    assert oc == pytest.approx(v / (a * (1 + ecc)), rel=1e-13)
+
+
+def test_guiding_centre_frame_follows_the_mean_motion():
+    """Over a few orbital periods without disk forces the guiding-centre
+    frame of an eccentric planet advances at the mean motion: the planet's
+    frame-relative azimuth oscillates (the epicycle) but does not drift."""
+    o = orc(C1, "Nx=8 Ny=20 Frame=G Eccentricity=0.2 PlanetMass=1e-12")
+    L = o.L
+    L.orc_psys_integrate.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+    # the planets alone, integrated one orbital period of the guiding centre
+    p0 = o.planets()[0].copy()
+    period = 2 * math.pi / math.sqrt(1 + 1e-12)
+    L.orc_psys_integrate(o._s, period, 4000)
+    p1 = o.planets()[0]
+    assert abs(p1[0] - p0[0]) < 1e-8 and abs(p1[1] - p0[1]) < 1e-8

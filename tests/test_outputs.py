@@ -68,34 +68,6 @@ def test_active_file_layout(fa, tmp_path, dimz):
                              str(tmp_path / "none.dat").encode()) == -1
 
 
-def _elements(fa, sv, m=1.0):
-    L = fa.lib()
-    L.f3d_orbital_elements.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p]
-    a = np.ascontiguousarray(sv, dtype=np.float64)
-    out = np.zeros(8)
-    assert L.f3d_orbital_elements(a.ctypes.data, m, out.ctypes.data) == 0
-    return dict(zip(["e", "a", "M", "ta", "per", "i", "an", "phi"], out))
-
-
-def test_orbital_elements_closed_forms(fa):
-    """SV2OE (src/planets.c:117-195): a circular orbit, an ellipse at
-    perihelion and half an orbit later, and an inclined circle."""
-    o = _elements(fa, [1.0, 0, 0, 0, 1.0, 0])
-    assert abs(o["e"]) < 1e-15 and o["a"] == pytest.approx(1.0, rel=1e-15) and o["i"] == 0.0
-    a, e, gm = 2.0, 0.3, 1.5
-    rp, vp = a * (1 - e), np.sqrt(gm * (1 + e) / (a * (1 - e)))
-    o = _elements(fa, [0, rp, 0, -vp, 0, 0], m=gm)            # perihelion on +y, moving prograde
-    assert o["e"] == pytest.approx(e, rel=1e-13) and o["a"] == pytest.approx(a, rel=1e-13)
-    assert abs(o["M"]) < 1e-7 and abs(o["ta"]) < 1e-7
-    assert o["phi"] == pytest.approx(np.pi / 2, rel=1e-12)      # perihelion azimuth
-    ra, va = a * (1 + e), np.sqrt(gm * (1 - e) / (a * (1 + e)))
-    o = _elements(fa, [0, -ra, 0, va, 0, 0], m=gm)             # aphelion: E = pi, true anomaly pi
-    assert o["M"] == pytest.approx(np.pi, rel=1e-7) and o["ta"] == pytest.approx(np.pi, rel=1e-7)
-    inc = 0.2
-    o = _elements(fa, [1.0, 0, 0, 0, np.cos(inc), np.sin(inc)])
-    assert o["i"] == pytest.approx(inc, rel=1e-13) and abs(o["e"]) < 1e-14
-
-
 def _active(s, x):
     return x[3:-3, 3:-3, :] if s.dimz else x[:, 3:-3, :]
 
