@@ -78,3 +78,17 @@ def test_nsnap_is_refused(lib):
     h = lib.f3d_sim_create(b"fargo3d_amd/setups/c1_fargo2d.par", b"Nx=32 Ny=20 Nsnap=5", err, 512)
     assert not h
     assert b"NSNAP" in err.value
+
+
+def test_binding_compiles_against_the_pointer_types():
+    """tests/bind/hip_bind.c assigns every *_hip wrapper to a pointer of the
+    reference's table (restated from src/global.h:303-416); with
+    -Werror=incompatible-pointer-types a wrapper whose type differs from its
+    pointer's fails to compile."""
+    d = os.path.join(ROOT, "tests", "bind")
+    r = subprocess.run(["gcc", "-fsyntax-only", "-std=gnu11", "-Wall", "-Werror=incompatible-pointer-types",
+                        "-Werror=implicit-function-declaration", "-I" + os.path.join(ROOT, "include"),
+                        "-I" + os.path.join(ROOT, "fargo3d_amd", "csrc", "host"),
+                        os.path.join(d, "hip_bind.c"), os.path.join(d, "fargo_host.c")],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout
