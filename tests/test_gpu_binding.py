@@ -43,6 +43,7 @@ def run_bound(path, par, ov):
     L.bind_step.restype = ctypes.c_double
     L.bind_error.restype = ctypes.c_char_p
     L.bind_field.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
+    L.bind_planets.argtypes = [ctypes.c_void_p]
     err = ctypes.create_string_buffer(512)
     assert L.bind_setup(par.encode(), ov.encode(), err, 512) == 0, err.value
     return L
