@@ -1990,8 +1990,7 @@ static int cfl_mode(const dmesh& m, int nf) {
 // reads ring j+1), for the CFL kernel's pre-pass form: one wave per ring, four
 // for rings of 1024 cells or more (measured on MI355X: their loads, issued
 // by one wave, would take longer than the sum)
-static int vmed_prepass(f3d_hip_ctx* ctx, Scratch* S, const real* v) {
-  const dmesh& m = S->m;
+static int vmed_prepass_m(f3d_hip_ctx* ctx, const dmesh& m, const real* v) {
   const int nry = m.ny + (ctx->ph.fargo ? 1 : 0);
   const int nr = nry * (m.dimz ? m.nz : 1);
   if (m.nx >= 1024)
@@ -2000,6 +1999,7 @@ static int vmed_prepass(f3d_hip_ctx* ctx, Scratch* S, const real* v) {
     FDISPATCH_T(ctx, kf_vmed_rings, 0, dim3(nr), dim3(64), m.nx * sizeof(double), v, ctx->f.VxMed, nry);
   return 0;
 }
+static int vmed_prepass(f3d_hip_ctx* ctx, Scratch* S, const real* v) { return vmed_prepass_m(ctx, S->m, v); }
 
 // The arrays one fluid's transport writes between its kernels: the Y-swept
 // density and energy, the swept momenta (Mpx, Mmx, Mpy, Mmy, Mpz, Mmz), the

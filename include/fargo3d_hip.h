@@ -290,6 +290,16 @@ int f3d_fused_cfl_fluids(f3d_hip_ctx *ctx, int nf, double *const *cs, double *co
 int f3d_fused_cfl_fluids_async(f3d_hip_ctx *ctx, int nf, double *const *cs, double *const *rho,
                                double *const *vx, double *const *vy, double *const *vz, const double *vxm);
 int f3d_fused_cfl_wait(f3d_hip_ctx *ctx, double *step_min);
+/* f3d_fused_cfl_fluids with no allocation (no counterpart in the reference;
+ * for callers that cannot allocate device memory: code under stream capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and the
+ * same time step bit for bit (vxm must not be NULL); the per-ring minima and
+ * result slots live in a fixed device array of the library, at most 262144
+ * rings x fluids (larger meshes return -2 with a message), shared by every
+ * call on a GPU: calls that overlap on one GPU race.  Synchronous. */
+int f3d_fused_cfl_fluids_noalloc(f3d_hip_ctx *ctx, int nf, double *const *cs, double *const *rho,
+                                 double *const *vx, double *const *vy, double *const *vz, const double *vxm,
+                                 double *step_min);
 
 /* ---- coupling of several fluids (src/main.c:412-418) ---- */
 /* Collisions(dt, 1): src/collisions.c:156 (+ _collisions_cpu :10, ColRate
