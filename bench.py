@@ -10,11 +10,17 @@ step, sources, transport, damping and ghost fill, including the one device->
 host read of dt the host needs for the planet integrator.  Fields are device
 resident before the timed region starts.
 
-Prints one JSON line: the FP64 step's cell-updates/s, the HBM roofline of the
-stage with the largest time (algorithmic bytes per launch over its HIP-event
-duration, with the traffic rocprofv3 measured for it), the same for the whole
-step, the per-stage averages, the precision, batched and multifluid forms and
-the CPU baseline (the oracle on one host core).
+Prints one JSON line: the FP64 step's cell-updates/s; the HBM roofline of the
+stage with the largest time (the step's algorithmic bytes per cell-update,
+B_alg, times the cell-updates of one launch, over its HIP-event duration;
+beside it the same with the kernel's own algorithmic bytes, and the traffic
+rocprofv3 measured for it); the same for the whole step; the per-stage
+averages; the precision form (speed and per-field error against the FP64
+path) and the batched form (64 uneven planet-disk runs over 1000 steps,
+against looping the FP64 path, with its error); the other workloads at their
+full sizes (the 2-D 384x128 disk, the 2-D adiabatic 4096x1024 disk in FP64 and
+in the precision form, gas + 4 dust 1024x512 in both), each with its own
+roofline; and the CPU baseline (the oracle on one host core).
 """
 import argparse
 import ctypes
@@ -30,7 +36,6 @@ os.chdir(ROOT)
 HBM_PEAK_GBS = 8000.0
 PAR = {"c1": "fargo3d_amd/setups/c1_fargo2d.par", "c2": "fargo3d_amd/setups/c2_p3d.par",
        "c3": "fargo3d_amd/setups/c3_adiabatic2d.par", "c4": "fargo3d_amd/setups/c4_multifluid2d.par"}
-PAR_MULTIFLUID = PAR["c4"]
 WORKLOAD = {
     "c1": "2-D polar isothermal disk 384x128, FP64, FARGO, nu=1e-5, 1e-3 Mstar planet",
     "c2": "3-D spherical isothermal disk 256x96x32, FP64, FARGO, alpha=1e-3, 3e-4 Mstar planet",
@@ -70,12 +75,16 @@ STAGE_KERNEL = {"cfl": ["kf_cfl", "kf_cfl_fluids", "kf_vmed_rings", "kf_cfl_fin"
 PROFILE = {"c2": "profiles/fused_step_c2_3d_256x96x32.md", "c1": "profiles/fused_step_c1_2d_384x128.md",
            "c3": "profiles/fused_step_c3_2d_adiabatic_4096x1024.md",
            "c4": "profiles/fused_step_c4_multifluid_1024x512.md"}
+PROFILE32 = {"c2": "profiles/fused_step_fp32_c2_3d_256x96x32.md",
+             "c3": "profiles/fused_step_fp32_c3_2d_adiabatic_4096x1024.md"}
+# stages whose one launch covers every fluid
+FLUID_BATCHED = {"cfl", "sweep_x", "finish", "sweep_y", "sources_b"}
 
 
-def measured_traffic(config, stage):
+def measured_traffic(config, stage, esize=8):
     """HBM bytes per launch of the stage's kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (scaled by a copy of known size), or None."""
-    path = os.path.join(ROOT, PROFILE.get(config, ""))
+    path = os.path.join(ROOT, (PROFILE if esize == 8 else PROFILE32).get(config, ""))
     if not os.path.isfile(path):
         return None
     for line in open(path):
@@ -163,19 +172,58 @@ def timed_run(fa, L, names, parfile, overrides, arch, steps, warmup, probe=20):
     return elapsed, stages, sim
 
 
-def roofline_of(stages, cells, dimz, ad, alpha, esize, config):
-    """Roofline figures of the stage with the largest total time."""
+def b_alg_arrays(dimz, ad, nfl=1):
+    """Arrays per cell-update of the compulsory state traffic: rho, v (, e)
+    read and written, the sound speed read (isothermal); with several fluids,
+    per fluid-cell update, the gas's arrays and each pressureless dust
+    fluid's rho, v read and written, averaged."""
+    nv = 3 if dimz else 2
+    n = 2 * (1 + nv + ad) + (0 if ad else 1)
+    if nfl > 1:
+        n = (n + (nfl - 1) * 2 * (1 + nv)) / nfl
+    return n
+
+
+def roofline_of(stages, cells, dimz, ad, alpha, esize, config, b_alg, nfl=1):
+    """Roofline figures of the stage with the largest total time: achieved =
+    B_alg (the step's algorithmic bytes per cell-update) x the cell-updates
+    one launch performs / its average HIP-event duration; the same with the
+    kernel's own algorithmic bytes (every array it must read and write once)
+    beside it; traffic = its HBM bytes per launch from the committed rocprofv3
+    counters."""
     if not stages:
         return None
     doms = [n for n in stages if stages[n].get("dominant")]
     dom = doms[0] if doms else max(stages, key=lambda n: stages[n]["avg_us"] * stages[n]["launches"])
-    per_cell = esize * stage_arrays(dom, dimz, ad, alpha)
     avg_s = stages[dom]["avg_us"] * 1e-6
-    achieved = per_cell * cells / avg_s / 1e9 if avg_s > 0 else 0.0
+    units = cells * nfl if dom in FLUID_BATCHED else cells
+    achieved = b_alg * units / avg_s / 1e9 if avg_s > 0 else 0.0
+    per_cell = esize * stage_arrays(dom, dimz, ad, alpha)
+    k_achieved = per_cell * units / avg_s / 1e9 if avg_s > 0 else 0.0
     return {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(config, dom) if esize == 8 else None,
-            "bytes_per_cell": per_cell, "bytes_per_launch": per_cell * cells}
+            "traffic": measured_traffic(config, dom, esize),
+            "bytes_per_cell_update": b_alg, "cell_updates_per_launch": units, "avg_us": stages[dom]["avg_us"],
+            "kernel_bytes_per_cell": per_cell, "kernel_achieved": round(k_achieved, 1),
+            "kernel_frac": round(k_achieved / HBM_PEAK_GBS, 4)}
+
+
+def workload_run(fa, L, names, config, overrides, steps, warmup, esize):
+    """cell-updates/s of one workload (fluid-cell-updates with several
+    fluids), its step roofline under B_alg and the dominant stage's roofline."""
+    el, st, sim = timed_run(fa, L, names, PAR[config], overrides, {}, steps, warmup)
+    cells = sim.nx * sim.ny * (sim.nz if sim.dimz else 1)
+    nfl, ad, dimz = sim.nfluids, 1 if sim.adiabatic else 0, sim.dimz
+    sim.close()
+    v = cells * nfl * steps / el
+    b_alg = esize * b_alg_arrays(dimz, ad, nfl)
+    return {"workload": WORKLOAD[config], "value": round(v, 1),
+            "unit": "fluid-cell-updates/s" if nfl > 1 else "cell-updates/s", "cells": cells, "fluids": nfl,
+            "steps": steps, "ms_per_step": round(1e3 * el / steps, 4),
+            "step_roofline": {"bytes_per_cell_update": b_alg, "achieved": round(v * b_alg / 1e9, 1),
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(v * b_alg / 1e9 / HBM_PEAK_GBS, 4)},
+            "roofline": roofline_of(st, cells, dimz, ad, 1 if config == "c2" else 0, esize, config, b_alg, nfl),
+            "stages": st}
 
 
 def fp32_errors(fa, parfile, steps):
@@ -199,29 +247,14 @@ def fp32_errors(fa, parfile, steps):
     return out
 
 
-def batch_specs(n, seed=20240605):
-    """n 2-D isothermal disk set-ups with uneven grids and seeded parameters:
-    Nx in {256,384,768,1536}, Ny in {128,192,384}, planet mass log-uniform in
-    [1e-5, 3e-3], viscosity log-uniform in [1e-6, 1e-4], aspect ratio uniform
-    in [0.03, 0.08]."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    specs = []
-    for _ in range(n):
-        nx = int(rng.choice([256, 384, 768, 1536]))
-        ny = int(rng.choice([128, 192, 384]))
-        mp = 10 ** rng.uniform(np.log10(1e-5), np.log10(3e-3))
-        nu = 10 ** rng.uniform(-6, -4)
-        h = rng.uniform(0.03, 0.08)
-        specs.append((PAR["c1"], "Nx=%d Ny=%d PlanetMass=%.6e Nu=%.6e AspectRatio=%.6f" % (nx, ny, mp, nu, h)))
-    return specs
-
-
-def batched_form(fa, n, steps, loop_steps):
+def batched_form(fa, n, steps, loop_steps, err_every=4):
     """Aggregate cell-updates/s of n disks advanced together (FP32, one graph
-    replay per step, autotuned tiles) and of looping the plain FP64 path over
-    the same disks."""
-    specs = batch_specs(n)
+    replay per step, autotuned tiles, planets integrated on the device) and of
+    looping the plain FP64 path over the same disks; then the FP32 form's
+    error against the FP64 path on every err_every-th disk at equal step
+    count (each run on its own time steps)."""
+    import numpy as np
+    specs = fa.sweep_specs(n)
     b = fa.Batch(specs)
     cells = sum(b.cells)
     b.step(5)
@@ -233,8 +266,22 @@ def batched_form(fa, n, steps, loop_steps):
     b.step(steps)
     b.sync()
     el = time.perf_counter() - t0
-    b.close()
     v = cells * steps / el
+    done = 10 + steps
+    errs = {"Density": [], "Vx": []}
+    tdiff = []
+    for k in range(0, n, err_every):
+        s = fa.Simulation(*specs[k])
+        for _ in range(done):
+            s.step()
+        t64 = s.scalars()["PhysicalTime"]
+        tdiff.append(abs(b.clock(k)["PhysicalTime"] - t64) / t64)
+        for name in errs:
+            a = b.instances[k].field(name)[:, 3:-3, :]
+            c = s.field(name)[:, 3:-3, :]
+            errs[name].append((float(np.abs(a - c).sum() / np.abs(c).sum()), float(np.abs(a - c).max() / np.abs(c).max())))
+        s.close()
+    b.close()
     # the plain FP64 path, one disk after the other
     t_loop = 0.0
     for par, ov in specs:
@@ -249,40 +296,21 @@ def batched_form(fa, n, steps, loop_steps):
         t_loop += time.perf_counter() - t1
         s.close()
     v_loop = cells * loop_steps / t_loop
+    error = {"steps": done, "instances": len(tdiff), "time_rel_max": float("%.2e" % max(tdiff))}
+    for name, e in errs.items():
+        l1 = [x[0] for x in e]
+        error[name] = {"L1_max": float("%.3e" % max(l1)), "L1_median": float("%.3e" % float(np.median(l1))),
+                       "Linf_max": float("%.3e" % max(x[1] for x in e))}
     return {"value": round(v, 1), "unit": "cell-updates/s", "instances": n, "cells": cells, "steps": steps,
-            "ms_per_step": round(1e3 * el / steps, 4), "dtype": "f32 (f64 sums, residual, cfl)",
+            "ms_per_step": round(1e3 * el / steps, 4), "dtype": "f32 (f64 sums, residual, cfl, planets)",
             "tuned": {"rows_y": rows_y, "x_tile": tile},
             "step_roofline": {"bytes_per_cell_update": 28, "achieved": round(v * 28 / 1e9, 1), "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": round(v * 28 / 1e9 / HBM_PEAK_GBS, 4)},
             "fp64_loop": {"value": round(v_loop, 1), "steps_per_instance": loop_steps},
             "speedup_vs_fp64_loop": round(v / v_loop, 2),
-            "workload": "%d 2-D isothermal disks, Nx in {256,384,768,1536} x Ny in {128,192,384}, seeded planet "
-                        "mass, viscosity and aspect ratio; planets held on their orbits" % n}
-
-
-def multifluid_form(fa, L, names, steps, warmup):
-    """The 2-D disk of one gas and four dust fluids (1024x512, implicit drag,
-    dust diffusion), FP64 and FP32: fluid-cell-updates/s (active cells x
-    fluids x steps / wall time).  Bytes per cell: the gas reads and writes
-    rho, vx, vy and reads cs (7 arrays), each dust fluid reads and writes rho,
-    vx, vy (6 arrays)."""
-    out = {"workload": "2-D polar disk, gas + 4 dust fluids (St 1e-3..1), 1024x512, FARGO, implicit drag with "
-                       "feedback, dust diffusion (alpha 1e-4), 1e-3 Mstar planet"}
-    for tag, ov, es in (("fp64", "", 8), ("fp32", "Precision=32", 4)):
-        el, st, sim = timed_run(fa, L, names, PAR_MULTIFLUID, ov, {}, steps, warmup)
-        nf = sim.nfluids
-        cells = sim.nx * sim.ny
-        sim.close()
-        v = cells * nf * steps / el
-        b_cell = es * (7 + 6 * (nf - 1))
-        gbs = cells * steps / el * b_cell / 1e9
-        out[tag] = {"value": round(v, 1), "unit": "fluid-cell-updates/s", "fluids": nf, "cells": cells,
-                    "steps": steps, "ms_per_step": round(1e3 * el / steps, 4),
-                    "step_roofline": {"bytes_per_cell": b_cell, "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
-                                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)},
-                    "stages": st}
-    out["fp32"]["speedup_vs_fp64"] = round(out["fp32"]["value"] / out["fp64"]["value"], 3)
-    return out
+            "error_vs_fp64": error,
+            "workload": "%d 2-D isothermal planet-disk runs, Nx in {256,384,768,1536} x Ny in {128,192,384}, seeded "
+                        "planet mass, viscosity and aspect ratio; planets and frame integrated on the device" % n}
 
 
 def output_rate(sim, reps=3):
@@ -319,9 +347,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-fp32", action="store_true", help="skip the reduced-precision form")
     ap.add_argument("--batch", type=int, default=64, help="instances of the batched form (0: skip it)")
-    ap.add_argument("--batch-steps", type=int, default=200)
-    ap.add_argument("--multifluid-steps", type=int, default=100,
-                    help="timed steps of the multifluid form (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--no-workloads", dest="workloads", action="store_false",
+                    help="skip the other workloads (2-D disk, large adiabatic disk, gas + dust)")
+    ap.add_argument("--workload-steps", type=int, default=200)
     ap.add_argument("--error-steps", type=int, default=200,
                     help="steps of the paired FP64/FP32 run that measures the FP32 form's error")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -359,15 +388,9 @@ def main():
 
     value = cells * nfl * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
-    # bytes per cell-update of the state: rho, v (, e) read and written, the
-    # sound speed read (isothermal); with several fluids, per fluid-cell
-    # update: the gas's 7 arrays and each dust fluid's 6, averaged
-    nv = 3 if dimz else 2
-    n_arrays = 2 * (1 + nv + ad) + (0 if ad else 1)
-    if nfl > 1:
-        n_arrays = (n_arrays + (nfl - 1) * 2 * (1 + nv)) / nfl
+    n_arrays = b_alg_arrays(dimz, ad, nfl)
     b_alg = 8 * n_arrays
-    roofline = roofline_of(stages, cells, dimz, ad, alpha, 8, args.config)
+    roofline = roofline_of(stages, cells, dimz, ad, alpha, 8, args.config, b_alg, nfl)
     step_gbs = value * b_alg / 1e9
 
     # the reduced-precision form on the same workload (FP32 fields, FP64 sums,
@@ -382,15 +405,27 @@ def main():
                           "step_roofline": {"bytes_per_cell_update": 4 * n_arrays,
                                             "achieved": round(v32 * 4 * n_arrays / 1e9, 1), "peak": HBM_PEAK_GBS,
                                             "unit": "GB/s", "frac": round(v32 * 4 * n_arrays / 1e9 / HBM_PEAK_GBS, 4)},
-                          "roofline": roofline_of(st32, cells, dimz, ad, alpha, 4, args.config),
+                          "roofline": roofline_of(st32, cells, dimz, ad, alpha, 4, args.config,
+                                                  4 * n_arrays, nfl),
                           "stages": st32,
                           "error_vs_fp64": fp32_errors(fa, parfile, args.error_steps)}}
     if not args.routines and args.batch > 0:
         forms = forms or {}
         forms["batched"] = batched_form(fa, args.batch, args.batch_steps, 10)
-    if not args.routines and args.multifluid_steps > 0:
-        forms = forms or {}
-        forms["multifluid"] = multifluid_form(fa, L, names, args.multifluid_steps, 10)
+
+    # the other single-disk workloads at their full sizes: the 2-D disk, the
+    # large adiabatic disk (FP64 and the precision form) and gas + 4 dust
+    workloads = None
+    if not args.routines and args.workloads:
+        workloads = {}
+        for cfg in ("c1", "c3", "c4"):
+            if cfg == args.config:
+                continue
+            workloads[cfg] = {"fp64": workload_run(fa, L, names, cfg, "", args.workload_steps, args.warmup, 8)}
+            if cfg in ("c3", "c4"):
+                w32 = workload_run(fa, L, names, cfg, "Precision=32", args.workload_steps, args.warmup, 4)
+                w32["speedup_vs_fp64"] = round(w32["value"] / workloads[cfg]["fp64"]["value"], 3)
+                workloads[cfg]["fp32"] = w32
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -419,6 +454,7 @@ def main():
                           "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "stages": stages,
         "forms": forms,
+        "workloads": workloads,
         "outputs": outputs,
         "cpu_baseline": cpu,
     }

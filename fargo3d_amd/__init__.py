@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-__all__ = ["Simulation", "Batch", "lib", "library_path", "build", "HipError", "SCALAR_KEYS"]
+__all__ = ["Simulation", "Batch", "sweep_specs", "lib", "library_path", "build", "HipError", "SCALAR_KEYS"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "lib", "libfargo3d_amd.so")
@@ -295,6 +295,7 @@ class Batch:
         L.f3d_batch_clock.argtypes = [vp, i, vp]
         L.f3d_batch_set_tiling.argtypes = [vp, i, i]
         L.f3d_batch_tune.argtypes = [vp, i, vp, vp]
+        L.f3d_batch_planets.argtypes = [vp, i, vp, i, vp]
         self.L = L
         n = len(specs)
         pars = (cp * n)(*[p.encode() for p, _ in specs])
@@ -322,6 +323,16 @@ class Batch:
         self._check(self.L.f3d_batch_clock(self._b, k, out))
         return {"dt": out[0], "dtemp": out[1], "PhysicalTime": out[2]}
 
+    def planets(self, k, maxn=8):
+        """Planets of instance k (x, y, z, vx, vy, vz, mass per row) and the
+        frame rate, from the device."""
+        out = (ctypes.c_double * (8 * maxn))()
+        of = ctypes.c_double()
+        nb = self.L.f3d_batch_planets(self._b, k, out, maxn, ctypes.byref(of))
+        if nb < 0:
+            raise HipError(self.L.f3d_batch_error(self._b).decode())
+        return np.array(out).reshape(maxn, 8)[:nb, :7], of.value
+
     def set_tiling(self, rows_y, tile):
         self._check(self.L.f3d_batch_set_tiling(self._b, int(rows_y), int(tile)))
 
@@ -343,3 +354,24 @@ class Batch:
         except Exception:
             pass
 
+
+
+C1_PAR = os.path.join("fargo3d_amd", "setups", "c1_fargo2d.par")
+
+
+def sweep_specs(n, seed=20240605, parfile=C1_PAR):
+    """A parameter sweep of n 2-D isothermal planet-disk set-ups for the
+    batched form: grids Nx in {256, 384, 768, 1536} x Ny in {128, 192, 384},
+    planet mass log-uniform in [1e-5, 3e-3] M*, viscosity log-uniform in
+    [1e-6, 1e-4] and aspect ratio uniform in [0.03, 0.08], drawn from a seeded
+    generator.  Returns (parfile, overrides) pairs."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(n):
+        nx = int(rng.choice([256, 384, 768, 1536]))
+        ny = int(rng.choice([128, 192, 384]))
+        mp = 10 ** rng.uniform(np.log10(1e-5), np.log10(3e-3))
+        nu = 10 ** rng.uniform(-6, -4)
+        h = rng.uniform(0.03, 0.08)
+        specs.append((parfile, "Nx=%d Ny=%d PlanetMass=%.6e Nu=%.6e AspectRatio=%.6f" % (nx, ny, mp, nu, h)))
+    return specs

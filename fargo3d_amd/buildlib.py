@@ -62,19 +62,23 @@ def build(force=True, verbose=False):
                os.path.join(ROOT, "include", "fargo3d_hip.h")]
     headers += [os.path.join(CSRC, "hip", h) for h in os.listdir(os.path.join(CSRC, "hip"))
                 if h.endswith(".h")]
-    objs = []
+    objs, jobs = [], []
     for s in HOST_SRCS:
         src = os.path.join(CSRC, s)
         obj = os.path.join(OBJDIR, os.path.basename(s) + ".o")
         if force or _stale(obj, [src] + headers):
-            _run(["gcc"] + CFLAGS + ["-I" + os.path.join(ROOT, "include"), "-c", src, "-o", obj])
+            jobs.append(["gcc"] + CFLAGS + ["-I" + os.path.join(ROOT, "include"), "-c", src, "-o", obj])
         objs.append(obj)
     for s in HIP_SRCS:
         src = os.path.join(CSRC, s)
         obj = os.path.join(OBJDIR, os.path.basename(s) + ".o")
         if force or _stale(obj, [src] + headers):
-            _run([HIPCC] + HIPFLAGS + EXTRA.get(s, []) + ["-c", src, "-o", obj])
+            jobs.append([HIPCC] + HIPFLAGS + EXTRA.get(s, []) + ["-c", src, "-o", obj])
         objs.append(obj)
+    # the translation units are independent: compile them side by side
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=min(len(jobs), 8) or 1) as pool:
+        list(pool.map(_run, jobs))
     lib = os.path.join(LIBDIR, "libfargo3d_amd.so")
     tmp = lib + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lm"])

@@ -5,17 +5,35 @@
 // the same device function the single-disk kernels run.  The time step of
 // each instance is computed and clamped on the device (src/main.c:383-395),
 // so a step needs no host round trip and is captured once in a hipGraph.
+// The planets of each instance live on the device too, in FP64: the
+// potential of the step from their positions, then the RK5 N-body step and
+// the rotation of the frame that follows the first planet (src/potential.c
+// :10-49), inside the same graph.
 // Included by f3d_fused_body.h in the FP32 build only.
 #include "../host/f3d_driver.h"
 
 namespace F3D_NS {
 namespace fused {
 
+constexpr int kBP = cell::kMaxPlanets;
+
+// The planetary system of one instance, FP64, in device memory (the host's
+// f3d_psys and the frame's scalars).  pm/ps2: tapered mass and squared
+// smoothing length of each planet for the next Potential.
+struct BPsys {
+  int n, indirect, corotating, guiding;
+  double x[kBP], y[kBP], z[kBP], vx[kBP], vy[kBP], vz[kBP], m[kBP];
+  int feelothers[kBP];
+  double pm[kBP], ps2[kBP];
+  double masstaper, rochesmoothing, aspectratio, flaringindex, thicknesssmoothing;
+  double of;  // OMEGAFRAME
+};
+
 // one instance as the batched kernels see it (device memory)
 struct BInst {
   dmesh m;
   real *rho, *cs, *vx, *vy, *vxt, *vyt, *mpx, *mmx, *mpy, *mmy, *rho0, *vx0, *vy0;
-  const double* pot;  // FP64 potential
+  double* pot;  // FP64 potential
   real *rhoy, *rhox, *e2, *ey, *xm0, *xm1, *xm2, *xm3;
   double* vxmed;
   int* nshift;
@@ -24,14 +42,18 @@ struct BInst {
   cell::ViscPar vp;
   cell::DampPar dp;
   cell::BcPar b0, b1;
-  real of;
+  real of;                  // OMEGAFRAME of the step (written by kb_dt with the planets)
+  real domega;              // CorrectVtheta's change of the frame rate this step
   int visc, potential, stockholm, fargo;
   double* clk;              // [0] dt, [1] dtemp, [2] PhysicalTime, [3] DT, [4] null-dt flag
   unsigned long long* res;  // the CFL atomic MIN (two slots; kd_cfl resets the other)
+  BPsys* ps;                // planets (nullptr: no potential)
+  const double *ymed64, *cosx64, *sinx64;  // FP64 mesh of the potential
+  double zc;                // Zmed of the single plane
 };
 
 // the stages, each with its own per-instance workgroup counts
-enum { BK_CFL = 0, BK_SRCA, BK_SRCB, BK_BND, BK_SWY, BK_VMED, BK_XT, BK_FIN, BK_N };
+enum { BK_CFL = 0, BK_SRCA, BK_SRCB, BK_BND, BK_SWY, BK_VMED, BK_XT, BK_FIN, BK_POT, BK_N };
 
 __device__ __forceinline__ int find_inst(const int* off, int n, int b) {
   int lo = 0, hi = n;  // off[lo] <= b < off[hi]
@@ -53,12 +75,200 @@ __global__ void __launch_bounds__(WG) kb_cfl(const BInst* I, const int* off, int
                     in.res, 0, in.cp, nullptr);
 }
 
+// Potential (src/potential.c:51-208) of every instance with planets, FP64:
+// the star's term, each planet's softened term and its indirect term, at the
+// planets' positions at the start of the step (cylindrical 2-D disks).
+__global__ void __launch_bounds__(BXF) kb_potential(const BInst* I, const int* off, int n) {
+  const int ii = find_inst(off, n, blockIdx.x);
+  const BInst& in = I[ii];
+  const int gx = (in.m.nx + BXF - 1) / BXF;
+  const dim3 b = local2(blockIdx.x - off[ii], gx);
+  const int i = b.x * BXF + threadIdx.x;
+  if (i >= in.m.nx) return;
+  const int j = b.y;
+  const BPsys& P = *in.ps;
+  const double r = in.ymed64[j], XC = r * in.cosx64[i], YC = r * in.sinx64[i], ZC = in.zc;
+  double pot = -1.0 / sqrt(r * r + ZC * ZC);
+  for (int p = 0; p < P.n; p++) {
+    const double xp = P.x[p], yp = P.y[p], zp = P.z[p], mp = P.pm[p];
+    const double d = sqrt(xp * xp + yp * yp + zp * zp);
+    const double dist = (XC - xp) * (XC - xp) + (YC - yp) * (YC - yp) + (ZC - zp) * (ZC - zp);
+    if (P.indirect) pot += mp * (XC * xp + YC * yp + ZC * zp) / (d * d * d);
+    pot += -mp / sqrt(dist + P.ps2[p]);
+  }
+  in.pot[(size_t)j * in.m.pitch + i] = pot;
+}
+
+// Tapered mass and squared smoothing length of each planet at time t
+// (src/potential.c:56-59, 115-131).
+__device__ void bp_prepare(BPsys& P, double t) {
+  const double taper = (P.masstaper == 0.0) ? 1.0
+                       : (t >= P.masstaper ? 1.0 : .5 * (1.0 - cos(M_PI * t / P.masstaper)));
+  for (int p = 0; p < P.n; p++) {
+    const double mp = P.m[p] * taper;
+    const double d = sqrt(P.x[p] * P.x[p] + P.y[p] * P.y[p] + P.z[p] * P.z[p]);
+    double h = (P.rochesmoothing != 0) ? d * pow(1.0 / 3.0 * mp, 1.0 / 3.0) * P.rochesmoothing
+                                       : P.aspectratio * pow(d, P.flaringindex) * d * P.thicknesssmoothing;
+    P.pm[p] = mp;
+    P.ps2[p] = h * h;
+  }
+}
+
+// Time derivative of the state q (x, y, z, vx, vy, vz blocks of NP) times
+// dt: the star's pull, the indirect acceleration of the star by every
+// planet, and the pull of the other planets on those that feel them
+// (src/runge_kutta.c:10-58).  NP, an upper bound of the planet count P.n,
+// is a template parameter and every loop is unrolled to it, so the state
+// stays in registers; slots at or beyond P.n are skipped (and hold 1.0).
+template <int NP>
+__device__ __forceinline__ void bp_deriv(const BPsys& P, const double (&q)[6 * NP], double (&d)[6 * NP], double dt) {
+  double rad[NP];
+#pragma unroll
+  for (int i = 0; i < NP; i++) rad[i] = sqrt(q[i] * q[i] + q[NP + i] * q[NP + i] + q[2 * NP + i] * q[2 * NP + i]);
+#pragma unroll
+  for (int i = 0; i < NP; i++) {
+    double a[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) a[c] = -q[c * NP + i] / rad[i] / rad[i] / rad[i];
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+      if (k >= P.n) continue;
+      if (P.indirect) {
+        const double cf = P.m[k] / rad[k] / rad[k] / rad[k];
+#pragma unroll
+        for (int c = 0; c < 3; c++) a[c] -= cf * q[c * NP + k];
+      }
+      if (k != i && P.feelothers[i]) {
+        const double dx = q[k] - q[i], dy = q[NP + k] - q[NP + i], dz = q[2 * NP + k] - q[2 * NP + i];
+        const double sd = sqrt(dx * dx + dy * dy + dz * dz);
+        const double cf = P.m[k] / sd / sd / sd;
+        a[0] += cf * dx;
+        a[1] += cf * dy;
+        a[2] += cf * dz;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      d[c * NP + i] = q[(3 + c) * NP + i] * dt;
+      d[(3 + c) * NP + i] = a[c] * dt;
+    }
+  }
+}
+
+// One Cash-Karp step of the planets over dt (src/runge_kutta.c:120-157):
+// five stages from the tableau's rows, the fifth-order combination.
+template <int NP>
+__device__ __forceinline__ void bp_rk5(BPsys& P, double dt) {
+  constexpr int S = 6 * NP;
+  double q0[S], q[S], k[6][S];
+#pragma unroll
+  for (int i = 0; i < NP; i++) {
+    const bool on = i < P.n;
+    q0[i] = on ? P.x[i] : 1.0; q0[NP + i] = on ? P.y[i] : 1.0; q0[2 * NP + i] = on ? P.z[i] : 1.0;
+    q0[3 * NP + i] = on ? P.vx[i] : 1.0; q0[4 * NP + i] = on ? P.vy[i] : 1.0; q0[5 * NP + i] = on ? P.vz[i] : 1.0;
+  }
+  constexpr double A[5][5] = {{0.2, 0.0, 0.0, 0.0, 0.0},
+                              {0.075, 0.225, 0.0, 0.0, 0.0},
+                              {0.3, -0.9, 1.2, 0.0, 0.0},
+                              {-11.0 / 54.0, 2.5, -70.0 / 27.0, 35.0 / 27.0, 0.0},
+                              {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0,
+                               253.0 / 4096.0}};
+  bp_deriv<NP>(P, q0, k[0], dt);
+#pragma unroll
+  for (int s = 1; s < 6; s++) {
+#pragma unroll
+    for (int a = 0; a < S; a++) {
+      double v = q0[a];
+#pragma unroll
+      for (int r = 0; r < s; r++) v += A[s - 1][r] * k[r][a];
+      q[a] = v;
+    }
+    bp_deriv<NP>(P, q, k[s], dt);
+  }
+#pragma unroll
+  for (int a = 0; a < S; a++)
+    q[a] = q0[a] + 37.0 / 378.0 * k[0][a] + 250.0 / 621.0 * k[2][a] + 125.0 / 594.0 * k[3][a] +
+           512.0 / 1771.0 * k[5][a];
+#pragma unroll
+  for (int i = 0; i < NP; i++) {
+    if (i >= P.n) continue;
+    P.x[i] = q[i]; P.y[i] = q[NP + i]; P.z[i] = q[2 * NP + i];
+    P.vx[i] = q[3 * NP + i]; P.vy[i] = q[4 * NP + i]; P.vz[i] = q[5 * NP + i];
+  }
+}
+
+// The point the frame follows (GetPsysInfo, src/psys.c:362-415): the first
+// planet itself, or (Frame G) its guiding centre, the point at mean anomaly M
+// on the circle of radius a of its two-body orbit about star + planet,
+// azimuth from the perihelion, tilted by the inclination.  Elements from the
+// angular momentum h = r x v and the Laplace-Runge-Lenz vector
+// A = v x h - G m r/|r| (|A| = G m e), a = h^2 / (G m (1 - e^2)), the
+// eccentric anomaly E from r = a (1 - e cos E) (negative while r.v < 0;
+// 0 or pi where rounding leaves acos's domain), M = E - e sin E; a circular
+// orbit (e < 1e-8) follows the planet's position.
+__device__ void bp_follow(const BPsys& P, double& xc, double& yc) {
+  const double x = P.x[0], y = P.y[0], z = P.z[0], vx = P.vx[0], vy = P.vy[0], vz = P.vz[0];
+  xc = x;
+  yc = y;
+  if (!P.guiding) return;
+  const double gm = 1.0 + P.m[0];
+  const double d = sqrt(x * x + y * y + z * z);
+  const double hx = y * vz - z * vy, hy = z * vx - x * vz, hz = x * vy - y * vx;
+  const double hh = sqrt(hx * hx + hy * hy), h = sqrt(hx * hx + hy * hy + hz * hz);
+  const double inc = asin(hh / h);
+  const double Ax = vy * hz - vz * hy - gm * x / d, Ay = vz * hx - vx * hz - gm * y / d,
+               Az = vx * hy - vy * hx - gm * z / d;
+  const double e = sqrt(Ax * Ax + Ay * Ay + Az * Az) / gm;
+  if (e < 1e-8) return;
+  const double a = h * h / (gm * (1. - e * e));
+  double E = acos((1.0 - d / a) / e);
+  if (x * vx + y * vy + z * vz < 0) E = -E;
+  if (isnan(E)) E = (d < a) ? 0.0 : M_PI;
+  const double M = E - e * sin(E);
+  const double phi = (Ax * Ax + Ay * Ay > 0.0) ? atan2(Ay, Ax) : atan2(y, x);
+  xc = a * cos(M + phi) * cos(inc);
+  yc = a * sin(M + phi) * cos(inc);
+}
+
+// compute_potential's planet part after Potential (src/potential.c:32-48):
+// five RK5 substeps over dt; with a frame that follows the first planet, the
+// new frame rate from the angle the followed point swept (GetPsysInfo
+// MARK/GET) and its change for CorrectVtheta; then the rotation of the
+// system into the new frame (src/psys.c:417-440).
+template <int NP>
+__device__ void bp_advance(BPsys& P, double dt, double& domega) {
+  double x0, y0;
+  bp_follow(P, x0, y0);
+  for (int s = 0; s < 5; s++) bp_rk5<NP>(P, 1.0 / 5.0 * dt);
+  domega = 0.0;
+  if (P.corotating) {
+    double x1, y1;
+    bp_follow(P, x1, y1);
+    const double d1 = sqrt(x0 * x0 + y0 * y0), d2 = sqrt(x1 * x1 + y1 * y1);
+    const double omeganew = asin((x0 * y1 - x1 * y0) / (d1 * d2)) / dt;
+    domega = omeganew - P.of;
+    P.of = omeganew;
+  }
+  const double ang = P.of * dt, sn = sin(ang), cs = cos(ang);
+  for (int i = 0; i < P.n; i++) {
+    const double x = P.x[i], y = P.y[i], vx = P.vx[i], vy = P.vy[i];
+    P.x[i] = x * cs + y * sn;
+    P.y[i] = -x * sn + y * cs;
+    P.vx[i] = vx * cs + vy * sn;
+    P.vy[i] = -vx * sn + vy * cs;
+  }
+}
+
 // CflFluidsMin and the output-time clamp (src/cfl_fluids_min.c:3-22,
 // src/main.c:383-395, 423), one thread per instance; clears the MIN slot.
-__global__ void kb_dt(const BInst* I, int n) {
+// With planets, then their step over dt and the frame's new rate, which the
+// rest of the step's kernels read from the instance table.  NP: the largest
+// planet count of the batch (1, 2, 4 or 8).
+template <int NP>
+__global__ void __launch_bounds__(64) kb_dt(BInst* I, int n) {
   const int ii = blockIdx.x * blockDim.x + threadIdx.x;
   if (ii >= n) return;
-  const BInst& in = I[ii];
+  BInst& in = I[ii];
   double* c = in.clk;
   const double DT = c[3];
   if (c[1] >= DT) c[1] = 0.0;
@@ -71,6 +281,18 @@ __global__ void kb_dt(const BInst* I, int n) {
   c[1] += dt;
   if (c[1] > DT) dt = DT - (c[1] - dt);
   c[0] = dt;
+  if (in.ps) {
+    BPsys& P = *in.ps;
+    double domega = 0.0;
+    bp_advance<NP>(P, dt, domega);
+    const real of = (real)P.of;
+    in.of = of;
+    in.dp.of = of;
+    in.b0.of = of;
+    in.b1.of = of;
+    in.domega = (real)domega;
+    bp_prepare(P, c[2] + dt);  // the next step's Potential
+  }
   c[2] += dt;
 }
 
@@ -79,7 +301,7 @@ __global__ void __launch_bounds__(BXF) kb_src_a(const BInst* I, const int* off, 
   const BInst& in = I[ii];
   const int gx = (in.m.nx + BXF - 1) / BXF;
   kd_src_a<CYL, 0>(local2(blockIdx.x - off[ii], gx), in.m, in.rho, in.cs, in.vx, in.vy, nullptr, in.mmx, in.mmy,
-                   nullptr, in.mpx, in.mpy, nullptr, in.vp, in.visc, (real)0);
+                   nullptr, in.mpx, in.mpy, nullptr, in.vp, in.visc, in.domega);
 }
 
 __global__ void __launch_bounds__(BXF) kb_src_b(const BInst* I, const int* off, int n) {
@@ -88,7 +310,7 @@ __global__ void __launch_bounds__(BXF) kb_src_b(const BInst* I, const int* off, 
   const int gx = (in.m.nx + BXF - 1) / BXF;
   kd_src_b<CYL, 0>(local2(blockIdx.x - off[ii], gx), in.m, (real)in.clk[0], in.rho, in.cs, in.vx, in.vy, nullptr,
                    in.pot, in.mmx, in.mmy, nullptr, in.mpx, in.mpy, nullptr, in.vxt, in.vyt,
-                   nullptr, in.e2, in.visc, in.potential, 0, 1, (real)0, in.of, (real)0);
+                   nullptr, in.e2, in.visc, in.potential, 0, 1, (real)0, in.of, in.domega);
 }
 
 // the ghost fill in Y (both sides in one launch): mid = 1 fills V_temp
@@ -143,6 +365,7 @@ struct f3d_batch {
   F3D_NS::fused::BInst* dI = nullptr;
   double* dclk = nullptr;
   unsigned long long* dres = nullptr;
+  F3D_NS::fused::BPsys* dps = nullptr;  // planetary systems, one per instance
   int* doff[F3D_NS::fused::BK_N] = {};
   int total[F3D_NS::fused::BK_N] = {};
   int* dswy = nullptr;  // workgroup offsets of the Y sweep for the chosen rows per thread
@@ -151,6 +374,8 @@ struct f3d_batch {
   int xt_total = 0;
   int nxmax = 0;
   int rows_y = 4, tile = 128;
+  std::vector<int> has_ps;  // instance has a planetary system on the device
+  int npmax = 0;            // the largest planet count
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<double*> save;  // state copies used by the tuner
@@ -192,6 +417,7 @@ static int batch_layout(f3d_batch* B) {
     c[BK_BND][i] = gx * 2;  // one block row per side, all ghost layers per thread
     c[BK_VMED][i] = m.ny;
     c[BK_FIN][i] = gx * m.ny;
+    c[BK_POT][i] = B->has_ps[i] ? gx * m.sy : 0;
     swy[i] = gx * ((m.ny + B->rows_y - 1) / B->rows_y);
     xt[i] = ((m.nx + x_tile_cells(B->tile) - 1) / x_tile_cells(B->tile)) * m.ny;
   }
@@ -207,8 +433,13 @@ static void batch_launch(f3d_batch* B) {
   const int n = B->n;
   hipStream_t s = B->stream;
   const size_t cfl_lds = (2 * (size_t)B->nxmax + 16) * sizeof(double);
+  if (B->total[BK_POT])
+    hipLaunchKernelGGL(kb_potential, dim3(B->total[BK_POT]), dim3(BXF), 0, s, B->dI, B->doff[BK_POT], n);
   hipLaunchKernelGGL(kb_cfl, dim3(B->total[BK_CFL]), dim3(WG), cfl_lds, s, B->dI, B->doff[BK_CFL], n);
-  hipLaunchKernelGGL(kb_dt, dim3((n + 63) / 64), dim3(64), 0, s, B->dI, n);
+  if (B->npmax <= 1) hipLaunchKernelGGL(kb_dt<1>, dim3((n + 63) / 64), dim3(64), 0, s, B->dI, n);
+  else if (B->npmax <= 2) hipLaunchKernelGGL(kb_dt<2>, dim3((n + 63) / 64), dim3(64), 0, s, B->dI, n);
+  else if (B->npmax <= 4) hipLaunchKernelGGL(kb_dt<4>, dim3((n + 63) / 64), dim3(64), 0, s, B->dI, n);
+  else hipLaunchKernelGGL(kb_dt<kBP>, dim3((n + 63) / 64), dim3(64), 0, s, B->dI, n);
   hipLaunchKernelGGL(kb_src_a, dim3(B->total[BK_SRCA]), dim3(BXF), 0, s, B->dI, B->doff[BK_SRCA], n);
   hipLaunchKernelGGL(kb_src_b, dim3(B->total[BK_SRCB]), dim3(BXF), 0, s, B->dI, B->doff[BK_SRCB], n);
   hipLaunchKernelGGL(kb_bound, dim3(B->total[BK_BND]), dim3(BXF), 0, s, B->dI, B->doff[BK_BND], n, 1);
@@ -234,31 +465,32 @@ static int batch_capture(f3d_batch* B) {
   return 0;
 }
 
-// copies of the evolving state (rho, vx, vy and the clocks) for the tuner
+// copies of the evolving state for the tuner: rho, vx, vy of every
+// instance, the clocks, the planetary systems and the instance table (its
+// frame rates change with the planets)
 static int batch_state(f3d_batch* B, int restore) {
-  size_t k = 0;
+  std::vector<std::pair<void*, size_t>> reg;
   for (int i = 0; i < B->n; i++) {
     f3d_hip_ctx* c = f3d_sim_ctx(B->sims[i]);
-    double* f[3] = {c->f.Density, c->f.Vx, c->f.Vy};
     const size_t bytes = c->n3 * sizeof(real);
-    for (int a = 0; a < 3; a++, k++) {
-      if (!restore && k >= B->save.size()) {
-        double* p = nullptr;
-        BCHECK(B, hipMalloc(&p, bytes));
-        B->save.push_back(p);
-      }
-      if (restore) BCHECK(B, hipMemcpyAsync(f[a], B->save[k], bytes, hipMemcpyDeviceToDevice, B->stream));
-      else BCHECK(B, hipMemcpyAsync(B->save[k], f[a], bytes, hipMemcpyDeviceToDevice, B->stream));
+    reg.push_back({c->f.Density, bytes});
+    reg.push_back({c->f.Vx, bytes});
+    reg.push_back({c->f.Vy, bytes});
+  }
+  reg.push_back({B->dclk, (size_t)B->n * 5 * sizeof(double)});
+  reg.push_back({B->dps, (size_t)B->n * sizeof(BPsys)});
+  reg.push_back({B->dI, (size_t)B->n * sizeof(BInst)});
+  for (size_t k = 0; k < reg.size(); k++) {
+    if (!restore && k >= B->save.size()) {
+      double* p = nullptr;
+      BCHECK(B, hipMalloc(&p, reg[k].second));
+      B->save.push_back(p);
     }
+    if (restore)
+      BCHECK(B, hipMemcpyAsync(reg[k].first, B->save[k], reg[k].second, hipMemcpyDeviceToDevice, B->stream));
+    else
+      BCHECK(B, hipMemcpyAsync(B->save[k], reg[k].first, reg[k].second, hipMemcpyDeviceToDevice, B->stream));
   }
-  const size_t cb = (size_t)B->n * 5 * sizeof(double);
-  if (!restore && k >= B->save.size()) {
-    double* p = nullptr;
-    BCHECK(B, hipMalloc(&p, cb));
-    B->save.push_back(p);
-  }
-  if (restore) BCHECK(B, hipMemcpyAsync(B->dclk, B->save[k], cb, hipMemcpyDeviceToDevice, B->stream));
-  else BCHECK(B, hipMemcpyAsync(B->save[k], B->dclk, cb, hipMemcpyDeviceToDevice, B->stream));
   BCHECK(B, hipStreamSynchronize(B->stream));
   return 0;
 }
@@ -281,6 +513,7 @@ void f3d_batch_destroy(f3d_batch* B) {
   if (B->dI) (void)hipFree(B->dI);
   if (B->dclk) (void)hipFree(B->dclk);
   if (B->dres) (void)hipFree(B->dres);
+  if (B->dps) (void)hipFree(B->dps);
   if (B->sims)
     for (int i = 0; i < B->n; i++)
       if (B->sims[i]) f3d_sim_destroy(B->sims[i]);
@@ -305,8 +538,11 @@ f3d_batch* f3d_batch_create(int n, const char* const* parfiles, const char* cons
   std::vector<BInst> h(n);
   std::vector<double> clk(5 * (size_t)n, 0.0);
   if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) return fail("stream creation failed");
+  std::vector<BPsys> hps(n);
+  B->has_ps.assign(n, 0);
   if (hipMalloc(&B->dclk, clk.size() * sizeof(double)) != hipSuccess ||
-      hipMalloc(&B->dres, 2 * (size_t)n * sizeof(unsigned long long)) != hipSuccess)
+      hipMalloc(&B->dres, 2 * (size_t)n * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&B->dps, (size_t)n * sizeof(BPsys)) != hipSuccess)
     return fail("device allocation failed");
   for (int i = 0; i < n; i++) {
     char ov[1024];
@@ -346,6 +582,46 @@ f3d_batch* f3d_batch_create(int n, const char* const* parfiles, const char* cons
     b.potential = c->ph.potential;
     b.stockholm = c->ph.stockholm;
     b.fargo = c->ph.fargo;
+    b.domega = (real)0;
+    b.ps = nullptr;
+    b.ymed64 = c->m.ymed;
+    b.cosx64 = c->m.cosxmed;
+    b.sinx64 = c->m.sinxmed;
+    b.zc = 0.0;
+    {
+      const f3d_params* p = f3d_sim_params(B->sims[i]);
+      const f3d_psys* sys = f3d_sim_psys(B->sims[i]);
+      const f3d_clock* ck = f3d_sim_clock(B->sims[i]);
+      BPsys& P = hps[i];
+      memset(&P, 0, sizeof(P));
+      P.of = ck->OMEGAFRAME;
+      if (c->ph.potential && sys->nb > 0) {
+        if (p->releasedate > 0.0) return fail("the batched form's planets have no release date");
+        for (int k = 0; k < sys->nb; k++)
+          if (sys->feeldisk[k]) return fail("the batched form's planets do not feel the disk");
+        P.n = sys->nb;
+        P.indirect = p->indirectterm;
+        P.corotating = p->corotating;
+        P.guiding = p->guidingcenter;
+        P.masstaper = p->masstaper;
+        P.rochesmoothing = p->rochesmoothing;
+        P.aspectratio = p->aspectratio;
+        P.flaringindex = p->flaringindex;
+        P.thicknesssmoothing = p->thicknesssmoothing;
+        const double taper = f3d_planet_taper(p, ck->PhysicalTime);
+        for (int k = 0; k < sys->nb; k++) {
+          P.x[k] = sys->x[k]; P.y[k] = sys->y[k]; P.z[k] = sys->z[k];
+          P.vx[k] = sys->vx[k]; P.vy[k] = sys->vy[k]; P.vz[k] = sys->vz[k];
+          P.m[k] = sys->mass[k];
+          P.feelothers[k] = sys->feelothers[k];
+          P.pm[k] = sys->mass[k] * taper;
+          P.ps2[k] = f3d_planet_smoothing2(sys, p, k, taper);
+        }
+        b.ps = B->dps + i;
+        B->has_ps[i] = 1;
+        if (P.n > B->npmax) B->npmax = P.n;
+      }
+    }
     b.clk = B->dclk + 5 * (size_t)i;
     b.res = B->dres + 2 * (size_t)i;
     clk[5 * (size_t)i + 3] = f3d_sim_dt_out(B->sims[i]);
@@ -355,6 +631,7 @@ f3d_batch* f3d_batch_create(int n, const char* const* parfiles, const char* cons
   if (hipMalloc(&B->dI, n * sizeof(BInst)) != hipSuccess ||
       hipMemcpy(B->dI, h.data(), n * sizeof(BInst), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(B->dclk, clk.data(), clk.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(B->dps, hps.data(), hps.size() * sizeof(BPsys), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(B->dres, res.data(), res.size() * sizeof(unsigned long long), hipMemcpyHostToDevice) != hipSuccess)
     return fail("device upload failed");
   if (batch_layout(B) || batch_capture(B)) return fail(B->err);
@@ -389,6 +666,21 @@ int f3d_batch_clock(f3d_batch* B, int i, double out[3]) {
   BCHECK(B, hipStreamSynchronize(B->stream));
   BCHECK(B, hipMemcpy(out, B->dclk + 5 * (size_t)i, 3 * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
+}
+
+int f3d_batch_planets(f3d_batch* B, int i, double* out, int maxn, double* omegaframe) {
+  using namespace F3D_NS::fused;
+  if (i < 0 || i >= B->n) return -2;
+  BPsys P;
+  BCHECK(B, hipStreamSynchronize(B->stream));
+  BCHECK(B, hipMemcpy(&P, B->dps + i, sizeof(BPsys), hipMemcpyDeviceToHost));
+  if (omegaframe) *omegaframe = P.of;
+  for (int k = 0; k < P.n && k < maxn; k++) {
+    double* o = out + 8 * (size_t)k;
+    o[0] = P.x[k]; o[1] = P.y[k]; o[2] = P.z[k]; o[3] = P.vx[k]; o[4] = P.vy[k]; o[5] = P.vz[k];
+    o[6] = P.m[k]; o[7] = (double)P.n;
+  }
+  return P.n;
 }
 
 int f3d_batch_set_tiling(f3d_batch* B, int rows_y, int tile) {

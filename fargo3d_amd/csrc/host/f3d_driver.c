@@ -86,6 +86,9 @@ int f3d_sim_select_fluid(f3d_sim *S, int n) {
   return 0;
 }
 f3d_hip_ctx *f3d_sim_ctx(f3d_sim *S) { return S->ctx; }
+const f3d_params *f3d_sim_params(f3d_sim *S) { return &S->p; }
+const f3d_psys *f3d_sim_psys(f3d_sim *S) { return &S->sys; }
+const f3d_clock *f3d_sim_clock(f3d_sim *S) { return &S->clk; }
 
 int f3d_sim_force_dt(f3d_sim *S, double dt) {
   S->dt_force = dt;

@@ -58,6 +58,11 @@ int f3d_sim_nfluids(f3d_sim *S);
 int f3d_sim_select_fluid(f3d_sim *S, int n);
 double f3d_sim_dt_out(f3d_sim *S);           /* DT, the output interval the step clamps to */
 f3d_hip_ctx *f3d_sim_ctx(f3d_sim *S);
+/* read-only views of the set-up, the planetary system and the clocks (the
+   batched form copies them to the device) */
+const f3d_params *f3d_sim_params(f3d_sim *S);
+const f3d_psys *f3d_sim_psys(f3d_sim *S);
+const f3d_clock *f3d_sim_clock(f3d_sim *S);
 /* refresh the kernels' copy of the run parameters and host scalars */
 int f3d_sim_push_phys(f3d_sim *S);
 
@@ -105,11 +110,15 @@ int f3d_read_active(double *h, int nx, int ny, int nz, int nghy, int nghz, long 
                     const char *path);
 
 /* ---- batched form (no counterpart in the reference): n independent 2-D
- * isothermal disks, each its own .par set-up, run in the FP32 form with
- * their planets held on their orbits (potential computed once); every stage
- * of the step is one launch over all instances and one step is one hipGraph
- * replay, time steps included (per-instance CFL and output-time clamp on the
- * device).  Instances are read through f3d_batch_instance + f3d_sim_get_field. */
+ * isothermal disks, each its own .par set-up, run in the FP32 form; every
+ * stage of the step is one launch over all instances and one step is one
+ * hipGraph replay, time steps included (per-instance CFL and output-time
+ * clamp on the device), and so is each instance's planetary system (FP64 on
+ * the device: the potential from the planets' positions, RK5, the frame that
+ * follows the first planet, CorrectVtheta).  Instances are read through
+ * f3d_batch_instance + f3d_sim_get_field; their planets and frame rate through
+ * f3d_batch_planets (the instance's host-side planets stay at their initial
+ * state). */
 typedef struct f3d_batch f3d_batch;
 f3d_batch *f3d_batch_create(int n, const char *const *parfiles, const char *const *overrides, char *err,
                             int errlen);
@@ -120,6 +129,10 @@ f3d_sim *f3d_batch_instance(f3d_batch *B, int i);
 int f3d_batch_step(f3d_batch *B, int nsteps);
 int f3d_batch_sync(f3d_batch *B);                        /* also reports a null time step */
 int f3d_batch_clock(f3d_batch *B, int i, double out[3]); /* dt, dtemp, PhysicalTime */
+/* planets of instance i as f3d_sim_get_planets lays them out (8 doubles per
+ * planet: x, y, z, vx, vy, vz, mass, count), at most maxn; *omegaframe gets
+ * the frame rate.  Returns the planet count, or -1/-2 on error. */
+int f3d_batch_planets(f3d_batch *B, int i, double *out, int maxn, double *omegaframe);
 /* tuning knobs (rows per thread of the Y sweep; 64/128/256-cell X tiles) and
  * the autotuner that times every combination on the batch itself, restoring
  * the state afterwards */
