@@ -226,24 +226,33 @@ def workload_run(fa, L, names, config, overrides, steps, warmup, esize):
             "stages": st}
 
 
-def fp32_errors(fa, parfile, steps):
+def fp32_errors(fa, parfile, steps=None, orbits=None):
     """Relative L1 and L-infinity errors of every field of the FP32 form against
-    the FP64 path after `steps` steps on one time-step sequence."""
+    the FP64 path after `steps` steps, or after `orbits` orbits at r = 1, on one
+    sequence of time steps (each FP32 step takes the FP64 step before the
+    output-time clamp, so both clamp alike)."""
     import numpy as np
     s64 = fa.Simulation(parfile)
     s32 = fa.Simulation(parfile, "Precision=32")
-    for _ in range(steps):
-        dt = s64.step()
-        s32.force_dt(dt)
+    n = 0
+    t_end = 2 * np.pi * orbits if orbits else None
+    while (n < steps) if steps else (s64.scalars()["PhysicalTime"] < t_end * (1 - 1e-12)):
+        s64.step()
+        s32.force_dt(s64.step_time())
         s32.step()
+        n += 1
     fields = ["Density", "Vx", "Vy"] + (["Vz"] if s64.dimz else []) + (["Energy"] if s64.adiabatic else [])
-    out = {"steps": steps, "time": round(s64.scalars()["PhysicalTime"], 6)}
+    out = {"steps": n, "time": round(s64.scalars()["PhysicalTime"], 6)}
+    if orbits:
+        out["orbits"] = orbits
     for f in fields:
         a, b = s32.field(f), s64.field(f)
         a = a[3:-3, 3:-3, :] if s64.dimz else a[:, 3:-3, :]
         b = b[3:-3, 3:-3, :] if s64.dimz else b[:, 3:-3, :]
         d = np.abs(a - b)
         out[f] = {"L1": float("%.3e" % (d.sum() / np.abs(b).sum())), "Linf": float("%.3e" % (d.max() / np.abs(b).max()))}
+    s64.close()
+    s32.close()
     return out
 
 
@@ -351,7 +360,10 @@ def main():
     ap.add_argument("--no-workloads", dest="workloads", action="store_false",
                     help="skip the other workloads (2-D disk, large adiabatic disk, gas + dust)")
     ap.add_argument("--workload-steps", type=int, default=200)
-    ap.add_argument("--error-steps", type=int, default=200,
+    ap.add_argument("--error-orbits", type=float, default=10.0,
+                    help="orbits at r = 1 of the paired FP64/FP32 runs of the 3-D disk and the large adiabatic "
+                         "disk that measure the precision form's error (0: skip them)")
+    ap.add_argument("--error-steps", type=int, default=500,
                     help="steps of the paired FP64/FP32 run that measures the FP32 form's error")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run three device-to-device copies of MIB MiB on the library stream (a copy "
@@ -408,7 +420,9 @@ def main():
                           "roofline": roofline_of(st32, cells, dimz, ad, alpha, 4, args.config,
                                                   4 * n_arrays, nfl),
                           "stages": st32,
-                          "error_vs_fp64": fp32_errors(fa, parfile, args.error_steps)}}
+                          "error_vs_fp64": fp32_errors(fa, parfile, steps=args.error_steps)}}
+        if args.error_orbits > 0:
+            forms["fp32"]["error_vs_fp64_orbits"] = fp32_errors(fa, parfile, orbits=args.error_orbits)
     if not args.routines and args.batch > 0:
         forms = forms or {}
         forms["batched"] = batched_form(fa, args.batch, args.batch_steps, 10)
@@ -425,6 +439,9 @@ def main():
             if cfg in ("c3", "c4"):
                 w32 = workload_run(fa, L, names, cfg, "Precision=32", args.workload_steps, args.warmup, 4)
                 w32["speedup_vs_fp64"] = round(w32["value"] / workloads[cfg]["fp64"]["value"], 3)
+                if cfg == "c3" and args.error_orbits > 0:
+                    # the large adiabatic disk's own horizon: ten orbits at r = 1
+                    w32["error_vs_fp64_orbits"] = fp32_errors(fa, PAR[cfg], orbits=args.error_orbits)
                 workloads[cfg]["fp32"] = w32
 
     cpu = None

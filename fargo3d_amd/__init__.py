@@ -154,8 +154,15 @@ class Simulation:
         return dt.value
 
     def force_dt(self, dt):
-        """Use dt as the next step's time step (paired-run measurements)."""
+        """Use dt as the next step's time step before the output-time clamp
+        (paired-run measurements: pass another run's step_time(), so that
+        both clamp to the output times identically)."""
         self._check(self.L.f3d_sim_force_dt(self._s, float(dt)))
+
+    def step_time(self):
+        """The last step's time step before the output-time clamp (StepTime);
+        step() returns the clamped one."""
+        return self.scalars()["StepTime"]
 
     def run(self, nsteps):
         self._check(self.L.f3d_sim_run(self._s, int(nsteps)))
@@ -278,9 +285,11 @@ class Simulation:
 class Batch:
     """Many independent 2-D isothermal disks advanced together in the FP32
     form: each stage of the step is one launch over every instance, and the
-    step (time steps included) is one hipGraph replay."""
+    step (time steps included) is one hipGraph replay, planets included.
+    The initial states are computed on the device (CondInitOnDevice) unless
+    condinit_on_device is False."""
 
-    def __init__(self, specs):
+    def __init__(self, specs, condinit_on_device=True):
         L = lib()
         vp, cp, i = ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int
         L.f3d_batch_create.restype = vp
@@ -299,7 +308,8 @@ class Batch:
         self.L = L
         n = len(specs)
         pars = (cp * n)(*[p.encode() for p, _ in specs])
-        ovs = (cp * n)(*[o.encode() for _, o in specs])
+        extra = " CondInitOnDevice=%s" % ("yes" if condinit_on_device else "no")
+        ovs = (cp * n)(*[(o + extra).encode() for _, o in specs])
         err = ctypes.create_string_buffer(512)
         self._b = L.f3d_batch_create(n, pars, ovs, err, 512)
         if not self._b:

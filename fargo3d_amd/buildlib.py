@@ -34,8 +34,11 @@ HIPFLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contra
             "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-result", "-I" + os.path.join(ROOT, "include")]
 # The reduced-precision build is not bit-matched to anything: it may fuse
-# multiply-adds and divide through the hardware reciprocal (about 1 ulp).
-EXTRA = {"hip/f3d_fused32.hip": ["-ffp-contract=fast", "-freciprocal-math", "-fapprox-func"]}
+# multiply-adds.  Its divisions stay IEEE-rounded: the hardware reciprocal's
+# error is biased, and over thousands of steps of a quasi-steady disk a bias
+# of the same sign in every cell acts like a source (measured: the density
+# error after 10 orbits of the adiabatic disk halves without it).
+EXTRA = {"hip/f3d_fused32.hip": ["-ffp-contract=fast"]}
 
 
 def _run(cmd):

@@ -43,6 +43,7 @@ struct BInst {
   cell::DampPar dp;
   cell::BcPar b0, b1;
   real of;                  // OMEGAFRAME of the step (written by kb_dt with the planets)
+  double of64;              // the same in FP64 (the centrifugal terms)
   real domega;              // CorrectVtheta's change of the frame rate this step
   int visc, potential, stockholm, fargo;
   double* clk;              // [0] dt, [1] dtemp, [2] PhysicalTime, [3] DT, [4] null-dt flag
@@ -270,6 +271,7 @@ __global__ void __launch_bounds__(64) kb_dt(BInst* I, int n) {
   if (ii >= n) return;
   BInst& in = I[ii];
   double* c = in.clk;
+  in.m.salt = cell::sr_hash(in.m.salt + 0x9E3779B9U);  // the step's key of the stochastic rounding
   const double DT = c[3];
   if (c[1] >= DT) c[1] = 0.0;
   const double fluid_min = __longlong_as_double((long long)in.res[0]);
@@ -285,11 +287,11 @@ __global__ void __launch_bounds__(64) kb_dt(BInst* I, int n) {
     BPsys& P = *in.ps;
     double domega = 0.0;
     bp_advance<NP>(P, dt, domega);
-    const real of = (real)P.of;
-    in.of = of;
-    in.dp.of = of;
-    in.b0.of = of;
-    in.b1.of = of;
+    in.of = (real)P.of;
+    in.of64 = P.of;
+    in.dp.of = P.of;
+    in.b0.of = P.of;
+    in.b1.of = P.of;
     in.domega = (real)domega;
     bp_prepare(P, c[2] + dt);  // the next step's Potential
   }
@@ -310,7 +312,7 @@ __global__ void __launch_bounds__(BXF) kb_src_b(const BInst* I, const int* off, 
   const int gx = (in.m.nx + BXF - 1) / BXF;
   kd_src_b<CYL, 0>(local2(blockIdx.x - off[ii], gx), in.m, (real)in.clk[0], in.rho, in.cs, in.vx, in.vy, nullptr,
                    in.pot, in.mmx, in.mmy, nullptr, in.mpx, in.mpy, nullptr, in.vxt, in.vyt,
-                   nullptr, in.e2, in.visc, in.potential, 0, 1, (real)0, in.of, in.domega);
+                   nullptr, in.e2, in.visc, in.potential, 0, 1, (real)0, in.of64, in.domega);
 }
 
 // the ghost fill in Y (both sides in one launch): mid = 1 fills V_temp
@@ -574,10 +576,11 @@ f3d_batch* f3d_batch_create(int n, const char* const* parfiles, const char* cons
     b.dp.Y_inf = (real)c->ph.Y_inf; b.dp.Y_sup = (real)c->ph.Y_sup; b.dp.Z_inf = (real)c->ph.Z_inf;
     b.dp.Z_sup = (real)c->ph.Z_sup; b.dp.y_min = (real)c->ph.ymin_par; b.dp.y_max = (real)c->ph.ymax_par;
     b.dp.z_min = (real)c->ph.zmin_par; b.dp.z_max = (real)c->ph.zmax_par; b.dp.ds = (real)c->ph.taudamp;
-    b.dp.of = (real)c->ph.OMEGAFRAME; b.dp.of0 = (real)c->ph.OMEGAFRAME0; b.dp.periodic_z = c->ph.periodicz;
+    b.dp.of = c->ph.OMEGAFRAME; b.dp.of0 = c->ph.OMEGAFRAME0; b.dp.periodic_z = c->ph.periodicz;
     b.b0 = bc_par_f(c, 0);
     b.b1 = bc_par_f(c, 1);
     b.of = (real)c->ph.OMEGAFRAME;
+    b.of64 = c->ph.OMEGAFRAME;
     b.visc = c->ph.visc != 0 && c->ph.fluidtype == 1;
     b.potential = c->ph.potential;
     b.stockholm = c->ph.stockholm;

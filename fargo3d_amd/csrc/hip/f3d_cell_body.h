@@ -45,6 +45,77 @@ __device__ __forceinline__ GLT<double> gld(const double* p, const dmesh& m, int 
 // their sum (of the order of the pressure support) is rounded to FP32.
 constexpr bool kFix64 = sizeof(real) == 4;
 
+// The FP32 build also stores the azimuthal velocity as its deviation from an
+// FP64 profile of the rings, vx = V0(j,k) + vx' (the fields Vx, Vx_temp and
+// the damping reference Vx0 hold vx'), and the azimuthal momenta
+// rho (vx + r Omega_f) r as their deviation rho vx' r from rho l0(j,k), with
+// l0 = (V0 + r Omega_f) r.  A disk in near-Keplerian rotation has |vx'| of
+// the order of its pressure support and perturbations, so the FP32 rounding
+// of the stored and transported values is that much smaller than on the
+// full rotation, whose rounding acts as a torque that never averages out.
+// The azimuthal X sweep is invariant under a constant added along a ring
+// (its slopes, faces, limiters and fluxes see only differences), so it runs
+// on the deviations unchanged; the terms that need the full velocity (the
+// centrifugal acceleration, the viscous stress, the ghost formulas, the CFL,
+// the ring means) add V0 in FP64.  In the FP64 build these return 0 and no
+// code path reads them.
+__device__ __forceinline__ double vbase(const dmesh& m, int j, int k) {
+  if constexpr (kFix64) return m.vb[j + k * m.sy];
+  return 0.0;
+}
+// State updates x + inc of the precision form (the sweeps' flux updates,
+// the compression work) round stochastically to FP32: up or down with the
+// probabilities that make the expected result the exact sum.  In a disk that
+// evolves slowly a cell's update is often below half an FP32 unit of its
+// value and nearly the same step after step; rounded to nearest it would be
+// lost the same way every step (the slow heating or mass trend of the FP64
+// run stalls in FP32), rounded stochastically it is kept on average.  The
+// random number is a hash of the cell, the quantity and the step's key, so a
+// run is reproducible.  In the FP64 build upd is the plain sum.
+__host__ __device__ __forceinline__ unsigned sr_hash(unsigned x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+__device__ __forceinline__ real sr_round(const dmesh& m, double s, int l, int q) {
+  const float f = (float)s;
+  const double df = (double)f, ds = s - df;
+  if (ds == 0.0) return f;
+  // the neighbouring float on the side of s (finite f): one unit of the
+  // bit pattern, away from zero where s is farther from zero than f
+  const bool up = ds > 0.0;
+  float g;
+  if (f == 0.0f) g = up ? __uint_as_float(1u) : -__uint_as_float(1u);
+  else g = __uint_as_float(__float_as_uint(f) + (((f > 0.0f) == up) ? 1u : 0xFFFFFFFFu));
+  const unsigned h = sr_hash((unsigned)l * 0x9E3779B1U ^ m.salt ^ ((unsigned)q * 0x85EBCA77U));
+  const double u = (double)(h >> 8) * 0x1p-24;
+  return (fabs(ds) > u * fabs((double)g - df)) ? g : f;
+}
+__device__ __forceinline__ real upd(const dmesh& m, real x, real inc, int l, int q) {
+  if constexpr (kFix64) return sr_round(m, (double)x + (double)inc, l, q);
+  return x + inc;
+}
+// the same with an FP64 increment (precision form only)
+__device__ __forceinline__ real upd64(const dmesh& m, real x, double inc, int l, int q) {
+  if constexpr (kFix64) return sr_round(m, (double)x + inc, l, q);
+  return (real)((double)x + inc);
+}
+
+// the cylindrical radius of ring (j,k) in FP64
+template <int G>
+__device__ __forceinline__ double rcyl64(const dmesh& m, int j, int k) {
+  return (G == SPH) ? m.ymed64[j] * m.sinzmed64[k] : m.ymed64[j];
+}
+// l0(j,k) = (V0 + r Omega_f) r, the specific angular momentum of the profile
+template <int G>
+__device__ __forceinline__ double lbase(const dmesh& m, int j, int k, double omegaframe) {
+  const double r = rcyl64<G>(m, j, k);
+  return (vbase(m, j, k) + r * omegaframe) * r;
+}
+
 // ---------------------------------------------------------------- SubStep1
 // src/substep1_x.c:112-120
 template <int G, class LR, class LP, class LPOT, class LV>
@@ -64,13 +135,14 @@ __device__ __forceinline__ real substep1_x(const dmesh& m, int i, int j, int k, 
 template <int G, int DZ, class LR, class LP, class LPOT, class LVX, class LVY, class LVZ>
 __device__ __forceinline__ real substep1_y(const dmesh& m, int i, int j, int k, real dt, LR rho, LP p,
                                              LPOT pot, LVX vx, LVY vy, LVZ vz, int dust, int potential,
-                                             real omegaframe) {
+                                             double omegaframe) {
   real dtOVERrhom = real(2.0) * dt / (rho(0, 0, 0) * (m.ymin[j + 1] - m.ymin[j]) + rho(0, -1, 0) * (m.ymin[j] - m.ymin[j - 1])) *
                       (m.ymed[j] - m.ymed[j - 1]);
   real v = vy(0, 0, 0);
   if (!dust) v -= dtOVERrhom * (p(0, 0, 0) - p(0, -1, 0)) / (m.ymed[j] - m.ymed[j - 1]);
   if constexpr (kFix64) {
-    double vphi = 0.25 * ((double)vx(0, 0, 0) + (double)vx(1, 0, 0) + (double)vx(0, -1, 0) + (double)vx(1, -1, 0));
+    double vphi = 0.25 * ((double)vx(0, 0, 0) + (double)vx(1, 0, 0) + (double)vx(0, -1, 0) + (double)vx(1, -1, 0)) +
+                  0.5 * (vbase(m, j, k) + vbase(m, j - 1, k));
     double acc;
     if (G == SPH) {
       vphi += m.ymin64[j] * m.sinzmed64[k] * (double)omegaframe;
@@ -103,12 +175,13 @@ __device__ __forceinline__ real substep1_y(const dmesh& m, int i, int j, int k, 
 template <int G, class LR, class LP, class LPOT, class LVX, class LVZ>
 __device__ __forceinline__ real substep1_z(const dmesh& m, int i, int j, int k, real dt, LR rho, LP p,
                                              LPOT pot, LVX vx, LVZ vz, int dust, int potential,
-                                             real omegaframe) {
+                                             double omegaframe) {
   real dtOVERrhom = real(2.0) * dt / (rho(0, 0, 0) * (m.zmin[k + 1] - m.zmin[k]) + rho(0, 0, -1) * (m.zmin[k] - m.zmin[k - 1])) *
                       (m.zmed[k] - m.zmed[k - 1]);
   real v;
   if (G == SPH && kFix64) {
-    double vphi = 0.25 * ((double)vx(0, 0, 0) + (double)vx(1, 0, 0) + (double)vx(0, 0, -1) + (double)vx(1, 0, -1));
+    double vphi = 0.25 * ((double)vx(0, 0, 0) + (double)vx(1, 0, 0) + (double)vx(0, 0, -1) + (double)vx(1, 0, -1)) +
+                  0.5 * (vbase(m, j, k) + vbase(m, j, k - 1));
     const double smin = m.sinzmin64[k], r = m.ymed64[j];
     vphi += r * smin * (double)omegaframe;
     v = vz(0, 0, 0);
@@ -188,6 +261,13 @@ __device__ __forceinline__ real substep3(const dmesh& m, int i, int j, int k, re
                                            LVY vy, LVZ vz, real gamma) {
   real dv = div_v<DZ>(m, i, j, k, vx, vy, vz);
   real term = mul_invvol(m, real(0.5) * dt * (gamma - real(1.)) * dv, i, j, k);
+  if constexpr (kFix64) {
+    // e (1 - t)/(1 + t) = e - e 2t/(1 + t): the change of e formed directly.
+    // In FP32 the rounding of 1 - t is the same for every cell of a ring
+    // with the same t and would heat or cool the ring by a fixed amount
+    // each step; here it is relative to the change only.
+    return upd(m, e, -(e * (real(2.0) * term / (real(1.0) + term))), i + j * m.pitch + k * m.stride, 7);
+  }
   e *= (real(1.0) - term) / (real(1.0) + term);
   return e;
 }
@@ -370,12 +450,11 @@ template <int G>
 __device__ __forceinline__ void momenta_x(const dmesh& m, int i, int j, int k, real rho, real vx, real vxp,
                                           real omegaframe, real& mmx, real& mpx) {
   if constexpr (kFix64) {
-    // angular momentum with the FP64 radius: the FP32 radius of a ring would
-    // give a fixed relative bias between rings to every transported momentum
-    const double rcyl = (G == SPH) ? m.ymed64[j] * m.sinzmed64[k] : m.ymed64[j];
-    const double of = omegaframe;
-    mmx = (real)(((double)vx + rcyl * of) * rcyl * (double)rho);
-    mpx = (real)(((double)vxp + rcyl * of) * rcyl * (double)rho);
+    // deviations from rho l0: rho vx' r (vx, vxp are deviations), with the
+    // FP64 radius
+    const double rcyl = rcyl64<G>(m, j, k);
+    mmx = (real)((double)vx * rcyl * (double)rho);
+    mpx = (real)((double)vxp * rcyl * (double)rho);
     return;
   }
   if (G == SPH) {
@@ -394,10 +473,11 @@ __device__ __forceinline__ real newvel_x(const dmesh& m, int i, int j, int k, re
                                            real mpxm, real omegaframe) {
   const int im = ixm_of(i, m.nx);
   if constexpr (kFix64) {
-    const double r = (G == SPH) ? m.ymed64[j] * m.sinzmed64[k] : m.ymed64[j];
+    // from the momenta's deviations: l0/r - Omega_f r = V0 drops out, and the
+    // result is the deviation vx'
+    const double r = rcyl64<G>(m, j, k);
     return (real)(((double)mul_vol(m, mmx, i, j, k) + (double)mul_vol(m, mpxm, im, j, k)) /
-                      (((double)mul_vol(m, rho, i, j, k) + (double)mul_vol(m, rhoxm, im, j, k)) * r) -
-                  (double)omegaframe * r);
+                  (((double)mul_vol(m, rho, i, j, k) + (double)mul_vol(m, rhoxm, im, j, k)) * r));
   }
   real r = (G == SPH) ? m.ymed[j] * m.sinzmed[k] : m.ymed[j];
   return (mul_vol(m, mmx, i, j, k) + mul_vol(m, mpxm, im, j, k)) /
@@ -484,7 +564,11 @@ __device__ __forceinline__ double cfl_sumsq(const dmesh& m, int i, int j, int k,
                                             const real* rho, const real* vx, const real* vy, const real* vz,
                                             double vxmed, const CflPar& cp) {
   const int lxp = lxp_of(l, i, m.nx), lyp = l + m.pitch, lzp = l + m.stride;
-  const double vx0 = vx[l], vx1 = vx[lxp];
+  double vx0 = vx[l], vx1 = vx[lxp];
+  if constexpr (kFix64) {
+    vx0 += vbase(m, j, k);
+    vx1 += vbase(m, j, k);
+  }
   double vxx, vxxp;
   if (!cp.fargo) {
     vxx = vx0;
@@ -544,7 +628,8 @@ __device__ __forceinline__ double cfl_cell(const dmesh& m, int i, int j, int k, 
 
 // ---------------------------------------------------------------- Stockholm damping
 struct DampPar {
-  real Y_inf, Y_sup, Z_inf, Z_sup, y_min, y_max, z_min, z_max, ds, of, of0;
+  real Y_inf, Y_sup, Z_inf, Z_sup, y_min, y_max, z_min, z_max, ds;
+  double of, of0;  // the frame rate and its initial value, FP64 in every build
   int periodic_z;
 };
 
@@ -581,7 +666,7 @@ __device__ __forceinline__ void stockholm(const dmesh& m, int j, int k, int l, r
     real vx0_target = vx0[l2];
     real radius = ym;
     if (G == SPH) radius *= m.sinzmed[k];
-    vx0_target -= (d.of - d.of0) * radius;
+    vx0_target -= (real)((d.of - d.of0) * radius);
     vx[l] = (vx[l] * taud + vx0_target * dt) / (dt + taud);
     vy[l] = (vy[l] * taud + vy0[l2] * dt) / (dt + taud);
   }
@@ -597,7 +682,7 @@ __device__ __forceinline__ void stockholm(const dmesh& m, int j, int k, int l, r
 struct BcPar {
   int p3d;
   int energy;  // Energy ghosts SYMMETRIC (centered |a|a|, std/boundaries.txt:55-57)
-  real of;
+  double of;   // the frame rate, FP64 in every build
   real dens[3], vsq[3], vfac[3], zvfac[3];
 };
 
@@ -645,14 +730,29 @@ __device__ __forceinline__ GhostCell ghost_y_load(const dmesh& m, int side, int 
 
 __device__ __forceinline__ void ghost_y_store(const dmesh& m, int mm, int k, const GhostCell& c, real* dens, real* vx,
                                               real* vy, real* vz, real* e, const BcPar& b, const real* dexp) {
-  const real of = b.of;
   real dg, xg;
-  if (!b.p3d) {
-    dg = c.d0 * b.dens[mm];
-    xg = (c.x0 + m.ymed[c.act] * of) * b.vsq[mm] - m.ymed[c.gh] * of;
+  if constexpr (kFix64) {
+    // the full velocities in FP64, the ghost's deviation from its own V0
+    const double of = b.of, xa = (double)c.x0 + vbase(m, c.act, k);
+    double x;
+    if (!b.p3d) {
+      dg = c.d0 * b.dens[mm];
+      x = (xa + m.ymed64[c.act] * of) * (double)b.vsq[mm] - m.ymed64[c.gh] * of;
+    } else {
+      dg = c.d0 * b.dens[mm] * dexp[mm * m.sz + k];
+      x = (xa + of * m.ymed64[c.act] * m.sinzmed64[k]) * (double)b.vsq[mm] * (double)b.vfac[mm] -
+          m.ymed64[c.gh] * of * m.sinzmed64[k];
+    }
+    xg = (real)(x - vbase(m, c.gh, k));
   } else {
-    dg = c.d0 * b.dens[mm] * dexp[mm * m.sz + k];
-    xg = (c.x0 + of * m.ymed[c.act] * m.sinzmed[k]) * b.vsq[mm] * b.vfac[mm] - m.ymed[c.gh] * of * m.sinzmed[k];
+    const real of = b.of;
+    if (!b.p3d) {
+      dg = c.d0 * b.dens[mm];
+      xg = (c.x0 + m.ymed[c.act] * of) * b.vsq[mm] - m.ymed[c.gh] * of;
+    } else {
+      dg = c.d0 * b.dens[mm] * dexp[mm * m.sz + k];
+      xg = (c.x0 + of * m.ymed[c.act] * m.sinzmed[k]) * b.vsq[mm] * b.vfac[mm] - m.ymed[c.gh] * of * m.sinzmed[k];
+    }
   }
   dens[c.lgh] = dg;
   vx[c.lgh] = xg;
@@ -700,9 +800,16 @@ __device__ __forceinline__ GhostCell ghost_z_load(const dmesh& m, int side, int 
 
 __device__ __forceinline__ void ghost_z_store(const dmesh& m, int mm, int j, const GhostCell& c, real* dens, real* vx,
                                               real* vy, real* vz, real* e, const BcPar& b, const real* zdens) {
-  const real of = b.of;
   const real dg = c.d0 * zdens[mm * m.sy + j];
-  const real xg = (c.x0 + of * m.ymed[j] * m.sinzmed[c.act]) * b.zvfac[mm] - m.ymed[j] * of * m.sinzmed[c.gh];
+  real xg;
+  if constexpr (kFix64) {
+    const double of = b.of, xa = (double)c.x0 + vbase(m, j, c.act);
+    xg = (real)((xa + of * m.ymed64[j] * m.sinzmed64[c.act]) * (double)b.zvfac[mm] -
+                m.ymed64[j] * of * m.sinzmed64[c.gh] - vbase(m, j, c.gh));
+  } else {
+    const real of = b.of;
+    xg = (c.x0 + of * m.ymed[j] * m.sinzmed[c.act]) * b.zvfac[mm] - m.ymed[j] * of * m.sinzmed[c.gh];
+  }
   dens[c.lgh] = dg;
   vx[c.lgh] = xg;
   vy[c.lgh] = c.y0;

@@ -1,6 +1,7 @@
 // f3d_fused.hip -- the FP64 build of the fused step (entry points f3d_fused_*):
 // bit-for-bit the reference's arithmetic.  The kernels are in f3d_fused_body.h.
 #include <cstring>
+#include <type_traits>
 
 #include "f3d_internal.h"
 #include "f3d_cell.h"

@@ -2,6 +2,7 @@
 // points f3d32_fused_*): FP32 fields and stencil arithmetic, FP64 ring sums,
 // residual/shift split and CFL.  The kernels are in f3d_fused_body.h.
 #include <cstring>
+#include <type_traits>
 #include <utility>
 #include <vector>
 

@@ -119,6 +119,9 @@ int make_mesh(f3d_hip_ctx* ctx, Scratch* S) {
   m.sinzmed = conv(d.sinzmed); m.coszmed = conv(d.coszmed); m.sinzmin = conv(d.sinzmin); m.coszmin = conv(d.coszmin);
   m.xmin64 = d.xmin; m.ymin64 = d.ymin; m.zmin64 = d.zmin; m.ymed64 = d.ymed; m.sinzmed64 = d.sinzmed;
   m.sxi64 = d.sxi; m.zmed64 = d.zmed; m.sinzmin64 = d.sinzmin; m.coszmin64 = d.coszmin;
+  m.sxj64 = d.sxj; m.sxk64 = d.sxk; m.syk64 = d.syk; m.invvj64 = d.invvj;
+  if (!ctx->vbase) return -1;  // set with f3d_hip_set_vbase before the first FP32 stage
+  m.vb = ctx->vbase;
   if (ctx->bctab) {
     const size_t ny3 = 3 * (size_t)d.sz, nz3 = 3 * (size_t)d.sy, nb = 2 * ny3 + 2 * nz3;
     if (hipMalloc(&S->bctab_r, nb * sizeof(real)) != hipSuccess) return -1;
@@ -301,6 +304,13 @@ __device__ __forceinline__ double seq_sum_pairs(const double* w, int n) {
   return acc;
 }
 
+// VxMed of ring (j,k) from the ring mean of the stored velocity: the
+// precision form stores deviations from V0(j,k), so the full mean adds V0
+__device__ __forceinline__ double vmed_full(double stored_mean, const dmesh& m, int j, int k) {
+  if constexpr (cell::kFix64) return stored_mean + cell::vbase(m, j, k);
+  return stored_mean;
+}
+
 }  // namespace
 
 // ----------------------------------------------------------------- kf_cfl
@@ -337,8 +347,8 @@ __device__ __forceinline__ void kd_cfl(int ring, const dmesh& m, const real* cs,
     ring_weights(w0, vxm + base, m.xmin64, m.nx);
     if (cp.fargo) ring_weights(w1, vxm + base + m.pitch, m.xmin64, m.nx);
     __syncthreads();
-    if (threadIdx.x == 0) sh[8] = seq_sum(w0, m.nx) / (m.XMAX - m.XMIN);
-    if (threadIdx.x == 64 && cp.fargo) sh[9] = seq_sum(w1, m.nx) / (m.XMAX - m.XMIN);
+    if (threadIdx.x == 0) sh[8] = vmed_full(seq_sum(w0, m.nx) / (m.XMAX - m.XMIN), m, j, k);
+    if (threadIdx.x == 64 && cp.fargo) sh[9] = vmed_full(seq_sum(w1, m.nx) / (m.XMAX - m.XMIN), m, j + 1, k);
   }
   __syncthreads();
   const double vm0 = sh[8];
@@ -388,7 +398,9 @@ __global__ void __launch_bounds__(WG) kf_vmed_rings(dmesh m, const real* __restr
   const int base = j * m.pitch + k * m.stride;
   ring_weights(w, v + base, m.xmin64, m.nx);
   __syncthreads();
-  if (threadIdx.x == 0) vxmed[j + k * m.sy] = (LONG ? seq_sum_pairs<16>(w, m.nx) : seq_sum_pairs<8>(w, m.nx)) / (m.XMAX - m.XMIN);
+  if (threadIdx.x == 0)
+    vxmed[j + k * m.sy] =
+        vmed_full((LONG ? seq_sum_pairs<16>(w, m.nx) : seq_sum_pairs<8>(w, m.nx)) / (m.XMAX - m.XMIN), m, j, k);
 }
 
 // The cfl of nf fluids in one launch: nf consecutive blocks of nrings
@@ -472,6 +484,27 @@ __device__ __forceinline__ VxcLoader<G> vxc(const real* vx, const dmesh& m, int 
   return VxcLoader<G>{cell::gl(vx, m, l, i), m.ymed, m.sinzmed, j, k, dom};
 }
 
+// The same for the viscous stress in the precision form, which stores the
+// deviation vx' from V0(j,k): the stress differentiates vx across rings and
+// planes, where V0 varies, so the loader returns the full V0 + vx'.
+template <int G>
+struct VxfLoader {
+  cell::GL v;
+  const dmesh* m;
+  int j, k;
+  real domega;
+  __device__ __forceinline__ real operator()(int di, int dj, int dk) const {
+    const real full = (real)((double)v(di, dj, dk) + cell::vbase(*m, j + dj, k + dk));
+    if (G == SPH) return full - domega * m->ymed[j + dj] * m->sinzmed[k + dk];
+    return full - domega * m->ymed[j + dj];
+  }
+};
+template <int G>
+__device__ __forceinline__ auto vx_stress(const real* vx, const dmesh& m, int l, int i, int j, int k, real dom) {
+  if constexpr (cell::kFix64) return VxfLoader<G>{cell::gl(vx, m, l, i), &m, j, k, dom};
+  else return vxc<G>(vx, m, l, i, j, k, dom);
+}
+
 // The artificial pressure of SubStep2_a (src/substep2_a.c:103-113) at a
 // cell, computed where SubStep2_b reads it, from the same loaders and with
 // the same arithmetic as the array the reference stores: q = art_q(rho,
@@ -494,7 +527,7 @@ __device__ __forceinline__ void kd_src_a(const dim3 bid, const dmesh& m, const r
   const int j = 1 + (int)bid.y;       // 1 .. sy-2
   const int k = DZ ? 1 + (int)bid.z : 0;
   const int l = i + j * m.pitch + k * m.stride;
-  const auto VX = vxc<G>(vx, m, l, i, j, k, domega);
+  const auto VX = vx_stress<G>(vx, m, l, i, j, k, domega);
   const cell::GL VY = cell::gl(vy, m, l, i);
   const cell::GL VZ = cell::gl(DZ ? vz : vy, m, l, i);
   const cell::GL R = cell::gl(rho, m, l, i);
@@ -521,7 +554,7 @@ __global__ void __launch_bounds__(WG) kf_src_a(dmesh m, const real* __restrict__
 // kf_src_b: the velocity update of SubStep1 + addviscosity + SubStep2_b on
 // active cells, in the order the reference applies them.
 template <int G, int DZ>
-__device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt, const real* __restrict__ rho, const real* __restrict__ energy, const real* __restrict__ vx, const real* __restrict__ vy, const real* __restrict__ vz, const double* __restrict__ pot, const real* __restrict__ txx, const real* __restrict__ tyy, const real* __restrict__ tzz, const real* __restrict__ txz, const real* __restrict__ tyx, const real* __restrict__ tzy, real* __restrict__ vxt, real* __restrict__ vyt, real* __restrict__ vzt, real* __restrict__ e2, int visc, int potential, int dust, int iso, real gm1, real omegaframe, real domega) {
+__device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt, const real* __restrict__ rho, const real* __restrict__ energy, const real* __restrict__ vx, const real* __restrict__ vy, const real* __restrict__ vz, const double* __restrict__ pot, const real* __restrict__ txx, const real* __restrict__ tyy, const real* __restrict__ tzz, const real* __restrict__ txz, const real* __restrict__ tyx, const real* __restrict__ tzy, real* __restrict__ vxt, real* __restrict__ vyt, real* __restrict__ vzt, real* __restrict__ e2, int visc, int potential, int dust, int iso, real gm1, double omegaframe, real domega) {
   const int i = bid.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int j = m.nghy + (int)bid.y;
@@ -559,7 +592,7 @@ __global__ void __launch_bounds__(WG) kf_src_b(dmesh m, real dt, const real* __r
                                                const double* __restrict__ pot, const real* __restrict__ txx, const real* __restrict__ tyy,
                                                const real* __restrict__ tzz, const real* __restrict__ txz, const real* __restrict__ tyx,
                                                const real* __restrict__ tzy, real* __restrict__ vxt, real* __restrict__ vyt, real* __restrict__ vzt, real* __restrict__ e2,
-                                               int visc, int potential, int dust, int iso, real gm1, real omegaframe,
+                                               int visc, int potential, int dust, int iso, real gm1, double omegaframe,
                                                real domega) {
   kd_src_b<G, DZ>(xcd_bid(), m, dt, rho, energy, vx, vy, vz, pot, txx, tyy, tzz, txz, tyx, tzy, vxt, vyt, vzt, e2, visc, potential, dust, iso, gm1, omegaframe, domega);
 }
@@ -583,7 +616,7 @@ __global__ void __launch_bounds__(WG) kf_src_b_fluids(dmesh m, real dt, SrcTab T
                                                       const real* __restrict__ txx, const real* __restrict__ tyy,
                                                       const real* __restrict__ tzz, const real* __restrict__ txz,
                                                       const real* __restrict__ tyx, const real* __restrict__ tzy,
-                                                      int potential, real gm1, real omegaframe, real domega) {
+                                                      int potential, real gm1, double omegaframe, real domega) {
   const dim3 b0 = xcd_bid();
   const int fl = (int)b0.z / nzb;
   const dim3 b(b0.x, b0.y, b0.z % nzb);
@@ -621,10 +654,34 @@ __global__ void __launch_bounds__(WG) kf_substep3(dmesh m, real dt, const real* 
 //   col.v(r)        velocity at the lower face of row r
 //   col.flux(r, a, b) conservative increment of row r from face fluxes a, b
 //   col.out(c, r, value) store of the updated quantity (c == NQ: density)
+// In the precision form the first Col::NXQ quantities (the azimuthal momenta)
+// are deviations Q' = Q - rho l0(r) from the profile's angular momentum
+// (col.base(r) = l0 at row r, FP64); their q' = Q'/rho is carried, and
+//   - a gradient takes the profile's jump across the face, db(r) =
+//     l0(r) - l0(r-1), so the limiter sees the full gradient;
+//   - a face value is held relative to l0 of the row below the face (the
+//     upper row's q' enters with + db(r));
+//   - the update of row r uses its lower face relative to l0(r) (- db(r)):
+//     Q'(r) + dt/V (F_rho(r) (q*(r) - l0(r)) - F_rho(r+1) (q*(r+1) - l0(r))),
+//     which is Q(r) + dt/V (F_rho q* - F_rho q*) - l0(r) (rho_new - rho):
+// the reference's update of the full momentum, rearranged so that every
+// FP32 operand is of the order of the deviation and of l0's jump across one
+// row, not of l0.  In the FP64 build NXQ is 0 and the arithmetic is the
+// reference's.
+// x + d for a deviation quantity; x itself (no addition) otherwise
+__device__ __forceinline__ real dsh(real x, real d, bool dev) { return dev ? x + d : x; }
+
 template <int NQ, class Col>
 __device__ __forceinline__ void column_sweep(Col& col, int r0, int r1, real dt) {
   constexpr int N = NQ + 1;
+  constexpr int NXQ = cell::kFix64 ? Col::NXQ : 0;
+  auto db = [&](int r) -> real {
+    if constexpr (NXQ > 0) return (real)(col.base(r) - col.base(r - 1));
+    return real(0.0);
+  };
   real A1[N], A2[N], G2[N], S1[N], F0[N], F1[N];
+  // db of the rows r, r+1, r+2 of the loop below (deviation quantities only)
+  real D0 = real(0.0), D1 = real(0.0), D2 = real(0.0);
   {
     real a[N][5];
 #pragma unroll
@@ -635,17 +692,27 @@ __device__ __forceinline__ void column_sweep(Col& col, int r0, int r1, real dt) 
       for (int c = 0; c < NQ; c++) a[c][t] = col.Q(c, r) / rr;
       a[NQ][t] = rr;
     }
+    real dm = real(0.0);
+    if constexpr (NXQ > 0) {
+      dm = db(r0 - 1);
+      D0 = db(r0);
+      D1 = db(r0 + 1);
+      D2 = db(r0 + 2);
+    }
 #pragma unroll
     for (int c = 0; c < N; c++) {
-      const real g0 = (a[c][1] - a[c][0]) / col.zs(r0 - 1);
-      const real g1 = (a[c][2] - a[c][1]) / col.zs(r0);
-      const real g2 = (a[c][3] - a[c][2]) / col.zs(r0 + 1);
-      const real g3 = (a[c][4] - a[c][3]) / col.zs(r0 + 2);
+      const bool dev = c < NXQ;
+      const real g0 = dsh(a[c][1] - a[c][0], dm, dev) / col.zs(r0 - 1);
+      const real g1 = dsh(a[c][2] - a[c][1], D0, dev) / col.zs(r0);
+      const real g2 = dsh(a[c][3] - a[c][2], D1, dev) / col.zs(r0 + 1);
+      const real g3 = dsh(a[c][4] - a[c][3], D2, dev) / col.zs(r0 + 2);
       const real sm = vanleer_slope(g0, g1);
       const real s0 = vanleer_slope(g1, g2);
       const real s1 = vanleer_slope(g2, g3);
-      F0[c] = vanleer_face(col.v(r0), dt, a[c][1], sm, col.zs(r0 - 1), a[c][2], s0, col.zs(r0));
-      F1[c] = vanleer_face(col.v(r0 + 1), dt, a[c][2], s0, col.zs(r0), a[c][3], s1, col.zs(r0 + 1));
+      F0[c] = vanleer_face(col.v(r0), dt, a[c][1], sm, col.zs(r0 - 1), dsh(a[c][2], D0, dev), s0,
+                           col.zs(r0));
+      F1[c] = vanleer_face(col.v(r0 + 1), dt, a[c][2], s0, col.zs(r0), dsh(a[c][3], D1, dev), s1,
+                           col.zs(r0 + 1));
       A1[c] = a[c][3];
       A2[c] = a[c][4];
       G2[c] = g3;
@@ -656,20 +723,26 @@ __device__ __forceinline__ void column_sweep(Col& col, int r0, int r1, real dt) 
     const real v0 = col.v(r), v1 = col.v(r + 1);
     // UpdateY/Z (src/update_y.c:68-70), then the density last (src/update_density_y.c:66-68)
 #pragma unroll
-    for (int c = 0; c < NQ; c++)
-      col.out(c, r, col.Q(c, r) + col.flux(r, v0 * F0[c] * F0[NQ], v1 * F1[c] * F1[NQ]));
-    col.out(NQ, r, col.rho(r) + col.flux(r, v0 * F0[NQ], v1 * F1[NQ]));
+    for (int c = 0; c < NQ; c++) {
+      const real f0 = (c < NXQ) ? F0[c] - D0 : F0[c];
+      col.out(c, r, cell::upd(col.mesh(), col.Q(c, r), col.flux(r, v0 * f0 * F0[NQ], v1 * F1[c] * F1[NQ]), col.L(r),
+                              Col::kSweep + c));
+    }
+    col.out(NQ, r, cell::upd(col.mesh(), col.rho(r), col.flux(r, v0 * F0[NQ], v1 * F1[NQ]), col.L(r), Col::kSweep + 15));
     if (r + 1 >= r1) break;
     // advance the window by one row: row r+3 enters
     const int rn = r + 3;
     const real rr = col.rho(rn);
     const real zsn = col.zs(rn), zs2 = col.zs(r + 2), zs1 = col.zs(r + 1), v2 = col.v(r + 2);
+    real D3 = real(0.0);
+    if constexpr (NXQ > 0) D3 = db(rn);
 #pragma unroll
     for (int c = 0; c < N; c++) {
+      const bool dev = c < NXQ;
       const real an = (c < NQ) ? col.Q(c, rn) / rr : rr;
-      const real g3 = (an - A2[c]) / zsn;
+      const real g3 = dsh(an - A2[c], D3, dev) / zsn;
       const real s2 = vanleer_slope(G2[c], g3);
-      const real f2 = vanleer_face(v2, dt, A1[c], S1[c], zs1, A2[c], s2, zs2);
+      const real f2 = vanleer_face(v2, dt, A1[c], S1[c], zs1, dsh(A2[c], D2, dev), s2, zs2);
       F0[c] = F1[c];
       F1[c] = f2;
       A1[c] = A2[c];
@@ -677,6 +750,9 @@ __device__ __forceinline__ void column_sweep(Col& col, int r0, int r1, real dt) 
       G2[c] = g3;
       S1[c] = s2;
     }
+    D0 = D1;
+    D1 = D2;
+    D2 = D3;
   }
 }
 
@@ -687,7 +763,14 @@ __device__ __forceinline__ void column_sweep(Col& col, int r0, int r1, real dt) 
 // hide the FP64 latency) for one extra density sweep per column.
 template <class Col, int NQ, int FIRST, int COUNT, bool WRITE_RHO>
 struct ColPart {
+  // the deviation quantities of the column among those carried here
+  static constexpr int NXQ = FIRST < Col::NXQ ? (Col::NXQ - FIRST < COUNT ? Col::NXQ - FIRST : COUNT) : 0;
+  static_assert(FIRST == 0 || FIRST >= Col::NXQ, "the deviation quantities lead each part");
+  static constexpr int kSweep = Col::kSweep + FIRST;
   const Col& c;
+  __device__ __forceinline__ double base(int r) const { return c.base(r); }
+  __device__ __forceinline__ const dmesh& mesh() const { return c.mesh(); }
+  __device__ __forceinline__ int L(int r) const { return c.L(r); }
   __device__ __forceinline__ real rho(int r) const { return c.rho(r); }
   __device__ __forceinline__ real Q(int q, int r) const { return c.Q(FIRST + q, r); }
   __device__ __forceinline__ real zs(int r) const { return c.zs(r); }
@@ -718,11 +801,15 @@ __device__ __forceinline__ void column_sweep_part(const Col& col, int grp, int r
 // (adiabatic).  out_[c] receives quantity c, out_[NQ] the density.
 template <int G>
 struct ColY2 {
+  static constexpr int NXQ = 2;  // Mpx, Mmx: deviations in the precision form
+  static constexpr int kSweep = 16;  // stream of the stochastic rounding
   const dmesh* m;
   const real *rho_in, *vxt, *vyt, *e_in;
   real* out_[6];
   int i, ixp;
   real of, dt;
+  __device__ __forceinline__ double base(int r) const { return cell::lbase<G>(*m, r, 0, (double)of); }
+  __device__ __forceinline__ const dmesh& mesh() const { return *m; }
   __device__ __forceinline__ int L(int r) const { return i + r * m->pitch; }
   __device__ __forceinline__ real rho(int r) const { return rho_in[L(r)]; }
   __device__ __forceinline__ real Q(int c, int r) const {
@@ -748,11 +835,15 @@ struct ColY2 {
 // quantity 6 is the energy (adiabatic).
 template <int G>
 struct ColZ {
+  static constexpr int NXQ = 2;  // Mpx, Mmx: deviations in the precision form
+  static constexpr int kSweep = 32;  // stream of the stochastic rounding
   const dmesh* m;
   const real *rho_in, *vxt, *vyt, *vzt, *e_in;
   real* out_[8];
   int i, j, ixp;
   real of, dt;
+  __device__ __forceinline__ double base(int r) const { return cell::lbase<G>(*m, j, r, (double)of); }
+  __device__ __forceinline__ const dmesh& mesh() const { return *m; }
   __device__ __forceinline__ int L(int r) const { return i + j * m->pitch + r * m->stride; }
   __device__ __forceinline__ real rho(int r) const { return rho_in[L(r)]; }
   __device__ __forceinline__ real Q(int c, int r) const {
@@ -790,14 +881,19 @@ struct ColZ {
 };
 
 // Y column of the 3-D step: reads the Z-swept density, momenta (and energy).
+template <int G>
 struct ColY3 {
+  static constexpr int NXQ = 2;  // Mpx, Mmx: deviations in the precision form
+  static constexpr int kSweep = 16;  // stream of the stochastic rounding
   const dmesh* m;
   const real* rho_in;
   const real* q_in[7];
   const real* vyt;
   real* out_[8];
   int i, k;
-  real dt;
+  real dt, of;
+  __device__ __forceinline__ double base(int r) const { return cell::lbase<G>(*m, r, k, (double)of); }
+  __device__ __forceinline__ const dmesh& mesh() const { return *m; }
   __device__ __forceinline__ int L(int r) const { return i + r * m->pitch + k * m->stride; }
   __device__ __forceinline__ real rho(int r) const { return rho_in[L(r)]; }
   __device__ __forceinline__ real Q(int c, int r) const { return q_in[c][L(r)]; }
@@ -846,7 +942,7 @@ __global__ void __launch_bounds__(BXF) kf_sweep_y3(dmesh m, real dt, const real*
                                                    const real* __restrict__ z4, const real* __restrict__ z5, const real* __restrict__ ze,
                                                    const real* __restrict__ vyt, real* __restrict__ rho_out, real* __restrict__ q0, real* __restrict__ q1,
                                                    real* __restrict__ q2, real* __restrict__ q3, real* __restrict__ q4, real* __restrict__ q5, real* __restrict__ e_out,
-                                                   int rows_per) {
+                                                   int rows_per, real of) {
   const dim3 b = xcd_bid();
   const int i = b.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
@@ -854,8 +950,8 @@ __global__ void __launch_bounds__(BXF) kf_sweep_y3(dmesh m, real dt, const real*
   const int seg = SPLIT ? (int)b.y >> 1 : (int)b.y;
   const int r0 = m.nghy + seg * rows_per;
   const int r1 = min(r0 + rows_per, m.ny + m.nghy);
-  ColY3 col{&m, rho_in, {z0, z1, z2, z3, z4, z5, ze}, vyt,
-            {q0, q1, q2, q3, q4, q5, AD ? e_out : rho_out, rho_out}, i, k, dt};
+  ColY3<G> col{&m, rho_in, {z0, z1, z2, z3, z4, z5, ze}, vyt,
+               {q0, q1, q2, q3, q4, q5, AD ? e_out : rho_out, rho_out}, i, k, dt, of};
   if (SPLIT) column_sweep_part<6 + AD>(col, 1 + ((int)b.y & 1), r0, r1, dt);
   else column_sweep<6 + AD>(col, r0, r1, dt);
 }
@@ -897,6 +993,25 @@ struct XQuant {
   static constexpr int NL = GRP == 0 ? NQ : GRP == 1 ? NA : NQ - NA;  // carried here
   static constexpr int first = GRP == 2 ? NA : 0;  // global index of local quantity 0
 };
+
+// The precision form's update of the PPA pass of the FARGO shift.  The
+// advection velocity of that pass, the fractional part of VxMed dt / dx, is
+// the same for every cell of a ring and nearly the same step after step, and
+// so are the mesh factors SurfX dt InvVol of the update: rounded to FP32 they
+// would move each ring by the same tiny fraction of a cell too far or too
+// short every step, a phase error that grows with the step count.  So the
+// velocity, recomputed in FP64 from VxMed, and the mesh factors multiply the
+// FP32 face terms in FP64.
+template <int G>
+__device__ __forceinline__ double ppa_uvel64(const dmesh& m, int i, int j, int k, double vmed, double dt64) {
+  const double zs = zone_size_x64<G>(m, i, j, k);
+  const double ntilde = vmed * dt64 / zs;
+  return (ntilde - floor(ntilde + 0.5)) * zs / dt64;
+}
+// SurfX dt InvVol of cell i in FP64 (the factors of flux_x)
+__device__ __forceinline__ double surfx_dt_invvol64(const dmesh& m, int i, int j, int k, double dt64) {
+  return (((m.sxj64[j] * m.sxk64[k]) * dt64) * m.invvj64[j]) / (m.syk64[k] * m.sxi64[i]);
+}
 
 // An empty asm that "changes" the LDS row stride after each barrier: the
 // compiler then computes each phase's LDS addresses in that phase instead of
@@ -953,12 +1068,14 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
   } else {
     __syncthreads();
     // ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30), FP64
-    if (t == 0) sh[0] = seq_sum(Wt, nx) / (m.XMAX - m.XMIN);
+    if (t == 0) sh[0] = vmed_full(seq_sum(Wt, nx) / (m.XMAX - m.XMIN), m, j, k);
     __syncthreads();
     vmed = sh[0];
   }
   // ComputeResidual (src/compute_residual.c:68-73), FP64: cell residual into V
+  // (the stored velocity is the deviation from V0 in the precision form)
   int ns = 0;
+  const double vb = cell::vbase(m, j, k);
 #pragma unroll
   for (int s = 0; s < CPT; s++) {
     const int i = t + s * nt;
@@ -968,10 +1085,12 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
         const double ntilde = vmed * dt64 / zs;
         const double nround = floor(ntilde + 0.5);
         if (i == 0) sh[1] = nround;
-        V[i] = (real)(vt[s] - vmed);
+        if constexpr (cell::kFix64) V[i] = (real)(vt[s] - (vmed - vb));
+        else V[i] = (real)(vt[s] - vmed);
         vu[s] = (real)((ntilde - nround) * zs / dt64);
       } else {
-        V[i] = (real)vt[s];
+        if constexpr (cell::kFix64) V[i] = (real)(vt[s] + vb);
+        else V[i] = (real)vt[s];
       }
     }
   }
@@ -1160,12 +1279,25 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
         const int ip = ixp_of(i, nx);
         const real v0 = V[i], v1 = V[ip];
         const real* ds = FC + NL * nxv;
+        if (cell::kFix64 && ppa) {
+          const double u0 = ppa_uvel64<G>(m, i, j, k, vmed, dt64), u1 = ppa_uvel64<G>(m, ip, j, k, vmed, dt64);
+          const double met = surfx_dt_invvol64(m, i, j, k, dt64);
 #pragma unroll
-        for (int c = 0; c < NL; c++) {
-          const real* f = FC + c * nxv;
-          st[s][c] += cell::flux_x(m, i, j, k, dt, v0 * f[i] * ds[i], v1 * f[ip] * ds[ip]);
+          for (int c = 0; c < NL; c++) {
+            const real* f = FC + c * nxv;
+            st[s][c] = cell::upd64(m, st[s][c], ((double)(f[i] * ds[i]) * u0 - (double)(f[ip] * ds[ip]) * u1) * met,
+                                   base + i, 64 + XQ::first + c);
+          }
+          st[s][NL] = cell::upd64(m, st[s][NL], ((double)ds[i] * u0 - (double)ds[ip] * u1) * met, base + i, 79);
+        } else {
+#pragma unroll
+          for (int c = 0; c < NL; c++) {
+            const real* f = FC + c * nxv;
+            st[s][c] = cell::upd(m, st[s][c], cell::flux_x(m, i, j, k, dt, v0 * f[i] * ds[i], v1 * f[ip] * ds[ip]),
+                                 base + i, 48 + XQ::first + c);
+          }
+          st[s][NL] = cell::upd(m, st[s][NL], cell::flux_x(m, i, j, k, dt, v0 * ds[i], v1 * ds[ip]), base + i, 63);
         }
-        st[s][NL] += cell::flux_x(m, i, j, k, dt, v0 * ds[i], v1 * ds[ip]);
       }
     }
   }
@@ -1233,7 +1365,9 @@ __device__ __forceinline__ void kd_vmed(const dim3 bid, const dmesh& m, const re
   const int base = j * m.pitch + k * m.stride;
   ring_weights(w, vxt + base, m.xmin64, m.nx);
   __syncthreads();
-  if (threadIdx.x == 0) vxmed[j + k * m.sy] = (LONG ? seq_sum_pairs<16>(w, m.nx) : seq_sum_pairs<8>(w, m.nx)) / (m.XMAX - m.XMIN);
+  if (threadIdx.x == 0)
+    vxmed[j + k * m.sy] =
+        vmed_full((LONG ? seq_sum_pairs<16>(w, m.nx) : seq_sum_pairs<8>(w, m.nx)) / (m.XMAX - m.XMIN), m, j, k);
 }
 template <int G, int DZ, int LONG>
 __global__ void __launch_bounds__(64) kf_vmed(dmesh m, VmTab T, int nrings) {
@@ -1295,10 +1429,12 @@ __device__ __forceinline__ void kd_sweep_x_tile(const dim3 bid, const dmesh& m, 
       const double zs64 = zone_size_x64<G>(m, g, j, k);
       const double ntilde = vmed * dt64 / zs64;
       const double nround = floor(ntilde + 0.5);
-      vp = (real)(v - vmed);
+      if constexpr (cell::kFix64) vp = (real)(v - (vmed - cell::vbase(m, j, k)));
+      else vp = (real)(v - vmed);
       vu = (real)((ntilde - nround) * zs64 / dt64);
     } else {
-      vp = (real)v;
+      if constexpr (cell::kFix64) vp = (real)(v + cell::vbase(m, j, k));
+      else vp = (real)v;
     }
     V[p] = vp;
   }
@@ -1407,12 +1543,26 @@ __device__ __forceinline__ void kd_sweep_x_tile(const dim3 bid, const dmesh& m, 
     if (p >= ulo && p < uhi) {
       const real v0 = vp, v1 = V[p + 1];
       const real* ds = FC + NQ * W;
+      if (cell::kFix64 && pass == 1) {
+        // the precision form's PPA update in FP64 factors (see ppa_uvel64)
+        const double u0 = ppa_uvel64<G>(m, g, j, k, vmed, dt64), u1 = ppa_uvel64<G>(m, gidx(p + 1), j, k, vmed, dt64);
+        const double met = surfx_dt_invvol64(m, g, j, k, dt64);
 #pragma unroll
-      for (int c = 0; c < NQ; c++) {
-        const real* f = FC + c * W;
-        st[c] += cell::flux_x(m, g, j, k, dt, v0 * f[p] * ds[p], v1 * f[p + 1] * ds[p + 1]);
+        for (int c = 0; c < NQ; c++) {
+          const real* f = FC + c * W;
+          st[c] = cell::upd64(m, st[c], ((double)(f[p] * ds[p]) * u0 - (double)(f[p + 1] * ds[p + 1]) * u1) * met,
+                              base + g, 64 + c);
+        }
+        st[NQ] = cell::upd64(m, st[NQ], ((double)ds[p] * u0 - (double)ds[p + 1] * u1) * met, base + g, 79);
+      } else {
+#pragma unroll
+        for (int c = 0; c < NQ; c++) {
+          const real* f = FC + c * W;
+          st[c] = cell::upd(m, st[c], cell::flux_x(m, g, j, k, dt, v0 * f[p] * ds[p], v1 * f[p + 1] * ds[p + 1]),
+                            base + g, 48 + c);
+        }
+        st[NQ] = cell::upd(m, st[NQ], cell::flux_x(m, g, j, k, dt, v0 * ds[p], v1 * ds[p + 1]), base + g, 63);
       }
-      st[NQ] += cell::flux_x(m, g, j, k, dt, v0 * ds[p], v1 * ds[p + 1]);
     }
     if (pass == 0 && fargo) __syncthreads();  // the PPA pass rewrites AQ and V
   }
@@ -1591,36 +1741,40 @@ struct FluidSet {
 template <int NF>
 __device__ __forceinline__ void drag_solve(const FluidSet& F, real* const* v, int l, int idm, double dt,
                                            double omega) {
-  real m[NF * NF], b[NF];
+  // the precision form solves in FP64: the stiff drag of small grains
+  // couples the fluids' velocities so strongly that an FP32 solve rounds
+  // every ring's cells alike, a torque that accumulates step after step
+  using dreal = std::conditional_t<cell::kFix64, double, real>;
+  dreal m[NF * NF], b[NF];
 #pragma unroll
   for (int o = 0; o < NF; o++) {
 #pragma unroll
     for (int p = 0; p < NF; p++) {
-      const real rho_p = real(0.5) * (F.rho[p][l] + F.rho[p][idm]);
+      const dreal rho_p = dreal(0.5) * ((dreal)F.rho[p][l] + (dreal)F.rho[p][idm]);
       if (p != o) {
-        const real rho_o = real(0.5) * (F.rho[o][l] + F.rho[o][idm]);
-        if (p > o) m[p + o * NF] = -(real)dt * (real)omega * (real)F.alpha[p + o * NF] * rho_p / rho_o;
-        else m[p + o * NF] = -(real)dt * (real)omega * (real)F.alpha[p + o * NF];
+        const dreal rho_o = dreal(0.5) * ((dreal)F.rho[o][l] + (dreal)F.rho[o][idm]);
+        if (p > o) m[p + o * NF] = -(dreal)dt * (dreal)omega * (dreal)F.alpha[p + o * NF] * rho_p / rho_o;
+        else m[p + o * NF] = -(dreal)dt * (dreal)omega * (dreal)F.alpha[p + o * NF];
       } else {
-        real sum = real(0.0);
+        dreal sum = dreal(0.0);
 #pragma unroll
         for (int q = 0; q < NF; q++) {
           if (q != p) {
-            const real rho_q = real(0.5) * (F.rho[q][l] + F.rho[q][idm]);
-            if (q > p) sum += (real)F.alpha[q + p * NF] * rho_q / rho_p;
-            else sum += (real)F.alpha[q + p * NF];
+            const dreal rho_q = dreal(0.5) * ((dreal)F.rho[q][l] + (dreal)F.rho[q][idm]);
+            if (q > p) sum += (dreal)F.alpha[q + p * NF] * rho_q / rho_p;
+            else sum += (dreal)F.alpha[q + p * NF];
           }
         }
-        m[p + p * NF] = real(1.0) + (real)dt * (real)omega * sum;
+        m[p + p * NF] = dreal(1.0) + (dreal)dt * (dreal)omega * sum;
       }
     }
-    b[o] = v[o][l];
+    b[o] = (dreal)v[o][l];
   }
 #pragma unroll
   for (int ir = 0; ir < NF - 1; ir++) {
     // pivot as src/gauss.h:8-14: the comparison uses |m|, the stored maximum
     // the signed value
-    real max_value = real(-1);
+    dreal max_value = dreal(-1);
     int ir_max = ir;
 #pragma unroll
     for (int ir2 = ir; ir2 < NF; ir2++) {
@@ -1634,18 +1788,18 @@ __device__ __forceinline__ void drag_solve(const FluidSet& F, real* const* v, in
       if (ir_max == r) {
 #pragma unroll
         for (int ic = 0; ic < NF; ic++) {
-          const real temp = m[ic + ir * NF];
+          const dreal temp = m[ic + ir * NF];
           m[ic + ir * NF] = m[ic + r * NF];
           m[ic + r * NF] = temp;
         }
-        const real temp = b[ir];
+        const dreal temp = b[ir];
         b[ir] = b[r];
         b[r] = temp;
       }
     }
 #pragma unroll
     for (int ir2 = ir + 1; ir2 < NF; ir2++) {
-      const real factor = m[ir + ir2 * NF] / m[ir + ir * NF];
+      const dreal factor = m[ir + ir2 * NF] / m[ir + ir * NF];
 #pragma unroll
       for (int ic = 0; ic < NF; ic++) m[ic + ir2 * NF] -= m[ic + ir * NF] * factor;
       b[ir2] -= b[ir] * factor;
@@ -1658,7 +1812,7 @@ __device__ __forceinline__ void drag_solve(const FluidSet& F, real* const* v, in
     b[ir] /= m[ir + ir * NF];
   }
 #pragma unroll
-  for (int o = 0; o < NF; o++) v[o][l] = b[o];
+  for (int o = 0; o < NF; o++) v[o][l] = (real)b[o];
 }
 
 // Collisions(dt, 1) (src/collisions.c:156-189) on V_temp in place over the
@@ -1901,7 +2055,7 @@ static cell::BcPar bc_par_f(const f3d_hip_ctx* ctx, int side) {
   cell::BcPar b;
   b.p3d = (ctx->ph.bc == 2);
   b.energy = ctx->ph.energy_bc && ctx->f.Energy != nullptr;
-  b.of = (real)ctx->ph.OMEGAFRAME;
+  b.of = ctx->ph.OMEGAFRAME;
   for (int q = 0; q < 3; q++) {
     b.dens[q] = (real)(side ? ctx->ph.ymax_dens[q] : ctx->ph.ymin_dens[q]);
     b.vsq[q] = (real)(side ? ctx->ph.ymax_vsq[q] : ctx->ph.ymin_vsq[q]);
@@ -2059,7 +2213,7 @@ static int finish_step(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step,
   dp.Z_inf = (real)ctx->ph.Z_inf; dp.Z_sup = (real)ctx->ph.Z_sup;
   dp.y_min = (real)ctx->ph.ymin_par; dp.y_max = (real)ctx->ph.ymax_par;
   dp.z_min = (real)ctx->ph.zmin_par; dp.z_max = (real)ctx->ph.zmax_par;
-  dp.ds = (real)ctx->ph.taudamp; dp.of = (real)ctx->ph.OMEGAFRAME; dp.of0 = (real)ctx->ph.OMEGAFRAME0;
+  dp.ds = (real)ctx->ph.taudamp; dp.of = ctx->ph.OMEGAFRAME; dp.of0 = ctx->ph.OMEGAFRAME0;
   dp.periodic_z = ctx->ph.periodicz;
   FinTab T;
   memset(&T, 0, sizeof(T));
@@ -2126,7 +2280,7 @@ static int yz_sweeps(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const RF* 
       dim3 gy((m.nx + BXF - 1) / BXF, ((m.ny + ry - 1) / ry) * sp, m.nz);
       f3d_stage_begin(ctx, F3D_ST_SWEEP_Y);
       FDISPATCH_ADS(ctx, kf_sweep_y3, gy, rdt, Z[6], Z[0], Z[1], Z[2], Z[3], Z[4], Z[5], Z[7], f.Vy_temp, t.rhoy,
-                    t.m[0], t.m[1], t.m[2], t.m[3], t.m[4], t.m[5], t.ey, ry);
+                    t.m[0], t.m[1], t.m[2], t.m[3], t.m[4], t.m[5], t.ey, ry, of);
       f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
     }
   } else {
@@ -2346,10 +2500,20 @@ static int x_sweep_ring(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const R
 // fluid's tiles), then each fluid's NewVelocity_y/z, damping and ghost fill.
 // Fluids do not interact in Transport, so this equals transporting them one
 // after the other (src/main.c:420).
+// the key of the stochastic rounding of a stage (precision form): a hash of
+// the step's start time and the stage, so that a run restarted from an
+// output continues with the same keys as the run that wrote it
+static void next_salt(const f3d_hip_ctx* ctx, Scratch* S, unsigned stage) {
+  unsigned long long b;
+  memcpy(&b, &ctx->ph.PhysicalTime, sizeof(b));
+  S->m.salt = cell::sr_hash((unsigned)b ^ cell::sr_hash((unsigned)(b >> 32) + 0x9E3779B9U * stage));
+}
+
 static int transport_fluids(f3d_hip_ctx* ctx, double dt, int nf, const f3d_fields* fl, int end_of_step) {
   Scratch* S;
   F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
   F3D_REQUIRE(ctx, nf >= 1 && nf <= 8, "transport: 1 to 8 fluids");
+  next_salt(ctx, S, 2);
   RF f[8];
   TSet t[8];
   for (int k = 0; k < nf; k++) {
@@ -2670,6 +2834,7 @@ int F3D_API(fused_dust_diffusion)(f3d_hip_ctx* ctx, double dt, int nf, double* c
 int F3D_API(fused_sources)(f3d_hip_ctx* ctx, double dt, double domega) {
   Scratch* S;
   F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
+  next_salt(ctx, S, 1);
   const dmesh& m = S->m;
   const RF f = rfields(ctx);
   cell::ViscPar vp;
@@ -2698,7 +2863,7 @@ int F3D_API(fused_sources)(f3d_hip_ctx* ctx, double dt, double domega) {
   FDISPATCH(ctx, kf_src_b, gb, dim3(BXF), 0, rdt, f.Density, f.Energy, f.Vx, f.Vy, f.Vz, ctx->f.Pot, f.Mmx, f.Mmy, f.Mmz,
             f.Mpx, f.Mpy, f.Mpz, f.Vx_temp, f.Vy_temp, f.Vz_temp,
             S->f[FS_E2], visc, pot, (int)(ctx->ph.fluidtype == 2), iso, (real)(ctx->ph.gamma - 1.0),
-            (real)ctx->ph.OMEGAFRAME, rdom);
+            ctx->ph.OMEGAFRAME, rdom);
   f3d_stage_end(ctx, F3D_ST_SRC_B);
   if (!iso && ctx->ph.fluidtype == 1) {
     // SubStep3 (src/algogas.c:110-112)
@@ -2723,6 +2888,7 @@ int F3D_API(fused_sources_fluids)(f3d_hip_ctx* ctx, double dt, double domega, in
   Scratch* S;
   F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
   F3D_REQUIRE(ctx, nf >= 1 && nf <= 8 && fl && type, "sources: 1 to 8 fluids");
+  next_salt(ctx, S, 1);
   F3D_REQUIRE(ctx, ctx->ph.eos != 2, "sources of several fluids in one launch: isothermal runs");
   const dmesh& m = S->m;
   RF f[8];
@@ -2763,7 +2929,7 @@ int F3D_API(fused_sources_fluids)(f3d_hip_ctx* ctx, double dt, double domega, in
   dim3 gb((m.nx + BXF - 1) / BXF, m.ny, nzb * nf);
   f3d_stage_begin(ctx, F3D_ST_SRC_B);
   FDISPATCH(ctx, kf_src_b_fluids, gb, dim3(BXF), 0, rdt, T, nzb, ctx->f.Pot, sh.Mmx, sh.Mmy, sh.Mmz, sh.Mpx, sh.Mpy,
-            sh.Mpz, ctx->ph.potential, (real)(ctx->ph.gamma - 1.0), (real)ctx->ph.OMEGAFRAME, rdom);
+            sh.Mpz, ctx->ph.potential, (real)(ctx->ph.gamma - 1.0), ctx->ph.OMEGAFRAME, rdom);
   f3d_stage_end(ctx, F3D_ST_SRC_B);
   // mid-step ghost fill (src/algogas.c:116-118) on every fluid's V_temp
   f3d_stage_begin(ctx, F3D_ST_BC_MID);

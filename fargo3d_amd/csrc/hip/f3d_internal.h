@@ -40,6 +40,9 @@ struct f3d_hip_ctx {
   f3d_fields f{};                   // bound device buffers (selected fluid + scratch)
   f3d_phys ph{};                    // host copy of the run parameters
   double* bctab = nullptr;          // device copy of the per-(m,k)/(m,j) boundary factors
+  // precision form: the FP64 ring profile V0(j,k) (sy*sz) that its azimuthal
+  // velocities are stored as FP32 deviations from (f3d_hip_set_vbase)
+  double* vbase = nullptr;
   const double* d_ymin_dexp = nullptr;
   const double* d_ymax_dexp = nullptr;
   const double* d_zmin_dens = nullptr;

@@ -10,6 +10,7 @@
 // path live in f3d_fused.hip and reuse the per-cell arithmetic written here.
 #include "f3d_internal.h"
 #include "f3d_cell.h"
+#include "../host/f3d_host.h"
 
 using namespace f3d;
 
@@ -548,6 +549,68 @@ __global__ void k_bound_z(dmesh m, int side0, double* dens, double* vx, double* 
   cell::bound_z_cell(m, side, i, mm, j, dens, vx, vy, vz, e, side ? bp1 : bp0, side ? zdens1 : zdens0);
 }
 
+// CondInit on the device (f3d_CondInit): the jump-ahead of glibc's drand48
+// and the per-cell arithmetic of the host's f3d_ic_cell, operation for
+// operation (FP64, no contraction), so the host and the device give the same
+// bits -- except the log-normal set-up, whose exp/log/cos are the device's
+// own (within a few ulp of glibc's).
+struct IcJump {
+  unsigned long long x0;                  // state before the first draw
+  unsigned long long A[48], C[48];        // X -> A X + C: 2^b * ndraw draws ahead
+};
+struct IcPar {
+  int mode, ndraw, adiabatic;
+  double noise, h, gamma;
+};
+template <typename T>
+__global__ void k_condinit(dmesh m, IcPar d, IcJump jp, const double* __restrict__ R, const double* __restrict__ V,
+                           const double* __restrict__ CS, const double* __restrict__ E, T* __restrict__ dens,
+                           T* __restrict__ en, T* __restrict__ vx, T* __restrict__ vy, T* __restrict__ vz,
+                           const double* __restrict__ vb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nx) return;
+  const int j = blockIdx.y, k = blockIdx.z;
+  const size_t l = (size_t)i + (size_t)j * m.pitch + (size_t)k * m.stride, l2 = (size_t)j + (size_t)k * m.sy;
+  constexpr unsigned long long MASK = (1ULL << 48) - 1;
+  double u[3] = {0.0, 0.0, 0.0};
+  if (d.ndraw) {
+    // the cell's place in the set-ups' loop order (x fastest, then y, then z)
+    unsigned long long n = (unsigned long long)l2 * (unsigned long long)m.nx + (unsigned long long)i;
+    unsigned long long X = jp.x0;
+    for (int b = 0; n; b++, n >>= 1)
+      if (n & 1) X = (jp.A[b] * X + jp.C[b]) & MASK;
+    for (int q = 0; q < 3; q++) {
+      if (q >= d.ndraw) break;
+      X = (0x5DEECE66DULL * X + 0xBULL) & MASK;
+      u[q] = (double)X * 0x1p-48;  // erand48: the 48 bits as the fraction of 1 + x, minus 1
+    }
+  }
+  const double Rv = R[l2], Vv = V[l2];
+  double r = Rv, x = Vv, y = 0.0, e;
+  if (d.mode == F3D_IC_PLAIN) {
+    e = E[l2];
+  } else {
+    if (d.mode == F3D_IC_KEPLER) {
+      r = Rv * (1.0 + d.noise * (u[0] - .5));
+    } else if (d.mode == F3D_IC_FARGO) {
+      r = Rv * (1.0 + d.noise * (u[0] - .5));
+      x = Vv * (1. + d.h * d.noise * (u[1] - .5));
+      y = CS[l2] * d.noise * (u[2] - .5);
+    } else if (d.noise != 0.0) {
+      double u1 = u[0];
+      const double u2 = u[1];
+      if (u1 < 1e-300) u1 = 1e-300;
+      r *= exp(d.noise * (sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2)));
+    }
+    e = d.adiabatic ? E[l2] * r / (d.gamma - 1.0) : CS[l2];
+  }
+  dens[l] = (T)r;
+  en[l] = (T)e;
+  vx[l] = vb ? (T)(x - vb[l2]) : (T)x;
+  vy[l] = (T)y;
+  if (vz) vz[l] = (T)0.0;
+}
+
 __global__ void k_init_stockholm(dmesh m, const double* rho, const double* vx, const double* vy, const double* vz,
                                  const double* e, double* rho0, double* vx0, double* vy0, double* vz0, double* e0,
                                  int adiabatic) {
@@ -568,19 +631,28 @@ __global__ void k_init_stockholm(dmesh m, const double* rho, const double* vx, c
 // each active cell into an FP64 array, with the reference's association
 // (Vol(i,j,k) expands to Syk(k)*Sxi(i)/InvVj(j) inside the product).  T is the
 // field type: double, or float for the precision form, whose values are
-// widened before any arithmetic so that its monitors are FP64 sums.
+// widened before any arithmetic so that its monitors are FP64 sums; vb is the
+// precision form's ring profile of the azimuthal velocity (its Vx holds the
+// deviation from it), null for the FP64 form.
 enum { MON_DENS, MON_MOMX, MON_MOMY, MON_MOMZ, MON_TORQ };
 
 template <int G, int DZ, int Q, typename T>
 __global__ void k_mon(dmesh m, Range r, const T* __restrict__ dens, const T* __restrict__ v,
-                      double* __restrict__ out, double omegaframe, double xp, double yp, double zp, double rsm2) {
+                      double* __restrict__ out, double omegaframe, double xp, double yp, double zp, double rsm2,
+                      const double* __restrict__ vb) {
   CELL_INDEX;
   const double rho = (double)dens[l];
   double q;
   if (Q == MON_DENS) {
     q = mul_vol(m, rho, i, j, k);
   } else if (Q == MON_MOMX) {
-    const double vm = .5 * ((double)v[l] + (double)v[lxp_of(l, i, m.nx)]);
+    double v0 = (double)v[l], v1 = (double)v[lxp_of(l, i, m.nx)];
+    if (vb) {
+      // the full velocities, as a host read-back of the field forms them
+      v0 += vb[j + k * m.sy];
+      v1 += vb[j + k * m.sy];
+    }
+    const double vm = .5 * (v0 + v1);
     if (G == SPH) {
       const double rcyl = m.ymed[j] * m.sinzmed[k];
       q = mul_vol(m, ((vm + rcyl * omegaframe) * rcyl) * rho, i, j, k);
@@ -1161,6 +1233,56 @@ int f3d_StockholmBoundary(f3d_hip_ctx* ctx, double dt) {
   return 0;
 }
 
+// the device CondInit of either form (fp32: FP32 fields, vx as the deviation
+// from the handle's profile)
+static int condinit(f3d_hip_ctx* ctx, const f3d_ic_desc* d, void* dens, void* energy, void* vx, void* vy, void* vz,
+                    int fp32) {
+  F3D_REQUIRE(ctx, d && d->R && d->V && d->CS && d->E, "CondInit: missing profiles");
+  F3D_REQUIRE(ctx, dens && energy && vx && vy && (vz || !ctx->m.dimz), "CondInit: missing fields");
+  F3D_REQUIRE(ctx, d->ndraw >= 0 && d->ndraw <= 3, "CondInit: at most three draws per cell");
+  F3D_REQUIRE(ctx, !fp32 || ctx->vbase, "CondInit: the precision form's profile is not set");
+  const dmesh& m = ctx->m;
+  const size_t n2 = (size_t)m.sy * m.sz;
+  double* prof = nullptr;
+  F3D_CHECK(ctx, hipMalloc(&prof, 4 * n2 * sizeof(double)));
+  const double* src[4] = {d->R, d->V, d->CS, d->E};
+  for (int a = 0; a < 4; a++) {
+    const hipError_t e = hipMemcpyAsync(prof + a * n2, src[a], n2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+      (void)hipFree(prof);
+      F3D_CHECK(ctx, e);
+    }
+  }
+  IcPar p{d->mode, d->ndraw, d->adiabatic, d->noise, d->aspectratio, d->gamma};
+  IcJump jp;
+  jp.x0 = d->x0 & ((1ULL << 48) - 1);
+  f3d_drand48_jumps(d->ndraw ? d->ndraw : 1, jp.A, jp.C);
+  const dim3 g((m.nx + BX - 1) / BX, m.sy, m.sz);
+  if (fp32)
+    hipLaunchKernelGGL(k_condinit<float>, g, dim3(BX), 0, ctx->stream, m, p, jp, prof, prof + n2, prof + 2 * n2,
+                       prof + 3 * n2, (float*)dens, (float*)energy, (float*)vx, (float*)vy, (float*)vz, ctx->vbase);
+  else
+    hipLaunchKernelGGL(k_condinit<double>, g, dim3(BX), 0, ctx->stream, m, p, jp, prof, prof + n2, prof + 2 * n2,
+                       prof + 3 * n2, (double*)dens, (double*)energy, (double*)vx, (double*)vy, (double*)vz,
+                       (const double*)nullptr);
+  const hipError_t le = hipGetLastError();
+  const hipError_t se = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(prof);
+  F3D_CHECK(ctx, le);
+  F3D_CHECK(ctx, se);
+  return 0;
+}
+
+int f3d_CondInit(f3d_hip_ctx* ctx, const f3d_ic_desc* d, double* dens, double* energy, double* vx, double* vy,
+                 double* vz) {
+  return condinit(ctx, d, dens, energy, vx, vy, vz, 0);
+}
+
+int f3d32_CondInit(f3d_hip_ctx* ctx, const f3d_ic_desc* d, double* dens, double* energy, double* vx, double* vy,
+                   double* vz) {
+  return condinit(ctx, d, dens, energy, vx, vy, vz, 1);
+}
+
 int f3d_init_stockholm(f3d_hip_ctx* ctx) {
   NEED(ctx->f.Density0, "Density0"); NEED(ctx->f.Vx0, "Vx0"); NEED(ctx->f.Vy0, "Vy0");
   int n = ctx->m.sy * (ctx->m.dimz ? ctx->m.sz : 1);
@@ -1195,10 +1317,10 @@ static int mon_launch(f3d_hip_ctx* ctx, int q, int fp32, double xp, double yp, d
     if (fp32)                                                                                                 \
       hipLaunchKernelGGL((k_mon<G_, DZ_, Q_, float>), g, dim3(BX), 0, ctx->stream, m, r,                      \
                          reinterpret_cast<const float*>(ctx->f.Density), reinterpret_cast<const float*>(v),   \
-                         out, of, xp, yp, zp, rsm2);                                                          \
+                         out, of, xp, yp, zp, rsm2, ctx->vbase);                                              \
     else                                                                                                      \
       hipLaunchKernelGGL((k_mon<G_, DZ_, Q_, double>), g, dim3(BX), 0, ctx->stream, m, r, ctx->f.Density, v, \
-                         out, of, xp, yp, zp, rsm2);                                                          \
+                         out, of, xp, yp, zp, rsm2, nullptr);                                                 \
   } while (0)
 #define MON_G(G_, DZ_)                                  \
   do {                                                  \

@@ -80,6 +80,9 @@ int f3d_hip_create(const f3d_mesh_desc* d, f3d_hip_ctx** out) {
   m.xmin64 = m.xmin; m.ymin64 = m.ymin; m.zmin64 = m.zmin;
   m.ymed64 = m.ymed; m.sinzmed64 = m.sinzmed; m.sxi64 = m.sxi;
   m.zmed64 = m.zmed; m.sinzmin64 = m.sinzmin; m.coszmin64 = m.coszmin;
+  m.sxj64 = m.sxj; m.sxk64 = m.sxk; m.syk64 = m.syk; m.invvj64 = m.invvj;
+  m.vb = nullptr;  // the FP64 form stores full azimuthal velocities
+  m.salt = 0;
   c->meshlen = total;
   *out = c;
   return 0;
@@ -136,6 +139,7 @@ void f3d_hip_destroy(f3d_hip_ctx* c) {
     if (c->fused[b] && c->fused_free[b]) c->fused_free[b](c->fused[b]);
   if (c->meshbuf) hipFree(c->meshbuf);
   if (c->bctab) hipFree(c->bctab);
+  if (c->vbase) hipFree(c->vbase);
   if (c->dred) hipFree(c->dred);
   if (c->hres) hipHostFree(c->hres);
   if (c->side) hipStreamSynchronize(c->side);
@@ -150,6 +154,20 @@ void f3d_hip_destroy(f3d_hip_ctx* c) {
 int f3d_hip_bind(f3d_hip_ctx* c, const f3d_fields* f) {
   if (!c || !f) return -2;
   c->f = *f;
+  return 0;
+}
+
+int f3d_hip_set_vbase(f3d_hip_ctx* ctx, const double* host_v0) {
+  if (!ctx || !host_v0) return -1;
+  const size_t bytes = (size_t)ctx->m.sy * (size_t)ctx->m.sz * sizeof(double);
+  if (!ctx->vbase && hipMalloc(&ctx->vbase, bytes) != hipSuccess) {
+    snprintf(ctx->err, sizeof(ctx->err), "f3d_hip_set_vbase: device allocation failed");
+    return -1;
+  }
+  if (hipMemcpy(ctx->vbase, host_v0, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    snprintf(ctx->err, sizeof(ctx->err), "f3d_hip_set_vbase: copy failed");
+    return -1;
+  }
   return 0;
 }
 

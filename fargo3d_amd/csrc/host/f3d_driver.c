@@ -48,6 +48,7 @@ struct f3d_sim {
   long pot_step;                                 /* Timestepcount of the last Potential launch */
   double *monbuf;                                /* FP64 cell values of a monitored quantity */
   int mon_counter;                               /* MonCounter (src/monitor.c:14) */
+  double *vb; /* precision form: the ring profile V0[j + k*sy] its azimuthal velocities deviate from */
   char err[512];
 };
 
@@ -116,12 +117,33 @@ static double *dalloc_bytes(size_t bytes, int *fail) {
 }
 #define dalloc(n, fail) dalloc_bytes((n) * S->es, fail)
 
+/* The precision form stores every fluid's azimuthal velocity (Vx, Vx_temp)
+   and its damping reference (Vx0) as the FP32 deviation from the FP64 ring
+   profile S->vb: 1 for such a 3-D array, 2 for the 2-D reference, 0 for
+   every other array. */
+static int vx_kind(f3d_sim *S, const void *dev) {
+  if (!S->prec32 || !S->vb) return 0;
+  for (int k = 0; k < S->nfl; k++) {
+    if (dev == S->fl[k].Vx || dev == S->fl[k].Vx_temp) return 1;
+    if (dev == S->fl[k].Vx0) return 2;
+  }
+  return 0;
+}
+
+/* the profile value of element idx of an array of that kind */
+static double vb_at(const f3d_sim *S, size_t idx, int kind) {
+  if (kind == 2) return S->vb[idx];
+  size_t j = (idx / (size_t)S->g.pitch) % (size_t)S->g.sy, k = idx / (size_t)S->g.stride;
+  return S->vb[j + k * (size_t)S->g.sy];
+}
+
 /* host buffer of n doubles -> device array of the simulation's element type */
 static int upload(f3d_sim *S, void *dev, const double *h, size_t n) {
   if (!S->prec32) return f3d_hip_memcpy_h2d(S->ctx, dev, h, n * sizeof(double));
   float *t = (float *)malloc(n * sizeof(float));
   if (!t) return -1;
-  for (size_t i = 0; i < n; i++) t[i] = (float)h[i];
+  const int kind = vx_kind(S, dev);
+  for (size_t i = 0; i < n; i++) t[i] = (float)(kind ? h[i] - vb_at(S, i, kind) : h[i]);
   int rc = f3d_hip_memcpy_h2d(S->ctx, dev, t, n * sizeof(float));
   if (rc == 0) rc = f3d_hip_sync(S->ctx);
   free(t);
@@ -134,7 +156,8 @@ static int download(f3d_sim *S, double *h, const void *dev, size_t n) {
   if (!t) return -1;
   int rc = f3d_hip_memcpy_d2h(S->ctx, t, dev, n * sizeof(float));
   if (rc == 0) rc = f3d_hip_sync(S->ctx);
-  for (size_t i = 0; rc == 0 && i < n; i++) h[i] = t[i];
+  const int kind = vx_kind(S, dev);
+  for (size_t i = 0; rc == 0 && i < n; i++) h[i] = kind ? (double)t[i] + vb_at(S, i, kind) : (double)t[i];
   free(t);
   return rc;
 }
@@ -332,12 +355,39 @@ f3d_sim *f3d_sim_create(const char *parfile, const char *overrides, char *err, i
     select_fluid(S, 0);
   }
   {
-    /* CondInit of every fluid on the host, then upload (src/main.c:265) */
-    size_t n = S->g.ncell;
-    double *h = (double *)calloc(5 * n, sizeof(double));
-    int bad = 0;
+    /* CondInit of every fluid (src/main.c:265): on the host and uploaded, or
+       (CondInitOnDevice) on the GPU from the same per-ring profiles and the
+       same drand48 stream */
+    size_t n = S->g.ncell, n2 = S->g.n2d;
+    double *prof = (double *)malloc(4 * n2 * sizeof(double));
+    double *h = S->p.ic_device ? NULL : (double *)calloc(5 * n, sizeof(double));
+    int bad = !prof || (!S->p.ic_device && !h);
     for (int k = 0; k < S->nfl && !bad; k++) {
       const f3d_fields *F = &S->fl[k];
+      f3d_ic_desc d;
+      if (f3d_ic_profiles(&S->g, &S->p, &S->clk, S->ftype[k], &d, prof, prof + n2, prof + 2 * n2, prof + 3 * n2)) {
+        bad = 1;
+        break;
+      }
+      d.R = prof;
+      d.V = prof + n2;
+      d.CS = prof + 2 * n2;
+      d.E = prof + 3 * n2;
+      if (k == 0 && S->prec32) {
+        /* the profile the precision form's azimuthal velocities deviate
+           from: the gas's initial V(j,k), FP64 */
+        S->vb = (double *)malloc(n2 * sizeof(double));
+        if (!S->vb || f3d_hip_set_vbase(S->ctx, d.V) != 0) {
+          bad = 1;
+          break;
+        }
+        memcpy(S->vb, d.V, n2 * sizeof(double));
+      }
+      if (S->p.ic_device) {
+        bad = (S->prec32 ? f3d32_CondInit : f3d_CondInit)(S->ctx, &d, F->Density, F->Energy, F->Vx, F->Vy,
+                                                           S->g.dimz ? F->Vz : NULL) != 0;
+        continue;
+      }
       f3d_initial_conditions_fluid(&S->g, &S->p, &S->clk, S->ftype[k], h, h + n, h + 2 * n, h + 3 * n,
                                    S->g.dimz ? h + 4 * n : NULL);
       bad = upload(S, F->Density, h, n) | upload(S, F->Energy, h + n, n) | upload(S, F->Vx, h + 2 * n, n) |
@@ -345,8 +395,10 @@ f3d_sim *f3d_sim_create(const char *parfile, const char *overrides, char *err, i
       if (S->g.dimz) bad |= upload(S, F->Vz, h + 4 * n, n);
     }
     free(h);
+    free(prof);
     if (bad) {
-      if (err) snprintf(err, errlen, "upload of the initial conditions failed");
+      if (err) snprintf(err, errlen, "the initial conditions failed%s%s", S->p.ic_device ? ": " : "",
+                        S->p.ic_device ? f3d_hip_last_error(S->ctx) : "");
       goto fail;
     }
   }
@@ -394,6 +446,7 @@ void f3d_sim_destroy(f3d_sim *S) {
     if (all[a]) f3d_hip_free(all[a]);
   if (S->ctx) f3d_hip_destroy(S->ctx);
   if (S->monbuf) f3d_hip_free(S->monbuf);
+  free(S->vb);
   f3d_bctab_free(&S->bct);
   f3d_grid_free(&S->g);
   free(S);

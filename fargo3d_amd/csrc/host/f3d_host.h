@@ -16,6 +16,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "fargo3d_hip.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -95,6 +97,7 @@ typedef struct f3d_params {
   int ic_keplerian;            /* 1: v_phi = r^-1/2 - Omega_f r, v_r = 0, noise on density only */
   int mon_2d, mon_y_raw, mon_scalar; /* MONITOR_2D, MONITOR_Y_RAW, MONITOR_SCALAR of the .opt file */
   uint64_t seed;               /* srand48 seed for the noise (0 = glibc default state) */
+  int ic_device;               /* CondInitOnDevice: the initial state computed on the GPU (f3d_CondInit) */
   /* derived (src/param.c:236-240) */
   int corotating, guidingcenter;
 } f3d_params;
@@ -185,6 +188,20 @@ int f3d_initial_conditions(const f3d_grid *g, const f3d_params *p, const f3d_clo
 /* the same for fluid type F3D_GAS or F3D_DUST of a multifluid setup */
 int f3d_initial_conditions_fluid(const f3d_grid *g, const f3d_params *p, const f3d_clock *c, int fluidtype,
                                  double *dens, double *energy, double *vx, double *vy, double *vz);
+/* The two parts of those: the per-ring profiles R, V, CS, E (sy*sz each,
+ * caller's arrays) and the scalars of the set-up's per-cell noise into d
+ * (its R/V/CS/E pointers are left for the caller to set), and the
+ * arithmetic of one cell from its draws u[0..ndraw) -- the device CondInit
+ * (f3d_CondInit) repeats it operation for operation. */
+int f3d_ic_profiles(const f3d_grid *g, const f3d_params *p, const f3d_clock *c, int fluidtype, f3d_ic_desc *d,
+                    double *R, double *V, double *CS, double *E);
+void f3d_ic_cell(const f3d_ic_desc *d, size_t l2, const double *u, double *rho, double *e, double *vx,
+                 double *vy);
+/* drand48's state after srand48(seed) (glibc's initial state for seed 0),
+ * and the jump table of ndraw draws: entry b maps a state X to the state
+ * 2^b * ndraw draws later, X -> (A[b] X + C[b]) mod 2^48 */
+unsigned long long f3d_drand48_state(uint64_t seed);
+void f3d_drand48_jumps(int ndraw, unsigned long long A[48], unsigned long long C[48]);
 
 /* ---- boundary tables ----
  * The generated boundary kernels (std/boundaries.txt, scripts/boundparser.py)

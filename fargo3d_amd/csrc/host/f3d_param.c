@@ -250,6 +250,7 @@ int f3d_params_set(f3d_params *p, const char *name_in, const char *value) {
   }
   if (strcmp(name, "ENERGYBOUNDARY") == 0) { p->energy_bc = (toupper((unsigned char)value[0]) == 'S'); return 0; }
   if (strcmp(name, "SEED") == 0) { p->seed = strtoull(value, NULL, 10); return 0; }
+  if (strcmp(name, "CONDINITONDEVICE") == 0) { p->ic_device = (toupper((unsigned char)value[0]) == 'Y'); return 0; }
   for (int i = 0; IGNORED[i]; i++)
     if (strcmp(name, IGNORED[i]) == 0) return 0;
   return -1;

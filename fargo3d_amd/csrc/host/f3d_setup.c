@@ -21,11 +21,200 @@ void f3d_seed(uint64_t seed) {
   if (seed) srand48((long)seed);
 }
 
-/* Standard normal deviate from two drand48 draws (Box-Muller). */
-static double gauss48(void) {
-  double u1 = drand48(), u2 = drand48();
-  if (u1 < 1e-300) u1 = 1e-300;
-  return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+/* ---- glibc's drand48 generator: X' = (a X + c) mod 2^48, a draw is X'/2^48 ---- */
+#define LCG_A 0x5DEECE66DULL
+#define LCG_C 0xBULL
+#define LCG_MASK ((1ULL << 48) - 1)
+
+/* the n-fold composition of X -> (a X + c) mod 2^48, as X -> (A X + C):
+   binary powering (every power of one affine map commutes with the others) */
+static void lcg_pow(unsigned long long a, unsigned long long c, unsigned long long n, unsigned long long *A,
+                    unsigned long long *C) {
+  unsigned long long ra = 1, rc = 0;
+  while (n) {
+    if (n & 1) {
+      rc = (a * rc + c) & LCG_MASK;
+      ra = (a * ra) & LCG_MASK;
+    }
+    c = (a * c + c) & LCG_MASK;
+    a = (a * a) & LCG_MASK;
+    n >>= 1;
+  }
+  *A = ra;
+  *C = rc;
+}
+
+unsigned long long f3d_drand48_skip(unsigned long long x0, unsigned long long n) {
+  unsigned long long A, C;
+  lcg_pow(LCG_A, LCG_C, n, &A, &C);
+  return (A * (x0 & LCG_MASK) + C) & LCG_MASK;
+}
+
+/* the state srand48(seed) sets (glibc keeps the low 32 bits of the seed
+   above 0x330E), or glibc's initial state without srand48 */
+unsigned long long f3d_drand48_state(uint64_t seed) {
+  if (!seed) return 0x1234ABCD330EULL;
+  return ((((unsigned long long)seed) & 0xffffffffULL) << 16) | 0x330EULL;
+}
+
+void f3d_drand48_jumps(int ndraw, unsigned long long A[48], unsigned long long C[48]) {
+  unsigned long long a, c;
+  lcg_pow(LCG_A, LCG_C, (unsigned long long)ndraw, &a, &c);
+  for (int b = 0; b < 48; b++) {
+    A[b] = a;
+    C[b] = c;
+    c = (a * c + c) & LCG_MASK;
+    a = (a * a) & LCG_MASK;
+  }
+}
+
+/* ---- initial conditions: per-ring profiles and per-cell noise ----
+ * Every set-up below is a profile of the rings (j,k) -- the density factor R,
+ * the azimuthal velocity V in the frame, the sound speed CS and E, the
+ * energy (PLAIN) or cs^2 (adiabatic) -- times noise drawn per cell in the
+ * set-up's drand48 order.  f3d_ic_profiles evaluates the profiles with the
+ * set-up's expressions (glibc transcendentals); f3d_ic_cell is the per-cell
+ * arithmetic, which the device CondInit repeats operation for operation. */
+int f3d_ic_profiles(const f3d_grid *g, const f3d_params *p, const f3d_clock *c, int fluidtype, f3d_ic_desc *d,
+                    double *R, double *V, double *CS, double *E) {
+  const double OMEGAFRAME = c->OMEGAFRAME;
+  memset(d, 0, sizeof(*d));
+  d->noise = p->noise;
+  d->aspectratio = p->aspectratio;
+  d->gamma = p->gamma;
+  d->adiabatic = (p->eos == F3D_ADIABATIC);
+  d->x0 = f3d_drand48_state(p->seed);
+  size_t n2 = g->n2d;
+  memset(R, 0, n2 * sizeof(double));
+  memset(V, 0, n2 * sizeof(double));
+  memset(CS, 0, n2 * sizeof(double));
+  memset(E, 0, n2 * sizeof(double));
+  if (strcmp(p->setup, "fargo_multifluid") == 0) {
+    /* setups/fargo_multifluid/condinit.c:8-52: gas in pressure-supported
+       rotation, each dust fluid Keplerian and pressureless with EPSILON times
+       the gas density; no noise */
+    if (g->dimz) return -1;
+    d->mode = F3D_IC_PLAIN;
+    d->adiabatic = 0;
+    for (int j = 0; j < g->sy; j++) {
+      double r = g->ymed[j];
+      double omega = sqrt(G * MSTAR / r / r / r);
+      double rhog = p->sigma0 * pow(r / R0, -p->sigmaslope);
+      double v;
+      if (fluidtype == F3D_GAS) {
+        R[j] = rhog;
+        v = omega * r * sqrt(1.0 + pow(p->aspectratio, 2) * pow(r / R0, 2 * p->flaringindex) *
+                                       (2.0 * p->flaringindex - 1.0 - p->sigmaslope));
+        E[j] = p->aspectratio * pow(r / R0, p->flaringindex) * omega * r;
+      } else {
+        R[j] = rhog * p->epsilon;
+        v = omega * r;
+        E[j] = 0.0;
+      }
+      v -= OMEGAFRAME * r;
+      V[j] = v;
+    }
+    return 0;
+  }
+  if (!g->dimz) {
+    /* 2-D cylindrical disks: setups/fargo/condinit.c:25-50 */
+    if (p->ic_keplerian) {
+      /* v_phi = r^-1/2 - Omega_f r, v_r = 0, uniform noise on the density only */
+      d->mode = F3D_IC_KEPLER;
+      d->ndraw = 1;
+    } else if (p->eos == F3D_ADIABATIC) {
+      /* log-normal density perturbations, pressure-supported rotation */
+      d->mode = F3D_IC_LOGNORMAL;
+      d->ndraw = p->noise != 0.0 ? 2 : 0;
+    } else {
+      d->mode = F3D_IC_FARGO;
+      d->ndraw = 3;
+    }
+    for (int j = 0; j < g->sy; j++) {
+      double r = g->ymed[j];
+      double omega = sqrt(G * MSTAR / r / r / r);
+      double soundspeed = p->aspectratio * pow(r / R0, p->flaringindex) * omega * r;
+      double v;
+      R[j] = p->sigma0 * pow(r / R0, -p->sigmaslope);
+      if (p->ic_keplerian)
+        v = omega * r;
+      else
+        v = omega * r * sqrt(1.0 + pow(p->aspectratio, 2) * pow(r / R0, 2 * p->flaringindex) *
+                                       (2.0 * p->flaringindex - 1.0 - p->sigmaslope));
+      v -= OMEGAFRAME * r;
+      V[j] = v;
+      CS[j] = soundspeed;
+      E[j] = pow(soundspeed, 2);
+    }
+    return 0;
+  }
+  if (g->geometry != F3D_SPHERICAL) return -1;
+  /* 3-D spherical disk: setups/p3disof/condinit.c:14-52, no noise */
+  d->mode = F3D_IC_PLAIN;
+  d->adiabatic = 0;
+  for (int k = 0; k < g->sz; k++) {
+    for (int j = 0; j < g->sy; j++) {
+      size_t l2 = (size_t)j + (size_t)k * g->sy;
+      double r = g->ymed[j];
+      double r3 = r * r * r;
+      double omega = sqrt(G * MSTAR / (r3));
+      double v = omega * r;
+      double xi = p->sigmaslope + 1. + p->flaringindex;
+      double beta = 1. - 2 * p->flaringindex;
+      double h = p->aspectratio * pow(r / R0, p->flaringindex);
+      double sz = sin(g->zmed[k]);
+      double dens;
+      if (p->flaringindex == 0.0) {
+        dens = p->sigma0 / sqrt(2.0 * M_PI) / (R0 * p->aspectratio) * pow(r / R0, -xi) *
+               pow(sz, -beta - xi + 1. / (h * h));
+      } else {
+        dens = p->sigma0 / sqrt(2.0 * M_PI) / (R0 * p->aspectratio) * pow(r / R0, -xi) * pow(sz, -xi - beta) *
+               exp((1. - pow(sz, -2. * p->flaringindex)) / 2. / p->flaringindex / (h * h));
+      }
+      v *= sqrt(pow(sz, -2. * p->flaringindex) - (beta + xi) * h * h);
+      v -= OMEGAFRAME * r * sz;
+      R[l2] = dens;
+      V[l2] = v;
+      if (p->eos == F3D_ISOTHERMAL)
+        E[l2] = h * sqrt(G * MSTAR / r);
+      else
+        E[l2] = dens * h * h * G * MSTAR / r / (p->gamma - 1.0);
+    }
+  }
+  return 0;
+}
+
+void f3d_ic_cell(const f3d_ic_desc *d, size_t l2, const double *u, double *rho, double *e, double *vx,
+                 double *vy) {
+  const double Rv = d->R[l2], Vv = d->V[l2];
+  double r = Rv, x = Vv, y = 0.0;
+  switch (d->mode) {
+    case F3D_IC_PLAIN:
+      *rho = Rv;
+      *vx = Vv;
+      *vy = 0.0;
+      *e = d->E[l2];
+      return;
+    case F3D_IC_KEPLER:
+      r = Rv * (1.0 + d->noise * (u[0] - .5));
+      break;
+    case F3D_IC_FARGO:
+      r = Rv * (1.0 + d->noise * (u[0] - .5));
+      x = Vv * (1. + d->aspectratio * d->noise * (u[1] - .5));
+      y = d->CS[l2] * d->noise * (u[2] - .5);
+      break;
+    default: /* F3D_IC_LOGNORMAL */
+      if (d->noise != 0.0) {
+        double u1 = u[0], u2 = u[1];
+        if (u1 < 1e-300) u1 = 1e-300;
+        r *= exp(d->noise * (sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2)));
+      }
+      break;
+  }
+  *rho = r;
+  *vx = x;
+  *vy = y;
+  *e = d->adiabatic ? d->E[l2] * r / (d->gamma - 1.0) : d->CS[l2];
 }
 
 int f3d_initial_conditions(const f3d_grid *g, const f3d_params *p, const f3d_clock *c,
@@ -33,123 +222,37 @@ int f3d_initial_conditions(const f3d_grid *g, const f3d_params *p, const f3d_clo
   return f3d_initial_conditions_fluid(g, p, c, F3D_GAS, dens, energy, vx, vy, vz);
 }
 
-/* setups/fargo_multifluid/condinit.c:8-52: gas in pressure-supported
-   rotation, each dust fluid Keplerian and pressureless with EPSILON times the
-   gas density; no noise */
-static void multifluid_ic(const f3d_grid *g, const f3d_params *p, double OMEGAFRAME, int fluidtype, double *dens,
-                          double *energy, double *vx, double *vy) {
-  for (int j = 0; j < g->sy; j++) {
-    for (int i = 0; i < g->sx; i++) {
-      size_t l = (size_t)i + (size_t)j * g->pitch;
-      double r = g->ymed[j];
-      double omega = sqrt(G * MSTAR / r / r / r);
-      double rhog = p->sigma0 * pow(r / R0, -p->sigmaslope);
-      double rhod = rhog * p->epsilon;
-      if (fluidtype == F3D_GAS) {
-        dens[l] = rhog;
-        vx[l] = omega * r * sqrt(1.0 + pow(p->aspectratio, 2) * pow(r / R0, 2 * p->flaringindex) *
-                                           (2.0 * p->flaringindex - 1.0 - p->sigmaslope));
-        vy[l] = 0.0;
-        energy[l] = p->aspectratio * pow(r / R0, p->flaringindex) * omega * r;
-      } else {
-        dens[l] = rhod;
-        vx[l] = omega * r;
-        vy[l] = 0.0;
-        energy[l] = 0.0;
-      }
-      vx[l] -= OMEGAFRAME * r;
-    }
-  }
-}
-
 int f3d_initial_conditions_fluid(const f3d_grid *g, const f3d_params *p, const f3d_clock *c, int fluidtype,
                                  double *dens, double *energy, double *vx, double *vy, double *vz) {
-  const double OMEGAFRAME = c->OMEGAFRAME;
+  size_t n = g->ncell, n2 = g->n2d;
+  double *buf = (double *)malloc(4 * n2 * sizeof(double));
+  if (!buf) return -1;
+  f3d_ic_desc d;
+  if (f3d_ic_profiles(g, p, c, fluidtype, &d, buf, buf + n2, buf + 2 * n2, buf + 3 * n2) != 0) {
+    free(buf);
+    return -1;
+  }
+  d.R = buf;
+  d.V = buf + n2;
+  d.CS = buf + 2 * n2;
+  d.E = buf + 3 * n2;
   f3d_seed(p->seed);
-  size_t n = g->ncell;
   memset(dens, 0, n * sizeof(double));
   memset(energy, 0, n * sizeof(double));
   memset(vx, 0, n * sizeof(double));
   memset(vy, 0, n * sizeof(double));
   if (vz) memset(vz, 0, n * sizeof(double));
-  if (strcmp(p->setup, "fargo_multifluid") == 0) {
-    if (g->dimz) return -1;
-    multifluid_ic(g, p, OMEGAFRAME, fluidtype, dens, energy, vx, vy);
-    return 0;
-  }
-  if (!g->dimz) {
-    /* 2-D cylindrical disks: setups/fargo/condinit.c:25-50 */
-    for (int j = 0; j < g->sy; j++) {
-      for (int i = 0; i < g->sx; i++) {
-        size_t l = (size_t)i + (size_t)j * g->pitch;
-        double r = g->ymed[j];
-        double omega = sqrt(G * MSTAR / r / r / r);
-        double soundspeed;
-        if (p->ic_keplerian) {
-          /* v_phi = r^-1/2 - Omega_f r, v_r = 0, uniform noise on density only */
-          dens[l] = p->sigma0 * pow(r / R0, -p->sigmaslope) * (1.0 + p->noise * (drand48() - .5));
-          soundspeed = p->aspectratio * pow(r / R0, p->flaringindex) * omega * r;
-          vx[l] = omega * r;
-          vx[l] -= OMEGAFRAME * r;
-          vy[l] = 0.0;
-        } else if (p->eos == F3D_ADIABATIC) {
-          /* log-normal density perturbations, pressure-supported rotation */
-          dens[l] = p->sigma0 * pow(r / R0, -p->sigmaslope);
-          if (p->noise != 0.0) dens[l] *= exp(p->noise * gauss48());
-          soundspeed = p->aspectratio * pow(r / R0, p->flaringindex) * omega * r;
-          vx[l] = omega * r * sqrt(1.0 + pow(p->aspectratio, 2) * pow(r / R0, 2 * p->flaringindex) *
-                                             (2.0 * p->flaringindex - 1.0 - p->sigmaslope));
-          vx[l] -= OMEGAFRAME * r;
-          vy[l] = 0.0;
-        } else {
-          dens[l] = p->sigma0 * pow(r / R0, -p->sigmaslope) * (1.0 + p->noise * (drand48() - .5));
-          soundspeed = p->aspectratio * pow(r / R0, p->flaringindex) * omega * r;
-          vx[l] = omega * r * sqrt(1.0 + pow(p->aspectratio, 2) * pow(r / R0, 2 * p->flaringindex) *
-                                             (2.0 * p->flaringindex - 1.0 - p->sigmaslope));
-          vx[l] -= OMEGAFRAME * r;
-          vx[l] *= (1. + p->aspectratio * p->noise * (drand48() - .5));
-          vy[l] = soundspeed * p->noise * (drand48() - .5);
-        }
-        if (p->eos == F3D_ISOTHERMAL)
-          energy[l] = soundspeed;
-        else
-          energy[l] = pow(soundspeed, 2) * dens[l] / (p->gamma - 1.0);
-      }
-    }
-    return 0;
-  }
-  if (g->geometry != F3D_SPHERICAL) return -1;
-  /* 3-D spherical disk: setups/p3disof/condinit.c:14-52 */
-  for (int k = 0; k < g->sz; k++) {
-    for (int j = 0; j < g->sy; j++) {
-      double r = g->ymed[j];
-      double r3 = r * r * r;
-      double omega = sqrt(G * MSTAR / (r3));
+  /* cells in the set-ups' loop order: x fastest, then y, then z; the draws
+     of one cell in the order its condinit.c makes them */
+  for (int k = 0; k < g->sz; k++)
+    for (int j = 0; j < g->sy; j++)
       for (int i = 0; i < g->nx; i++) {
         size_t l = (size_t)i + (size_t)j * g->pitch + (size_t)k * g->stride;
-        vy[l] = vz[l] = 0.0;
-        vx[l] = omega * r;
-        double xi = p->sigmaslope + 1. + p->flaringindex;
-        double beta = 1. - 2 * p->flaringindex;
-        double h = p->aspectratio * pow(r / R0, p->flaringindex);
-        double sz = sin(g->zmed[k]);
-        if (p->flaringindex == 0.0) {
-          dens[l] = p->sigma0 / sqrt(2.0 * M_PI) / (R0 * p->aspectratio) * pow(r / R0, -xi) *
-                    pow(sz, -beta - xi + 1. / (h * h));
-        } else {
-          dens[l] = p->sigma0 / sqrt(2.0 * M_PI) / (R0 * p->aspectratio) * pow(r / R0, -xi) *
-                    pow(sz, -xi - beta) *
-                    exp((1. - pow(sz, -2. * p->flaringindex)) / 2. / p->flaringindex / (h * h));
-        }
-        vx[l] *= sqrt(pow(sz, -2. * p->flaringindex) - (beta + xi) * h * h);
-        vx[l] -= OMEGAFRAME * r * sz;
-        if (p->eos == F3D_ISOTHERMAL)
-          energy[l] = h * sqrt(G * MSTAR / r);
-        else
-          energy[l] = dens[l] * h * h * G * MSTAR / r / (p->gamma - 1.0);
+        double u[3] = {0.0, 0.0, 0.0};
+        for (int q = 0; q < d.ndraw; q++) u[q] = drand48();
+        f3d_ic_cell(&d, (size_t)j + (size_t)k * g->sy, u, dens + l, energy + l, vx + l, vy + l);
       }
-    }
-  }
+  free(buf);
   return 0;
 }
 

@@ -86,6 +86,47 @@ int f3d_hip_create(const f3d_mesh_desc *mesh, f3d_hip_ctx **out);
 void f3d_hip_destroy(f3d_hip_ctx *ctx);
 int f3d_hip_bind(f3d_hip_ctx *ctx, const f3d_fields *fields);
 int f3d_hip_set_phys(f3d_hip_ctx *ctx, const f3d_phys *phys);
+/* Initial conditions on the device (the setups' CondInit: setups/fargo/
+ * condinit.c:25-50, setups/p3disof/condinit.c:14-52, setups/fargo_multifluid/
+ * condinit.c:8-52).  A set-up's initial state is a per-ring profile -- every
+ * transcendental of the coordinates, which the host evaluates with glibc --
+ * times the seeded noise of each cell.  The noise is glibc's drand48 stream
+ * in the set-up's call order (cells x fastest, then y, then z; ndraw draws per
+ * cell), which the device reproduces by jump-ahead of the 48-bit generator
+ * X' = (a X + c) mod 2^48 (a = 0x5DEECE66D, c = 0xB; a draw is X' / 2^48). */
+enum { F3D_IC_PLAIN = 0, F3D_IC_KEPLER = 1, F3D_IC_FARGO = 2, F3D_IC_LOGNORMAL = 3 };
+typedef struct f3d_ic_desc {
+  int mode;      /* F3D_IC_*: PLAIN rho = R, vx = V, vy = 0, e = E (no noise);
+                    KEPLER rho = R (1 + noise (U0 - .5)), vx = V, vy = 0;
+                    FARGO  rho = R (1 + noise (U0 - .5)), vx = V (1 + h noise (U1 - .5)),
+                           vy = CS noise (U2 - .5);
+                    LOGNORMAL rho = R exp(noise sqrt(-2 log U0) cos(2 pi U1)) (noise != 0), vx = V, vy = 0;
+                    energy (not PLAIN): E rho / (gamma - 1) if adiabatic (E = cs^2), else CS */
+  int ndraw;     /* drand48 draws per cell */
+  int adiabatic;
+  double noise, aspectratio, gamma;
+  unsigned long long x0;                 /* generator state before the first draw (48 bits) */
+  const double *R, *V, *CS, *E;          /* host arrays of sy*sz per-ring values (index j + k*sy) */
+} f3d_ic_desc;
+/* The state of every cell (ghosts included) of one fluid, FP64 fields.  No
+ * counterpart entry point in the reference (CondInit runs on its host). */
+int f3d_CondInit(f3d_hip_ctx *ctx, const f3d_ic_desc *d, double *dens, double *energy, double *vx, double *vy,
+                 double *vz);
+/* The same into the precision form's FP32 fields (behind the same pointer
+ * types), the azimuthal velocity as its deviation from the profile of
+ * f3d_hip_set_vbase. */
+int f3d32_CondInit(f3d_hip_ctx *ctx, const f3d_ic_desc *d, double *dens, double *energy, double *vx, double *vy,
+                   double *vz);
+/* drand48's state after n draws from state x0 (the generator's jump-ahead;
+ * host only, no device use) */
+unsigned long long f3d_drand48_skip(unsigned long long x0, unsigned long long n);
+
+/* Precision form (no counterpart in the reference, whose fields are all of
+ * one type): the FP64 ring profile V0[j + k*sy] (sy*sz values) that the FP32
+ * fields Vx, Vx_temp and the damping reference Vx0 are stored as deviations
+ * from, vx = V0(j,k) + (double)stored.  Set once, before the first f3d32_*
+ * stage on the handle and before any FP32 azimuthal velocity is written. */
+int f3d_hip_set_vbase(f3d_hip_ctx *ctx, const double *host_v0);
 int f3d_hip_sync(f3d_hip_ctx *ctx);
 const char *f3d_hip_last_error(f3d_hip_ctx *ctx);
 /* device memory helpers for callers without their own allocator */

@@ -127,7 +127,9 @@ def test_device_planets_follow_the_host_integrator(frame):
     time steps: the single disk integrates the planet on the host (glibc,
     the reference's RK5 and frame rotation), the batch on the device."""
     import fargo3d_amd as fa
-    ov = "Nx=384 Ny=192 PlanetMass=1e-3 Nu=1e-5 AspectRatio=0.05"
+    # no output-time clamp in these 40 steps (DT beyond the run), so both take
+    # exactly the batch's time steps
+    ov = "Nx=384 Ny=192 PlanetMass=1e-3 Nu=1e-5 AspectRatio=0.05 DT=100.0"
     if frame == "F":
         ov += " Frame=F OmegaFrame=0.0"
     b = fa.Batch([(C1, ov)])

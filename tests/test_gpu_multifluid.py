@@ -141,8 +141,8 @@ def test_multifluid_fp32(fa, orc):
     s64 = fa.Simulation(C4, C4_SMALL)
     s32 = fa.Simulation(C4, C4_SMALL + " Precision=32")
     for n in range(10):
-        dt = s64.step()
-        s32.force_dt(dt)
+        s64.step()
+        s32.force_dt(s64.step_time())
         s32.step()
     for k in range(5):
         s64.select_fluid(k)

@@ -140,17 +140,17 @@ MUTANTS = [
     (MESH, "for (int j = 0; j < sy + 1; j++) g->ymin[j] = exp(log(p->ymin) + dy * (j - g->nghy));",
      "for (int j = 0; j < sy + 1; j++) g->ymin[j] = exp(log(p->ymin) + dy * (j - g->nghy + 1));", 0, "mesh log offset"),
     (MESH, "g->syj[j] = g->ymin[j];", "g->syj[j] = g->ymed[j];", 0, "mesh cylindrical Syj at the centre"),
-    (SETUP, "vx[l] *= (1. + p->aspectratio * p->noise * (drand48() - .5));", "vx[l] *= (1. + p->aspectratio * p->noise * (drand48() - .6));",
+    (SETUP, "x = Vv * (1. + d->aspectratio * d->noise * (u[1] - .5));", "x = Vv * (1. + d->aspectratio * d->noise * (u[1] - .6));",
      0, "IC fargo azimuthal noise offset"),
-    (SETUP, "dens[l] = p->sigma0 * pow(r / R0, -p->sigmaslope) * (1.0 + p->noise * (drand48() - .5));\n          soundspeed",
-     "dens[l] = p->sigma0 * pow(r / R0, -p->sigmaslope) * (1.0 + p->noise * (drand48() - .4));\n          soundspeed", 0,
+    (SETUP, "r = Rv * (1.0 + d->noise * (u[0] - .5));\n      break;\n    case F3D_IC_FARGO:",
+     "r = Rv * (1.0 + d->noise * (u[0] - .4));\n      break;\n    case F3D_IC_FARGO:", 0,
      "IC Keplerian noise offset"),
-    (SETUP, "(2.0 * p->flaringindex - 1.0 - p->sigmaslope));\n          vx[l] -= OMEGAFRAME * r;\n          vx[l] *=",
-     "(2.0 * p->flaringindex - 1.0 + p->sigmaslope));\n          vx[l] -= OMEGAFRAME * r;\n          vx[l] *=", 0,
+    (SETUP, "(2.0 * p->flaringindex - 1.0 - p->sigmaslope));\n      v -= OMEGAFRAME * r;",
+     "(2.0 * p->flaringindex - 1.0 + p->sigmaslope));\n      v -= OMEGAFRAME * r;", 0,
      "IC fargo pressure support sign"),
     (SETUP, "pow(sz, -beta - xi + 1. / (h * h));", "pow(sz, -beta + xi + 1. / (h * h));", 0, "IC 3-D colatitude exponent"),
-    (SETUP, "if (p->noise != 0.0) dens[l] *= exp(p->noise * gauss48());", "if (p->noise != 0.0) dens[l] *= 1.0 + p->noise * gauss48();",
-     0, "IC adiabatic noise not log-normal"),
+    (SETUP, "r *= exp(d->noise * (sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2)));",
+     "r *= 1.0 + d->noise * (sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2));", 0, "IC adiabatic noise not log-normal"),
     (SETUP, "t->ymin_vsq[m] = sqrt(Ymed[jact] / Ymed[jgh]);", "t->ymin_vsq[m] = sqrt(Ymed[jgh] / Ymed[jact]);", 0,
      "boundary 2-D inner Keplerian factor inverted"),
     (SETUP, "double pw = pow(Ymed[jact] / Ymed[jgh], SS + FI + 1.);", "double pw = pow(Ymed[jact] / Ymed[jgh], SS + FI);", 0,
@@ -194,7 +194,7 @@ def build_copy(tmp, mutant):
     lib = os.path.join(tmp, "liboracle.so")
     srcs = [os.path.join(tmp, r) for r in SOURCES if r.endswith(".c")]
     subprocess.run(["gcc", "-O1", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-w", "-shared",
-                    "-o", lib] + srcs + ["-lm"], check=True)
+                    "-I" + os.path.join(ROOT, "include"), "-o", lib] + srcs + ["-lm"], check=True)
     return lib
 
 
