@@ -271,7 +271,6 @@ __global__ void __launch_bounds__(64) kb_dt(BInst* I, int n) {
   if (ii >= n) return;
   BInst& in = I[ii];
   double* c = in.clk;
-  in.m.salt = cell::sr_hash(in.m.salt + 0x9E3779B9U);  // the step's key of the stochastic rounding
   const double DT = c[3];
   if (c[1] >= DT) c[1] = 0.0;
   const double fluid_min = __longlong_as_double((long long)in.res[0]);

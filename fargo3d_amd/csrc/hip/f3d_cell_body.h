@@ -63,47 +63,6 @@ __device__ __forceinline__ double vbase(const dmesh& m, int j, int k) {
   if constexpr (kFix64) return m.vb[j + k * m.sy];
   return 0.0;
 }
-// State updates x + inc of the precision form (the sweeps' flux updates,
-// the compression work) round stochastically to FP32: up or down with the
-// probabilities that make the expected result the exact sum.  In a disk that
-// evolves slowly a cell's update is often below half an FP32 unit of its
-// value and nearly the same step after step; rounded to nearest it would be
-// lost the same way every step (the slow heating or mass trend of the FP64
-// run stalls in FP32), rounded stochastically it is kept on average.  The
-// random number is a hash of the cell, the quantity and the step's key, so a
-// run is reproducible.  In the FP64 build upd is the plain sum.
-__host__ __device__ __forceinline__ unsigned sr_hash(unsigned x) {
-  x ^= x >> 16;
-  x *= 0x7feb352dU;
-  x ^= x >> 15;
-  x *= 0x846ca68bU;
-  x ^= x >> 16;
-  return x;
-}
-__device__ __forceinline__ real sr_round(const dmesh& m, double s, int l, int q) {
-  const float f = (float)s;
-  const double df = (double)f, ds = s - df;
-  if (ds == 0.0) return f;
-  // the neighbouring float on the side of s (finite f): one unit of the
-  // bit pattern, away from zero where s is farther from zero than f
-  const bool up = ds > 0.0;
-  float g;
-  if (f == 0.0f) g = up ? __uint_as_float(1u) : -__uint_as_float(1u);
-  else g = __uint_as_float(__float_as_uint(f) + (((f > 0.0f) == up) ? 1u : 0xFFFFFFFFu));
-  const unsigned h = sr_hash((unsigned)l * 0x9E3779B1U ^ m.salt ^ ((unsigned)q * 0x85EBCA77U));
-  const double u = (double)(h >> 8) * 0x1p-24;
-  return (fabs(ds) > u * fabs((double)g - df)) ? g : f;
-}
-__device__ __forceinline__ real upd(const dmesh& m, real x, real inc, int l, int q) {
-  if constexpr (kFix64) return sr_round(m, (double)x + (double)inc, l, q);
-  return x + inc;
-}
-// the same with an FP64 increment (precision form only)
-__device__ __forceinline__ real upd64(const dmesh& m, real x, double inc, int l, int q) {
-  if constexpr (kFix64) return sr_round(m, (double)x + inc, l, q);
-  return (real)((double)x + inc);
-}
-
 // the cylindrical radius of ring (j,k) in FP64
 template <int G>
 __device__ __forceinline__ double rcyl64(const dmesh& m, int j, int k) {
@@ -266,7 +225,8 @@ __device__ __forceinline__ real substep3(const dmesh& m, int i, int j, int k, re
     // In FP32 the rounding of 1 - t is the same for every cell of a ring
     // with the same t and would heat or cool the ring by a fixed amount
     // each step; here it is relative to the change only.
-    return upd(m, e, -(e * (real(2.0) * term / (real(1.0) + term))), i + j * m.pitch + k * m.stride, 7);
+    e -= e * (real(2.0) * term / (real(1.0) + term));
+    return e;
   }
   e *= (real(1.0) - term) / (real(1.0) + term);
   return e;

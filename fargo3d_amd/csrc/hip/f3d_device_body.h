@@ -38,13 +38,9 @@ struct dmesh {
   // FP64 copies for the parts that stay FP64 in the reduced-precision form
   // (ring sums, residual/shift split, CFL); the same arrays in the FP64 build
   const double *xmin64, *ymin64, *zmin64, *ymed64, *sinzmed64, *sxi64, *zmed64, *sinzmin64, *coszmin64;
-  const double *sxj64, *sxk64, *syk64, *invvj64;
   // precision form only: V0(j,k), the FP64 ring profile the azimuthal
   // velocities are stored as deviations from (index j + k*sy); null in FP64
   const double* vb;
-  // precision form only: the step's key of the stochastic rounding of state
-  // updates (cell::upd), changed every step
-  unsigned salt;
 };
 
 __device__ __forceinline__ int lxp_of(int l, int i, int nx) { return (i < nx - 1) ? l + 1 : l - (nx - 1); }

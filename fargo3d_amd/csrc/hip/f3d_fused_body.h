@@ -119,7 +119,6 @@ int make_mesh(f3d_hip_ctx* ctx, Scratch* S) {
   m.sinzmed = conv(d.sinzmed); m.coszmed = conv(d.coszmed); m.sinzmin = conv(d.sinzmin); m.coszmin = conv(d.coszmin);
   m.xmin64 = d.xmin; m.ymin64 = d.ymin; m.zmin64 = d.zmin; m.ymed64 = d.ymed; m.sinzmed64 = d.sinzmed;
   m.sxi64 = d.sxi; m.zmed64 = d.zmed; m.sinzmin64 = d.sinzmin; m.coszmin64 = d.coszmin;
-  m.sxj64 = d.sxj; m.sxk64 = d.sxk; m.syk64 = d.syk; m.invvj64 = d.invvj;
   if (!ctx->vbase) return -1;  // set with f3d_hip_set_vbase before the first FP32 stage
   m.vb = ctx->vbase;
   if (ctx->bctab) {
@@ -725,10 +724,9 @@ __device__ __forceinline__ void column_sweep(Col& col, int r0, int r1, real dt) 
 #pragma unroll
     for (int c = 0; c < NQ; c++) {
       const real f0 = (c < NXQ) ? F0[c] - D0 : F0[c];
-      col.out(c, r, cell::upd(col.mesh(), col.Q(c, r), col.flux(r, v0 * f0 * F0[NQ], v1 * F1[c] * F1[NQ]), col.L(r),
-                              Col::kSweep + c));
+      col.out(c, r, col.Q(c, r) + col.flux(r, v0 * f0 * F0[NQ], v1 * F1[c] * F1[NQ]));
     }
-    col.out(NQ, r, cell::upd(col.mesh(), col.rho(r), col.flux(r, v0 * F0[NQ], v1 * F1[NQ]), col.L(r), Col::kSweep + 15));
+    col.out(NQ, r, col.rho(r) + col.flux(r, v0 * F0[NQ], v1 * F1[NQ]));
     if (r + 1 >= r1) break;
     // advance the window by one row: row r+3 enters
     const int rn = r + 3;
@@ -766,11 +764,8 @@ struct ColPart {
   // the deviation quantities of the column among those carried here
   static constexpr int NXQ = FIRST < Col::NXQ ? (Col::NXQ - FIRST < COUNT ? Col::NXQ - FIRST : COUNT) : 0;
   static_assert(FIRST == 0 || FIRST >= Col::NXQ, "the deviation quantities lead each part");
-  static constexpr int kSweep = Col::kSweep + FIRST;
   const Col& c;
   __device__ __forceinline__ double base(int r) const { return c.base(r); }
-  __device__ __forceinline__ const dmesh& mesh() const { return c.mesh(); }
-  __device__ __forceinline__ int L(int r) const { return c.L(r); }
   __device__ __forceinline__ real rho(int r) const { return c.rho(r); }
   __device__ __forceinline__ real Q(int q, int r) const { return c.Q(FIRST + q, r); }
   __device__ __forceinline__ real zs(int r) const { return c.zs(r); }
@@ -802,14 +797,12 @@ __device__ __forceinline__ void column_sweep_part(const Col& col, int grp, int r
 template <int G>
 struct ColY2 {
   static constexpr int NXQ = 2;  // Mpx, Mmx: deviations in the precision form
-  static constexpr int kSweep = 16;  // stream of the stochastic rounding
   const dmesh* m;
   const real *rho_in, *vxt, *vyt, *e_in;
   real* out_[6];
   int i, ixp;
   real of, dt;
   __device__ __forceinline__ double base(int r) const { return cell::lbase<G>(*m, r, 0, (double)of); }
-  __device__ __forceinline__ const dmesh& mesh() const { return *m; }
   __device__ __forceinline__ int L(int r) const { return i + r * m->pitch; }
   __device__ __forceinline__ real rho(int r) const { return rho_in[L(r)]; }
   __device__ __forceinline__ real Q(int c, int r) const {
@@ -836,14 +829,12 @@ struct ColY2 {
 template <int G>
 struct ColZ {
   static constexpr int NXQ = 2;  // Mpx, Mmx: deviations in the precision form
-  static constexpr int kSweep = 32;  // stream of the stochastic rounding
   const dmesh* m;
   const real *rho_in, *vxt, *vyt, *vzt, *e_in;
   real* out_[8];
   int i, j, ixp;
   real of, dt;
   __device__ __forceinline__ double base(int r) const { return cell::lbase<G>(*m, j, r, (double)of); }
-  __device__ __forceinline__ const dmesh& mesh() const { return *m; }
   __device__ __forceinline__ int L(int r) const { return i + j * m->pitch + r * m->stride; }
   __device__ __forceinline__ real rho(int r) const { return rho_in[L(r)]; }
   __device__ __forceinline__ real Q(int c, int r) const {
@@ -884,7 +875,6 @@ struct ColZ {
 template <int G>
 struct ColY3 {
   static constexpr int NXQ = 2;  // Mpx, Mmx: deviations in the precision form
-  static constexpr int kSweep = 16;  // stream of the stochastic rounding
   const dmesh* m;
   const real* rho_in;
   const real* q_in[7];
@@ -893,7 +883,6 @@ struct ColY3 {
   int i, k;
   real dt, of;
   __device__ __forceinline__ double base(int r) const { return cell::lbase<G>(*m, r, k, (double)of); }
-  __device__ __forceinline__ const dmesh& mesh() const { return *m; }
   __device__ __forceinline__ int L(int r) const { return i + r * m->pitch + k * m->stride; }
   __device__ __forceinline__ real rho(int r) const { return rho_in[L(r)]; }
   __device__ __forceinline__ real Q(int c, int r) const { return q_in[c][L(r)]; }
@@ -993,25 +982,6 @@ struct XQuant {
   static constexpr int NL = GRP == 0 ? NQ : GRP == 1 ? NA : NQ - NA;  // carried here
   static constexpr int first = GRP == 2 ? NA : 0;  // global index of local quantity 0
 };
-
-// The precision form's update of the PPA pass of the FARGO shift.  The
-// advection velocity of that pass, the fractional part of VxMed dt / dx, is
-// the same for every cell of a ring and nearly the same step after step, and
-// so are the mesh factors SurfX dt InvVol of the update: rounded to FP32 they
-// would move each ring by the same tiny fraction of a cell too far or too
-// short every step, a phase error that grows with the step count.  So the
-// velocity, recomputed in FP64 from VxMed, and the mesh factors multiply the
-// FP32 face terms in FP64.
-template <int G>
-__device__ __forceinline__ double ppa_uvel64(const dmesh& m, int i, int j, int k, double vmed, double dt64) {
-  const double zs = zone_size_x64<G>(m, i, j, k);
-  const double ntilde = vmed * dt64 / zs;
-  return (ntilde - floor(ntilde + 0.5)) * zs / dt64;
-}
-// SurfX dt InvVol of cell i in FP64 (the factors of flux_x)
-__device__ __forceinline__ double surfx_dt_invvol64(const dmesh& m, int i, int j, int k, double dt64) {
-  return (((m.sxj64[j] * m.sxk64[k]) * dt64) * m.invvj64[j]) / (m.syk64[k] * m.sxi64[i]);
-}
 
 // An empty asm that "changes" the LDS row stride after each barrier: the
 // compiler then computes each phase's LDS addresses in that phase instead of
@@ -1279,25 +1249,12 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
         const int ip = ixp_of(i, nx);
         const real v0 = V[i], v1 = V[ip];
         const real* ds = FC + NL * nxv;
-        if (cell::kFix64 && ppa) {
-          const double u0 = ppa_uvel64<G>(m, i, j, k, vmed, dt64), u1 = ppa_uvel64<G>(m, ip, j, k, vmed, dt64);
-          const double met = surfx_dt_invvol64(m, i, j, k, dt64);
 #pragma unroll
-          for (int c = 0; c < NL; c++) {
-            const real* f = FC + c * nxv;
-            st[s][c] = cell::upd64(m, st[s][c], ((double)(f[i] * ds[i]) * u0 - (double)(f[ip] * ds[ip]) * u1) * met,
-                                   base + i, 64 + XQ::first + c);
-          }
-          st[s][NL] = cell::upd64(m, st[s][NL], ((double)ds[i] * u0 - (double)ds[ip] * u1) * met, base + i, 79);
-        } else {
-#pragma unroll
-          for (int c = 0; c < NL; c++) {
-            const real* f = FC + c * nxv;
-            st[s][c] = cell::upd(m, st[s][c], cell::flux_x(m, i, j, k, dt, v0 * f[i] * ds[i], v1 * f[ip] * ds[ip]),
-                                 base + i, 48 + XQ::first + c);
-          }
-          st[s][NL] = cell::upd(m, st[s][NL], cell::flux_x(m, i, j, k, dt, v0 * ds[i], v1 * ds[ip]), base + i, 63);
+        for (int c = 0; c < NL; c++) {
+          const real* f = FC + c * nxv;
+          st[s][c] += cell::flux_x(m, i, j, k, dt, v0 * f[i] * ds[i], v1 * f[ip] * ds[ip]);
         }
+        st[s][NL] += cell::flux_x(m, i, j, k, dt, v0 * ds[i], v1 * ds[ip]);
       }
     }
   }
@@ -1543,26 +1500,12 @@ __device__ __forceinline__ void kd_sweep_x_tile(const dim3 bid, const dmesh& m, 
     if (p >= ulo && p < uhi) {
       const real v0 = vp, v1 = V[p + 1];
       const real* ds = FC + NQ * W;
-      if (cell::kFix64 && pass == 1) {
-        // the precision form's PPA update in FP64 factors (see ppa_uvel64)
-        const double u0 = ppa_uvel64<G>(m, g, j, k, vmed, dt64), u1 = ppa_uvel64<G>(m, gidx(p + 1), j, k, vmed, dt64);
-        const double met = surfx_dt_invvol64(m, g, j, k, dt64);
 #pragma unroll
-        for (int c = 0; c < NQ; c++) {
-          const real* f = FC + c * W;
-          st[c] = cell::upd64(m, st[c], ((double)(f[p] * ds[p]) * u0 - (double)(f[p + 1] * ds[p + 1]) * u1) * met,
-                              base + g, 64 + c);
-        }
-        st[NQ] = cell::upd64(m, st[NQ], ((double)ds[p] * u0 - (double)ds[p + 1] * u1) * met, base + g, 79);
-      } else {
-#pragma unroll
-        for (int c = 0; c < NQ; c++) {
-          const real* f = FC + c * W;
-          st[c] = cell::upd(m, st[c], cell::flux_x(m, g, j, k, dt, v0 * f[p] * ds[p], v1 * f[p + 1] * ds[p + 1]),
-                            base + g, 48 + c);
-        }
-        st[NQ] = cell::upd(m, st[NQ], cell::flux_x(m, g, j, k, dt, v0 * ds[p], v1 * ds[p + 1]), base + g, 63);
+      for (int c = 0; c < NQ; c++) {
+        const real* f = FC + c * W;
+        st[c] += cell::flux_x(m, g, j, k, dt, v0 * f[p] * ds[p], v1 * f[p + 1] * ds[p + 1]);
       }
+      st[NQ] += cell::flux_x(m, g, j, k, dt, v0 * ds[p], v1 * ds[p + 1]);
     }
     if (pass == 0 && fargo) __syncthreads();  // the PPA pass rewrites AQ and V
   }
@@ -2500,20 +2443,10 @@ static int x_sweep_ring(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const R
 // fluid's tiles), then each fluid's NewVelocity_y/z, damping and ghost fill.
 // Fluids do not interact in Transport, so this equals transporting them one
 // after the other (src/main.c:420).
-// the key of the stochastic rounding of a stage (precision form): a hash of
-// the step's start time and the stage, so that a run restarted from an
-// output continues with the same keys as the run that wrote it
-static void next_salt(const f3d_hip_ctx* ctx, Scratch* S, unsigned stage) {
-  unsigned long long b;
-  memcpy(&b, &ctx->ph.PhysicalTime, sizeof(b));
-  S->m.salt = cell::sr_hash((unsigned)b ^ cell::sr_hash((unsigned)(b >> 32) + 0x9E3779B9U * stage));
-}
-
 static int transport_fluids(f3d_hip_ctx* ctx, double dt, int nf, const f3d_fields* fl, int end_of_step) {
   Scratch* S;
   F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
   F3D_REQUIRE(ctx, nf >= 1 && nf <= 8, "transport: 1 to 8 fluids");
-  next_salt(ctx, S, 2);
   RF f[8];
   TSet t[8];
   for (int k = 0; k < nf; k++) {
@@ -2834,7 +2767,6 @@ int F3D_API(fused_dust_diffusion)(f3d_hip_ctx* ctx, double dt, int nf, double* c
 int F3D_API(fused_sources)(f3d_hip_ctx* ctx, double dt, double domega) {
   Scratch* S;
   F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
-  next_salt(ctx, S, 1);
   const dmesh& m = S->m;
   const RF f = rfields(ctx);
   cell::ViscPar vp;
@@ -2888,7 +2820,6 @@ int F3D_API(fused_sources_fluids)(f3d_hip_ctx* ctx, double dt, double domega, in
   Scratch* S;
   F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
   F3D_REQUIRE(ctx, nf >= 1 && nf <= 8 && fl && type, "sources: 1 to 8 fluids");
-  next_salt(ctx, S, 1);
   F3D_REQUIRE(ctx, ctx->ph.eos != 2, "sources of several fluids in one launch: isothermal runs");
   const dmesh& m = S->m;
   RF f[8];

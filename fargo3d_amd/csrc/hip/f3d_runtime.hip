@@ -80,9 +80,7 @@ int f3d_hip_create(const f3d_mesh_desc* d, f3d_hip_ctx** out) {
   m.xmin64 = m.xmin; m.ymin64 = m.ymin; m.zmin64 = m.zmin;
   m.ymed64 = m.ymed; m.sinzmed64 = m.sinzmed; m.sxi64 = m.sxi;
   m.zmed64 = m.zmed; m.sinzmin64 = m.sinzmin; m.coszmin64 = m.coszmin;
-  m.sxj64 = m.sxj; m.sxk64 = m.sxk; m.syk64 = m.syk; m.invvj64 = m.invvj;
   m.vb = nullptr;  // the FP64 form stores full azimuthal velocities
-  m.salt = 0;
   c->meshlen = total;
   *out = c;
   return 0;
