@@ -479,6 +479,21 @@ __device__ __forceinline__ void ppa_limit(real q, real& qL, real& qR) {
   if (real(6.0) * diff * cord > diff * diff) qL = real(3.0) * q - real(2.0) * qR;
   if (-diff * diff > real(6.0) * diff * cord) qR = real(3.0) * q - real(2.0) * qL;
 }
+// the swept fraction ksi of ppa_flux_value, the same for every quantity of a cell
+__device__ __forceinline__ real ppa_ksi(real v, real dt, real zsx) {
+  return (v > real(0.0)) ? v * dt / zsx : -v * dt / zsx;
+}
+__device__ __forceinline__ real ppa_flux_ksi(real v, real ksi, real qm, real qLm, real qRm, real q, real qL, real qR) {
+  real qs;
+  if (v > real(0.0)) {
+    qs = qRm + ksi * (qm - qRm);
+    qs += ksi * (real(1.0) - ksi) * (real(2.0) * qm - qRm - qLm);
+  } else {
+    qs = qL + ksi * (q - qL);
+    qs += ksi * (real(1.0) - ksi) * (real(2.0) * q - qR - qL);
+  }
+  return qs;
+}
 // parabolic average over the swept zone (src/fargo_ppa_d.c:62-70)
 __device__ __forceinline__ real ppa_flux_value(real v, real dt, real zsx, real qm, real qLm, real qRm,
                                                  real q, real qL, real qR) {
@@ -500,12 +515,23 @@ __device__ __forceinline__ real ppa_flux_value(real v, real dt, real zsx, real q
 __device__ __forceinline__ real flux_x(const dmesh& m, int i, int j, int k, real dt, real a, real b) {
   return mul_invvol(m, mul_surfx(m, a - b, j, k) * dt, i, j, k);
 }
+// the same with the cell's InvVol divisor shared by its quantities
+__device__ __forceinline__ real flux_x(const dmesh& m, int j, int k, real dt, real a, real b, const Recip& V) {
+  return mul_invvol(m, mul_surfx(m, a - b, j, k) * dt, j, V);
+}
 // src/update_y.c:68-70: (a*SurfY(j) - b*SurfY(j+1))*dt*InvVol
 __device__ __forceinline__ real flux_y(const dmesh& m, int i, int j, int k, real dt, real a, real b) {
   return mul_invvol(m, (mul_surfy(m, a, i, j, k) - mul_surfy(m, b, i, j + 1, k)) * dt, i, j, k);
 }
 __device__ __forceinline__ real flux_z(const dmesh& m, int i, int j, int k, real dt, real a, real b) {
   return mul_invvol(m, (mul_surfz(m, a, i, j, k) - mul_surfz(m, b, i, j, k + 1)) * dt, i, j, k);
+}
+// the same with the cell's InvVol divisor shared by its quantities
+__device__ __forceinline__ real flux_y(const dmesh& m, int i, int j, int k, real dt, real a, real b, const Recip& V) {
+  return mul_invvol(m, (mul_surfy(m, a, i, j, k) - mul_surfy(m, b, i, j + 1, k)) * dt, j, V);
+}
+__device__ __forceinline__ real flux_z(const dmesh& m, int i, int j, int k, real dt, real a, real b, const Recip& V) {
+  return mul_invvol(m, (mul_surfz(m, a, i, j, k) - mul_surfz(m, b, i, j, k + 1)) * dt, j, V);
 }
 
 // ---------------------------------------------------------------- cfl (src/cfl.c:174-253)
@@ -550,28 +576,32 @@ __device__ __forceinline__ double cfl_sumsq(const dmesh& m, int i, int j, int k,
     viscosity = cp.nu;
   }
   const double soundspeed = sqrt(soundspeed2);
+  // every quotient below divides by one of the three zone sizes: one
+  // reciprocal each, shared (RecipD), with zmin's the one of the smallest
   const double zx = zone_size_x64<G>(m, i, j, k), zy = zone_size_y64(m, j);
   const double zz = DZ ? zone_size_z64<G>(m, j, k) : 1.0;
-  double zmin = zx < zy ? zx : zy;
-  if (DZ && zz < zmin) zmin = zz;
-  const double cfl1 = soundspeed / zmin;
+  const RecipD Zx = recipd(zx), Zy = recipd(zy);
+  const RecipD Zz = DZ ? recipd(zz) : RecipD{1.0, 1.0};
+  RecipD Zmin = zx < zy ? Zx : Zy;
+  if (DZ && zz < Zmin.d) Zmin = Zz;
+  const double cfl1 = rdivd(soundspeed, Zmin);
   const double ax = fabs(vxx), axp = fabs(vxxp);
-  const double cfl2 = (ax > axp ? ax : axp) / zx;
+  const double cfl2 = rdivd(ax > axp ? ax : axp, Zx);
   const double vy0 = vy[l], vy1 = vy[lyp];
   const double ay = fabs(vy0), ayp = fabs(vy1);
-  const double cfl3 = (ay > ayp ? ay : ayp) / zy;
+  const double cfl3 = rdivd(ay > ayp ? ay : ayp, Zy);
   double cfl4 = 0.0, cfl5_c = 0.0;
   if (DZ) {
     const double vz0 = vz[l], vz1 = vz[lzp];
     const double az = fabs(vz0), azp = fabs(vz1);
-    cfl4 = (az > azp ? az : azp) / zz;
-    cfl5_c = fabs(vz1 - vz0) / zz;
+    cfl4 = rdivd(az > azp ? az : azp, Zz);
+    cfl5_c = rdivd(fabs(vz1 - vz0), Zz);
   }
-  const double cfl5_a = fabs(vx1 - vx0) / zx;
-  const double cfl5_b = fabs(vy1 - vy0) / zy;
+  const double cfl5_a = rdivd(fabs(vx1 - vx0), Zx);
+  const double cfl5_b = rdivd(fabs(vy1 - vy0), Zy);
   const double t5 = cfl5_a > cfl5_b ? cfl5_a : cfl5_b;
   const double cfl5 = (t5 > cfl5_c ? t5 : cfl5_c) * 4.0 * 1.41;  // 4 CVNR
-  const double m7 = 1.0 / zmin;
+  const double m7 = Zmin.r;  // 1.0 / zmin
   const double cfl7 = 4.0 * viscosity * (m7 * m7);
   // the reference sums ten squares; the MHD/strong-shock terms (cfl6, cfl8-10)
   // are exact zeros here and adding +0.0 does not change the sum

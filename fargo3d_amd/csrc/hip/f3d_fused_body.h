@@ -651,7 +651,8 @@ __global__ void __launch_bounds__(WG) kf_substep3(dmesh m, real dt, const real* 
 //   col.Q(c, r)     conserved quantity c at row r
 //   col.zs(r)       zone size of row r in the sweep direction
 //   col.v(r)        velocity at the lower face of row r
-//   col.flux(r, a, b) conservative increment of row r from face fluxes a, b
+//   col.vol(r)      the divisor of row r's InvVol (rdiv(), shared by its quantities)
+//   col.flux(r, a, b, V) conservative increment of row r from face fluxes a, b
 //   col.out(c, r, value) store of the updated quantity (c == NQ: density)
 // In the precision form the first Col::NXQ quantities (the azimuthal momenta)
 // are deviations Q' = Q - rho l0(r) from the profile's angular momentum
@@ -687,8 +688,9 @@ __device__ __forceinline__ void column_sweep(Col& col, int r0, int r1, real dt) 
     for (int t = 0; t < 5; t++) {
       const int r = r0 - 2 + t;
       const real rr = col.rho(r);
+      const Recip R = recip(rr);
 #pragma unroll
-      for (int c = 0; c < NQ; c++) a[c][t] = col.Q(c, r) / rr;
+      for (int c = 0; c < NQ; c++) a[c][t] = rdiv(col.Q(c, r), R);
       a[NQ][t] = rr;
     }
     real dm = real(0.0);
@@ -698,13 +700,15 @@ __device__ __forceinline__ void column_sweep(Col& col, int r0, int r1, real dt) 
       D1 = db(r0 + 1);
       D2 = db(r0 + 2);
     }
+    const Recip Z0 = recip(col.zs(r0 - 1)), Z1 = recip(col.zs(r0)), Z2 = recip(col.zs(r0 + 1)),
+                Z3 = recip(col.zs(r0 + 2));
 #pragma unroll
     for (int c = 0; c < N; c++) {
       const bool dev = c < NXQ;
-      const real g0 = dsh(a[c][1] - a[c][0], dm, dev) / col.zs(r0 - 1);
-      const real g1 = dsh(a[c][2] - a[c][1], D0, dev) / col.zs(r0);
-      const real g2 = dsh(a[c][3] - a[c][2], D1, dev) / col.zs(r0 + 1);
-      const real g3 = dsh(a[c][4] - a[c][3], D2, dev) / col.zs(r0 + 2);
+      const real g0 = rdiv(dsh(a[c][1] - a[c][0], dm, dev), Z0);
+      const real g1 = rdiv(dsh(a[c][2] - a[c][1], D0, dev), Z1);
+      const real g2 = rdiv(dsh(a[c][3] - a[c][2], D1, dev), Z2);
+      const real g3 = rdiv(dsh(a[c][4] - a[c][3], D2, dev), Z3);
       const real sm = vanleer_slope(g0, g1);
       const real s0 = vanleer_slope(g1, g2);
       const real s1 = vanleer_slope(g2, g3);
@@ -720,25 +724,28 @@ __device__ __forceinline__ void column_sweep(Col& col, int r0, int r1, real dt) 
   }
   for (int r = r0; r < r1; r++) {
     const real v0 = col.v(r), v1 = col.v(r + 1);
+    const Recip V = col.vol(r);
     // UpdateY/Z (src/update_y.c:68-70), then the density last (src/update_density_y.c:66-68)
 #pragma unroll
     for (int c = 0; c < NQ; c++) {
       const real f0 = (c < NXQ) ? F0[c] - D0 : F0[c];
-      col.out(c, r, col.Q(c, r) + col.flux(r, v0 * f0 * F0[NQ], v1 * F1[c] * F1[NQ]));
+      col.out(c, r, col.Q(c, r) + col.flux(r, v0 * f0 * F0[NQ], v1 * F1[c] * F1[NQ], V));
     }
-    col.out(NQ, r, col.rho(r) + col.flux(r, v0 * F0[NQ], v1 * F1[NQ]));
+    col.out(NQ, r, col.rho(r) + col.flux(r, v0 * F0[NQ], v1 * F1[NQ], V));
     if (r + 1 >= r1) break;
     // advance the window by one row: row r+3 enters
     const int rn = r + 3;
     const real rr = col.rho(rn);
+    const Recip R = recip(rr);
     const real zsn = col.zs(rn), zs2 = col.zs(r + 2), zs1 = col.zs(r + 1), v2 = col.v(r + 2);
+    const Recip Zn = recip(zsn);
     real D3 = real(0.0);
     if constexpr (NXQ > 0) D3 = db(rn);
 #pragma unroll
     for (int c = 0; c < N; c++) {
       const bool dev = c < NXQ;
-      const real an = (c < NQ) ? col.Q(c, rn) / rr : rr;
-      const real g3 = dsh(an - A2[c], D3, dev) / zsn;
+      const real an = (c < NQ) ? rdiv(col.Q(c, rn), R) : rr;
+      const real g3 = rdiv(dsh(an - A2[c], D3, dev), Zn);
       const real s2 = vanleer_slope(G2[c], g3);
       const real f2 = vanleer_face(v2, dt, A1[c], S1[c], zs1, dsh(A2[c], D2, dev), s2, zs2);
       F0[c] = F1[c];
@@ -770,7 +777,8 @@ struct ColPart {
   __device__ __forceinline__ real Q(int q, int r) const { return c.Q(FIRST + q, r); }
   __device__ __forceinline__ real zs(int r) const { return c.zs(r); }
   __device__ __forceinline__ real v(int r) const { return c.v(r); }
-  __device__ __forceinline__ real flux(int r, real a, real b) const { return c.flux(r, a, b); }
+  __device__ __forceinline__ Recip vol(int r) const { return c.vol(r); }
+  __device__ __forceinline__ real flux(int r, real a, real b, const Recip& V) const { return c.flux(r, a, b, V); }
   __device__ __forceinline__ void out(int q, int r, real val) const {
     if (q < COUNT) c.out(FIRST + q, r, val);
     else if (WRITE_RHO) c.out(NQ, r, val);
@@ -819,7 +827,10 @@ struct ColY2 {
   }
   __device__ __forceinline__ real zs(int r) const { return zone_size_y(*m, r); }
   __device__ __forceinline__ real v(int r) const { return vyt[L(r)]; }
-  __device__ __forceinline__ real flux(int r, real a, real b) const { return cell::flux_y(*m, i, r, 0, dt, a, b); }
+  __device__ __forceinline__ Recip vol(int r) const { return invvol_recip(*m, i, 0); }
+  __device__ __forceinline__ real flux(int r, real a, real b, const Recip& V) const {
+    return cell::flux_y(*m, i, r, 0, dt, a, b, V);
+  }
   __device__ __forceinline__ void out(int c, int r, real val) const { out_[c][L(r)] = val; }
 };
 
@@ -867,7 +878,10 @@ struct ColZ {
   }
   __device__ __forceinline__ real zs(int r) const { return zone_size_z<G>(*m, j, r); }
   __device__ __forceinline__ real v(int r) const { return vzt[L(r)]; }
-  __device__ __forceinline__ real flux(int r, real a, real b) const { return cell::flux_z(*m, i, j, r, dt, a, b); }
+  __device__ __forceinline__ Recip vol(int r) const { return invvol_recip(*m, i, r); }
+  __device__ __forceinline__ real flux(int r, real a, real b, const Recip& V) const {
+    return cell::flux_z(*m, i, j, r, dt, a, b, V);
+  }
   __device__ __forceinline__ void out(int c, int r, real val) const { out_[c][L(r)] = val; }
 };
 
@@ -888,7 +902,10 @@ struct ColY3 {
   __device__ __forceinline__ real Q(int c, int r) const { return q_in[c][L(r)]; }
   __device__ __forceinline__ real zs(int r) const { return zone_size_y(*m, r); }
   __device__ __forceinline__ real v(int r) const { return vyt[L(r)]; }
-  __device__ __forceinline__ real flux(int r, real a, real b) const { return cell::flux_y(*m, i, r, k, dt, a, b); }
+  __device__ __forceinline__ Recip vol(int r) const { return invvol_recip(*m, i, k); }
+  __device__ __forceinline__ real flux(int r, real a, real b, const Recip& V) const {
+    return cell::flux_y(*m, i, r, k, dt, a, b, V);
+  }
   __device__ __forceinline__ void out(int c, int r, real val) const { out_[c][L(r)] = val; }
 };
 
@@ -1092,8 +1109,9 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
       const int i = t + s * nt;
       if (i < nx) {
         const real r = st[s][NL];
+        const Recip R = recip(r);
 #pragma unroll
-        for (int c = 0; c < NL; c++) AQ[c * nxv + i] = st[s][c] / r;
+        for (int c = 0; c < NL; c++) AQ[c * nxv + i] = rdiv(st[s][c], R);
         AQ[NL * nxv + i] = r;
       }
     }
@@ -1108,11 +1126,11 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
         const int i = t + s * nt;
         if (i < nx) {
           const int im = ixm_of(i, nx);
-          const real zi = zone_size_x<G>(m, i, j, k);
+          const Recip Zi = recip(zone_size_x<G>(m, i, j, k));
 #pragma unroll
           for (int c = 0; c < N; c++) {
             const real* a = AQ + c * nxv;
-            FC[c * nxv + i] = (a[i] - a[im]) / zi;
+            FC[c * nxv + i] = rdiv(a[i] - a[im], Zi);
           }
         }
       }
@@ -1216,14 +1234,14 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
         const int i = t + s * nt;
         if (i < nx) {
           const int im = ixm_of(i, nx);
-          const real zi = zone_size_x<G>(m, i, j, k);
           const real v = V[i];
+          const real ksi = cell::ppa_ksi(v, dt, zone_size_x<G>(m, i, j, k));
 #pragma unroll
           for (int c = 0; c < N; c++) {
             const real* a = AQ + c * nxv;
             const real* qL = SL + c * nxv;
             const real* qR = FC + c * nxv;
-            qr[s][c] = cell::ppa_flux_value(v, dt, zi, a[im], qL[im], qR[im], a[i], qL[i], qR[i]);
+            qr[s][c] = cell::ppa_flux_ksi(v, ksi, a[im], qL[im], qR[im], a[i], qL[i], qR[i]);
           }
         }
       }
@@ -1249,12 +1267,13 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
         const int ip = ixp_of(i, nx);
         const real v0 = V[i], v1 = V[ip];
         const real* ds = FC + NL * nxv;
+        const Recip Vi = invvol_recip(m, i, k);
 #pragma unroll
         for (int c = 0; c < NL; c++) {
           const real* f = FC + c * nxv;
-          st[s][c] += cell::flux_x(m, i, j, k, dt, v0 * f[i] * ds[i], v1 * f[ip] * ds[ip]);
+          st[s][c] += cell::flux_x(m, j, k, dt, v0 * f[i] * ds[i], v1 * f[ip] * ds[ip], Vi);
         }
-        st[s][NL] += cell::flux_x(m, i, j, k, dt, v0 * ds[i], v1 * ds[ip]);
+        st[s][NL] += cell::flux_x(m, j, k, dt, v0 * ds[i], v1 * ds[ip], Vi);
       }
     }
   }
@@ -1400,8 +1419,9 @@ __device__ __forceinline__ void kd_sweep_x_tile(const dim3 bid, const dmesh& m, 
     const int lo = pass ? 2 : 0, hi = pass ? W - 2 : W;
     if (p >= lo && p < hi) {
       const real r = st[NQ];
+      const Recip R = recip(r);
 #pragma unroll
-      for (int c = 0; c < NQ; c++) AQ[c * W + p] = st[c] / r;
+      for (int c = 0; c < NQ; c++) AQ[c * W + p] = rdiv(st[c], R);
       AQ[NQ * W + p] = r;
     }
     __syncthreads();  // also orders the first pass after the stores of V
@@ -1416,10 +1436,11 @@ __device__ __forceinline__ void kd_sweep_x_tile(const dim3 bid, const dmesh& m, 
       // the gradient at face p, (q_p - q_{p-1}) / zone_p, once per face into
       // FC: the left gradient of cell p and the right one of cell p-1
       if (p >= 1) {
+        const Recip Z = recip(zs);
 #pragma unroll
         for (int c = 0; c < N; c++) {
           const real* a = AQ + c * W;
-          FC[c * W + p] = (a[p] - a[p - 1]) / zs;
+          FC[c * W + p] = rdiv(a[p] - a[p - 1], Z);
         }
       }
       __syncthreads();
@@ -1479,12 +1500,13 @@ __device__ __forceinline__ void kd_sweep_x_tile(const dim3 bid, const dmesh& m, 
       // face fluxes into FC once every neighbour has read qR
       const bool flx = p >= 5 && p < W - 4;
       if (flx) {
+        const real ksi = cell::ppa_ksi(vp, dt, zs);
 #pragma unroll
         for (int c = 0; c < N; c++) {
           const real* a = AQ + c * W;
           const real* qL = SL + c * W;
           const real* qR = FC + c * W;
-          qr[c] = cell::ppa_flux_value(vp, dt, zs, a[p - 1], qL[p - 1], qR[p - 1], a[p], qL[p], qR[p]);
+          qr[c] = cell::ppa_flux_ksi(vp, ksi, a[p - 1], qL[p - 1], qR[p - 1], a[p], qL[p], qR[p]);
         }
       }
       __syncthreads();
@@ -1500,12 +1522,13 @@ __device__ __forceinline__ void kd_sweep_x_tile(const dim3 bid, const dmesh& m, 
     if (p >= ulo && p < uhi) {
       const real v0 = vp, v1 = V[p + 1];
       const real* ds = FC + NQ * W;
+      const Recip Vg = invvol_recip(m, g, k);
 #pragma unroll
       for (int c = 0; c < NQ; c++) {
         const real* f = FC + c * W;
-        st[c] += cell::flux_x(m, g, j, k, dt, v0 * f[p] * ds[p], v1 * f[p + 1] * ds[p + 1]);
+        st[c] += cell::flux_x(m, j, k, dt, v0 * f[p] * ds[p], v1 * f[p + 1] * ds[p + 1], Vg);
       }
-      st[NQ] += cell::flux_x(m, g, j, k, dt, v0 * ds[p], v1 * ds[p + 1]);
+      st[NQ] += cell::flux_x(m, j, k, dt, v0 * ds[p], v1 * ds[p + 1], Vg);
     }
     if (pass == 0 && fargo) __syncthreads();  // the PPA pass rewrites AQ and V
   }
