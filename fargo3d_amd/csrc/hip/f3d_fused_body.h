@@ -2293,7 +2293,7 @@ static int x_vmed(f3d_hip_ctx* ctx, Scratch* S, int nf, const RF* f, const TSet*
 }
 
 // tile width of the tiled X sweep (cells per workgroup): 64, 128 or 256
-static int g_xtile = 256;
+static int g_xtile = 128;
 
 // Where the X sweep's VxMed comes from: 0 = the ring kernel's own sum,
 // 1 = kf_vmed on the side stream, overlapping the Z/Y sweeps, 2 = kf_vmed on

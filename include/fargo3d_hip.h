@@ -285,7 +285,7 @@ int f3d_fused_set_xmode(int mode);
  * workgroups per CU), 0 one workgroup per ring, -1 automatic; results do not
  * depend on it.  Returns -2 for other values. */
 int f3d_fused_set_xsplit(int split);
-/* Cells per workgroup of the tiled X sweep: 64, 128 or 256 (default);
+/* Cells per workgroup of the tiled X sweep: 64, 128 (default) or 256;
  * results do not depend on it.  Returns -2 for other values. */
 int f3d_fused_set_xtile(int cells);
 /* Where the ring X sweep takes each ring's VxMed (ComputeVmed of Vx_temp,
