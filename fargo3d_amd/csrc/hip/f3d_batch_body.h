@@ -331,10 +331,12 @@ __global__ void __launch_bounds__(BXF) kb_sweep_y(const BInst* I, const int* off
                          in.rhoy, in.mpx, in.mmx, in.mpy, in.mmy, in.ey, rows_per, in.of);
 }
 
-__global__ void __launch_bounds__(WG) kb_vmed(const BInst* I, const int* off, int n) {
+// one wave per ring, 8 terms per lane (the instances' rings hold 256 to a few
+// thousand cells)
+__global__ void __launch_bounds__(64) kb_vmed(const BInst* I, const int* off, int n) {
   const int ii = find_inst(off, n, blockIdx.x);
   const BInst& in = I[ii];
-  kd_vmed<CYL, 0, 0>(dim3(blockIdx.x - off[ii]), in.m, in.vxt, in.vxmed);
+  kd_vmed<CYL, 0, 8>(dim3(blockIdx.x - off[ii]), in.m, in.vxt, in.vxmed);
 }
 
 template <int T>
@@ -445,7 +447,7 @@ static void batch_launch(f3d_batch* B) {
   hipLaunchKernelGGL(kb_src_b, dim3(B->total[BK_SRCB]), dim3(BXF), 0, s, B->dI, B->doff[BK_SRCB], n);
   hipLaunchKernelGGL(kb_bound, dim3(B->total[BK_BND]), dim3(BXF), 0, s, B->dI, B->doff[BK_BND], n, 1);
   hipLaunchKernelGGL(kb_sweep_y, dim3(B->swy_total), dim3(BXF), 0, s, B->dI, B->dswy, n, B->rows_y);
-  hipLaunchKernelGGL(kb_vmed, dim3(B->total[BK_VMED]), dim3(WG), (size_t)B->nxmax * sizeof(double), s, B->dI,
+  hipLaunchKernelGGL(kb_vmed, dim3(B->total[BK_VMED]), dim3(64), 0, s, B->dI,
                      B->doff[BK_VMED], n);
   if (B->tile == 64) hipLaunchKernelGGL(kb_sweep_x<64>, dim3(B->xt_total), dim3(64), 0, s, B->dI, B->dxt, n);
   else if (B->tile == 256) hipLaunchKernelGGL(kb_sweep_x<256>, dim3(B->xt_total), dim3(256), 0, s, B->dI, B->dxt, n);

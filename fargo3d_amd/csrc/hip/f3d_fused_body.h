@@ -259,47 +259,46 @@ __device__ __forceinline__ double seq_sum(const double* w, int n) {
   return acc;
 }
 
-// seq_sum for the VxMed pass kernels: the same additions in the same order,
-// the LDS terms read in pairs (w 16-byte aligned), 2B terms per batch.
-// Measured on MI355X: 4096 terms 61.6 -> 51.7 us (B = 16), 256 terms
-// 11.9 -> 9.9 us (B = 8), for every ring of the grid at once.  B = 16 holds
-// 128 registers: only for the CFL's pass on long rings, which runs alone;
-// the X sweep's pass shares the CUs with the Z/Y sweeps and keeps B = 8.
-template <int B>
-__device__ __forceinline__ double seq_sum_pairs(const double* w, int n) {
-  const double2* w2 = reinterpret_cast<const double2*>(w);
+// The same sequential sum of a ring's weights w_i = v_i (x_{i+1} - x_i)
+// (src/compute_vmed.c:14-21), i in [0, n), by one wave (blockDim.x == 64)
+// with the terms in registers.  The ring goes in chunks of 64*S terms, lane L
+// holding terms [c + L*S, c + (L+1)*S) of the chunk at c.  The running sum
+// passes from lane to lane: every lane adds its own S terms to the sum so far
+// (one chain of dependent register additions), and lane L's result -- the
+// sequential sum through its last term -- is read out (v_readlane) as the
+// start of lane L+1's chain; the other lanes' results are discarded.  Terms
+// past the ring's end are +0.0, which leaves a sum started at +0.0 unchanged
+// (in round-to-nearest such a sum is never -0.0).  The next chunk's loads are
+// issued before the current chunk's chain.  Every lane returns the sum.
+// Measured on MI355X: a dependent FP64 addition in registers takes about
+// 4.5 ns, the same sum read from LDS about 12 ns per term.
+template <int S>
+__device__ __forceinline__ double wave_ring_sum(const real* __restrict__ v, const double* __restrict__ x, int n) {
+#pragma clang fp contract(off)
+  const int lane = (int)threadIdx.x;
+  double a[S], x0[S + 1];
+  auto load = [&](int c) {
+#pragma unroll
+    for (int s = 0; s < S; s++) a[s] = (double)v[min(c + lane * S + s, n - 1)];
+#pragma unroll
+    for (int s = 0; s <= S; s++) x0[s] = x[min(c + lane * S + s, n)];
+  };
+  load(0);
   double acc = 0.0;
-  int i = 0;  // in pairs
-  const int n2 = n / 2;
-  if (n2 >= 2 * B) {
-    double2 a[B], b[B];
+  for (int c = 0; c < n; c += 64 * S) {
+    double w[S];
 #pragma unroll
-    for (int q = 0; q < B; q++) a[q] = w2[q];
-    for (i = B; i + 2 * B <= n2; i += 2 * B) {
+    for (int s = 0; s < S; s++) w[s] = (c + lane * S + s < n) ? a[s] * (x0[s + 1] - x0[s]) : 0.0;
+    if (c + 64 * S < n) load(c + 64 * S);
+    const int lanes = min(64, (n - c + S - 1) / S);
+    for (int L = 0; L < lanes; L++) {
+      double t = acc;
 #pragma unroll
-      for (int q = 0; q < B; q++) b[q] = w2[i + q];
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int q = 0; q < B; q++) {
-        acc = acc + a[q].x;
-        acc = acc + a[q].y;
-      }
-#pragma unroll
-      for (int q = 0; q < B; q++) a[q] = w2[i + B + q];
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int q = 0; q < B; q++) {
-        acc = acc + b[q].x;
-        acc = acc + b[q].y;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < B; q++) {
-      acc = acc + a[q].x;
-      acc = acc + a[q].y;
+      for (int s = 0; s < S; s++) t = t + w[s];
+      acc = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(t), L),
+                             __builtin_amdgcn_readlane(__double2loint(t), L));
     }
   }
-  for (int e = 2 * i; e < n; e++) acc = acc + w[e];
   return acc;
 }
 
@@ -387,19 +386,15 @@ __global__ void __launch_bounds__(WG) kf_cfl(dmesh m, const real* __restrict__ c
 }
 
 // ComputeVmed (src/compute_vmed.c:14-30) of rings j = NGHY .. NGHY+nry-1 of
-// every active plane: one workgroup per ring, the weights in LDS, one
-// sequential sum in i order.
-template <int G, int DZ, int LONG>
-__global__ void __launch_bounds__(WG) kf_vmed_rings(dmesh m, const real* __restrict__ v, double* __restrict__ vxmed, int nry) {
-  extern __shared__ double w[];
+// every active plane: one wave per ring, one sequential sum in i order with
+// S terms per lane (wave_ring_sum).
+template <int G, int DZ, int S>
+__global__ void __launch_bounds__(64) kf_vmed_rings(dmesh m, const real* __restrict__ v, double* __restrict__ vxmed, int nry) {
   const int j = m.nghy + (int)(blockIdx.x % nry);
   const int k = DZ ? m.nghz + (int)(blockIdx.x / nry) : 0;
   const int base = j * m.pitch + k * m.stride;
-  ring_weights(w, v + base, m.xmin64, m.nx);
-  __syncthreads();
-  if (threadIdx.x == 0)
-    vxmed[j + k * m.sy] =
-        vmed_full((LONG ? seq_sum_pairs<16>(w, m.nx) : seq_sum_pairs<8>(w, m.nx)) / (m.XMAX - m.XMIN), m, j, k);
+  const double sum = wave_ring_sum<S>(v + base, m.xmin64, m.nx);
+  if (threadIdx.x == 0) vxmed[j + k * m.sy] = vmed_full(sum / (m.XMAX - m.XMIN), m, j, k);
 }
 
 // The cfl of nf fluids in one launch: nf consecutive blocks of nrings
@@ -586,7 +581,7 @@ __device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt
   if (!iso) e2[l] = e;
 }
 template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_src_b(dmesh m, real dt, const real* __restrict__ rho, const real* __restrict__ energy,
+__global__ void __launch_bounds__(WG, DZ ? 5 : 8) kf_src_b(dmesh m, real dt, const real* __restrict__ rho, const real* __restrict__ energy,
                                                const real* __restrict__ vx, const real* __restrict__ vy, const real* __restrict__ vz,
                                                const double* __restrict__ pot, const real* __restrict__ txx, const real* __restrict__ tyy,
                                                const real* __restrict__ tzz, const real* __restrict__ txz, const real* __restrict__ tyx,
@@ -611,7 +606,7 @@ struct SrcTab {
   int dust[8];
 };
 template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_src_b_fluids(dmesh m, real dt, SrcTab T, int nzb, const double* __restrict__ pot,
+__global__ void __launch_bounds__(WG, DZ ? 5 : 8) kf_src_b_fluids(dmesh m, real dt, SrcTab T, int nzb, const double* __restrict__ pot,
                                                       const real* __restrict__ txx, const real* __restrict__ tyy,
                                                       const real* __restrict__ tzz, const real* __restrict__ txz,
                                                       const real* __restrict__ tyx, const real* __restrict__ tzy,
@@ -934,7 +929,7 @@ struct Y2Tab {
   real* e_out[8];
 };
 template <int G, int DZ, int AD, int SPLIT>
-__global__ void __launch_bounds__(BXF) kf_sweep_y2(dmesh m, real dt, Y2Tab T, int rows_per, real of) {
+__global__ void __launch_bounds__(BXF, AD ? 1 : 4) kf_sweep_y2(dmesh m, real dt, Y2Tab T, int rows_per, real of) {
   const dim3 b = xcd_bid();
   const int fl = (int)b.z;
   kd_sweep_y2<G, DZ, AD, SPLIT>(b, m, dt, T.rho_in[fl], T.vxt[fl], T.vyt[fl], T.e_in[fl], T.rho_out[fl], T.q[fl][0],
@@ -1326,29 +1321,25 @@ inline size_t xsweep_ring_lds(int nx, int ncarried) {
   return (3 * P + nx) * sizeof(real) + 64;
 }
 
-// ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30) for the tiled X sweep: one
-// workgroup (one wave) per active ring and fluid, the ring's weights in LDS,
-// one sequential sum; blocks [f*nrings, (f+1)*nrings) take fluid f.
+// ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30) for the X sweeps: one wave
+// per active ring and fluid, one sequential sum with S terms per lane
+// (wave_ring_sum); blocks [f*nrings, (f+1)*nrings) take fluid f.
 struct VmTab {
   const real* vxt[8];
   double* vxmed[8];
 };
-template <int G, int DZ, int LONG>
+template <int G, int DZ, int S>
 __device__ __forceinline__ void kd_vmed(const dim3 bid, const dmesh& m, const real* vxt, double* vxmed) {
-  extern __shared__ double w[];
   const int j = m.nghy + (int)(bid.x % m.ny);
   const int k = DZ ? m.nghz + (int)(bid.x / m.ny) : 0;
   const int base = j * m.pitch + k * m.stride;
-  ring_weights(w, vxt + base, m.xmin64, m.nx);
-  __syncthreads();
-  if (threadIdx.x == 0)
-    vxmed[j + k * m.sy] =
-        vmed_full((LONG ? seq_sum_pairs<16>(w, m.nx) : seq_sum_pairs<8>(w, m.nx)) / (m.XMAX - m.XMIN), m, j, k);
+  const double sum = wave_ring_sum<S>(vxt + base, m.xmin64, m.nx);
+  if (threadIdx.x == 0) vxmed[j + k * m.sy] = vmed_full(sum / (m.XMAX - m.XMIN), m, j, k);
 }
-template <int G, int DZ, int LONG>
+template <int G, int DZ, int S>
 __global__ void __launch_bounds__(64) kf_vmed(dmesh m, VmTab T, int nrings) {
   const int fl = (int)blockIdx.x / nrings;
-  kd_vmed<G, DZ, LONG>(dim3(blockIdx.x % nrings), m, T.vxt[fl], T.vxmed[fl]);
+  kd_vmed<G, DZ, S>(dim3(blockIdx.x % nrings), m, T.vxt[fl], T.vxmed[fl]);
 }
 
 // X sweep of rings too long for LDS: a workgroup of T threads owns a tile of
@@ -2106,6 +2097,17 @@ static int cfl_mode(const dmesh& m, int nf) {
   return mode;
 }
 
+// Terms per lane of the one-wave ring sums (wave_ring_sum): 4, 8 or 16; -1
+// automatic.  Measured on MI355X (whole FP64 step): 8 is fastest on rings of
+// 256 (3-D disk, 0.2503/0.2488/0.2546 ms for 4/8/16), 1024 (gas + 4 dust,
+// 0.4682/0.4573/0.4616 ms) and 4096 cells (adiabatic disk, 0.6811/0.6639/
+// 0.6756 ms), against 0.2515 and 0.680 ms with the sums read from LDS.
+static int g_vmed_terms = -1;
+static int vmed_terms(int nx) {
+  (void)nx;
+  return g_vmed_terms > 0 ? g_vmed_terms : 8;
+}
+
 // VxMed of every active ring and of the ring above the last (the shear limit
 // reads ring j+1), for the CFL kernel's pre-pass form: one wave per ring, four
 // for rings of 1024 cells or more (measured on MI355X: their loads, issued
@@ -2113,10 +2115,11 @@ static int cfl_mode(const dmesh& m, int nf) {
 static int vmed_prepass_m(f3d_hip_ctx* ctx, const dmesh& m, const real* v) {
   const int nry = m.ny + (ctx->ph.fargo ? 1 : 0);
   const int nr = nry * (m.dimz ? m.nz : 1);
-  if (m.nx >= 1024)
-    FDISPATCH_T(ctx, kf_vmed_rings, 1, dim3(nr), dim3(256), m.nx * sizeof(double), v, ctx->f.VxMed, nry);
-  else
-    FDISPATCH_T(ctx, kf_vmed_rings, 0, dim3(nr), dim3(64), m.nx * sizeof(double), v, ctx->f.VxMed, nry);
+  switch (vmed_terms(m.nx)) {
+    case 4: FDISPATCH_T(ctx, kf_vmed_rings, 4, dim3(nr), dim3(64), 0, v, ctx->f.VxMed, nry); break;
+    case 8: FDISPATCH_T(ctx, kf_vmed_rings, 8, dim3(nr), dim3(64), 0, v, ctx->f.VxMed, nry); break;
+    default: FDISPATCH_T(ctx, kf_vmed_rings, 16, dim3(nr), dim3(64), 0, v, ctx->f.VxMed, nry); break;
+  }
   return 0;
 }
 static int vmed_prepass(f3d_hip_ctx* ctx, Scratch* S, const real* v) { return vmed_prepass_m(ctx, S->m, v); }
@@ -2280,14 +2283,21 @@ static int x_vmed(f3d_hip_ctx* ctx, Scratch* S, int nf, const RF* f, const TSet*
     T.vxt[q] = f[q].Vx_temp;
     T.vxmed[q] = t[q].vxmed;
   }
-#define XV(G_, DZ_) \
-  hipLaunchKernelGGL((kf_vmed<G_, DZ_, 0>), dim3(nrings * nf), dim3(64), m.nx * sizeof(double), st, m, T, nrings)
+#define XV1(G_, DZ_, S_) hipLaunchKernelGGL((kf_vmed<G_, DZ_, S_>), dim3(nrings * nf), dim3(64), 0, st, m, T, nrings)
+#define XV(G_, DZ_)                                \
+  do {                                             \
+    const int s_ = vmed_terms(m.nx);               \
+    if (s_ == 4) XV1(G_, DZ_, 4);                  \
+    else if (s_ == 8) XV1(G_, DZ_, 8);             \
+    else XV1(G_, DZ_, 16);                         \
+  } while (0)
   if (m.geom == SPH) {
     if (m.dimz) XV(SPH, 1); else XV(SPH, 0);
   } else {
     if (m.dimz) XV(CYL, 1); else XV(CYL, 0);
   }
 #undef XV
+#undef XV1
   F3D_LAUNCH_CHECK(ctx);
   return 0;
 }
@@ -2535,6 +2545,12 @@ int F3D_API(fused_set_xtile)(int cells) {
 int F3D_API(fused_set_xmode)(int mode) {
   if (mode < 0 || mode > 2) return -2;
   g_xmode = mode;
+  return 0;
+}
+
+int F3D_API(fused_set_vmedterms)(int terms) {
+  if (terms != -1 && terms != 4 && terms != 8 && terms != 16) return -2;
+  g_vmed_terms = terms;
   return 0;
 }
 

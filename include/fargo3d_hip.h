@@ -306,6 +306,11 @@ int f3d_fused_set_colsplit(int split);
  * workgroup.  -1 automatic.  The result is the same value in every form.
  * Returns -2 for other values. */
 int f3d_fused_set_cflmode(int mode);
+/* Terms per lane of the one-wave sequential ring sums of VxMed (the CFL's
+ * and the X sweep's passes): 4, 8 or 16, -1 automatic (by ring length); the
+ * sums are the same additions in the same order in every form.  Returns -2
+ * for other values. */
+int f3d_fused_set_vmedterms(int terms);
 /* Potential, the ghost fill and the damping references on the fused stages'
  * scratch-free kernels (the same formulas as f3d_Potential, f3d_boundaries,
  * f3d_init_stockholm) */
@@ -380,6 +385,7 @@ int f3d32_fused_set_xtile(int cells);
 int f3d32_fused_set_xvmed(int form);
 int f3d32_fused_set_colsplit(int split);
 int f3d32_fused_set_cflmode(int mode);
+int f3d32_fused_set_vmedterms(int terms);
 int f3d32_fused_cfl_vm(f3d_hip_ctx *ctx, const double *vxm, double *fluid_min);
 int f3d32_fused_cfl_fluids(f3d_hip_ctx *ctx, int nf, double *const *cs, double *const *rho, double *const *vx,
                            double *const *vy, double *const *vz, const double *vxm, double *step_min);

@@ -381,6 +381,47 @@ def test_whole_step_fused_xvmed_forms(fa, orc, par, ov, nsteps, form):
         L.f3d_fused_set_xvmed(-1)
 
 
+@pytest.mark.parametrize("terms", [4, 8, 16])
+def test_vmed_pass_terms_per_lane(fa, orc, terms):
+    """The one-wave ring sums of VxMed (the CFL's pass over Vx and the X
+    sweep's pass over Vx_temp) with 4, 8 or 16 terms per lane give the
+    oracle's time step and ring means bit for bit, on rings that fill whole
+    chunks of 64 x terms, part of one, and several with a partial last one."""
+    L = fa.lib()
+    L.f3d_fused_cfl_fluids.restype = ctypes.c_int
+    L.f3d_fused_cfl_fluids.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 6 + \
+        [ctypes.POINTER(ctypes.c_double)]
+    assert L.f3d_fused_set_vmedterms(terms) == 0
+    assert L.f3d_fused_set_cflmode(3) == 0      # VxMed of Vx from the pass
+    assert L.f3d_fused_set_xvmed(2) == 0        # VxMed of Vx_temp from the pass
+    P = ctypes.c_void_p * 8
+    try:
+        for par, ov in [(C1, ""), (C2, C2_SMALL), (C3, C3_SMALL), (C3, "Nx=1000 Ny=24"), (C3, "Nx=2500 Ny=16")]:
+            o, s = make_pair(fa, orc, par, ov)
+            for n in range(3):
+                # the CFL's pass alone: ComputeVmed(Vx) (src/main.c:366)
+                tab = [P(s.device_ptr(nm)) for nm in ("Energy", "Density", "Vx", "Vy", "Vz" if s.dimz else "Vy")]
+                got = ctypes.c_double()
+                assert L.f3d_fused_cfl_fluids(s.ctx, 1, *tab, s.device_ptr("Vx"), ctypes.byref(got)) == 0
+                o.call("ComputeVmed", "Vx")
+                vm_o, vm_s = o.field("VxMed"), s.field("VxMed")
+                say("vmed terms %d %s %s step %d: CFL pass VxMed bits %d" % (terms, par, ov, n, nbits(vm_s, vm_o)))
+                assert nbits(active_rings(vm_s, s.dimz), active_rings(vm_o, s.dimz)) == 0, (terms, par, ov, n)
+                dto, dts = o.step(), s.step()
+                vm_o, vm_s = o.field("VxMed"), s.field("VxMed")
+                say("vmed terms %d %s %s step %d dt %r VxMed bits %d" % (terms, par, ov, n, dts, nbits(vm_s, vm_o)))
+                assert dto == dts, (terms, par, ov, n, dts, dto)
+                assert nbits(active_rings(vm_s, s.dimz), active_rings(vm_o, s.dimz)) == 0, (terms, par, ov, n)
+                assert np.array_equal(active_rings(s.nshift(), s.dimz), active_rings(o.nshift(), s.dimz))
+            for f in fields_of(s):
+                assert nbits(active(s.field(f), s.dimz), active(o.field(f), s.dimz)) == 0, (terms, par, ov, f)
+            s.close()
+    finally:
+        L.f3d_fused_set_vmedterms(-1)
+        L.f3d_fused_set_cflmode(-1)
+        L.f3d_fused_set_xvmed(-1)
+
+
 @pytest.mark.parametrize("ov", [C2_SMALL, C2_AD], ids=["c2_3d_small", "c2_3d_adiabatic_small"])
 def test_fused_ghost_fill_random_interior(fa, orc, ov):
     """The fused ghost fill (Z then Y, src/boundary.c:2-23) on a random
