@@ -272,6 +272,20 @@ __device__ __forceinline__ double seq_sum(const double* w, int n) {
 // issued before the current chunk's chain.  Every lane returns the sum.
 // Measured on MI355X: a dependent FP64 addition in registers takes about
 // 4.5 ns, the same sum read from LDS about 12 ns per term.
+// The chain of wave_ring_sum over one chunk: lane L's S terms w follow the
+// terms of lanes 0 .. L-1, after acc; lanes 0 .. lanes-1 hold terms.
+template <int S>
+__device__ __forceinline__ double wave_chain(const double (&w)[S], double acc, int lanes) {
+#pragma clang fp contract(off)
+  for (int L = 0; L < lanes; L++) {
+    double t = acc;
+#pragma unroll
+    for (int s = 0; s < S; s++) t = t + w[s];
+    acc = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(t), L),
+                           __builtin_amdgcn_readlane(__double2loint(t), L));
+  }
+  return acc;
+}
 template <int S>
 __device__ __forceinline__ double wave_ring_sum(const real* __restrict__ v, const double* __restrict__ x, int n) {
 #pragma clang fp contract(off)
@@ -290,14 +304,7 @@ __device__ __forceinline__ double wave_ring_sum(const real* __restrict__ v, cons
 #pragma unroll
     for (int s = 0; s < S; s++) w[s] = (c + lane * S + s < n) ? a[s] * (x0[s + 1] - x0[s]) : 0.0;
     if (c + 64 * S < n) load(c + 64 * S);
-    const int lanes = min(64, (n - c + S - 1) / S);
-    for (int L = 0; L < lanes; L++) {
-      double t = acc;
-#pragma unroll
-      for (int s = 0; s < S; s++) t = t + w[s];
-      acc = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(t), L),
-                             __builtin_amdgcn_readlane(__double2loint(t), L));
-    }
+    acc = wave_chain<S>(w, acc, min(64, (n - c + S - 1) / S));
   }
   return acc;
 }
@@ -1315,6 +1322,225 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
   }
 }
 
+// Whole-wave rotations by one lane (DPP wave_ror:1 and wave_rol:1 of the
+// gfx9 family): from_left gives lane L the value of lane L-1 and lane 0 that
+// of lane 63; from_right gives lane L the value of lane L+1 and lane 63 that
+// of lane 0.  All 64 lanes must be active.
+__device__ __forceinline__ float from_left(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x13C, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_right(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x134, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double from_left(double x) {
+  return __hiloint2double(__builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x13C, 0xF, 0xF, false),
+                          __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x13C, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double from_right(double x) {
+  return __hiloint2double(__builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x134, 0xF, 0xF, false),
+                          __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x134, 0xF, 0xF, false));
+}
+
+// A ring of 64*CPT cells held by one wave, CPT consecutive cells per lane:
+// lane L holds cells L*CPT .. L*CPT+CPT-1.  The ring is periodic, so the left
+// neighbour of a lane's first cell is the last cell of the lane before (lane
+// 63 for lane 0) and the right neighbour of its last cell the first cell of
+// the lane after: one rotation per array, no LDS and no barrier.
+template <int CPT>
+struct WRing {
+  template <class T>
+  __device__ __forceinline__ static void left(const T (&a)[CPT], T (&out)[CPT]) {
+    out[0] = from_left(a[CPT - 1]);
+#pragma unroll
+    for (int s = 1; s < CPT; s++) out[s] = a[s - 1];
+  }
+  template <class T>
+  __device__ __forceinline__ static void right(const T (&a)[CPT], T (&out)[CPT]) {
+#pragma unroll
+    for (int s = 0; s < CPT - 1; s++) out[s] = a[s + 1];
+    out[CPT - 1] = from_right(a[0]);
+  }
+};
+
+// The X sweep of one ring by one wave (rings of exactly 64*CPT cells), the
+// cell state in registers: the same arithmetic as xsweep_ring, cell for cell
+// (ComputeResidual, VanLeerX_a/b, VanLeerX_PPA_a-d, UpdateX and
+// UpdateDensityX for both passes, AdvectSHIFT, NewVelocity_x), with every
+// neighbour read from a register or from the next lane.  Each quantity goes
+// through a pass on its own after the density's faces (which every update
+// reads), so only one quantity's stencil arrays are live at a time.  VxMed
+// is summed in the wave (wave_chain) from the velocities it holds.
+template <int G, int DZ, int AD, int CPT>
+__device__ __forceinline__ void xsweep_wave(const dmesh& m, int ring, double dt64, const real* __restrict__ rho_in,
+                                            const real* __restrict__ vxt, const real* const* qin,
+                                            const real* __restrict__ e_in, real* __restrict__ e_out,
+                                            real* __restrict__ rho_out, real* __restrict__ vx_out,
+                                            real* const* qout, double* __restrict__ vxmed,
+                                            int* __restrict__ nshift_out, real omegaframe, int fargo) {
+  using W = WRing<CPT>;
+  constexpr int NM = DZ ? 6 : 4;
+  constexpr int NQ = NM + AD;  // transported quantities besides the density
+  const int nx = m.nx;         // == 64 * CPT
+  const real dt = (real)dt64;
+  const int lane = (int)threadIdx.x;
+  const int j = m.nghy + ring % m.ny;
+  const int k = DZ ? m.nghz + ring / m.ny : 0;
+  const int base = j * m.pitch + k * m.stride;
+  const int i0 = lane * CPT;
+  real rho[CPT], st[NQ][CPT];
+  double vt[CPT];
+#pragma unroll
+  for (int s = 0; s < CPT; s++) {
+    rho[s] = rho_in[base + i0 + s];
+#pragma unroll
+    for (int c = 0; c < NQ; c++) st[c][s] = (c < NM) ? qin[c][base + i0 + s] : e_in[base + i0 + s];
+    vt[s] = vxt[base + i0 + s];
+  }
+  // ComputeResidual (src/compute_residual.c:68-73), FP64: the cell residual
+  // velocity for the van Leer pass, the uniform one for the PPA pass
+  real vel[CPT], vu[CPT];
+  int ns = 0;
+  if (fargo) {
+    // ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30): the ring's weights are
+    // in the lanes already, in the layout of wave_ring_sum's chunks
+    double w[CPT];
+#pragma unroll
+    for (int s = 0; s < CPT; s++) w[s] = vt[s] * (m.xmin64[i0 + s + 1] - m.xmin64[i0 + s]);
+    const double vmed = vmed_full(wave_chain<CPT>(w, 0.0, 64) / (m.XMAX - m.XMIN), m, j, k);
+    const double vb = cell::vbase(m, j, k);
+    double nr0 = 0.0;
+#pragma unroll
+    for (int s = 0; s < CPT; s++) {
+      const double zs = zone_size_x64<G>(m, i0 + s, j, k);
+      const double ntilde = vmed * dt64 / zs;
+      const double nround = floor(ntilde + 0.5);
+      if (s == 0) nr0 = nround;
+      if constexpr (cell::kFix64) vel[s] = (real)(vt[s] - (vmed - vb));
+      else vel[s] = (real)(vt[s] - vmed);
+      vu[s] = (real)((ntilde - nround) * zs / dt64);
+    }
+    // the shift of the ring is that of its cell 0 (lane 0)
+    ns = (int)__hiloint2double(__builtin_amdgcn_readlane(__double2hiint(nr0), 0),
+                               __builtin_amdgcn_readlane(__double2loint(nr0), 0));
+    if (lane == 0) {
+      nshift_out[j + k * m.sy] = ns;
+      vxmed[j + k * m.sy] = vmed;
+    }
+  } else {
+    const double vb = cell::vbase(m, j, k);
+#pragma unroll
+    for (int s = 0; s < CPT; s++) {
+      if constexpr (cell::kFix64) vel[s] = (real)(vt[s] + vb);
+      else vel[s] = (real)vt[s];
+    }
+  }
+  // per-cell mesh factors shared by the quantities
+  real zs[CPT], zsl[CPT];
+  Recip Vi[CPT];
+#pragma unroll
+  for (int s = 0; s < CPT; s++) {
+    zs[s] = zone_size_x<G>(m, i0 + s, j, k);
+    Vi[s] = invvol_recip(m, i0 + s, k);
+  }
+  W::left(zs, zsl);
+
+  // X_advection (src/transport.c:208-226), twice with FARGO
+  for (int pass = 0; pass < (fargo ? 2 : 1); pass++) {
+    const bool ppa = (pass == 1);
+    if (ppa) {
+#pragma unroll
+      for (int s = 0; s < CPT; s++) vel[s] = vu[s];
+    }
+    real vr[CPT];
+    W::right(vel, vr);
+    Recip Rr[CPT];
+    real ksi[CPT];
+#pragma unroll
+    for (int s = 0; s < CPT; s++) {
+      Rr[s] = recip(rho[s]);  // DivideByRho of every quantity (the density before this pass)
+      ksi[s] = ppa ? cell::ppa_ksi(vel[s], dt, zs[s]) : real(0.0);
+    }
+    // the face value (van Leer) or swept-zone average (PPA) of q at each cell's
+    // left face, for one quantity
+    auto faces = [&](const real (&q)[CPT], real (&fc)[CPT]) {
+      real ql[CPT], qr[CPT];
+      W::left(q, ql);
+      W::right(q, qr);
+      if (!ppa) {
+        // VanLeerX_a (src/vanleer_x_a.c:60-68), gradient once per face
+        real g[CPT], gr[CPT], sl[CPT], sll[CPT];
+#pragma unroll
+        for (int s = 0; s < CPT; s++) g[s] = rdiv(q[s] - ql[s], recip(zs[s]));
+        W::right(g, gr);
+#pragma unroll
+        for (int s = 0; s < CPT; s++) sl[s] = vanleer_slope(g[s], gr[s]);
+        W::left(sl, sll);
+        // VanLeerX_b (src/vanleer_x_b.c:60-65)
+#pragma unroll
+        for (int s = 0; s < CPT; s++) fc[s] = vanleer_face(vel[s], dt, ql[s], sll[s], zsl[s], q[s], sl[s], zs[s]);
+      } else {
+        // VanLeerX_PPA_a/b/c/d (src/fargo_ppa_a.c:58-69, src/fargo_ppa_b.c:52-55,
+        // src/fargo_ppa_c.c:52-63, src/fargo_ppa_d.c:62-70)
+        real sl[CPT], slr[CPT], fr[CPT], qL[CPT], qR[CPT];
+#pragma unroll
+        for (int s = 0; s < CPT; s++) sl[s] = cell::ppa_slope(ql[s], q[s], qr[s]);
+        W::right(sl, slr);
+#pragma unroll
+        for (int s = 0; s < CPT; s++) fr[s] = cell::ppa_face(q[s], qr[s], sl[s], slr[s]);  // q at face i+1/2
+        W::left(fr, qL);
+#pragma unroll
+        for (int s = 0; s < CPT; s++) {
+          qR[s] = fr[s];
+          cell::ppa_limit(q[s], qL[s], qR[s]);
+        }
+        real qLl[CPT], qRl[CPT];
+        W::left(qL, qLl);
+        W::left(qR, qRl);
+#pragma unroll
+        for (int s = 0; s < CPT; s++)
+          fc[s] = cell::ppa_flux_ksi(vel[s], ksi[s], ql[s], qLl[s], qRl[s], q[s], qL[s], qR[s]);
+      }
+    };
+    // the density's faces first: every quantity's update reads them
+    real fd[CPT], fdr[CPT];
+    faces(rho, fd);
+    W::right(fd, fdr);
+    // UpdateX of every quantity (src/update_x.c:69-71)
+#pragma unroll
+    for (int c = 0; c < NQ; c++) {
+      real q[CPT], fc[CPT], fcr[CPT];
+#pragma unroll
+      for (int s = 0; s < CPT; s++) q[s] = rdiv(st[c][s], Rr[s]);
+      faces(q, fc);
+      W::right(fc, fcr);
+#pragma unroll
+      for (int s = 0; s < CPT; s++)
+        st[c][s] += cell::flux_x(m, j, k, dt, vel[s] * fc[s] * fd[s], vr[s] * fcr[s] * fdr[s], Vi[s]);
+    }
+    // UpdateDensityX last (src/update_density_x.c:66-68)
+#pragma unroll
+    for (int s = 0; s < CPT; s++) rho[s] += cell::flux_x(m, j, k, dt, vel[s] * fd[s], vr[s] * fdr[s], Vi[s]);
+  }
+  // AdvectSHIFT (src/advect_shift.c:45-58): cell i moves to i + ns (mod nx);
+  // NewVelocity_x (src/newvel_x.c:78-86) at the destination, from the moved
+  // cell and its left neighbour
+  real rl[CPT], mpl[CPT];
+  W::left(rho, rl);
+  W::left(st[0], mpl);
+#pragma unroll
+  for (int s = 0; s < CPT; s++) {
+    int it = (i0 + s + ns) % nx;
+    if (it < 0) it += nx;
+#pragma unroll
+    for (int c = 2; c < NQ; c++) {
+      if (c < NM) qout[c - 2][base + it] = st[c][s];
+      else e_out[base + it] = st[c][s];
+    }
+    rho_out[base + it] = rho[s];
+    vx_out[base + it] = cell::newvel_x<G>(m, it, j, k, rho[s], rl[s], st[1][s], mpl[s], omegaframe);
+  }
+}
+
 // LDS bytes of xsweep_ring for n carried quantities + the density
 inline size_t xsweep_ring_lds(int nx, int ncarried) {
   const size_t P = ((size_t)(ncarried + 1) * nx + 1) & ~(size_t)1;
@@ -2074,7 +2300,8 @@ static int g_rows_y = 4, g_rows_z = 6;
 // the quantities), 0 one, -1 automatic: measured on MI355X, the split only
 // pays when a sweep has fewer waves than the GPU has SIMDs (small 2-D grids)
 static int g_colsplit = -1;
-// X-sweep kernel choice: 0 automatic, 1 ring in LDS, 2 tiles
+// X-sweep kernel choice: 0 automatic, 1 ring in LDS, 2 tiles, 3 one wave per
+// ring with the cells in registers (rings of 256 cells)
 static int g_xmode = 0;
 // ring X sweep: 1 two workgroups per ring (half of the quantities each), 0
 // one; -1 automatic
@@ -2400,6 +2627,17 @@ __global__ void __launch_bounds__(CPT == 2 ? 512 : 256, (CPT == 1 && VPRE && !AD
   }
 }
 
+// one wave per ring and fluid; blocks [f*nrings, (f+1)*nrings) take fluid f
+template <int G, int DZ, int AD, int CPT>
+__global__ void __launch_bounds__(64, 2) kf_sweep_xw(dmesh m, double dt64, XTab T, int nrings, real omegaframe,
+                                                  int fargo) {
+  const int lin = xcd_linear();
+  const int fl = lin / nrings, ring = lin % nrings;
+  real* qout[4] = {T.qo[fl][0], T.qo[fl][1], T.qo[fl][2], T.qo[fl][3]};
+  xsweep_wave<G, DZ, AD, CPT>(m, ring, dt64, T.rho_in[fl], T.vxt[fl], T.q[fl], T.e_in[fl], T.e_out[fl],
+                              T.rho_out[fl], T.vx_out[fl], qout, T.vxmed[fl], T.nshift[fl], omegaframe, fargo);
+}
+
 // ring X sweep form: the split (two workgroups per ring) and its LDS bytes
 static void x_ring_form(const f3d_hip_ctx* ctx, const dmesh& m, int* split, size_t* shx) {
   const int ad = (ctx->ph.eos == 2);
@@ -2417,7 +2655,57 @@ static bool x_ring_fits(const f3d_hip_ctx* ctx, Scratch* S) {
   size_t shx;
   x_ring_form(ctx, S->m, &split, &shx);
   const bool fits = shx <= 160 * 1024 && S->m.nx <= 1024;
-  return g_xmode == 1 || (g_xmode == 0 && fits);
+  return g_xmode == 1 || ((g_xmode == 0 || g_xmode == 3) && fits);
+}
+
+// the ring X sweeps' table of every fluid's arrays
+static void xtab_fill(XTab* T, int nf, const RF* f, const TSet* t) {
+  memset(T, 0, sizeof(*T));
+  for (int k = 0; k < nf; k++) {
+    T->rho_in[k] = t[k].rhoy; T->vxt[k] = f[k].Vx_temp;
+    for (int q = 0; q < 6; q++) T->q[k][q] = t[k].m[q];
+    T->e_in[k] = t[k].ey; T->e_out[k] = f[k].Energy; T->rho_out[k] = t[k].rhox; T->vx_out[k] = f[k].Vx;
+    for (int q = 0; q < 4; q++) T->qo[k][q] = t[k].xm[q];
+    T->vxmed[k] = t[k].vxmed; T->nshift[k] = t[k].nshift;
+  }
+}
+
+// The one-wave ring X sweep (kf_sweep_xw): rings of 256 cells (four per
+// lane; longer rings need more registers than two waves per SIMD leave),
+// chosen automatically from 2048 rings x fluids on (enough waves to fill the
+// GPU), or asked for with f3d_fused_set_xmode(3).
+static bool x_wave_form(const dmesh& m, int nf) {
+  if (m.nx != 256) return false;
+  if (g_xmode == 3) return true;
+  if (g_xmode != 0) return false;
+  return (long)m.ny * (m.dimz ? m.nz : 1) * nf >= 2048;
+}
+
+static int x_sweep_wave(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const RF* f, const TSet* t) {
+  const dmesh& m = S->m;
+  const real of = (real)ctx->ph.OMEGAFRAME;
+  const int ad = (ctx->ph.eos == 2);
+  F3D_REQUIRE(ctx, m.nx == 256, "one-wave X sweep: rings of 256 cells");
+  F3D_REQUIRE(ctx, nf >= 1 && nf <= 8, "X sweep: 1 to 8 fluids");
+  const int nrings = m.ny * (m.dimz ? m.nz : 1);
+  XTab T;
+  xtab_fill(&T, nf, f, t);
+  f3d_stage_begin(ctx, F3D_ST_SWEEP_X);
+#define XW0(G_, DZ_, AD_, CPT_)                                                                             \
+  hipLaunchKernelGGL((kf_sweep_xw<G_, DZ_, AD_, CPT_>), dim3(nrings * nf), dim3(64), 0, ctx->stream, m, dt, T, \
+                     nrings, of, ctx->ph.fargo)
+#define XW(G_, DZ_) \
+  do { if (ad) XW0(G_, DZ_, 1, 4); else XW0(G_, DZ_, 0, 4); } while (0)
+  if (m.geom == SPH) {
+    if (m.dimz) XW(SPH, 1); else XW(SPH, 0);
+  } else {
+    if (m.dimz) XW(CYL, 1); else XW(CYL, 0);
+  }
+#undef XW
+#undef XW0
+  F3D_LAUNCH_CHECK(ctx);
+  f3d_stage_end(ctx, F3D_ST_SWEEP_X);
+  return 0;
 }
 
 // Ring X sweep of nf fluids in one launch.
@@ -2435,14 +2723,7 @@ static int x_sweep_ring(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const R
   const int nthreads = (((m.nx + cpt - 1) / cpt + 63) / 64) * 64;
   const int per = m.ny * (m.dimz ? m.nz : 1) * (split ? 2 : 1);
   XTab T;
-  memset(&T, 0, sizeof(T));
-  for (int k = 0; k < nf; k++) {
-    T.rho_in[k] = t[k].rhoy; T.vxt[k] = f[k].Vx_temp;
-    for (int q = 0; q < 6; q++) T.q[k][q] = t[k].m[q];
-    T.e_in[k] = t[k].ey; T.e_out[k] = f[k].Energy; T.rho_out[k] = t[k].rhox; T.vx_out[k] = f[k].Vx;
-    for (int q = 0; q < 4; q++) T.qo[k][q] = t[k].xm[q];
-    T.vxmed[k] = t[k].vxmed; T.nshift[k] = t[k].nshift;
-  }
+  xtab_fill(&T, nf, f, t);
   dim3 gx(per * nf);
   f3d_stage_begin(ctx, F3D_ST_SWEEP_X);
 #define XSWEEP0(G_, DZ_, AD_, CPT_, SP_, VP_)                                                                  \
@@ -2486,8 +2767,10 @@ static int transport_fluids(f3d_hip_ctx* ctx, double dt, int nf, const f3d_field
     f[k] = fl ? rfields_of(fl[k], ctx->f) : rfields(ctx);
     if (tset_of(ctx, S, f[k], k, nf, &t[k])) return -1;
   }
-  const bool tiles = !x_ring_fits(ctx, S);
-  const int vpre = ctx->ph.fargo ? x_vmed_form(S->m, nf, tiles) : 0;
+  const bool wave = x_wave_form(S->m, nf);
+  const bool tiles = !wave && !x_ring_fits(ctx, S);
+  // the one-wave sweep sums VxMed itself; the tiled one takes it from the pass
+  const int vpre = (ctx->ph.fargo && !wave) ? x_vmed_form(S->m, nf, tiles) : 0;
   if (vpre == 1) {
     // VxMed of each fluid's Vx_temp (one long sequential sum per ring) on the
     // side stream, overlapping the Z/Y sweeps, which do not write Vx_temp
@@ -2500,7 +2783,9 @@ static int transport_fluids(f3d_hip_ctx* ctx, double dt, int nf, const f3d_field
   if (yz_sweeps(ctx, S, dt, nf, f, t)) return -1;
   if (vpre == 1) F3D_CHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->side_done, 0));
   if (vpre == 2 && x_vmed(ctx, S, nf, f, t, ctx->stream)) return -1;
-  if (!tiles) {
+  if (wave) {
+    if (x_sweep_wave(ctx, S, dt, nf, f, t)) return -1;
+  } else if (!tiles) {
     if (x_sweep_ring(ctx, S, dt, nf, f, t, vpre != 0)) return -1;
   } else {
     for (int k = 0; k < nf; k++)
@@ -2543,7 +2828,7 @@ int F3D_API(fused_set_xtile)(int cells) {
 }
 
 int F3D_API(fused_set_xmode)(int mode) {
-  if (mode < 0 || mode > 2) return -2;
+  if (mode < 0 || mode > 3) return -2;
   g_xmode = mode;
   return 0;
 }

@@ -331,6 +331,21 @@ def test_whole_step_fused_tiled_x(fa, orc, par, ov, nsteps):
         L.f3d_fused_set_xmode(0)
 
 
+@pytest.mark.parametrize("par,ov,nsteps", [(C1, "Nx=256 Ny=64", 8), (C2, "Nx=256 Ny=24 Nz=12", 6),
+                                           (C3, C3_SMALL, 10),
+                                           (C2, "Nx=256 Ny=24 Nz=12" + C2_AD[len(C2_SMALL):], 4)],
+                         ids=["c1_2d", "c2_3d", "c3_2d_adiabatic", "c2_3d_adiabatic"])
+def test_whole_step_fused_one_wave_x(fa, orc, par, ov, nsteps):
+    """The one-wave ring X sweep (cells in registers, neighbours across lanes
+    by wave rotations, rings of 256 cells) gives the oracle's results."""
+    L = fa.lib()
+    assert L.f3d_fused_set_xmode(3) == 0
+    try:
+        _fused_run(fa, orc, par, ov, nsteps)
+    finally:
+        L.f3d_fused_set_xmode(0)
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3, 6, 7], ids=["atomic", "two_pass", "vmed_pass_atomic",
                                                      "vmed_pass_two_pass", "vmed_pass_atomic_one_wave",
                                                      "vmed_pass_two_pass_one_wave"])
