@@ -526,6 +526,23 @@ __device__ __forceinline__ real flux_y(const dmesh& m, int i, int j, int k, real
 __device__ __forceinline__ real flux_z(const dmesh& m, int i, int j, int k, real dt, real a, real b) {
   return mul_invvol(m, (mul_surfz(m, a, i, j, k) - mul_surfz(m, b, i, j, k + 1)) * dt, i, j, k);
 }
+// The mesh factors of one cell's Y or Z flux difference, loaded once for
+// all its quantities: (a SurfA - b SurfB) dt InvVol with Surf = (. * fj) * fk
+// * Sxi[i] on either face, in the association of mul_surfy/mul_surfz and
+// mul_invvol.
+struct FluxMesh {
+  real aj, ak, bj, bk, si, iv;
+};
+__device__ __forceinline__ FluxMesh flux_y_mesh(const dmesh& m, int i, int j, int k) {
+  return FluxMesh{m.syj[j], m.syk[k], m.syj[j + 1], m.syk[k], m.sxi[i], m.invvj[j]};
+}
+__device__ __forceinline__ FluxMesh flux_z_mesh(const dmesh& m, int i, int j, int k) {
+  return FluxMesh{m.szj[j], m.szk[k], m.szj[j], m.szk[k + 1], m.sxi[i], m.invvj[j]};
+}
+// flux_y / flux_z below from those factors, with the InvVol divisor V
+__device__ __forceinline__ real flux_yz(const FluxMesh& F, real dt, real a, real b, const Recip& V) {
+  return rdiv(((((a * F.aj) * F.ak) * F.si - ((b * F.bj) * F.bk) * F.si) * dt) * F.iv, V);
+}
 // the same with the cell's InvVol divisor shared by its quantities
 __device__ __forceinline__ real flux_y(const dmesh& m, int i, int j, int k, real dt, real a, real b, const Recip& V) {
   return mul_invvol(m, (mul_surfy(m, a, i, j, k) - mul_surfy(m, b, i, j + 1, k)) * dt, j, V);

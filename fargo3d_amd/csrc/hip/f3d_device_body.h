@@ -139,10 +139,15 @@ __device__ __forceinline__ double zone_size_z64(const dmesh& m, int j, int k) {
   return m.zmin64[k + 1] - m.zmin64[k];
 }
 
-// van Leer harmonic slope (src/vanleer_y_a.c:61-68): 0 at extrema.
+// van Leer harmonic slope (src/vanleer_y_a.c:61-68): 0 at extrema.  The
+// quotient is formed in every lane and the extremum case selected after it
+// (the same value in both cases): a branch around the division would split
+// the sweeps' loops into blocks that the next row's loads cannot be moved
+// across.
 __device__ __forceinline__ real vanleer_slope(real dqm, real dqp) {
-  if (dqp * dqm <= real(0.0)) return real(0.0);
-  return real(2.) * dqp * dqm / (dqm + dqp);
+  const real p = dqp * dqm;
+  const real s = real(2.) * dqp * dqm / (dqm + dqp);
+  return (p <= real(0.0)) ? real(0.0) : s;
 }
 // upwind interface value (src/vanleer_y_b.c:60-65)
 __device__ __forceinline__ real vanleer_face(real v, real dt, real qm, real sm, real dzm,

@@ -654,7 +654,8 @@ __global__ void __launch_bounds__(WG) kf_substep3(dmesh m, real dt, const real* 
 //   col.zs(r)       zone size of row r in the sweep direction
 //   col.v(r)        velocity at the lower face of row r
 //   col.vol(r)      the divisor of row r's InvVol (rdiv(), shared by its quantities)
-//   col.flux(r, a, b, V) conservative increment of row r from face fluxes a, b
+//   col.fmesh(r)    the mesh factors of row r's flux difference (cell::FluxMesh)
+//   col.flux(F, a, b, V) conservative increment of a row from face fluxes a, b
 //   col.out(c, r, value) store of the updated quantity (c == NQ: density)
 // In the precision form the first Col::NXQ quantities (the azimuthal momenta)
 // are deviations Q' = Q - rho l0(r) from the profile's angular momentum
@@ -672,6 +673,10 @@ __global__ void __launch_bounds__(WG) kf_substep3(dmesh m, real dt, const real* 
 // reference's.
 // x + d for a deviation quantity; x itself (no addition) otherwise
 __device__ __forceinline__ real dsh(real x, real d, bool dev) { return dev ? x + d : x; }
+// Holds a value in its registers at this point of the program (an empty asm
+// statement that reads it), so that the load producing it stays above.
+__device__ __forceinline__ void pin(float x) { asm volatile("" ::"v"(x)); }
+__device__ __forceinline__ void pin(double x) { asm volatile("" ::"v"(x)); }
 
 template <int NQ, class Col>
 __device__ __forceinline__ void column_sweep(Col& col, int r0, int r1, real dt) {
@@ -725,28 +730,50 @@ __device__ __forceinline__ void column_sweep(Col& col, int r0, int r1, real dt) 
     }
   }
   for (int r = r0; r < r1; r++) {
-    const real v0 = col.v(r), v1 = col.v(r + 1);
+    // Every load of the iteration before its first store: the quantities of
+    // row r (its update) and of row r+3 (entering the window).  The vector
+    // memory counter retires loads and stores in issue order, so a load
+    // issued after a store would wait for that store as well.
+    // On the last row no row enters, and row r is loaded in its place (no
+    // branch around the loads; the values are not used).
+    const bool more = r + 1 < r1;
+    const int rn = r + 3, rl = more ? rn : r;
+    real Qr[NQ], Qn[NQ];
+#pragma unroll
+    for (int c = 0; c < NQ; c++) Qr[c] = col.Q(c, r);
+    const real rhor = col.rho(r);
+    const real rr = col.rho(rl);
+#pragma unroll
+    for (int c = 0; c < NQ; c++) Qn[c] = col.Q(c, rl);
+    const real zsn = col.zs(rl);
+    const real v0 = col.v(r), v1 = col.v(r + 1), v2 = col.v(r + 2);
+    const real zs1 = col.zs(r + 1), zs2 = col.zs(r + 2);
     const Recip V = col.vol(r);
+    const cell::FluxMesh FM = col.fmesh(r);
+    // the entering row's values are used only after the stores; held here, so
+    // that their loads are not moved below them
+#pragma unroll
+    for (int c = 0; c < NQ; c++) pin(Qn[c]);
+    pin(rr);
+    pin(zsn);
+    pin(v2);
     // UpdateY/Z (src/update_y.c:68-70), then the density last (src/update_density_y.c:66-68)
 #pragma unroll
     for (int c = 0; c < NQ; c++) {
       const real f0 = (c < NXQ) ? F0[c] - D0 : F0[c];
-      col.out(c, r, col.Q(c, r) + col.flux(r, v0 * f0 * F0[NQ], v1 * F1[c] * F1[NQ], V));
+      col.out(c, r, Qr[c] + col.flux(FM, v0 * f0 * F0[NQ], v1 * F1[c] * F1[NQ], V));
     }
-    col.out(NQ, r, col.rho(r) + col.flux(r, v0 * F0[NQ], v1 * F1[NQ], V));
-    if (r + 1 >= r1) break;
+    col.out(NQ, r, rhor + col.flux(FM, v0 * F0[NQ], v1 * F1[NQ], V));
+    if (!more) break;
     // advance the window by one row: row r+3 enters
-    const int rn = r + 3;
-    const real rr = col.rho(rn);
     const Recip R = recip(rr);
-    const real zsn = col.zs(rn), zs2 = col.zs(r + 2), zs1 = col.zs(r + 1), v2 = col.v(r + 2);
     const Recip Zn = recip(zsn);
     real D3 = real(0.0);
     if constexpr (NXQ > 0) D3 = db(rn);
 #pragma unroll
     for (int c = 0; c < N; c++) {
       const bool dev = c < NXQ;
-      const real an = (c < NQ) ? rdiv(col.Q(c, rn), R) : rr;
+      const real an = (c < NQ) ? rdiv(Qn[c], R) : rr;
       const real g3 = rdiv(dsh(an - A2[c], D3, dev), Zn);
       const real s2 = vanleer_slope(G2[c], g3);
       const real f2 = vanleer_face(v2, dt, A1[c], S1[c], zs1, dsh(A2[c], D2, dev), s2, zs2);
@@ -780,7 +807,10 @@ struct ColPart {
   __device__ __forceinline__ real zs(int r) const { return c.zs(r); }
   __device__ __forceinline__ real v(int r) const { return c.v(r); }
   __device__ __forceinline__ Recip vol(int r) const { return c.vol(r); }
-  __device__ __forceinline__ real flux(int r, real a, real b, const Recip& V) const { return c.flux(r, a, b, V); }
+  __device__ __forceinline__ cell::FluxMesh fmesh(int r) const { return c.fmesh(r); }
+  __device__ __forceinline__ real flux(const cell::FluxMesh& F, real a, real b, const Recip& V) const {
+    return c.flux(F, a, b, V);
+  }
   __device__ __forceinline__ void out(int q, int r, real val) const {
     if (q < COUNT) c.out(FIRST + q, r, val);
     else if (WRITE_RHO) c.out(NQ, r, val);
@@ -830,8 +860,9 @@ struct ColY2 {
   __device__ __forceinline__ real zs(int r) const { return zone_size_y(*m, r); }
   __device__ __forceinline__ real v(int r) const { return vyt[L(r)]; }
   __device__ __forceinline__ Recip vol(int r) const { return invvol_recip(*m, i, 0); }
-  __device__ __forceinline__ real flux(int r, real a, real b, const Recip& V) const {
-    return cell::flux_y(*m, i, r, 0, dt, a, b, V);
+  __device__ __forceinline__ cell::FluxMesh fmesh(int r) const { return cell::flux_y_mesh(*m, i, r, 0); }
+  __device__ __forceinline__ real flux(const cell::FluxMesh& F, real a, real b, const Recip& V) const {
+    return cell::flux_yz(F, dt, a, b, V);
   }
   __device__ __forceinline__ void out(int c, int r, real val) const { out_[c][L(r)] = val; }
 };
@@ -881,8 +912,9 @@ struct ColZ {
   __device__ __forceinline__ real zs(int r) const { return zone_size_z<G>(*m, j, r); }
   __device__ __forceinline__ real v(int r) const { return vzt[L(r)]; }
   __device__ __forceinline__ Recip vol(int r) const { return invvol_recip(*m, i, r); }
-  __device__ __forceinline__ real flux(int r, real a, real b, const Recip& V) const {
-    return cell::flux_z(*m, i, j, r, dt, a, b, V);
+  __device__ __forceinline__ cell::FluxMesh fmesh(int r) const { return cell::flux_z_mesh(*m, i, j, r); }
+  __device__ __forceinline__ real flux(const cell::FluxMesh& F, real a, real b, const Recip& V) const {
+    return cell::flux_yz(F, dt, a, b, V);
   }
   __device__ __forceinline__ void out(int c, int r, real val) const { out_[c][L(r)] = val; }
 };
@@ -905,8 +937,9 @@ struct ColY3 {
   __device__ __forceinline__ real zs(int r) const { return zone_size_y(*m, r); }
   __device__ __forceinline__ real v(int r) const { return vyt[L(r)]; }
   __device__ __forceinline__ Recip vol(int r) const { return invvol_recip(*m, i, k); }
-  __device__ __forceinline__ real flux(int r, real a, real b, const Recip& V) const {
-    return cell::flux_y(*m, i, r, k, dt, a, b, V);
+  __device__ __forceinline__ cell::FluxMesh fmesh(int r) const { return cell::flux_y_mesh(*m, i, r, k); }
+  __device__ __forceinline__ real flux(const cell::FluxMesh& F, real a, real b, const Recip& V) const {
+    return cell::flux_yz(F, dt, a, b, V);
   }
   __device__ __forceinline__ void out(int c, int r, real val) const { out_[c][L(r)] = val; }
 };
@@ -936,7 +969,7 @@ struct Y2Tab {
   real* e_out[8];
 };
 template <int G, int DZ, int AD, int SPLIT>
-__global__ void __launch_bounds__(BXF, AD ? 1 : 4) kf_sweep_y2(dmesh m, real dt, Y2Tab T, int rows_per, real of) {
+__global__ void __launch_bounds__(BXF) kf_sweep_y2(dmesh m, real dt, Y2Tab T, int rows_per, real of) {
   const dim3 b = xcd_bid();
   const int fl = (int)b.z;
   kd_sweep_y2<G, DZ, AD, SPLIT>(b, m, dt, T.rho_in[fl], T.vxt[fl], T.vyt[fl], T.e_in[fl], T.rho_out[fl], T.q[fl][0],
