@@ -640,11 +640,13 @@ struct DampPar {
   int periodic_z;
 };
 
-// src/stockholm.c:129-184
-template <int G, int DZ>
-__device__ __forceinline__ void stockholm(const dmesh& m, int j, int k, int l, real dt, real* rho, real* vx,
-                                          real* vy, real* vz, const real* rho0, const real* vx0,
-                                          const real* vy0, const real* vz0, const DampPar& d) {
+// src/stockholm.c:129-184: the ramps of ring (j,k) (ramp for rho, vx, vy;
+// rampi for vz) and the damping time, then the relaxation of the values
+struct Ramps {
+  real ramp, rampi, tau;
+};
+template <int DZ>
+__device__ __forceinline__ Ramps stockholm_ramps(const dmesh& m, int j, int k, const DampPar& d) {
   const real ym = m.ymed[j];
   real rampy = real(0.0), rampz = real(0.0), rampzz = real(0.0);
   if (ym > d.Y_sup) rampy = (ym - d.Y_sup) / (d.y_max - d.Y_sup);
@@ -663,25 +665,42 @@ __device__ __forceinline__ void stockholm(const dmesh& m, int j, int k, int l, r
     rampz = real(0.0);
     rampzz = real(0.0);
   }
-  real ramp = rampy + rampz;
-  real rampi = rampy + rampzz;
-  real tau = d.ds * sqrt(ym * ym * ym / real(1.0) / real(1.0));
+  return Ramps{rampy + rampz, rampy + rampzz, d.ds * sqrt(ym * ym * ym / real(1.0) / real(1.0))};
+}
+// rho, vx, vy of a cell where R.ramp > 0
+template <int G>
+__device__ __forceinline__ void stockholm_rxy(const dmesh& m, int j, int k, real dt, const Ramps& R, real& rho,
+                                              real& vx, real& vy, real rho0, real vx0, real vy0, const DampPar& d) {
+  real taud = R.tau / R.ramp;
+  rho = (rho * taud + rho0 * dt) / (dt + taud);
+  real vx0_target = vx0;
+  real radius = m.ymed[j];
+  if (G == SPH) radius *= m.sinzmed[k];
+  vx0_target -= (real)((d.of - d.of0) * radius);
+  vx = (vx * taud + vx0_target * dt) / (dt + taud);
+  vy = (vy * taud + vy0 * dt) / (dt + taud);
+}
+// vz of a cell where R.rampi > 0
+__device__ __forceinline__ real stockholm_z(real dt, const Ramps& R, real vz, real vz0) {
+  real taud = R.tau / R.rampi;
+  return (vz * taud + vz0 * dt) / (dt + taud);
+}
+// the same on the arrays (cell l of ring (j,k))
+template <int G, int DZ>
+__device__ __forceinline__ void stockholm(const dmesh& m, int j, int k, int l, real dt, real* rho, real* vx,
+                                          real* vy, real* vz, const real* rho0, const real* vx0,
+                                          const real* vy0, const real* vz0, const DampPar& d) {
+  const Ramps R = stockholm_ramps<DZ>(m, j, k, d);
   const int l2 = j + k * m.sy;
-  if (ramp > real(0.0)) {
-    real taud = tau / ramp;
-    rho[l] = (rho[l] * taud + rho0[l2] * dt) / (dt + taud);
-    real vx0_target = vx0[l2];
-    real radius = ym;
-    if (G == SPH) radius *= m.sinzmed[k];
-    vx0_target -= (real)((d.of - d.of0) * radius);
-    vx[l] = (vx[l] * taud + vx0_target * dt) / (dt + taud);
-    vy[l] = (vy[l] * taud + vy0[l2] * dt) / (dt + taud);
+  if (R.ramp > real(0.0)) {
+    real r = rho[l], x = vx[l], y = vy[l];
+    stockholm_rxy<G>(m, j, k, dt, R, r, x, y, rho0[l2], vx0[l2], vy0[l2], d);
+    rho[l] = r;
+    vx[l] = x;
+    vy[l] = y;
   }
   if (DZ) {
-    if (rampi > real(0.0)) {
-      real taud = tau / rampi;
-      vz[l] = (vz[l] * taud + vz0[l2] * dt) / (dt + taud);
-    }
+    if (R.rampi > real(0.0)) vz[l] = stockholm_z(dt, R, vz[l], vz0[l2]);
   }
 }
 

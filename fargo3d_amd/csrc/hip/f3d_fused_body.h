@@ -520,9 +520,16 @@ struct ArtQ {
 };
 
 // kf_src_a: stresses (visctensor ranges j,k in [1, s-2]) at every cell the
-// active update reads (launched only with viscosity).
-template <int G, int DZ>
+// active update reads (launched only with viscosity).  VM fixes the
+// viscosity model at compile time (1 constant nu, 2 alpha isothermal, 3 alpha
+// adiabatic; 0: from vp), so that no branch separates the stencil's loads.
+template <int G, int DZ, int VM = 0>
 __device__ __forceinline__ void kd_src_a(const dim3 bid, const dmesh& m, const real* rho, const real* energy, const real* vx, const real* vy, const real* vz, real* txx, real* tyy, real* tzz, real* txz, real* tyx, real* tzy, cell::ViscPar vp, int visc, real domega) {
+  if constexpr (VM != 0) {
+    visc = 1;
+    vp.model = VM == 1 ? 1 : 2;
+    vp.iso = VM != 3;
+  }
   const int i = bid.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int j = 1 + (int)bid.y;       // 1 .. sy-2
@@ -544,18 +551,27 @@ __device__ __forceinline__ void kd_src_a(const dim3 bid, const dmesh& m, const r
     }
   }
 }
-template <int G, int DZ>
+template <int G, int DZ, int VM>
 __global__ void __launch_bounds__(WG) kf_src_a(dmesh m, const real* __restrict__ rho, const real* __restrict__ energy, const real* __restrict__ vx,
                                                const real* __restrict__ vy, const real* __restrict__ vz, real* __restrict__ txx, real* __restrict__ tyy,
                                                real* __restrict__ tzz, real* __restrict__ txz, real* __restrict__ tyx, real* __restrict__ tzy,
                                                cell::ViscPar vp, int visc, real domega) {
-  kd_src_a<G, DZ>(xcd_bid(), m, rho, energy, vx, vy, vz, txx, tyy, tzz, txz, tyx, tzy, vp, visc, domega);
+  kd_src_a<G, DZ, VM>(xcd_bid(), m, rho, energy, vx, vy, vz, txx, tyy, tzz, txz, tyx, tzy, vp, visc, domega);
 }
 
 // kf_src_b: the velocity update of SubStep1 + addviscosity + SubStep2_b on
-// active cells, in the order the reference applies them.
-template <int G, int DZ>
+// active cells, in the order the reference applies them.  MODE fixes the
+// flags of a viscous gas with a potential at compile time (1 isothermal, 2
+// adiabatic; 0: the flags as passed), so that no branch separates the
+// stencil's loads.
+template <int G, int DZ, int MODE = 0>
 __device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt, const real* __restrict__ rho, const real* __restrict__ energy, const real* __restrict__ vx, const real* __restrict__ vy, const real* __restrict__ vz, const double* __restrict__ pot, const real* __restrict__ txx, const real* __restrict__ tyy, const real* __restrict__ tzz, const real* __restrict__ txz, const real* __restrict__ tyx, const real* __restrict__ tzy, real* __restrict__ vxt, real* __restrict__ vyt, real* __restrict__ vzt, real* __restrict__ e2, int visc, int potential, int dust, int iso, real gm1, double omegaframe, real domega) {
+  if constexpr (MODE != 0) {
+    visc = 1;
+    potential = 1;
+    dust = 0;
+    iso = MODE == 1;
+  }
   const int i = bid.x * blockDim.x + threadIdx.x;
   if (i >= m.nx) return;
   const int j = m.nghy + (int)bid.y;
@@ -587,7 +603,7 @@ __device__ __forceinline__ void kd_src_b(const dim3 bid, const dmesh& m, real dt
   if (DZ) vzt[l] = az;
   if (!iso) e2[l] = e;
 }
-template <int G, int DZ>
+template <int G, int DZ, int MODE>
 __global__ void __launch_bounds__(WG, DZ ? 5 : 8) kf_src_b(dmesh m, real dt, const real* __restrict__ rho, const real* __restrict__ energy,
                                                const real* __restrict__ vx, const real* __restrict__ vy, const real* __restrict__ vz,
                                                const double* __restrict__ pot, const real* __restrict__ txx, const real* __restrict__ tyy,
@@ -595,7 +611,7 @@ __global__ void __launch_bounds__(WG, DZ ? 5 : 8) kf_src_b(dmesh m, real dt, con
                                                const real* __restrict__ tzy, real* __restrict__ vxt, real* __restrict__ vyt, real* __restrict__ vzt, real* __restrict__ e2,
                                                int visc, int potential, int dust, int iso, real gm1, double omegaframe,
                                                real domega) {
-  kd_src_b<G, DZ>(xcd_bid(), m, dt, rho, energy, vx, vy, vz, pot, txx, tyy, tzz, txz, tyx, tzy, vxt, vyt, vzt, e2, visc, potential, dust, iso, gm1, omegaframe, domega);
+  kd_src_b<G, DZ, MODE>(xcd_bid(), m, dt, rho, energy, vx, vy, vz, pot, txx, tyy, tzz, txz, tyx, tzy, vxt, vyt, vzt, e2, visc, potential, dust, iso, gm1, omegaframe, domega);
 }
 
 // kf_src_b of every fluid in one launch (isothermal runs): blockIdx.z =
@@ -1827,18 +1843,38 @@ __device__ __forceinline__ void kd_finish(const dim3 bid, const dmesh& m, real d
   const int j = m.nghy + (int)bid.y;
   const int k = DZ ? m.nghz + (int)bid.z : 0;
   const int l = i + j * m.pitch + k * m.stride;
-  const real r = rho_x[l];
+  real r = rho_x[l];
   // the faces j = nghy and k = nghz lie on the mesh boundary; the ghost fill
-  // sets them to zero, so they are not computed here
-  if (j > m.nghy) vy[l] = cell::newvel_y(m, i, j, k, r, rho_x[l - m.pitch], mmy[l], mpy[l - m.pitch]);
+  // sets them to zero, so they are not computed here (the damping, which runs
+  // on the values in registers, reads the stored ones there)
+  real y = (j > m.nghy) ? cell::newvel_y(m, i, j, k, r, rho_x[l - m.pitch], mmy[l], mpy[l - m.pitch]) : real(0.0);
+  real z = real(0.0);
   if (DZ && k > m.nghz) {
-    real v = (mul_vol(m, mmz[l], i, j, k) + mul_vol(m, mpz[l - m.stride], i, j, k - 1)) /
-               (mul_vol(m, r, i, j, k) + mul_vol(m, rho_x[l - m.stride], i, j, k - 1));
-    if (G == SPH) v /= m.ymed[j];
-    vz[l] = v;
+    z = (mul_vol(m, mmz[l], i, j, k) + mul_vol(m, mpz[l - m.stride], i, j, k - 1)) /
+        (mul_vol(m, r, i, j, k) + mul_vol(m, rho_x[l - m.stride], i, j, k - 1));
+    if (G == SPH) z /= m.ymed[j];
   }
+  if (stockholm) {
+    // StockholmBoundary (src/stockholm.c:129-184) after NewVelocity_y/z; the
+    // ramps depend on the ring only, so the branches are uniform
+    const cell::Ramps R = cell::stockholm_ramps<DZ>(m, j, k, dp);
+    const int l2 = j + k * m.sy;
+    if (R.ramp > real(0.0)) {
+      if (j == m.nghy) y = vy[l];
+      real x = vx[l];
+      cell::stockholm_rxy<G>(m, j, k, dt, R, r, x, y, rho0[l2], vx0[l2], vy0[l2], dp);
+      vx[l] = x;
+      if (j == m.nghy) vy[l] = y;
+    }
+    if (DZ && R.rampi > real(0.0)) {
+      if (k == m.nghz) z = vz[l];
+      z = cell::stockholm_z(dt, R, z, vz0[l2]);
+      if (k == m.nghz) vz[l] = z;
+    }
+  }
+  if (j > m.nghy) vy[l] = y;
+  if (DZ && k > m.nghz) vz[l] = z;
   rho[l] = r;
-  if (stockholm) cell::stockholm<G, DZ>(m, j, k, l, dt, rho, vx, vy, vz, rho0, vx0, vy0, vz0, dp);
 }
 // every fluid in one launch: blockIdx.z = fluid * (planes) + plane
 struct FinTab {
@@ -3139,8 +3175,11 @@ int F3D_API(fused_sources)(f3d_hip_ctx* ctx, double dt, double domega) {
   dim3 ga((m.nx + BXF - 1) / BXF, m.sy - 2, m.dimz ? m.sz - 2 : 1);
   if (visc) {
     f3d_stage_begin(ctx, F3D_ST_SRC_A);
-    FDISPATCH(ctx, kf_src_a, ga, dim3(BXF), 0, f.Density, f.Energy, f.Vx, f.Vy, f.Vz, f.Mmx, f.Mmy, f.Mmz, f.Mpx,
-              f.Mpy, f.Mpz, vp, visc, rdom);
+    const int vm = vp.model == 1 ? 1 : vp.model == 2 ? (iso ? 2 : 3) : 0;
+#define SRCA(VM_) FDISPATCH_T(ctx, kf_src_a, VM_, ga, dim3(BXF), 0, f.Density, f.Energy, f.Vx, f.Vy, f.Vz, f.Mmx, \
+                              f.Mmy, f.Mmz, f.Mpx, f.Mpy, f.Mpz, vp, visc, rdom)
+    if (vm == 1) SRCA(1); else if (vm == 2) SRCA(2); else if (vm == 3) SRCA(3); else SRCA(0);
+#undef SRCA
     f3d_stage_end(ctx, F3D_ST_SRC_A);
   }
   // adiabatic: SubStep3 reads V_temp one row (and plane) past the active
@@ -3149,10 +3188,16 @@ int F3D_API(fused_sources)(f3d_hip_ctx* ctx, double dt, double domega) {
   const int ext = iso ? 0 : 1;
   dim3 gb((m.nx + BXF - 1) / BXF, m.ny + ext, m.dimz ? m.nz + ext : 1);
   f3d_stage_begin(ctx, F3D_ST_SRC_B);
-  FDISPATCH(ctx, kf_src_b, gb, dim3(BXF), 0, rdt, f.Density, f.Energy, f.Vx, f.Vy, f.Vz, ctx->f.Pot, f.Mmx, f.Mmy, f.Mmz,
-            f.Mpx, f.Mpy, f.Mpz, f.Vx_temp, f.Vy_temp, f.Vz_temp,
-            S->f[FS_E2], visc, pot, (int)(ctx->ph.fluidtype == 2), iso, (real)(ctx->ph.gamma - 1.0),
-            ctx->ph.OMEGAFRAME, rdom);
+  const int dust = (ctx->ph.fluidtype == 2);
+  // measured on MI355X: the fixed flags pay in 2-D (adiabatic 4096x1024 step
+  // 0.662 -> 0.645 ms); in 3-D the kernel took longer (34.5 -> 38.8 us)
+  const int mode = (visc && pot && !dust && !m.dimz) ? (iso ? 1 : 2) : 0;
+#define SRCB(MODE_) FDISPATCH_T(ctx, kf_src_b, MODE_, gb, dim3(BXF), 0, rdt, f.Density, f.Energy, f.Vx, f.Vy, f.Vz, \
+                                ctx->f.Pot, f.Mmx, f.Mmy, f.Mmz, f.Mpx, f.Mpy, f.Mpz, f.Vx_temp, f.Vy_temp,        \
+                                f.Vz_temp, S->f[FS_E2], visc, pot, dust, iso, (real)(ctx->ph.gamma - 1.0),        \
+                                ctx->ph.OMEGAFRAME, rdom)
+  if (mode == 1) SRCB(1); else if (mode == 2) SRCB(2); else SRCB(0);
+#undef SRCB
   f3d_stage_end(ctx, F3D_ST_SRC_B);
   if (!iso && ctx->ph.fluidtype == 1) {
     // SubStep3 (src/algogas.c:110-112)
@@ -3201,8 +3246,11 @@ int F3D_API(fused_sources_fluids)(f3d_hip_ctx* ctx, double dt, double domega, in
   if (vq >= 0) {
     dim3 ga((m.nx + BXF - 1) / BXF, m.sy - 2, m.dimz ? m.sz - 2 : 1);
     f3d_stage_begin(ctx, F3D_ST_SRC_A);
-    FDISPATCH(ctx, kf_src_a, ga, dim3(BXF), 0, sh.Density, sh.Energy, sh.Vx, sh.Vy, sh.Vz, sh.Mmx, sh.Mmy, sh.Mmz,
-              sh.Mpx, sh.Mpy, sh.Mpz, vp, 1, rdom);
+    // isothermal runs: the model alone selects the form
+#define SRCA(VM_) FDISPATCH_T(ctx, kf_src_a, VM_, ga, dim3(BXF), 0, sh.Density, sh.Energy, sh.Vx, sh.Vy, sh.Vz, \
+                              sh.Mmx, sh.Mmy, sh.Mmz, sh.Mpx, sh.Mpy, sh.Mpz, vp, 1, rdom)
+    if (vp.model == 1) SRCA(1); else if (vp.model == 2) SRCA(2); else SRCA(0);
+#undef SRCA
     f3d_stage_end(ctx, F3D_ST_SRC_A);
   }
   SrcTab T;
