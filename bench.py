@@ -34,6 +34,9 @@ sys.path.insert(0, ROOT)
 os.chdir(ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# the dominant stage's HIP events bracket every EVENT_EVERY-th launch in the
+# timed region (an event pair adds about 6 us to the step it brackets)
+EVENT_EVERY = 8
 PAR = {"c1": "fargo3d_amd/setups/c1_fargo2d.par", "c2": "fargo3d_amd/setups/c2_p3d.par",
        "c3": "fargo3d_amd/setups/c3_adiabatic2d.par", "c4": "fargo3d_amd/setups/c4_multifluid2d.par"}
 WORKLOAD = {
@@ -69,7 +72,7 @@ def stage_arrays(stage, dz, ad, alpha):
 # the kernels each stage's events bracket (names without namespace and template arguments)
 STAGE_KERNEL = {"cfl": ["kf_cfl", "kf_cfl_fluids", "kf_vmed_rings", "kf_cfl_fin"], "sources_a": ["kf_src_a"],
                 "sources_b": ["kf_src_b"], "sweep_z": ["kf_sweep_z"], "sweep_y": ["kf_sweep_y2", "kf_sweep_y3"],
-                "sweep_x": ["kf_sweep_xr_fluids", "kf_sweep_x_tile"], "finish": ["kf_finish"],
+                "sweep_x": ["kf_sweep_xw", "kf_sweep_xr_fluids", "kf_sweep_x_tile"], "finish": ["kf_finish"],
                 "potential": ["k_potential", "kf_potential"], "substep3": ["kf_substep3"]}
 # rocprofv3 summary with the measured HBM traffic of each kernel of this workload
 PROFILE = {"c2": "profiles/fused_step_c2_3d_256x96x32.md", "c1": "profiles/fused_step_c1_2d_384x128.md",
@@ -142,13 +145,14 @@ def _read_stages(L, sim, names):
 def timed_run(fa, L, names, parfile, overrides, arch, steps, warmup, probe=20):
     """Warm up, take every stage's HIP-event average over `probe` steps, then
     time exactly `steps` steps (synchronised on both sides) with events around
-    the dominant stage only (each event pair adds a gap between launches, so
-    the timed region carries one pair per step).  Returns the wall time, the
+    the dominant stage only, on every EVENT_EVERY-th of its launches (each
+    event pair adds a gap between launches).  Returns the wall time, the
     per-stage averages (the dominant stage's from the timed region) and the
     simulation."""
     # the handle is a pointer: every timing entry point gets its C signature
     L.f3d_hip_timing_mask.argtypes = [ctypes.c_void_p, ctypes.c_uint]
     L.f3d_hip_timing_read.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.f3d_hip_timing_every.argtypes = [ctypes.c_void_p, ctypes.c_int]
     sim = fa.Simulation(parfile, overrides, arch=arch)
     for _ in range(warmup):
         sim.step()
@@ -159,6 +163,9 @@ def timed_run(fa, L, names, parfile, overrides, arch, steps, warmup, probe=20):
     stages = _read_stages(L, sim, names)
     dom = max(stages, key=lambda n: stages[n]["avg_us"] * stages[n]["launches"]) if stages else None
     L.f3d_hip_timing_mask(sim.ctx, (1 << names.index(dom)) if dom else 0)
+    # the event pair goes around every EVENT_EVERY-th launch of the dominant
+    # stage only: each pair puts a gap of a few microseconds into the step
+    L.f3d_hip_timing_every(sim.ctx, EVENT_EVERY)
     sim.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -169,6 +176,7 @@ def timed_run(fa, L, names, parfile, overrides, arch, steps, warmup, probe=20):
         timed = _read_stages(L, sim, names)[dom]
         stages[dom] = dict(timed, dominant=True, probe_avg_us=stages[dom]["avg_us"])
     L.f3d_hip_timing_mask(sim.ctx, 0)
+    L.f3d_hip_timing_every(sim.ctx, 1)
     return elapsed, stages, sim
 
 

@@ -93,7 +93,7 @@ extern "C" int f3d_fused_cfl_fluids_noalloc(f3d_hip_ctx* ctx, int nf, double* co
   else
     FDISPATCH_T(ctx, kf_cfl_fluids, 0, dim3(nrings * nf), dim3(wg), shmem, T, nrings, vxm, ctx->f.VxMed, result, 0,
                 cfl_par_f(ctx));
-  hipLaunchKernelGGL(kf_cfl_fin, dim3(1), dim3(1024), 0, ctx->stream, sp, nf * nrings, result, 0, ctx->dhres);
+  hipLaunchKernelGGL(kf_cfl_fin, dim3(1), dim3(1024), 0, ctx->stream, sp, nf * nrings, result, 0, ctx->dhres, 0ull);
   F3D_LAUNCH_CHECK(ctx);
   F3D_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   *step_min = ctx->hres[0];

@@ -428,7 +428,7 @@ __global__ void __launch_bounds__(WG) kf_cfl_fluids(dmesh m, CflTab T, int nring
 // Second pass of the two-pass MIN: one workgroup takes the MIN of the n ring
 // minima (order independent, so equal to the atomic form) into result[slot].
 __global__ void __launch_bounds__(1024) kf_cfl_fin(const double* ringmin, int n, unsigned long long* result,
-                                                   int slot, double* host_out) {
+                                                   int slot, double* host_out, unsigned long long seq) {
   __shared__ double sh[16];
   double v = __longlong_as_double((long long)kInfBits);
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -444,9 +444,13 @@ __global__ void __launch_bounds__(1024) kf_cfl_fin(const double* ringmin, int n,
   if (threadIdx.x == 0) {
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) v = (sh[w] < v) ? sh[w] : v;
     result[slot] = (unsigned long long)__double_as_longlong(v);
-    // straight into pinned host memory: the host reads it after the stream
-    // synchronises, with no copy launch
-    if (host_out) *host_out = v;
+    // straight into pinned host memory, with no copy launch: the value, then
+    // (after a system-scope fence) the sequence number the host polls for
+    if (host_out) {
+      host_out[0] = v;
+      __threadfence_system();
+      *reinterpret_cast<volatile unsigned long long*>(host_out + 1) = seq;
+    }
   }
 }
 
@@ -2941,7 +2945,8 @@ int F3D_API(fused_cfl_vm)(f3d_hip_ctx* ctx, const double* vxm_in, double* fluid_
   else CFL1(0);
 #undef CFL1
   if (rm) {
-    hipLaunchKernelGGL(kf_cfl_fin, dim3(1), dim3(1024), 0, ctx->stream, rm, nrings, S->result, slot, ctx->dhres);
+    hipLaunchKernelGGL(kf_cfl_fin, dim3(1), dim3(1024), 0, ctx->stream, rm, nrings, S->result, slot, ctx->dhres,
+                       0ull);
     F3D_LAUNCH_CHECK(ctx);
   }
   f3d_stage_end(ctx, F3D_ST_CFL);
@@ -2989,16 +2994,23 @@ int F3D_API(fused_cfl_fluids_async)(f3d_hip_ctx* ctx, int nf, double* const* cs,
   if (pre) CFLF(1);
   else CFLF(0);
 #undef CFLF
+  ctx->cfl_spin = 0;
   if (mode & 1) {
+    ctx->cfl_spin = 1;
     hipLaunchKernelGGL(kf_cfl_fin, dim3(1), dim3(1024), 0, ctx->stream, S->ringmin, nf * nrings, S->result, slot,
-                       ctx->dhres);
+                       ctx->dhres, ++ctx->cfl_seq);
     F3D_LAUNCH_CHECK(ctx);
   }
   f3d_stage_end(ctx, F3D_ST_CFL);
-  if (!(mode & 1))
+  // the atomic form copies the step to the host and records an event for
+  // the wait; the two-pass form publishes it itself (an event record would
+  // hold the next launch back by a few microseconds)
+  if (!(mode & 1)) {
     F3D_CHECK(ctx, hipMemcpyAsync(ctx->hres, S->result + slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  if (!ctx->cfl_done) F3D_CHECK(ctx, hipEventCreateWithFlags(&ctx->cfl_done, hipEventDisableTiming));
-  F3D_CHECK(ctx, hipEventRecord(ctx->cfl_done, ctx->stream));
+    if (!ctx->cfl_done) F3D_CHECK(ctx, hipEventCreateWithFlags(&ctx->cfl_done, hipEventDisableTiming));
+    F3D_CHECK(ctx, hipEventRecord(ctx->cfl_done, ctx->stream));
+  }
+  ctx->cfl_queued = 1;
   S->slot ^= 1;
   return 0;
 }
@@ -3006,7 +3018,28 @@ int F3D_API(fused_cfl_fluids_async)(f3d_hip_ctx* ctx, int nf, double* const* cs,
 // The step queued by fused_cfl_fluids_async, once its launches are done
 // (work queued behind them may still run).
 int F3D_API(fused_cfl_wait)(f3d_hip_ctx* ctx, double* step_min) {
-  F3D_REQUIRE(ctx, ctx->cfl_done != nullptr, "cfl_wait: no CFL queued");
+  F3D_REQUIRE(ctx, ctx->cfl_queued, "cfl_wait: no CFL queued");
+  ctx->cfl_queued = 0;
+  if (ctx->cfl_spin) {
+    // poll the sequence number the last CFL kernel writes after dt: the
+    // value is there as soon as the kernel's stores land, before the
+    // runtime has processed the kernel's completion.  Past two seconds the
+    // wait goes to the stream, which also reports a failed launch.
+    const volatile unsigned long long* flag = reinterpret_cast<volatile unsigned long long*>(ctx->hres + 1);
+    const unsigned long long want = ctx->cfl_seq;
+    const auto t0 = std::chrono::steady_clock::now();
+    bool seen = true;
+    for (long n = 1; *flag != want; n++) {
+      if ((n & 0xFFFF) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
+        seen = false;
+        break;
+      }
+    }
+    if (!seen) F3D_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    std::atomic_thread_fence(std::memory_order_acquire);
+    *step_min = ctx->hres[0];
+    return 0;
+  }
   F3D_CHECK(ctx, hipEventSynchronize(ctx->cfl_done));
   *step_min = ctx->hres[0];
   return 0;

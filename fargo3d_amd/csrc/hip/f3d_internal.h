@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <chrono>
 #include <cstdio>
 
 #include "../../../include/fargo3d_hip.h"
@@ -34,6 +36,12 @@ struct f3d_hip_ctx {
   // recorded after the fused CFL's last launch: the dt read-back waits for
   // it, not for work queued behind the CFL (created on first use)
   hipEvent_t cfl_done = nullptr;
+  // the fused CFL's last kernel writes dt to hres[0] and then this sequence
+  // number to hres[1] (pinned host memory); the host polls it (cfl_spin: the
+  // queued CFL publishes the number)
+  unsigned long long cfl_seq = 0;
+  int cfl_spin = 0;
+  int cfl_queued = 0;  // a fused CFL is queued and not yet waited for
   int device = 0;
   f3d::dmesh m{};
   double* meshbuf = nullptr;        // one device buffer for all 1-D mesh arrays
@@ -63,12 +71,17 @@ struct f3d_hip_ctx {
   int pending[F3D_NSTAGES] = {};
   double stage_ms[F3D_NSTAGES] = {};
   long stage_n[F3D_NSTAGES] = {};
+  int timing_every = 1;             // time every n-th launch of a stage only
+  long stage_calls[F3D_NSTAGES] = {};
+  int skipped[F3D_NSTAGES] = {};
 };
 
 // Event pair around one launch of stage s when timing is on; the previous
 // pair of the same stage is complete by then (every step synchronises on dt).
 inline void f3d_stage_begin(f3d_hip_ctx* c, int s) {
   if (!((c->timing >> s) & 1u)) return;
+  c->skipped[s] = (c->stage_calls[s]++ % c->timing_every) != 0;
+  if (c->skipped[s]) return;
   if (c->pending[s]) {
     float ms = 0.f;
     if (hipEventSynchronize(c->ev[s][1]) == hipSuccess &&
@@ -81,7 +94,7 @@ inline void f3d_stage_begin(f3d_hip_ctx* c, int s) {
   (void)hipEventRecord(c->ev[s][0], c->stream);
 }
 inline void f3d_stage_end(f3d_hip_ctx* c, int s) {
-  if (!((c->timing >> s) & 1u)) return;
+  if (!((c->timing >> s) & 1u) || c->skipped[s]) return;
   (void)hipEventRecord(c->ev[s][1], c->stream);
   c->pending[s] = 1;
 }

@@ -105,8 +105,17 @@ int f3d_hip_timing_mask(f3d_hip_ctx* c, unsigned mask) {
     c->pending[s] = 0;
     c->stage_ms[s] = 0.0;
     c->stage_n[s] = 0;
+    c->stage_calls[s] = 0;
+    c->skipped[s] = 0;
   }
   c->timing = mask & ((1u << F3D_NSTAGES) - 1u);
+  return 0;
+}
+
+int f3d_hip_timing_every(f3d_hip_ctx* c, int n) {
+  if (n < 1) return -2;
+  c->timing_every = n;
+  for (int s = 0; s < F3D_NSTAGES; s++) c->stage_calls[s] = 0;
   return 0;
 }
 

@@ -147,6 +147,10 @@ int f3d_hip_timing_enable(f3d_hip_ctx *ctx, int on);
  * its timed region); enable(on) is mask ~0 or 0 */
 int f3d_hip_timing_mask(f3d_hip_ctx *ctx, unsigned mask);
 int f3d_hip_timing_read(f3d_hip_ctx *ctx, double *ms, long *counts);
+/* time only every n-th launch of each timed stage (n >= 1; -2 otherwise):
+ * the averages then come from a sample of the launches, and the others run
+ * without the event pair's gap */
+int f3d_hip_timing_every(f3d_hip_ctx *ctx, int n);
 
 /* ---- per-routine entry points (reference pointer table, src/global.h:303-416) ---- */
 /* ComputePressureFieldIso/Ad: src/CompPresIso.c:10, src/CompPresAd.c:10 */
