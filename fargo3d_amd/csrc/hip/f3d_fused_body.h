@@ -159,6 +159,25 @@ int scratch_of(f3d_hip_ctx* ctx, Scratch** out) {
   return 0;
 }
 
+// Whole-wave rotations by one lane (DPP wave_ror:1 and wave_rol:1 of the
+// gfx9 family): from_left gives lane L the value of lane L-1 and lane 0 that
+// of lane 63; from_right gives lane L the value of lane L+1 and lane 63 that
+// of lane 0.  All 64 lanes must be active.
+__device__ __forceinline__ float from_left(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x13C, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_right(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x134, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double from_left(double x) {
+  return __hiloint2double(__builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x13C, 0xF, 0xF, false),
+                          __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x13C, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double from_right(double x) {
+  return __hiloint2double(__builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x134, 0xF, 0xF, false),
+                          __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x134, 0xF, 0xF, false));
+}
+
 __device__ __forceinline__ double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) {
     double w = __shfl_down(v, o, 64);
@@ -616,6 +635,80 @@ __global__ void __launch_bounds__(WG, DZ ? 5 : 8) kf_src_b(dmesh m, real dt, con
                                                int visc, int potential, int dust, int iso, real gm1, double omegaframe,
                                                real domega) {
   kd_src_b<G, DZ, MODE>(xcd_bid(), m, dt, rho, energy, vx, vy, vz, pot, txx, tyy, tzz, txz, tyx, tzy, vxt, vyt, vzt, e2, visc, potential, dust, iso, gm1, omegaframe, domega);
+}
+
+// 2-D sources of a viscous gas with a potential in one pass, the viscous
+// stress computed where addviscosity reads it instead of going through the
+// stress arrays (kf_src_a + kf_src_b, without their 3 arrays written and read
+// back).  Each lane marches a column through rows_per rows with the stresses
+// of rows j-1, j, j+1 in registers (visctensor once per cell and row, plus two
+// rows per segment); the x neighbours' stresses come from the next lanes by
+// wave rotation.  A wave holds 64 stress columns, i0-1 .. i0+62, and updates
+// the 62 cells between (waves overlap by two columns), so every neighbour a
+// cell reads is in the wave.  MODE as kd_src_b (1 isothermal, 2 adiabatic),
+// VM as kd_src_a (1 constant nu, 2 alpha isothermal, 3 alpha adiabatic).
+struct StressAt {
+  real c, xm, xp, ym, yp;  // at the cell, its x-1, x+1, y-1 and y+1 neighbours
+  __device__ __forceinline__ real operator()(int di, int dj, int) const {
+    return di < 0 ? xm : di > 0 ? xp : dj < 0 ? ym : dj > 0 ? yp : c;
+  }
+};
+constexpr int kSrcCols = 62;  // cells a wave of kf_src_fused2 updates
+template <int G, int MODE, int VM>
+__global__ void __launch_bounds__(64) kf_src_fused2(dmesh m, real dt, const real* __restrict__ rho,
+                                                    const real* __restrict__ energy, const real* __restrict__ vx,
+                                                    const real* __restrict__ vy, const double* __restrict__ pot,
+                                                    real* __restrict__ vxt, real* __restrict__ vyt,
+                                                    real* __restrict__ e2, cell::ViscPar vp, real gm1,
+                                                    double omegaframe, real domega, int rows_per, int jend) {
+  constexpr int iso = MODE == 1;
+  vp.model = VM == 1 ? 1 : 2;
+  vp.iso = VM != 3;
+  const int lane = (int)threadIdx.x;
+  const int is = kSrcCols * (int)blockIdx.x - 1 + lane;  // this lane's stress column
+  const int i = is < 0 ? is + m.nx : (is >= m.nx ? is - m.nx : is);
+  const bool upd = lane >= 1 && lane <= kSrcCols && is < m.nx;
+  const int r0 = m.nghy + (int)blockIdx.y * rows_per;
+  const int r1 = min(r0 + rows_per, jend);
+  // visctensor (src/visctensor_cyl.c:140-186, src/visctensor_sph.c:126-206)
+  // at (i, row): the same loaders as kd_src_a
+  auto stress = [&](int row) -> cell::Taus {
+    const int l = i + row * m.pitch;
+    const auto VX = vx_stress<G>(vx, m, l, i, row, 0, domega);
+    const cell::GL VY = cell::gl(vy, m, l, i);
+    return cell::visctensor<G, 0>(m, i, row, 0, cell::gl(rho, m, l, i), cell::gl(energy, m, l, i), VX, VY, VY, vp);
+  };
+  cell::Taus Tm = stress(r0 - 1), T0 = stress(r0);
+  for (int j = r0; j < r1; j++) {
+    const cell::Taus Tp = stress(j + 1);
+    // the x neighbours' stresses (every lane takes part in the rotations)
+    const real txx_xm = from_left(T0.xx), tyx_xp = from_right(T0.yx);
+    if (upd) {
+      // kd_src_b at (i, j)
+      const int l = i + j * m.pitch;
+      const cell::GL R = cell::gl(rho, m, l, i);
+      const PresLoader P{R, cell::gl(energy, m, l, i), gm1, iso};
+      const cell::GLT<double> PT = cell::gld(pot, m, l, i);
+      const auto VX = vxc<G>(vx, m, l, i, j, 0, domega);
+      const cell::GL VY = cell::gl(vy, m, l, i);
+      real ax = cell::substep1_x<G>(m, i, j, 0, dt, R, P, PT, VX, 0, 1);
+      real ay = cell::substep1_y<G, 0>(m, i, j, 0, dt, R, P, PT, VX, VY, VY, 0, 1, omegaframe);
+      real az = real(0.0);
+      const StressAt TXX{T0.xx, txx_xm, real(0.0), Tm.xx, real(0.0)};
+      const StressAt TYY{T0.yy, real(0.0), real(0.0), Tm.yy, real(0.0)};
+      const StressAt TYX{T0.yx, real(0.0), tyx_xp, real(0.0), Tp.yx};
+      cell::addviscosity<G, 0>(m, i, j, 0, dt, R, TXX, TYY, TXX, TXX, TYX, TXX, ax, ay, az, ax, ay, az);
+      real e = iso ? real(0.0) : energy[l];
+      const ArtQ<cell::GL, decltype(VX), 0> QX{R, VX};
+      const ArtQ<cell::GL, cell::GL, 1> QY{R, VY};
+      cell::substep2_b<G, 0>(m, i, j, 0, dt, R, VX, VY, VY, QX, QY, QY, !iso, ax, ay, az, e, ax, ay, az, e);
+      vxt[l] = ax;
+      vyt[l] = ay;
+      if (!iso) e2[l] = e;
+    }
+    Tm = T0;
+    T0 = Tp;
+  }
 }
 
 // kf_src_b of every fluid in one launch (isothermal runs): blockIdx.z =
@@ -1373,25 +1466,6 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
       }
     }
   }
-}
-
-// Whole-wave rotations by one lane (DPP wave_ror:1 and wave_rol:1 of the
-// gfx9 family): from_left gives lane L the value of lane L-1 and lane 0 that
-// of lane 63; from_right gives lane L the value of lane L+1 and lane 63 that
-// of lane 0.  All 64 lanes must be active.
-__device__ __forceinline__ float from_left(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x13C, 0xF, 0xF, false));
-}
-__device__ __forceinline__ float from_right(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x134, 0xF, 0xF, false));
-}
-__device__ __forceinline__ double from_left(double x) {
-  return __hiloint2double(__builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x13C, 0xF, 0xF, false),
-                          __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x13C, 0xF, 0xF, false));
-}
-__device__ __forceinline__ double from_right(double x) {
-  return __hiloint2double(__builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x134, 0xF, 0xF, false),
-                          __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x134, 0xF, 0xF, false));
 }
 
 // A ring of 64*CPT cells held by one wave, CPT consecutive cells per lane:
@@ -2384,6 +2458,9 @@ static int g_xsplit = -1;
 // VxMed in a pass of their own (else two sums inside each ring's workgroup).
 // -1: chosen from the mesh shape.
 static int g_cflmode = -1;
+// 2-D sources of a viscous gas in one pass without the stress arrays
+// (kf_src_fused2): -1 automatic (on), 0 off (kf_src_a + kf_src_b)
+static int g_srcfuse = -1;
 
 // CFL form, bits: 1 ring minima to an array and a second pass (else one
 // atomic MIN per ring), 2 VxMed from kf_vmed_rings, 4 (with 2) one wave per
@@ -2906,6 +2983,12 @@ int F3D_API(fused_set_xmode)(int mode) {
   return 0;
 }
 
+int F3D_API(fused_set_srcfuse)(int on) {
+  if (on < -1 || on > 1) return -2;
+  g_srcfuse = on;
+  return 0;
+}
+
 int F3D_API(fused_set_vmedterms)(int terms) {
   if (terms != -1 && terms != 4 && terms != 8 && terms != 16) return -2;
   g_vmed_terms = terms;
@@ -3205,23 +3288,46 @@ int F3D_API(fused_sources)(f3d_hip_ctx* ctx, double dt, double domega) {
   const int visc = ctx->ph.visc != 0 && ctx->ph.fluidtype == 1;
   const int pot = ctx->ph.potential;
   const real rdt = (real)dt, rdom = (real)domega;
+  const int vm = vp.model == 1 ? 1 : vp.model == 2 ? (iso ? 2 : 3) : 0;
+  const int dust = (ctx->ph.fluidtype == 2);
+  // adiabatic: SubStep3 reads V_temp one row (and plane) past the active
+  // range, which the reference computes there with the same formulas
+  // (src/substep1_y.c:78-80, src/substep2_b.c:84-86), so those are included
+  const int ext = iso ? 0 : 1;
+  if (!m.dimz && visc && pot && !dust && vm != 0 && g_srcfuse != 0) {
+    // one pass without the stress arrays (kf_src_fused2); rows per lane
+    // halved from 8 while the launch has fewer than 4096 waves
+    const int wx = (m.nx + kSrcCols - 1) / kSrcCols, rows = m.ny + ext;
+    int rp = 8;
+    while (rp > 1 && (long)wx * ((rows + rp - 1) / rp) < 4096) rp /= 2;
+    dim3 g(wx, (rows + rp - 1) / rp);
+    f3d_stage_begin(ctx, F3D_ST_SRC_B);
+#define SRCF(G_, MODE_, VM_)                                                                                    \
+  hipLaunchKernelGGL((kf_src_fused2<G_, MODE_, VM_>), g, dim3(64), 0, ctx->stream, m, rdt, f.Density, f.Energy, \
+                     f.Vx, f.Vy, ctx->f.Pot, f.Vx_temp, f.Vy_temp, S->f[FS_E2], vp, (real)(ctx->ph.gamma - 1.0), \
+                     ctx->ph.OMEGAFRAME, rdom, rp, m.nghy + rows)
+#define SRCF_G(G_)                                                        \
+  do {                                                                    \
+    if (iso) { if (vm == 1) SRCF(G_, 1, 1); else SRCF(G_, 1, 2); }       \
+    else { if (vm == 1) SRCF(G_, 2, 1); else SRCF(G_, 2, 3); }           \
+  } while (0)
+    if (m.geom == SPH) SRCF_G(SPH); else SRCF_G(CYL);
+#undef SRCF_G
+#undef SRCF
+    F3D_LAUNCH_CHECK(ctx);
+    f3d_stage_end(ctx, F3D_ST_SRC_B);
+  } else {
   dim3 ga((m.nx + BXF - 1) / BXF, m.sy - 2, m.dimz ? m.sz - 2 : 1);
   if (visc) {
     f3d_stage_begin(ctx, F3D_ST_SRC_A);
-    const int vm = vp.model == 1 ? 1 : vp.model == 2 ? (iso ? 2 : 3) : 0;
 #define SRCA(VM_) FDISPATCH_T(ctx, kf_src_a, VM_, ga, dim3(BXF), 0, f.Density, f.Energy, f.Vx, f.Vy, f.Vz, f.Mmx, \
                               f.Mmy, f.Mmz, f.Mpx, f.Mpy, f.Mpz, vp, visc, rdom)
     if (vm == 1) SRCA(1); else if (vm == 2) SRCA(2); else if (vm == 3) SRCA(3); else SRCA(0);
 #undef SRCA
     f3d_stage_end(ctx, F3D_ST_SRC_A);
   }
-  // adiabatic: SubStep3 reads V_temp one row (and plane) past the active
-  // range, which the reference computes there with the same formulas
-  // (src/substep1_y.c:78-80, src/substep2_b.c:84-86), so those are included
-  const int ext = iso ? 0 : 1;
   dim3 gb((m.nx + BXF - 1) / BXF, m.ny + ext, m.dimz ? m.nz + ext : 1);
   f3d_stage_begin(ctx, F3D_ST_SRC_B);
-  const int dust = (ctx->ph.fluidtype == 2);
   // measured on MI355X: the fixed flags pay in 2-D (adiabatic 4096x1024 step
   // 0.662 -> 0.645 ms); in 3-D the kernel took longer (34.5 -> 38.8 us)
   const int mode = (visc && pot && !dust && !m.dimz) ? (iso ? 1 : 2) : 0;
@@ -3232,6 +3338,7 @@ int F3D_API(fused_sources)(f3d_hip_ctx* ctx, double dt, double domega) {
   if (mode == 1) SRCB(1); else if (mode == 2) SRCB(2); else SRCB(0);
 #undef SRCB
   f3d_stage_end(ctx, F3D_ST_SRC_B);
+  }
   if (!iso && ctx->ph.fluidtype == 1) {
     // SubStep3 (src/algogas.c:110-112)
     dim3 g3((m.nx + BXF - 1) / BXF, m.ny, m.dimz ? m.nz : 1);

@@ -346,6 +346,28 @@ def test_whole_step_fused_one_wave_x(fa, orc, par, ov, nsteps):
         L.f3d_fused_set_xmode(0)
 
 
+@pytest.mark.parametrize("fuse", [1, 0], ids=["one_pass", "stress_arrays"])
+@pytest.mark.parametrize("par,ov,nsteps", [(C1, "", 12), (C1, "ViscosityModel=alpha Alpha=1e-3", 8),
+                                           (C3, C3_SMALL, 10), (C3, C3_SMALL + " ViscosityModel=alpha Alpha=1e-3", 6),
+                                           (C1, "Nx=100 Ny=40", 6)],
+                         ids=["c1_2d_nu", "c1_2d_alpha", "c3_2d_adiabatic_nu", "c3_2d_adiabatic_alpha",
+                              "c1_2d_ragged"])
+def test_whole_step_fused_sources_one_pass(fa, orc, par, ov, nsteps, fuse):
+    """The 2-D sources of a viscous gas in one pass (the stress computed where
+    it is used, x neighbours across lanes, waves overlapping by two columns,
+    rings not a multiple of the 62 cells a wave updates) and through the
+    stress arrays give the oracle's results bit for bit."""
+    L = fa.lib()
+    assert L.f3d_fused_set_srcfuse(fuse) == 0
+    try:
+        o, s = _fused_run(fa, orc, par, ov, nsteps)
+        for f in fields_of(s):
+            assert nbits(active(s.field(f), s.dimz), active(o.field(f), s.dimz)) == 0, (par, ov, f)
+        s.close()
+    finally:
+        L.f3d_fused_set_srcfuse(-1)
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3, 6, 7], ids=["atomic", "two_pass", "vmed_pass_atomic",
                                                      "vmed_pass_two_pass", "vmed_pass_atomic_one_wave",
                                                      "vmed_pass_two_pass_one_wave"])
