@@ -562,10 +562,18 @@ struct CflPar {
 // of c/zone_size are formed as c/min(zone_size): IEEE division is monotone
 // in the divisor and rounding is monotone, so max_k fl(c/z_k) = fl(c/min z_k)
 // exactly (c >= 0, z_k > 0), with one division instead of three.
-template <int G, int DZ>
+// FX fixes the flags at compile time (1 isothermal, constant nu; 2
+// isothermal, alpha; 3 adiabatic, constant nu; FARGO in all three; 0: as
+// cp holds them), so that no branch separates the cells' loads.
+template <int G, int DZ, int FX = 0>
 __device__ __forceinline__ double cfl_sumsq(const dmesh& m, int i, int j, int k, int l, const real* cs,
                                             const real* rho, const real* vx, const real* vy, const real* vz,
-                                            double vxmed, const CflPar& cp) {
+                                            double vxmed, CflPar cp) {
+  if constexpr (FX != 0) {
+    cp.fargo = 1;
+    cp.iso = FX != 3;
+    cp.alpha_visc = FX == 2;
+  }
   const int lxp = lxp_of(l, i, m.nx), lyp = l + m.pitch, lzp = l + m.stride;
   double vx0 = vx[l], vx1 = vx[lxp];
   if constexpr (kFix64) {

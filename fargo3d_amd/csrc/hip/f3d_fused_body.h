@@ -347,7 +347,7 @@ __device__ __forceinline__ double vmed_full(double stored_mean, const dmesh& m, 
 // PRE: VxMed of the ring and the next from kf_vmed_rings; else summed here
 // (the in-kernel sum's registers would halve the occupancy of the PRE form,
 // hence a template parameter).
-template <int G, int DZ, int PRE>
+template <int G, int DZ, int PRE, int FX = 0>
 __device__ __forceinline__ void kd_cfl(int ring, const dmesh& m, const real* cs, const real* rho, const real* vx, const real* vy, const real* vz, const real* vxm, double* vxmed, unsigned long long* result, int slot, cell::CflPar cp, double* ringmin) {
   extern __shared__ double lds[];  // 2 nx weights + 16 (16 with pre)
   double* w0 = lds;
@@ -385,7 +385,7 @@ __device__ __forceinline__ void kd_cfl(int ring, const dmesh& m, const real* cs,
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int i = min(i0 + t + u * nt, m.nx - 1);
-      s4[u] = cell::cfl_sumsq<G, DZ>(m, i, j, k, base + i, cs, rho, vx, vy, vz, vm0, cp);
+      s4[u] = cell::cfl_sumsq<G, DZ, FX>(m, i, j, k, base + i, cs, rho, vx, vy, vz, vm0, cp);
     }
 #pragma unroll
     for (int u = 0; u < 4; u++)
@@ -435,13 +435,14 @@ struct CflTab {
   const real* vz[8];
   double* ringmin[8];
 };
-template <int G, int DZ, int PRE>
+// PF bit 0: PRE; PF >> 1: FX of cell::cfl_sumsq (the flags at compile time)
+template <int G, int DZ, int PF>
 __global__ void __launch_bounds__(WG) kf_cfl_fluids(dmesh m, CflTab T, int nrings, const real* vxm, double* vxmed,
                                                     unsigned long long* result, int slot, cell::CflPar cp) {
   const int b = xcd_linear();
   const int fl = b / nrings, r = b % nrings;
-  kd_cfl<G, DZ, PRE>(r, m, T.cs[fl], T.rho[fl], T.vx[fl], T.vy[fl], T.vz[fl], vxm, vxmed, result, slot, cp,
-                     T.ringmin[fl]);
+  kd_cfl<G, DZ, PF & 1, (PF >> 1)>(r, m, T.cs[fl], T.rho[fl], T.vx[fl], T.vy[fl], T.vz[fl], vxm, vxmed, result,
+                                   slot, cp, T.ringmin[fl]);
 }
 
 // Second pass of the two-pass MIN: one workgroup takes the MIN of the n ring
@@ -3072,10 +3073,22 @@ int F3D_API(fused_cfl_fluids_async)(f3d_hip_ctx* ctx, int nf, double* const* cs,
   }
   const int wg = (mode & 4) ? 64 : WG;
 #define CFLF(P_)                                                                                             \
-  FDISPATCH_T(ctx, kf_cfl_fluids, P_, dim3(nrings * nf), dim3(wg), P_ ? 16 * sizeof(double) : shmem, T, nrings, \
-              vxm, f.VxMed, S->result, slot, cfl_par_f(ctx))
-  if (pre) CFLF(1);
-  else CFLF(0);
+  FDISPATCH_T(ctx, kf_cfl_fluids, P_, dim3(nrings * nf), dim3(wg), (P_ & 1) ? 16 * sizeof(double) : shmem, T, \
+              nrings, vxm, f.VxMed, S->result, slot, cfl_par_f(ctx))
+  // the flag set of the common configurations at compile time (FARGO:
+  // isothermal with constant nu or alpha, adiabatic with constant nu)
+  const cell::CflPar cpf = cfl_par_f(ctx);
+  const int fx = !cpf.fargo ? 0 : cpf.iso ? (cpf.alpha_visc ? 2 : 1) : (cpf.alpha_visc ? 0 : 3);
+  switch (fx * 2 + pre) {
+    case 2: CFLF(2); break;
+    case 3: CFLF(3); break;
+    case 4: CFLF(4); break;
+    case 5: CFLF(5); break;
+    case 6: CFLF(6); break;
+    case 7: CFLF(7); break;
+    case 1: CFLF(1); break;
+    default: CFLF(0); break;
+  }
 #undef CFLF
   ctx->cfl_spin = 0;
   if (mode & 1) {
