@@ -41,6 +41,9 @@ namespace {
 
 constexpr int WG = 256;
 constexpr int BXF = 64;
+// halo cells on each side of an X-sweep tile: the van Leer pass reaches 2
+// cells, the PPA pass 3 more, NewVelocity_x the left neighbour
+constexpr int kXHalo = 6;
 constexpr int kNgh = 3;  // ghost layers in Y and Z (src/define.h:57-63)
 
 // Scratch of the fused stages, n3 doubles each, allocated on a handle's
@@ -1498,48 +1501,73 @@ struct WRing {
 // through a pass on its own after the density's faces (which every update
 // reads), so only one quantity's stencil arrays are live at a time.  VxMed
 // is summed in the wave (wave_chain) from the velocities it holds.
-template <int G, int DZ, int AD, int CPT>
+// TILE = 1: the wave holds a tile of a longer ring instead, the 64*CPT cells
+// from tile*(64*CPT - 2 kXHalo) - kXHalo on (indexes wrapped around the
+// ring), and updates the 64*CPT - 2 kXHalo cells inside the halos, the same
+// reach as kf_sweep_x_tile's: the rotations wrap the tile's two ends into
+// each other, and what that puts into the halo cells does not reach the
+// updated ones.  VxMed then comes from kf_vmed.
+template <int G, int DZ, int AD, int CPT, int TILE = 0>
 __device__ __forceinline__ void xsweep_wave(const dmesh& m, int ring, double dt64, const real* __restrict__ rho_in,
                                             const real* __restrict__ vxt, const real* const* qin,
                                             const real* __restrict__ e_in, real* __restrict__ e_out,
                                             real* __restrict__ rho_out, real* __restrict__ vx_out,
                                             real* const* qout, double* __restrict__ vxmed,
-                                            int* __restrict__ nshift_out, real omegaframe, int fargo) {
+                                            int* __restrict__ nshift_out, real omegaframe, int fargo,
+                                            int tile = 0) {
   using W = WRing<CPT>;
   constexpr int NM = DZ ? 6 : 4;
   constexpr int NQ = NM + AD;  // transported quantities besides the density
-  const int nx = m.nx;         // == 64 * CPT
+  constexpr int TOUT = 64 * CPT - 2 * kXHalo;  // cells a tile updates
+  const int nx = m.nx;         // == 64 * CPT for a whole ring
   const real dt = (real)dt64;
   const int lane = (int)threadIdx.x;
   const int j = m.nghy + ring % m.ny;
   const int k = DZ ? m.nghz + ring / m.ny : 0;
   const int base = j * m.pitch + k * m.stride;
-  const int i0 = lane * CPT;
+  // this lane's first cell on the ring (a tile's may lie outside [0, nx))
+  const int t0 = tile * TOUT;
+  const int u0 = TILE ? t0 - kXHalo + lane * CPT : lane * CPT;
+  int ix[CPT];  // the cells' indexes on the ring
+#pragma unroll
+  for (int s = 0; s < CPT; s++) {
+    int g = u0 + s;
+    if (TILE) {
+      g %= nx;
+      if (g < 0) g += nx;
+    }
+    ix[s] = g;
+  }
   real rho[CPT], st[NQ][CPT];
   double vt[CPT];
 #pragma unroll
   for (int s = 0; s < CPT; s++) {
-    rho[s] = rho_in[base + i0 + s];
+    rho[s] = rho_in[base + ix[s]];
 #pragma unroll
-    for (int c = 0; c < NQ; c++) st[c][s] = (c < NM) ? qin[c][base + i0 + s] : e_in[base + i0 + s];
-    vt[s] = vxt[base + i0 + s];
+    for (int c = 0; c < NQ; c++) st[c][s] = (c < NM) ? qin[c][base + ix[s]] : e_in[base + ix[s]];
+    vt[s] = vxt[base + ix[s]];
   }
   // ComputeResidual (src/compute_residual.c:68-73), FP64: the cell residual
   // velocity for the van Leer pass, the uniform one for the PPA pass
   real vel[CPT], vu[CPT];
   int ns = 0;
   if (fargo) {
-    // ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30): the ring's weights are
-    // in the lanes already, in the layout of wave_ring_sum's chunks
-    double w[CPT];
+    double vmed;
+    if constexpr (TILE) {
+      vmed = vxmed[j + k * m.sy];  // from kf_vmed
+    } else {
+      // ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30): the ring's weights
+      // are in the lanes already, in the layout of wave_ring_sum's chunks
+      double w[CPT];
 #pragma unroll
-    for (int s = 0; s < CPT; s++) w[s] = vt[s] * (m.xmin64[i0 + s + 1] - m.xmin64[i0 + s]);
-    const double vmed = vmed_full(wave_chain<CPT>(w, 0.0, 64) / (m.XMAX - m.XMIN), m, j, k);
+      for (int s = 0; s < CPT; s++) w[s] = vt[s] * (m.xmin64[ix[s] + 1] - m.xmin64[ix[s]]);
+      vmed = vmed_full(wave_chain<CPT>(w, 0.0, 64) / (m.XMAX - m.XMIN), m, j, k);
+    }
     const double vb = cell::vbase(m, j, k);
     double nr0 = 0.0;
 #pragma unroll
     for (int s = 0; s < CPT; s++) {
-      const double zs = zone_size_x64<G>(m, i0 + s, j, k);
+      const double zs = zone_size_x64<G>(m, ix[s], j, k);
       const double ntilde = vmed * dt64 / zs;
       const double nround = floor(ntilde + 0.5);
       if (s == 0) nr0 = nround;
@@ -1547,12 +1575,15 @@ __device__ __forceinline__ void xsweep_wave(const dmesh& m, int ring, double dt6
       else vel[s] = (real)(vt[s] - vmed);
       vu[s] = (real)((ntilde - nround) * zs / dt64);
     }
-    // the shift of the ring is that of its cell 0 (lane 0)
-    ns = (int)__hiloint2double(__builtin_amdgcn_readlane(__double2hiint(nr0), 0),
-                               __builtin_amdgcn_readlane(__double2loint(nr0), 0));
-    if (lane == 0) {
+    // the shift of the ring is that of its cell 0: lane 0's, or for a tile
+    // the same expression at i = 0
+    if constexpr (TILE) ns = (int)floor(vmed * dt64 / zone_size_x64<G>(m, 0, j, k) + 0.5);
+    else
+      ns = (int)__hiloint2double(__builtin_amdgcn_readlane(__double2hiint(nr0), 0),
+                                 __builtin_amdgcn_readlane(__double2loint(nr0), 0));
+    if (lane == 0 && (!TILE || tile == 0)) {
       nshift_out[j + k * m.sy] = ns;
-      vxmed[j + k * m.sy] = vmed;
+      if (!TILE) vxmed[j + k * m.sy] = vmed;
     }
   } else {
     const double vb = cell::vbase(m, j, k);
@@ -1567,8 +1598,8 @@ __device__ __forceinline__ void xsweep_wave(const dmesh& m, int ring, double dt6
   Recip Vi[CPT];
 #pragma unroll
   for (int s = 0; s < CPT; s++) {
-    zs[s] = zone_size_x<G>(m, i0 + s, j, k);
-    Vi[s] = invvol_recip(m, i0 + s, k);
+    zs[s] = zone_size_x<G>(m, ix[s], j, k);
+    Vi[s] = invvol_recip(m, ix[s], k);
   }
   W::left(zs, zsl);
 
@@ -1657,7 +1688,9 @@ __device__ __forceinline__ void xsweep_wave(const dmesh& m, int ring, double dt6
   W::left(st[0], mpl);
 #pragma unroll
   for (int s = 0; s < CPT; s++) {
-    int it = (i0 + s + ns) % nx;
+    // a tile writes the cells inside its halos that lie on the ring
+    if (TILE && (u0 + s < t0 || u0 + s >= t0 + TOUT || u0 + s >= nx)) continue;
+    int it = (ix[s] + ns) % nx;
     if (it < 0) it += nx;
 #pragma unroll
     for (int c = 2; c < NQ; c++) {
@@ -1703,7 +1736,6 @@ __global__ void __launch_bounds__(64) kf_vmed(dmesh m, VmTab T, int nrings) {
 // VxMed comes from kf_vmed.  The outputs go to their shifted positions
 // directly (AdvectSHIFT, src/advect_shift.c:45-58) with NewVelocity_x computed
 // at the destination cell from the source cell and its left neighbour.
-constexpr int kXHalo = 6;
 __host__ __device__ constexpr int x_tile_cells(int threads) { return threads - 2 * kXHalo; }
 template <int G, int DZ, int AD, int T>
 __device__ __forceinline__ void kd_sweep_x_tile(const dim3 bid, const dmesh& m, double dt64, const real* rho_in, const real* vxt, const real* mpx, const real* mmx, const real* mpy, const real* mmy, const real* mpz, const real* mmz, const real* e_in, real* e_out, real* rho_out, real* vx_out, real* mpy_o, real* mmy_o, real* mpz_o, real* mmz_o, const double* vxmed, int* nshift_out, real omegaframe, int fargo) {
@@ -2448,8 +2480,9 @@ static int g_rows_y = 4, g_rows_z = 6;
 // the quantities), 0 one, -1 automatic: measured on MI355X, the split only
 // pays when a sweep has fewer waves than the GPU has SIMDs (small 2-D grids)
 static int g_colsplit = -1;
-// X-sweep kernel choice: 0 automatic, 1 ring in LDS, 2 tiles, 3 one wave per
-// ring with the cells in registers (rings of 256 cells)
+// X-sweep kernel choice: 0 automatic, 1 ring in LDS, 2 tiles in LDS, 3 one
+// wave per ring with the cells in registers (rings of 256 cells), 4 tiles of
+// one wave each with the cells in registers
 static int g_xmode = 0;
 // ring X sweep: 1 two workgroups per ring (half of the quantities each), 0
 // one; -1 automatic
@@ -2789,6 +2822,21 @@ __global__ void __launch_bounds__(64, 2) kf_sweep_xw(dmesh m, double dt64, XTab 
                               T.rho_out[fl], T.vx_out[fl], qout, T.vxmed[fl], T.nshift[fl], omegaframe, fargo);
 }
 
+// tiles of long rings, one wave each (xsweep_wave with TILE = 1, four cells
+// per lane: 244 cells updated per tile); blocks [f*nrings*ntiles, ...) take
+// fluid f, a ring's tiles consecutive
+template <int G, int DZ, int AD>
+__global__ void __launch_bounds__(64, 2) kf_sweep_xtw(dmesh m, double dt64, XTab T, int nrings, int ntiles,
+                                                      real omegaframe, int fargo) {
+  const int lin = xcd_linear();
+  const int per = nrings * ntiles;
+  const int fl = lin / per, rt = lin % per;
+  real* qout[4] = {T.qo[fl][0], T.qo[fl][1], T.qo[fl][2], T.qo[fl][3]};
+  xsweep_wave<G, DZ, AD, 4, 1>(m, rt / ntiles, dt64, T.rho_in[fl], T.vxt[fl], T.q[fl], T.e_in[fl], T.e_out[fl],
+                               T.rho_out[fl], T.vx_out[fl], qout, T.vxmed[fl], T.nshift[fl], omegaframe, fargo,
+                               rt % ntiles);
+}
+
 // ring X sweep form: the split (two workgroups per ring) and its LDS bytes
 static void x_ring_form(const f3d_hip_ctx* ctx, const dmesh& m, int* split, size_t* shx) {
   const int ad = (ctx->ph.eos == 2);
@@ -2860,6 +2908,39 @@ static int x_sweep_wave(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const R
 }
 
 // Ring X sweep of nf fluids in one launch.
+// Tiles of one wave each (kf_sweep_xtw) for the rings the LDS ring kernel
+// does not take: automatic in 2-D (the 3-D forms need more registers than
+// two waves per SIMD leave and spill), or asked for with
+// f3d_fused_set_xmode(4).
+static bool x_wtile_form(const dmesh& m) { return g_xmode == 4 || (g_xmode == 0 && !m.dimz); }
+
+static int x_sweep_wtiles(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const RF* f, const TSet* t) {
+  const dmesh& m = S->m;
+  const real of = (real)ctx->ph.OMEGAFRAME;
+  const int ad = (ctx->ph.eos == 2);
+  F3D_REQUIRE(ctx, nf >= 1 && nf <= 8, "X sweep: 1 to 8 fluids");
+  const int nrings = m.ny * (m.dimz ? m.nz : 1);
+  const int ntiles = (m.nx + 243) / 244;
+  XTab T;
+  xtab_fill(&T, nf, f, t);
+  f3d_stage_begin(ctx, F3D_ST_SWEEP_X);
+#define XTW(G_, DZ_, AD_)                                                                                     \
+  hipLaunchKernelGGL((kf_sweep_xtw<G_, DZ_, AD_>), dim3(nrings * ntiles * nf), dim3(64), 0, ctx->stream, m, dt, \
+                     T, nrings, ntiles, of, ctx->ph.fargo)
+#define XTW_G(G_, DZ_) \
+  do { if (ad) XTW(G_, DZ_, 1); else XTW(G_, DZ_, 0); } while (0)
+  if (m.geom == SPH) {
+    if (m.dimz) XTW_G(SPH, 1); else XTW_G(SPH, 0);
+  } else {
+    if (m.dimz) XTW_G(CYL, 1); else XTW_G(CYL, 0);
+  }
+#undef XTW_G
+#undef XTW
+  F3D_LAUNCH_CHECK(ctx);
+  f3d_stage_end(ctx, F3D_ST_SWEEP_X);
+  return 0;
+}
+
 static int x_sweep_ring(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const RF* f, const TSet* t, int vpre) {
   const dmesh& m = S->m;
   const real of = (real)ctx->ph.OMEGAFRAME;
@@ -2938,6 +3019,8 @@ static int transport_fluids(f3d_hip_ctx* ctx, double dt, int nf, const f3d_field
     if (x_sweep_wave(ctx, S, dt, nf, f, t)) return -1;
   } else if (!tiles) {
     if (x_sweep_ring(ctx, S, dt, nf, f, t, vpre != 0)) return -1;
+  } else if (x_wtile_form(S->m)) {
+    if (x_sweep_wtiles(ctx, S, dt, nf, f, t)) return -1;
   } else {
     for (int k = 0; k < nf; k++)
       if (x_sweep_tiled(ctx, S, dt, f[k], t[k])) return -1;
@@ -2979,7 +3062,7 @@ int F3D_API(fused_set_xtile)(int cells) {
 }
 
 int F3D_API(fused_set_xmode)(int mode) {
-  if (mode < 0 || mode > 3) return -2;
+  if (mode < 0 || mode > 4) return -2;
   g_xmode = mode;
   return 0;
 }

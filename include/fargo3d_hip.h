@@ -282,8 +282,9 @@ int f3d_fused_transport_fluids(f3d_hip_ctx *ctx, double dt, int nf, const f3d_fi
 int f3d_fused_set_tiling(int rows_y, int rows_z);
 /* X-sweep kernel: 0 automatic (one wave per ring with the cells in
  * registers for rings of 256 cells from 2048 rings x fluids on, else the
- * ring in LDS when it fits, else tiles of 128 cells with halos), 1 ring in
- * LDS, 2 tiles, 3 one wave per ring (rings of 256 cells; others as 0);
+ * ring in LDS when it fits, else tiles of 256 cells with halos, one wave
+ * each, the cells in registers), 1 ring in LDS, 2 tiles of 128 cells in LDS,
+ * 3 one wave per ring (rings of 256 cells; others as 0), 4 one-wave tiles;
  * results do not depend on it.  Returns -2 for other values. */
 int f3d_fused_set_xmode(int mode);
 /* Ring X sweep: 1 two workgroups per ring, each carrying the density and

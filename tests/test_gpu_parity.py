@@ -19,6 +19,7 @@ C2 = "fargo3d_amd/setups/c2_p3d.par"
 C2_SMALL = "Nx=64 Ny=24 Nz=12"
 C3 = "fargo3d_amd/setups/c3_adiabatic2d.par"
 C3_SMALL = "Nx=256 Ny=96"
+C4 = "fargo3d_amd/setups/c4_multifluid2d.par"
 # the 3-D disk with the adiabatic equation of state (energy transported)
 C2_AD = C2_SMALL + " Eos=adiabatic EnergyBoundary=symmetric Gamma=1.4"
 RTOL = 1e-12
@@ -342,6 +343,26 @@ def test_whole_step_fused_one_wave_x(fa, orc, par, ov, nsteps):
     assert L.f3d_fused_set_xmode(3) == 0
     try:
         _fused_run(fa, orc, par, ov, nsteps)
+    finally:
+        L.f3d_fused_set_xmode(0)
+
+
+@pytest.mark.parametrize("par,ov,nsteps", [(C3, C3_SMALL, 10), (C3, "Nx=1000 Ny=24", 6), (C3, "Nx=2500 Ny=48", 4),
+                                           (C1, "", 8), (C2, C2_SMALL, 4), (C2, C2_AD, 4), (C4, "Nx=128 Ny=64", 4)],
+                         ids=["c3_2d_adiabatic_256", "c3_2d_adiabatic_1000", "c3_2d_adiabatic_2500", "c1_2d_384",
+                              "c2_3d_64", "c2_3d_adiabatic_64", "c4_five_fluids_128"])
+def test_whole_step_fused_one_wave_tiles(fa, orc, par, ov, nsteps):
+    """The X sweep by one-wave tiles (256 cells in registers, 244 updated,
+    neighbours across lanes by wave rotations, the last tile of a ring
+    partial, rings shorter than a tile covered more than once) gives the
+    oracle's results bit for bit."""
+    L = fa.lib()
+    assert L.f3d_fused_set_xmode(4) == 0
+    try:
+        o, s = _fused_run(fa, orc, par, ov, nsteps)
+        for f in fields_of(s):
+            assert nbits(active(s.field(f), s.dimz), active(o.field(f), s.dimz)) == 0, (par, ov, f)
+        s.close()
     finally:
         L.f3d_fused_set_xmode(0)
 
