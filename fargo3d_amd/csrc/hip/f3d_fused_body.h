@@ -2913,6 +2913,16 @@ static int x_sweep_wave(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const R
 // two waves per SIMD leave and spill), or asked for with
 // f3d_fused_set_xmode(4).
 static bool x_wtile_form(const dmesh& m) { return g_xmode == 4 || (g_xmode == 0 && !m.dimz); }
+// ... and in place of the LDS ring kernel where the tiles make enough waves:
+// measured on MI355X, gas + 4 dust 1024x512 (12,800 waves) FP64 step 0.442 ->
+// 0.383 ms, FP32 0.299 -> 0.284 ms; the 384x128 disk (256 waves) 0.064 ->
+// 0.091 ms, so from 2048 waves on, in 2-D
+static bool x_wtile_pays(const dmesh& m, int nf) {
+  if (g_xmode == 4) return true;
+  if (g_xmode != 0 || m.dimz) return false;
+  const long ntiles = (m.nx + 243) / 244;
+  return (long)m.ny * ntiles * nf >= 2048;
+}
 
 static int x_sweep_wtiles(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const RF* f, const TSet* t) {
   const dmesh& m = S->m;
@@ -3001,8 +3011,9 @@ static int transport_fluids(f3d_hip_ctx* ctx, double dt, int nf, const f3d_field
   }
   const bool wave = x_wave_form(S->m, nf);
   const bool tiles = !wave && !x_ring_fits(ctx, S);
-  // the one-wave sweep sums VxMed itself; the tiled one takes it from the pass
-  const int vpre = (ctx->ph.fargo && !wave) ? x_vmed_form(S->m, nf, tiles) : 0;
+  const bool wtiles = !wave && (tiles ? x_wtile_form(S->m) : x_wtile_pays(S->m, nf));
+  // the one-wave sweep sums VxMed itself; the tiled ones take it from the pass
+  const int vpre = (ctx->ph.fargo && !wave) ? x_vmed_form(S->m, nf, tiles || wtiles) : 0;
   if (vpre == 1) {
     // VxMed of each fluid's Vx_temp (one long sequential sum per ring) on the
     // side stream, overlapping the Z/Y sweeps, which do not write Vx_temp
@@ -3017,10 +3028,10 @@ static int transport_fluids(f3d_hip_ctx* ctx, double dt, int nf, const f3d_field
   if (vpre == 2 && x_vmed(ctx, S, nf, f, t, ctx->stream)) return -1;
   if (wave) {
     if (x_sweep_wave(ctx, S, dt, nf, f, t)) return -1;
+  } else if (wtiles) {
+    if (x_sweep_wtiles(ctx, S, dt, nf, f, t)) return -1;
   } else if (!tiles) {
     if (x_sweep_ring(ctx, S, dt, nf, f, t, vpre != 0)) return -1;
-  } else if (x_wtile_form(S->m)) {
-    if (x_sweep_wtiles(ctx, S, dt, nf, f, t)) return -1;
   } else {
     for (int k = 0; k < nf; k++)
       if (x_sweep_tiled(ctx, S, dt, f[k], t[k])) return -1;
