@@ -669,10 +669,11 @@ __global__ void __launch_bounds__(64) kf_src_fused2(dmesh m, real dt, const real
   vp.model = VM == 1 ? 1 : 2;
   vp.iso = VM != 3;
   const int lane = (int)threadIdx.x;
-  const int is = kSrcCols * (int)blockIdx.x - 1 + lane;  // this lane's stress column
+  const dim3 b = xcd_bid();
+  const int is = kSrcCols * (int)b.x - 1 + lane;  // this lane's stress column
   const int i = is < 0 ? is + m.nx : (is >= m.nx ? is - m.nx : is);
   const bool upd = lane >= 1 && lane <= kSrcCols && is < m.nx;
-  const int r0 = m.nghy + (int)blockIdx.y * rows_per;
+  const int r0 = m.nghy + (int)b.y * rows_per;
   const int r1 = min(r0 + rows_per, jend);
   // visctensor (src/visctensor_cyl.c:140-186, src/visctensor_sph.c:126-206)
   // at (i, row): the same loaders as kd_src_a
