@@ -2499,12 +2499,14 @@ static int g_srcfuse = -1;
 
 // CFL form, bits: 1 ring minima to an array and a second pass (else one
 // atomic MIN per ring), 2 VxMed from kf_vmed_rings, 4 (with 2) one wave per
-// workgroup.  Measured on MI355X: the array pays from about 2048 rings, the
+// workgroup.  Measured on MI355X: the array and its MIN pass, which hands
+// dt to the host itself, pay on every mesh (the atomic form's copy and event
+// cost more: 384x128 step 0.0609 -> 0.0561 ms, 4096x1024 alike); the VxMed
 // pass for many or long rings, one wave per ring for rings of <= 256 cells.
 static int cfl_mode(const dmesh& m, int nf) {
   if (g_cflmode >= 0) return g_cflmode;
   const int nrings = m.ny * (m.dimz ? m.nz : 1) * nf;
-  int mode = (nrings >= 2048 ? 3 : 0) | (m.nx >= 1024 ? 2 : 0);
+  int mode = 1 | (nrings >= 2048 ? 2 : 0) | (m.nx >= 1024 ? 2 : 0);
   if ((mode & 2) && m.nx <= 256) mode |= 4;
   return mode;
 }
