@@ -435,7 +435,7 @@ static int batch_layout(f3d_batch* B) {
 static void batch_launch(f3d_batch* B) {
   const int n = B->n;
   hipStream_t s = B->stream;
-  const size_t cfl_lds = (2 * (size_t)B->nxmax + 16) * sizeof(double);
+  const size_t cfl_lds = 16 * sizeof(double);
   if (B->total[BK_POT])
     hipLaunchKernelGGL(kb_potential, dim3(B->total[BK_POT]), dim3(BXF), 0, s, B->dI, B->doff[BK_POT], n);
   hipLaunchKernelGGL(kb_cfl, dim3(B->total[BK_CFL]), dim3(WG), cfl_lds, s, B->dI, B->doff[BK_CFL], n);

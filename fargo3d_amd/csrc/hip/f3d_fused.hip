@@ -86,7 +86,7 @@ extern "C" int f3d_fused_cfl_fluids_noalloc(f3d_hip_ctx* ctx, int nf, double* co
     T.ringmin[k] = sp + (size_t)k * nrings;
   }
   const int wg = (mode & 4) ? 64 : WG;
-  const size_t shmem = pre ? 16 * sizeof(double) : (2 * m.nx + 16) * sizeof(double);
+  const size_t shmem = 16 * sizeof(double);
   if (pre)
     FDISPATCH_T(ctx, kf_cfl_fluids, 1, dim3(nrings * nf), dim3(wg), shmem, T, nrings, vxm, ctx->f.VxMed, result, 0,
                 cfl_par_f(ctx));

@@ -224,67 +224,12 @@ __device__ __forceinline__ dim3 xcd_bid() {
   return dim3(p % gridDim.x, (p / gridDim.x) % gridDim.y, p / (gridDim.x * gridDim.y));
 }
 
-// The weights v_i (x_{i+1} - x_i) of one ring (ComputeVweight,
-// src/compute_vmed.c:14-21) into w[0, n), in FP64.  The loads of eight cells
-// per thread are issued before any is used (indexes clamped into the ring, so
-// no branch separates them); a guarded loop would wait for each in turn.
-__device__ __forceinline__ void ring_weights(double* w, const real* v, const double* x, int n) {
-  const int t = (int)threadIdx.x, nt = (int)blockDim.x;
-  for (int i0 = 0; i0 < n; i0 += 8 * nt) {
-    double a[8], x0[8], x1[8];
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int i = min(i0 + t + u * nt, n - 1);
-      a[u] = (double)v[i];
-      x0[u] = x[i];
-      x1[u] = x[i + 1];
-    }
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int i = i0 + t + u * nt;
-      if (i < n) w[i] = a[u] * (x1[u] - x0[u]);
-    }
-  }
-}
-
-// Sequential sum in index order (the reduction order of src/reduction_generic.c
-// :18-20), in FP64.  Software-pipelined: the LDS loads of the next 16 terms
-// are issued before the 16 dependent additions of the current ones, so the
-// chain of additions, not the LDS latency, sets the pace.
-__device__ __forceinline__ double seq_sum(const double* w, int n) {
-  constexpr int B = 16;
-  double acc = 0.0;
-  int i = 0;
-  if (n >= 2 * B) {
-    // ping-pong between two register batches: a holds terms [i-B, i)
-    double a[B], b[B];
-#pragma unroll
-    for (int q = 0; q < B; q++) a[q] = w[q];
-    for (i = B; i + 2 * B <= n; i += 2 * B) {
-      // the scheduling barriers keep each batch's loads ahead of the other
-      // batch's additions
-#pragma unroll
-      for (int q = 0; q < B; q++) b[q] = w[i + q];
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int q = 0; q < B; q++) acc = acc + a[q];
-#pragma unroll
-      for (int q = 0; q < B; q++) a[q] = w[i + B + q];
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int q = 0; q < B; q++) acc = acc + b[q];
-    }
-#pragma unroll
-    for (int q = 0; q < B; q++) acc = acc + a[q];
-  }
-  for (; i < n; i++) acc = acc + w[i];
-  return acc;
-}
-
-// The same sequential sum of a ring's weights w_i = v_i (x_{i+1} - x_i)
-// (src/compute_vmed.c:14-21), i in [0, n), by one wave (blockDim.x == 64)
-// with the terms in registers.  The ring goes in chunks of 64*S terms, lane L
-// holding terms [c + L*S, c + (L+1)*S) of the chunk at c.  The running sum
+// The sequential sum in index order (the reduction order of
+// src/reduction_generic.c:18-20), in FP64, of a ring's weights
+// w_i = v_i (x_{i+1} - x_i) (src/compute_vmed.c:14-21), i in [0, n), by one
+// wave (all 64 lanes taking part) with the terms in registers.  The ring goes
+// in chunks of 64*S terms, lane L holding terms [c + L*S, c + (L+1)*S) of the
+// chunk at c.  The running sum
 // passes from lane to lane: every lane adds its own S terms to the sum so far
 // (one chain of dependent register additions), and lane L's result -- the
 // sequential sum through its last term -- is read out (v_readlane) as the
@@ -311,7 +256,7 @@ __device__ __forceinline__ double wave_chain(const double (&w)[S], double acc, i
 template <int S>
 __device__ __forceinline__ double wave_ring_sum(const real* __restrict__ v, const double* __restrict__ x, int n) {
 #pragma clang fp contract(off)
-  const int lane = (int)threadIdx.x;
+  const int lane = (int)threadIdx.x & 63;
   double a[S], x0[S + 1];
   auto load = [&](int c) {
 #pragma unroll
@@ -352,10 +297,8 @@ __device__ __forceinline__ double vmed_full(double stored_mean, const dmesh& m, 
 // hence a template parameter).
 template <int G, int DZ, int PRE, int FX = 0>
 __device__ __forceinline__ void kd_cfl(int ring, const dmesh& m, const real* cs, const real* rho, const real* vx, const real* vy, const real* vz, const real* vxm, double* vxmed, unsigned long long* result, int slot, cell::CflPar cp, double* ringmin) {
-  extern __shared__ double lds[];  // 2 nx weights + 16 (16 with pre)
-  double* w0 = lds;
-  double* w1 = lds + m.nx;
-  double* sh = PRE ? lds : lds + 2 * m.nx;
+  extern __shared__ double lds[];  // 16
+  double* sh = lds;
   const int j = m.nghy + ring % m.ny;
   const int k = DZ ? m.nghz + ring / m.ny : 0;
   const int base = j * m.pitch + k * m.stride;
@@ -370,12 +313,17 @@ __device__ __forceinline__ void kd_cfl(int ring, const dmesh& m, const real* cs,
   } else {
     // ComputeVweight + reduction_SUM (src/compute_vmed.c:14-30) of rings j and j+1
     // of vxm (the velocity whose VxMed the reference's cfl reads: with several
-    // fluids, the last fluid's, src/main.c:366 then :381)
-    ring_weights(w0, vxm + base, m.xmin64, m.nx);
-    if (cp.fargo) ring_weights(w1, vxm + base + m.pitch, m.xmin64, m.nx);
-    __syncthreads();
-    if (threadIdx.x == 0) sh[8] = vmed_full(seq_sum(w0, m.nx) / (m.XMAX - m.XMIN), m, j, k);
-    if (threadIdx.x == 64 && cp.fargo) sh[9] = vmed_full(seq_sum(w1, m.nx) / (m.XMAX - m.XMIN), m, j + 1, k);
+    // fluids, the last fluid's, src/main.c:366 then :381): the first wave sums
+    // ring j, the second ring j+1, each with its terms in registers
+    // (wave_ring_sum; this form runs with workgroups of four waves)
+    const int wv = (int)threadIdx.x >> 6;
+    if (wv == 0) {
+      const double s0 = wave_ring_sum<4>(vxm + base, m.xmin64, m.nx);
+      if (threadIdx.x == 0) sh[8] = vmed_full(s0 / (m.XMAX - m.XMIN), m, j, k);
+    } else if (wv == 1 && cp.fargo) {
+      const double s1 = wave_ring_sum<4>(vxm + base + m.pitch, m.xmin64, m.nx);
+      if (threadIdx.x == 64) sh[9] = vmed_full(s1 / (m.XMAX - m.XMIN), m, j + 1, k);
+    }
   }
   __syncthreads();
   const double vm0 = sh[8];
@@ -1175,11 +1123,10 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
   const real dt = (real)dt64;
   const int P = (N * nx + 1) & ~1;  // N * nx rounded up to 8-byte multiples
   real* AQ = (real*)lds_d;   // q = Q/rho (or rho): N * nx
-  real* SL = AQ + P;         // slopes, then limited qL: N * nx (FP64 ring weights before the passes)
+  real* SL = AQ + P;         // slopes, then limited qL: N * nx
   real* FC = SL + P;         // faces, then qR, then face fluxes: N * nx
   real* V = FC + P;          // advection velocity: nx
   double* sh = (double*)(((size_t)(V + nx) + 15) & ~(size_t)15);  // 4
-  double* Wt = (double*)SL;
   const int j = m.nghy + ring % m.ny;
   const int k = DZ ? m.nghz + ring / m.ny : 0;
   const int base = j * m.pitch + k * m.stride;
@@ -1198,7 +1145,6 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
         st[s][c] = (q < NM) ? qin[q][base + i] : e_in[base + i];
       }
       vt[s] = vxt[base + i];
-      if (!VPRE) Wt[i] = vt[s] * (m.xmin64[i + 1] - m.xmin64[i]);
     }
   }
   double vmed;
@@ -1206,9 +1152,12 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
     // VxMed of this ring from kf_vmed, run before this launch
     vmed = vxmed_out[j + k * m.sy];
   } else {
-    __syncthreads();
-    // ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30), FP64
-    if (t == 0) sh[0] = vmed_full(seq_sum(Wt, nx) / (m.XMAX - m.XMIN), m, j, k);
+    // ComputeVmed(Vx_temp) (src/compute_vmed.c:14-30), FP64, by the first
+    // wave with the terms in registers (wave_ring_sum)
+    if (t < 64) {
+      const double sum = wave_ring_sum<4>(vxt + base, m.xmin64, nx);
+      if (t == 0) sh[0] = vmed_full(sum / (m.XMAX - m.XMIN), m, j, k);
+    }
     __syncthreads();
     vmed = sh[0];
   }
@@ -1234,7 +1183,7 @@ __device__ __forceinline__ void xsweep_ring(const dmesh& m, int ring, double dt6
       }
     }
   }
-  __syncthreads();  // V complete; the ring weights in SL are no longer read
+  __syncthreads();  // V complete
   if (fargo) {
     ns = (int)sh[1];
     if (GRP != 2 && t == 0) {
@@ -3111,7 +3060,6 @@ int F3D_API(fused_cfl_vm)(f3d_hip_ctx* ctx, const double* vxm_in, double* fluid_
   const dmesh& m = S->m;
   const RF f = rfields(ctx);
   const int nrings = m.ny * (m.dimz ? m.nz : 1);
-  const size_t shmem = (2 * m.nx + 16) * sizeof(double);
   const int slot = S->slot;
   f3d_stage_begin(ctx, F3D_ST_CFL);
   const int mode = cfl_mode(m, 1);
@@ -3120,7 +3068,7 @@ int F3D_API(fused_cfl_vm)(f3d_hip_ctx* ctx, const double* vxm_in, double* fluid_
   if (pre && vmed_prepass(ctx, S, vxm ? vxm : f.Vx)) return -1;
   const int wg = (mode & 4) ? 64 : WG;
 #define CFL1(P_)                                                                                                 \
-  FDISPATCH_T(ctx, kf_cfl, P_, dim3(nrings), dim3(wg), P_ ? 16 * sizeof(double) : shmem, f.Energy, f.Density, \
+  FDISPATCH_T(ctx, kf_cfl, P_, dim3(nrings), dim3(wg), 16 * sizeof(double), f.Energy, f.Density,            \
               f.Vx, f.Vy, f.Vz, vxm ? vxm : f.Vx, f.VxMed, S->result, slot, cfl_par_f(ctx), rm)
   if (pre) CFL1(1);
   else CFL1(0);
@@ -3150,7 +3098,6 @@ int F3D_API(fused_cfl_fluids_async)(f3d_hip_ctx* ctx, int nf, double* const* cs,
   const dmesh& m = S->m;
   const RF f = rfields(ctx);
   const int nrings = m.ny * (m.dimz ? m.nz : 1);
-  const size_t shmem = (2 * m.nx + 16) * sizeof(double);
   const int slot = S->slot;
   const real* vxm = reinterpret_cast<const real*>(vxm_in);
   auto R = [](double* p) { return reinterpret_cast<real*>(p); };
@@ -3170,7 +3117,7 @@ int F3D_API(fused_cfl_fluids_async)(f3d_hip_ctx* ctx, int nf, double* const* cs,
   }
   const int wg = (mode & 4) ? 64 : WG;
 #define CFLF(P_)                                                                                             \
-  FDISPATCH_T(ctx, kf_cfl_fluids, P_, dim3(nrings * nf), dim3(wg), (P_ & 1) ? 16 * sizeof(double) : shmem, T, \
+  FDISPATCH_T(ctx, kf_cfl_fluids, P_, dim3(nrings * nf), dim3(wg), 16 * sizeof(double), T,                  \
               nrings, vxm, f.VxMed, S->result, slot, cfl_par_f(ctx))
   // the flag set of the common configurations at compile time (FARGO:
   // isothermal with constant nu or alpha, adiabatic with constant nu)
