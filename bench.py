@@ -73,7 +73,7 @@ def stage_arrays(stage, dz, ad, alpha):
 STAGE_KERNEL = {"cfl": ["kf_cfl", "kf_cfl_fluids", "kf_vmed_rings", "kf_cfl_fin"], "sources_a": ["kf_src_a"],
                 "sources_b": ["kf_src_b", "kf_src_b_fluids", "kf_src_fused2"], "sweep_z": ["kf_sweep_z"], "sweep_y": ["kf_sweep_y2", "kf_sweep_y3"],
                 "sweep_x": ["kf_sweep_xw", "kf_sweep_xtw", "kf_sweep_xr_fluids", "kf_sweep_x_tile"], "finish": ["kf_finish"],
-                "potential": ["k_potential", "kf_potential"], "substep3": ["kf_substep3"]}
+                "potential": ["k_potential", "k_potential_fin", "kf_potential"], "substep3": ["kf_substep3"]}
 # rocprofv3 summary with the measured HBM traffic of each kernel of this workload
 PROFILE = {"c2": "profiles/fused_step_c2_3d_256x96x32.md", "c1": "profiles/fused_step_c1_2d_384x128.md",
            "c3": "profiles/fused_step_c3_2d_adiabatic_4096x1024.md",

@@ -139,6 +139,40 @@ __device__ __forceinline__ double zone_size_z64(const dmesh& m, int j, int k) {
   return m.zmin64[k + 1] - m.zmin64[k];
 }
 
+// The MIN of the CFL's n ring minima by one workgroup (blockDim.x a multiple
+// of 64, at most 1024; MIN is order independent, so this equals one atomic
+// MIN per ring) into result[slot], and, when host_out is given, straight into
+// pinned host memory: the value, then, after a system-scope fence, the
+// sequence number the host polls for.  sh: blockDim.x / 64 doubles of LDS.
+__device__ inline void cfl_min_publish(const double* ringmin, int n, unsigned long long* result, int slot,
+                                       double* host_out, unsigned long long seq, double* sh) {
+  double v = __longlong_as_double(0x7FF0000000000000ll);  // +inf
+  // eight loads per thread issued before any is compared; an index past the
+  // end reads the last minimum again, which leaves the MIN unchanged
+  for (int i0 = threadIdx.x; i0 < n; i0 += 8 * blockDim.x) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) x[u] = ringmin[min(i0 + u * (int)blockDim.x, n - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v = (x[u] < v) ? x[u] : v;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double w = __shfl_down(v, o, 64);
+    v = (w < v) ? w : v;
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) v = (sh[w] < v) ? sh[w] : v;
+    result[slot] = (unsigned long long)__double_as_longlong(v);
+    if (host_out) {
+      host_out[0] = v;
+      __threadfence_system();
+      *reinterpret_cast<volatile unsigned long long*>(host_out + 1) = seq;
+    }
+  }
+}
+
 // van Leer harmonic slope (src/vanleer_y_a.c:61-68): 0 at extrema.  The
 // quotient is formed in every lane and the extremum case selected after it
 // (the same value in both cases): a branch around the division would split

@@ -401,28 +401,7 @@ __global__ void __launch_bounds__(WG) kf_cfl_fluids(dmesh m, CflTab T, int nring
 __global__ void __launch_bounds__(1024) kf_cfl_fin(const double* ringmin, int n, unsigned long long* result,
                                                    int slot, double* host_out, unsigned long long seq) {
   __shared__ double sh[16];
-  double v = __longlong_as_double((long long)kInfBits);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const double x = ringmin[i];
-    v = (x < v) ? x : v;
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    const double w = __shfl_down(v, o, 64);
-    v = (w < v) ? w : v;
-  }
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < (int)(blockDim.x >> 6); w++) v = (sh[w] < v) ? sh[w] : v;
-    result[slot] = (unsigned long long)__double_as_longlong(v);
-    // straight into pinned host memory, with no copy launch: the value, then
-    // (after a system-scope fence) the sequence number the host polls for
-    if (host_out) {
-      host_out[0] = v;
-      __threadfence_system();
-      *reinterpret_cast<volatile unsigned long long*>(host_out + 1) = seq;
-    }
-  }
+  cfl_min_publish(ringmin, n, result, slot, host_out, seq, sh);
 }
 
 // ------------------------------------------------------------------ sources
@@ -2421,11 +2400,19 @@ static int fused_boundaries(f3d_hip_ctx* ctx, Scratch* S, real* dens, real* vx, 
   return 0;
 }
 
-// tuning knobs of the column sweeps (rows per thread, f3d_fused_set_tiling).
-// Measured on the 256x96x32 disk: 4 rows in Y; 6 planes in Z (the 32 planes
-// then take 6 workgroup rows, one wave of launches at 3 waves per SIMD:
-// 42.6 -> 41.3 us against 3 planes)
-static int g_rows_y = 4, g_rows_z = 6;
+// tuning knobs of the column sweeps (rows per thread, f3d_fused_set_tiling;
+// rows_y 0: automatic).  Measured on MI355X: 4 rows in Y on the 256x96x32,
+// 4096x1024 and 1024x512 disks (2 rows: 4096x1024 step 0.554 -> 0.579 ms),
+// 2 rows on the 384x128 disk (0.0587 -> 0.0571 ms), so 2 rows where 4 would
+// leave the launch under 1024 waves; 6 planes in Z (the 32 planes then take
+// 6 workgroup rows, one wave of launches at 3 waves per SIMD: 42.6 -> 41.3 us
+// against 3 planes)
+static int g_rows_y = 0, g_rows_z = 6;
+static int rows_y_of(const dmesh& m, int nf) {
+  if (g_rows_y > 0) return g_rows_y;
+  const long w4 = (long)((m.nx + BXF - 1) / BXF) * (m.dimz ? m.nz : nf) * ((m.ny + 3) / 4);
+  return w4 < 1024 ? 2 : 4;
+}
 // column sweeps: 1 two threads per column (each with the density and half of
 // the quantities), 0 one, -1 automatic: measured on MI355X, the split only
 // pays when a sweep has fewer waves than the GPU has SIMDs (small 2-D grids)
@@ -2588,7 +2575,7 @@ static int yz_sweeps(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const RF* 
   const dmesh& m = S->m;
   const real of = (real)ctx->ph.OMEGAFRAME;
   const real rdt = (real)dt;
-  const int ry = g_rows_y, rz = g_rows_z;
+  const int ry = rows_y_of(m, nf), rz = g_rows_z;
   const long waves = (long)((m.nx + BXF - 1) / BXF) * (m.dimz ? (long)m.nz * ((m.ny + ry - 1) / ry)
                                                                : (long)nf * ((m.ny + ry - 1) / ry));
   const int split = g_colsplit >= 0 ? g_colsplit : (waves < 1024 ? 1 : 0);
@@ -3043,7 +3030,7 @@ int F3D_API(fused_set_vmedterms)(int terms) {
 }
 
 int F3D_API(fused_set_tiling)(int rows_y, int rows_z) {
-  if (rows_y < 1 || rows_z < 1) return -2;
+  if (rows_y < 0 || rows_z < 1) return -2;
   g_rows_y = rows_y;
   g_rows_z = rows_z;
   return 0;
@@ -3087,6 +3074,17 @@ int F3D_API(fused_cfl_vm)(f3d_hip_ctx* ctx, const double* vxm_in, double* fluid_
   return 0;
 }
 
+// Launches the queued CFL's MIN pass if no potential launch has carried it.
+static int cfl_fin_flush(f3d_hip_ctx* ctx) {
+  if (!ctx->fin_pending) return 0;
+  ctx->fin_pending = 0;
+  const f3d_hip_ctx::CflFin& F = ctx->fin;
+  hipLaunchKernelGGL(kf_cfl_fin, dim3(1), dim3(1024), 0, ctx->stream, F.ringmin, F.n, F.result, F.slot, ctx->dhres,
+                     ctx->cfl_seq);
+  F3D_LAUNCH_CHECK(ctx);
+  return 0;
+}
+
 // cfl of nf fluids with one read-back: every launch takes its MIN into the
 // same slot (MIN is order independent, so the result is CflFluidsMin's).
 // The launches are queued here; fused_cfl_wait reads the step back.
@@ -3095,6 +3093,7 @@ int F3D_API(fused_cfl_fluids_async)(f3d_hip_ctx* ctx, int nf, double* const* cs,
   Scratch* S;
   F3D_REQUIRE(ctx, scratch_of(ctx, &S) == 0, "fused scratch allocation failed");
   F3D_REQUIRE(ctx, nf >= 1 && nf <= 8, "cfl: 1 to 8 fluids");
+  if (cfl_fin_flush(ctx)) return -1;  // a CFL queued before and not waited for
   const dmesh& m = S->m;
   const RF f = rfields(ctx);
   const int nrings = m.ny * (m.dimz ? m.nz : 1);
@@ -3136,10 +3135,13 @@ int F3D_API(fused_cfl_fluids_async)(f3d_hip_ctx* ctx, int nf, double* const* cs,
 #undef CFLF
   ctx->cfl_spin = 0;
   if (mode & 1) {
+    // the MIN pass rides on the next launch of the potential (which needs
+    // no dt; f3d_Potential), else fused_cfl_wait launches it: one launch
+    // fewer per step
     ctx->cfl_spin = 1;
-    hipLaunchKernelGGL(kf_cfl_fin, dim3(1), dim3(1024), 0, ctx->stream, S->ringmin, nf * nrings, S->result, slot,
-                       ctx->dhres, ++ctx->cfl_seq);
-    F3D_LAUNCH_CHECK(ctx);
+    ++ctx->cfl_seq;
+    ctx->fin = f3d_hip_ctx::CflFin{S->ringmin, nf * nrings, slot, S->result};
+    ctx->fin_pending = 1;
   }
   f3d_stage_end(ctx, F3D_ST_CFL);
   // the atomic form copies the step to the host and records an event for
@@ -3159,6 +3161,7 @@ int F3D_API(fused_cfl_fluids_async)(f3d_hip_ctx* ctx, int nf, double* const* cs,
 // (work queued behind them may still run).
 int F3D_API(fused_cfl_wait)(f3d_hip_ctx* ctx, double* step_min) {
   F3D_REQUIRE(ctx, ctx->cfl_queued, "cfl_wait: no CFL queued");
+  if (cfl_fin_flush(ctx)) return -1;
   ctx->cfl_queued = 0;
   if (ctx->cfl_spin) {
     // poll the sequence number the last CFL kernel writes after dt: the

@@ -42,6 +42,15 @@ struct f3d_hip_ctx {
   unsigned long long cfl_seq = 0;
   int cfl_spin = 0;
   int cfl_queued = 0;  // a fused CFL is queued and not yet waited for
+  // the queued CFL's MIN pass over its ring minima, not yet launched: the
+  // next potential launch carries it (f3d_Potential), else the wait launches
+  // it; ringmin[0, n) -> result[slot], then hres with sequence number cfl_seq
+  struct CflFin {
+    const double* ringmin;
+    int n, slot;
+    unsigned long long* result;
+  } fin{};
+  int fin_pending = 0;
   int device = 0;
   f3d::dmesh m{};
   double* meshbuf = nullptr;        // one device buffer for all 1-D mesh arrays

@@ -195,6 +195,19 @@ __global__ void k_potential(dmesh m, Range r, double* pot, cell::PlanetSet ps) {
   CELL_INDEX;
   pot[l] = cell::potential<G>(m, i, j, k, ps);
 }
+// the same, its first workgroup also taking the queued fused CFL's MIN pass
+// (which would otherwise be a launch of its own; the potential needs no dt)
+template <int G, int DZ>
+__global__ void __launch_bounds__(BX) k_potential_fin(dmesh m, Range r, double* pot, cell::PlanetSet ps,
+                                                      f3d_hip_ctx::CflFin fin, double* host_out,
+                                                      unsigned long long seq) {
+  if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) {
+    __shared__ double sh[BX / 64];
+    cfl_min_publish(fin.ringmin, fin.n, fin.result, fin.slot, host_out, seq, sh);
+  }
+  CELL_INDEX;
+  pot[l] = cell::potential<G>(m, i, j, k, ps);
+}
 
 template <int G, int DZ>
 __global__ void k_correct_vtheta(dmesh m, Range r, double* vt, double domega) {
@@ -832,7 +845,13 @@ int f3d_Potential(f3d_hip_ctx* ctx) {
   F3D_REQUIRE(ctx, ctx->ph.nplanets <= cell::kMaxPlanets, "Potential: too many planets");
   Range r = rng(ctx, 0, ctx->m.sy, 0, ctx->m.sz);
   f3d_stage_begin(ctx, F3D_ST_POTENTIAL);
-  DISPATCH(ctx, k_potential, r, ctx->f.Pot, planet_set(ctx));
+  if (ctx->fin_pending && !empty(r)) {
+    // the fused CFL queued before this launch left its MIN pass to it
+    ctx->fin_pending = 0;
+    DISPATCH(ctx, k_potential_fin, r, ctx->f.Pot, planet_set(ctx), ctx->fin, ctx->dhres, ctx->cfl_seq);
+  } else {
+    DISPATCH(ctx, k_potential, r, ctx->f.Pot, planet_set(ctx));
+  }
   f3d_stage_end(ctx, F3D_ST_POTENTIAL);
   return 0;
 }

@@ -277,8 +277,9 @@ int f3d_fused_transport(f3d_hip_ctx *ctx, double dt, int end_of_step);
  * from the handle's bound set): every fluid's sweeps, with the X sweep of all
  * fluids in one launch.  VxMed and Nshift end with the last fluid's values. */
 int f3d_fused_transport_fluids(f3d_hip_ctx *ctx, double dt, int nf, const f3d_fields *fl);
-/* Rows (Y sweep) and planes (Z sweep) one thread marches through; a tuning
- * knob that does not change results.  Returns -2 for values below 1. */
+/* Rows (Y sweep; 0: automatic, the default) and planes (Z sweep) one thread
+ * marches through; a tuning knob that does not change results.  Returns -2
+ * for rows_y below 0 or rows_z below 1. */
 int f3d_fused_set_tiling(int rows_y, int rows_z);
 /* X-sweep kernel: 0 automatic (one wave per ring with the cells in
  * registers for rings of 256 cells from 2048 rings x fluids on, else the
