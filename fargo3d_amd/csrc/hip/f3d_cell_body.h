@@ -832,18 +832,23 @@ __device__ __forceinline__ GhostCell ghost_z_load(const dmesh& m, int side, int 
   return c;
 }
 
+// the azimuthal velocity of Z ghost layer mm (plane gh) from x0, the stored
+// velocity of its active plane act
+__device__ __forceinline__ real ghost_z_vx(const dmesh& m, const BcPar& b, int mm, int j, int act, int gh, real x0) {
+  if constexpr (kFix64) {
+    const double of = b.of, xa = (double)x0 + vbase(m, j, act);
+    return (real)((xa + of * m.ymed64[j] * m.sinzmed64[act]) * (double)b.zvfac[mm] -
+                  m.ymed64[j] * of * m.sinzmed64[gh] - vbase(m, j, gh));
+  } else {
+    const real of = b.of;
+    return (x0 + of * m.ymed[j] * m.sinzmed[act]) * b.zvfac[mm] - m.ymed[j] * of * m.sinzmed[gh];
+  }
+}
+
 __device__ __forceinline__ void ghost_z_store(const dmesh& m, int mm, int j, const GhostCell& c, real* dens, real* vx,
                                               real* vy, real* vz, real* e, const BcPar& b, const real* zdens) {
   const real dg = c.d0 * zdens[mm * m.sy + j];
-  real xg;
-  if constexpr (kFix64) {
-    const double of = b.of, xa = (double)c.x0 + vbase(m, j, c.act);
-    xg = (real)((xa + of * m.ymed64[j] * m.sinzmed64[c.act]) * (double)b.zvfac[mm] -
-                m.ymed64[j] * of * m.sinzmed64[c.gh] - vbase(m, j, c.gh));
-  } else {
-    const real of = b.of;
-    xg = (c.x0 + of * m.ymed[j] * m.sinzmed[c.act]) * b.zvfac[mm] - m.ymed[j] * of * m.sinzmed[c.gh];
-  }
+  const real xg = ghost_z_vx(m, b, mm, j, c.act, c.gh, c.x0);
   dens[c.lgh] = dg;
   vx[c.lgh] = xg;
   vy[c.lgh] = c.y0;

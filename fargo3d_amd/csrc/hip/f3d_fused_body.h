@@ -1932,12 +1932,64 @@ struct FinTab {
   const real* vy0[8];
   const real* vz0[8];
 };
+// The end-of-step Z ghost fill (bound_z_cell, both sides) inside the
+// finishing kernel: the thread that finishes plane k of column (i, j) writes
+// the ghost values that plane is the source of, from its own final values.
+// Lower side: ghost plane mm takes rho, vx, vy (and e) of plane
+// 2 nghz - 1 - mm and vz of plane 2 nghz - mm, negated; upper side: ghost
+// plane nz + nghz + mm takes all of plane nz + nghz - 1 - mm (vz one plane
+// further out, for mm < nghz - 1); the boundary faces vz(nghz) and
+// vz(nz + nghz) become zero.  Every source cell is active and written only by
+// its own thread, so no thread reads what another writes.  What the separate
+// Z fill also wrote in the Y ghost rows of the Z ghost planes, the Y fill that
+// follows rewrites in full (rho, vx, vz, e of every Y ghost row; vy of the
+// ghost faces and the boundary face).
+struct ZFill {
+  int on;
+  real* e[8];
+  cell::BcPar b0, b1;
+  const real* zd0;  // density factors of the lower and upper ghost layers
+  const real* zd1;
+};
+__device__ __forceinline__ void zfill_from_plane(const dmesh& m, int i, int j, int k, real* dens, real* vx, real* vy,
+                                                 real* vz, real* e, const ZFill& z) {
+  const int nghz = m.nghz, nz = m.nz, stride = m.stride;
+  const int col = i + j * m.pitch, l = col + k * stride;
+  const int mlo = 2 * nghz - 1 - k;  // lower ghost plane whose rho, vx, vy, e this plane gives
+  if (mlo >= 0 && mlo < nghz) {
+    const int lg = col + mlo * stride;
+    dens[lg] = dens[l] * z.zd0[mlo * m.sy + j];
+    vx[lg] = cell::ghost_z_vx(m, z.b0, mlo, j, k, mlo, vx[l]);
+    vy[lg] = vy[l];
+    if (z.b0.energy) e[lg] = e[l];
+  }
+  const int mlz = 2 * nghz - k;  // lower ghost plane whose vz this plane gives
+  if (mlz >= 0 && mlz < nghz) vz[col + mlz * stride] = -vz[l];
+  if (k == nghz) vz[l] = 0;  // after this thread's own store there
+  const int mup = nz + nghz - 1 - k;  // upper ghost layer this plane gives
+  if (mup >= 0 && mup < nghz) {
+    const int gh = nz + nghz + mup, lg = col + gh * stride;
+    dens[lg] = dens[l] * z.zd1[mup * m.sy + j];
+    vx[lg] = cell::ghost_z_vx(m, z.b1, mup, j, k, gh, vx[l]);
+    vy[lg] = vy[l];
+    if (mup < nghz - 1) vz[lg + stride] = -vz[l];
+    if (mup == 0) vz[col + (nz + nghz) * stride] = 0;
+    if (z.b1.energy) e[lg] = e[l];
+  }
+}
 template <int G, int DZ>
-__global__ void __launch_bounds__(WG) kf_finish(dmesh m, real dt, FinTab T, int nzb, cell::DampPar dp, int stockholm) {
+__global__ void __launch_bounds__(WG) kf_finish(dmesh m, real dt, FinTab T, int nzb, cell::DampPar dp, int stockholm,
+                                                ZFill zf) {
   const int fl = (int)blockIdx.z / nzb;
   const dim3 b(blockIdx.x, blockIdx.y, blockIdx.z % nzb);
   kd_finish<G, DZ>(b, m, dt, T.rho_x[fl], T.mmy[fl], T.mpy[fl], T.mmz[fl], T.mpz[fl], T.rho[fl], T.vx[fl], T.vy[fl],
                    T.vz[fl], T.rho0[fl], T.vx0[fl], T.vy0[fl], T.vz0[fl], dp, stockholm);
+  if (DZ && zf.on) {
+    const int i = (int)b.x * (int)blockDim.x + (int)threadIdx.x;
+    if (i < m.nx)
+      zfill_from_plane(m, i, m.nghy + (int)b.y, m.nghz + (int)b.z, T.rho[fl], T.vx[fl], T.vy[fl], T.vz[fl],
+                       zf.e[fl], zf);
+  }
 }
 
 // ------------------------------------------------------------------ ghost fill, potential, damping references
@@ -2359,13 +2411,15 @@ static cell::BcPar bc_par_f(const f3d_hip_ctx* ctx, int side) {
 
 // The ghost fill (src/boundary.c:2-23: zmin, zmax, then ymin, ymax) on the
 // given density/velocity buffers and the energy.
-static int fused_boundaries(f3d_hip_ctx* ctx, Scratch* S, real* dens, real* vx, real* vy, real* vz, real* e) {
+// skipz: the Z fill was done by the kernel before (kf_finish with ZFill).
+static int fused_boundaries(f3d_hip_ctx* ctx, Scratch* S, real* dens, real* vx, real* vy, real* vz, real* e,
+                            int skipz = 0) {
   const dmesh& m = S->m;
   constexpr int BX = 64;
   dim3 blk(BX);
   const cell::BcPar b0 = bc_par_f(ctx, 0), b1 = bc_par_f(ctx, 1);
   F3D_REQUIRE(ctx, m.nghy == kNgh && (!m.dimz || m.nghz == kNgh), "ghost fill: three ghost layers");
-  if (!ctx->ph.periodicz && m.dimz) {
+  if (!ctx->ph.periodicz && m.dimz && !skipz) {
     F3D_REQUIRE(ctx, ctx->ph.bc == 2, "3-D boundaries need the spherical-disk family");
     // both sides in one launch when they read and write disjoint planes
     if (m.nz > m.nghz) {
@@ -2432,6 +2486,9 @@ static int g_cflmode = -1;
 // 2-D sources of a viscous gas in one pass without the stress arrays
 // (kf_src_fused2): -1 automatic (on), 0 off (kf_src_a + kf_src_b)
 static int g_srcfuse = -1;
+// 3-D end-of-step Z ghost fill inside kf_finish: -1 automatic (on), 0 off
+// (its own launch)
+static int g_zfill = -1;
 
 // CFL form, bits: 1 ring minima to an array and a second pass (else one
 // atomic MIN per ring), 2 VxMed from kf_vmed_rings, 4 (with 2) one wave per
@@ -2544,8 +2601,22 @@ static int finish_step(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step,
   }
   const int nzb = m.dimz ? m.nz : 1;
   dim3 gf((m.nx + BXF - 1) / BXF, m.ny, nzb * nf);
+  // 3-D: the end-of-step Z fill inside the finishing kernel (one launch
+  // fewer); the Y fill follows as its own launch
+  ZFill zf;
+  memset(&zf, 0, sizeof(zf));
+  const int zmerge = end_of_step && m.dimz && !ctx->ph.periodicz && ctx->ph.bc == 2 && m.nghz == kNgh &&
+                     m.nz >= 2 * m.nghz && g_zfill != 0;
+  if (zmerge) {
+    zf.on = 1;
+    for (int q = 0; q < nf; q++) zf.e[q] = f[q].Energy;
+    zf.b0 = bc_par_f(ctx, 0);
+    zf.b1 = bc_par_f(ctx, 1);
+    zf.zd0 = S->zmin_dens;
+    zf.zd1 = S->zmax_dens;
+  }
   f3d_stage_begin(ctx, F3D_ST_FINISH);
-  FDISPATCH(ctx, kf_finish, gf, dim3(BXF), 0, (real)dt, T, nzb, dp, ctx->ph.stockholm);
+  FDISPATCH(ctx, kf_finish, gf, dim3(BXF), 0, (real)dt, T, nzb, dp, ctx->ph.stockholm, zf);
   f3d_stage_end(ctx, F3D_ST_FINISH);
   if (end_of_step) {
     // end-of-step ghost fill (src/main.c:431)
@@ -2562,7 +2633,7 @@ static int finish_step(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step,
       F3D_LAUNCH_CHECK(ctx);
     } else {
       for (int q = 0; q < nf; q++)
-        if (fused_boundaries(ctx, S, f[q].Density, f[q].Vx, f[q].Vy, f[q].Vz, f[q].Energy)) return -1;
+        if (fused_boundaries(ctx, S, f[q].Density, f[q].Vx, f[q].Vy, f[q].Vz, f[q].Energy, zmerge)) return -1;
     }
     f3d_stage_end(ctx, F3D_ST_BC_END);
   }
@@ -3020,6 +3091,12 @@ int F3D_API(fused_set_xmode)(int mode) {
 int F3D_API(fused_set_srcfuse)(int on) {
   if (on < -1 || on > 1) return -2;
   g_srcfuse = on;
+  return 0;
+}
+
+int F3D_API(fused_set_zfill)(int on) {
+  if (on < -1 || on > 1) return -2;
+  g_zfill = on;
   return 0;
 }
 

@@ -324,6 +324,11 @@ int f3d_fused_set_vmedterms(int terms);
  * 0 the stress pass and the update pass of their own; the same arithmetic
  * in both.  Returns -2 for other values. */
 int f3d_fused_set_srcfuse(int on);
+/* 3-D end-of-step ghost fill: -1 automatic (on): the Z sides are filled by
+ * the finishing kernel from the values it has just written, then the Y fill
+ * runs; 0: the Z fill as a launch of its own.  The same values in both.
+ * Returns -2 for other values. */
+int f3d_fused_set_zfill(int on);
 /* Potential, the ghost fill and the damping references on the fused stages'
  * scratch-free kernels (the same formulas as f3d_Potential, f3d_boundaries,
  * f3d_init_stockholm) */
@@ -400,6 +405,7 @@ int f3d32_fused_set_colsplit(int split);
 int f3d32_fused_set_cflmode(int mode);
 int f3d32_fused_set_vmedterms(int terms);
 int f3d32_fused_set_srcfuse(int on);
+int f3d32_fused_set_zfill(int on);
 int f3d32_fused_cfl_vm(f3d_hip_ctx *ctx, const double *vxm, double *fluid_min);
 int f3d32_fused_cfl_fluids(f3d_hip_ctx *ctx, int nf, double *const *cs, double *const *rho, double *const *vx,
                            double *const *vy, double *const *vz, const double *vxm, double *step_min);

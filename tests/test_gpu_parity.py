@@ -389,6 +389,31 @@ def test_whole_step_fused_sources_one_pass(fa, orc, par, ov, nsteps, fuse):
         L.f3d_fused_set_srcfuse(-1)
 
 
+@pytest.mark.parametrize("par,ov,nsteps", [(C2, C2_SMALL, 6), (C2, C2_AD, 4), (C2, "Nx=64 Ny=24 Nz=6", 5),
+                                           (C2, "Nx=64 Ny=24 Nz=5", 3)],
+                         ids=["c2_3d_small", "c2_3d_adiabatic_small", "c2_nz_6", "c2_nz_5_own_launch"])
+def test_whole_step_fused_z_fill_in_finish(fa, orc, par, ov, nsteps):
+    """The 3-D end-of-step Z ghost fill done by the finishing kernel (each
+    thread filling the ghost cells its plane is the source of) leaves every
+    cell, ghosts included, as the Z fill of its own launch does, and both give
+    the oracle's active cells bit for bit (with 5 planes, fewer than the six
+    the merged fill needs, both settings take the separate launch)."""
+    L = fa.lib()
+    full = {}
+    for zf in (-1, 0):
+        assert L.f3d_fused_set_zfill(zf) == 0
+        try:
+            o, s = _fused_run(fa, orc, par, ov, nsteps)
+            for f in fields_of(s):
+                assert nbits(active(s.field(f), s.dimz), active(o.field(f), s.dimz)) == 0, (ov, zf, f)
+            full[zf] = {f: s.field(f) for f in fields_of(s) + ["Energy"]}
+            s.close()
+        finally:
+            L.f3d_fused_set_zfill(-1)
+    for f in full[0]:
+        assert nbits(full[-1][f], full[0][f]) == 0, (ov, f)
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3, 6, 7], ids=["atomic", "two_pass", "vmed_pass_atomic",
                                                      "vmed_pass_two_pass", "vmed_pass_atomic_one_wave",
                                                      "vmed_pass_two_pass_one_wave"])
