@@ -1658,6 +1658,31 @@ __global__ void __launch_bounds__(64) kf_vmed(dmesh m, VmTab T, int nrings) {
   kd_vmed<G, DZ, S>(dim3(blockIdx.x % nrings), m, T.vxt[fl], T.vxmed[fl]);
 }
 
+// The 2-D Y sweep with the X sweep's VxMed pass in the same launch: the first
+// nvm workgroups (one wave each) sum one ring of one fluid's Vx_temp
+// (kd_vmed), the others sweep (workgroup L of the sweep's grid gy, in
+// XCD-contiguous order).  The Y sweep neither reads what the sums write nor
+// writes the Vx_temp they read.  The ring sums are one dependent chain each,
+// latency bound, so their waves run beside the sweep's without the second
+// stream and its events.
+template <int G, int AD, int SPLIT>
+__global__ void __launch_bounds__(BXF) kf_sweep_y2v(dmesh m, real dt, Y2Tab T, int rows_per, real of, VmTab V,
+                                                    int nrings, int nvm, dim3 gy) {
+  const int lin = (int)blockIdx.x;
+  if (lin < nvm) {
+    const int fl = lin / nrings;
+    kd_vmed<G, 0, 8>(dim3(lin % nrings), m, V.vxt[fl], V.vxmed[fl]);
+    return;
+  }
+  const int N = (int)(gy.x * gy.y * gy.z), L = lin - nvm;
+  const int x = L & 7, q = N >> 3, r = N & 7;
+  const int p = x * q + min(x, r) + (L >> 3);
+  const dim3 b(p % gy.x, (p / gy.x) % gy.y, p / (gy.x * gy.y));
+  const int fl = (int)b.z;
+  kd_sweep_y2<G, 0, AD, SPLIT>(b, m, dt, T.rho_in[fl], T.vxt[fl], T.vyt[fl], T.e_in[fl], T.rho_out[fl], T.q[fl][0],
+                               T.q[fl][1], T.q[fl][2], T.q[fl][3], T.e_out[fl], rows_per, of);
+}
+
 // X sweep of rings too long for LDS: a workgroup of T threads owns a tile of
 // T - 2H cells and loads H = 6 halo cells on each side (the van Leer pass
 // needs 2, the PPA pass 3 more of the van Leer result, NewVelocity_x the left
@@ -2489,6 +2514,9 @@ static int g_srcfuse = -1;
 // 3-D end-of-step Z ghost fill inside kf_finish: -1 automatic (on), 0 off
 // (its own launch)
 static int g_zfill = -1;
+// 2-D: the X sweep's VxMed pass inside the Y sweep's launch where it would
+// run on the second stream: -1 automatic (on), 0 off
+static int g_y2v = -1;
 
 // CFL form, bits: 1 ring minima to an array and a second pass (else one
 // atomic MIN per ring), 2 VxMed from kf_vmed_rings, 4 (with 2) one wave per
@@ -2642,7 +2670,9 @@ static int finish_step(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step,
 
 // momenta and the Z (3-D) and Y sweeps of every fluid into its transport set
 // (2-D: one Y-sweep launch over the fluids; 3-D: one pair of launches per fluid)
-static int yz_sweeps(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const RF* fl, const TSet* tl) {
+// vm (2-D only): the X sweep's VxMed pass rides in the Y sweep's launch
+static int yz_sweeps(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const RF* fl, const TSet* tl,
+                     const VmTab* vm = nullptr) {
   const dmesh& m = S->m;
   const real of = (real)ctx->ph.OMEGAFRAME;
   const real rdt = (real)dt;
@@ -2684,7 +2714,22 @@ static int yz_sweeps(f3d_hip_ctx* ctx, Scratch* S, double dt, int nf, const RF* 
     }
     dim3 gy((m.nx + BXF - 1) / BXF, ((m.ny + ry - 1) / ry) * sp, nf);
     f3d_stage_begin(ctx, F3D_ST_SWEEP_Y);
-    FDISPATCH_ADS(ctx, kf_sweep_y2, gy, rdt, T, ry, of);
+    if (vm) {
+      const int nrings = m.ny, nvm = nrings * nf;
+      const dim3 g1(nvm + gy.x * gy.y * gy.z);
+      const int ad = (ctx->ph.eos == 2);
+#define Y2V(G_, AD_, SP_)                                                                                     \
+  hipLaunchKernelGGL((kf_sweep_y2v<G_, AD_, SP_>), g1, dim3(BXF), 0, ctx->stream, m, rdt, T, ry, of, *vm, nrings, \
+                     nvm, gy)
+#define Y2V_G(G_) \
+  do { if (ad) { if (split) Y2V(G_, 1, 1); else Y2V(G_, 1, 0); } else { if (split) Y2V(G_, 0, 1); else Y2V(G_, 0, 0); } } while (0)
+      if (m.geom == SPH) Y2V_G(SPH); else Y2V_G(CYL);
+#undef Y2V_G
+#undef Y2V
+      F3D_LAUNCH_CHECK(ctx);
+    } else {
+      FDISPATCH_ADS(ctx, kf_sweep_y2, gy, rdt, T, ry, of);
+    }
     f3d_stage_end(ctx, F3D_ST_SWEEP_Y);
   }
 #undef FDISPATCH_ADS
@@ -3024,7 +3069,16 @@ static int transport_fluids(f3d_hip_ctx* ctx, double dt, int nf, const f3d_field
   const bool wtiles = !wave && (tiles ? x_wtile_form(S->m) : x_wtile_pays(S->m, nf));
   // the one-wave sweep sums VxMed itself; the tiled ones take it from the pass
   const int vpre = (ctx->ph.fargo && !wave) ? x_vmed_form(S->m, nf, tiles || wtiles) : 0;
-  if (vpre == 1) {
+  // 2-D: the pass in the Y sweep's launch instead of on the second stream
+  const bool vin = vpre == 1 && !S->m.dimz && g_y2v != 0 && vmed_terms(S->m.nx) == 8 &&
+                   (size_t)S->m.nx * sizeof(double) <= 64 * 1024;
+  VmTab VT;
+  memset(&VT, 0, sizeof(VT));
+  for (int q = 0; q < nf; q++) {
+    VT.vxt[q] = f[q].Vx_temp;
+    VT.vxmed[q] = t[q].vxmed;
+  }
+  if (vpre == 1 && !vin) {
     // VxMed of each fluid's Vx_temp (one long sequential sum per ring) on the
     // side stream, overlapping the Z/Y sweeps, which do not write Vx_temp
     if (side_stream(ctx)) return -1;
@@ -3033,8 +3087,8 @@ static int transport_fluids(f3d_hip_ctx* ctx, double dt, int nf, const f3d_field
     if (x_vmed(ctx, S, nf, f, t, ctx->side)) return -1;
     F3D_CHECK(ctx, hipEventRecord(ctx->side_done, ctx->side));
   }
-  if (yz_sweeps(ctx, S, dt, nf, f, t)) return -1;
-  if (vpre == 1) F3D_CHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->side_done, 0));
+  if (yz_sweeps(ctx, S, dt, nf, f, t, vin ? &VT : nullptr)) return -1;
+  if (vpre == 1 && !vin) F3D_CHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->side_done, 0));
   if (vpre == 2 && x_vmed(ctx, S, nf, f, t, ctx->stream)) return -1;
   if (wave) {
     if (x_sweep_wave(ctx, S, dt, nf, f, t)) return -1;
@@ -3091,6 +3145,12 @@ int F3D_API(fused_set_xmode)(int mode) {
 int F3D_API(fused_set_srcfuse)(int on) {
   if (on < -1 || on > 1) return -2;
   g_srcfuse = on;
+  return 0;
+}
+
+int F3D_API(fused_set_y2v)(int on) {
+  if (on < -1 || on > 1) return -2;
+  g_y2v = on;
   return 0;
 }
 

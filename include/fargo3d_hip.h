@@ -329,6 +329,11 @@ int f3d_fused_set_srcfuse(int on);
  * runs; 0: the Z fill as a launch of its own.  The same values in both.
  * Returns -2 for other values. */
 int f3d_fused_set_zfill(int on);
+/* 2-D X sweep of long or many rings: the VxMed pass it reads runs in the Y
+ * sweep's launch (its own workgroups beside the sweep's), -1 automatic (on),
+ * or 0 on the second stream; the same sums in both.  Returns -2 for other
+ * values. */
+int f3d_fused_set_y2v(int on);
 /* Potential, the ghost fill and the damping references on the fused stages'
  * scratch-free kernels (the same formulas as f3d_Potential, f3d_boundaries,
  * f3d_init_stockholm) */
@@ -406,6 +411,7 @@ int f3d32_fused_set_cflmode(int mode);
 int f3d32_fused_set_vmedterms(int terms);
 int f3d32_fused_set_srcfuse(int on);
 int f3d32_fused_set_zfill(int on);
+int f3d32_fused_set_y2v(int on);
 int f3d32_fused_cfl_vm(f3d_hip_ctx *ctx, const double *vxm, double *fluid_min);
 int f3d32_fused_cfl_fluids(f3d_hip_ctx *ctx, int nf, double *const *cs, double *const *rho, double *const *vx,
                            double *const *vy, double *const *vz, const double *vxm, double *step_min);

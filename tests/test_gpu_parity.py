@@ -367,6 +367,28 @@ def test_whole_step_fused_one_wave_tiles(fa, orc, par, ov, nsteps):
         L.f3d_fused_set_xmode(0)
 
 
+@pytest.mark.parametrize("y2v", [-1, 0], ids=["pass_in_y_sweep_launch", "pass_on_second_stream"])
+@pytest.mark.parametrize("par,ov,nsteps", [(C3, C3_SMALL, 8), (C3, "Nx=1000 Ny=24", 5), (C4, "Nx=128 Ny=64", 4),
+                                           (C1, "Nx=100 Ny=40", 5)],
+                         ids=["c3_2d_adiabatic_256", "c3_2d_adiabatic_1000", "c4_five_fluids_128", "c1_2d_ragged"])
+def test_whole_step_fused_vmed_in_y_sweep(fa, orc, par, ov, nsteps, y2v):
+    """The X sweep's VxMed pass run by workgroups of the 2-D Y sweep's launch
+    (before the sweep's own workgroups, every fluid) or on the second stream
+    gives the oracle's results bit for bit (the tiled X sweep, which takes
+    VxMed from the pass)."""
+    L = fa.lib()
+    assert L.f3d_fused_set_xmode(4) == 0
+    assert L.f3d_fused_set_y2v(y2v) == 0
+    try:
+        o, s = _fused_run(fa, orc, par, ov, nsteps)
+        for f in fields_of(s):
+            assert nbits(active(s.field(f), s.dimz), active(o.field(f), s.dimz)) == 0, (par, ov, y2v, f)
+        s.close()
+    finally:
+        L.f3d_fused_set_xmode(0)
+        L.f3d_fused_set_y2v(-1)
+
+
 @pytest.mark.parametrize("fuse", [1, 0], ids=["one_pass", "stress_arrays"])
 @pytest.mark.parametrize("par,ov,nsteps", [(C1, "", 12), (C1, "ViscosityModel=alpha Alpha=1e-3", 8),
                                            (C3, C3_SMALL, 10), (C3, C3_SMALL + " ViscosityModel=alpha Alpha=1e-3", 6),
