@@ -71,7 +71,7 @@ def stage_arrays(stage, dz, ad, alpha):
 
 # the kernels each stage's events bracket (names without namespace and template arguments)
 STAGE_KERNEL = {"cfl": ["kf_cfl", "kf_cfl_fluids", "kf_vmed_rings", "kf_cfl_fin"], "sources_a": ["kf_src_a"],
-                "sources_b": ["kf_src_b", "kf_src_b_fluids", "kf_src_fused2"], "sweep_z": ["kf_sweep_z"], "sweep_y": ["kf_sweep_y2", "kf_sweep_y3"],
+                "sources_b": ["kf_src_b", "kf_src_b_fluids", "kf_src_fused2"], "sweep_z": ["kf_sweep_z"], "sweep_y": ["kf_sweep_y2", "kf_sweep_y2v", "kf_sweep_y3"],
                 "sweep_x": ["kf_sweep_xw", "kf_sweep_xtw", "kf_sweep_xr_fluids", "kf_sweep_x_tile"], "finish": ["kf_finish"],
                 "potential": ["k_potential", "k_potential_fin", "kf_potential"], "substep3": ["kf_substep3"]}
 # rocprofv3 summary with the measured HBM traffic of each kernel of this workload
