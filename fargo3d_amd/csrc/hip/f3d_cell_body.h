@@ -762,32 +762,39 @@ __device__ __forceinline__ GhostCell ghost_y_load(const dmesh& m, int side, int 
   return c;
 }
 
-__device__ __forceinline__ void ghost_y_store(const dmesh& m, int mm, int k, const GhostCell& c, real* dens, real* vx,
-                                              real* vy, real* vz, real* e, const BcPar& b, const real* dexp) {
-  real dg, xg;
+// the density and azimuthal velocity of Y ghost layer mm (row gh, plane k)
+// from d0 and x0, the stored values of its active row act
+__device__ __forceinline__ void ghost_y_dx(const dmesh& m, const BcPar& b, int mm, int k, int act, int gh, real d0,
+                                           real x0, const real* dexp, real& dg, real& xg) {
   if constexpr (kFix64) {
     // the full velocities in FP64, the ghost's deviation from its own V0
-    const double of = b.of, xa = (double)c.x0 + vbase(m, c.act, k);
+    const double of = b.of, xa = (double)x0 + vbase(m, act, k);
     double x;
     if (!b.p3d) {
-      dg = c.d0 * b.dens[mm];
-      x = (xa + m.ymed64[c.act] * of) * (double)b.vsq[mm] - m.ymed64[c.gh] * of;
+      dg = d0 * b.dens[mm];
+      x = (xa + m.ymed64[act] * of) * (double)b.vsq[mm] - m.ymed64[gh] * of;
     } else {
-      dg = c.d0 * b.dens[mm] * dexp[mm * m.sz + k];
-      x = (xa + of * m.ymed64[c.act] * m.sinzmed64[k]) * (double)b.vsq[mm] * (double)b.vfac[mm] -
-          m.ymed64[c.gh] * of * m.sinzmed64[k];
+      dg = d0 * b.dens[mm] * dexp[mm * m.sz + k];
+      x = (xa + of * m.ymed64[act] * m.sinzmed64[k]) * (double)b.vsq[mm] * (double)b.vfac[mm] -
+          m.ymed64[gh] * of * m.sinzmed64[k];
     }
-    xg = (real)(x - vbase(m, c.gh, k));
+    xg = (real)(x - vbase(m, gh, k));
   } else {
     const real of = b.of;
     if (!b.p3d) {
-      dg = c.d0 * b.dens[mm];
-      xg = (c.x0 + m.ymed[c.act] * of) * b.vsq[mm] - m.ymed[c.gh] * of;
+      dg = d0 * b.dens[mm];
+      xg = (x0 + m.ymed[act] * of) * b.vsq[mm] - m.ymed[gh] * of;
     } else {
-      dg = c.d0 * b.dens[mm] * dexp[mm * m.sz + k];
-      xg = (c.x0 + of * m.ymed[c.act] * m.sinzmed[k]) * b.vsq[mm] * b.vfac[mm] - m.ymed[c.gh] * of * m.sinzmed[k];
+      dg = d0 * b.dens[mm] * dexp[mm * m.sz + k];
+      xg = (x0 + of * m.ymed[act] * m.sinzmed[k]) * b.vsq[mm] * b.vfac[mm] - m.ymed[gh] * of * m.sinzmed[k];
     }
   }
+}
+
+__device__ __forceinline__ void ghost_y_store(const dmesh& m, int mm, int k, const GhostCell& c, real* dens, real* vx,
+                                              real* vy, real* vz, real* e, const BcPar& b, const real* dexp) {
+  real dg, xg;
+  ghost_y_dx(m, b, mm, k, c.act, c.gh, c.d0, c.x0, dexp, dg, xg);
   dens[c.lgh] = dg;
   vx[c.lgh] = xg;
   if (b.p3d) vz[c.lgh] = c.z0;

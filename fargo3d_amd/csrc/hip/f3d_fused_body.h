@@ -1969,11 +1969,13 @@ struct FinTab {
 // Z fill also wrote in the Y ghost rows of the Z ghost planes, the Y fill that
 // follows rewrites in full (rho, vx, vz, e of every Y ghost row; vy of the
 // ghost faces and the boundary face).
+// on: 1 the Z sides (3-D), 2 the Y sides (2-D, yfill_from_row); zd0/zd1:
+// the density factors of the lower and upper Z ghost layers
 struct ZFill {
   int on;
   real* e[8];
   cell::BcPar b0, b1;
-  const real* zd0;  // density factors of the lower and upper ghost layers
+  const real* zd0;
   const real* zd1;
 };
 __device__ __forceinline__ void zfill_from_plane(const dmesh& m, int i, int j, int k, real* dens, real* vx, real* vy,
@@ -2002,6 +2004,40 @@ __device__ __forceinline__ void zfill_from_plane(const dmesh& m, int i, int j, i
     if (z.b1.energy) e[lg] = e[l];
   }
 }
+// The 2-D end-of-step Y ghost fill (bound_y_cell of both sides, the disk
+// family's rules without the 3-D factors) inside the finishing kernel, as the
+// Z fill above: row j of column i gives the lower ghost row 2 nghy - 1 - j
+// its rho, vx (and e) and ghost row 2 nghy - j its vy, negated; the upper
+// ghost row ny + nghy + (ny + nghy - 1 - j) all of them (vy one row further
+// out); the boundary faces vy(nghy) and vy(ny + nghy) become zero.
+__device__ __forceinline__ void yfill_from_row(const dmesh& m, int i, int j, real* dens, real* vx, real* vy, real* e,
+                                               const ZFill& z) {
+  const int nghy = m.nghy, ny = m.ny, pitch = m.pitch;
+  const int l = i + j * pitch;
+  const int mlo = 2 * nghy - 1 - j;
+  if (mlo >= 0 && mlo < nghy) {
+    const int lg = i + mlo * pitch;
+    real dg, xg;
+    cell::ghost_y_dx(m, z.b0, mlo, 0, j, mlo, dens[l], vx[l], nullptr, dg, xg);
+    dens[lg] = dg;
+    vx[lg] = xg;
+    if (z.b0.energy) e[lg] = e[l];
+  }
+  const int mly = 2 * nghy - j;
+  if (mly >= 0 && mly < nghy) vy[i + mly * pitch] = -vy[l];
+  if (j == nghy) vy[l] = 0;  // after this thread's own store there
+  const int mup = ny + nghy - 1 - j;
+  if (mup >= 0 && mup < nghy) {
+    const int gh = ny + nghy + mup, lg = i + gh * pitch;
+    real dg, xg;
+    cell::ghost_y_dx(m, z.b1, mup, 0, j, gh, dens[l], vx[l], nullptr, dg, xg);
+    dens[lg] = dg;
+    vx[lg] = xg;
+    if (mup < nghy - 1) vy[lg + pitch] = -vy[l];
+    if (mup == 0) vy[i + (ny + nghy) * pitch] = 0;
+    if (z.b1.energy) e[lg] = e[l];
+  }
+}
 template <int G, int DZ>
 __global__ void __launch_bounds__(WG) kf_finish(dmesh m, real dt, FinTab T, int nzb, cell::DampPar dp, int stockholm,
                                                 ZFill zf) {
@@ -2009,11 +2045,15 @@ __global__ void __launch_bounds__(WG) kf_finish(dmesh m, real dt, FinTab T, int 
   const dim3 b(blockIdx.x, blockIdx.y, blockIdx.z % nzb);
   kd_finish<G, DZ>(b, m, dt, T.rho_x[fl], T.mmy[fl], T.mpy[fl], T.mmz[fl], T.mpz[fl], T.rho[fl], T.vx[fl], T.vy[fl],
                    T.vz[fl], T.rho0[fl], T.vx0[fl], T.vy0[fl], T.vz0[fl], dp, stockholm);
-  if (DZ && zf.on) {
+  if (zf.on) {
     const int i = (int)b.x * (int)blockDim.x + (int)threadIdx.x;
-    if (i < m.nx)
-      zfill_from_plane(m, i, m.nghy + (int)b.y, m.nghz + (int)b.z, T.rho[fl], T.vx[fl], T.vy[fl], T.vz[fl],
-                       zf.e[fl], zf);
+    if (i < m.nx) {
+      if (DZ)
+        zfill_from_plane(m, i, m.nghy + (int)b.y, m.nghz + (int)b.z, T.rho[fl], T.vx[fl], T.vy[fl], T.vz[fl],
+                         zf.e[fl], zf);
+      else
+        yfill_from_row(m, i, m.nghy + (int)b.y, T.rho[fl], T.vx[fl], T.vy[fl], zf.e[fl], zf);
+    }
   }
 }
 
@@ -2511,8 +2551,8 @@ static int g_cflmode = -1;
 // 2-D sources of a viscous gas in one pass without the stress arrays
 // (kf_src_fused2): -1 automatic (on), 0 off (kf_src_a + kf_src_b)
 static int g_srcfuse = -1;
-// 3-D end-of-step Z ghost fill inside kf_finish: -1 automatic (on), 0 off
-// (its own launch)
+// end-of-step ghost fill inside kf_finish (3-D: the Z sides, 2-D: the Y
+// sides): -1 automatic (on), 0 off (launches of its own)
 static int g_zfill = -1;
 // 2-D: the X sweep's VxMed pass inside the Y sweep's launch where it would
 // run on the second stream: -1 automatic (on), 0 off
@@ -2629,14 +2669,17 @@ static int finish_step(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step,
   }
   const int nzb = m.dimz ? m.nz : 1;
   dim3 gf((m.nx + BXF - 1) / BXF, m.ny, nzb * nf);
-  // 3-D: the end-of-step Z fill inside the finishing kernel (one launch
-  // fewer); the Y fill follows as its own launch
+  // the end-of-step ghost fill inside the finishing kernel, one launch
+  // fewer: in 3-D the Z sides (the Y fill follows as its own launch), in 2-D
+  // the Y sides (the disk family's rules)
   ZFill zf;
   memset(&zf, 0, sizeof(zf));
   const int zmerge = end_of_step && m.dimz && !ctx->ph.periodicz && ctx->ph.bc == 2 && m.nghz == kNgh &&
                      m.nz >= 2 * m.nghz && g_zfill != 0;
-  if (zmerge) {
-    zf.on = 1;
+  const int ymerge = end_of_step && !m.dimz && !ctx->ph.periodicy && ctx->ph.bc != 2 && m.nghy == kNgh &&
+                     m.ny >= 2 * m.nghy && g_zfill != 0;
+  if (zmerge || ymerge) {
+    zf.on = zmerge ? 1 : 2;
     for (int q = 0; q < nf; q++) zf.e[q] = f[q].Energy;
     zf.b0 = bc_par_f(ctx, 0);
     zf.b1 = bc_par_f(ctx, 1);
@@ -2646,7 +2689,7 @@ static int finish_step(f3d_hip_ctx* ctx, Scratch* S, double dt, int end_of_step,
   f3d_stage_begin(ctx, F3D_ST_FINISH);
   FDISPATCH(ctx, kf_finish, gf, dim3(BXF), 0, (real)dt, T, nzb, dp, ctx->ph.stockholm, zf);
   f3d_stage_end(ctx, F3D_ST_FINISH);
-  if (end_of_step) {
+  if (end_of_step && !ymerge) {
     // end-of-step ghost fill (src/main.c:431)
     f3d_stage_begin(ctx, F3D_ST_BC_END);
     if (nf > 1 && !m.dimz && !ctx->ph.periodicy && m.ny > m.nghy) {

@@ -324,10 +324,11 @@ int f3d_fused_set_vmedterms(int terms);
  * 0 the stress pass and the update pass of their own; the same arithmetic
  * in both.  Returns -2 for other values. */
 int f3d_fused_set_srcfuse(int on);
-/* 3-D end-of-step ghost fill: -1 automatic (on): the Z sides are filled by
- * the finishing kernel from the values it has just written, then the Y fill
- * runs; 0: the Z fill as a launch of its own.  The same values in both.
- * Returns -2 for other values. */
+/* End-of-step ghost fill: -1 automatic (on): in 3-D the Z sides, in 2-D the
+ * Y sides, are filled by the finishing kernel from the values it has just
+ * written (in 3-D the Y fill then runs as its own launch); 0: the fills as
+ * launches of their own.  The same values in both.  Returns -2 for other
+ * values. */
 int f3d_fused_set_zfill(int on);
 /* 2-D X sweep of long or many rings: the VxMed pass it reads runs in the Y
  * sweep's launch (its own workgroups beside the sweep's), -1 automatic (on),

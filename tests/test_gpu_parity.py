@@ -412,14 +412,18 @@ def test_whole_step_fused_sources_one_pass(fa, orc, par, ov, nsteps, fuse):
 
 
 @pytest.mark.parametrize("par,ov,nsteps", [(C2, C2_SMALL, 6), (C2, C2_AD, 4), (C2, "Nx=64 Ny=24 Nz=6", 5),
-                                           (C2, "Nx=64 Ny=24 Nz=5", 3)],
-                         ids=["c2_3d_small", "c2_3d_adiabatic_small", "c2_nz_6", "c2_nz_5_own_launch"])
+                                           (C2, "Nx=64 Ny=24 Nz=5", 3), (C1, "", 8), (C3, C3_SMALL, 6),
+                                           (C4, "Nx=128 Ny=64", 4), (C1, "Nx=96 Ny=6 Ymin=1.0 Ymax=1.3", 4),
+                                           (C1, "Nx=96 Ny=5 Ymin=1.0 Ymax=1.3", 3)],
+                         ids=["c2_3d_small", "c2_3d_adiabatic_small", "c2_nz_6", "c2_nz_5_own_launch", "c1_2d",
+                              "c3_2d_adiabatic_small", "c4_five_fluids_128", "c1_ny_6", "c1_ny_5_own_launch"])
 def test_whole_step_fused_z_fill_in_finish(fa, orc, par, ov, nsteps):
-    """The 3-D end-of-step Z ghost fill done by the finishing kernel (each
-    thread filling the ghost cells its plane is the source of) leaves every
-    cell, ghosts included, as the Z fill of its own launch does, and both give
-    the oracle's active cells bit for bit (with 5 planes, fewer than the six
-    the merged fill needs, both settings take the separate launch)."""
+    """The end-of-step ghost fill done by the finishing kernel (3-D: the Z
+    sides; 2-D: the Y sides; each thread filling the ghost cells its plane or
+    row is the source of) leaves every cell, ghosts included, as the fill of
+    its own launch does, and both give the oracle's active cells bit for bit
+    (with 5 planes or rows, fewer than the six the merged fill needs, both
+    settings take the separate launch)."""
     L = fa.lib()
     full = {}
     for zf in (-1, 0):
