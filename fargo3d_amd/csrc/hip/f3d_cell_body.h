@@ -404,6 +404,51 @@ __device__ __forceinline__ real potential(const dmesh& m, int i, int j, int k, c
   return pot;
 }
 
+// The same for C cells i0 + u * di of one ring (j, k), what depends on the
+// ring alone (the star term, each planet's distance and the indirect term's
+// divisor, shared through rdiv()) evaluated once: every cell's value is
+// potential()'s, bit for bit.  Cells with i >= nx are skipped.
+template <int G, int C>
+__device__ __forceinline__ void potential_cells(const dmesh& m, int i0, int di, int j, int k, const PlanetSet& ps,
+                                                real* pot, int l0) {
+  real ZC, pot0;
+  if (G == SPH) {
+    ZC = m.ymed[j] * m.coszmed[k];
+    pot0 = -real(1.0) * real(1.0) / m.ymed[j];
+  } else {
+    ZC = m.zmed[k];
+    pot0 = -real(1.0) * real(1.0) / sqrt(m.ymed[j] * m.ymed[j] + ZC * ZC);
+  }
+  real acc[C], XC[C], YC[C];
+#pragma unroll
+  for (int u = 0; u < C; u++) {
+    const int i = min(i0 + u * di, m.nx - 1);
+    acc[u] = pot0;
+    if (G == SPH) {
+      XC[u] = m.ymed[j] * m.cosxmed[i] * m.sinzmed[k];
+      YC[u] = m.ymed[j] * m.sinxmed[i] * m.sinzmed[k];
+    } else {
+      XC[u] = m.ymed[j] * m.cosxmed[i];
+      YC[u] = m.ymed[j] * m.sinxmed[i];
+    }
+  }
+  for (int n = 0; n < ps.n; n++) {
+    const real mp = ps.mp[n];
+    const real xp = ps.x[n], yp = ps.y[n], zp = ps.z[n];
+    const real planetdistance = sqrt(xp * xp + yp * yp + zp * zp);
+    const Recip D = recip(planetdistance * planetdistance * planetdistance);
+#pragma unroll
+    for (int u = 0; u < C; u++) {
+      const real dist = ((XC[u] - xp) * (XC[u] - xp) + (YC[u] - yp) * (YC[u] - yp) + (ZC - zp) * (ZC - zp));
+      if (ps.indirect) acc[u] += rdiv(real(1.0) * mp * (XC[u] * xp + YC[u] * yp + ZC * zp), D);
+      acc[u] += -real(1.0) * mp / sqrt(dist + ps.smooth2[n]);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < C; u++)
+    if (i0 + u * di < m.nx) pot[l0 + u * di] = acc[u];
+}
+
 // ---------------------------------------------------------------- momenta / new velocities
 // src/momenta_x.c:76-84
 template <int G>

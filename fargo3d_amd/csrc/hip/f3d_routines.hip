@@ -17,6 +17,7 @@ using namespace f3d;
 namespace {
 
 constexpr int BX = 128;
+constexpr int kPotCells = 4;  // cells per thread of the potential kernels
 
 struct Range {
   int jb, je, kb, ke;
@@ -43,10 +44,13 @@ inline Range rng(const f3d_hip_ctx* c, int jb, int je, int kb, int ke) {
   const int l = i + j * m.pitch + k * m.stride;          \
   (void)l
 
-#define DISPATCH(ctx, KERN, r, ...)                                                              \
+#define DISPATCH(ctx, KERN, r, ...) DISPATCH_X(ctx, KERN, r, 1, __VA_ARGS__)
+// the same with cpt cells per thread along x (a workgroup covers cpt * BX)
+#define DISPATCH_X(ctx, KERN, r, cpt, ...)                                                       \
   do {                                                                                           \
     if (empty(r)) break;                                                                         \
     dim3 g_ = grid_of(ctx, r);                                                                   \
+    g_.x = ((ctx)->m.nx + (cpt) * BX - 1) / ((cpt) * BX);                                        \
     if ((ctx)->m.geom == SPH) {                                                                  \
       if ((ctx)->m.dimz)                                                                         \
         hipLaunchKernelGGL((KERN<SPH, 1>), g_, dim3(BX), 0, (ctx)->stream, (ctx)->m, r, __VA_ARGS__); \
@@ -191,9 +195,12 @@ __global__ void k_addviscosity(dmesh m, Range r, double dt, const double* rho, d
 }
 
 template <int G, int DZ>
-__global__ void k_potential(dmesh m, Range r, double* pot, cell::PlanetSet ps) {
-  CELL_INDEX;
-  pot[l] = cell::potential<G>(m, i, j, k, ps);
+__global__ void __launch_bounds__(BX) k_potential(dmesh m, Range r, double* pot, cell::PlanetSet ps) {
+  // kPotCells cells per thread, BX apart (a workgroup covers kPotCells * BX
+  // cells of one ring): what depends on the ring alone is evaluated once
+  const int i0 = (int)blockIdx.x * BX * kPotCells + (int)threadIdx.x;
+  const int j = r.jb + (int)blockIdx.y, k = r.kb + (int)blockIdx.z;
+  cell::potential_cells<G, kPotCells>(m, i0, BX, j, k, ps, pot, i0 + j * m.pitch + k * m.stride);
 }
 // the same, its first workgroup also taking the queued fused CFL's MIN pass
 // (which would otherwise be a launch of its own; the potential needs no dt)
@@ -205,8 +212,9 @@ __global__ void __launch_bounds__(BX) k_potential_fin(dmesh m, Range r, double* 
     __shared__ double sh[BX / 64];
     cfl_min_publish(fin.ringmin, fin.n, fin.result, fin.slot, host_out, seq, sh);
   }
-  CELL_INDEX;
-  pot[l] = cell::potential<G>(m, i, j, k, ps);
+  const int i0 = (int)blockIdx.x * BX * kPotCells + (int)threadIdx.x;
+  const int j = r.jb + (int)blockIdx.y, k = r.kb + (int)blockIdx.z;
+  cell::potential_cells<G, kPotCells>(m, i0, BX, j, k, ps, pot, i0 + j * m.pitch + k * m.stride);
 }
 
 template <int G, int DZ>
@@ -848,9 +856,10 @@ int f3d_Potential(f3d_hip_ctx* ctx) {
   if (ctx->fin_pending && !empty(r)) {
     // the fused CFL queued before this launch left its MIN pass to it
     ctx->fin_pending = 0;
-    DISPATCH(ctx, k_potential_fin, r, ctx->f.Pot, planet_set(ctx), ctx->fin, ctx->dhres, ctx->cfl_seq);
+    DISPATCH_X(ctx, k_potential_fin, r, kPotCells, ctx->f.Pot, planet_set(ctx), ctx->fin, ctx->dhres,
+               ctx->cfl_seq);
   } else {
-    DISPATCH(ctx, k_potential, r, ctx->f.Pot, planet_set(ctx));
+    DISPATCH_X(ctx, k_potential, r, kPotCells, ctx->f.Pot, planet_set(ctx));
   }
   f3d_stage_end(ctx, F3D_ST_POTENTIAL);
   return 0;
