@@ -17,7 +17,9 @@ using namespace f3d;
 namespace {
 
 constexpr int BX = 128;
-constexpr int kPotCells = 4;  // cells per thread of the potential kernels
+// cells per thread of the potential kernels: four on rings of 1024 cells or
+// more, else one (shorter rings would leave threads idle)
+inline int pot_cells(const f3d_hip_ctx* c) { return c->m.nx >= 8 * BX ? 4 : 1; }
 
 struct Range {
   int jb, je, kb, ke;
@@ -195,26 +197,30 @@ __global__ void k_addviscosity(dmesh m, Range r, double dt, const double* rho, d
 }
 
 template <int G, int DZ>
-__global__ void __launch_bounds__(BX) k_potential(dmesh m, Range r, double* pot, cell::PlanetSet ps) {
-  // kPotCells cells per thread, BX apart (a workgroup covers kPotCells * BX
+__global__ void __launch_bounds__(BX) k_potential(dmesh m, Range r, double* pot, cell::PlanetSet ps, int cpt) {
+  // cpt (1 or 4) cells per thread, BX apart (a workgroup covers cpt * BX
   // cells of one ring): what depends on the ring alone is evaluated once
-  const int i0 = (int)blockIdx.x * BX * kPotCells + (int)threadIdx.x;
+  const int i0 = (int)blockIdx.x * BX * cpt + (int)threadIdx.x;
   const int j = r.jb + (int)blockIdx.y, k = r.kb + (int)blockIdx.z;
-  cell::potential_cells<G, kPotCells>(m, i0, BX, j, k, ps, pot, i0 + j * m.pitch + k * m.stride);
+  const int l0 = i0 + j * m.pitch + k * m.stride;
+  if (cpt == 4) cell::potential_cells<G, 4>(m, i0, BX, j, k, ps, pot, l0);
+  else cell::potential_cells<G, 1>(m, i0, BX, j, k, ps, pot, l0);
 }
 // the same, its first workgroup also taking the queued fused CFL's MIN pass
 // (which would otherwise be a launch of its own; the potential needs no dt)
 template <int G, int DZ>
 __global__ void __launch_bounds__(BX) k_potential_fin(dmesh m, Range r, double* pot, cell::PlanetSet ps,
                                                       f3d_hip_ctx::CflFin fin, double* host_out,
-                                                      unsigned long long seq) {
+                                                      unsigned long long seq, int cpt) {
   if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) {
     __shared__ double sh[BX / 64];
     cfl_min_publish(fin.ringmin, fin.n, fin.result, fin.slot, host_out, seq, sh);
   }
-  const int i0 = (int)blockIdx.x * BX * kPotCells + (int)threadIdx.x;
+  const int i0 = (int)blockIdx.x * BX * cpt + (int)threadIdx.x;
   const int j = r.jb + (int)blockIdx.y, k = r.kb + (int)blockIdx.z;
-  cell::potential_cells<G, kPotCells>(m, i0, BX, j, k, ps, pot, i0 + j * m.pitch + k * m.stride);
+  const int l0 = i0 + j * m.pitch + k * m.stride;
+  if (cpt == 4) cell::potential_cells<G, 4>(m, i0, BX, j, k, ps, pot, l0);
+  else cell::potential_cells<G, 1>(m, i0, BX, j, k, ps, pot, l0);
 }
 
 template <int G, int DZ>
@@ -853,13 +859,13 @@ int f3d_Potential(f3d_hip_ctx* ctx) {
   F3D_REQUIRE(ctx, ctx->ph.nplanets <= cell::kMaxPlanets, "Potential: too many planets");
   Range r = rng(ctx, 0, ctx->m.sy, 0, ctx->m.sz);
   f3d_stage_begin(ctx, F3D_ST_POTENTIAL);
+  const int cpt = pot_cells(ctx);
   if (ctx->fin_pending && !empty(r)) {
     // the fused CFL queued before this launch left its MIN pass to it
     ctx->fin_pending = 0;
-    DISPATCH_X(ctx, k_potential_fin, r, kPotCells, ctx->f.Pot, planet_set(ctx), ctx->fin, ctx->dhres,
-               ctx->cfl_seq);
+    DISPATCH_X(ctx, k_potential_fin, r, cpt, ctx->f.Pot, planet_set(ctx), ctx->fin, ctx->dhres, ctx->cfl_seq, cpt);
   } else {
-    DISPATCH_X(ctx, k_potential, r, kPotCells, ctx->f.Pot, planet_set(ctx));
+    DISPATCH_X(ctx, k_potential, r, cpt, ctx->f.Pot, planet_set(ctx), cpt);
   }
   f3d_stage_end(ctx, F3D_ST_POTENTIAL);
   return 0;
